@@ -1,0 +1,282 @@
+#!/usr/bin/env python3
+"""Connected-components throughput on one MI355X, beside the host CPU.
+
+A step is one full local-CC solve (labels = minimum vertex id per component)
+of the workload graph, CSR already resident in HBM.  TEPS = m / t where m is
+the directed CSR entry count row_ptr[n] (the reference's m).
+
+Prints one JSON line (rank 0) with the metric, a `roofline` object for the
+dominant kernel (algorithmic bytes / HIP-event-timed average duration, vs the
+8 TB/s HBM peak) and a `cpu_baseline` object: the reference's pthreads label
+propagation, compiled from its own sources into oracle/_ref, timed on this
+host's cores on the same graph.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0  # MI355X HBM3E peak, GB/s
+
+
+def log(msg: str) -> None:
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+WORKLOADS = {
+    # R-MAT scale 22, edge factor 16, (a,b,c,d) = (0.57,0.19,0.19,0.05),
+    # symmetrised, deduplicated, ids permuted from the seed.
+    "rmat22": dict(kind="rmat", scale=22, ef=16, seed=1002),
+    # Erdos-Renyi 65,536 vertices, average degree 8.
+    "er64k": dict(kind="er", n=65536, deg=8.0, seed=1001),
+    # 8192 x 8192 lattice with 30% of edges deleted.
+    "grid8k": dict(kind="grid", rows=8192, cols=8192, keep=0.7, seed=1004),
+    # Chung-Lu power law (gamma 2.1), 65,601,024 vertices, 1.8e9 pairs.
+    "chunglu65m": dict(kind="chunglu", n=65601024, pairs=1800000000, gamma=2.1, seed=1003),
+}
+
+
+def make_graph(name: str):
+    import cc_mpi_amd as cm
+    w = WORKLOADS[name]
+    if w["kind"] == "rmat":
+        return cm.rmat(w["scale"], w["ef"], seed=w["seed"]), (
+            f"R-MAT scale {w['scale']} ef {w['ef']} (0.57,0.19,0.19,0.05) sym+dedup, "
+            f"permuted ids, seed {w['seed']}")
+    if w["kind"] == "er":
+        return cm.erdos_renyi(w["n"], w["deg"], seed=w["seed"]), (
+            f"Erdos-Renyi n={w['n']} avg degree {w['deg']:g}, seed {w['seed']}")
+    if w["kind"] == "grid":
+        return cm.grid(w["rows"], w["cols"], w["keep"], seed=w["seed"]), (
+            f"{w['rows']}x{w['cols']} grid, {1 - w['keep']:.0%} edges deleted, permuted, "
+            f"seed {w['seed']}")
+    return cm.chung_lu(w["n"], w["pairs"], w["gamma"], seed=w["seed"]), (
+        f"Chung-Lu n={w['n']} pairs={w['pairs']} gamma {w['gamma']}, permuted, seed {w['seed']}")
+
+
+def frontier_stats(G):
+    """Size of the frontier the hooking sweep visits: vertices not in the
+    sampled giant tree after hooking each vertex with neighbours 0 and 1.
+    The trees of that sampled forest are the components of the sampled edge
+    set, so they are computed here independently with scipy."""
+    import scipy.sparse as sp
+    from scipy.sparse.csgraph import connected_components
+    n = G.n
+    deg = np.diff(G.row_ptr).astype(np.int64)
+    src, dst = [], []
+    for r in (0, 1):
+        has = np.flatnonzero(deg > r)
+        src.append(has)
+        dst.append(G.col_idx[G.row_ptr[has].astype(np.int64) + r].astype(np.int64))
+    src = np.concatenate(src)
+    dst = np.concatenate(dst)
+    A = sp.coo_matrix((np.ones(src.size, dtype=np.int8), (src, dst)), shape=(n, n)).tocsr()
+    _, comp = connected_components(A, directed=False)
+    # the same 1024 sample positions the device uses (SplitMix64 finaliser)
+    i = np.arange(1024, dtype=np.uint64) + np.uint64(1)
+    with np.errstate(over="ignore"):
+        z = i * np.uint64(0x9E3779B97F4A7C15)
+        z = (z ^ (z >> np.uint64(30))) * np.uint64(0xBF58476D1CE4E5B9)
+        z = (z ^ (z >> np.uint64(27))) * np.uint64(0x94D049BB133111EB)
+        z ^= z >> np.uint64(31)
+    pos = (((z >> np.uint64(32)) * np.uint64(n)) >> np.uint64(32)).astype(np.int64)
+    vals, counts = np.unique(comp[pos], return_counts=True)
+    giant = vals[np.argmax(counts)]
+    front = comp != giant
+    rem = np.maximum(deg - 2, 0)
+    active = front & (rem > 0)
+    return dict(frontier_vertices=int(active.sum()), frontier_edges=int(rem[active].sum()),
+                giant_fraction=float((~front).mean()),
+                deg_gt0=int((deg > 0).sum()), deg_gt1=int((deg > 1).sum()))
+
+
+def kernel_bytes(name: str, n: int, st: dict, launches_per_solve: int) -> float | None:
+    """Algorithmic HBM bytes of one launch of kernel `name` (model stated in
+    the design notes): each array element the algorithm must touch, counted
+    once, 4 B per id, 8 B per offset."""
+    if name == "cc_init":
+        return 4.0 * n
+    if name == "cc_hook_sample":
+        # two launches (r=0,1): row_ptr, own parent, one col entry + one parent gather
+        # per vertex that has neighbour r; averaged over the two launches
+        a = (st["deg_gt0"] + st["deg_gt1"]) / 2.0
+        return 8.0 * (n + 1) + 4.0 * n + 8.0 * a
+    if name == "cc_compress":
+        return 12.0 * n
+    if name == "cc_hook_frontier":
+        return 4.0 * n + 16.0 * st["frontier_vertices"] + 8.0 * st["frontier_edges"]
+    return None
+
+
+def cpu_baseline(G, budget_s: float):
+    """The reference's pthreads CC on this host (oracle/_ref), or the oracle
+    restatement if the reference build is absent.  Median of up to 3 timed
+    runs after one warm-up, bounded by budget_s."""
+    import oracle
+    aff = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else os.cpu_count()
+    threads = max(1, min(aff or 1, int(os.environ.get("OMP_NUM_THREADS", "16") or 16), 16))
+    chunk = 8192
+    ref = oracle.ref_lib()
+    kind = "reference" if ref is not None else "port"
+    fn = oracle.ref_cc_pthreads if ref is not None else oracle.cc_lp_pthreads
+    times = []
+    t_start = time.perf_counter()
+    labels = fn(G.row_ptr, G.col_idx, threads, chunk)  # warm-up
+    warm = time.perf_counter() - t_start
+    for _ in range(3):
+        if time.perf_counter() - t_start + warm > budget_s and times:
+            break
+        t0 = time.perf_counter()
+        labels = fn(G.row_ptr, G.col_idx, threads, chunk)
+        times.append(time.perf_counter() - t0)
+    if not times:
+        times = [warm]
+    t = float(np.median(times))
+    cpu_model = ""
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                cpu_model = line.split(":", 1)[1].strip()
+                break
+    except OSError:
+        pass
+    return labels, dict(value=G.m / t / 1e9, unit="GTEPS", cores=threads, kind=kind,
+                        sample=(f"full workload graph, {len(times)} timed run(s) after 1 warm-up, "
+                                f"median {t:.3f} s, chunk {chunk}, {threads} threads on "
+                                f"{cpu_model or 'host CPU'}"),
+                        seconds=t)
+
+
+def main() -> int:
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=50)
+    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--workload", default="rmat22", choices=sorted(WORKLOADS))
+    ap.add_argument("--cpu-budget", type=float, default=30.0,
+                    help="seconds of CPU work for the cpu_baseline leg")
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    args = ap.parse_args()
+
+    if args.gpus != 1:
+        print(f"bench.py: this build runs on one GPU only (asked for --gpus {args.gpus}); "
+              "the multi-GPU path is a design, not a built component.", file=sys.stderr)
+        return 2
+
+    import cc_mpi_amd as cm
+    from cc_mpi_amd import _native as nat
+    lib = nat.gpu_lib()
+    nat.check(lib.cc_hip_init(0), "cc_hip_init")
+
+    t0 = time.perf_counter()
+    G, desc = make_graph(args.workload)
+    log(f"generated {desc}: n={G.n} m={G.m} in {time.perf_counter() - t0:.1f}s")
+
+    t0 = time.perf_counter()
+    D = cm.DeviceGraph(G)
+    h2d_s = time.perf_counter() - t0
+    log(f"uploaded CSR ({(G.row_ptr.nbytes + G.col_idx.nbytes) / 1e9:.2f} GB) in {h2d_s:.2f}s")
+
+    for _ in range(args.warmup):
+        D.solve_async()
+    nat.check(lib.cc_hip_device_sync(), "sync")
+
+    # timed region: K solves back to back on the library stream
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        D.solve_async()
+    nat.check(lib.cc_hip_device_sync(), "sync")
+    elapsed = time.perf_counter() - t0
+    ms = elapsed * 1e3 / args.steps
+    gteps = G.m / (ms / 1e3) / 1e9
+    log(f"{args.steps} solves: {ms:.3f} ms/solve, {gteps:.2f} GTEPS")
+
+    # same K solves again with every kernel bracketed by HIP events on its stream
+    import ctypes
+    lib.cc_hip_prof_reset()
+    lib.cc_hip_prof_enable(1)
+    for _ in range(args.steps):
+        D.solve_async()
+    nat.check(lib.cc_hip_device_sync(), "sync")
+    lib.cc_hip_prof_enable(0)
+    names = [x for x in lib.cc_hip_prof_names().decode().split(",") if x]
+    per = {}
+    for nm in names:
+        cnt, tot = ctypes.c_uint64(), ctypes.c_double()
+        lib.cc_hip_prof_query(nm.encode(), ctypes.byref(cnt), ctypes.byref(tot))
+        per[nm] = dict(launches=int(cnt.value), total_ms=float(tot.value),
+                       avg_us=1e3 * float(tot.value) / max(1, int(cnt.value)))
+    lib.cc_hip_prof_reset()
+    dominant = max(per, key=lambda k: per[k]["total_ms"]) if per else None
+
+    labels = D.solve()
+    D.close()
+
+    st = frontier_stats(G)
+    log(f"frontier: {st}")
+    roofline = None
+    if dominant:
+        b = kernel_bytes(dominant, G.n, st, per[dominant]["launches"] // max(1, args.steps))
+        avg_s = per[dominant]["avg_us"] / 1e6
+        traffic = None
+        tpath = os.path.join(ROOT, "profiles", f"traffic_{args.workload}_{dominant}.json")
+        if os.path.exists(tpath):
+            with open(tpath) as f:
+                traffic = json.load(f).get("bytes_per_launch")
+        if b is not None:
+            achieved = b / avg_s / 1e9
+            roofline = dict(bound="hbm", kernel=dominant, achieved=round(achieved, 1),
+                            peak=HBM_PEAK_GBS, unit="GB/s", frac=round(achieved / HBM_PEAK_GBS, 4),
+                            traffic=traffic, algorithmic_bytes=b,
+                            avg_us=round(per[dominant]["avg_us"], 2))
+
+    cpu = None
+    parity = None
+    if not args.no_cpu_baseline:
+        log("timing the CPU baseline ...")
+        ref_labels, cpu = cpu_baseline(G, args.cpu_budget)
+        parity = bool(np.array_equal(ref_labels, labels))
+        log(f"cpu baseline: {cpu['value']:.4f} GTEPS ({cpu['seconds']:.2f} s/solve, "
+            f"{cpu['cores']} threads); GPU labels equal to the reference's: {parity}")
+        cpu.pop("seconds")
+    else:
+        import oracle
+        parity = bool(np.array_equal(oracle.cc_min_label(G.row_ptr, G.col_idx), labels))
+
+    out = {
+        "metric": "CC GTEPS on 1 MI355X vs host CPU",
+        "value": round(gteps, 4),
+        "unit": "GTEPS",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "u32",
+        "data": "synthetic",
+        "config": {"workload": desc, "n": G.n, "m_directed": G.m,
+                   "teps": "m/t, m = row_ptr[n] directed CSR entries, CSR resident in HBM",
+                   "h2d_seconds": round(h2d_s, 3)},
+        "roofline": roofline,
+        "cpu_baseline": cpu,
+        "kernels": {k: {"launches_per_solve": v["launches"] / args.steps,
+                        "avg_us": round(v["avg_us"], 2)} for k, v in per.items()},
+        "frontier": st,
+        "parity_bit_exact": parity,
+    }
+    print(json.dumps(out), flush=True)
+    return 0 if parity else 1
+
+
+if __name__ == "__main__":
+    sys.exit(main())

@@ -1,0 +1,130 @@
+"""ctypes bindings to the in-tree native libraries.
+
+``libccmi355x.so`` is the HIP library (kernels + C ABI, gfx950 only) and
+``libcchost.so`` the host C library (generators, CSR builder).  Both are built
+in-tree by ``__graft_entry__.build()`` (``make -C cc_mpi_amd/csrc``).  Nothing
+here falls back to a CPU implementation: a missing library raises.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIBDIR = os.path.join(_HERE, "lib")
+
+c_u32 = ctypes.c_uint32
+c_u64 = ctypes.c_uint64
+c_int = ctypes.c_int
+c_void_p = ctypes.c_void_p
+c_size_t = ctypes.c_size_t
+c_double = ctypes.c_double
+P_u32 = ctypes.POINTER(ctypes.c_uint32)
+P_u64 = ctypes.POINTER(ctypes.c_uint64)
+P_int = ctypes.POINTER(ctypes.c_int)
+
+
+class NativeLibraryMissing(RuntimeError):
+    pass
+
+
+class CSRGraphC(ctypes.Structure):
+    """ABI twin of CSRGraph (include/graph.h; reference include/graph.h:8-14)."""
+
+    _fields_ = [
+        ("n", c_u32),
+        ("m", c_u64),
+        ("row_ptr", P_u64),
+        ("col_idx", P_u32),
+    ]
+
+
+def _load(name: str) -> ctypes.CDLL:
+    path = os.path.join(LIBDIR, name)
+    if not os.path.exists(path):
+        raise NativeLibraryMissing(
+            f"{path} is not built; run __graft_entry__.build() (make -C cc_mpi_amd/csrc)"
+        )
+    return ctypes.CDLL(path, mode=ctypes.RTLD_LOCAL)
+
+
+_gpu = None
+_host = None
+
+
+def _sig(lib, name, restype, argtypes):
+    fn = getattr(lib, name)
+    fn.restype = restype
+    fn.argtypes = argtypes
+    return fn
+
+
+def gpu_lib() -> ctypes.CDLL:
+    """The HIP library with argument types declared (loads once)."""
+    global _gpu
+    if _gpu is not None:
+        return _gpu
+    lib = _load("libccmi355x.so")
+    _sig(lib, "cc_hip_init", c_int, [c_int])
+    _sig(lib, "cc_hip_device_count", c_int, [])
+    _sig(lib, "cc_hip_last_error", ctypes.c_char_p, [])
+    _sig(lib, "cc_hip_build_info", ctypes.c_char_p, [])
+    _sig(lib, "cc_hip_malloc", c_int, [ctypes.POINTER(c_void_p), c_size_t])
+    _sig(lib, "cc_hip_free", c_int, [c_void_p])
+    _sig(lib, "cc_hip_host_alloc", c_int, [ctypes.POINTER(c_void_p), c_size_t])
+    _sig(lib, "cc_hip_host_free", c_int, [c_void_p])
+    for nm in ("cc_hip_memcpy_h2d", "cc_hip_memcpy_d2h", "cc_hip_memcpy_d2d"):
+        _sig(lib, nm, c_int, [c_void_p, c_void_p, c_size_t, c_void_p])
+    _sig(lib, "cc_hip_stream_create", c_int, [ctypes.POINTER(c_void_p)])
+    _sig(lib, "cc_hip_stream_destroy", c_int, [c_void_p])
+    _sig(lib, "cc_hip_stream_sync", c_int, [c_void_p])
+    _sig(lib, "cc_hip_device_sync", c_int, [])
+    _sig(lib, "cc_hip_prof_enable", c_int, [c_int])
+    _sig(lib, "cc_hip_prof_reset", c_int, [])
+    _sig(lib, "cc_hip_prof_query", c_int, [ctypes.c_char_p, P_u64, ctypes.POINTER(c_double)])
+    _sig(lib, "cc_hip_prof_names", ctypes.c_char_p, [])
+    _sig(lib, "cc_hip_components_device", c_int,
+         [c_void_p, c_void_p, c_u32, c_u64, c_void_p, c_void_p])
+    _sig(lib, "cc_hip_components_host", c_int, [P_u64, P_u32, c_u32, c_u64, P_u32])
+    _sig(lib, "compute_connected_components_pthreads", None,
+         [ctypes.POINTER(CSRGraphC), P_u32, c_int, c_int])
+    _gpu = lib
+    return lib
+
+
+def host_lib() -> ctypes.CDLL:
+    global _host
+    if _host is not None:
+        return _host
+    lib = _load("libcchost.so")
+    _sig(lib, "free_csr", None, [ctypes.POINTER(CSRGraphC)])
+    _sig(lib, "ccgen_rand_at", c_u64, [c_u64, c_u64])
+    _sig(lib, "ccgen_csr_from_pairs", c_int,
+         [c_u32, P_u32, P_u32, c_u64, P_u32, ctypes.POINTER(CSRGraphC)])
+    _sig(lib, "ccgen_erdos_renyi", c_int, [c_u32, c_u64, c_u64, ctypes.POINTER(CSRGraphC)])
+    _sig(lib, "ccgen_rmat", c_int,
+         [c_int, c_u32, c_double, c_double, c_double, c_u64, c_int, ctypes.POINTER(CSRGraphC)])
+    _sig(lib, "ccgen_grid", c_int,
+         [c_u32, c_u32, c_double, c_u64, c_int, ctypes.POINTER(CSRGraphC)])
+    _sig(lib, "ccgen_chung_lu", c_int,
+         [c_u32, c_u64, c_double, c_u64, c_int, ctypes.POINTER(CSRGraphC)])
+    _host = lib
+    return lib
+
+
+def ptr(a: np.ndarray, ctype=None):
+    """ctypes pointer to a contiguous numpy array's data."""
+    if not a.flags["C_CONTIGUOUS"]:
+        raise ValueError("array must be C-contiguous")
+    if ctype is None:
+        return ctypes.c_void_p(a.ctypes.data)
+    return a.ctypes.data_as(ctypes.POINTER(ctype))
+
+
+def check(rc: int, what: str) -> None:
+    """Raise with the library's message if a C-ABI call failed."""
+    if rc != 0:
+        msg = gpu_lib().cc_hip_last_error().decode(errors="replace")
+        raise RuntimeError(f"{what} failed (rc={rc}): {msg}")

@@ -1,0 +1,303 @@
+// Local connected components on gfx950: labels[v] = smallest vertex id in
+// v's component.
+//
+// Replaces the reference's pthreads label propagation
+// (src/cc_pthreads.c:32-229), which lowers every vertex's label to the
+// minimum over its neighbourhood round after round until nothing changes.
+// The fixed point of that process is unique (the per-component minimum id),
+// so any algorithm reaching the same fixed point is bit-exact with it.  This
+// one is a parent-pointer forest that keeps the invariant p[v] <= v:
+//
+//   1. p[v] = v.
+//   2. Sampled hooking: every vertex hooks with its neighbour 0, pointer
+//      jumping, then with its neighbour 1, pointer jumping.  After this the
+//      forest already holds the giant component of a power-law or random
+//      graph in one tree.
+//   3. The most frequent root among 1024 sampled vertices is taken as the
+//      giant root g.
+//   4. Frontier sweep: every vertex whose parent is not g hooks with all its
+//      remaining neighbours (indices 2..deg-1).  Edges between two vertices
+//      already under g are skipped; an edge between a g-vertex and another
+//      vertex is still seen from the other end because the graph is
+//      symmetric.  Each wave64 takes 64 consecutive vertices, compacts the
+//      active ones with a wave prefix sum of their remaining degrees and
+//      sweeps the concatenated neighbour lists 64 entries at a time, so the
+//      col_idx reads of a wave are contiguous runs regardless of how the
+//      degrees are spread over its lanes.
+//   5. Pointer jumping to the root: labels[v] = root(v).
+//
+// Hooking: for an edge (a, b) with roots ra != rb, let hi = max, lo = min;
+// old = atomicMin(&p[hi], lo).  If old == hi, hi was still a root and is now
+// under lo.  Otherwise hi had meanwhile been hooked elsewhere (old < hi) and
+// the pair (lo, old) is hooked in the next iteration, which keeps every
+// connection the overwritten pointer carried.  max(ra, rb) strictly
+// decreases per iteration, so the loop ends.  Because a pointer only ever
+// moves to a smaller id, each tree's root is its minimum id, hence the final
+// label is the component minimum whatever order the atomics land in.
+
+#include <hip/hip_runtime.h>
+
+#include <mutex>
+#include <unordered_map>
+
+#include "ccx_internal.h"
+#include "../../../include/cc_hip.h"
+#include "../../../include/cc.h"
+
+namespace {
+
+constexpr int kBlock = 256;
+constexpr int kWave = 64;
+constexpr int kSampledNeighbours = 2;
+constexpr int kGiantSamples = 1024;
+
+// Root of x with path halving.  Plain loads may return stale parents; a
+// stale parent is still an ancestor (pointers only decrease), so the walk
+// still ends at a root or at a vertex that the caller's atomicMin will reveal
+// as hooked.  The halving store only ever writes a grandparent.
+__device__ __forceinline__ uint32_t find_root(uint32_t x, uint32_t *__restrict__ p) {
+  uint32_t px = p[x];
+  while (px != x) {
+    uint32_t ppx = p[px];
+    if (ppx == px) return px;
+    p[x] = ppx;
+    x = ppx;
+    px = p[x];
+  }
+  return x;
+}
+
+__device__ __forceinline__ void hook(uint32_t a, uint32_t b, uint32_t *__restrict__ p) {
+  uint32_t ra = find_root(a, p);
+  uint32_t rb = find_root(b, p);
+  while (ra != rb) {
+    const uint32_t hi = ra > rb ? ra : rb;
+    const uint32_t lo = ra > rb ? rb : ra;
+    const uint32_t old = atomicMin(&p[hi], lo);
+    if (old == hi || old == lo) return;
+    ra = find_root(old, p);
+    rb = find_root(lo, p);
+  }
+}
+
+__global__ __launch_bounds__(kBlock) void k_init(uint32_t n, uint32_t *__restrict__ p) {
+  const uint32_t v = blockIdx.x * kBlock + threadIdx.x;
+  if (v < n) p[v] = v;
+}
+
+// Hook every vertex with its neighbour number r (if it has one).
+__global__ __launch_bounds__(kBlock) void k_hook_neighbour(const uint64_t *__restrict__ row_ptr,
+                                                           const uint32_t *__restrict__ col,
+                                                           uint32_t n, uint32_t r,
+                                                           uint32_t *__restrict__ p) {
+  const uint32_t v = blockIdx.x * kBlock + threadIdx.x;
+  if (v >= n) return;
+  const uint64_t b = row_ptr[v] + r;
+  if (b >= row_ptr[v + 1]) return;
+  const uint32_t w = col[b];
+  if (w < n) hook(v, w, p);
+}
+
+// Pointer jumping: p[v] = root(v).  Run with no hooks in flight.
+__global__ __launch_bounds__(kBlock) void k_compress(uint32_t n, uint32_t *__restrict__ p) {
+  const uint32_t v = blockIdx.x * kBlock + threadIdx.x;
+  if (v >= n) return;
+  uint32_t r = p[v];
+  if (r == v) return;
+  uint32_t rr = p[r];
+  while (rr != r) {
+    r = rr;
+    rr = p[r];
+  }
+  p[v] = r;
+}
+
+__device__ __forceinline__ uint32_t sample_vertex(uint32_t i, uint32_t n) {
+  // SplitMix64 finaliser of the sample index, reduced to [0, n).
+  uint64_t z = (uint64_t)(i + 1) * 0x9E3779B97F4A7C15ull;
+  z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+  z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+  z ^= z >> 31;
+  return (uint32_t)(((z >> 32) * (uint64_t)n) >> 32);
+}
+
+// One workgroup: the most frequent root among kGiantSamples sampled
+// vertices (ties: the larger run found first in sorted order).
+__global__ __launch_bounds__(kBlock) void k_sample_giant(uint32_t n, const uint32_t *__restrict__ p,
+                                                         uint32_t *__restrict__ giant) {
+  __shared__ uint32_t s[kGiantSamples];
+  __shared__ uint64_t best[kBlock];
+  const int t = threadIdx.x;
+  for (int i = t; i < kGiantSamples; i += kBlock) s[i] = p[sample_vertex(i, n)];
+  __syncthreads();
+  // Bitonic sort of the 1024 samples in LDS.
+  for (int k = 2; k <= kGiantSamples; k <<= 1) {
+    for (int j = k >> 1; j > 0; j >>= 1) {
+      for (int q = t; q < kGiantSamples / 2; q += kBlock) {
+        const int i = (q / j) * 2 * j + (q % j);
+        const int l = i + j;
+        const bool up = (i & k) == 0;
+        const uint32_t a = s[i], b = s[l];
+        if ((a > b) == up) {
+          s[i] = b;
+          s[l] = a;
+        }
+      }
+      __syncthreads();
+    }
+  }
+  uint64_t mine = 0;
+  for (int i = t * (kGiantSamples / kBlock); i < (t + 1) * (kGiantSamples / kBlock); ++i) {
+    if (i > 0 && s[i] == s[i - 1]) continue;
+    uint32_t len = 1;
+    while (i + (int)len < kGiantSamples && s[i + len] == s[i]) ++len;
+    const uint64_t cand = ((uint64_t)len << 32) | s[i];
+    if ((cand >> 32) > (mine >> 32)) mine = cand;
+  }
+  best[t] = mine;
+  __syncthreads();
+  for (int w = kBlock / 2; w > 0; w >>= 1) {
+    if (t < w && (best[t + w] >> 32) > (best[t] >> 32)) best[t] = best[t + w];
+    __syncthreads();
+  }
+  if (t == 0) giant[0] = (uint32_t)best[0];
+}
+
+// Frontier sweep over the vertices not under the giant root.
+__global__ __launch_bounds__(kBlock) void k_hook_frontier(const uint64_t *__restrict__ row_ptr,
+                                                          const uint32_t *__restrict__ col,
+                                                          uint32_t n, uint32_t *__restrict__ p,
+                                                          const uint32_t *__restrict__ giant_slot) {
+  __shared__ uint64_t s_excl[kBlock / kWave][kWave];
+  __shared__ uint64_t s_begin[kBlock / kWave][kWave];
+  const int lane = threadIdx.x & (kWave - 1);
+  const int wid = threadIdx.x / kWave;
+  const uint32_t giant = giant_slot[0];
+  const uint64_t base = (uint64_t)blockIdx.x * kBlock + (uint64_t)wid * kWave;
+  const uint64_t v = base + lane;
+
+  uint64_t work = 0, begin = 0;
+  if (v < n && p[v] != giant) {
+    const uint64_t b = row_ptr[v] + kSampledNeighbours;
+    const uint64_t e = row_ptr[v + 1];
+    if (e > b) {
+      work = e - b;
+      begin = b;
+    }
+  }
+  // Wave-inclusive prefix sum of the remaining degrees.
+  uint64_t incl = work;
+#pragma unroll
+  for (int d = 1; d < kWave; d <<= 1) {
+    const uint64_t up = __shfl_up(incl, d, kWave);
+    if (lane >= d) incl += up;
+  }
+  const uint64_t total = __shfl(incl, kWave - 1, kWave);
+  if (total == 0) return;
+  s_excl[wid][lane] = incl - work;
+  s_begin[wid][lane] = begin;
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+
+  const uint64_t *ex = s_excl[wid];
+  for (uint64_t e = lane; e < total; e += kWave) {
+    int j = 0;
+#pragma unroll
+    for (int step = kWave / 2; step > 0; step >>= 1)
+      if (ex[j + step] <= e) j += step;
+    const uint64_t idx = s_begin[wid][j] + (e - ex[j]);
+    const uint32_t w = col[idx];
+    if (w < n) hook((uint32_t)(base + j), w, p);
+  }
+}
+
+std::mutex g_scratch_mu;
+std::unordered_map<hipStream_t, uint32_t *> g_scratch;
+
+// A few device words per stream, so solves on different streams never share
+// their sampled-root slot.
+uint32_t *stream_scratch(hipStream_t s) {
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  auto it = g_scratch.find(s);
+  if (it != g_scratch.end()) return it->second;
+  uint32_t *d = nullptr;
+  if (hipMalloc(&d, 256) != hipSuccess) return nullptr;
+  g_scratch[s] = d;
+  return d;
+}
+
+inline unsigned blocks_for(uint64_t n) { return (unsigned)((n + kBlock - 1) / kBlock); }
+
+}  // namespace
+
+extern "C" int cc_hip_components_device(const uint64_t *d_row_ptr, const uint32_t *d_col_idx,
+                                        uint32_t n, uint64_t m, uint32_t *d_labels, void *stream) {
+  ccx::Context *c = ccx::ctx();
+  if (!c) return 2;
+  if (n == 0) return 0;
+  if (!d_row_ptr || !d_labels || (m > 0 && !d_col_idx))
+    return ccx::set_error(4, "cc_hip_components_device: null pointer");
+  hipStream_t s = ccx::pick_stream(stream);
+  uint32_t *giant = stream_scratch(s);
+  if (!giant) return ccx::set_error(3, "cc_hip_components_device: scratch allocation failed");
+  const unsigned nb = blocks_for(n);
+
+  CCX_LAUNCH("cc_init", s, k_init<<<nb, kBlock, 0, s>>>(n, d_labels));
+  if (m > 0) {
+    for (uint32_t r = 0; r < (uint32_t)kSampledNeighbours; ++r) {
+      CCX_LAUNCH("cc_hook_sample", s,
+                 k_hook_neighbour<<<nb, kBlock, 0, s>>>(d_row_ptr, d_col_idx, n, r, d_labels));
+      CCX_LAUNCH("cc_compress", s, k_compress<<<nb, kBlock, 0, s>>>(n, d_labels));
+    }
+    CCX_LAUNCH("cc_sample_giant", s, k_sample_giant<<<1, kBlock, 0, s>>>(n, d_labels, giant));
+    CCX_LAUNCH("cc_hook_frontier", s,
+               k_hook_frontier<<<nb, kBlock, 0, s>>>(d_row_ptr, d_col_idx, n, d_labels, giant));
+    CCX_LAUNCH("cc_compress", s, k_compress<<<nb, kBlock, 0, s>>>(n, d_labels));
+  }
+  return 0;
+}
+
+extern "C" int cc_hip_components_host(const uint64_t *row_ptr, const uint32_t *col_idx, uint32_t n,
+                                      uint64_t m, uint32_t *labels) {
+  ccx::Context *c = ccx::ctx();
+  if (!c) return 2;
+  if (n == 0) return 0;
+  if (!row_ptr || !labels || (m > 0 && !col_idx))
+    return ccx::set_error(4, "cc_hip_components_host: null pointer");
+  if (row_ptr[n] != m)
+    return ccx::set_error(4, "cc_hip_components_host: row_ptr[n]=%llu but m=%llu",
+                          (unsigned long long)row_ptr[n], (unsigned long long)m);
+  const size_t rp_bytes = ((size_t)n + 1) * sizeof(uint64_t);
+  const size_t col_bytes = (size_t)m * sizeof(uint32_t);
+  const size_t lab_bytes = (size_t)n * sizeof(uint32_t);
+  const size_t off_col = (rp_bytes + 255) & ~(size_t)255;
+  const size_t off_lab = (off_col + col_bytes + 255) & ~(size_t)255;
+  hipStream_t s = c->stream;
+  char *ws = (char *)ccx::workspace(off_lab + lab_bytes, s);
+  if (!ws) return 3;
+  uint64_t *d_rp = (uint64_t *)ws;
+  uint32_t *d_col = (uint32_t *)(ws + off_col);
+  uint32_t *d_lab = (uint32_t *)(ws + off_lab);
+  CCX_TRY(hipMemcpyAsync(d_rp, row_ptr, rp_bytes, hipMemcpyHostToDevice, s));
+  if (m > 0) CCX_TRY(hipMemcpyAsync(d_col, col_idx, col_bytes, hipMemcpyHostToDevice, s));
+  int rc = cc_hip_components_device(d_rp, d_col, n, m, d_lab, s);
+  if (rc) return rc;
+  CCX_TRY(hipMemcpyAsync(labels, d_lab, lab_bytes, hipMemcpyDeviceToHost, s));
+  CCX_TRY(hipStreamSynchronize(s));
+  return 0;
+}
+
+// Drop-in for include/cc.h:27-28 (reference src/cc_pthreads.c:152-229).
+extern "C" void compute_connected_components_pthreads(const CSRGraph *CC_RESTRICT G,
+                                                      uint32_t *CC_RESTRICT labels, int num_threads,
+                                                      int chunk_size) {
+  (void)num_threads;
+  (void)chunk_size;
+  if (!G || !labels || G->n == 0) return;
+  const int rc = cc_hip_components_host(G->row_ptr, G->col_idx, G->n, G->m, labels);
+  if (rc != 0) {
+    fprintf(stderr, "compute_connected_components_pthreads: %s\n", cc_hip_last_error());
+    exit(EXIT_FAILURE);
+  }
+}

@@ -1,0 +1,315 @@
+// Runtime layer of libccmi355x.so: device context, errors, memory helpers
+// and per-kernel HIP-event timing.  One context per process (device chosen
+// by cc_hip_init or device 0), one library stream at default priority.
+
+#include <hip/hip_runtime.h>
+
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <map>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "ccx_internal.h"
+#include "../../../include/cc_hip.h"
+
+namespace ccx {
+
+static thread_local char g_err[512];
+
+int set_error(int code, const char *fmt, ...) {
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(g_err, sizeof(g_err), fmt, ap);
+  va_end(ap);
+  return code;
+}
+
+void clear_error() { g_err[0] = '\0'; }
+
+static std::mutex g_ctx_mu;
+static Context g_ctx;
+static bool g_ctx_ok = false;
+
+static int init_locked(int device) {
+  if (g_ctx_ok) {
+    if (device >= 0 && device != g_ctx.device)
+      return set_error(2, "cc_hip_init: already initialised on device %d", g_ctx.device);
+    return 0;
+  }
+  int count = 0;
+  hipError_t e = hipGetDeviceCount(&count);
+  if (e != hipSuccess || count <= 0)
+    return set_error(2, "no HIP device available (%s)",
+                     e == hipSuccess ? "count=0" : hipGetErrorString(e));
+  if (device < 0) device = 0;
+  if (device >= count) return set_error(2, "device %d out of range (%d devices)", device, count);
+  CCX_TRY(hipSetDevice(device));
+  hipDeviceProp_t prop;
+  CCX_TRY(hipGetDeviceProperties(&prop, device));
+  if (std::strncmp(prop.gcnArchName, "gfx950", 6) != 0)
+    return set_error(2, "device %d is %s; this library is built for gfx950 only", device,
+                     prop.gcnArchName);
+  g_ctx.device = device;
+  g_ctx.cu_count = prop.multiProcessorCount;
+  CCX_TRY(hipStreamCreateWithFlags(&g_ctx.stream, hipStreamNonBlocking));
+  CCX_TRY(hipMalloc(&g_ctx.d_small, 4096));
+  CCX_TRY(hipMemset(g_ctx.d_small, 0, 4096));
+  CCX_TRY(hipHostMalloc(&g_ctx.h_small, 4096, hipHostMallocDefault));
+  g_ctx_ok = true;
+  return 0;
+}
+
+Context *ctx() {
+  if (g_ctx_ok) {
+    // Every call runs on the context's device, whatever the caller set.
+    int cur = -1;
+    if (hipGetDevice(&cur) == hipSuccess && cur != g_ctx.device) (void)hipSetDevice(g_ctx.device);
+    return &g_ctx;
+  }
+  std::lock_guard<std::mutex> lk(g_ctx_mu);
+  if (init_locked(-1) != 0) return nullptr;
+  return &g_ctx;
+}
+
+void *workspace(size_t bytes, hipStream_t s) {
+  Context *c = ctx();
+  if (!c) return nullptr;
+  if (c->ws_stream && c->ws_stream != s) {
+    if (hipStreamSynchronize(c->ws_stream) != hipSuccess) {
+      set_error(1, "workspace: synchronize of previous stream failed");
+      return nullptr;
+    }
+  }
+  c->ws_stream = s;
+  if (bytes <= c->ws_bytes) return c->ws;
+  if (c->ws) {
+    // Pending work may still use the old buffer.
+    if (hipDeviceSynchronize() != hipSuccess) {
+      set_error(1, "workspace: synchronize failed");
+      return nullptr;
+    }
+    (void)hipFree(c->ws);
+    c->ws = nullptr;
+    c->ws_bytes = 0;
+  }
+  size_t want = bytes + bytes / 8 + 4096;
+  if (hipMalloc(&c->ws, want) != hipSuccess) {
+    c->ws = nullptr;
+    set_error(3, "workspace: hipMalloc of %zu bytes failed", want);
+    return nullptr;
+  }
+  c->ws_bytes = want;
+  return c->ws;
+}
+
+hipStream_t pick_stream(void *stream) {
+  if (stream) return (hipStream_t)stream;
+  Context *c = ctx();
+  return c ? c->stream : nullptr;
+}
+
+// ---- kernel timing -------------------------------------------------------
+
+struct PendingPair {
+  std::string name;
+  hipEvent_t a, b;
+};
+static std::mutex g_prof_mu;
+static bool g_prof_on = false;
+static std::vector<PendingPair> g_pending;
+static std::vector<hipEvent_t> g_event_pool;
+static std::map<std::string, std::pair<uint64_t, double>> g_prof_acc;
+static thread_local PendingPair *g_open = nullptr;
+static thread_local PendingPair g_open_slot;
+static std::string g_names;
+
+static hipEvent_t take_event() {
+  if (!g_event_pool.empty()) {
+    hipEvent_t e = g_event_pool.back();
+    g_event_pool.pop_back();
+    return e;
+  }
+  hipEvent_t e = nullptr;
+  if (hipEventCreate(&e) != hipSuccess) return nullptr;
+  return e;
+}
+
+void prof_begin(const char *name, hipStream_t s) {
+  if (!g_prof_on) return;
+  std::lock_guard<std::mutex> lk(g_prof_mu);
+  g_open_slot.name = name;
+  g_open_slot.a = take_event();
+  g_open_slot.b = take_event();
+  if (!g_open_slot.a || !g_open_slot.b) return;
+  (void)hipEventRecord(g_open_slot.a, s);
+  g_open = &g_open_slot;
+}
+
+void prof_end(hipStream_t s) {
+  if (!g_prof_on || !g_open) return;
+  std::lock_guard<std::mutex> lk(g_prof_mu);
+  (void)hipEventRecord(g_open->b, s);
+  g_pending.push_back(*g_open);
+  g_open = nullptr;
+}
+
+static void prof_collect_locked() {
+  for (auto &p : g_pending) {
+    (void)hipEventSynchronize(p.b);
+    float ms = 0.f;
+    if (hipEventElapsedTime(&ms, p.a, p.b) == hipSuccess) {
+      auto &acc = g_prof_acc[p.name];
+      acc.first += 1;
+      acc.second += ms;
+    }
+    g_event_pool.push_back(p.a);
+    g_event_pool.push_back(p.b);
+  }
+  g_pending.clear();
+}
+
+}  // namespace ccx
+
+using namespace ccx;
+
+extern "C" {
+
+int cc_hip_init(int device) {
+  std::lock_guard<std::mutex> lk(g_ctx_mu);
+  clear_error();
+  return init_locked(device);
+}
+
+int cc_hip_device_count(void) {
+  int n = 0;
+  if (hipGetDeviceCount(&n) != hipSuccess) return 0;
+  return n;
+}
+
+const char *cc_hip_last_error(void) { return g_err; }
+
+const char *cc_hip_build_info(void) {
+  return "libccmi355x: gfx950, wave64, HIP " __DATE__;
+}
+
+int cc_hip_malloc(void **d_ptr, size_t bytes) {
+  if (!ctx()) return 2;
+  if (!d_ptr) return set_error(4, "cc_hip_malloc: null out pointer");
+  *d_ptr = nullptr;
+  if (bytes == 0) bytes = 1;
+  CCX_TRY(hipMalloc(d_ptr, bytes));
+  return 0;
+}
+
+int cc_hip_free(void *d_ptr) {
+  if (!d_ptr) return 0;
+  if (!ctx()) return 2;
+  CCX_TRY(hipFree(d_ptr));
+  return 0;
+}
+
+int cc_hip_host_alloc(void **h_ptr, size_t bytes) {
+  if (!ctx()) return 2;
+  if (!h_ptr) return set_error(4, "cc_hip_host_alloc: null out pointer");
+  *h_ptr = nullptr;
+  if (bytes == 0) bytes = 1;
+  CCX_TRY(hipHostMalloc(h_ptr, bytes, hipHostMallocDefault));
+  return 0;
+}
+
+int cc_hip_host_free(void *h_ptr) {
+  if (!h_ptr) return 0;
+  if (!ctx()) return 2;
+  CCX_TRY(hipHostFree(h_ptr));
+  return 0;
+}
+
+int cc_hip_memcpy_h2d(void *d_dst, const void *src, size_t bytes, void *stream) {
+  if (!ctx()) return 2;
+  if (bytes == 0) return 0;
+  hipStream_t s = pick_stream(stream);
+  CCX_TRY(hipMemcpyAsync(d_dst, src, bytes, hipMemcpyHostToDevice, s));
+  CCX_TRY(hipStreamSynchronize(s));
+  return 0;
+}
+
+int cc_hip_memcpy_d2h(void *dst, const void *d_src, size_t bytes, void *stream) {
+  if (!ctx()) return 2;
+  if (bytes == 0) return 0;
+  hipStream_t s = pick_stream(stream);
+  CCX_TRY(hipMemcpyAsync(dst, d_src, bytes, hipMemcpyDeviceToHost, s));
+  CCX_TRY(hipStreamSynchronize(s));
+  return 0;
+}
+
+int cc_hip_memcpy_d2d(void *d_dst, const void *d_src, size_t bytes, void *stream) {
+  if (!ctx()) return 2;
+  if (bytes == 0) return 0;
+  CCX_TRY(hipMemcpyAsync(d_dst, d_src, bytes, hipMemcpyDeviceToDevice, pick_stream(stream)));
+  return 0;
+}
+
+int cc_hip_stream_create(void **stream) {
+  if (!ctx()) return 2;
+  hipStream_t s = nullptr;
+  CCX_TRY(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
+  *stream = (void *)s;
+  return 0;
+}
+
+int cc_hip_stream_destroy(void *stream) {
+  if (!stream) return 0;
+  if (!ctx()) return 2;
+  CCX_TRY(hipStreamDestroy((hipStream_t)stream));
+  return 0;
+}
+
+int cc_hip_stream_sync(void *stream) {
+  if (!ctx()) return 2;
+  CCX_TRY(hipStreamSynchronize(pick_stream(stream)));
+  return 0;
+}
+
+int cc_hip_device_sync(void) {
+  if (!ctx()) return 2;
+  CCX_TRY(hipDeviceSynchronize());
+  return 0;
+}
+
+int cc_hip_prof_enable(int on) {
+  std::lock_guard<std::mutex> lk(g_prof_mu);
+  g_prof_on = (on != 0);
+  return 0;
+}
+
+int cc_hip_prof_reset(void) {
+  std::lock_guard<std::mutex> lk(g_prof_mu);
+  prof_collect_locked();
+  g_prof_acc.clear();
+  return 0;
+}
+
+int cc_hip_prof_query(const char *name, uint64_t *launches, double *total_ms) {
+  std::lock_guard<std::mutex> lk(g_prof_mu);
+  prof_collect_locked();
+  auto it = g_prof_acc.find(name ? name : "");
+  if (launches) *launches = (it == g_prof_acc.end()) ? 0 : it->second.first;
+  if (total_ms) *total_ms = (it == g_prof_acc.end()) ? 0.0 : it->second.second;
+  return 0;
+}
+
+const char *cc_hip_prof_names(void) {
+  std::lock_guard<std::mutex> lk(g_prof_mu);
+  prof_collect_locked();
+  g_names.clear();
+  for (auto &kv : g_prof_acc) {
+    if (!g_names.empty()) g_names += ",";
+    g_names += kv.first;
+  }
+  return g_names.c_str();
+}
+
+}  // extern "C"

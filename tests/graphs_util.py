@@ -1,0 +1,63 @@
+"""Small hand-built graphs for edge cases (shared by CPU and GPU tests)."""
+import numpy as np
+
+from cc_mpi_amd.graphs import Graph, from_pairs
+
+
+def csr_raw(n, adj):
+    """CSR exactly as given (no symmetrisation/dedup): adj is a list of rows."""
+    rp = np.zeros(n + 1, dtype=np.uint64)
+    for i, row in enumerate(adj):
+        rp[i + 1] = rp[i] + len(row)
+    ci = np.array([c for row in adj for c in row], dtype=np.uint32)
+    return Graph(rp, ci, name="raw")
+
+
+def path(n, perm_seed=None):
+    src = np.arange(n - 1, dtype=np.uint32)
+    dst = src + 1
+    perm = None
+    if perm_seed is not None:
+        perm = np.random.default_rng(perm_seed).permutation(n).astype(np.uint32)
+    return from_pairs(n, src, dst, perm=perm, name=f"path-{n}")
+
+
+def star(n, centre=0):
+    leaves = np.array([v for v in range(n) if v != centre], dtype=np.uint32)
+    return from_pairs(n, np.full(leaves.size, centre, dtype=np.uint32), leaves, name=f"star-{n}")
+
+
+def complete(n):
+    s, d = np.triu_indices(n, 1)
+    return from_pairs(n, s.astype(np.uint32), d.astype(np.uint32), name=f"K{n}")
+
+
+def disjoint_cliques(k, size, seed=0):
+    """k cliques of `size` vertices with ids shuffled over [0, k*size)."""
+    n = k * size
+    perm = np.random.default_rng(seed).permutation(n).astype(np.uint32)
+    s, d = np.triu_indices(size, 1)
+    src = np.concatenate([s + i * size for i in range(k)]).astype(np.uint32)
+    dst = np.concatenate([d + i * size for i in range(k)]).astype(np.uint32)
+    return from_pairs(n, src, dst, perm=perm, name=f"{k}xK{size}")
+
+
+def bfs_min_labels(G):
+    """Brute force: BFS from each unvisited vertex in id order (so the
+    starting vertex is the component minimum)."""
+    n = G.n
+    lab = np.full(n, 0xFFFFFFFF, dtype=np.uint32)
+    rp, ci = G.row_ptr, G.col_idx
+    for s in range(n):
+        if lab[s] != 0xFFFFFFFF:
+            continue
+        lab[s] = s
+        stack = [s]
+        while stack:
+            u = stack.pop()
+            for j in range(int(rp[u]), int(rp[u + 1])):
+                w = int(ci[j])
+                if w < n and lab[w] == 0xFFFFFFFF:
+                    lab[w] = s
+                    stack.append(w)
+    return lab

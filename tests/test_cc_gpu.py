@@ -1,0 +1,116 @@
+"""Parity of the HIP local-CC path with the oracle (bit-exact labels).
+
+Runs through the C ABI: the drop-in compute_connected_components_pthreads
+(host buffers) and cc_hip_components_device (CSR resident in HBM).
+"""
+import ctypes
+
+import numpy as np
+import pytest
+
+import cc_mpi_amd as cm
+from cc_mpi_amd import _native as nat
+import oracle
+from graphs_util import complete, csr_raw, disjoint_cliques, path, star
+
+pytestmark = pytest.mark.gpu
+
+
+def check_labels(G, got):
+    want = oracle.cc_min_label(G.row_ptr, G.col_idx)
+    bad = np.flatnonzero(got != want)
+    assert bad.size == 0, f"{G.name}: {bad.size}/{G.n} labels differ, first at {bad[:5]}"
+    return want
+
+
+def dropin(G):
+    lab = np.full(G.n, 0xDEADBEEF, dtype=np.uint32)
+    Gc = G.as_c()
+    nat.gpu_lib().compute_connected_components_pthreads(ctypes.byref(Gc),
+                                                        nat.ptr(lab, ctypes.c_uint32), 8, 8192)
+    return lab
+
+
+@pytest.mark.parametrize("make", [
+    lambda: path(2), lambda: path(1000), lambda: path(5000, perm_seed=1),
+    lambda: star(3000, centre=2999), lambda: star(3000, centre=0), lambda: complete(64),
+    lambda: disjoint_cliques(300, 7, seed=3),
+])
+def test_small_structured(gpu, make):
+    G = make()
+    check_labels(G, dropin(G))
+
+
+def test_edge_cases(gpu):
+    # single vertex, no edges, isolated vertices
+    G = csr_raw(1, [[]])
+    assert dropin(G).tolist() == [0]
+    G = csr_raw(5, [[], [], [], [], []])
+    assert dropin(G).tolist() == [0, 1, 2, 3, 4]
+    # non-simple but symmetric CSR: duplicates, self-loops, unsorted rows;
+    # entries >= n are ignored
+    G = csr_raw(6, [[1, 1, 0], [0, 0, 1], [77], [4], [3, 3, 4], [5]])
+    assert dropin(G).tolist() == [0, 0, 2, 3, 3, 5]
+    G = csr_raw(4, [[3, 2], [2], [1, 0], [0]])
+    assert dropin(G).tolist() == [0, 0, 0, 0]
+
+
+def test_empty_graph(gpu):
+    G = cm.Graph(np.zeros(1, dtype=np.uint64), np.zeros(0, dtype=np.uint32))
+    assert cm.compute_connected_components_pthreads(G).size == 0
+
+
+@pytest.mark.parametrize("seed", [1, 2, 3])
+def test_er_64k(gpu, seed):
+    # Erdos-Renyi 65,536 vertices, average degree 8 (the reference's CPU case)
+    G = cm.erdos_renyi(65536, 8.0, seed=1000 + seed)
+    check_labels(G, dropin(G))
+
+
+@pytest.mark.parametrize("avg", [0.5, 1.0, 2.0])
+def test_er_near_threshold(gpu, avg):
+    # many mid-sized components around the giant-component threshold
+    G = cm.erdos_renyi(200000, avg, seed=17)
+    check_labels(G, dropin(G))
+
+
+@pytest.mark.parametrize("scale", [10, 14, 18])
+def test_rmat(gpu, scale):
+    G = cm.rmat(scale, 16, seed=1002)
+    check_labels(G, dropin(G))
+
+
+@pytest.mark.parametrize("keep", [0.3, 0.5, 0.7, 1.0])
+def test_grid(gpu, keep):
+    G = cm.grid(300, 400, keep, seed=1004)
+    check_labels(G, dropin(G))
+
+
+def test_chung_lu(gpu):
+    G = cm.chung_lu(1 << 18, 3 << 19, 2.1, seed=1003)
+    check_labels(G, dropin(G))
+
+
+def test_device_resident_repeat(gpu):
+    # same labels on repeated solves of a resident graph, and equal to the oracle
+    G = cm.rmat(16, 16, seed=5)
+    D = cm.DeviceGraph(G)
+    first = D.solve()
+    check_labels(G, first)
+    for _ in range(5):
+        assert np.array_equal(D.solve(), first)
+    D.close()
+
+
+def test_rmat22_full_size(gpu):
+    # R-MAT scale 22, edge factor 16, (0.57,0.19,0.19,0.05), permuted ids
+    G = cm.rmat(22, 16, seed=1002)
+    D = cm.DeviceGraph(G)
+    got = D.solve()
+    D.close()
+    want = check_labels(G, got)
+    # component count and size histogram agree
+    roots = np.flatnonzero(want == np.arange(G.n))
+    assert np.count_nonzero(got == np.arange(G.n)) == roots.size
+    assert np.array_equal(np.bincount(got, minlength=G.n)[roots],
+                          np.bincount(want, minlength=G.n)[roots])

@@ -1,0 +1,99 @@
+"""CPU-side checks of the native libraries: every symbol declared in
+include/*.h is exported by the library that implements it, and the host
+generators produce well-formed, deterministic inputs."""
+import ctypes
+import os
+import re
+import subprocess
+
+import numpy as np
+import pytest
+
+import cc_mpi_amd as cm
+from cc_mpi_amd import _native as nat
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+# header -> library that must export every function it declares
+HEADER_LIBS = {
+    "cc.h": "libccmi355x.so",
+    "cc_hip.h": "libccmi355x.so",
+    "graph.h": "libcchost.so",
+    "graphgen.h": "libcchost.so",
+}
+
+
+def declared_functions(header):
+    text = open(os.path.join(ROOT, "include", header)).read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    text = re.sub(r"//[^\n]*", "", text)
+    text = re.sub(r"#[^\n]*", "", text)
+    names = re.findall(r"\b([A-Za-z_][A-Za-z0-9_]*)\s*\([^;{)]*(?:\([^)]*\)[^;{)]*)*\)\s*;", text)
+    return sorted(set(n for n in names if n not in ("if", "while", "for", "return", "sizeof")))
+
+
+def exported(lib):
+    out = subprocess.run(["nm", "-D", "--defined-only", os.path.join(nat.LIBDIR, lib)],
+                         capture_output=True, text=True, check=True).stdout
+    return {line.split()[-1] for line in out.splitlines() if line.strip()}
+
+
+@pytest.mark.parametrize("header,lib", sorted(HEADER_LIBS.items()))
+def test_header_symbols_exported(header, lib):
+    names = declared_functions(header)
+    assert names, f"no declarations parsed from {header}"
+    missing = [n for n in names if n not in exported(lib)]
+    assert not missing, f"{lib} lacks {missing} declared in include/{header}"
+
+
+def test_libraries_load_without_gpu():
+    lib = nat.gpu_lib()
+    assert lib.cc_hip_build_info().startswith(b"libccmi355x")
+    nat.host_lib()
+    # the offload bundle of this library (built here) targets gfx950
+    blob = open(os.path.join(nat.LIBDIR, "libccmi355x.so"), "rb").read()
+    assert b"amdgcn-amd-amdhsa--gfx950" in blob
+
+
+def test_no_device_reports_error_not_fallback():
+    lib = nat.gpu_lib()
+    if lib.cc_hip_device_count() > 0:
+        pytest.skip("a GPU is visible")
+    G = cm.rmat(8, 4, seed=1)
+    with pytest.raises(RuntimeError, match="no HIP device"):
+        cm.compute_connected_components_pthreads(G)
+
+
+def well_formed(G):
+    rp, ci = G.row_ptr, G.col_idx
+    assert rp[0] == 0 and (np.diff(rp.astype(np.int64)) >= 0).all() and rp[-1] == ci.size
+    src = np.repeat(np.arange(G.n, dtype=np.uint32), np.diff(rp).astype(np.int64))
+    assert (ci < G.n).all() and (src != ci).all()
+    key = (src.astype(np.uint64) << np.uint64(32)) | ci
+    assert (np.diff(key.astype(np.uint64)) > 0).all()  # sorted rows, no duplicates
+    rev = (ci.astype(np.uint64) << np.uint64(32)) | src
+    assert np.array_equal(np.sort(rev), key)  # symmetric
+
+
+@pytest.mark.parametrize("make", [
+    lambda: cm.erdos_renyi(5000, 8.0, seed=1),
+    lambda: cm.rmat(12, 16, seed=2),
+    lambda: cm.grid(50, 70, 0.7, seed=3),
+    lambda: cm.chung_lu(5000, 20000, 2.1, seed=4),
+])
+def test_generators_well_formed_and_deterministic(make):
+    a, b = make(), make()
+    well_formed(a)
+    assert np.array_equal(a.row_ptr, b.row_ptr) and np.array_equal(a.col_idx, b.col_idx)
+
+
+def test_grid_structure():
+    G = cm.grid(4, 5, 1.0, seed=0, permute=False)
+    assert G.m == 2 * (4 * 4 + 3 * 5)
+    G = cm.grid(4, 5, 0.0, seed=0, permute=False)
+    assert G.m == 0
+
+
+def test_rmat_sizes():
+    G = cm.rmat(10, 16, seed=1002)
+    assert G.n == 1024 and G.m <= 2 * 16 * 1024

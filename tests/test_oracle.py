@@ -1,0 +1,238 @@
+"""Pins the oracle (CPU restatement) before it is trusted as the checker.
+
+Each oracle function is checked against something independent of its own
+formula: the reference itself compiled from its sources (oracle/_ref), a
+brute-force search, numpy/scipy library routines, or closed-form facts.
+"""
+import numpy as np
+import pytest
+import scipy.sparse as sp
+from scipy.sparse.csgraph import connected_components
+
+import cc_mpi_amd as cm
+import oracle
+from graphs_util import bfs_min_labels, complete, csr_raw, disjoint_cliques, path, star
+
+SMALL = [
+    lambda: path(50),
+    lambda: path(300, perm_seed=3),
+    lambda: star(40, centre=17),
+    lambda: complete(12),
+    lambda: disjoint_cliques(9, 5, seed=2),
+    lambda: cm.erdos_renyi(500, 1.2, seed=4),
+    lambda: cm.grid(20, 20, 0.55, seed=8),
+    lambda: cm.rmat(9, 4, seed=1),
+]
+
+
+def scipy_min_labels(G):
+    n = G.n
+    A = sp.csr_matrix((np.ones(G.m, dtype=np.int8), G.col_idx.astype(np.int64),
+                       G.row_ptr.astype(np.int64)), shape=(n, n))
+    k, comp = connected_components(A, directed=False)
+    mins = np.full(k, n, dtype=np.int64)
+    np.minimum.at(mins, comp, np.arange(n))
+    return mins[comp].astype(np.uint32)
+
+
+@pytest.mark.parametrize("make", SMALL)
+def test_min_label_vs_bfs(make):
+    G = make()
+    assert np.array_equal(oracle.cc_min_label(G.row_ptr, G.col_idx), bfs_min_labels(G))
+
+
+@pytest.mark.parametrize("seed", [1, 2])
+def test_min_label_vs_scipy(seed):
+    for G in (cm.rmat(14, 8, seed=seed), cm.erdos_renyi(20000, 1.5, seed=seed),
+              cm.grid(100, 120, 0.6, seed=seed)):
+        assert np.array_equal(oracle.cc_min_label(G.row_ptr, G.col_idx), scipy_min_labels(G))
+
+
+def test_min_label_raw_csr_edge_cases():
+    # duplicates, self-loops, out-of-range entries (ignored), isolated vertex
+    G = csr_raw(6, [[1, 1, 0], [0, 0], [9], [4], [3, 3], []])
+    assert oracle.cc_min_label(G.row_ptr, G.col_idx).tolist() == [0, 0, 2, 3, 3, 5]
+    G = csr_raw(1, [[]])
+    assert oracle.cc_min_label(G.row_ptr, G.col_idx).tolist() == [0]
+
+
+def test_min_label_closed_form_properties():
+    G = cm.rmat(13, 8, seed=9)
+    lab = oracle.cc_min_label(G.row_ptr, G.col_idx)
+    v = np.arange(G.n, dtype=np.uint32)
+    assert (lab <= v).all()
+    assert (lab[lab] == lab).all()
+    src = np.repeat(v, np.diff(G.row_ptr).astype(np.int64))
+    assert (lab[src] == lab[G.col_idx]).all()
+
+
+needs_ref = pytest.mark.skipif(oracle.ref_lib() is None,
+                               reason="oracle/_ref not built (reference sources absent)")
+
+
+@needs_ref
+@pytest.mark.parametrize("threads,chunk", [(1, 4096), (4, 1), (4, 8192), (3, 0), (8, 2048)])
+def test_lp_restatement_matches_reference(threads, chunk):
+    for G in (cm.erdos_renyi(65536, 8.0, seed=1001), cm.rmat(14, 16, seed=3),
+              cm.grid(64, 64, 0.7, seed=5)):
+        ref = oracle.ref_cc_pthreads(G.row_ptr, G.col_idx, threads, chunk)
+        mine = oracle.cc_lp_pthreads(G.row_ptr, G.col_idx, threads, chunk)
+        uf = oracle.cc_min_label(G.row_ptr, G.col_idx)
+        assert np.array_equal(ref, uf)
+        assert np.array_equal(mine, uf)
+
+
+def test_lp_restatement_without_reference():
+    G = cm.rmat(12, 16, seed=2)
+    for t, c in ((1, 4096), (5, 1), (6, 7)):
+        assert np.array_equal(oracle.cc_lp_pthreads(G.row_ptr, G.col_idx, t, c),
+                              oracle.cc_min_label(G.row_ptr, G.col_idx))
+
+
+def random_records(count, seed, rep_bits=20, remote_bits=26):
+    rng = np.random.default_rng(seed)
+    rep = rng.integers(0, 1 << rep_bits, count, dtype=np.uint64)
+    remote = rng.integers(0, 1 << remote_bits, count, dtype=np.uint64)
+    return (remote << np.uint64(32)) | rep
+
+
+@pytest.mark.parametrize("count", [0, 1, 2, 63, 64, 65, 1000, 200000])
+def test_sort_vs_numpy(count):
+    k = random_records(count, seed=count)
+    assert np.array_equal(oracle.sort_u64(k), np.sort(k))
+
+
+def test_sort_full_width_and_constant_digits():
+    rng = np.random.default_rng(5)
+    k = rng.integers(0, np.iinfo(np.uint64).max, 5000, dtype=np.uint64, endpoint=True)
+    assert np.array_equal(oracle.sort_u64(k), np.sort(k))
+    k = np.full(100, 0xABCD00000000EF01, dtype=np.uint64)
+    assert np.array_equal(oracle.sort_u64(k), k)
+
+
+@needs_ref
+@pytest.mark.parametrize("count", [2, 64, 65, 5000, 300000])
+def test_sort_and_dedup_match_reference(count):
+    k = random_records(count, seed=11 + count, rep_bits=6, remote_bits=8)  # many duplicates
+    ref = k.copy()
+    L = oracle.ref_lib()
+    L.boundary_edges_sort(oracle._p(ref, oracle.c_u64), ref.size)
+    assert np.array_equal(oracle.sort_u64(k), ref)
+    nd = L.boundary_edges_dedup(oracle._p(ref, oracle.c_u64), ref.size)
+    mine = oracle.dedup_u64(oracle.sort_u64(k))
+    assert np.array_equal(mine, ref[:nd])
+    assert np.array_equal(mine, np.unique(k))
+
+
+def test_ghost_extract_vs_numpy():
+    k = np.unique(random_records(20000, seed=7, rep_bits=10, remote_bits=12))
+    rec, ghosts = oracle.ghost_extract(k)
+    remote = (k >> np.uint64(32)).astype(np.uint32)
+    uq, inv = np.unique(remote, return_inverse=True)
+    assert np.array_equal(ghosts, uq)
+    assert np.array_equal((rec >> np.uint64(32)).astype(np.int64), inv)
+    assert np.array_equal(rec & np.uint64(0xFFFFFFFF), k & np.uint64(0xFFFFFFFF))
+
+
+def brute_delta(stf, seg, comp_label, comp_of, vs, ve, prev):
+    pairs, counts = [], []
+    for k in range(len(seg) - 1):
+        c = 0
+        for i in range(seg[k + 1] - seg[k]):
+            idx = seg[k] + i
+            v = int(stf[idx])
+            lbl = int(comp_label[comp_of[v - vs]]) if vs <= v < ve else v
+            if lbl < prev[idx]:
+                prev[idx] = lbl
+                pairs.append((i << 32) | lbl)
+                c += 1
+        counts.append(c)
+    return np.array(pairs, dtype=np.uint64), np.array(counts, dtype=np.int32)
+
+
+@pytest.mark.parametrize("seed", range(4))
+def test_delta_pack_vs_brute_force(seed):
+    rng = np.random.default_rng(seed)
+    n_local, vs = 300, 1000
+    ve = vs + n_local
+    comp_of = rng.integers(0, n_local, n_local).astype(np.uint32)
+    comp_label = rng.integers(0, 5000, n_local).astype(np.uint32)
+    stf = rng.integers(vs - 50, ve + 50, 700).astype(np.uint32)  # some not owned
+    cuts = np.sort(rng.choice(np.arange(1, 700), 3, replace=False))
+    seg = [0, *cuts.tolist(), 700]
+    prev = np.full(700, 0xFFFFFFFF, dtype=np.uint32)
+    prev[rng.random(700) < 0.3] = rng.integers(0, 3000, 1).astype(np.uint32)[0]
+    prev_b = prev.astype(np.int64).tolist()
+    pairs, cnt, dis = oracle.delta_pack(stf, seg, comp_label, comp_of, vs, ve, prev)
+    want_pairs, want_cnt = brute_delta(stf, seg, comp_label, comp_of, vs, ve, prev_b)
+    assert np.array_equal(pairs, want_pairs)
+    assert np.array_equal(cnt, want_cnt)
+    assert np.array_equal(dis, np.concatenate([[0], np.cumsum(want_cnt)[:-1]]).astype(np.int32))
+    assert np.array_equal(prev.astype(np.int64), np.array(prev_b))
+    # a second pack with nothing changed emits nothing
+    pairs2, cnt2, _ = oracle.delta_pack(stf, seg, comp_label, comp_of, vs, ve, prev)
+    assert pairs2.size == 0 and cnt2.sum() == 0
+
+
+def test_partition_math():
+    L = oracle.lib()
+    for n in (1, 7, 100, 1001):
+        for P in (1, 2, 3, 4, 8):
+            if P > n:
+                continue
+            starts, ends = [], []
+            for r in range(P):
+                a, b = oracle.c_u32(), oracle.c_u32()
+                L.oracle_vertex_range(n, P, r, oracle.ctypes.byref(a), oracle.ctypes.byref(b))
+                starts.append(a.value)
+                ends.append(b.value)
+            assert starts[0] == 0 and ends[-1] == n
+            assert all(ends[i] == starts[i + 1] for i in range(P - 1))
+            sizes = [e - s for s, e in zip(starts, ends)]
+            assert max(sizes) - min(sizes) <= 1 and sizes == sorted(sizes, reverse=True)
+            for v in range(n):
+                own = L.oracle_owner_of_vertex(v, n, P, None, 0)
+                assert starts[own] <= v < ends[own]
+    G = cm.rmat(10, 8, seed=3)
+    for P in (2, 3, 4, 7):
+        b = np.zeros(P + 1, dtype=np.uint32)
+        L.oracle_weighted_bounds(oracle._p(G.row_ptr, oracle.c_u64), G.n, P, 1,
+                                 oracle._p(b, oracle.c_u32))
+        assert b[0] == 0 and b[-1] == G.n and (np.diff(b.astype(np.int64)) >= 1).all()
+        w = np.diff(G.row_ptr).astype(np.int64) + 1
+        cum = np.concatenate([[0], np.cumsum(w)])
+        for r in range(1, P):
+            # first vertex whose prefix weight reaches r/P of the total
+            target = cum[-1] * r // P
+            assert b[r] == max(int(np.searchsorted(cum, target, side="left")), int(b[r - 1]) + 1)
+        for v in range(G.n):
+            own = L.oracle_owner_of_vertex(v, G.n, P, oracle._p(b, oracle.c_u32), 1)
+            assert b[own] <= v < b[own + 1]
+
+
+def test_local_split_and_boundary_build():
+    G = cm.rmat(11, 8, seed=6)
+    n, vs, ve = G.n, 500, 1300
+    rows = slice(vs, ve + 1)
+    rp = (G.row_ptr[rows] - G.row_ptr[vs]).astype(np.uint64)
+    ci = G.col_idx[int(G.row_ptr[vs]):int(G.row_ptr[ve])].copy()
+    nl = ve - vs
+    lrow = np.zeros(nl + 1, dtype=np.uint64)
+    lcol = np.zeros(max(1, ci.size), dtype=np.uint32)
+    braw = oracle.c_u64()
+    L = oracle.lib()
+    mll = L.oracle_local_split(nl, oracle._p(rp, oracle.c_u64), oracle._p(ci, oracle.c_u32), vs,
+                               ve, n, oracle._p(lrow, oracle.c_u64), oracle._p(lcol, oracle.c_u32),
+                               oracle.ctypes.byref(braw))
+    src = np.repeat(np.arange(nl), np.diff(rp).astype(np.int64))
+    loc = (ci >= vs) & (ci < ve)
+    assert mll == loc.sum() and braw.value == (~loc).sum()
+    assert np.array_equal(lcol[:mll], ci[loc] - vs)
+    assert np.array_equal(np.diff(lrow).astype(np.int64), np.bincount(src[loc], minlength=nl))
+    comp_of = np.random.default_rng(0).integers(0, nl, nl).astype(np.uint32)
+    rec = np.zeros(max(1, ci.size), dtype=np.uint64)
+    nb = L.oracle_boundary_build(nl, oracle._p(rp, oracle.c_u64), oracle._p(ci, oracle.c_u32), vs,
+                                 ve, n, oracle._p(comp_of, oracle.c_u32),
+                                 oracle._p(rec, oracle.c_u64))
+    want = (ci[~loc].astype(np.uint64) << np.uint64(32)) | comp_of[src[~loc]].astype(np.uint64)
+    assert np.array_equal(rec[:nb], want)
