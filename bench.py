@@ -104,6 +104,9 @@ def kernel_bytes(name: str, n: int, st: dict, launches_per_solve: int) -> float 
     once, 4 B per id, 8 B per offset."""
     if name == "cc_init":
         return 4.0 * n
+    if name == "cc_init_min":
+        # row_ptr, first col entry of each non-isolated vertex, own parent write
+        return 8.0 * (n + 1) + 4.0 * st["deg_gt0"] + 4.0 * n
     if name == "cc_hook_sample":
         # two launches (r=0,1): row_ptr, own parent, one col entry + one parent gather
         # per vertex that has neighbour r; averaged over the two launches

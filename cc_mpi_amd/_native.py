@@ -90,6 +90,16 @@ def gpu_lib() -> ctypes.CDLL:
     _sig(lib, "cc_hip_components_host", c_int, [P_u64, P_u32, c_u32, c_u64, P_u32])
     _sig(lib, "compute_connected_components_pthreads", None,
          [ctypes.POINTER(CSRGraphC), P_u32, c_int, c_int])
+    _sig(lib, "cc_hip_sort_u64_device", c_int,
+         [c_void_p, c_void_p, c_u64, ctypes.POINTER(c_int), c_void_p])
+    _sig(lib, "cc_hip_dedup_u64_device", c_int, [c_void_p, c_u64, c_void_p, P_u64, c_void_p])
+    _sig(lib, "cc_hip_ghost_extract_device", c_int,
+         [c_void_p, c_u64, c_void_p, c_void_p, P_u64, c_void_p])
+    _sig(lib, "cc_hip_delta_pack_device", c_int,
+         [c_void_p, c_u64, P_int, c_int, c_void_p, c_void_p, c_u32, c_u32, c_void_p, c_void_p,
+          P_int, P_int, P_u64, c_void_p])
+    _sig(lib, "boundary_edges_sort", None, [P_u64, c_u64])
+    _sig(lib, "boundary_edges_dedup", c_u64, [P_u64, c_u64])
     _gpu = lib
     return lib
 

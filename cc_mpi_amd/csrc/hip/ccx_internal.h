@@ -32,6 +32,34 @@ struct Context {
   int cu_count = 0;
 };
 
+// State private to one stream, so work enqueued on different streams never
+// shares scratch memory.
+struct StreamState {
+  hipStream_t stream = nullptr;
+  uint32_t *d_words = nullptr;  // 256 device words: [0] sampled root, [8..] tile counters
+  uint64_t *d_status = nullptr;  // look-back status words (epoch-tagged)
+  size_t status_words = 0;
+  uint32_t epoch = 0;           // last epoch handed out; 0 is never valid
+  uint64_t *h_stage = nullptr;  // 64 KiB pinned host staging
+  void *tmp = nullptr;          // grow-only temp buffer (sort ping-pong etc.)
+  size_t tmp_bytes = 0;
+};
+
+// Look-back status word layout: [63:42] epoch, [41:40] flag, [39:0] value.
+constexpr int kEpochShift = 42;
+constexpr uint32_t kEpochMax = (1u << 22) - 1;
+constexpr uint64_t kFlagAggregate = 1ull << 40;
+constexpr uint64_t kFlagInclusive = 2ull << 40;
+constexpr uint64_t kValueMask = (1ull << 40) - 1;
+
+StreamState *stream_state(hipStream_t s);
+// Grow the status array to `words` (new memory zeroed, epochs restart).
+int ensure_status(StreamState *st, size_t words);
+// Fresh epoch for one look-back pass; recycles the array on wrap-around.
+int next_epoch(StreamState *st, uint32_t *epoch);
+// Grow the per-stream temp buffer to at least `bytes`.
+void *stream_tmp(StreamState *st, size_t bytes);
+
 // Returns the context, initialising device 0 on first use.  nullptr on
 // failure (error recorded).
 Context *ctx();

@@ -8,11 +8,16 @@
 // so any algorithm reaching the same fixed point is bit-exact with it.  This
 // one is a parent-pointer forest that keeps the invariant p[v] <= v:
 //
-//   1. p[v] = v.
-//   2. Sampled hooking: every vertex hooks with its neighbour 0, pointer
-//      jumping, then with its neighbour 1, pointer jumping.  After this the
-//      forest already holds the giant component of a power-law or random
-//      graph in one tree.
+//   1. p[v] = min(v, first listed neighbour of v): an own-entry write with
+//      no atomics.  Rows are sorted, so this is v's smallest neighbour; the
+//      point is that afterwards every vertex already hangs at or below its
+//      smallest neighbour, so a low-id vertex that later hooks onto a hub
+//      finds the hub's root at or below its own id and lowers its own entry
+//      instead of every such vertex hammering the hub's entry.
+//   2. Sampled hooking: every vertex hooks with its neighbour 0 (only where
+//      step 1 did not already link them), pointer jumping, then with its
+//      neighbour 1, pointer jumping.  After this the forest already holds
+//      the giant component of a power-law or random graph in one tree.
 //   3. The most frequent root among 1024 sampled vertices is taken as the
 //      giant root g.
 //   4. Frontier sweep: every vertex whose parent is not g hooks with all its
@@ -85,6 +90,22 @@ __global__ __launch_bounds__(kBlock) void k_init(uint32_t n, uint32_t *__restric
   if (v < n) p[v] = v;
 }
 
+// p[v] = min(v, col[row_ptr[v]]): links v to its first listed neighbour when
+// that one is smaller.  Writes only v's own entry.
+__global__ __launch_bounds__(kBlock) void k_init_min(const uint64_t *__restrict__ row_ptr,
+                                                     const uint32_t *__restrict__ col,
+                                                     uint32_t n, uint32_t *__restrict__ p) {
+  const uint32_t v = blockIdx.x * kBlock + threadIdx.x;
+  if (v >= n) return;
+  uint32_t x = v;
+  const uint64_t b = row_ptr[v];
+  if (b < row_ptr[v + 1]) {
+    const uint32_t w = col[b];
+    if (w < x) x = w;
+  }
+  p[v] = x;
+}
+
 // Hook every vertex with its neighbour number r (if it has one).
 __global__ __launch_bounds__(kBlock) void k_hook_neighbour(const uint64_t *__restrict__ row_ptr,
                                                            const uint32_t *__restrict__ col,
@@ -95,7 +116,8 @@ __global__ __launch_bounds__(kBlock) void k_hook_neighbour(const uint64_t *__res
   const uint64_t b = row_ptr[v] + r;
   if (b >= row_ptr[v + 1]) return;
   const uint32_t w = col[b];
-  if (w < n) hook(v, w, p);
+  // neighbour 0 below v was linked by k_init_min already
+  if (w < n && !(r == 0 && w < v)) hook(v, w, p);
 }
 
 // Pointer jumping: p[v] = root(v).  Run with no hooks in flight.
@@ -122,45 +144,54 @@ __device__ __forceinline__ uint32_t sample_vertex(uint32_t i, uint32_t n) {
 }
 
 // One workgroup: the most frequent root among kGiantSamples sampled
-// vertices (ties: the larger run found first in sorted order).
+// vertices, counted in an LDS open-addressing table; ties go to the smaller
+// root so the choice is a function of the forest alone.
 __global__ __launch_bounds__(kBlock) void k_sample_giant(uint32_t n, const uint32_t *__restrict__ p,
                                                          uint32_t *__restrict__ giant) {
-  __shared__ uint32_t s[kGiantSamples];
-  __shared__ uint64_t best[kBlock];
+  constexpr uint32_t kSlots = 2 * kGiantSamples;
+  constexpr uint32_t kEmpty = 0xFFFFFFFFu;
+  __shared__ uint32_t keys[kSlots];
+  __shared__ uint32_t cnts[kSlots];
+  __shared__ uint64_t best[kBlock / kWave];
   const int t = threadIdx.x;
-  for (int i = t; i < kGiantSamples; i += kBlock) s[i] = p[sample_vertex(i, n)];
+  for (uint32_t i = t; i < kSlots; i += kBlock) {
+    keys[i] = kEmpty;
+    cnts[i] = 0;
+  }
   __syncthreads();
-  // Bitonic sort of the 1024 samples in LDS.
-  for (int k = 2; k <= kGiantSamples; k <<= 1) {
-    for (int j = k >> 1; j > 0; j >>= 1) {
-      for (int q = t; q < kGiantSamples / 2; q += kBlock) {
-        const int i = (q / j) * 2 * j + (q % j);
-        const int l = i + j;
-        const bool up = (i & k) == 0;
-        const uint32_t a = s[i], b = s[l];
-        if ((a > b) == up) {
-          s[i] = b;
-          s[l] = a;
-        }
+  for (int i = t; i < kGiantSamples; i += kBlock) {
+    const uint32_t x = p[sample_vertex(i, n)];
+    uint32_t h = (x * 0x9E3779B1u) >> 21;  // 11-bit slot
+    for (;;) {
+      const uint32_t prev = atomicCAS(&keys[h], kEmpty, x);
+      if (prev == kEmpty || prev == x) {
+        atomicAdd(&cnts[h], 1u);
+        break;
       }
-      __syncthreads();
+      h = (h + 1) & (kSlots - 1);
     }
   }
-  uint64_t mine = 0;
-  for (int i = t * (kGiantSamples / kBlock); i < (t + 1) * (kGiantSamples / kBlock); ++i) {
-    if (i > 0 && s[i] == s[i - 1]) continue;
-    uint32_t len = 1;
-    while (i + (int)len < kGiantSamples && s[i + len] == s[i]) ++len;
-    const uint64_t cand = ((uint64_t)len << 32) | s[i];
-    if ((cand >> 32) > (mine >> 32)) mine = cand;
-  }
-  best[t] = mine;
   __syncthreads();
-  for (int w = kBlock / 2; w > 0; w >>= 1) {
-    if (t < w && (best[t + w] >> 32) > (best[t] >> 32)) best[t] = best[t + w];
-    __syncthreads();
+  // (count, ~root) packed so the max picks the largest count, then the
+  // smallest root.
+  uint64_t mine = 0;
+  for (uint32_t i = t; i < kSlots; i += kBlock)
+    if (cnts[i]) {
+      const uint64_t cand = ((uint64_t)cnts[i] << 32) | (uint32_t)~keys[i];
+      if (cand > mine) mine = cand;
+    }
+  for (int d = kWave / 2; d > 0; d >>= 1) {
+    const uint64_t o = __shfl_xor(mine, d, kWave);
+    if (o > mine) mine = o;
   }
-  if (t == 0) giant[0] = (uint32_t)best[0];
+  if ((t & (kWave - 1)) == 0) best[t / kWave] = mine;
+  __syncthreads();
+  if (t == 0) {
+    uint64_t b = best[0];
+    for (int w = 1; w < kBlock / kWave; ++w)
+      if (best[w] > b) b = best[w];
+    giant[0] = ~(uint32_t)b;
+  }
 }
 
 // Frontier sweep over the vertices not under the giant root.
@@ -212,21 +243,6 @@ __global__ __launch_bounds__(kBlock) void k_hook_frontier(const uint64_t *__rest
   }
 }
 
-std::mutex g_scratch_mu;
-std::unordered_map<hipStream_t, uint32_t *> g_scratch;
-
-// A few device words per stream, so solves on different streams never share
-// their sampled-root slot.
-uint32_t *stream_scratch(hipStream_t s) {
-  std::lock_guard<std::mutex> lk(g_scratch_mu);
-  auto it = g_scratch.find(s);
-  if (it != g_scratch.end()) return it->second;
-  uint32_t *d = nullptr;
-  if (hipMalloc(&d, 256) != hipSuccess) return nullptr;
-  g_scratch[s] = d;
-  return d;
-}
-
 inline unsigned blocks_for(uint64_t n) { return (unsigned)((n + kBlock - 1) / kBlock); }
 
 }  // namespace
@@ -239,12 +255,18 @@ extern "C" int cc_hip_components_device(const uint64_t *d_row_ptr, const uint32_
   if (!d_row_ptr || !d_labels || (m > 0 && !d_col_idx))
     return ccx::set_error(4, "cc_hip_components_device: null pointer");
   hipStream_t s = ccx::pick_stream(stream);
-  uint32_t *giant = stream_scratch(s);
-  if (!giant) return ccx::set_error(3, "cc_hip_components_device: scratch allocation failed");
+  // The sampled giant root lives in this stream's own scratch word, so
+  // solves on different streams never share it.
+  ccx::StreamState *st = ccx::stream_state(s);
+  if (!st) return 3;
+  uint32_t *giant = st->d_words;
   const unsigned nb = blocks_for(n);
 
-  CCX_LAUNCH("cc_init", s, k_init<<<nb, kBlock, 0, s>>>(n, d_labels));
-  if (m > 0) {
+  if (m == 0) {
+    CCX_LAUNCH("cc_init", s, k_init<<<nb, kBlock, 0, s>>>(n, d_labels));
+  } else {
+    CCX_LAUNCH("cc_init_min", s,
+               k_init_min<<<nb, kBlock, 0, s>>>(d_row_ptr, d_col_idx, n, d_labels));
     for (uint32_t r = 0; r < (uint32_t)kSampledNeighbours; ++r) {
       CCX_LAUNCH("cc_hook_sample", s,
                  k_hook_neighbour<<<nb, kBlock, 0, s>>>(d_row_ptr, d_col_idx, n, r, d_labels));

@@ -105,6 +105,68 @@ void *workspace(size_t bytes, hipStream_t s) {
   return c->ws;
 }
 
+static std::mutex g_ss_mu;
+static std::map<hipStream_t, StreamState *> g_ss;
+
+StreamState *stream_state(hipStream_t s) {
+  std::lock_guard<std::mutex> lk(g_ss_mu);
+  auto it = g_ss.find(s);
+  if (it != g_ss.end()) return it->second;
+  StreamState *st = new StreamState();
+  st->stream = s;
+  if (hipMalloc(&st->d_words, 256 * sizeof(uint32_t)) != hipSuccess ||
+      hipMemset(st->d_words, 0, 256 * sizeof(uint32_t)) != hipSuccess ||
+      hipHostMalloc(&st->h_stage, 65536, hipHostMallocDefault) != hipSuccess) {
+    set_error(3, "stream_state: allocation failed");
+    delete st;
+    return nullptr;
+  }
+  g_ss[s] = st;
+  return st;
+}
+
+int ensure_status(StreamState *st, size_t words) {
+  if (words <= st->status_words) return 0;
+  CCX_TRY(hipStreamSynchronize(st->stream));
+  if (st->d_status) CCX_TRY(hipFree(st->d_status));
+  st->d_status = nullptr;
+  st->status_words = 0;
+  size_t want = words + words / 4 + 4096;
+  CCX_TRY(hipMalloc(&st->d_status, want * sizeof(uint64_t)));
+  CCX_TRY(hipMemsetAsync(st->d_status, 0, want * sizeof(uint64_t), st->stream));
+  st->status_words = want;
+  st->epoch = 0;
+  return 0;
+}
+
+int next_epoch(StreamState *st, uint32_t *epoch) {
+  if (st->epoch >= kEpochMax) {
+    // Every tag value has been used: clear so no stale word can match.
+    CCX_TRY(hipMemsetAsync(st->d_status, 0, st->status_words * sizeof(uint64_t), st->stream));
+    st->epoch = 0;
+  }
+  *epoch = ++st->epoch;
+  return 0;
+}
+
+void *stream_tmp(StreamState *st, size_t bytes) {
+  if (bytes <= st->tmp_bytes) return st->tmp;
+  if (hipStreamSynchronize(st->stream) != hipSuccess) {
+    set_error(1, "stream_tmp: synchronize failed");
+    return nullptr;
+  }
+  if (st->tmp) (void)hipFree(st->tmp);
+  st->tmp = nullptr;
+  st->tmp_bytes = 0;
+  size_t want = bytes + bytes / 8 + 4096;
+  if (hipMalloc(&st->tmp, want) != hipSuccess) {
+    set_error(3, "stream_tmp: hipMalloc of %zu bytes failed", want);
+    return nullptr;
+  }
+  st->tmp_bytes = want;
+  return st->tmp;
+}
+
 hipStream_t pick_stream(void *stream) {
   if (stream) return (hipStream_t)stream;
   Context *c = ctx();

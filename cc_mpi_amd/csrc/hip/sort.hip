@@ -1,0 +1,527 @@
+// Boundary-record sort and the ordered-compaction kernels behind
+// include/boundary_sort.h (reference src/boundary_sort.c:5-119) and the
+// ghost extraction of src/cc_mpi.c:431-457.
+//
+// Sort: the reference sorts BoundaryEdge records by the 64-bit key
+// (remote << 32) | rep with an in-place MSD American-flag radix sort.  The
+// order is total on the key and the key is the whole record, so any correct
+// sort gives the same bytes.  Here it is an LSD radix sort on the raw
+// little-endian records (8-bit digits), one "onesweep" pass per digit:
+//   * one pass over the keys builds all eight 256-bin histograms (wave64
+//     ballot matching, so skewed digits do not serialise LDS atomics);
+//     digits that are constant over the input are skipped;
+//   * each digit pass: a workgroup takes the next 4096-key tile (atomic tile
+//     counter), ranks its keys stably with wave64 ballot matching into
+//     per-wave LDS counters, publishes its 256 digit counts, resolves the
+//     global offsets by decoupled look-back over the preceding tiles, and
+//     scatters through LDS so that each digit's run of the tile is written
+//     to HBM contiguously.
+//
+// Compaction: an order-preserving stream compaction (decoupled look-back
+// over 4096-item tiles, each thread owning 16 consecutive items) drives
+// dedup (adjacent-unique) and ghost extraction.
+
+#include <hip/hip_runtime.h>
+
+#include <cstdio>
+#include <cstring>
+
+#include "ccx_internal.h"
+#include "lookback.h"
+#include "../../../include/cc_hip.h"
+#include "../../../include/boundary_sort.h"
+
+using ccx::kWave;
+
+namespace {
+
+constexpr int kThreads = 256;
+constexpr int kWaves = kThreads / kWave;
+constexpr int kItems = 16;
+constexpr int kTile = kThreads * kItems;  // 4096 keys
+constexpr int kBins = 256;
+
+// Lanes (among `active`) whose 8-bit digit equals this lane's.  Only the
+// digit bits in `vary` can differ between keys; the rest are skipped.
+__device__ __forceinline__ uint64_t match_digit(uint32_t d, uint64_t active, uint32_t vary) {
+  uint64_t peers = active;
+#pragma unroll
+  for (int b = 0; b < 8; ++b) {
+    if (!((vary >> b) & 1u)) continue;
+    const bool bit = (d >> b) & 1u;
+    const uint64_t bal = __ballot(bit);
+    peers &= bit ? bal : ~bal;
+  }
+  return peers;
+}
+
+__device__ __forceinline__ uint64_t lanemask_lt(int lane) {
+  return (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+}
+
+// AND and OR of all keys: a bit that is equal in both is constant.
+__global__ __launch_bounds__(kThreads) void k_and_or(const uint64_t *__restrict__ keys, uint64_t n,
+                                                     unsigned long long *__restrict__ out) {
+  uint64_t a = ~0ull, o = 0;
+  for (uint64_t i = (uint64_t)blockIdx.x * kThreads + threadIdx.x; i < n;
+       i += (uint64_t)gridDim.x * kThreads) {
+    const uint64_t k = keys[i];
+    a &= k;
+    o |= k;
+  }
+#pragma unroll
+  for (int d = kWave / 2; d > 0; d >>= 1) {
+    a &= __shfl_xor(a, d, kWave);
+    o |= __shfl_xor(o, d, kWave);
+  }
+  if ((threadIdx.x & (kWave - 1)) == 0) {
+    atomicAnd(&out[0], (unsigned long long)a);
+    atomicOr(&out[1], (unsigned long long)o);
+  }
+}
+
+struct DigitPlan {
+  uint32_t vary[8];  // varying bits of each digit (0 = constant digit)
+};
+
+// 256-bin histograms of every non-constant digit, one pass over the keys.
+__global__ __launch_bounds__(kThreads) void k_histograms(const uint64_t *__restrict__ keys, uint64_t n,
+                                                         DigitPlan plan,
+                                                         unsigned long long *__restrict__ hist) {
+  __shared__ uint32_t h[8][kBins];
+  for (int i = threadIdx.x; i < 8 * kBins; i += kThreads) (&h[0][0])[i] = 0;
+  __syncthreads();
+  const int lane = threadIdx.x & (kWave - 1);
+  const uint64_t lt = lanemask_lt(lane);
+  const uint64_t stride = (uint64_t)gridDim.x * kThreads;
+  for (uint64_t base = (uint64_t)blockIdx.x * kThreads; base < n; base += stride) {
+    const uint64_t i = base + threadIdx.x;
+    const bool valid = i < n;
+    const uint64_t k = valid ? keys[i] : 0;
+    const uint64_t active = __ballot(valid);
+#pragma unroll
+    for (int d = 0; d < 8; ++d) {
+      if (!plan.vary[d]) continue;
+      const uint32_t dig = (uint32_t)(k >> (8 * d)) & 0xFFu;
+      const uint64_t peers = match_digit(dig, active, plan.vary[d]);
+      // the lowest peer adds the group's count
+      if (valid && (peers & lt) == 0) atomicAdd(&h[d][dig], (uint32_t)__popcll(peers));
+    }
+  }
+  __syncthreads();
+  for (int i = threadIdx.x; i < 8 * kBins; i += kThreads) {
+    const uint32_t c = (&h[0][0])[i];
+    if (c) atomicAdd(&hist[i], (unsigned long long)c);
+  }
+}
+
+// One LSD pass on digit `shift/8`: stable scatter of in[] into out[].
+// goff[d] = first output position of digit d (exclusive scan of the global
+// histogram).  tile_ctr hands out tiles in order; status holds per-tile,
+// per-digit look-back words (kBins per tile).
+__global__ __launch_bounds__(kThreads) void k_onesweep(const uint64_t *__restrict__ in,
+                                                       uint64_t *__restrict__ out, uint64_t n,
+                                                       int shift, uint32_t vary,
+                                                       const uint64_t *__restrict__ goff,
+                                                       uint32_t *__restrict__ tile_ctr,
+                                                       uint64_t *__restrict__ status, uint32_t epoch) {
+  __shared__ uint64_t s_keys[kTile];
+  __shared__ uint32_t s_wcnt[kWaves][kBins];  // per-wave digit counts, then exclusive offsets
+  __shared__ uint32_t s_dstart[kBins];        // tile-local start of each digit
+  __shared__ uint64_t s_gbase[kBins];         // global position of tile-local index 0 of digit d
+  __shared__ uint32_t s_wsum[kWaves];
+  __shared__ uint32_t s_tile;
+
+  const int t = threadIdx.x;
+  const int lane = t & (kWave - 1);
+  const int w = t / kWave;
+  if (t == 0) s_tile = atomicAdd(tile_ctr, 1u);
+  for (int i = t; i < kWaves * kBins; i += kThreads) (&s_wcnt[0][0])[i] = 0;
+  __syncthreads();
+  const uint64_t tile = s_tile;
+  // Tile ids come from a counter zeroed before the launch and run 0..tiles-1;
+  // an id past the input (never expected) must not touch memory.
+  if (tile * kTile >= n) return;
+  const uint64_t tile_base = tile * kTile;
+  const uint64_t lt = lanemask_lt(lane);
+
+  uint64_t key[kItems];
+  uint32_t rank[kItems];
+#pragma unroll
+  for (int i = 0; i < kItems; ++i) {
+    const uint64_t pos = tile_base + (uint64_t)w * (kWave * kItems) + (uint64_t)i * kWave + lane;
+    key[i] = (pos < n) ? in[pos] : ~0ull;
+  }
+#pragma unroll
+  for (int i = 0; i < kItems; ++i) {
+    const uint64_t pos = tile_base + (uint64_t)w * (kWave * kItems) + (uint64_t)i * kWave + lane;
+    const bool valid = pos < n;
+    const uint32_t d = (uint32_t)(key[i] >> shift) & 0xFFu;
+    const uint64_t active = __ballot(valid);
+    const uint64_t peers = match_digit(d, active, vary);
+    uint32_t old = 0;
+    if (valid) old = s_wcnt[w][d];
+    __builtin_amdgcn_wave_barrier();
+    // the highest peer bumps the wave's counter after every peer has read it
+    if (valid && (peers >> lane) == 1ull) s_wcnt[w][d] = old + (uint32_t)__popcll(peers);
+    __builtin_amdgcn_wave_barrier();
+    rank[i] = old + (uint32_t)__popcll(peers & lt);
+  }
+  __syncthreads();
+
+  // Thread t owns digit t: per-wave exclusive offsets and the tile count.
+  uint32_t cnt = 0;
+#pragma unroll
+  for (int ww = 0; ww < kWaves; ++ww) {
+    const uint32_t c = s_wcnt[ww][t];
+    s_wcnt[ww][t] = cnt;
+    cnt += c;
+  }
+  // publish this tile's count for digit t as early as possible
+  uint64_t *my_status = status + tile * kBins + t;
+  if (tile == 0)
+    ccx::status_store(my_status, ccx::tag_word(epoch, ccx::kFlagInclusive, cnt));
+  else
+    ccx::status_store(my_status, ccx::tag_word(epoch, ccx::kFlagAggregate, cnt));
+
+  // tile-local exclusive scan over digits
+  const uint32_t incl = ccx::wave_incl_scan<uint32_t>(cnt, lane);
+  if (lane == kWave - 1) s_wsum[w] = incl;
+  __syncthreads();
+  uint32_t wprefix = 0;
+  for (int ww = 0; ww < w; ++ww) wprefix += s_wsum[ww];
+  const uint32_t dstart = wprefix + incl - cnt;
+  s_dstart[t] = dstart;
+
+  uint64_t excl = 0;
+  if (tile > 0) {
+    excl = ccx::lookback(status, tile, kBins, t, epoch);
+    ccx::status_store(my_status, ccx::tag_word(epoch, ccx::kFlagInclusive, excl + cnt));
+  }
+  s_gbase[t] = goff[t] + excl - dstart;
+  __syncthreads();
+
+  // Stable scatter into LDS in digit order.
+#pragma unroll
+  for (int i = 0; i < kItems; ++i) {
+    const uint64_t pos = tile_base + (uint64_t)w * (kWave * kItems) + (uint64_t)i * kWave + lane;
+    if (pos < n) {
+      const uint32_t d = (uint32_t)(key[i] >> shift) & 0xFFu;
+      s_keys[s_dstart[d] + s_wcnt[w][d] + rank[i]] = key[i];
+    }
+  }
+  __syncthreads();
+  const uint64_t valid_in_tile = (n - tile_base) < (uint64_t)kTile ? (n - tile_base) : kTile;
+  for (uint32_t j = t; j < valid_in_tile; j += kThreads) {
+    const uint64_t k = s_keys[j];
+    const uint32_t d = (uint32_t)(k >> shift) & 0xFFu;
+    out[s_gbase[d] + j] = k;
+  }
+}
+
+// ---- ordered compaction --------------------------------------------------
+
+// Op interface: bool eval(uint64_t j, uint64_t &val) decides whether item j
+// is kept (val carries data to apply); void apply(uint64_t j, bool keep,
+// uint64_t rank, uint64_t val) is called for every item in order, rank =
+// number of kept items before j.  begin(first_item) runs once per thread.
+template <class Op>
+__global__ __launch_bounds__(kThreads) void k_compact(Op op, uint64_t n, uint32_t *__restrict__ tile_ctr,
+                                                      uint64_t *__restrict__ status, uint32_t epoch,
+                                                      unsigned long long *__restrict__ total) {
+  __shared__ uint32_t s_tile;
+  __shared__ uint32_t s_wsum[kWaves];
+  __shared__ uint64_t s_base;
+  const int t = threadIdx.x;
+  const int lane = t & (kWave - 1);
+  const int w = t / kWave;
+  if (t == 0) s_tile = atomicAdd(tile_ctr, 1u);
+  __syncthreads();
+  const uint64_t tile = s_tile;
+  // Tile ids come from a counter zeroed before the launch and run 0..tiles-1;
+  // an id past the input (never expected) must not touch memory.
+  if (tile * kTile >= n) return;
+  const uint64_t first = tile * kTile + (uint64_t)t * kItems;
+  uint32_t keep = 0;
+  uint64_t val[kItems];
+#pragma unroll
+  for (int k = 0; k < kItems; ++k) {
+    const uint64_t j = first + k;
+    val[k] = 0;
+    if (j < n && op.eval(j, val[k])) keep |= 1u << k;
+  }
+  const uint32_t cnt = (uint32_t)__popc(keep);
+  const uint32_t incl = ccx::wave_incl_scan<uint32_t>(cnt, lane);
+  if (lane == kWave - 1) s_wsum[w] = incl;
+  __syncthreads();
+  uint32_t wprefix = 0, tile_total = 0;
+  for (int ww = 0; ww < kWaves; ++ww) {
+    if (ww < w) wprefix += s_wsum[ww];
+    tile_total += s_wsum[ww];
+  }
+  if (t == 0) {
+    uint64_t *my = status + tile;
+    uint64_t excl = 0;
+    if (tile == 0) {
+      ccx::status_store(my, ccx::tag_word(epoch, ccx::kFlagInclusive, tile_total));
+    } else {
+      ccx::status_store(my, ccx::tag_word(epoch, ccx::kFlagAggregate, tile_total));
+      excl = ccx::lookback(status, tile, 1, 0, epoch);
+      ccx::status_store(my, ccx::tag_word(epoch, ccx::kFlagInclusive, excl + tile_total));
+    }
+    s_base = excl;
+    if ((tile + 1) * kTile >= n) *total = excl + tile_total;
+  }
+  __syncthreads();
+  uint64_t rank = s_base + wprefix + incl - cnt;
+  op.begin(first);
+#pragma unroll
+  for (int k = 0; k < kItems; ++k) {
+    const uint64_t j = first + k;
+    if (j < n) {
+      const bool kf = (keep >> k) & 1u;
+      op.apply(j, kf, rank, val[k]);
+      rank += kf;
+    }
+  }
+}
+
+// Adjacent-unique (reference src/boundary_sort.c:100-119).
+struct DedupOp {
+  const uint64_t *in;
+  uint64_t *out;
+  __device__ void begin(uint64_t) {}
+  __device__ bool eval(uint64_t j, uint64_t &v) const {
+    v = in[j];
+    return j == 0 || v != in[j - 1];
+  }
+  __device__ void apply(uint64_t, bool keep, uint64_t rank, uint64_t v) const {
+    if (keep) out[rank] = v;
+  }
+};
+
+// Ghost extraction over sorted unique records (reference src/cc_mpi.c:431-457):
+// ghosts[g] = g-th distinct remote; each record's remote becomes its ghost
+// index g.  Records are (remote << 32) | rep.
+struct GhostOp {
+  const uint64_t *in;
+  uint64_t *out;
+  uint32_t *ghosts;
+  __device__ void begin(uint64_t) {}
+  __device__ bool eval(uint64_t j, uint64_t &v) const {
+    v = in[j];
+    return j == 0 || (uint32_t)(v >> 32) != (uint32_t)(in[j - 1] >> 32);
+  }
+  __device__ void apply(uint64_t j, bool keep, uint64_t rank, uint64_t v) const {
+    if (keep) ghosts[rank] = (uint32_t)(v >> 32);
+    const uint64_t g = rank + (keep ? 1 : 0) - 1;
+    out[j] = (g << 32) | (uint32_t)v;
+  }
+};
+
+inline unsigned tiles_for(uint64_t n) { return (unsigned)((n + kTile - 1) / kTile); }
+
+template <class Op>
+int run_compact(const char *name, Op op, uint64_t n, hipStream_t s, ccx::StreamState *st,
+                uint64_t *count_out) {
+  if (n == 0) {
+    *count_out = 0;
+    return 0;
+  }
+  const uint64_t tiles = tiles_for(n);
+  if (tiles > 0xFFFFFFFFull) return ccx::set_error(4, "%s: input too large", name);
+  if (int rc = ccx::ensure_status(st, tiles)) return rc;
+  uint32_t epoch;
+  if (int rc = ccx::next_epoch(st, &epoch)) return rc;
+  uint32_t *ctr = st->d_words + 8;
+  unsigned long long *tot = (unsigned long long *)(st->d_words + 16);
+  CCX_TRY(hipMemsetAsync(ctr, 0, sizeof(uint32_t), s));
+  CCX_LAUNCH(name, s,
+             k_compact<Op><<<(unsigned)tiles, kThreads, 0, s>>>(op, n, ctr, st->d_status, epoch, tot));
+  CCX_TRY(hipMemcpyAsync(st->h_stage, tot, sizeof(uint64_t), hipMemcpyDeviceToHost, s));
+  CCX_TRY(hipStreamSynchronize(s));
+  *count_out = st->h_stage[0];
+  return 0;
+}
+
+}  // namespace
+
+// ---- sort ------------------------------------------------------------------
+
+namespace ccx {
+// Sorts keys[0..n) ascending; alt is a buffer of n keys.  On return
+// *in_alt says which buffer holds the result.  Blocks until the digit plan
+// is known (one small device-to-host read), then enqueues the passes.
+int sort_u64(uint64_t *keys, uint64_t *alt, uint64_t n, int *in_alt, hipStream_t s) {
+  *in_alt = 0;
+  if (n <= 1) return 0;
+  StreamState *st = stream_state(s);
+  if (!st) return 3;
+  unsigned long long *d_ao = (unsigned long long *)(st->d_words + 32);    // 2 x u64
+  unsigned long long *d_hist = nullptr;
+  uint64_t *d_goff = nullptr;
+  // histograms (8 x 256 u64) and offsets (8 x 256 u64) live in the temp buffer
+  char *tmp = (char *)stream_tmp(st, 2 * 8 * kBins * sizeof(uint64_t));
+  if (!tmp) return 3;
+  d_hist = (unsigned long long *)tmp;
+  d_goff = (uint64_t *)(tmp + 8 * kBins * sizeof(uint64_t));
+
+  uint64_t init_ao[2] = {~0ull, 0ull};
+  memcpy(st->h_stage, init_ao, sizeof(init_ao));
+  CCX_TRY(hipMemcpyAsync(d_ao, st->h_stage, sizeof(init_ao), hipMemcpyHostToDevice, s));
+  const unsigned red_blocks = (unsigned)((n + kThreads - 1) / kThreads < 2048 ? (n + kThreads - 1) / kThreads : 2048);
+  CCX_LAUNCH("sort_and_or", s, k_and_or<<<red_blocks, kThreads, 0, s>>>(keys, n, d_ao));
+  CCX_TRY(hipMemcpyAsync(st->h_stage, d_ao, 2 * sizeof(uint64_t), hipMemcpyDeviceToHost, s));
+  CCX_TRY(hipStreamSynchronize(s));
+  const uint64_t diff = st->h_stage[0] ^ st->h_stage[1];
+  DigitPlan plan;
+  int passes = 0;
+  for (int d = 0; d < 8; ++d) {
+    plan.vary[d] = (uint32_t)(diff >> (8 * d)) & 0xFFu;
+    passes += plan.vary[d] != 0;
+  }
+  if (passes == 0) return 0;  // all keys equal
+
+  CCX_TRY(hipMemsetAsync(d_hist, 0, 8 * kBins * sizeof(uint64_t), s));
+  CCX_LAUNCH("sort_histograms", s, k_histograms<<<red_blocks, kThreads, 0, s>>>(keys, n, plan, d_hist));
+  uint64_t *h = st->h_stage;
+  CCX_TRY(hipMemcpyAsync(h, d_hist, 8 * kBins * sizeof(uint64_t), hipMemcpyDeviceToHost, s));
+  CCX_TRY(hipStreamSynchronize(s));
+  uint64_t *offs = st->h_stage + 8 * kBins;
+  for (int d = 0; d < 8; ++d) {
+    uint64_t acc = 0;
+    for (int b = 0; b < kBins; ++b) {
+      offs[d * kBins + b] = acc;
+      acc += h[d * kBins + b];
+    }
+  }
+  CCX_TRY(hipMemcpyAsync(d_goff, offs, 8 * kBins * sizeof(uint64_t), hipMemcpyHostToDevice, s));
+
+  const uint64_t tiles = tiles_for(n);
+  if (tiles > 0xFFFFFFFFull) return set_error(4, "sort: input too large");
+  if (int rc = ensure_status(st, tiles * kBins)) return rc;
+  uint64_t *src = keys, *dst = alt;
+  int pass = 0;
+  for (int d = 0; d < 8; ++d) {
+    if (!plan.vary[d]) continue;
+    uint32_t epoch;
+    if (int rc = next_epoch(st, &epoch)) return rc;
+    uint32_t *ctr = st->d_words + 8 + pass;
+    CCX_TRY(hipMemsetAsync(ctr, 0, sizeof(uint32_t), s));
+    CCX_LAUNCH("sort_onesweep", s,
+               k_onesweep<<<(unsigned)tiles, kThreads, 0, s>>>(src, dst, n, 8 * d, plan.vary[d],
+                                                               d_goff + d * kBins, ctr,
+                                                               st->d_status, epoch));
+    uint64_t *tt = src;
+    src = dst;
+    dst = tt;
+    ++pass;
+  }
+  *in_alt = (src == alt);
+  return 0;
+}
+}  // namespace ccx
+
+extern "C" int cc_hip_sort_u64_device(uint64_t *d_keys, uint64_t *d_alt, uint64_t count,
+                                      int *result_in_alt, void *stream) {
+  if (!ccx::ctx()) return 2;
+  int in_alt = 0;
+  if (result_in_alt) *result_in_alt = 0;
+  if (count <= 1) return 0;
+  if (!d_keys || !d_alt) return ccx::set_error(4, "cc_hip_sort_u64_device: null pointer");
+  int rc = ccx::sort_u64(d_keys, d_alt, count, &in_alt, ccx::pick_stream(stream));
+  if (rc) return rc;
+  if (result_in_alt) {
+    *result_in_alt = in_alt;
+  } else if (in_alt) {
+    CCX_TRY(hipMemcpyAsync(d_keys, d_alt, count * sizeof(uint64_t), hipMemcpyDeviceToDevice,
+                           ccx::pick_stream(stream)));
+  }
+  return 0;
+}
+
+extern "C" int cc_hip_dedup_u64_device(const uint64_t *d_in, uint64_t count, uint64_t *d_out,
+                                       uint64_t *out_count, void *stream) {
+  if (!ccx::ctx()) return 2;
+  if (!out_count) return ccx::set_error(4, "cc_hip_dedup_u64_device: null count");
+  if (count > 0 && (!d_in || !d_out)) return ccx::set_error(4, "cc_hip_dedup_u64_device: null pointer");
+  if (d_in == d_out && count > 1)
+    return ccx::set_error(4, "cc_hip_dedup_u64_device: input and output must differ");
+  hipStream_t s = ccx::pick_stream(stream);
+  ccx::StreamState *st = ccx::stream_state(s);
+  if (!st) return 3;
+  return run_compact("dedup_compact", DedupOp{d_in, d_out}, count, s, st, out_count);
+}
+
+extern "C" int cc_hip_ghost_extract_device(const uint64_t *d_records, uint64_t count,
+                                           uint64_t *d_out_records, uint32_t *d_ghost_vertices,
+                                           uint64_t *ghost_count, void *stream) {
+  if (!ccx::ctx()) return 2;
+  if (!ghost_count) return ccx::set_error(4, "cc_hip_ghost_extract_device: null count");
+  if (count > 0 && (!d_records || !d_out_records || !d_ghost_vertices))
+    return ccx::set_error(4, "cc_hip_ghost_extract_device: null pointer");
+  if (d_records == d_out_records && count > 1)
+    return ccx::set_error(4, "cc_hip_ghost_extract_device: input and output must differ");
+  hipStream_t s = ccx::pick_stream(stream);
+  ccx::StreamState *st = ccx::stream_state(s);
+  if (!st) return 3;
+  return run_compact("ghost_compact", GhostOp{d_records, d_out_records, d_ghost_vertices}, count,
+                     s, st, ghost_count);
+}
+
+// ---- host drop-ins (include/boundary_sort.h) --------------------------------
+
+static int host_sort(BoundaryEdge *edges, uint64_t count) {
+  ccx::Context *c = ccx::ctx();
+  if (!c) return 2;
+  hipStream_t s = c->stream;
+  const size_t bytes = count * sizeof(uint64_t);
+  char *ws = (char *)ccx::workspace(2 * bytes + 256, s);
+  if (!ws) return 3;
+  uint64_t *a = (uint64_t *)ws;
+  uint64_t *b = (uint64_t *)(ws + ((bytes + 255) & ~(size_t)255));
+  CCX_TRY(hipMemcpyAsync(a, edges, bytes, hipMemcpyHostToDevice, s));
+  int in_alt = 0;
+  if (int rc = ccx::sort_u64(a, b, count, &in_alt, s)) return rc;
+  CCX_TRY(hipMemcpyAsync(edges, in_alt ? b : a, bytes, hipMemcpyDeviceToHost, s));
+  CCX_TRY(hipStreamSynchronize(s));
+  return 0;
+}
+
+static int host_dedup(BoundaryEdge *edges, uint64_t count, uint64_t *out) {
+  ccx::Context *c = ccx::ctx();
+  if (!c) return 2;
+  hipStream_t s = c->stream;
+  const size_t bytes = count * sizeof(uint64_t);
+  char *ws = (char *)ccx::workspace(2 * bytes + 256, s);
+  if (!ws) return 3;
+  uint64_t *a = (uint64_t *)ws;
+  uint64_t *b = (uint64_t *)(ws + ((bytes + 255) & ~(size_t)255));
+  CCX_TRY(hipMemcpyAsync(a, edges, bytes, hipMemcpyHostToDevice, s));
+  ccx::StreamState *st = ccx::stream_state(s);
+  if (!st) return 3;
+  if (int rc = run_compact("dedup_compact", DedupOp{a, b}, count, s, st, out)) return rc;
+  CCX_TRY(hipMemcpyAsync(edges, b, *out * sizeof(uint64_t), hipMemcpyDeviceToHost, s));
+  CCX_TRY(hipStreamSynchronize(s));
+  return 0;
+}
+
+// Drop-in for include/boundary_sort.h:8 (reference src/boundary_sort.c:93-98).
+extern "C" void boundary_edges_sort(BoundaryEdge *edges, uint64_t count) {
+  if (!edges || count <= 1) return;
+  if (int rc = host_sort(edges, count)) {
+    fprintf(stderr, "boundary_edges_sort: %s (rc=%d)\n", cc_hip_last_error(), rc);
+    exit(EXIT_FAILURE);
+  }
+}
+
+// Drop-in for include/boundary_sort.h:9 (reference src/boundary_sort.c:100-119).
+extern "C" uint64_t boundary_edges_dedup(BoundaryEdge *edges, uint64_t count) {
+  if (!edges || count <= 1) return count;
+  uint64_t out = count;
+  if (int rc = host_dedup(edges, count, &out)) {
+    fprintf(stderr, "boundary_edges_dedup: %s (rc=%d)\n", cc_hip_last_error(), rc);
+    exit(EXIT_FAILURE);
+  }
+  return out;
+}

@@ -78,6 +78,56 @@ int cc_hip_components_device(const uint64_t *d_row_ptr, const uint32_t *d_col_id
 int cc_hip_components_host(const uint64_t *row_ptr, const uint32_t *col_idx,
                            uint32_t n, uint64_t m, uint32_t *labels);
 
+/* ---- boundary records (device-resident forms of boundary_sort.h) -------- */
+
+/*
+ * Ascending sort of count 64-bit keys.  A BoundaryEdge array is such a key
+ * array (see vec_helpers.h).  d_alt is a second buffer of count keys used as
+ * the ping-pong target; if result_in_alt is non-NULL it receives 1 when the
+ * sorted keys ended in d_alt, else the result is copied back into d_keys.
+ * Blocks until the digit plan is read back (one small copy), then enqueues
+ * the passes on `stream`.  Replaces src/boundary_sort.c:93-98.
+ */
+int cc_hip_sort_u64_device(uint64_t *d_keys, uint64_t *d_alt, uint64_t count,
+                           int *result_in_alt, void *stream);
+
+/* Adjacent-unique of d_in[0..count) into d_out (must not alias d_in);
+ * *out_count receives the kept count (blocking).  src/boundary_sort.c:100-119. */
+int cc_hip_dedup_u64_device(const uint64_t *d_in, uint64_t count, uint64_t *d_out,
+                            uint64_t *out_count, void *stream);
+
+/*
+ * Ghost extraction over sorted, deduplicated records (remote << 32) | rep:
+ * d_ghost_vertices receives the distinct remotes in order (*ghost_count of
+ * them) and d_out_records the records with remote replaced by its ghost
+ * index (d_out_records must not alias d_records; blocking).
+ * src/cc_mpi.c:431-457.
+ */
+int cc_hip_ghost_extract_device(const uint64_t *d_records, uint64_t count,
+                                uint64_t *d_out_records, uint32_t *d_ghost_vertices,
+                                uint64_t *ghost_count, void *stream);
+
+/* ---- delta exchange pack ---------------------------------------------- */
+
+/*
+ * The pack half of exchangeplan_exchange_delta (src/exchange_plan.c:325-357)
+ * with the plan's send list and the label arrays resident on the device.
+ * seg_start (host, nseg+1 ascending entries from 0 to total_send_to) gives
+ * each destination's segment of d_send_to_flat in dests order.  For each
+ * entry whose current label (comp_label[comp_of[v - v_start]] if v is in
+ * [v_start, v_end), else v) is below d_prev_sent, d_prev_sent is lowered and
+ * the pair ((idx - segment start) << 32) | label is written to h_sendbuf
+ * (pinned host memory from cc_hip_host_alloc, or device memory; room for
+ * total_send_to pairs) in ascending idx order.  sendcounts/sdispls (nseg
+ * ints each) and *total_pairs receive the layout.  Blocking.
+ */
+int cc_hip_delta_pack_device(const uint32_t *d_send_to_flat, uint64_t total_send_to,
+                             const int *seg_start, int nseg,
+                             const uint32_t *d_comp_label, const uint32_t *d_comp_of,
+                             uint32_t v_start, uint32_t v_end, uint32_t *d_prev_sent,
+                             uint64_t *h_sendbuf, int *sendcounts, int *sdispls,
+                             uint64_t *total_pairs, void *stream);
+
 #ifdef __cplusplus
 }
 #endif

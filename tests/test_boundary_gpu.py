@@ -1,0 +1,163 @@
+"""Parity of the device boundary-record path (sort, dedup, ghost extraction)
+and the delta pack with the oracle, bit-exact."""
+import numpy as np
+import pytest
+
+import oracle
+from cc_mpi_amd import boundary as bd
+from cc_mpi_amd.cc import DeviceBuffer
+
+pytestmark = pytest.mark.gpu
+
+
+def records(count, seed, rep_bits=24, remote_bits=26):
+    """Records shaped like one rank's boundary list: rep < 2^rep_bits,
+    remote < 2^remote_bits."""
+    rng = np.random.default_rng(seed)
+    rep = rng.integers(0, 1 << rep_bits, count, dtype=np.uint64)
+    remote = rng.integers(0, 1 << remote_bits, count, dtype=np.uint64)
+    return (remote << np.uint64(32)) | rep
+
+
+SIZES = [0, 1, 2, 3, 64, 4095, 4096, 4097, 100_000, 1_000_003]
+
+
+@pytest.mark.parametrize("count", SIZES)
+def test_dropin_sort_matches_oracle(gpu, count):
+    k = records(count, seed=count)
+    got = k.copy()
+    bd.boundary_edges_sort(got)
+    assert np.array_equal(got, oracle.sort_u64(k))
+
+
+@pytest.mark.parametrize("count", SIZES)
+def test_dropin_dedup_matches_oracle(gpu, count):
+    k = oracle.sort_u64(records(count, seed=count + 1, rep_bits=8, remote_bits=9))
+    got = k.copy()
+    n = bd.boundary_edges_dedup(got)
+    want = oracle.dedup_u64(k)
+    assert n == want.size
+    assert np.array_equal(got[:n], want)
+
+
+@pytest.mark.parametrize("kind", ["full64", "equal", "high_only", "low_only", "two_values",
+                                  "descending", "sorted"])
+def test_sort_key_shapes(gpu, kind):
+    rng = np.random.default_rng(7)
+    n = 300_001
+    if kind == "full64":
+        k = rng.integers(0, np.iinfo(np.uint64).max, n, dtype=np.uint64, endpoint=True)
+    elif kind == "equal":
+        k = np.full(n, 0x0123456789ABCDEF, dtype=np.uint64)
+    elif kind == "high_only":
+        k = rng.integers(0, 256, n, dtype=np.uint64) << np.uint64(56)
+    elif kind == "low_only":
+        k = rng.integers(0, 4, n, dtype=np.uint64)
+    elif kind == "two_values":
+        k = np.where(rng.random(n) < 0.5, np.uint64(5), np.uint64(1 << 40)).astype(np.uint64)
+    elif kind == "descending":
+        k = np.arange(n, dtype=np.uint64)[::-1].copy() * np.uint64(0x10001)
+    else:
+        k = np.arange(n, dtype=np.uint64) * np.uint64(977)
+    got = k.copy()
+    bd.boundary_edges_sort(got)
+    assert np.array_equal(got, np.sort(k))
+
+
+def test_device_sort_ping_pong_and_repeat(gpu):
+    k = records(2_000_000, seed=3)
+    want = oracle.sort_u64(k)
+    a, b = DeviceBuffer.from_array(k), DeviceBuffer(k.nbytes)
+    for _ in range(3):
+        a.upload(k)
+        which = bd.sort_device(a, b, k.size)
+        out = np.empty_like(k)
+        (b if which else a).download(out)
+        assert np.array_equal(out, want)
+
+
+def test_device_dedup_and_ghosts(gpu):
+    k = oracle.sort_u64(records(1_500_000, seed=5, rep_bits=12, remote_bits=14))
+    a, b, c = DeviceBuffer.from_array(k), DeviceBuffer(k.nbytes), DeviceBuffer(k.nbytes)
+    n = bd.dedup_device(a, k.size, b)
+    uniq = oracle.dedup_u64(k)
+    assert n == uniq.size
+    got = np.empty(n, dtype=np.uint64)
+    b.download(got)
+    assert np.array_equal(got, uniq)
+    ghosts_d = DeviceBuffer(4 * n)
+    g = bd.ghost_extract_device(b, n, c, ghosts_d)
+    want_rec, want_ghosts = oracle.ghost_extract(uniq)
+    assert g == want_ghosts.size
+    gg = np.empty(g, dtype=np.uint32)
+    ghosts_d.download(gg)
+    rr = np.empty(n, dtype=np.uint64)
+    c.download(rr)
+    assert np.array_equal(gg, want_ghosts)
+    assert np.array_equal(rr, want_rec)
+
+
+def test_large_sort_properties(gpu):
+    # 200M records of the one-rank boundary shape: sortedness and an
+    # order-independent checksum (size-independent properties)
+    n = 200_000_000
+    rng = np.random.default_rng(11)
+    k = (rng.integers(0, 1 << 26, n, dtype=np.uint64) << np.uint64(32)) | \
+        rng.integers(0, 1 << 24, n, dtype=np.uint64)
+    xs, ss = np.bitwise_xor.reduce(k), int(k.sum(dtype=np.uint64))
+    a, b = DeviceBuffer.from_array(k), DeviceBuffer(k.nbytes)
+    which = bd.sort_device(a, b, n)
+    (b if which else a).download(k)
+    assert (k[1:] >= k[:-1]).all()
+    assert np.bitwise_xor.reduce(k) == xs and int(k.sum(dtype=np.uint64)) == ss
+
+
+def delta_case(seed, n_send=50_000, nseg=4, empty=()):
+    rng = np.random.default_rng(seed)
+    n_local, vs = 40_000, 1_000_000
+    ve = vs + n_local
+    comp_of = rng.integers(0, n_local, n_local).astype(np.uint32)
+    comp_label = rng.integers(0, 2_000_000, n_local).astype(np.uint32)
+    stf = rng.integers(vs - 3000, ve + 3000, n_send).astype(np.uint32)
+    cuts = np.sort(rng.choice(np.arange(1, n_send), nseg - 1, replace=False)).tolist()
+    seg = [0, *cuts, n_send]
+    for e in empty:  # make segment e empty
+        seg[e + 1] = seg[e]
+    seg = np.maximum.accumulate(np.array(seg)).astype(np.int32)
+    seg[-1] = n_send
+    prev = np.full(n_send, 0xFFFFFFFF, dtype=np.uint32)
+    return stf, seg, comp_label, comp_of, vs, ve, prev
+
+
+@pytest.mark.parametrize("seed,nseg,empty", [(0, 1, ()), (1, 4, ()), (2, 5, (0, 2)),
+                                             (3, 3, (2,)), (4, 200, ())])
+def test_delta_pack_rounds(gpu, seed, nseg, empty):
+    stf, seg, cl, co, vs, ve, prev = delta_case(seed, nseg=nseg, empty=empty)
+    d_stf, d_cl, d_co = (DeviceBuffer.from_array(x) for x in (stf, cl, co))
+    d_prev = DeviceBuffer.from_array(prev)
+    sendbuf = bd.PinnedBuffer(stf.size)
+    rng = np.random.default_rng(seed + 100)
+    prev_o = prev.copy()
+    for rnd in range(4):
+        pairs, cnt, dis = bd.delta_pack_device(d_stf, stf.size, seg, d_cl, d_co, vs, ve, d_prev,
+                                               sendbuf)
+        wp, wc, wd = oracle.delta_pack(stf, seg, cl, co, vs, ve, prev_o)
+        assert np.array_equal(pairs, wp), f"round {rnd}"
+        assert np.array_equal(cnt, wc) and np.array_equal(dis, wd)
+        got_prev = np.empty_like(prev)
+        d_prev.download(got_prev)
+        assert np.array_equal(got_prev, prev_o)
+        # labels only go down between exchanges; lower a random subset
+        cl = np.minimum(cl, rng.integers(0, 2_000_000, cl.size).astype(np.uint32)
+                        | (rng.random(cl.size) > 0.2).astype(np.uint32) * np.uint32(0xFFFFFFFF))
+        d_cl.upload(cl)
+    sendbuf.close()
+
+
+def test_delta_pack_nothing_to_send(gpu):
+    stf, seg, cl, co, vs, ve, prev = delta_case(9)
+    prev[:] = 0  # everything already sent at label 0
+    d = [DeviceBuffer.from_array(x) for x in (stf, cl, co, prev)]
+    sendbuf = bd.PinnedBuffer(stf.size)
+    pairs, cnt, dis = bd.delta_pack_device(d[0], stf.size, seg, d[1], d[2], vs, ve, d[3], sendbuf)
+    assert pairs.size == 0 and cnt.sum() == 0 and (dis == 0).all()

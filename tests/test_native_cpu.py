@@ -18,6 +18,7 @@ ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 HEADER_LIBS = {
     "cc.h": "libccmi355x.so",
     "cc_hip.h": "libccmi355x.so",
+    "boundary_sort.h": "libccmi355x.so",
     "graph.h": "libcchost.so",
     "graphgen.h": "libcchost.so",
 }
