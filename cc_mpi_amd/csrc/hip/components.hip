@@ -42,9 +42,6 @@
 
 #include <hip/hip_runtime.h>
 
-#include <mutex>
-#include <unordered_map>
-
 #include "ccx_internal.h"
 #include "../../../include/cc_hip.h"
 #include "../../../include/cc.h"
@@ -53,22 +50,40 @@ namespace {
 
 constexpr int kBlock = 256;
 constexpr int kWave = 64;
-constexpr int kSampledNeighbours = 2;
 constexpr int kGiantSamples = 1024;
 
-// Root of x with path halving.  Plain loads may return stale parents; a
-// stale parent is still an ancestor (pointers only decrease), so the walk
-// still ends at a root or at a vertex that the caller's atomicMin will reveal
-// as hooked.  The halving store only ever writes a grandparent.
-__device__ __forceinline__ uint32_t find_root(uint32_t x, uint32_t *__restrict__ p) {
+// Tuning knobs (process-wide; cc_hip_components_tune).  Neither changes the
+// result: sampling only decides how much of the work the cheap
+// per-vertex rounds do before the frontier sweep.
+int g_sampled_neighbours = 2;  // sampled hooking rounds (1..4)
+int g_blocks_per_cu = 8;       // grid cap for the vertex-parallel kernels
+
+// Root of x: follow parents until a fixed point.  Read-only: concurrent
+// walkers through a hub would otherwise all store to the hub's entry, and
+// same-address stores serialise in L2 while loads are served from L1.
+// Trees are flattened by the separate pointer-jumping passes instead.  A
+// stale parent read is still an ancestor (pointers only decrease), so the
+// walk ends at a root or at a vertex the caller's atomicMin reveals as hooked.
+__device__ __forceinline__ uint32_t find_root(uint32_t x, const uint32_t *__restrict__ p) {
   uint32_t px = p[x];
   while (px != x) {
-    uint32_t ppx = p[px];
-    if (ppx == px) return px;
-    p[x] = ppx;
-    x = ppx;
+    x = px;
     px = p[x];
   }
+  return x;
+}
+
+// Root of x, and whether the walk passed through (or ended at) g.
+__device__ __forceinline__ uint32_t find_root_via(uint32_t x, const uint32_t *__restrict__ p,
+                                                  uint32_t g, bool &via_g) {
+  bool hit = (x == g);
+  uint32_t px = p[x];
+  while (px != x) {
+    x = px;
+    hit |= (x == g);
+    px = p[x];
+  }
+  via_g = hit;
   return x;
 }
 
@@ -85,9 +100,12 @@ __device__ __forceinline__ void hook(uint32_t a, uint32_t b, uint32_t *__restric
   }
 }
 
+#define CCX_GRID_STRIDE(v, n)                                                       \
+  for (uint32_t v = blockIdx.x * kBlock + threadIdx.x; v < (n);                     \
+       v += gridDim.x * kBlock)
+
 __global__ __launch_bounds__(kBlock) void k_init(uint32_t n, uint32_t *__restrict__ p) {
-  const uint32_t v = blockIdx.x * kBlock + threadIdx.x;
-  if (v < n) p[v] = v;
+  CCX_GRID_STRIDE(v, n) p[v] = v;
 }
 
 // p[v] = min(v, col[row_ptr[v]]): links v to its first listed neighbour when
@@ -95,15 +113,15 @@ __global__ __launch_bounds__(kBlock) void k_init(uint32_t n, uint32_t *__restric
 __global__ __launch_bounds__(kBlock) void k_init_min(const uint64_t *__restrict__ row_ptr,
                                                      const uint32_t *__restrict__ col,
                                                      uint32_t n, uint32_t *__restrict__ p) {
-  const uint32_t v = blockIdx.x * kBlock + threadIdx.x;
-  if (v >= n) return;
-  uint32_t x = v;
-  const uint64_t b = row_ptr[v];
-  if (b < row_ptr[v + 1]) {
-    const uint32_t w = col[b];
-    if (w < x) x = w;
+  CCX_GRID_STRIDE(v, n) {
+    uint32_t x = v;
+    const uint64_t b = row_ptr[v];
+    if (b < row_ptr[v + 1]) {
+      const uint32_t w = col[b];
+      if (w < x) x = w;
+    }
+    p[v] = x;
   }
-  p[v] = x;
 }
 
 // Hook every vertex with its neighbour number r (if it has one).
@@ -111,27 +129,23 @@ __global__ __launch_bounds__(kBlock) void k_hook_neighbour(const uint64_t *__res
                                                            const uint32_t *__restrict__ col,
                                                            uint32_t n, uint32_t r,
                                                            uint32_t *__restrict__ p) {
-  const uint32_t v = blockIdx.x * kBlock + threadIdx.x;
-  if (v >= n) return;
-  const uint64_t b = row_ptr[v] + r;
-  if (b >= row_ptr[v + 1]) return;
-  const uint32_t w = col[b];
-  // neighbour 0 below v was linked by k_init_min already
-  if (w < n && !(r == 0 && w < v)) hook(v, w, p);
+  CCX_GRID_STRIDE(v, n) {
+    const uint64_t b = row_ptr[v] + r;
+    if (b >= row_ptr[v + 1]) continue;
+    const uint32_t w = col[b];
+    // neighbour 0 below v was linked by k_init_min already
+    if (w < n && !(r == 0 && w < v)) hook(v, w, p);
+  }
 }
 
 // Pointer jumping: p[v] = root(v).  Run with no hooks in flight.
 __global__ __launch_bounds__(kBlock) void k_compress(uint32_t n, uint32_t *__restrict__ p) {
-  const uint32_t v = blockIdx.x * kBlock + threadIdx.x;
-  if (v >= n) return;
-  uint32_t r = p[v];
-  if (r == v) return;
-  uint32_t rr = p[r];
-  while (rr != r) {
-    r = rr;
-    rr = p[r];
+  CCX_GRID_STRIDE(v, n) {
+    const uint32_t pv = p[v];
+    if (pv == v) continue;
+    const uint32_t r = find_root(pv, p);
+    if (r != pv) p[v] = r;
   }
-  p[v] = r;
 }
 
 __device__ __forceinline__ uint32_t sample_vertex(uint32_t i, uint32_t n) {
@@ -160,7 +174,7 @@ __global__ __launch_bounds__(kBlock) void k_sample_giant(uint32_t n, const uint3
   }
   __syncthreads();
   for (int i = t; i < kGiantSamples; i += kBlock) {
-    const uint32_t x = p[sample_vertex(i, n)];
+    const uint32_t x = find_root(sample_vertex(i, n), p);
     uint32_t h = (x * 0x9E3779B1u) >> 21;  // 11-bit slot
     for (;;) {
       const uint32_t prev = atomicCAS(&keys[h], kEmpty, x);
@@ -194,58 +208,87 @@ __global__ __launch_bounds__(kBlock) void k_sample_giant(uint32_t n, const uint3
   }
 }
 
-// Frontier sweep over the vertices not under the giant root.
+// Frontier sweep over the vertices not under the giant root g.  Each
+// vertex first resolves its root (and writes it back to its own entry,
+// which does the pointer jumping the sampled rounds left); a walk that
+// passes through g marks the vertex as a giant member and skips it.
 __global__ __launch_bounds__(kBlock) void k_hook_frontier(const uint64_t *__restrict__ row_ptr,
                                                           const uint32_t *__restrict__ col,
-                                                          uint32_t n, uint32_t *__restrict__ p,
+                                                          uint32_t n, uint32_t k_first,
+                                                          uint32_t *__restrict__ p,
                                                           const uint32_t *__restrict__ giant_slot) {
   __shared__ uint64_t s_excl[kBlock / kWave][kWave];
   __shared__ uint64_t s_begin[kBlock / kWave][kWave];
   const int lane = threadIdx.x & (kWave - 1);
   const int wid = threadIdx.x / kWave;
   const uint32_t giant = giant_slot[0];
-  const uint64_t base = (uint64_t)blockIdx.x * kBlock + (uint64_t)wid * kWave;
-  const uint64_t v = base + lane;
-
-  uint64_t work = 0, begin = 0;
-  if (v < n && p[v] != giant) {
-    const uint64_t b = row_ptr[v] + kSampledNeighbours;
-    const uint64_t e = row_ptr[v + 1];
-    if (e > b) {
-      work = e - b;
-      begin = b;
+  const uint64_t waves = (uint64_t)gridDim.x * (kBlock / kWave);
+  for (uint64_t chunk = (uint64_t)blockIdx.x * (kBlock / kWave) + wid; chunk * kWave < n;
+       chunk += waves) {
+    const uint64_t base = chunk * kWave;
+    const uint64_t v = base + lane;
+    uint64_t work = 0, begin = 0;
+    if (v < n) {
+      const uint32_t pv = p[v];
+      bool in_giant = false;
+      const uint32_t r = find_root_via((uint32_t)v, p, giant, in_giant);
+      if (r != pv) p[v] = r;
+      if (!in_giant) {
+        const uint64_t b = row_ptr[v] + k_first;
+        const uint64_t e = row_ptr[v + 1];
+        if (e > b) {
+          work = e - b;
+          begin = b;
+        }
+      }
     }
-  }
-  // Wave-inclusive prefix sum of the remaining degrees.
-  uint64_t incl = work;
+    // Wave-inclusive prefix sum of the remaining degrees.
+    uint64_t incl = work;
 #pragma unroll
-  for (int d = 1; d < kWave; d <<= 1) {
-    const uint64_t up = __shfl_up(incl, d, kWave);
-    if (lane >= d) incl += up;
-  }
-  const uint64_t total = __shfl(incl, kWave - 1, kWave);
-  if (total == 0) return;
-  s_excl[wid][lane] = incl - work;
-  s_begin[wid][lane] = begin;
-  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-  __builtin_amdgcn_wave_barrier();
-  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-
-  const uint64_t *ex = s_excl[wid];
-  for (uint64_t e = lane; e < total; e += kWave) {
-    int j = 0;
+    for (int d = 1; d < kWave; d <<= 1) {
+      const uint64_t up = __shfl_up(incl, d, kWave);
+      if (lane >= d) incl += up;
+    }
+    const uint64_t total = __shfl(incl, kWave - 1, kWave);
+    if (total == 0) continue;
+    s_excl[wid][lane] = incl - work;
+    s_begin[wid][lane] = begin;
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    const uint64_t *ex = s_excl[wid];
+    for (uint64_t e = lane; e < total; e += kWave) {
+      int j = 0;
 #pragma unroll
-    for (int step = kWave / 2; step > 0; step >>= 1)
-      if (ex[j + step] <= e) j += step;
-    const uint64_t idx = s_begin[wid][j] + (e - ex[j]);
-    const uint32_t w = col[idx];
-    if (w < n) hook((uint32_t)(base + j), w, p);
+      for (int step = kWave / 2; step > 0; step >>= 1)
+        if (ex[j + step] <= e) j += step;
+      const uint64_t idx = s_begin[wid][j] + (e - ex[j]);
+      const uint32_t w = col[idx];
+      if (w < n) hook((uint32_t)(base + j), w, p);
+    }
+    // the LDS rows are rewritten by the next chunk of this wave
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
   }
 }
 
-inline unsigned blocks_for(uint64_t n) { return (unsigned)((n + kBlock - 1) / kBlock); }
+inline unsigned blocks_for(uint64_t n, int cu_count) {
+  const uint64_t need = (n + kBlock - 1) / kBlock;
+  const uint64_t cap = (uint64_t)(cu_count > 0 ? cu_count : 256) * (uint64_t)g_blocks_per_cu;
+  return (unsigned)(need < cap ? need : cap);
+}
 
 }  // namespace
+
+extern "C" int cc_hip_components_tune(int sampled_neighbours, int blocks_per_cu) {
+  if (sampled_neighbours < 0 || sampled_neighbours > 4 || blocks_per_cu < 0 || blocks_per_cu > 64)
+    return ccx::set_error(4, "cc_hip_components_tune: out of range (%d, %d)", sampled_neighbours,
+                          blocks_per_cu);
+  if (sampled_neighbours > 0) g_sampled_neighbours = sampled_neighbours;
+  if (blocks_per_cu > 0) g_blocks_per_cu = blocks_per_cu;
+  return 0;
+}
 
 extern "C" int cc_hip_components_device(const uint64_t *d_row_ptr, const uint32_t *d_col_idx,
                                         uint32_t n, uint64_t m, uint32_t *d_labels, void *stream) {
@@ -260,23 +303,23 @@ extern "C" int cc_hip_components_device(const uint64_t *d_row_ptr, const uint32_
   ccx::StreamState *st = ccx::stream_state(s);
   if (!st) return 3;
   uint32_t *giant = st->d_words;
-  const unsigned nb = blocks_for(n);
+  const unsigned nb = blocks_for(n, c->cu_count);
+  const uint32_t rounds = (uint32_t)g_sampled_neighbours;
 
   if (m == 0) {
     CCX_LAUNCH("cc_init", s, k_init<<<nb, kBlock, 0, s>>>(n, d_labels));
-  } else {
-    CCX_LAUNCH("cc_init_min", s,
-               k_init_min<<<nb, kBlock, 0, s>>>(d_row_ptr, d_col_idx, n, d_labels));
-    for (uint32_t r = 0; r < (uint32_t)kSampledNeighbours; ++r) {
-      CCX_LAUNCH("cc_hook_sample", s,
-                 k_hook_neighbour<<<nb, kBlock, 0, s>>>(d_row_ptr, d_col_idx, n, r, d_labels));
-      CCX_LAUNCH("cc_compress", s, k_compress<<<nb, kBlock, 0, s>>>(n, d_labels));
-    }
-    CCX_LAUNCH("cc_sample_giant", s, k_sample_giant<<<1, kBlock, 0, s>>>(n, d_labels, giant));
-    CCX_LAUNCH("cc_hook_frontier", s,
-               k_hook_frontier<<<nb, kBlock, 0, s>>>(d_row_ptr, d_col_idx, n, d_labels, giant));
-    CCX_LAUNCH("cc_compress", s, k_compress<<<nb, kBlock, 0, s>>>(n, d_labels));
+    return 0;
   }
+  CCX_LAUNCH("cc_init_min", s, k_init_min<<<nb, kBlock, 0, s>>>(d_row_ptr, d_col_idx, n, d_labels));
+  for (uint32_t r = 0; r < rounds; ++r) {
+    if (r > 0) CCX_LAUNCH("cc_compress", s, k_compress<<<nb, kBlock, 0, s>>>(n, d_labels));
+    CCX_LAUNCH("cc_hook_sample", s,
+               k_hook_neighbour<<<nb, kBlock, 0, s>>>(d_row_ptr, d_col_idx, n, r, d_labels));
+  }
+  CCX_LAUNCH("cc_sample_giant", s, k_sample_giant<<<1, kBlock, 0, s>>>(n, d_labels, giant));
+  CCX_LAUNCH("cc_hook_frontier", s,
+             k_hook_frontier<<<nb, kBlock, 0, s>>>(d_row_ptr, d_col_idx, n, rounds, d_labels, giant));
+  CCX_LAUNCH("cc_compress", s, k_compress<<<nb, kBlock, 0, s>>>(n, d_labels));
   return 0;
 }
 
