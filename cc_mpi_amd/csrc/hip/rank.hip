@@ -1,0 +1,467 @@
+// Per-rank preprocessing and merge kernels of the distributed CC driver
+// (reference src/cc_mpi.c), so the driver keeps the owned CSR, the local
+// labels and the boundary records resident in HBM:
+//
+//   * split (src/cc_mpi.c:252-304 and 364-394): one pass counts, per owned
+//     row, the entries whose column is owned (local-local) and those that are
+//     not but are < n_global (boundary); exclusive scans give the local CSR
+//     offsets and the boundary-record offsets; a second pass emits the local
+//     CSR (columns rebased to local ids) and the boundary records
+//     (remote << 32) | li in row order;
+//   * rep fill: li -> comp_of[li] in each boundary record (:379, :391);
+//   * merge step (:547-622): comp_min = comp_label over the reps, hook every
+//     boundary record (comp_min[rep] = min(comp_min[rep], ghost_labels[gidx])),
+//     then for each rep follow up to four label hops (owned vertex:
+//     comp_label[comp_of[x - v_start]]; ghost: binary search of the sorted
+//     ghost list; anything else stops the walk) and lower comp_label;
+//   * labels + root count (:730-742): label of owned li is
+//     comp_label[comp_of[li]]; li is a root when that equals v_start + li;
+//   * delta unpack (src/exchange_plan.c:390-413).
+//
+// Every step computes a minimum or a pure function of its inputs, so the
+// atomics' order never changes a result.
+
+#include <hip/hip_runtime.h>
+
+#include <cstring>
+
+#include "ccx_internal.h"
+#include "lookback.h"
+#include "../../../include/cc_hip.h"
+
+using ccx::kWave;
+
+namespace {
+
+constexpr int kThreads = 256;
+constexpr int kWaves = kThreads / kWave;
+constexpr int kItems = 16;
+constexpr int kTile = kThreads * kItems;
+
+#define GRID_STRIDE(i, n) \
+  for (uint64_t i = (uint64_t)blockIdx.x * kThreads + threadIdx.x; i < (n); i += (uint64_t)gridDim.x * kThreads)
+
+unsigned grid_for(uint64_t n) {
+  const uint64_t need = (n + kThreads - 1) / kThreads;
+  ccx::Context *c = ccx::ctx();
+  const uint64_t cap = (uint64_t)((c && c->cu_count > 0) ? c->cu_count : 256) * 8;
+  return (unsigned)(need == 0 ? 1 : (need < cap ? need : cap));
+}
+
+// Exclusive scan of n uint32 counts into n+1 uint64 offsets
+// (out[0] = 0, out[n] = total), decoupled look-back over 4096-item tiles.
+__global__ __launch_bounds__(kThreads) void k_scan_u32(const uint32_t *__restrict__ in, uint64_t n,
+                                                       uint64_t *__restrict__ out,
+                                                       uint32_t *__restrict__ tile_ctr,
+                                                       uint64_t *__restrict__ status, uint32_t epoch) {
+  __shared__ uint32_t s_tile;
+  __shared__ uint64_t s_wsum[kWaves];
+  __shared__ uint64_t s_base;
+  const int t = threadIdx.x;
+  const int lane = t & (kWave - 1);
+  const int w = t / kWave;
+  if (t == 0) s_tile = atomicAdd(tile_ctr, 1u);
+  __syncthreads();
+  const uint64_t tile = s_tile;
+  if (tile * kTile >= n) return;
+  const uint64_t first = tile * kTile + (uint64_t)t * kItems;
+  uint32_t v[kItems];
+  uint64_t sum = 0;
+#pragma unroll
+  for (int k = 0; k < kItems; ++k) {
+    v[k] = (first + k < n) ? in[first + k] : 0u;
+    sum += v[k];
+  }
+  const uint64_t incl = ccx::wave_incl_scan<uint64_t>(sum, lane);
+  if (lane == kWave - 1) s_wsum[w] = incl;
+  __syncthreads();
+  uint64_t wprefix = 0, tile_total = 0;
+  for (int ww = 0; ww < kWaves; ++ww) {
+    if (ww < w) wprefix += s_wsum[ww];
+    tile_total += s_wsum[ww];
+  }
+  if (t == 0) {
+    uint64_t *my = status + tile;
+    uint64_t excl = 0;
+    if (tile == 0) {
+      ccx::status_store(my, ccx::tag_word(epoch, ccx::kFlagInclusive, tile_total));
+    } else {
+      ccx::status_store(my, ccx::tag_word(epoch, ccx::kFlagAggregate, tile_total));
+      excl = ccx::lookback(status, tile, 1, 0, epoch);
+      ccx::status_store(my, ccx::tag_word(epoch, ccx::kFlagInclusive, excl + tile_total));
+    }
+    s_base = excl;
+    if ((tile + 1) * kTile >= n) out[n] = excl + tile_total;
+  }
+  __syncthreads();
+  uint64_t run = s_base + wprefix + incl - sum;
+#pragma unroll
+  for (int k = 0; k < kItems; ++k) {
+    if (first + k < n) out[first + k] = run;
+    run += v[k];
+  }
+}
+
+__global__ __launch_bounds__(kThreads) void k_split_count(const uint64_t *__restrict__ row_ptr,
+                                                          const uint32_t *__restrict__ col,
+                                                          uint32_t n_local, uint32_t v_start,
+                                                          uint32_t v_end, uint32_t n_global,
+                                                          uint32_t *__restrict__ lcnt,
+                                                          uint32_t *__restrict__ bcnt) {
+  GRID_STRIDE(li, n_local) {
+    uint32_t l = 0, b = 0;
+    for (uint64_t j = row_ptr[li]; j < row_ptr[li + 1]; ++j) {
+      const uint32_t v = col[j];
+      if (v >= v_start && v < v_end) ++l;
+      else if (v < n_global) ++b;
+    }
+    lcnt[li] = l;
+    bcnt[li] = b;
+  }
+}
+
+__global__ __launch_bounds__(kThreads) void k_split_emit(const uint64_t *__restrict__ row_ptr,
+                                                         const uint32_t *__restrict__ col,
+                                                         uint32_t n_local, uint32_t v_start,
+                                                         uint32_t v_end, uint32_t n_global,
+                                                         const uint64_t *__restrict__ loff,
+                                                         const uint64_t *__restrict__ boff,
+                                                         uint32_t *__restrict__ lcol,
+                                                         uint64_t *__restrict__ brec) {
+  GRID_STRIDE(li, n_local) {
+    uint64_t wl = loff[li], wb = boff[li];
+    for (uint64_t j = row_ptr[li]; j < row_ptr[li + 1]; ++j) {
+      const uint32_t v = col[j];
+      if (v >= v_start && v < v_end) lcol[wl++] = v - v_start;
+      else if (v < n_global) brec[wb++] = ((uint64_t)v << 32) | (uint32_t)li;
+    }
+  }
+}
+
+__global__ __launch_bounds__(kThreads) void k_set_rep(uint64_t *__restrict__ rec, uint64_t nb,
+                                                      const uint32_t *__restrict__ comp_of) {
+  GRID_STRIDE(i, nb) {
+    const uint64_t r = rec[i];
+    rec[i] = (r & 0xFFFFFFFF00000000ull) | comp_of[(uint32_t)r];
+  }
+}
+
+__global__ __launch_bounds__(kThreads) void k_iota(uint32_t *__restrict__ a, uint32_t n, uint32_t base) {
+  GRID_STRIDE(i, n) a[i] = base + (uint32_t)i;
+}
+
+// comp_min[r] = comp_label[r] for every rep r (comp_of[r] == r).
+__global__ __launch_bounds__(kThreads) void k_merge_init(const uint32_t *__restrict__ comp_of,
+                                                         const uint32_t *__restrict__ comp_label,
+                                                         uint32_t *__restrict__ comp_min,
+                                                         uint32_t n_local) {
+  GRID_STRIDE(i, n_local) {
+    if (comp_of[i] == (uint32_t)i) comp_min[i] = comp_label[i];
+  }
+}
+
+// comp_min[rep] = min(comp_min[rep], ghost_labels[gidx]) over the records
+// (gidx << 32) | rep.
+__global__ __launch_bounds__(kThreads) void k_merge_hook(const uint64_t *__restrict__ rec, uint64_t nb,
+                                                         const uint32_t *__restrict__ ghost_labels,
+                                                         uint32_t *__restrict__ comp_min) {
+  GRID_STRIDE(i, nb) {
+    const uint64_t r = rec[i];
+    const uint32_t rep = (uint32_t)r;
+    const uint32_t lbl = ghost_labels[r >> 32];
+    if (lbl < comp_min[rep]) atomicMin(&comp_min[rep], lbl);
+  }
+}
+
+__device__ __forceinline__ bool label_of(uint32_t x, const uint32_t *comp_label,
+                                         const uint32_t *comp_of, uint32_t v_start, uint32_t v_end,
+                                         const uint32_t *ghosts, uint32_t ghost_count,
+                                         const uint32_t *ghost_labels, uint32_t &y) {
+  if (x >= v_start && x < v_end) {
+    y = comp_label[comp_of[x - v_start]];
+    return true;
+  }
+  uint32_t lo = 0, hi = ghost_count;
+  while (lo < hi) {
+    const uint32_t mid = lo + (hi - lo) / 2;
+    if (ghosts[mid] < x) lo = mid + 1;
+    else hi = mid;
+  }
+  if (lo < ghost_count && ghosts[lo] == x) {
+    y = ghost_labels[lo];
+    return true;
+  }
+  return false;
+}
+
+// Label hops and the comp_label update for every rep (src/cc_mpi.c:588-621).
+__global__ __launch_bounds__(kThreads) void k_merge_update(const uint32_t *__restrict__ comp_of,
+                                                           uint32_t *__restrict__ comp_label,
+                                                           uint32_t *__restrict__ comp_min,
+                                                           uint32_t n_local, uint32_t v_start,
+                                                           uint32_t v_end,
+                                                           const uint32_t *__restrict__ ghosts,
+                                                           uint32_t ghost_count,
+                                                           const uint32_t *__restrict__ ghost_labels,
+                                                           uint32_t *__restrict__ changed) {
+  bool any = false;
+  GRID_STRIDE(i, n_local) {
+    if (comp_of[i] != (uint32_t)i) continue;
+    uint32_t x = comp_min[i];
+    for (int t = 0; t < 4; ++t) {
+      uint32_t y;
+      if (!label_of(x, comp_label, comp_of, v_start, v_end, ghosts, ghost_count, ghost_labels, y))
+        break;
+      if (y >= x) break;
+      x = y;
+    }
+    if (x < comp_min[i]) comp_min[i] = x;
+    if (comp_min[i] < comp_label[i]) {
+      comp_label[i] = comp_min[i];
+      any = true;
+    }
+  }
+  if (__ballot(any) != 0 && (threadIdx.x & (kWave - 1)) == 0) atomicOr(changed, 1u);
+}
+
+__global__ __launch_bounds__(kThreads) void k_rank_labels(const uint32_t *__restrict__ comp_label,
+                                                          const uint32_t *__restrict__ comp_of,
+                                                          uint32_t n_local, uint32_t v_start,
+                                                          uint32_t *__restrict__ labels,
+                                                          unsigned long long *__restrict__ roots) {
+  uint32_t mine = 0;
+  GRID_STRIDE(i, n_local) {
+    const uint32_t l = comp_label[comp_of[i]];
+    if (labels) labels[i] = l;
+    mine += (l == v_start + (uint32_t)i);
+  }
+  for (int d = kWave / 2; d > 0; d >>= 1) mine += __shfl_xor(mine, d, kWave);
+  if ((threadIdx.x & (kWave - 1)) == 0 && mine) atomicAdd(roots, (unsigned long long)mine);
+}
+
+// Received pairs (pos << 32) | lbl, grouped per source k at rdispls[k];
+// ghost_labels[need_from_gidx_flat[need_base[k] + pos]] = lbl.
+__global__ __launch_bounds__(kThreads) void k_delta_unpack(const uint64_t *__restrict__ pairs,
+                                                           uint64_t total,
+                                                           const int *__restrict__ rdispls,
+                                                           const int *__restrict__ need_base,
+                                                           int nsrc,
+                                                           const uint32_t *__restrict__ gidx_flat,
+                                                           uint32_t *__restrict__ ghost_labels) {
+  GRID_STRIDE(i, total) {
+    // last source whose displacement is <= i
+    int lo = 0, hi = nsrc - 1;
+    while (lo < hi) {
+      const int mid = (lo + hi + 1) >> 1;
+      if ((uint64_t)rdispls[mid] <= i) lo = mid;
+      else hi = mid - 1;
+    }
+    const uint64_t pr = pairs[i];
+    ghost_labels[gidx_flat[need_base[lo] + (int)(pr >> 32)]] = (uint32_t)pr;
+  }
+}
+
+}  // namespace
+
+extern "C" int cc_hip_exclusive_scan_u32_device(const uint32_t *d_in, uint64_t n, uint64_t *d_out,
+                                                void *stream) {
+  if (!ccx::ctx()) return 2;
+  hipStream_t s = ccx::pick_stream(stream);
+  if (n == 0) {
+    CCX_TRY(hipMemsetAsync(d_out, 0, sizeof(uint64_t), s));
+    return 0;
+  }
+  ccx::StreamState *st = ccx::stream_state(s);
+  if (!st) return 3;
+  const uint64_t tiles = (n + kTile - 1) / kTile;
+  if (int rc = ccx::ensure_status(st, tiles)) return rc;
+  uint32_t epoch;
+  if (int rc = ccx::next_epoch(st, &epoch)) return rc;
+  uint32_t *ctr = st->d_words + 9;
+  CCX_TRY(hipMemsetAsync(ctr, 0, sizeof(uint32_t), s));
+  CCX_LAUNCH("scan_u32", s,
+             k_scan_u32<<<(unsigned)tiles, kThreads, 0, s>>>(d_in, n, d_out, ctr, st->d_status, epoch));
+  return 0;
+}
+
+extern "C" int cc_hip_split_device(const uint64_t *d_row_ptr, const uint32_t *d_col_idx,
+                                   uint32_t n_local, uint32_t v_start, uint32_t v_end,
+                                   uint32_t n_global, uint64_t *d_lrow, uint32_t *d_lcol,
+                                   uint64_t *d_boff, uint64_t *d_brec, uint64_t *m_local_local,
+                                   uint64_t *n_boundary, void *stream) {
+  if (!ccx::ctx()) return 2;
+  hipStream_t s = ccx::pick_stream(stream);
+  ccx::StreamState *st = ccx::stream_state(s);
+  if (!st) return 3;
+  if (n_local == 0) {
+    CCX_TRY(hipMemsetAsync(d_lrow, 0, sizeof(uint64_t), s));
+    *m_local_local = 0;
+    *n_boundary = 0;
+    return 0;
+  }
+  // per-row counts in the stream temp buffer: two uint32 arrays
+  const size_t cbytes = ((size_t)n_local * sizeof(uint32_t) + 255) & ~(size_t)255;
+  char *tmp = (char *)ccx::stream_tmp(st, 2 * cbytes);
+  if (!tmp) return 3;
+  uint32_t *lcnt = (uint32_t *)tmp, *bcnt = (uint32_t *)(tmp + cbytes);
+  const unsigned g = grid_for(n_local);
+  CCX_LAUNCH("split_count", s,
+             k_split_count<<<g, kThreads, 0, s>>>(d_row_ptr, d_col_idx, n_local, v_start, v_end,
+                                                  n_global, lcnt, bcnt));
+  if (int rc = cc_hip_exclusive_scan_u32_device(lcnt, n_local, d_lrow, s)) return rc;
+  if (int rc = cc_hip_exclusive_scan_u32_device(bcnt, n_local, d_boff, s)) return rc;
+  CCX_LAUNCH("split_emit", s,
+             k_split_emit<<<g, kThreads, 0, s>>>(d_row_ptr, d_col_idx, n_local, v_start, v_end,
+                                                 n_global, d_lrow, d_boff, d_lcol, d_brec));
+  uint64_t *h = st->h_stage;
+  CCX_TRY(hipMemcpyAsync(h, d_lrow + n_local, sizeof(uint64_t), hipMemcpyDeviceToHost, s));
+  CCX_TRY(hipMemcpyAsync(h + 1, d_boff + n_local, sizeof(uint64_t), hipMemcpyDeviceToHost, s));
+  CCX_TRY(hipStreamSynchronize(s));
+  *m_local_local = h[0];
+  *n_boundary = h[1];
+  return 0;
+}
+
+extern "C" int cc_hip_split_count_device(const uint64_t *d_row_ptr, const uint32_t *d_col_idx,
+                                         uint32_t n_local, uint32_t v_start, uint32_t v_end,
+                                         uint32_t n_global, uint64_t *d_lrow, uint64_t *d_boff,
+                                         uint64_t *m_local_local, uint64_t *n_boundary,
+                                         void *stream) {
+  if (!ccx::ctx()) return 2;
+  hipStream_t s = ccx::pick_stream(stream);
+  ccx::StreamState *st = ccx::stream_state(s);
+  if (!st) return 3;
+  if (n_local == 0) {
+    *m_local_local = 0;
+    *n_boundary = 0;
+    return 0;
+  }
+  const size_t cbytes = ((size_t)n_local * sizeof(uint32_t) + 255) & ~(size_t)255;
+  char *tmp = (char *)ccx::stream_tmp(st, 2 * cbytes);
+  if (!tmp) return 3;
+  uint32_t *lcnt = (uint32_t *)tmp, *bcnt = (uint32_t *)(tmp + cbytes);
+  CCX_LAUNCH("split_count", s,
+             k_split_count<<<grid_for(n_local), kThreads, 0, s>>>(d_row_ptr, d_col_idx, n_local,
+                                                                  v_start, v_end, n_global, lcnt,
+                                                                  bcnt));
+  if (int rc = cc_hip_exclusive_scan_u32_device(lcnt, n_local, d_lrow, s)) return rc;
+  if (int rc = cc_hip_exclusive_scan_u32_device(bcnt, n_local, d_boff, s)) return rc;
+  uint64_t *h = st->h_stage;
+  CCX_TRY(hipMemcpyAsync(h, d_lrow + n_local, sizeof(uint64_t), hipMemcpyDeviceToHost, s));
+  CCX_TRY(hipMemcpyAsync(h + 1, d_boff + n_local, sizeof(uint64_t), hipMemcpyDeviceToHost, s));
+  CCX_TRY(hipStreamSynchronize(s));
+  *m_local_local = h[0];
+  *n_boundary = h[1];
+  return 0;
+}
+
+extern "C" int cc_hip_split_emit_device(const uint64_t *d_row_ptr, const uint32_t *d_col_idx,
+                                        uint32_t n_local, uint32_t v_start, uint32_t v_end,
+                                        uint32_t n_global, const uint64_t *d_lrow,
+                                        const uint64_t *d_boff, uint32_t *d_lcol,
+                                        uint64_t *d_brec, void *stream) {
+  if (!ccx::ctx()) return 2;
+  if (n_local == 0) return 0;
+  hipStream_t s = ccx::pick_stream(stream);
+  CCX_LAUNCH("split_emit", s,
+             k_split_emit<<<grid_for(n_local), kThreads, 0, s>>>(d_row_ptr, d_col_idx, n_local,
+                                                                 v_start, v_end, n_global, d_lrow,
+                                                                 d_boff, d_lcol, d_brec));
+  return 0;
+}
+
+extern "C" int cc_hip_boundary_set_rep_device(uint64_t *d_records, uint64_t count,
+                                              const uint32_t *d_comp_of, void *stream) {
+  if (!ccx::ctx()) return 2;
+  if (count == 0) return 0;
+  hipStream_t s = ccx::pick_stream(stream);
+  CCX_LAUNCH("boundary_set_rep", s,
+             k_set_rep<<<grid_for(count), kThreads, 0, s>>>(d_records, count, d_comp_of));
+  return 0;
+}
+
+extern "C" int cc_hip_iota_u32_device(uint32_t *d_out, uint32_t n, uint32_t base, void *stream) {
+  if (!ccx::ctx()) return 2;
+  if (n == 0) return 0;
+  hipStream_t s = ccx::pick_stream(stream);
+  CCX_LAUNCH("iota", s, k_iota<<<grid_for(n), kThreads, 0, s>>>(d_out, n, base));
+  return 0;
+}
+
+extern "C" int cc_hip_merge_step_device(const uint64_t *d_records, uint64_t count,
+                                        const uint32_t *d_ghost_vertices,
+                                        const uint32_t *d_ghost_labels, uint32_t ghost_count,
+                                        const uint32_t *d_comp_of, uint32_t *d_comp_label,
+                                        uint32_t *d_comp_min, uint32_t n_local, uint32_t v_start,
+                                        uint32_t v_end, int *changed, void *stream) {
+  if (!ccx::ctx()) return 2;
+  if (changed) *changed = 0;
+  if (n_local == 0) return 0;
+  hipStream_t s = ccx::pick_stream(stream);
+  ccx::StreamState *st = ccx::stream_state(s);
+  if (!st) return 3;
+  uint32_t *d_changed = st->d_words + 24;
+  CCX_TRY(hipMemsetAsync(d_changed, 0, sizeof(uint32_t), s));
+  const unsigned g = grid_for(n_local);
+  CCX_LAUNCH("merge_init", s,
+             k_merge_init<<<g, kThreads, 0, s>>>(d_comp_of, d_comp_label, d_comp_min, n_local));
+  if (count > 0)
+    CCX_LAUNCH("merge_hook", s,
+               k_merge_hook<<<grid_for(count), kThreads, 0, s>>>(d_records, count, d_ghost_labels,
+                                                                 d_comp_min));
+  CCX_LAUNCH("merge_update", s,
+             k_merge_update<<<g, kThreads, 0, s>>>(d_comp_of, d_comp_label, d_comp_min, n_local,
+                                                   v_start, v_end, d_ghost_vertices, ghost_count,
+                                                   d_ghost_labels, d_changed));
+  uint32_t *h = (uint32_t *)st->h_stage;
+  CCX_TRY(hipMemcpyAsync(h, d_changed, sizeof(uint32_t), hipMemcpyDeviceToHost, s));
+  CCX_TRY(hipStreamSynchronize(s));
+  if (changed) *changed = (int)h[0];
+  return 0;
+}
+
+extern "C" int cc_hip_rank_labels_device(const uint32_t *d_comp_label, const uint32_t *d_comp_of,
+                                         uint32_t n_local, uint32_t v_start, uint32_t *d_labels,
+                                         uint64_t *roots, void *stream) {
+  if (!ccx::ctx()) return 2;
+  if (roots) *roots = 0;
+  if (n_local == 0) return 0;
+  hipStream_t s = ccx::pick_stream(stream);
+  ccx::StreamState *st = ccx::stream_state(s);
+  if (!st) return 3;
+  unsigned long long *d_roots = (unsigned long long *)(st->d_words + 26);
+  CCX_TRY(hipMemsetAsync(d_roots, 0, sizeof(uint64_t), s));
+  CCX_LAUNCH("rank_labels", s,
+             k_rank_labels<<<grid_for(n_local), kThreads, 0, s>>>(d_comp_label, d_comp_of, n_local,
+                                                                  v_start, d_labels, d_roots));
+  CCX_TRY(hipMemcpyAsync(st->h_stage, d_roots, sizeof(uint64_t), hipMemcpyDeviceToHost, s));
+  CCX_TRY(hipStreamSynchronize(s));
+  if (roots) *roots = st->h_stage[0];
+  return 0;
+}
+
+extern "C" int cc_hip_delta_unpack_device(const uint64_t *h_pairs, uint64_t total,
+                                          const int *rdispls, const int *need_base, int nsrc,
+                                          const uint32_t *d_need_from_gidx_flat,
+                                          uint32_t *d_ghost_labels, void *stream) {
+  if (!ccx::ctx()) return 2;
+  if (total == 0 || nsrc <= 0) return 0;
+  if ((size_t)nsrc * 2 * sizeof(int) > 65536)
+    return ccx::set_error(4, "cc_hip_delta_unpack_device: too many sources (%d)", nsrc);
+  hipStream_t s = ccx::pick_stream(stream);
+  ccx::StreamState *st = ccx::stream_state(s);
+  if (!st) return 3;
+  int *d_meta = (int *)ccx::stream_tmp(st, (size_t)nsrc * 2 * sizeof(int));
+  if (!d_meta) return 3;
+  int *h = (int *)st->h_stage;
+  memcpy(h, rdispls, (size_t)nsrc * sizeof(int));
+  memcpy(h + nsrc, need_base, (size_t)nsrc * sizeof(int));
+  CCX_TRY(hipMemcpyAsync(d_meta, h, (size_t)nsrc * 2 * sizeof(int), hipMemcpyHostToDevice, s));
+  CCX_LAUNCH("delta_unpack", s,
+             k_delta_unpack<<<grid_for(total), kThreads, 0, s>>>(h_pairs, total, d_meta,
+                                                                 d_meta + nsrc, nsrc,
+                                                                 d_need_from_gidx_flat,
+                                                                 d_ghost_labels));
+  CCX_TRY(hipStreamSynchronize(s));
+  return 0;
+}

@@ -396,6 +396,8 @@ int sort_u64(uint64_t *keys, uint64_t *alt, uint64_t n, int *in_alt, hipStream_t
     }
   }
   CCX_TRY(hipMemcpyAsync(d_goff, offs, 8 * kBins * sizeof(uint64_t), hipMemcpyHostToDevice, s));
+  // the staging buffer is reused by the next call: let the copy land first
+  CCX_TRY(hipStreamSynchronize(s));
 
   const uint64_t tiles = tiles_for(n);
   if (tiles > 0xFFFFFFFFull) return set_error(4, "sort: input too large");

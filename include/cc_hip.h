@@ -134,6 +134,68 @@ int cc_hip_delta_pack_device(const uint32_t *d_send_to_flat, uint64_t total_send
                              uint64_t *h_sendbuf, int *sendcounts, int *sdispls,
                              uint64_t *total_pairs, void *stream);
 
+/* Received-pair unpack (src/exchange_plan.c:390-413): pairs (pos << 32) |
+ * lbl grouped per source k from rdispls[k] (h_pairs: pinned host or device
+ * memory); d_ghost_labels[d_need_from_gidx_flat[need_base[k] + pos]] = lbl.
+ * Blocking. */
+int cc_hip_delta_unpack_device(const uint64_t *h_pairs, uint64_t total, const int *rdispls,
+                               const int *need_base, int nsrc,
+                               const uint32_t *d_need_from_gidx_flat, uint32_t *d_ghost_labels,
+                               void *stream);
+
+/* ---- per-rank driver building blocks (src/cc_mpi.c) -------------------- */
+
+/* d_out[0..n] = exclusive prefix sums of d_in[0..n) (d_out[n] = total). */
+int cc_hip_exclusive_scan_u32_device(const uint32_t *d_in, uint64_t n, uint64_t *d_out,
+                                     void *stream);
+
+/*
+ * Local-local split of an owned CSR with global column ids
+ * (src/cc_mpi.c:252-304, 364-394).  Count phase: d_lrow[0..n_local] gets the
+ * local CSR offsets (columns in [v_start, v_end)), d_boff[0..n_local] the
+ * boundary-record offsets (other columns < n_global); the totals are
+ * returned (blocking).  Emit phase: d_lcol gets the rebased local columns
+ * and d_brec the records (remote << 32) | li, both in row order.
+ */
+int cc_hip_split_count_device(const uint64_t *d_row_ptr, const uint32_t *d_col_idx,
+                              uint32_t n_local, uint32_t v_start, uint32_t v_end,
+                              uint32_t n_global, uint64_t *d_lrow, uint64_t *d_boff,
+                              uint64_t *m_local_local, uint64_t *n_boundary, void *stream);
+int cc_hip_split_emit_device(const uint64_t *d_row_ptr, const uint32_t *d_col_idx,
+                             uint32_t n_local, uint32_t v_start, uint32_t v_end,
+                             uint32_t n_global, const uint64_t *d_lrow, const uint64_t *d_boff,
+                             uint32_t *d_lcol, uint64_t *d_brec, void *stream);
+/* Both phases; d_boff is caller-provided scratch of n_local+1 entries. */
+int cc_hip_split_device(const uint64_t *d_row_ptr, const uint32_t *d_col_idx, uint32_t n_local,
+                        uint32_t v_start, uint32_t v_end, uint32_t n_global, uint64_t *d_lrow,
+                        uint32_t *d_lcol, uint64_t *d_boff, uint64_t *d_brec,
+                        uint64_t *m_local_local, uint64_t *n_boundary, void *stream);
+
+/* records (remote << 32) | li  ->  (remote << 32) | comp_of[li]. */
+int cc_hip_boundary_set_rep_device(uint64_t *d_records, uint64_t count,
+                                   const uint32_t *d_comp_of, void *stream);
+
+/* d_out[i] = base + i. */
+int cc_hip_iota_u32_device(uint32_t *d_out, uint32_t n, uint32_t base, void *stream);
+
+/*
+ * One merge round (src/cc_mpi.c:547-622) over records (gidx << 32) | rep:
+ * comp_min = comp_label on the reps, comp_min[rep] lowered by every
+ * record's ghost label, then per rep up to four label hops and the
+ * comp_label update.  *changed = 1 if any comp_label dropped (blocking).
+ */
+int cc_hip_merge_step_device(const uint64_t *d_records, uint64_t count,
+                             const uint32_t *d_ghost_vertices, const uint32_t *d_ghost_labels,
+                             uint32_t ghost_count, const uint32_t *d_comp_of,
+                             uint32_t *d_comp_label, uint32_t *d_comp_min, uint32_t n_local,
+                             uint32_t v_start, uint32_t v_end, int *changed, void *stream);
+
+/* d_labels[li] = comp_label[comp_of[li]] (if d_labels non-NULL) and *roots =
+ * #{li : that label == v_start + li} (src/cc_mpi.c:730-736; blocking). */
+int cc_hip_rank_labels_device(const uint32_t *d_comp_label, const uint32_t *d_comp_of,
+                              uint32_t n_local, uint32_t v_start, uint32_t *d_labels,
+                              uint64_t *roots, void *stream);
+
 #ifdef __cplusplus
 }
 #endif
