@@ -86,11 +86,31 @@ class DeviceBuffer:
 class DeviceGraph:
     """A CSR resident in device memory plus a device label array."""
 
-    def __init__(self, G: Graph):
+    def __init__(self, G: Graph | None):
+        if G is None:
+            return
         self.n, self.m = G.n, G.m
         self.row_ptr = DeviceBuffer.from_array(G.row_ptr)
         self.col_idx = DeviceBuffer.from_array(G.col_idx) if G.m else DeviceBuffer(4)
         self.labels = DeviceBuffer(max(4, 4 * G.n))
+
+    @classmethod
+    def adopt(cls, n: int, m: int, row_ptr: "DeviceBuffer", col_idx: "DeviceBuffer") -> "DeviceGraph":
+        """Wrap device arrays built elsewhere (takes ownership)."""
+        self = cls(None)
+        self.n, self.m = n, m
+        self.row_ptr, self.col_idx = row_ptr, col_idx
+        self.labels = DeviceBuffer(max(4, 4 * n))
+        return self
+
+    def to_host(self) -> Graph:
+        """Copy the CSR back to host memory."""
+        rp = np.empty(self.n + 1, dtype=np.uint64)
+        ci = np.empty(self.m, dtype=np.uint32)
+        self.row_ptr.download(rp)
+        if self.m:
+            self.col_idx.download(ci)
+        return Graph(rp, ci)
 
     def solve_async(self, stream=None) -> None:
         """Enqueue a solve on `stream` (None = the library stream)."""

@@ -195,6 +195,32 @@ int ccgen_csr_from_pairs(uint32_t n, const uint32_t *src, const uint32_t *dst, u
   return 0;
 }
 
+/* The vertex renaming the generators apply when `permute` is set. */
+int ccgen_permutation(uint32_t n, uint64_t seed, uint32_t *out)
+{
+  uint32_t *perm = make_permutation(n, seed ^ 0x5EED5EED5EED5EEDull);
+  if (!perm)
+    return 1;
+  memcpy(out, perm, (size_t)n * sizeof(uint32_t));
+  free(perm);
+  return 0;
+}
+
+/* Chung-Lu weight prefix sums (the inverse-CDF table ccgen_chung_lu
+ * searches); returns the total. */
+double ccgen_chung_lu_cdf(uint32_t n, double gamma, double *cdf)
+{
+  const double expo = -1.0 / (gamma - 1.0);
+  const double i0 = (double)(n >> 20) + 1.0;
+  double acc = 0.0;
+  for (uint32_t i = 0; i < n; ++i)
+  {
+    acc += pow((double)i + i0, expo);
+    cdf[i] = acc;
+  }
+  return acc;
+}
+
 static int finish_pairs(uint32_t n, uint32_t *src, uint32_t *dst, uint64_t count, int permute,
                         uint64_t seed, CSRGraph *out)
 {
@@ -349,14 +375,7 @@ int ccgen_chung_lu(uint32_t n, uint64_t pairs, double gamma, uint64_t seed, int 
     free(dst);
     return 1;
   }
-  const double expo = -1.0 / (gamma - 1.0);
-  const double i0 = (double)(n >> 20) + 1.0;
-  double acc = 0.0;
-  for (uint32_t i = 0; i < n; ++i)
-  {
-    acc += pow((double)i + i0, expo);
-    cdf[i] = acc;
-  }
+  const double acc = ccgen_chung_lu_cdf(n, gamma, cdf);
 #pragma omp parallel for schedule(static)
   for (uint64_t e = 0; e < 2 * pairs; ++e)
   {

@@ -196,6 +196,33 @@ int cc_hip_rank_labels_device(const uint32_t *d_comp_label, const uint32_t *d_co
                               uint32_t n_local, uint32_t v_start, uint32_t *d_labels,
                               uint64_t *roots, void *stream);
 
+/* ---- device-side construction of the seeded inputs -------------------- */
+
+/*
+ * Undirected pairs of a seeded generator, identical pair for pair to the
+ * host generators of graphgen.h.  kind 0 = Erdos-Renyi {n, pairs};
+ * 1 = R-MAT {scale, edge_factor, ta, tab, tabc} (16-bit thresholds);
+ * 2 = grid {rows, cols, keep threshold} (2 slots per vertex);
+ * 3 = Chung-Lu {n, pairs, bit pattern of the double cdf total} with d_cdf
+ * from ccgen_chung_lu_cdf.  *count receives the number of pairs written.
+ */
+int cc_hip_gen_pairs_device(int kind, const uint64_t *params, const double *d_cdf,
+                            uint64_t seed, uint32_t *d_src, uint32_t *d_dst, uint64_t *count,
+                            void *stream);
+
+/*
+ * Symmetric simple CSR from `count` undirected pairs: self-loops and
+ * out-of-range pairs dropped, both directions kept, duplicates removed,
+ * rows sorted; optional renaming d_perm (n entries).  d_keys and d_alt
+ * hold 2*count keys each; d_col needs room for 2*count entries (it may be
+ * the pair buffer itself, read before it is written); d_row_ptr n+1.
+ * *m_out receives the entry count (blocking).
+ */
+int cc_hip_csr_from_pairs_device(uint32_t n, const uint32_t *d_src, const uint32_t *d_dst,
+                                 uint64_t count, const uint32_t *d_perm, uint64_t *d_keys,
+                                 uint64_t *d_alt, uint64_t *d_row_ptr, uint32_t *d_col,
+                                 uint64_t *m_out, void *stream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -29,6 +29,11 @@ int ccgen_grid(uint32_t rows, uint32_t cols, double keep, uint64_t seed, int per
 int ccgen_chung_lu(uint32_t n, uint64_t pairs, double gamma, uint64_t seed, int permute,
                    CSRGraph *out);
 
+/* The vertex renaming the generators apply when `permute` is set. */
+int ccgen_permutation(uint32_t n, uint64_t seed, uint32_t *out);
+/* Chung-Lu inverse-CDF table (n doubles); returns the weight total. */
+double ccgen_chung_lu_cdf(uint32_t n, double gamma, double *cdf);
+
 #ifdef __cplusplus
 }
 #endif
