@@ -57,6 +57,13 @@ constexpr int kGiantSamples = 1024;
 // per-vertex rounds do before the frontier sweep.
 int g_sampled_neighbours = 2;  // sampled hooking rounds (1..4)
 int g_blocks_per_cu = 8;       // grid cap for the vertex-parallel kernels
+int g_hook_mode = 1;           // 0: plain loads, 1: L2-confirmed roots, 2: all loads from L2
+
+// Parent read that bypasses the CU's L1 (an agent-scope atomic load), so it
+// sees every hook that has reached L2.
+__device__ __forceinline__ uint32_t load_l2(const uint32_t *p) {
+  return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
 
 // Root of x: follow parents until a fixed point.  Read-only: concurrent
 // walkers through a hub would otherwise all store to the hub's entry, and
@@ -69,6 +76,15 @@ __device__ __forceinline__ uint32_t find_root(uint32_t x, const uint32_t *__rest
   while (px != x) {
     x = px;
     px = p[x];
+  }
+  return x;
+}
+
+__device__ __forceinline__ uint32_t find_root_l2(uint32_t x, const uint32_t *__restrict__ p) {
+  uint32_t px = load_l2(&p[x]);
+  while (px != x) {
+    x = px;
+    px = load_l2(&p[x]);
   }
   return x;
 }
@@ -87,16 +103,31 @@ __device__ __forceinline__ uint32_t find_root_via(uint32_t x, const uint32_t *__
   return x;
 }
 
+// Union of the trees of a and b.  Many members of one tree often hook
+// towards the same other tree in the same launch; with plain loads their
+// CUs keep seeing the tree's root as unhooked (stale L1 lines) and every one
+// of them pays a same-address atomic.  Mode 1 confirms, with one L2 read,
+// that the larger root is still a root before the atomic, and follows any
+// change with L2 reads; mode 2 walks with L2 reads throughout.
+template <int Mode>
 __device__ __forceinline__ void hook(uint32_t a, uint32_t b, uint32_t *__restrict__ p) {
-  uint32_t ra = find_root(a, p);
-  uint32_t rb = find_root(b, p);
+  uint32_t ra = Mode == 2 ? find_root_l2(a, p) : find_root(a, p);
+  uint32_t rb = Mode == 2 ? find_root_l2(b, p) : find_root(b, p);
   while (ra != rb) {
     const uint32_t hi = ra > rb ? ra : rb;
     const uint32_t lo = ra > rb ? rb : ra;
+    if (Mode != 0) {
+      const uint32_t cur = load_l2(&p[hi]);
+      if (cur != hi) {  // hooked meanwhile: cur < hi, so max(ra, rb) still drops
+        ra = find_root_l2(cur, p);
+        rb = find_root_l2(lo, p);
+        continue;
+      }
+    }
     const uint32_t old = atomicMin(&p[hi], lo);
     if (old == hi || old == lo) return;
-    ra = find_root(old, p);
-    rb = find_root(lo, p);
+    ra = Mode == 0 ? find_root(old, p) : find_root_l2(old, p);
+    rb = Mode == 0 ? find_root(lo, p) : find_root_l2(lo, p);
   }
 }
 
@@ -125,6 +156,7 @@ __global__ __launch_bounds__(kBlock) void k_init_min(const uint64_t *__restrict_
 }
 
 // Hook every vertex with its neighbour number r (if it has one).
+template <int Mode>
 __global__ __launch_bounds__(kBlock) void k_hook_neighbour(const uint64_t *__restrict__ row_ptr,
                                                            const uint32_t *__restrict__ col,
                                                            uint32_t n, uint32_t r,
@@ -134,7 +166,7 @@ __global__ __launch_bounds__(kBlock) void k_hook_neighbour(const uint64_t *__res
     if (b >= row_ptr[v + 1]) continue;
     const uint32_t w = col[b];
     // neighbour 0 below v was linked by k_init_min already
-    if (w < n && !(r == 0 && w < v)) hook(v, w, p);
+    if (w < n && !(r == 0 && w < v)) hook<Mode>(v, w, p);
   }
 }
 
@@ -212,6 +244,7 @@ __global__ __launch_bounds__(kBlock) void k_sample_giant(uint32_t n, const uint3
 // vertex first resolves its root (and writes it back to its own entry,
 // which does the pointer jumping the sampled rounds left); a walk that
 // passes through g marks the vertex as a giant member and skips it.
+template <int Mode>
 __global__ __launch_bounds__(kBlock) void k_hook_frontier(const uint64_t *__restrict__ row_ptr,
                                                           const uint32_t *__restrict__ col,
                                                           uint32_t n, uint32_t k_first,
@@ -264,7 +297,7 @@ __global__ __launch_bounds__(kBlock) void k_hook_frontier(const uint64_t *__rest
         if (ex[j + step] <= e) j += step;
       const uint64_t idx = s_begin[wid][j] + (e - ex[j]);
       const uint32_t w = col[idx];
-      if (w < n) hook((uint32_t)(base + j), w, p);
+      if (w < n) hook<Mode>((uint32_t)(base + j), w, p);
     }
     // the LDS rows are rewritten by the next chunk of this wave
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
@@ -281,12 +314,14 @@ inline unsigned blocks_for(uint64_t n, int cu_count) {
 
 }  // namespace
 
-extern "C" int cc_hip_components_tune(int sampled_neighbours, int blocks_per_cu) {
-  if (sampled_neighbours < 0 || sampled_neighbours > 4 || blocks_per_cu < 0 || blocks_per_cu > 64)
+extern "C" int cc_hip_components_tune(int sampled_neighbours, int blocks_per_cu, int hook_mode) {
+  if (sampled_neighbours < 0 || sampled_neighbours > 4 || blocks_per_cu < 0 || blocks_per_cu > 64 ||
+      hook_mode < -1 || hook_mode > 2)
     return ccx::set_error(4, "cc_hip_components_tune: out of range (%d, %d)", sampled_neighbours,
                           blocks_per_cu);
   if (sampled_neighbours > 0) g_sampled_neighbours = sampled_neighbours;
   if (blocks_per_cu > 0) g_blocks_per_cu = blocks_per_cu;
+  if (hook_mode >= 0) g_hook_mode = hook_mode;
   return 0;
 }
 
@@ -311,14 +346,23 @@ extern "C" int cc_hip_components_device(const uint64_t *d_row_ptr, const uint32_
     return 0;
   }
   CCX_LAUNCH("cc_init_min", s, k_init_min<<<nb, kBlock, 0, s>>>(d_row_ptr, d_col_idx, n, d_labels));
+  const int mode = g_hook_mode;
   for (uint32_t r = 0; r < rounds; ++r) {
     if (r > 0) CCX_LAUNCH("cc_compress", s, k_compress<<<nb, kBlock, 0, s>>>(n, d_labels));
-    CCX_LAUNCH("cc_hook_sample", s,
-               k_hook_neighbour<<<nb, kBlock, 0, s>>>(d_row_ptr, d_col_idx, n, r, d_labels));
+    if (mode == 0)
+      CCX_LAUNCH("cc_hook_sample", s, k_hook_neighbour<0><<<nb, kBlock, 0, s>>>(d_row_ptr, d_col_idx, n, r, d_labels));
+    else if (mode == 1)
+      CCX_LAUNCH("cc_hook_sample", s, k_hook_neighbour<1><<<nb, kBlock, 0, s>>>(d_row_ptr, d_col_idx, n, r, d_labels));
+    else
+      CCX_LAUNCH("cc_hook_sample", s, k_hook_neighbour<2><<<nb, kBlock, 0, s>>>(d_row_ptr, d_col_idx, n, r, d_labels));
   }
   CCX_LAUNCH("cc_sample_giant", s, k_sample_giant<<<1, kBlock, 0, s>>>(n, d_labels, giant));
-  CCX_LAUNCH("cc_hook_frontier", s,
-             k_hook_frontier<<<nb, kBlock, 0, s>>>(d_row_ptr, d_col_idx, n, rounds, d_labels, giant));
+  if (mode == 0)
+    CCX_LAUNCH("cc_hook_frontier", s, k_hook_frontier<0><<<nb, kBlock, 0, s>>>(d_row_ptr, d_col_idx, n, rounds, d_labels, giant));
+  else if (mode == 1)
+    CCX_LAUNCH("cc_hook_frontier", s, k_hook_frontier<1><<<nb, kBlock, 0, s>>>(d_row_ptr, d_col_idx, n, rounds, d_labels, giant));
+  else
+    CCX_LAUNCH("cc_hook_frontier", s, k_hook_frontier<2><<<nb, kBlock, 0, s>>>(d_row_ptr, d_col_idx, n, rounds, d_labels, giant));
   CCX_LAUNCH("cc_compress", s, k_compress<<<nb, kBlock, 0, s>>>(n, d_labels));
   return 0;
 }

@@ -78,7 +78,7 @@ int cc_hip_components_device(const uint64_t *d_row_ptr, const uint32_t *d_col_id
  * of sampled hooking rounds (1..4, default 2) and the grid cap of the
  * vertex-parallel kernels in workgroups per CU (1..64, default 8).  Results
  * never depend on either. */
-int cc_hip_components_tune(int sampled_neighbours, int blocks_per_cu);
+int cc_hip_components_tune(int sampled_neighbours, int blocks_per_cu, int hook_mode);
 
 /* Same result through host buffers (H2D, solve, D2H; blocking). */
 int cc_hip_components_host(const uint64_t *row_ptr, const uint32_t *col_idx,
