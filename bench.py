@@ -44,21 +44,23 @@ WORKLOADS = {
 }
 
 
-def make_graph(name: str):
-    import cc_mpi_amd as cm
+def make_device_graph(name: str):
+    """Build the workload graph directly in HBM (the device builder produces
+    the same CSR as the host generators, entry for entry)."""
+    from cc_mpi_amd import devgen
     w = WORKLOADS[name]
     if w["kind"] == "rmat":
-        return cm.rmat(w["scale"], w["ef"], seed=w["seed"]), (
+        return devgen.rmat(w["scale"], w["ef"], seed=w["seed"]), (
             f"R-MAT scale {w['scale']} ef {w['ef']} (0.57,0.19,0.19,0.05) sym+dedup, "
             f"permuted ids, seed {w['seed']}")
     if w["kind"] == "er":
-        return cm.erdos_renyi(w["n"], w["deg"], seed=w["seed"]), (
+        return devgen.erdos_renyi(w["n"], w["deg"], seed=w["seed"]), (
             f"Erdos-Renyi n={w['n']} avg degree {w['deg']:g}, seed {w['seed']}")
     if w["kind"] == "grid":
-        return cm.grid(w["rows"], w["cols"], w["keep"], seed=w["seed"]), (
+        return devgen.grid(w["rows"], w["cols"], w["keep"], seed=w["seed"]), (
             f"{w['rows']}x{w['cols']} grid, {1 - w['keep']:.0%} edges deleted, permuted, "
             f"seed {w['seed']}")
-    return cm.chung_lu(w["n"], w["pairs"], w["gamma"], seed=w["seed"]), (
+    return devgen.chung_lu(w["n"], w["pairs"], w["gamma"], seed=w["seed"]), (
         f"Chung-Lu n={w['n']} pairs={w['pairs']} gamma {w['gamma']}, permuted, seed {w['seed']}")
 
 
@@ -115,33 +117,39 @@ def kernel_bytes(name: str, n: int, st: dict, launches_per_solve: int) -> float 
     if name == "cc_compress":
         return 12.0 * n
     if name == "cc_hook_frontier":
-        return 4.0 * n + 16.0 * st["frontier_vertices"] + 8.0 * st["frontier_edges"]
+        # every vertex resolves its root (own entry + one gather, written back);
+        # frontier vertices read their row offsets, then one col entry and one
+        # parent gather per remaining neighbour
+        return 12.0 * n + 16.0 * st["frontier_vertices"] + 8.0 * st["frontier_edges"]
     return None
 
 
-def cpu_baseline(G, budget_s: float):
+def cpu_baseline(G, budget_s: float, runs: int = 3, impl: str = "reference"):
     """The reference's pthreads CC on this host (oracle/_ref), or the oracle
-    restatement if the reference build is absent.  Median of up to 3 timed
-    runs after one warm-up, bounded by budget_s."""
+    restatement if the reference build is absent.  Median of up to `runs`
+    timed runs after one warm-up, bounded by budget_s; runs=1 times a single
+    cold run (for graphs where one solve already fills the budget)."""
     import oracle
     aff = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else os.cpu_count()
     threads = max(1, min(aff or 1, int(os.environ.get("OMP_NUM_THREADS", "16") or 16), 16))
     chunk = 8192
-    ref = oracle.ref_lib()
+    ref = oracle.ref_lib() if impl == "reference" else None
     kind = "reference" if ref is not None else "port"
     fn = oracle.ref_cc_pthreads if ref is not None else oracle.cc_lp_pthreads
+    if ref is None:
+        oracle.lib().oracle_set_verbose(1)  # one progress line per round
     times = []
     t_start = time.perf_counter()
-    labels = fn(G.row_ptr, G.col_idx, threads, chunk)  # warm-up
-    warm = time.perf_counter() - t_start
-    for _ in range(3):
-        if time.perf_counter() - t_start + warm > budget_s and times:
+    warm = 0.0
+    if runs > 1:
+        labels = fn(G.row_ptr, G.col_idx, threads, chunk)  # warm-up
+        warm = time.perf_counter() - t_start
+    for _ in range(max(1, runs)):
+        if times and time.perf_counter() - t_start + max(warm, times[-1]) > budget_s:
             break
         t0 = time.perf_counter()
         labels = fn(G.row_ptr, G.col_idx, threads, chunk)
         times.append(time.perf_counter() - t0)
-    if not times:
-        times = [warm]
     t = float(np.median(times))
     cpu_model = ""
     try:
@@ -152,7 +160,8 @@ def cpu_baseline(G, budget_s: float):
     except OSError:
         pass
     return labels, dict(value=G.m / t / 1e9, unit="GTEPS", cores=threads, kind=kind,
-                        sample=(f"full workload graph, {len(times)} timed run(s) after 1 warm-up, "
+                        sample=(f"full workload graph, {len(times)} timed run(s)"
+                                f"{' after 1 warm-up' if runs > 1 else ' (cold)'}, "
                                 f"median {t:.3f} s, chunk {chunk}, {threads} threads on "
                                 f"{cpu_model or 'host CPU'}"),
                         seconds=t)
@@ -164,9 +173,17 @@ def main() -> int:
     ap.add_argument("--steps", type=int, default=50)
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--workload", default="rmat22", choices=sorted(WORKLOADS))
+    ap.add_argument("--cpu-impl", choices=["reference", "port"], default="reference",
+                    help="time the reference compiled from its sources, or the oracle's "
+                         "restatement (prints a line per round)")
+    ap.add_argument("--cpu-runs", type=int, default=3,
+                    help="timed CPU runs (after a warm-up when > 1)")
     ap.add_argument("--cpu-budget", type=float, default=30.0,
                     help="seconds of CPU work for the cpu_baseline leg")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--copy-probe-mib", type=int, default=0,
+                    help="run one device-to-device copy of this size before the solves "
+                         "(reference point for HBM traffic counters)")
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -180,13 +197,27 @@ def main() -> int:
     nat.check(lib.cc_hip_init(0), "cc_hip_init")
 
     t0 = time.perf_counter()
-    G, desc = make_graph(args.workload)
-    log(f"generated {desc}: n={G.n} m={G.m} in {time.perf_counter() - t0:.1f}s")
-
+    D, desc = make_device_graph(args.workload)
+    log(f"built on the device: {desc}: n={D.n} m={D.m} in {time.perf_counter() - t0:.1f}s")
     t0 = time.perf_counter()
-    D = cm.DeviceGraph(G)
+    G = D.to_host()
+    d2h_s = time.perf_counter() - t0
+    # the host-to-device upload a caller with host buffers would pay
+    t0 = time.perf_counter()
+    D.row_ptr.upload(G.row_ptr)
+    D.col_idx.upload(G.col_idx)
     h2d_s = time.perf_counter() - t0
-    log(f"uploaded CSR ({(G.row_ptr.nbytes + G.col_idx.nbytes) / 1e9:.2f} GB) in {h2d_s:.2f}s")
+    log(f"CSR {(G.row_ptr.nbytes + G.col_idx.nbytes) / 1e9:.2f} GB: H2D {h2d_s:.2f}s, "
+        f"D2H {d2h_s:.2f}s")
+
+    if args.copy_probe_mib > 0:
+        # one plain device copy of known size, for calibrating HBM counters
+        nb = args.copy_probe_mib << 20
+        a, b = cm.DeviceBuffer(nb), cm.DeviceBuffer(nb)
+        nat.check(lib.cc_hip_memcpy_d2d(b.ptr, a.ptr, nb, None), "copy probe")
+        nat.check(lib.cc_hip_device_sync(), "sync")
+        a.close()
+        b.close()
 
     for _ in range(args.warmup):
         D.solve_async()
@@ -245,7 +276,7 @@ def main() -> int:
     parity = None
     if not args.no_cpu_baseline:
         log("timing the CPU baseline ...")
-        ref_labels, cpu = cpu_baseline(G, args.cpu_budget)
+        ref_labels, cpu = cpu_baseline(G, args.cpu_budget, args.cpu_runs, args.cpu_impl)
         parity = bool(np.array_equal(ref_labels, labels))
         log(f"cpu baseline: {cpu['value']:.4f} GTEPS ({cpu['seconds']:.2f} s/solve, "
             f"{cpu['cores']} threads); GPU labels equal to the reference's: {parity}")
@@ -269,7 +300,8 @@ def main() -> int:
         "data": "synthetic",
         "config": {"workload": desc, "n": G.n, "m_directed": G.m,
                    "teps": "m/t, m = row_ptr[n] directed CSR entries, CSR resident in HBM",
-                   "h2d_seconds": round(h2d_s, 3)},
+                   "h2d_seconds": round(h2d_s, 3),
+                   "h2d_inclusive_gteps": round(G.m / (h2d_s + ms / 1e3) / 1e9, 4)},
         "roofline": roofline,
         "cpu_baseline": cpu,
         "kernels": {k: {"launches_per_solve": v["launches"] / args.steps,

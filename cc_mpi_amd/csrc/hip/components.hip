@@ -56,7 +56,7 @@ constexpr int kGiantSamples = 1024;
 // result: sampling only decides how much of the work the cheap
 // per-vertex rounds do before the frontier sweep.
 int g_sampled_neighbours = 2;  // sampled hooking rounds (1..4)
-int g_blocks_per_cu = 8;       // grid cap for the vertex-parallel kernels
+int g_blocks_per_cu = 4;       // grid cap for the vertex-parallel kernels (measured best)
 int g_hook_mode = 1;           // 0: plain loads, 1: L2-confirmed roots, 2: all loads from L2
 
 // Parent read that bypasses the CU's L1 (an agent-scope atomic load), so it
@@ -131,6 +131,20 @@ __device__ __forceinline__ void hook(uint32_t a, uint32_t b, uint32_t *__restric
   }
 }
 
+// Lossy union used only by the sampling rounds of mode 3: one
+// fire-and-forget atomicMin on the larger root, no confirmation and no
+// retry.  p[hi] only ever drops to an id connected to hi, so the forest
+// stays valid (every tree inside one component); a link it loses is
+// recovered by the frontier sweep, which in mode 3 hooks every neighbour of
+// every vertex outside the giant tree.
+__device__ __forceinline__ void hook_lossy(uint32_t a, uint32_t b, uint32_t *__restrict__ p) {
+  const uint32_t ra = find_root(a, p);
+  const uint32_t rb = find_root(b, p);
+  if (ra == rb) return;
+  (void)__hip_atomic_fetch_min(&p[ra > rb ? ra : rb], ra > rb ? rb : ra, __ATOMIC_RELAXED,
+                               __HIP_MEMORY_SCOPE_AGENT);
+}
+
 #define CCX_GRID_STRIDE(v, n)                                                       \
   for (uint32_t v = blockIdx.x * kBlock + threadIdx.x; v < (n);                     \
        v += gridDim.x * kBlock)
@@ -166,7 +180,10 @@ __global__ __launch_bounds__(kBlock) void k_hook_neighbour(const uint64_t *__res
     if (b >= row_ptr[v + 1]) continue;
     const uint32_t w = col[b];
     // neighbour 0 below v was linked by k_init_min already
-    if (w < n && !(r == 0 && w < v)) hook<Mode>(v, w, p);
+    if (w < n && !(r == 0 && w < v)) {
+      if (Mode == 3) hook_lossy(v, w, p);
+      else hook<Mode>(v, w, p);
+    }
   }
 }
 
@@ -297,7 +314,7 @@ __global__ __launch_bounds__(kBlock) void k_hook_frontier(const uint64_t *__rest
         if (ex[j + step] <= e) j += step;
       const uint64_t idx = s_begin[wid][j] + (e - ex[j]);
       const uint32_t w = col[idx];
-      if (w < n) hook<Mode>((uint32_t)(base + j), w, p);
+      if (w < n) hook<(Mode == 3 ? 1 : Mode)>((uint32_t)(base + j), w, p);
     }
     // the LDS rows are rewritten by the next chunk of this wave
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
@@ -316,7 +333,7 @@ inline unsigned blocks_for(uint64_t n, int cu_count) {
 
 extern "C" int cc_hip_components_tune(int sampled_neighbours, int blocks_per_cu, int hook_mode) {
   if (sampled_neighbours < 0 || sampled_neighbours > 4 || blocks_per_cu < 0 || blocks_per_cu > 64 ||
-      hook_mode < -1 || hook_mode > 2)
+      hook_mode < -1 || hook_mode > 3)
     return ccx::set_error(4, "cc_hip_components_tune: out of range (%d, %d)", sampled_neighbours,
                           blocks_per_cu);
   if (sampled_neighbours > 0) g_sampled_neighbours = sampled_neighbours;
@@ -353,16 +370,24 @@ extern "C" int cc_hip_components_device(const uint64_t *d_row_ptr, const uint32_
       CCX_LAUNCH("cc_hook_sample", s, k_hook_neighbour<0><<<nb, kBlock, 0, s>>>(d_row_ptr, d_col_idx, n, r, d_labels));
     else if (mode == 1)
       CCX_LAUNCH("cc_hook_sample", s, k_hook_neighbour<1><<<nb, kBlock, 0, s>>>(d_row_ptr, d_col_idx, n, r, d_labels));
-    else
+    else if (mode == 2)
       CCX_LAUNCH("cc_hook_sample", s, k_hook_neighbour<2><<<nb, kBlock, 0, s>>>(d_row_ptr, d_col_idx, n, r, d_labels));
+    else
+      CCX_LAUNCH("cc_hook_sample", s, k_hook_neighbour<3><<<nb, kBlock, 0, s>>>(d_row_ptr, d_col_idx, n, r, d_labels));
   }
   CCX_LAUNCH("cc_sample_giant", s, k_sample_giant<<<1, kBlock, 0, s>>>(n, d_labels, giant));
+  // Exact sampling (modes 0-2) links neighbours 0..rounds-1 of every vertex,
+  // so the sweep starts after them; lossy sampling (mode 3) links nothing
+  // for sure, so the sweep covers every neighbour.
+  const uint32_t first = (mode == 3) ? 0u : rounds;
   if (mode == 0)
-    CCX_LAUNCH("cc_hook_frontier", s, k_hook_frontier<0><<<nb, kBlock, 0, s>>>(d_row_ptr, d_col_idx, n, rounds, d_labels, giant));
+    CCX_LAUNCH("cc_hook_frontier", s, k_hook_frontier<0><<<nb, kBlock, 0, s>>>(d_row_ptr, d_col_idx, n, first, d_labels, giant));
   else if (mode == 1)
-    CCX_LAUNCH("cc_hook_frontier", s, k_hook_frontier<1><<<nb, kBlock, 0, s>>>(d_row_ptr, d_col_idx, n, rounds, d_labels, giant));
+    CCX_LAUNCH("cc_hook_frontier", s, k_hook_frontier<1><<<nb, kBlock, 0, s>>>(d_row_ptr, d_col_idx, n, first, d_labels, giant));
+  else if (mode == 2)
+    CCX_LAUNCH("cc_hook_frontier", s, k_hook_frontier<2><<<nb, kBlock, 0, s>>>(d_row_ptr, d_col_idx, n, first, d_labels, giant));
   else
-    CCX_LAUNCH("cc_hook_frontier", s, k_hook_frontier<2><<<nb, kBlock, 0, s>>>(d_row_ptr, d_col_idx, n, rounds, d_labels, giant));
+    CCX_LAUNCH("cc_hook_frontier", s, k_hook_frontier<3><<<nb, kBlock, 0, s>>>(d_row_ptr, d_col_idx, n, first, d_labels, giant));
   CCX_LAUNCH("cc_compress", s, k_compress<<<nb, kBlock, 0, s>>>(n, d_labels));
   return 0;
 }

@@ -74,10 +74,13 @@ const char *cc_hip_prof_names(void);
 int cc_hip_components_device(const uint64_t *d_row_ptr, const uint32_t *d_col_idx,
                              uint32_t n, uint64_t m, uint32_t *d_labels, void *stream);
 
-/* Process-wide tuning of the solve (0 keeps the current value): the number
- * of sampled hooking rounds (1..4, default 2) and the grid cap of the
- * vertex-parallel kernels in workgroups per CU (1..64, default 8).  Results
- * never depend on either. */
+/* Process-wide tuning of the solve: the number of sampled hooking rounds
+ * (1..4, default 2; 0 keeps the current value), the grid cap of the
+ * vertex-parallel kernels in workgroups per CU (1..64, default 4; 0 keeps),
+ * and the hooking mode (-1 keeps; 0 plain loads, 1 roots confirmed with an
+ * L2 read before each atomic (default), 2 all parent reads from L2, 3 lossy
+ * sampling with every neighbour swept outside the giant tree).  Results
+ * never depend on any of them. */
 int cc_hip_components_tune(int sampled_neighbours, int blocks_per_cu, int hook_mode);
 
 /* Same result through host buffers (H2D, solve, D2H; blocking). */

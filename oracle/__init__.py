@@ -44,6 +44,8 @@ def lib():
         L.oracle_cc_min_label.restype = None
         L.oracle_cc_lp_pthreads.argtypes = [c_u32, P_u64, P_u32, P_u32, c_int, c_int]
         L.oracle_cc_lp_pthreads.restype = c_int
+        L.oracle_set_verbose.argtypes = [c_int]
+        L.oracle_set_verbose.restype = None
         L.oracle_sort_u64.argtypes = [P_u64, c_u64]
         L.oracle_sort_u64.restype = c_int
         L.oracle_dedup_u64.argtypes = [P_u64, c_u64]

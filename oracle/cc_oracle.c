@@ -17,6 +17,7 @@
 #include <pthread.h>
 #include <stdatomic.h>
 #include <stdint.h>
+#include <stdio.h>
 #include <stdlib.h>
 #include <string.h>
 
@@ -84,7 +85,14 @@ typedef struct
   uint32_t chunk;              /* 0 = static blocks */
   uint32_t lo, hi;             /* static block */
   int tid;
+  int rounds;                  /* rounds completed (thread 0 only) */
 } LpWorker;
+
+static int g_lp_verbose = 0;
+
+/* Print one line to stderr per completed round of oracle_cc_lp_pthreads
+ * (progress for long runs). */
+void oracle_set_verbose(int on) { g_lp_verbose = on; }
 
 /* One vertex: pull-min over the row, lower own label, push the new label to
  * every neighbour (src/cc_pthreads.c:32-73).  Returns 1 if u improved. */
@@ -157,6 +165,9 @@ static void *lp_thread(void *arg)
     {
       int f = atomic_load_explicit(w->flag, memory_order_acquire);
       atomic_store_explicit(w->flag, f == 0 ? -1 : 0, memory_order_release);
+      if (g_lp_verbose)
+        fprintf(stderr, "[oracle lp] round %d done (%s)\n", ++w->rounds,
+                f == 0 ? "converged" : "changed");
     }
     pthread_barrier_wait(w->bar);
     if (atomic_load_explicit(w->flag, memory_order_acquire) == -1)
@@ -209,7 +220,7 @@ int oracle_cc_lp_pthreads(uint32_t n, const uint64_t *row_ptr, const uint32_t *c
     if (hi > n)
       hi = n;
     w[t] = (LpWorker){row_ptr, col_idx, lab, &flag, &cursor, &bar, n, chunk,
-                      (uint32_t)lo, (uint32_t)hi, t};
+                      (uint32_t)lo, (uint32_t)hi, t, 0};
   }
   for (int t = 1; t < num_threads; ++t)
     pthread_create(&th[t], NULL, lp_thread, &w[t]);
