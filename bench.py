@@ -167,6 +167,50 @@ def cpu_baseline(G, budget_s: float, runs: int = 3, impl: str = "reference"):
                         seconds=t)
 
 
+def serving_leg(args) -> dict:
+    """Batched serving vs one-graph-per-launch on the same Poisson stream:
+    requests name graphs from a pool (Erdos-Renyi d=8, R-MAT ef 16, 2-D grid
+    with 30% deletion; n log-uniform in [1K, 4M]) resident in HBM."""
+    from cc_mpi_amd import serving
+    t0 = time.perf_counter()
+    keys, loader = serving.make_pool(per_type=args.serving_pool, seed=2024)
+    store = serving.GraphStore(budget_bytes=64 << 30, loader=loader)
+    entries, nmax = {}, 0
+    for k in keys:  # admit every pool graph once (built in HBM)
+        g = store.acquire(k)
+        entries[k] = g.m
+        nmax = max(nmax, g.n)
+        store.release(k)
+    log(f"serving pool: {len(keys)} graphs, {sum(entries.values()) / 1e9:.2f}e9 entries, "
+        f"built in {time.perf_counter() - t0:.1f}s")
+    lanes, max_vertices = 2, max(1 << 24, nmax)
+    srv = serving.Server(store, lanes=lanes, max_vertices=max_vertices, max_items=1 << 16,
+                         max_graphs=2048)
+    tune_set = [store.graphs[k] for k in keys]
+    tune = serving.autotune_item_vertices(srv.engines[0], tune_set)
+    srv.item_vertices = tune["item_vertices"]
+    log(f"autotuned work-item size: {tune}")
+    reqs = serving.poisson_requests(args.serving_requests, args.serving_qps, keys, seed=7)
+    srv.run(reqs[: min(200, len(reqs))])  # warm-up on a prefix of the stream
+    reqs = serving.poisson_requests(args.serving_requests, args.serving_qps, keys, seed=7)
+    stats = srv.run(reqs)
+    batched = serving.latency_summary(reqs, entries)
+    batched["batches"] = stats["batches"]
+    log(f"batched: {batched}")
+    srv.close()
+    base = serving.poisson_requests(args.serving_requests, args.serving_qps, keys, seed=7)
+    serving.serve_one_by_one(store, base[: min(200, len(base))])
+    base = serving.poisson_requests(args.serving_requests, args.serving_qps, keys, seed=7)
+    serving.serve_one_by_one(store, base)
+    single = serving.latency_summary(base, entries)
+    log(f"one-graph-per-launch: {single}")
+    store.close()
+    return {"qps": args.serving_qps, "requests": args.serving_requests,
+            "pool": f"{len(keys)} graphs: ER d8 / R-MAT ef16 / grid 30% deleted, n log-uniform 1K-4M",
+            "lanes": lanes, "item_vertices": tune["item_vertices"], "autotune": tune,
+            "batched": batched, "one_graph_per_launch": single}
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -181,6 +225,11 @@ def main() -> int:
     ap.add_argument("--cpu-budget", type=float, default=30.0,
                     help="seconds of CPU work for the cpu_baseline leg")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--serving-requests", type=int, default=2000,
+                    help="requests in the batched-serving leg (0 = skip; 10000 = the full stream)")
+    ap.add_argument("--serving-qps", type=float, default=2000.0)
+    ap.add_argument("--serving-pool", type=int, default=16,
+                    help="graphs per type in the request pool")
     ap.add_argument("--copy-probe-mib", type=int, default=0,
                     help="run one device-to-device copy of this size before the solves "
                          "(reference point for HBM traffic counters)")
@@ -309,6 +358,8 @@ def main() -> int:
         "frontier": st,
         "parity_bit_exact": parity,
     }
+    if args.serving_requests > 0:
+        out["serving"] = serving_leg(args)
     print(json.dumps(out), flush=True)
     return 0 if parity else 1
 

@@ -1,0 +1,398 @@
+// Batched local CC: many graphs of uneven size solved by one replay of a
+// captured HIP graph.
+//
+// Each graph keeps its own CSR and label array in HBM (GraphDesc).  A
+// block-to-graph schedule (WorkItem) gives every workgroup one graph and a
+// vertex range of at most `item_vertices` vertices, so a batch of a 4M-vertex
+// graph and a thousand 1K-vertex graphs fills the chip with no per-graph
+// launches.  The stages are those of the single-graph solve
+// (components.hip): parent = min(v, first neighbour), exact sampled hooking
+// of neighbours 0 and 1 with L2-confirmed roots, pointer jumping, the most
+// frequent root of each graph (one workgroup per graph), the frontier sweep
+// of every vertex outside its graph's giant tree, pointer jumping.  Labels
+// are per graph: the minimum vertex id of the component inside that graph.
+//
+// The tables (header, graph descriptors, schedule) live in device memory
+// and are refreshed by a host-to-device copy node at the head of the
+// captured graph, from pinned host mirrors, so one instantiated graph
+// serves every batch up to its capacity.  The graph holds no memset nodes.
+
+#include <hip/hip_runtime.h>
+
+#include <cstring>
+#include <vector>
+
+#include "ccx_internal.h"
+#include "../../../include/cc_hip.h"
+
+namespace {
+
+constexpr int kBlock = 256;
+constexpr int kWave = 64;
+constexpr int kSamples = 1024;
+
+struct GraphDesc {
+  const uint64_t *row_ptr;
+  const uint32_t *col;
+  uint32_t *labels;
+  uint32_t n;
+  uint32_t giant;  // written on the device by kb_sample_giant
+};
+
+struct WorkItem {
+  uint32_t graph;
+  uint32_t v_begin;
+  uint32_t v_end;
+  uint32_t pad;
+};
+
+struct Header {
+  uint32_t n_items;
+  uint32_t n_graphs;
+  uint32_t pad[2];
+};
+
+__device__ __forceinline__ uint32_t load_l2(const uint32_t *p) {
+  return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+__device__ __forceinline__ uint32_t root_of(uint32_t x, const uint32_t *p) {
+  uint32_t px = p[x];
+  while (px != x) {
+    x = px;
+    px = p[x];
+  }
+  return x;
+}
+
+__device__ __forceinline__ uint32_t root_of_l2(uint32_t x, const uint32_t *p) {
+  uint32_t px = load_l2(&p[x]);
+  while (px != x) {
+    x = px;
+    px = load_l2(&p[x]);
+  }
+  return x;
+}
+
+__device__ __forceinline__ uint32_t root_via(uint32_t x, const uint32_t *p, uint32_t g, bool &hit) {
+  bool h = (x == g);
+  uint32_t px = p[x];
+  while (px != x) {
+    x = px;
+    h |= (x == g);
+    px = p[x];
+  }
+  hit = h;
+  return x;
+}
+
+__device__ __forceinline__ void unite(uint32_t a, uint32_t b, uint32_t *p) {
+  uint32_t ra = root_of(a, p), rb = root_of(b, p);
+  while (ra != rb) {
+    const uint32_t hi = ra > rb ? ra : rb, lo = ra > rb ? rb : ra;
+    const uint32_t cur = load_l2(&p[hi]);
+    if (cur != hi) {
+      ra = root_of_l2(cur, p);
+      rb = root_of_l2(lo, p);
+      continue;
+    }
+    const uint32_t old = atomicMin(&p[hi], lo);
+    if (old == hi || old == lo) return;
+    ra = root_of_l2(old, p);
+    rb = root_of_l2(lo, p);
+  }
+}
+
+#define ITEM_OR_EXIT()                                   \
+  if (blockIdx.x >= hdr->n_items) return;                \
+  const WorkItem it = items[blockIdx.x];                 \
+  const GraphDesc gd = graphs[it.graph];
+
+__global__ __launch_bounds__(kBlock) void kb_init_min(const Header *__restrict__ hdr,
+                                                      const GraphDesc *__restrict__ graphs,
+                                                      const WorkItem *__restrict__ items) {
+  ITEM_OR_EXIT();
+  for (uint32_t v = it.v_begin + threadIdx.x; v < it.v_end; v += kBlock) {
+    uint32_t x = v;
+    const uint64_t b = gd.row_ptr[v];
+    if (b < gd.row_ptr[v + 1]) {
+      const uint32_t w = gd.col[b];
+      if (w < x) x = w;
+    }
+    gd.labels[v] = x;
+  }
+}
+
+__global__ __launch_bounds__(kBlock) void kb_hook_neighbour(const Header *__restrict__ hdr,
+                                                            const GraphDesc *__restrict__ graphs,
+                                                            const WorkItem *__restrict__ items,
+                                                            uint32_t r) {
+  ITEM_OR_EXIT();
+  for (uint32_t v = it.v_begin + threadIdx.x; v < it.v_end; v += kBlock) {
+    const uint64_t b = gd.row_ptr[v] + r;
+    if (b >= gd.row_ptr[v + 1]) continue;
+    const uint32_t w = gd.col[b];
+    if (w < gd.n && !(r == 0 && w < v)) unite(v, w, gd.labels);
+  }
+}
+
+__global__ __launch_bounds__(kBlock) void kb_compress(const Header *__restrict__ hdr,
+                                                      const GraphDesc *__restrict__ graphs,
+                                                      const WorkItem *__restrict__ items) {
+  ITEM_OR_EXIT();
+  uint32_t *p = gd.labels;
+  for (uint32_t v = it.v_begin + threadIdx.x; v < it.v_end; v += kBlock) {
+    const uint32_t pv = p[v];
+    if (pv == v) continue;
+    const uint32_t r = root_of(pv, p);
+    if (r != pv) p[v] = r;
+  }
+}
+
+__device__ __forceinline__ uint32_t sample_index(uint32_t i, uint32_t n) {
+  uint64_t z = (uint64_t)(i + 1) * 0x9E3779B97F4A7C15ull;
+  z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+  z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+  z ^= z >> 31;
+  return (uint32_t)(((z >> 32) * (uint64_t)n) >> 32);
+}
+
+// One workgroup per graph: its most frequent root (ties: smaller root).
+__global__ __launch_bounds__(kBlock) void kb_sample_giant(const Header *__restrict__ hdr,
+                                                          GraphDesc *__restrict__ graphs) {
+  constexpr uint32_t kSlots = 2 * kSamples;
+  constexpr uint32_t kEmpty = 0xFFFFFFFFu;
+  __shared__ uint32_t keys[kSlots];
+  __shared__ uint32_t cnts[kSlots];
+  __shared__ uint64_t best[kBlock / kWave];
+  if (blockIdx.x >= hdr->n_graphs) return;
+  GraphDesc &gd = graphs[blockIdx.x];
+  const uint32_t n = gd.n;
+  if (n == 0) return;
+  const int t = threadIdx.x;
+  for (uint32_t i = t; i < kSlots; i += kBlock) {
+    keys[i] = kEmpty;
+    cnts[i] = 0;
+  }
+  __syncthreads();
+  for (int i = t; i < kSamples; i += kBlock) {
+    const uint32_t x = root_of(sample_index(i, n), gd.labels);
+    uint32_t h = (x * 0x9E3779B1u) >> 21;
+    for (;;) {
+      const uint32_t prev = atomicCAS(&keys[h], kEmpty, x);
+      if (prev == kEmpty || prev == x) {
+        atomicAdd(&cnts[h], 1u);
+        break;
+      }
+      h = (h + 1) & (kSlots - 1);
+    }
+  }
+  __syncthreads();
+  uint64_t mine = 0;
+  for (uint32_t i = t; i < kSlots; i += kBlock)
+    if (cnts[i]) {
+      const uint64_t cand = ((uint64_t)cnts[i] << 32) | (uint32_t)~keys[i];
+      if (cand > mine) mine = cand;
+    }
+  for (int d = kWave / 2; d > 0; d >>= 1) {
+    const uint64_t o = __shfl_xor(mine, d, kWave);
+    if (o > mine) mine = o;
+  }
+  if ((t & (kWave - 1)) == 0) best[t / kWave] = mine;
+  __syncthreads();
+  if (t == 0) {
+    uint64_t b = best[0];
+    for (int w = 1; w < kBlock / kWave; ++w)
+      if (best[w] > b) b = best[w];
+    gd.giant = ~(uint32_t)b;
+  }
+}
+
+__global__ __launch_bounds__(kBlock) void kb_hook_frontier(const Header *__restrict__ hdr,
+                                                           const GraphDesc *__restrict__ graphs,
+                                                           const WorkItem *__restrict__ items,
+                                                           uint32_t k_first) {
+  __shared__ uint64_t s_excl[kBlock / kWave][kWave];
+  __shared__ uint64_t s_begin[kBlock / kWave][kWave];
+  ITEM_OR_EXIT();
+  uint32_t *p = gd.labels;
+  const int lane = threadIdx.x & (kWave - 1);
+  const int wid = threadIdx.x / kWave;
+  for (uint32_t base = it.v_begin + wid * kWave; base < it.v_end; base += kBlock) {
+    const uint32_t v = base + lane;
+    uint64_t work = 0, begin = 0;
+    if (v < it.v_end) {
+      const uint32_t pv = p[v];
+      bool in_giant = false;
+      const uint32_t r = root_via(v, p, gd.giant, in_giant);
+      if (r != pv) p[v] = r;
+      if (!in_giant) {
+        const uint64_t b = gd.row_ptr[v] + k_first, e = gd.row_ptr[v + 1];
+        if (e > b) {
+          work = e - b;
+          begin = b;
+        }
+      }
+    }
+    uint64_t incl = work;
+#pragma unroll
+    for (int d = 1; d < kWave; d <<= 1) {
+      const uint64_t up = __shfl_up(incl, d, kWave);
+      if (lane >= d) incl += up;
+    }
+    const uint64_t total = __shfl(incl, kWave - 1, kWave);
+    if (total == 0) continue;
+    s_excl[wid][lane] = incl - work;
+    s_begin[wid][lane] = begin;
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    const uint64_t *ex = s_excl[wid];
+    for (uint64_t e = lane; e < total; e += kWave) {
+      int j = 0;
+#pragma unroll
+      for (int step = kWave / 2; step > 0; step >>= 1)
+        if (ex[j + step] <= e) j += step;
+      const uint32_t w = gd.col[s_begin[wid][j] + (e - ex[j])];
+      if (w < gd.n) unite(base + j, w, p);
+    }
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+  }
+}
+
+}  // namespace
+
+struct cc_hip_batch_s {
+  uint32_t max_items = 0, max_graphs = 0;
+  hipStream_t stream = nullptr;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+  // device tables: [Header][GraphDesc x max_graphs][WorkItem x max_items]
+  char *d_tables = nullptr;
+  char *h_tables = nullptr;  // pinned mirror
+  size_t table_bytes = 0;
+  size_t off_graphs = 0, off_items = 0;
+  bool pending = false;
+};
+
+static size_t align_up(size_t x, size_t a) { return (x + a - 1) & ~(a - 1); }
+
+extern "C" int cc_hip_batch_create(uint32_t max_items, uint32_t max_graphs, cc_hip_batch_t *out) {
+  if (!ccx::ctx()) return 2;
+  if (!out || max_items == 0 || max_graphs == 0)
+    return ccx::set_error(4, "cc_hip_batch_create: bad arguments");
+  *out = nullptr;
+  cc_hip_batch_s *b = new cc_hip_batch_s();
+  b->max_items = max_items;
+  b->max_graphs = max_graphs;
+  b->off_graphs = align_up(sizeof(Header), 64);
+  b->off_items = align_up(b->off_graphs + (size_t)max_graphs * sizeof(GraphDesc), 64);
+  b->table_bytes = b->off_items + (size_t)max_items * sizeof(WorkItem);
+  CCX_TRY(hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking));
+  CCX_TRY(hipMalloc(&b->d_tables, b->table_bytes));
+  CCX_TRY(hipHostMalloc(&b->h_tables, b->table_bytes, hipHostMallocDefault));
+  memset(b->h_tables, 0, b->table_bytes);
+  CCX_TRY(hipMemcpy(b->d_tables, b->h_tables, b->table_bytes, hipMemcpyHostToDevice));
+
+  const Header *hdr = (const Header *)b->d_tables;
+  GraphDesc *graphs = (GraphDesc *)(b->d_tables + b->off_graphs);
+  const WorkItem *items = (const WorkItem *)(b->d_tables + b->off_items);
+  hipStream_t s = b->stream;
+  CCX_TRY(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+  hipError_t e = hipMemcpyAsync(b->d_tables, b->h_tables, b->table_bytes, hipMemcpyHostToDevice, s);
+  if (e == hipSuccess) {
+    kb_init_min<<<max_items, kBlock, 0, s>>>(hdr, graphs, items);
+    kb_hook_neighbour<<<max_items, kBlock, 0, s>>>(hdr, graphs, items, 0u);
+    kb_compress<<<max_items, kBlock, 0, s>>>(hdr, graphs, items);
+    kb_hook_neighbour<<<max_items, kBlock, 0, s>>>(hdr, graphs, items, 1u);
+    kb_sample_giant<<<max_graphs, kBlock, 0, s>>>(hdr, graphs);
+    kb_hook_frontier<<<max_items, kBlock, 0, s>>>(hdr, graphs, items, 2u);
+    kb_compress<<<max_items, kBlock, 0, s>>>(hdr, graphs, items);
+  }
+  hipError_t e2 = hipStreamEndCapture(s, &b->graph);
+  if (e != hipSuccess || e2 != hipSuccess) {
+    delete b;
+    return ccx::set_error(1, "cc_hip_batch_create: capture failed (%s)",
+                          hipGetErrorString(e != hipSuccess ? e : e2));
+  }
+  CCX_TRY(hipGraphInstantiate(&b->exec, b->graph, nullptr, nullptr, 0));
+  *out = b;
+  return 0;
+}
+
+extern "C" int cc_hip_batch_destroy(cc_hip_batch_t b) {
+  if (!b) return 0;
+  if (!ccx::ctx()) return 2;
+  (void)hipStreamSynchronize(b->stream);
+  if (b->exec) (void)hipGraphExecDestroy(b->exec);
+  if (b->graph) (void)hipGraphDestroy(b->graph);
+  (void)hipFree(b->d_tables);
+  (void)hipHostFree(b->h_tables);
+  (void)hipStreamDestroy(b->stream);
+  delete b;
+  return 0;
+}
+
+extern "C" int cc_hip_batch_launch(cc_hip_batch_t b, uint32_t n_graphs,
+                                   const uint64_t *const *d_row_ptr,
+                                   const uint32_t *const *d_col_idx, const uint32_t *n,
+                                   uint32_t *const *d_labels, uint32_t item_vertices,
+                                   uint32_t *items_used) {
+  if (!ccx::ctx()) return 2;
+  if (!b) return ccx::set_error(4, "cc_hip_batch_launch: null batch");
+  if (n_graphs > b->max_graphs)
+    return ccx::set_error(4, "cc_hip_batch_launch: %u graphs > capacity %u", n_graphs, b->max_graphs);
+  if (item_vertices == 0 || item_vertices % kBlock)
+    return ccx::set_error(4, "cc_hip_batch_launch: item_vertices must be a positive multiple of %d",
+                          kBlock);
+  uint64_t need = 0;
+  for (uint32_t g = 0; g < n_graphs; ++g) need += (n[g] + (uint64_t)item_vertices - 1) / item_vertices;
+  if (need > b->max_items)
+    return ccx::set_error(4, "cc_hip_batch_launch: %llu work items > capacity %u",
+                          (unsigned long long)need, b->max_items);
+  // the pinned mirror is read by the previous replay's copy node
+  if (b->pending) CCX_TRY(hipStreamSynchronize(b->stream));
+  Header *hdr = (Header *)b->h_tables;
+  GraphDesc *gds = (GraphDesc *)(b->h_tables + b->off_graphs);
+  WorkItem *its = (WorkItem *)(b->h_tables + b->off_items);
+  uint32_t k = 0;
+  for (uint32_t g = 0; g < n_graphs; ++g) {
+    gds[g] = GraphDesc{d_row_ptr[g], d_col_idx[g], d_labels[g], n[g], 0u};
+    for (uint64_t v = 0; v < n[g]; v += item_vertices) {
+      const uint64_t e = v + item_vertices < n[g] ? v + item_vertices : n[g];
+      its[k++] = WorkItem{g, (uint32_t)v, (uint32_t)e, 0u};
+    }
+  }
+  hdr->n_items = k;
+  hdr->n_graphs = n_graphs;
+  if (items_used) *items_used = k;
+  ccx::prof_begin("cc_batch_graph", b->stream);
+  hipError_t e = hipGraphLaunch(b->exec, b->stream);
+  ccx::prof_end(b->stream);
+  if (e != hipSuccess) return ccx::set_error(1, "hipGraphLaunch: %s", hipGetErrorString(e));
+  b->pending = true;
+  return 0;
+}
+
+extern "C" int cc_hip_batch_query(cc_hip_batch_t b) {
+  if (!b) return 1;
+  hipError_t e = hipStreamQuery(b->stream);
+  if (e == hipSuccess) {
+    b->pending = false;
+    return 1;
+  }
+  if (e == hipErrorNotReady) return 0;
+  return -ccx::set_error(1, "cc_hip_batch_query: %s", hipGetErrorString(e));
+}
+
+extern "C" int cc_hip_batch_sync(cc_hip_batch_t b) {
+  if (!b) return 0;
+  if (!ccx::ctx()) return 2;
+  CCX_TRY(hipStreamSynchronize(b->stream));
+  b->pending = false;
+  return 0;
+}
+
+extern "C" void *cc_hip_batch_stream(cc_hip_batch_t b) { return b ? (void *)b->stream : nullptr; }

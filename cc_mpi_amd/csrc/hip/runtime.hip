@@ -307,6 +307,13 @@ int cc_hip_memcpy_d2h(void *dst, const void *d_src, size_t bytes, void *stream) 
   return 0;
 }
 
+int cc_hip_memcpy_d2h_async(void *h_dst, const void *d_src, size_t bytes, void *stream) {
+  if (!ctx()) return 2;
+  if (bytes == 0) return 0;
+  CCX_TRY(hipMemcpyAsync(h_dst, d_src, bytes, hipMemcpyDeviceToHost, pick_stream(stream)));
+  return 0;
+}
+
 int cc_hip_memcpy_d2d(void *d_dst, const void *d_src, size_t bytes, void *stream) {
   if (!ctx()) return 2;
   if (bytes == 0) return 0;

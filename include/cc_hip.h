@@ -42,6 +42,9 @@ int cc_hip_host_free(void *h_ptr);
 int cc_hip_memcpy_h2d(void *d_dst, const void *src, size_t bytes, void *stream);
 int cc_hip_memcpy_d2h(void *dst, const void *d_src, size_t bytes, void *stream);
 int cc_hip_memcpy_d2d(void *d_dst, const void *d_src, size_t bytes, void *stream);
+/* Device-to-host copy enqueued on `stream` without waiting (h_dst should be
+ * pinned for the copy to be asynchronous). */
+int cc_hip_memcpy_d2h_async(void *h_dst, const void *d_src, size_t bytes, void *stream);
 int cc_hip_stream_create(void **stream);
 int cc_hip_stream_destroy(void *stream);
 int cc_hip_stream_sync(void *stream);
@@ -198,6 +201,32 @@ int cc_hip_merge_step_device(const uint64_t *d_records, uint64_t count,
 int cc_hip_rank_labels_device(const uint32_t *d_comp_label, const uint32_t *d_comp_of,
                               uint32_t n_local, uint32_t v_start, uint32_t *d_labels,
                               uint64_t *roots, void *stream);
+
+/* ---- batched solves (many graphs, one captured HIP graph replay) -------- */
+
+typedef struct cc_hip_batch_s *cc_hip_batch_t;
+
+/* A batch engine with room for max_items work items (workgroups) and
+ * max_graphs graphs per launch: its own stream, device tables, pinned
+ * mirrors and an instantiated HIP graph of the whole solve. */
+int cc_hip_batch_create(uint32_t max_items, uint32_t max_graphs, cc_hip_batch_t *out);
+int cc_hip_batch_destroy(cc_hip_batch_t b);
+/*
+ * Solve n_graphs resident graphs (symmetric CSRs d_row_ptr[g], d_col_idx[g]
+ * of n[g] vertices) into d_labels[g] (per-graph minimum ids) with one graph
+ * replay; each workgroup gets at most item_vertices (multiple of 256)
+ * vertices of one graph.  Asynchronous on the batch's stream; waits for the
+ * previous launch of the same batch first.  *items_used = work items.
+ */
+int cc_hip_batch_launch(cc_hip_batch_t b, uint32_t n_graphs, const uint64_t *const *d_row_ptr,
+                        const uint32_t *const *d_col_idx, const uint32_t *n,
+                        uint32_t *const *d_labels, uint32_t item_vertices,
+                        uint32_t *items_used);
+/* 1 when the last launch has finished, 0 while it runs, < 0 on error. */
+int cc_hip_batch_query(cc_hip_batch_t b);
+int cc_hip_batch_sync(cc_hip_batch_t b);
+/* The batch's stream (hipStream_t), for copies ordered after its solves. */
+void *cc_hip_batch_stream(cc_hip_batch_t b);
 
 /* ---- device-side construction of the seeded inputs -------------------- */
 

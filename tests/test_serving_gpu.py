@@ -1,0 +1,95 @@
+"""Batched solves and the serving loop: every request's labels equal the
+oracle's for its graph, whether solved alone, in a batch with others, or
+through the continuous-batching server (exact: integer labels)."""
+import numpy as np
+import pytest
+
+import cc_mpi_amd as cm
+from cc_mpi_amd import devgen, serving
+import oracle
+
+pytestmark = pytest.mark.gpu
+
+
+def mixed_graphs():
+    return [devgen.rmat(14, 16, seed=1), devgen.erdos_renyi(5000, 1.5, seed=2),
+            devgen.grid(90, 70, 0.6, seed=3), devgen.erdos_renyi(1, 0.0, seed=4),
+            devgen.rmat(10, 4, seed=5), devgen.grid(300, 300, 0.7, seed=6),
+            devgen.erdos_renyi(70000, 8.0, seed=7)]
+
+
+def oracle_of(g):
+    h = g.to_host()
+    return oracle.cc_min_label(h.row_ptr, h.col_idx)
+
+
+@pytest.mark.parametrize("item_vertices", [256, 1024, 65536])
+def test_batch_equals_alone_equals_oracle(gpu, item_vertices):
+    gs = mixed_graphs()
+    want = [oracle_of(g) for g in gs]
+    alone = [g.solve() for g in gs]
+    eng = serving.BatchEngine(max_vertices=1 << 20, max_items=4096, max_graphs=64)
+    # several replays of the captured graph with different contents, with
+    # eager work in between
+    for order in ([0, 1, 2, 3, 4, 5, 6], [6, 5, 4], [2, 2, 0, 3], [1]):
+        sel = [gs[i] for i in order]
+        offs = eng.launch(sel, item_vertices)
+        eng.sync()
+        for i, (off, n) in zip(order, offs):
+            got = eng.out.array[off:off + n]
+            assert np.array_equal(got, want[i]), f"graph {i} in batch {order}"
+            assert np.array_equal(got, alone[i])
+        gs[0].solve()  # eager single-graph work between replays
+    eng.close()
+
+
+def test_batch_capacity_errors(gpu):
+    eng = serving.BatchEngine(max_vertices=1 << 16, max_items=8, max_graphs=2)
+    g = devgen.erdos_renyi(5000, 2.0, seed=1)
+    with pytest.raises(RuntimeError, match="work items"):
+        eng.launch([g], 256)  # 20 items > 8
+    with pytest.raises(RuntimeError, match="graphs"):
+        eng.launch([g, g, g], 4096)
+    eng.close()
+
+
+def test_server_requests_match_oracle(gpu):
+    keys, loader = serving.make_pool(per_type=3, seed=11, n_min=1024, n_max=200000)
+    want = {}
+    store = serving.GraphStore(budget_bytes=1 << 32, loader=loader)
+    for k in keys:
+        g = store.acquire(k)
+        want[k] = oracle_of(g)
+        store.release(k)
+    reqs = serving.poisson_requests(300, qps=3000.0, keys=keys, seed=5)
+    srv = serving.Server(store, lanes=2, max_vertices=1 << 22, max_items=1 << 13,
+                         max_graphs=256, item_vertices=1024, keep_labels=True)
+    stats = srv.run(reqs)
+    srv.close()
+    assert stats["batches"] >= 1
+    for r in reqs:
+        assert np.isfinite(r.done) and r.done >= r.arrival
+        assert np.array_equal(r.labels, want[r.key]), f"request {r.rid} ({r.key})"
+    # the one-graph-per-launch baseline returns the same labels
+    base = [serving.Request(r.rid, r.arrival, r.key) for r in reqs[:60]]
+    serving.serve_one_by_one(store, base, keep_labels=True)
+    for r in base:
+        assert np.array_equal(r.labels, want[r.key])
+    store.close()
+
+
+def test_store_evicts_under_budget(gpu):
+    keys, loader = serving.make_pool(per_type=2, seed=3, n_min=50000, n_max=60000)
+    g0 = loader(keys[0])
+    one = serving.footprint_bytes(g0.n, g0.m)
+    g0.close()
+    store = serving.GraphStore(budget_bytes=int(one * 2.5), loader=loader)
+    a = store.acquire(keys[0])
+    b = store.acquire(keys[1])
+    assert a is not None and b is not None
+    assert store.acquire(keys[2]) is None or store.used <= store.budget
+    store.release(keys[0])
+    store.release(keys[1])
+    c = store.acquire(keys[3])
+    assert c is not None and store.used <= store.budget
+    store.close()
