@@ -184,7 +184,7 @@ def serving_leg(args) -> dict:
     log(f"serving pool: {len(keys)} graphs, {sum(entries.values()) / 1e9:.2f}e9 entries, "
         f"built in {time.perf_counter() - t0:.1f}s")
     lanes, max_vertices = 2, max(1 << 24, nmax)
-    srv = serving.Server(store, lanes=lanes, max_vertices=max_vertices, max_items=1 << 16,
+    srv = serving.Server(store, lanes=lanes, max_vertices=max_vertices, max_items=20000,
                          max_graphs=2048)
     tune_set = [store.graphs[k] for k in keys]
     tune = serving.autotune_item_vertices(srv.engines[0], tune_set)

@@ -272,6 +272,8 @@ class Server:
                     offs = eng.launch(graphs, self.item_vertices)
                     busy[li] = list(zip(chosen, graphs, offs))
                     batches += 1
+            if queue and not busy:
+                raise MemoryError(f"request {queue[0].rid} does not fit the graph store budget")
             if not busy and not queue and pending:
                 wait = pending[0].arrival - (time.perf_counter() - t0)
                 if wait > 0:

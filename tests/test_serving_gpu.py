@@ -87,9 +87,15 @@ def test_store_evicts_under_budget(gpu):
     a = store.acquire(keys[0])
     b = store.acquire(keys[1])
     assert a is not None and b is not None
-    assert store.acquire(keys[2]) is None or store.used <= store.budget
+    third = store.acquire(keys[2])
+    assert store.used <= store.budget
+    if third is not None:
+        store.release(keys[2])
     store.release(keys[0])
     store.release(keys[1])
+    # with nothing pinned, any single graph that fits the budget is admitted
+    # (older ones are evicted)
     c = store.acquire(keys[3])
     assert c is not None and store.used <= store.budget
+    store.release(keys[3])
     store.close()
