@@ -78,24 +78,17 @@ def test_server_requests_match_oracle(gpu):
     store.close()
 
 
-def test_store_evicts_under_budget(gpu):
+def test_store_with_device_graphs(gpu):
     keys, loader = serving.make_pool(per_type=2, seed=3, n_min=50000, n_max=60000)
-    g0 = loader(keys[0])
-    one = serving.footprint_bytes(g0.n, g0.m)
-    g0.close()
-    store = serving.GraphStore(budget_bytes=int(one * 2.5), loader=loader)
-    a = store.acquire(keys[0])
-    b = store.acquire(keys[1])
-    assert a is not None and b is not None
-    third = store.acquire(keys[2])
-    assert store.used <= store.budget
-    if third is not None:
-        store.release(keys[2])
-    store.release(keys[0])
-    store.release(keys[1])
-    # with nothing pinned, any single graph that fits the budget is admitted
-    # (older ones are evicted)
-    c = store.acquire(keys[3])
-    assert c is not None and store.used <= store.budget
-    store.release(keys[3])
+    sizes = {}
+    for k in keys:
+        g = loader(k)
+        sizes[k] = serving.footprint_bytes(g.n, g.m)
+        g.close()
+    store = serving.GraphStore(budget_bytes=2 * max(sizes.values()), loader=loader)
+    for k in keys:  # each admitted in turn, evicting older ones as needed
+        g = store.acquire(k)
+        assert g is not None and store.used <= store.budget
+        assert np.array_equal(g.solve(), oracle_of(g))
+        store.release(k)
     store.close()
