@@ -193,6 +193,12 @@ def autotune_item_vertices(engine: BatchEngine, graphs: list[DeviceGraph],
                            candidates=(1024, 4096, 16384, 65536), reps: int = 5) -> dict:
     """Time a representative batch under each work-item size; returns the
     timings and the fastest size."""
+    fit, verts = [], 0
+    for g in graphs:  # the representative batch: as much of the set as one lane holds
+        if verts + g.n <= engine.max_vertices and len(fit) < engine.max_graphs:
+            fit.append(g)
+            verts += g.n
+    graphs = fit
     timings = {}
     for iv in candidates:
         if engine.items_for(graphs, iv) > engine.max_items:
