@@ -211,6 +211,78 @@ def serving_leg(args) -> dict:
             "batched": batched, "one_graph_per_launch": single}
 
 
+def rankshare_leg(args) -> dict:
+    """One rank's share of the 4-rank vertex-block partition of the
+    65.6M-vertex Chung-Lu graph, at real shape: split, local CC, send lists,
+    boundary sort/dedup, ghost extraction, then the merge loop with injected
+    ghost deltas for exchange intervals 1, 2 and 4 (delta pack every
+    exchange; the interval-1 packs are checked against the oracle)."""
+    import ctypes
+    import oracle
+    from cc_mpi_amd import _native as nat
+    from cc_mpi_amd import devgen
+    from cc_mpi_amd.rankshare import RankShare
+    lib = nat.gpu_lib()
+    t0 = time.perf_counter()
+    full = devgen.chung_lu(65601024, 1800000000, 2.1, seed=1003)
+    full.solve_async()
+    nat.check(lib.cc_hip_device_sync(), "sync")
+    log(f"rankshare: full graph n={full.n} m={full.m} built+solved in "
+        f"{time.perf_counter() - t0:.1f}s")
+    lib.cc_hip_prof_reset()
+    lib.cc_hip_prof_enable(1)
+    rs = RankShare(full, P=4, rank=0)
+    rs.preprocess()
+    log(f"rankshare: preprocess {rs.times} {rs.stats}")
+    checked = [0]
+
+    def check(stf, seg, cl, co, vs, ve, before, pairs, counts, displs):
+        wp, wc, wd = oracle.delta_pack(stf, seg, cl, co, vs, ve, before.copy())
+        if not (np.array_equal(pairs, wp) and np.array_equal(counts, wc)
+                and np.array_equal(displs, wd)):
+            raise AssertionError("delta pack differs from the oracle")
+        checked[0] += 1
+
+    runs = {}
+    truth = full.labels
+    want = np.empty(rs.nl, dtype=np.uint32)
+    truth.download(want)  # rank 0's block is vertices [0, n_local)
+    for interval in (1, 2, 4):
+        t0 = time.perf_counter()
+        got = rs.merge(truth, exchange_interval=interval, seed=11,
+                       check_pack=check if interval == 1 else None)
+        ok = bool(np.array_equal(got, want))
+        runs[str(interval)] = {"seconds": round(time.perf_counter() - t0, 3), "labels_exact": ok,
+                               **{k: rs.stats[k] for k in ("merge_iterations", "exchanges",
+                                                           "packs", "pairs_emitted")}}
+        log(f"rankshare interval {interval}: {runs[str(interval)]}")
+        if not ok:
+            raise AssertionError(f"rank-share labels differ (interval {interval})")
+    lib.cc_hip_prof_enable(0)
+    kern = {}
+    for nm in [x for x in lib.cc_hip_prof_names().decode().split(",") if x]:
+        c, t = ctypes.c_uint64(), ctypes.c_double()
+        lib.cc_hip_prof_query(nm.encode(), ctypes.byref(c), ctypes.byref(t))
+        kern[nm] = {"launches": int(c.value), "avg_us": round(1e3 * t.value / max(1, c.value), 2)}
+    lib.cc_hip_prof_reset()
+    out = {"graph": "Chung-Lu n=65601024 pairs=1.8e9 gamma 2.1 seed 1003, rank 0 of 4 "
+                    "(vertex blocks)",
+           "stats": rs.stats, "preprocess_seconds": {k: round(v, 4) for k, v in rs.times.items()
+                                                      if k not in ("merge_steps", "delta_packs")},
+           "intervals": runs, "packs_checked_against_oracle": checked[0], "kernels": kern}
+    if "delta_pack" in kern:
+        b = 16.0 * rs.total_send  # + 12 per emitted pair, averaged below
+        emitted = runs["1"]["pairs_emitted"] / max(1, runs["1"]["packs"])
+        out["delta_pack_algorithmic_gbs"] = round((b + 12.0 * emitted) /
+                                                  (kern["delta_pack"]["avg_us"] * 1e3), 1)
+    if "sort_onesweep" in kern:
+        out["boundary_sort_onesweep_gbs"] = round(16.0 * rs.nb / (kern["sort_onesweep"]["avg_us"]
+                                                                  * 1e3), 1)
+    rs.close()
+    full.close()
+    return out
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -225,9 +297,11 @@ def main() -> int:
     ap.add_argument("--cpu-budget", type=float, default=30.0,
                     help="seconds of CPU work for the cpu_baseline leg")
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--serving-requests", type=int, default=2000,
+    ap.add_argument("--serving-requests", type=int, default=3000,
                     help="requests in the batched-serving leg (0 = skip; 10000 = the full stream)")
-    ap.add_argument("--serving-qps", type=float, default=2000.0)
+    ap.add_argument("--serving-qps", type=float, default=8000.0)
+    ap.add_argument("--rankshare", action="store_true",
+                    help="also run one rank's share of the 65.6M-vertex graph (4-rank blocks)")
     ap.add_argument("--serving-pool", type=int, default=16,
                     help="graphs per type in the request pool")
     ap.add_argument("--copy-probe-mib", type=int, default=0,
@@ -360,6 +434,8 @@ def main() -> int:
     }
     if args.serving_requests > 0:
         out["serving"] = serving_leg(args)
+    if args.rankshare:
+        out["rankshare"] = rankshare_leg(args)
     print(json.dumps(out), flush=True)
     return 0 if parity else 1
 

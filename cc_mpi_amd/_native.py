@@ -101,6 +101,22 @@ def gpu_lib() -> ctypes.CDLL:
           P_int, P_int, P_u64, c_void_p])
     _sig(lib, "boundary_edges_sort", None, [P_u64, c_u64])
     _sig(lib, "boundary_edges_dedup", c_u64, [P_u64, c_u64])
+    # per-rank driver building blocks
+    V = c_void_p
+    _sig(lib, "cc_hip_split_count_device", c_int,
+         [V, V, c_u32, c_u32, c_u32, c_u32, V, V, P_u64, P_u64, V])
+    _sig(lib, "cc_hip_split_emit_device", c_int,
+         [V, V, c_u32, c_u32, c_u32, c_u32, V, V, V, V, V])
+    _sig(lib, "cc_hip_boundary_set_rep_device", c_int, [V, c_u64, V, V])
+    _sig(lib, "cc_hip_iota_u32_device", c_int, [V, c_u32, c_u32, V])
+    _sig(lib, "cc_hip_merge_step_device", c_int,
+         [V, c_u64, V, V, c_u32, V, V, V, c_u32, c_u32, c_u32, P_int, V])
+    _sig(lib, "cc_hip_rank_labels_device", c_int, [V, V, c_u32, c_u32, V, P_u64, V])
+    _sig(lib, "cc_hip_owner_keys_device", c_int, [V, c_u64, V, c_int, V, V])
+    _sig(lib, "cc_hip_segments_device", c_int, [V, c_u64, c_int, c_u32, V, V, V])
+    _sig(lib, "cc_hip_gather_u32_device", c_int, [V, V, c_u64, V, V])
+    _sig(lib, "cc_hip_exclusive_scan_u32_device", c_int, [V, c_u64, V, V])
+    _sig(lib, "cc_hip_delta_unpack_device", c_int, [V, c_u64, P_int, P_int, c_int, V, V, V])
     _gpu = lib
     return lib
 

@@ -261,7 +261,86 @@ __global__ __launch_bounds__(kThreads) void k_delta_unpack(const uint64_t *__res
   }
 }
 
+// key = (owner(remote) << 32) | li for raw records (remote << 32) | li;
+// owner by binary search of bounds[0..P].
+__global__ __launch_bounds__(kThreads) void k_owner_keys(const uint64_t *__restrict__ rec, uint64_t nb,
+                                                         const uint32_t *__restrict__ bounds, int P,
+                                                         uint64_t *__restrict__ keys) {
+  GRID_STRIDE(i, nb) {
+    const uint64_t r = rec[i];
+    const uint32_t v = (uint32_t)(r >> 32);
+    int lo = 0, hi = P - 1;
+    while (lo < hi) {  // last q with bounds[q] <= v
+      const int mid = (lo + hi + 1) >> 1;
+      if (bounds[mid] <= v) lo = mid;
+      else hi = mid - 1;
+    }
+    keys[i] = ((uint64_t)lo << 32) | (uint32_t)r;
+  }
+}
+
+// Segments of sorted unique keys (q << 32) | x: starts[q] = first index
+// with high word >= q (q in [0, P]); vals[i] = low word + base.
+__global__ __launch_bounds__(kThreads) void k_segments(const uint64_t *__restrict__ keys, uint64_t m,
+                                                       int P, uint32_t base,
+                                                       int *__restrict__ starts,
+                                                       uint32_t *__restrict__ vals) {
+  GRID_STRIDE(i, m) {
+    const uint64_t q = keys[i] >> 32;
+    const uint64_t prev = (i == 0) ? 0 : (keys[i - 1] >> 32) + 1;
+    for (uint64_t r = prev; r <= q && r <= (uint64_t)P; ++r) starts[r] = (int)i;
+    vals[i] = (uint32_t)keys[i] + base;
+    if (i == m - 1)
+      for (uint64_t r = q + 1; r <= (uint64_t)P; ++r) starts[r] = (int)m;
+  }
+}
+
+// out[i] = table[idx[i]].
+__global__ __launch_bounds__(kThreads) void k_gather_u32(const uint32_t *__restrict__ table,
+                                                         const uint32_t *__restrict__ idx, uint64_t n,
+                                                         uint32_t *__restrict__ out) {
+  GRID_STRIDE(i, n) out[i] = table[idx[i]];
+}
+
 }  // namespace
+
+extern "C" int cc_hip_owner_keys_device(const uint64_t *d_records, uint64_t count,
+                                        const uint32_t *d_bounds, int comm_size,
+                                        uint64_t *d_keys, void *stream) {
+  if (!ccx::ctx()) return 2;
+  if (count == 0) return 0;
+  if (comm_size < 1) return ccx::set_error(4, "cc_hip_owner_keys_device: comm_size < 1");
+  hipStream_t s = ccx::pick_stream(stream);
+  CCX_LAUNCH("owner_keys", s,
+             k_owner_keys<<<grid_for(count), kThreads, 0, s>>>(d_records, count, d_bounds, comm_size,
+                                                               d_keys));
+  return 0;
+}
+
+extern "C" int cc_hip_segments_device(const uint64_t *d_keys, uint64_t count, int nseg,
+                                      uint32_t base, int *d_starts, uint32_t *d_vals,
+                                      void *stream) {
+  if (!ccx::ctx()) return 2;
+  hipStream_t s = ccx::pick_stream(stream);
+  if (count == 0) {
+    CCX_TRY(hipMemsetAsync(d_starts, 0, (size_t)(nseg + 1) * sizeof(int), s));
+    return 0;
+  }
+  if (count > 0x7FFFFFFFull) return ccx::set_error(4, "cc_hip_segments_device: count exceeds int");
+  CCX_LAUNCH("segments", s,
+             k_segments<<<grid_for(count), kThreads, 0, s>>>(d_keys, count, nseg, base, d_starts,
+                                                             d_vals));
+  return 0;
+}
+
+extern "C" int cc_hip_gather_u32_device(const uint32_t *d_table, const uint32_t *d_idx,
+                                        uint64_t count, uint32_t *d_out, void *stream) {
+  if (!ccx::ctx()) return 2;
+  if (count == 0) return 0;
+  hipStream_t s = ccx::pick_stream(stream);
+  CCX_LAUNCH("gather", s, k_gather_u32<<<grid_for(count), kThreads, 0, s>>>(d_table, d_idx, count, d_out));
+  return 0;
+}
 
 extern "C" int cc_hip_exclusive_scan_u32_device(const uint32_t *d_in, uint64_t n, uint64_t *d_out,
                                                 void *stream) {

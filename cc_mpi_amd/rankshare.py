@@ -1,0 +1,231 @@
+"""One rank's share of a distributed CC run, at its real shape, on one GPU.
+
+The per-rank path of the reference's driver (src/cc_mpi.c:221-763) for rank
+`rank` of a P-rank vertex-block partition of a graph resident in HBM, with
+the MPI transport replaced by a seeded injection schedule: every exchange
+interval the ghosts receive label updates (each ghost reaches its true
+global label at a seeded exchange index, with an intermediate value half
+way), exactly as deltas from the other ranks would arrive.  Everything the
+rank computes runs on the GPU through the C ABI:
+
+  split (local CSR + raw boundary records) -> local CC -> send lists (owner
+  keys, sort, dedup, segments) -> rep fill, sort, dedup, ghost extraction
+  -> merge loop (init/hook/<=4-hop update) with the reference's exchange
+  rule, a delta pack into pinned memory at every exchange -> labels.
+
+At convergence the owned labels must equal the true global labels (the
+minimum vertex id of each global component), which the caller supplies from
+a full-graph solve; every pack can be checked against the oracle's pack.
+"""
+from __future__ import annotations
+
+import ctypes
+import time
+
+import numpy as np
+
+from . import _native as nat
+from .boundary import PinnedBuffer, delta_pack_device, sort_device, dedup_device, ghost_extract_device
+from .cc import DeviceBuffer, DeviceGraph
+
+
+def block_bounds(n: int, P: int) -> np.ndarray:
+    """Vertex-block partition bounds (first n % P ranks one vertex more;
+    src/graph_dist.c:22-51)."""
+    base, rem = divmod(n, P)
+    sizes = [base + 1 if r < rem else base for r in range(P)]
+    return np.concatenate([[0], np.cumsum(sizes)]).astype(np.uint32)
+
+
+class RankShare:
+    def __init__(self, full: DeviceGraph, P: int = 4, rank: int = 0):
+        self.full, self.P, self.rank = full, P, rank
+        self.bounds = block_bounds(full.n, P)
+        self.vs, self.ve = int(self.bounds[rank]), int(self.bounds[rank + 1])
+        self.nl = self.ve - self.vs
+        self.times: dict[str, float] = {}
+        self.stats: dict[str, int] = {}
+
+    def _t(self, name, t0):
+        nat.check(nat.gpu_lib().cc_hip_device_sync(), "sync")
+        self.times[name] = self.times.get(name, 0.0) + (time.perf_counter() - t0)
+
+    def preprocess(self) -> None:
+        lib = nat.gpu_lib()
+        nl, vs, ve, n = self.nl, self.vs, self.ve, self.full.n
+        # the owned rows are a window of the full CSR (offsets stay global,
+        # so the column base is the full graph's)
+        d_row = self.full.row_ptr.ptr + 8 * vs
+        d_col = self.full.col_idx.ptr
+        t0 = time.perf_counter()
+        self.lrow = DeviceBuffer(8 * (nl + 1))
+        boff = DeviceBuffer(8 * (nl + 1))
+        mll, nb = ctypes.c_uint64(), ctypes.c_uint64()
+        nat.check(lib.cc_hip_split_count_device(d_row, d_col, nl, vs, ve, n, self.lrow.ptr, boff.ptr,
+                                                ctypes.byref(mll), ctypes.byref(nb), None), "split count")
+        self.m_ll, self.nb = int(mll.value), int(nb.value)
+        self.lcol = DeviceBuffer(max(4, 4 * self.m_ll))
+        rec = DeviceBuffer(max(8, 8 * self.nb))
+        nat.check(lib.cc_hip_split_emit_device(d_row, d_col, nl, vs, ve, n, self.lrow.ptr, boff.ptr,
+                                               self.lcol.ptr, rec.ptr, None), "split emit")
+        boff.close()
+        self._t("split", t0)
+
+        t0 = time.perf_counter()
+        self.comp_of = DeviceBuffer(4 * max(1, nl))
+        nat.check(lib.cc_hip_components_device(self.lrow.ptr, self.lcol.ptr, nl, self.m_ll,
+                                               self.comp_of.ptr, None), "local CC")
+        self._t("local_cc", t0)
+        self.lrow.close()
+        self.lcol.close()
+
+        # send lists: the owned vertices each other rank has as ghosts
+        t0 = time.perf_counter()
+        d_bounds = DeviceBuffer.from_array(self.bounds)
+        keys = DeviceBuffer(max(8, 8 * self.nb))
+        alt = DeviceBuffer(max(8, 8 * self.nb))
+        nat.check(lib.cc_hip_owner_keys_device(rec.ptr, self.nb, d_bounds.ptr, self.P, keys.ptr,
+                                               None), "owner keys")
+        which = sort_device(keys, alt, self.nb)
+        srt, oth = (alt, keys) if which else (keys, alt)
+        nsend = dedup_device(srt, self.nb, oth)
+        d_starts = DeviceBuffer(4 * (self.P + 1))
+        self.send_to_flat = DeviceBuffer(max(4, 4 * nsend))
+        nat.check(lib.cc_hip_segments_device(oth.ptr, nsend, self.P, vs, d_starts.ptr,
+                                             self.send_to_flat.ptr, None), "segments")
+        starts = np.empty(self.P + 1, dtype=np.int32)
+        d_starts.download(starts)
+        self.total_send = nsend
+        # destination segments in dests order (ranks with something to send)
+        self.dests = [q for q in range(self.P) if starts[q + 1] > starts[q]]
+        self.seg_start = np.array([starts[q] for q in self.dests] + [nsend], dtype=np.int32)
+        self._t("send_lists", t0)
+
+        # boundary records: rep fill, sort, dedup, ghost extraction
+        t0 = time.perf_counter()
+        nat.check(lib.cc_hip_boundary_set_rep_device(rec.ptr, self.nb, self.comp_of.ptr, None),
+                  "rep fill")
+        which = sort_device(rec, keys, self.nb)
+        srt, oth = (keys, rec) if which else (rec, keys)
+        self._t("boundary_sort", t0)
+        t0 = time.perf_counter()
+        self.nbd = dedup_device(srt, self.nb, oth)
+        self._t("boundary_dedup", t0)
+        t0 = time.perf_counter()
+        self.ghosts = DeviceBuffer(4 * max(1, self.nbd))
+        self.ng = ghost_extract_device(oth, self.nbd, srt, self.ghosts)
+        self.bnd = srt
+        self._t("ghost_extract", t0)
+        for b in (oth, alt, d_bounds, d_starts):
+            if b is not srt:
+                b.close()
+        self.stats.update(n_local=nl, m_local_local=self.m_ll, boundary_raw=self.nb,
+                          boundary_dedup=self.nbd, ghosts=self.ng, total_send_to=nsend,
+                          outdegree=len(self.dests))
+
+    def merge(self, truth_labels: DeviceBuffer, exchange_interval: int = 1, seed: int = 1,
+              check_pack=None, max_iters: int = 100000) -> np.ndarray:
+        """Run the merge loop with injected ghost deltas; returns the owned
+        labels (comp_label[comp_of[li]]).  check_pack(seg_start, comp_label,
+        comp_of, prev_before, pairs, counts, displs) is called per exchange
+        if given."""
+        lib = nat.gpu_lib()
+        nl, vs, ve, ng = self.nl, self.vs, self.ve, self.ng
+        interval = max(1, exchange_interval)
+        comp_label = DeviceBuffer(4 * max(1, nl))
+        comp_min = DeviceBuffer(4 * max(1, nl))
+        nat.check(lib.cc_hip_iota_u32_device(comp_label.ptr, nl, vs, None), "iota")
+        ghost_labels = DeviceBuffer(4 * max(1, ng))
+        nat.check(lib.cc_hip_memcpy_d2d(ghost_labels.ptr, self.ghosts.ptr, 4 * ng, None), "copy")
+        # the injection schedule: ghost g gets an intermediate label at
+        # exchange mid[g] and its true global label at exchange due[g]
+        ghost_ids = np.empty(ng, dtype=np.uint32)
+        self.ghosts.download(ghost_ids)
+        truth = np.empty(ng, dtype=np.uint32)
+        tmp = DeviceBuffer(4 * max(1, ng))
+        nat.check(lib.cc_hip_gather_u32_device(truth_labels.ptr, self.ghosts.ptr, ng, tmp.ptr, None),
+                  "gather")
+        tmp.download(truth)
+        tmp.close()
+        rng = np.random.default_rng(seed)
+        due = rng.geometric(0.5, ng).astype(np.int64) - 1
+        mid = due // 2
+        midval = truth + ((ghost_ids.astype(np.int64) - truth) * rng.random(ng)).astype(np.uint32)
+        cur = ghost_ids.copy()
+        last_exchange = int(due.max()) if ng else -1
+        prev_sent = DeviceBuffer(4 * max(1, self.total_send))
+        ones = np.full(max(1, self.total_send), 0xFFFFFFFF, dtype=np.uint32)
+        prev_sent.upload(ones[: max(1, self.total_send)])
+        sendbuf = PinnedBuffer(max(1, self.total_send))
+        comp_of_h = None
+        if check_pack is not None:
+            comp_of_h = np.empty(nl, dtype=np.uint32)
+            self.comp_of.download(comp_of_h)
+        pending_since = 1 if (ng and self.nbd and nl) else 0
+        steps, k, iters, packs, emitted_total = 0, 0, 0, 0, 0
+        t_merge = t_pack = 0.0
+        changed = ctypes.c_int(0)
+        while True:
+            t0 = time.perf_counter()
+            nat.check(lib.cc_hip_merge_step_device(self.bnd.ptr, self.nbd, self.ghosts.ptr,
+                                                   ghost_labels.ptr, ng, self.comp_of.ptr,
+                                                   comp_label.ptr, comp_min.ptr, nl, vs, ve,
+                                                   ctypes.byref(changed), None), "merge step")
+            t_merge += time.perf_counter() - t0
+            iters += 1
+            if changed.value:
+                pending_since = 1
+            others_pending = k <= last_exchange
+            if not (pending_since or others_pending) or iters >= max_iters:
+                break
+            steps += 1
+            if steps >= interval or not changed.value:
+                t0 = time.perf_counter()
+                if self.dests:
+                    before = None
+                    if check_pack is not None:
+                        before = np.empty(self.total_send, dtype=np.uint32)
+                        prev_sent.download(before)
+                    pairs, counts, displs = delta_pack_device(
+                        self.send_to_flat, self.total_send, self.seg_start, comp_label,
+                        self.comp_of, vs, ve, prev_sent, sendbuf)
+                    emitted_total += int(pairs.size)
+                    packs += 1
+                    t_pack += time.perf_counter() - t0
+                    if check_pack is not None:
+                        cl = np.empty(nl, dtype=np.uint32)
+                        comp_label.download(cl)
+                        stf = np.empty(self.total_send, dtype=np.uint32)
+                        self.send_to_flat.download(stf)
+                        check_pack(stf, self.seg_start, cl, comp_of_h, vs, ve, before,
+                                   pairs.copy(), counts, displs)
+                # deltas from the other ranks for this exchange
+                upd = (due == k)
+                cur[upd] = truth[upd]
+                half = (mid == k) & (due > k)
+                cur[half] = np.minimum(cur[half], midval[half])
+                if upd.any() or half.any():
+                    ghost_labels.upload(cur)
+                k += 1
+                pending_since = 0
+                steps = 0
+        labels = DeviceBuffer(4 * max(1, nl))
+        roots = ctypes.c_uint64()
+        nat.check(lib.cc_hip_rank_labels_device(comp_label.ptr, self.comp_of.ptr, nl, vs,
+                                                labels.ptr, ctypes.byref(roots), None), "labels")
+        out = np.empty(nl, dtype=np.uint32)
+        labels.download(out)
+        for b in (labels, comp_label, comp_min, ghost_labels, prev_sent):
+            b.close()
+        sendbuf.close()
+        self.times["merge_steps"] = t_merge
+        self.times["delta_packs"] = t_pack
+        self.stats.update(merge_iterations=iters, exchanges=k, packs=packs,
+                          pairs_emitted=emitted_total, local_roots=int(roots.value))
+        return out
+
+    def close(self) -> None:
+        for name in ("comp_of", "send_to_flat", "ghosts", "bnd"):
+            b = getattr(self, name, None)
+            if b is not None:
+                b.close()

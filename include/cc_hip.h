@@ -184,6 +184,20 @@ int cc_hip_boundary_set_rep_device(uint64_t *d_records, uint64_t count,
 /* d_out[i] = base + i. */
 int cc_hip_iota_u32_device(uint32_t *d_out, uint32_t n, uint32_t base, void *stream);
 
+/* Send-list construction for the exchange plan: keys (owner(remote) << 32)
+ * | li from raw records (remote << 32) | li, owner by the partition bounds
+ * d_bounds[0..comm_size] (src/cc_mpi.c:41-76). */
+int cc_hip_owner_keys_device(const uint64_t *d_records, uint64_t count, const uint32_t *d_bounds,
+                             int comm_size, uint64_t *d_keys, void *stream);
+/* Segments of sorted unique keys (q << 32) | x for q in [0, nseg):
+ * d_starts[q] = first index of segment q (d_starts[nseg] = count) and
+ * d_vals[i] = x + base. */
+int cc_hip_segments_device(const uint64_t *d_keys, uint64_t count, int nseg, uint32_t base,
+                           int *d_starts, uint32_t *d_vals, void *stream);
+/* d_out[i] = d_table[d_idx[i]]. */
+int cc_hip_gather_u32_device(const uint32_t *d_table, const uint32_t *d_idx, uint64_t count,
+                             uint32_t *d_out, void *stream);
+
 /*
  * One merge round (src/cc_mpi.c:547-622) over records (gidx << 32) | rep:
  * comp_min = comp_label on the reps, comp_min[rep] lowered by every
