@@ -123,6 +123,38 @@ class RankShare:
                           boundary_dedup=self.nbd, ghosts=self.ng, total_send_to=nsend,
                           outdegree=len(self.dests))
 
+    def owner_local_labels(self) -> np.ndarray:
+        """For every vertex, the label its owning rank holds right after its
+        own local CC (v_start_q + comp_of_q[v - v_start_q]): what that rank
+        sends at its first exchange.  Computed with the same split and
+        local-CC kernels for every rank's block."""
+        lib = nat.gpu_lib()
+        out = np.empty(self.full.n, dtype=np.uint32)
+        d_col = self.full.col_idx.ptr
+        for q in range(self.P):
+            a, b = int(self.bounds[q]), int(self.bounds[q + 1])
+            nl = b - a
+            if nl == 0:
+                continue
+            lrow, boff = DeviceBuffer(8 * (nl + 1)), DeviceBuffer(8 * (nl + 1))
+            mll, nb = ctypes.c_uint64(), ctypes.c_uint64()
+            d_row = self.full.row_ptr.ptr + 8 * a
+            nat.check(lib.cc_hip_split_count_device(d_row, d_col, nl, a, b, self.full.n, lrow.ptr,
+                                                    boff.ptr, ctypes.byref(mll), ctypes.byref(nb),
+                                                    None), "split count")
+            lcol, rec = DeviceBuffer(max(4, 4 * mll.value)), DeviceBuffer(max(8, 8 * nb.value))
+            nat.check(lib.cc_hip_split_emit_device(d_row, d_col, nl, a, b, self.full.n, lrow.ptr,
+                                                   boff.ptr, lcol.ptr, rec.ptr, None), "split emit")
+            comp = DeviceBuffer(4 * nl)
+            nat.check(lib.cc_hip_components_device(lrow.ptr, lcol.ptr, nl, mll.value, comp.ptr,
+                                                   None), "local CC")
+            part = np.empty(nl, dtype=np.uint32)
+            comp.download(part)
+            out[a:b] = part + np.uint32(a)
+            for buf in (lrow, boff, lcol, rec, comp):
+                buf.close()
+        return out
+
     def merge(self, truth_labels: DeviceBuffer, exchange_interval: int = 1, seed: int = 1,
               check_pack=None, max_iters: int = 100000) -> np.ndarray:
         """Run the merge loop with injected ghost deltas; returns the owned
@@ -137,8 +169,9 @@ class RankShare:
         nat.check(lib.cc_hip_iota_u32_device(comp_label.ptr, nl, vs, None), "iota")
         ghost_labels = DeviceBuffer(4 * max(1, ng))
         nat.check(lib.cc_hip_memcpy_d2d(ghost_labels.ptr, self.ghosts.ptr, 4 * ng, None), "copy")
-        # the injection schedule: ghost g gets an intermediate label at
-        # exchange mid[g] and its true global label at exchange due[g]
+        # the injection schedule: ghost g gets its owner's local-CC label at
+        # exchange mid[g] and its true global label at exchange due[g]; both
+        # are ids of vertices in g's component, as every real delta is
         ghost_ids = np.empty(ng, dtype=np.uint32)
         self.ghosts.download(ghost_ids)
         truth = np.empty(ng, dtype=np.uint32)
@@ -150,7 +183,9 @@ class RankShare:
         rng = np.random.default_rng(seed)
         due = rng.geometric(0.5, ng).astype(np.int64) - 1
         mid = due // 2
-        midval = truth + ((ghost_ids.astype(np.int64) - truth) * rng.random(ng)).astype(np.uint32)
+        if not hasattr(self, "_owner_local"):
+            self._owner_local = self.owner_local_labels()
+        midval = self._owner_local[ghost_ids]
         cur = ghost_ids.copy()
         last_exchange = int(due.max()) if ng else -1
         prev_sent = DeviceBuffer(4 * max(1, self.total_send))
