@@ -104,7 +104,7 @@ def gpu_lib() -> ctypes.CDLL:
     # per-rank driver building blocks
     V = c_void_p
     _sig(lib, "cc_hip_split_count_device", c_int,
-         [V, V, c_u32, c_u32, c_u32, c_u32, V, V, P_u64, P_u64, V])
+         [V, V, c_u32, c_u32, c_u32, c_u32, P_u64, P_u64, V])
     _sig(lib, "cc_hip_split_emit_device", c_int,
          [V, V, c_u32, c_u32, c_u32, c_u32, V, V, V, V, V])
     _sig(lib, "cc_hip_boundary_set_rep_device", c_int, [V, c_u64, V, V])

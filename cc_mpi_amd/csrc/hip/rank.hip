@@ -102,38 +102,132 @@ __global__ __launch_bounds__(kThreads) void k_scan_u32(const uint32_t *__restric
   }
 }
 
-__global__ __launch_bounds__(kThreads) void k_split_count(const uint64_t *__restrict__ row_ptr,
-                                                          const uint32_t *__restrict__ col,
-                                                          uint32_t n_local, uint32_t v_start,
-                                                          uint32_t v_end, uint32_t n_global,
-                                                          uint32_t *__restrict__ lcnt,
-                                                          uint32_t *__restrict__ bcnt) {
-  GRID_STRIDE(li, n_local) {
-    uint32_t l = 0, b = 0;
-    for (uint64_t j = row_ptr[li]; j < row_ptr[li + 1]; ++j) {
-      const uint32_t v = col[j];
-      if (v >= v_start && v < v_end) ++l;
-      else if (v < n_global) ++b;
-    }
-    lcnt[li] = l;
-    bcnt[li] = b;
+// Entries j in [e0, e1) of the owned rows (row_ptr may carry global
+// offsets: e0 = row_ptr[0]).  Totals of local-local and boundary entries.
+__global__ __launch_bounds__(kThreads) void k_split_count(const uint32_t *__restrict__ col,
+                                                          uint64_t e0, uint64_t e1,
+                                                          uint32_t v_start, uint32_t v_end,
+                                                          uint32_t n_global,
+                                                          unsigned long long *__restrict__ totals) {
+  uint64_t l = 0, b = 0;
+  for (uint64_t j = e0 + (uint64_t)blockIdx.x * kThreads + threadIdx.x; j < e1;
+       j += (uint64_t)gridDim.x * kThreads) {
+    const uint32_t v = col[j];
+    const bool loc = v >= v_start && v < v_end;
+    l += loc;
+    b += (!loc && v < n_global);
+  }
+  for (int d = kWave / 2; d > 0; d >>= 1) {
+    l += __shfl_xor(l, d, kWave);
+    b += __shfl_xor(b, d, kWave);
+  }
+  if ((threadIdx.x & (kWave - 1)) == 0) {
+    if (l) atomicAdd(&totals[0], (unsigned long long)l);
+    if (b) atomicAdd(&totals[1], (unsigned long long)b);
   }
 }
 
+// First index r in [0, n] with row_ptr[r] >= x (row_ptr ascending).
+__device__ __forceinline__ uint32_t rows_lower_bound(const uint64_t *row_ptr, uint32_t n, uint64_t x) {
+  uint32_t lo = 0, hi = n + 1;
+  while (lo < hi) {
+    const uint32_t mid = lo + (hi - lo) / 2;
+    if (row_ptr[mid] < x) lo = mid + 1;
+    else hi = mid;
+  }
+  return lo;
+}
+
+// The split as one order-preserving dual stream compaction over the
+// entries: each thread owns 16 consecutive entries, a 4096-entry tile
+// publishes both of its counts through decoupled look-back (status words
+// 2*tile and 2*tile+1), and a per-thread cursor over the row starts gives
+// each row's local/boundary offsets and each boundary record's row li.
+// col_idx is read once, coalesced, whatever the degree distribution.
 __global__ __launch_bounds__(kThreads) void k_split_emit(const uint64_t *__restrict__ row_ptr,
                                                          const uint32_t *__restrict__ col,
                                                          uint32_t n_local, uint32_t v_start,
                                                          uint32_t v_end, uint32_t n_global,
-                                                         const uint64_t *__restrict__ loff,
-                                                         const uint64_t *__restrict__ boff,
+                                                         uint64_t *__restrict__ lrow,
+                                                         uint64_t *__restrict__ boff,
                                                          uint32_t *__restrict__ lcol,
-                                                         uint64_t *__restrict__ brec) {
-  GRID_STRIDE(li, n_local) {
-    uint64_t wl = loff[li], wb = boff[li];
-    for (uint64_t j = row_ptr[li]; j < row_ptr[li + 1]; ++j) {
-      const uint32_t v = col[j];
-      if (v >= v_start && v < v_end) lcol[wl++] = v - v_start;
-      else if (v < n_global) brec[wb++] = ((uint64_t)v << 32) | (uint32_t)li;
+                                                         uint64_t *__restrict__ brec,
+                                                         uint32_t *__restrict__ tile_ctr,
+                                                         uint64_t *__restrict__ status, uint32_t epoch) {
+  __shared__ uint32_t s_tile;
+  __shared__ uint64_t s_wsum[kWaves];
+  __shared__ uint64_t s_base[2];
+  const int t = threadIdx.x;
+  const int lane = t & (kWave - 1);
+  const int w = t / kWave;
+  const uint64_t e0 = row_ptr[0], e1 = row_ptr[n_local];
+  const uint64_t n = e1 - e0;
+  if (t == 0) s_tile = atomicAdd(tile_ctr, 1u);
+  __syncthreads();
+  const uint64_t tile = s_tile;
+  if (tile * kTile >= n) return;
+  const uint64_t first = e0 + tile * kTile + (uint64_t)t * kItems;
+  uint32_t v[kItems];
+  uint32_t lmask = 0, bmask = 0;
+#pragma unroll
+  for (int k = 0; k < kItems; ++k) {
+    v[k] = (first + k < e1) ? col[first + k] : 0xFFFFFFFFu;
+    const bool in = first + k < e1;
+    const bool loc = in && v[k] >= v_start && v[k] < v_end;
+    if (loc) lmask |= 1u << k;
+    else if (in && v[k] < n_global) bmask |= 1u << k;
+  }
+  // both counts in one word: local low 32 bits, boundary high 32 bits
+  const uint64_t mine = (uint64_t)__popc(lmask) | ((uint64_t)__popc(bmask) << 32);
+  const uint64_t incl = ccx::wave_incl_scan<uint64_t>(mine, lane);
+  if (lane == kWave - 1) s_wsum[w] = incl;
+  __syncthreads();
+  uint64_t wprefix = 0, tile_total = 0;
+  for (int ww = 0; ww < kWaves; ++ww) {
+    if (ww < w) wprefix += s_wsum[ww];
+    tile_total += s_wsum[ww];
+  }
+  if (t < 2) {  // thread 0: local count, thread 1: boundary count
+    const uint64_t cnt = (t == 0) ? (tile_total & 0xFFFFFFFFull) : (tile_total >> 32);
+    uint64_t *my = status + 2 * tile + t;
+    uint64_t excl = 0;
+    if (tile == 0) {
+      ccx::status_store(my, ccx::tag_word(epoch, ccx::kFlagInclusive, cnt));
+    } else {
+      ccx::status_store(my, ccx::tag_word(epoch, ccx::kFlagAggregate, cnt));
+      excl = ccx::lookback(status, tile, 2, t, epoch);
+      ccx::status_store(my, ccx::tag_word(epoch, ccx::kFlagInclusive, excl + cnt));
+    }
+    s_base[t] = excl;
+  }
+  __syncthreads();
+  const uint64_t ex = wprefix + incl - mine;
+  uint64_t lr = s_base[0] + (ex & 0xFFFFFFFFull);
+  uint64_t br = s_base[1] + (ex >> 32);
+  if (first >= e1) return;
+  // rows starting inside [first, first + 16) get their offsets here
+  uint32_t r = rows_lower_bound(row_ptr, n_local, first);
+  // row of entry `first`: the last row starting at or before it
+  uint32_t cur = (r <= n_local && row_ptr[r] == first) ? r : r - 1;
+  const uint64_t stop = (first + kItems < e1) ? first + kItems : e1;
+#pragma unroll
+  for (int k = 0; k < kItems; ++k) {
+    const uint64_t j = first + k;
+    if (j >= stop) break;
+    while (r <= n_local && row_ptr[r] == j) {  // rows (possibly empty) starting at j
+      lrow[r] = lr;
+      boff[r] = br;
+      cur = r;
+      ++r;
+    }
+    if ((lmask >> k) & 1u) lcol[lr++] = v[k] - v_start;
+    else if ((bmask >> k) & 1u) brec[br++] = ((uint64_t)v[k] << 32) | cur;
+  }
+  if (stop == e1) {  // trailing rows start at e1 (at least row n_local)
+    while (r <= n_local) {
+      lrow[r] = lr;
+      boff[r] = br;
+      ++r;
     }
   }
 }
@@ -143,6 +237,14 @@ __global__ __launch_bounds__(kThreads) void k_set_rep(uint64_t *__restrict__ rec
   GRID_STRIDE(i, nb) {
     const uint64_t r = rec[i];
     rec[i] = (r & 0xFFFFFFFF00000000ull) | comp_of[(uint32_t)r];
+  }
+}
+
+__global__ __launch_bounds__(kThreads) void k_fill_u64(uint64_t *__restrict__ a, uint64_t *__restrict__ b,
+                                                       uint64_t n) {
+  GRID_STRIDE(i, n) {
+    a[i] = 0;
+    b[i] = 0;
   }
 }
 
@@ -363,71 +465,29 @@ extern "C" int cc_hip_exclusive_scan_u32_device(const uint32_t *d_in, uint64_t n
   return 0;
 }
 
-extern "C" int cc_hip_split_device(const uint64_t *d_row_ptr, const uint32_t *d_col_idx,
-                                   uint32_t n_local, uint32_t v_start, uint32_t v_end,
-                                   uint32_t n_global, uint64_t *d_lrow, uint32_t *d_lcol,
-                                   uint64_t *d_boff, uint64_t *d_brec, uint64_t *m_local_local,
-                                   uint64_t *n_boundary, void *stream) {
-  if (!ccx::ctx()) return 2;
-  hipStream_t s = ccx::pick_stream(stream);
-  ccx::StreamState *st = ccx::stream_state(s);
-  if (!st) return 3;
-  if (n_local == 0) {
-    CCX_TRY(hipMemsetAsync(d_lrow, 0, sizeof(uint64_t), s));
-    *m_local_local = 0;
-    *n_boundary = 0;
-    return 0;
-  }
-  // per-row counts in the stream temp buffer: two uint32 arrays
-  const size_t cbytes = ((size_t)n_local * sizeof(uint32_t) + 255) & ~(size_t)255;
-  char *tmp = (char *)ccx::stream_tmp(st, 2 * cbytes);
-  if (!tmp) return 3;
-  uint32_t *lcnt = (uint32_t *)tmp, *bcnt = (uint32_t *)(tmp + cbytes);
-  const unsigned g = grid_for(n_local);
-  CCX_LAUNCH("split_count", s,
-             k_split_count<<<g, kThreads, 0, s>>>(d_row_ptr, d_col_idx, n_local, v_start, v_end,
-                                                  n_global, lcnt, bcnt));
-  if (int rc = cc_hip_exclusive_scan_u32_device(lcnt, n_local, d_lrow, s)) return rc;
-  if (int rc = cc_hip_exclusive_scan_u32_device(bcnt, n_local, d_boff, s)) return rc;
-  CCX_LAUNCH("split_emit", s,
-             k_split_emit<<<g, kThreads, 0, s>>>(d_row_ptr, d_col_idx, n_local, v_start, v_end,
-                                                 n_global, d_lrow, d_boff, d_lcol, d_brec));
-  uint64_t *h = st->h_stage;
-  CCX_TRY(hipMemcpyAsync(h, d_lrow + n_local, sizeof(uint64_t), hipMemcpyDeviceToHost, s));
-  CCX_TRY(hipMemcpyAsync(h + 1, d_boff + n_local, sizeof(uint64_t), hipMemcpyDeviceToHost, s));
-  CCX_TRY(hipStreamSynchronize(s));
-  *m_local_local = h[0];
-  *n_boundary = h[1];
-  return 0;
-}
-
 extern "C" int cc_hip_split_count_device(const uint64_t *d_row_ptr, const uint32_t *d_col_idx,
                                          uint32_t n_local, uint32_t v_start, uint32_t v_end,
-                                         uint32_t n_global, uint64_t *d_lrow, uint64_t *d_boff,
-                                         uint64_t *m_local_local, uint64_t *n_boundary,
-                                         void *stream) {
+                                         uint32_t n_global, uint64_t *m_local_local,
+                                         uint64_t *n_boundary, void *stream) {
   if (!ccx::ctx()) return 2;
+  *m_local_local = 0;
+  *n_boundary = 0;
+  if (n_local == 0) return 0;
   hipStream_t s = ccx::pick_stream(stream);
   ccx::StreamState *st = ccx::stream_state(s);
   if (!st) return 3;
-  if (n_local == 0) {
-    *m_local_local = 0;
-    *n_boundary = 0;
-    return 0;
-  }
-  const size_t cbytes = ((size_t)n_local * sizeof(uint32_t) + 255) & ~(size_t)255;
-  char *tmp = (char *)ccx::stream_tmp(st, 2 * cbytes);
-  if (!tmp) return 3;
-  uint32_t *lcnt = (uint32_t *)tmp, *bcnt = (uint32_t *)(tmp + cbytes);
-  CCX_LAUNCH("split_count", s,
-             k_split_count<<<grid_for(n_local), kThreads, 0, s>>>(d_row_ptr, d_col_idx, n_local,
-                                                                  v_start, v_end, n_global, lcnt,
-                                                                  bcnt));
-  if (int rc = cc_hip_exclusive_scan_u32_device(lcnt, n_local, d_lrow, s)) return rc;
-  if (int rc = cc_hip_exclusive_scan_u32_device(bcnt, n_local, d_boff, s)) return rc;
   uint64_t *h = st->h_stage;
-  CCX_TRY(hipMemcpyAsync(h, d_lrow + n_local, sizeof(uint64_t), hipMemcpyDeviceToHost, s));
-  CCX_TRY(hipMemcpyAsync(h + 1, d_boff + n_local, sizeof(uint64_t), hipMemcpyDeviceToHost, s));
+  CCX_TRY(hipMemcpyAsync(h, d_row_ptr, sizeof(uint64_t), hipMemcpyDeviceToHost, s));
+  CCX_TRY(hipMemcpyAsync(h + 1, d_row_ptr + n_local, sizeof(uint64_t), hipMemcpyDeviceToHost, s));
+  CCX_TRY(hipStreamSynchronize(s));
+  const uint64_t e0 = h[0], e1 = h[1];
+  if (e1 <= e0) return 0;
+  unsigned long long *tot = (unsigned long long *)(st->d_words + 28);
+  CCX_TRY(hipMemsetAsync(tot, 0, 2 * sizeof(uint64_t), s));
+  CCX_LAUNCH("split_count", s,
+             k_split_count<<<grid_for(e1 - e0), kThreads, 0, s>>>(d_col_idx, e0, e1, v_start,
+                                                                  v_end, n_global, tot));
+  CCX_TRY(hipMemcpyAsync(h, tot, 2 * sizeof(uint64_t), hipMemcpyDeviceToHost, s));
   CCX_TRY(hipStreamSynchronize(s));
   *m_local_local = h[0];
   *n_boundary = h[1];
@@ -436,16 +496,34 @@ extern "C" int cc_hip_split_count_device(const uint64_t *d_row_ptr, const uint32
 
 extern "C" int cc_hip_split_emit_device(const uint64_t *d_row_ptr, const uint32_t *d_col_idx,
                                         uint32_t n_local, uint32_t v_start, uint32_t v_end,
-                                        uint32_t n_global, const uint64_t *d_lrow,
-                                        const uint64_t *d_boff, uint32_t *d_lcol,
-                                        uint64_t *d_brec, void *stream) {
+                                        uint32_t n_global, uint64_t *d_lrow, uint64_t *d_boff,
+                                        uint32_t *d_lcol, uint64_t *d_brec, void *stream) {
   if (!ccx::ctx()) return 2;
-  if (n_local == 0) return 0;
   hipStream_t s = ccx::pick_stream(stream);
+  ccx::StreamState *st = ccx::stream_state(s);
+  if (!st) return 3;
+  uint64_t *h = st->h_stage;
+  CCX_TRY(hipMemcpyAsync(h, d_row_ptr, sizeof(uint64_t), hipMemcpyDeviceToHost, s));
+  CCX_TRY(hipMemcpyAsync(h + 1, d_row_ptr + n_local, sizeof(uint64_t), hipMemcpyDeviceToHost, s));
+  CCX_TRY(hipStreamSynchronize(s));
+  const uint64_t n = h[1] - h[0];
+  if (n_local == 0 || n == 0) {
+    // no entries: every offset is 0
+    CCX_LAUNCH("split_zero", s, k_fill_u64<<<grid_for((uint64_t)n_local + 1), kThreads, 0, s>>>(d_lrow, d_boff, (uint64_t)n_local + 1));
+    return 0;
+  }
+  const uint64_t tiles = (n + kTile - 1) / kTile;
+  if (tiles > 0xFFFFFFFFull) return ccx::set_error(4, "cc_hip_split_emit_device: input too large");
+  if (int rc = ccx::ensure_status(st, 2 * tiles)) return rc;
+  uint32_t epoch;
+  if (int rc = ccx::next_epoch(st, &epoch)) return rc;
+  uint32_t *ctr = st->d_words + 10;
+  CCX_TRY(hipMemsetAsync(ctr, 0, sizeof(uint32_t), s));
   CCX_LAUNCH("split_emit", s,
-             k_split_emit<<<grid_for(n_local), kThreads, 0, s>>>(d_row_ptr, d_col_idx, n_local,
-                                                                 v_start, v_end, n_global, d_lrow,
-                                                                 d_boff, d_lcol, d_brec));
+             k_split_emit<<<(unsigned)tiles, kThreads, 0, s>>>(d_row_ptr, d_col_idx, n_local, v_start,
+                                                               v_end, n_global, d_lrow, d_boff,
+                                                               d_lcol, d_brec, ctr, st->d_status,
+                                                               epoch));
   return 0;
 }
 

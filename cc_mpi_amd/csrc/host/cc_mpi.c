@@ -102,8 +102,8 @@ void compute_connected_components_mpi_advanced(const DistCSRGraph *Gd, uint32_t 
   uint64_t *d_lrow = (uint64_t *)dalloc(((size_t)n_local + 1) * 8, comm, "alloc lrow");
   uint64_t *d_boff = (uint64_t *)dalloc(((size_t)n_local + 1) * 8, comm, "alloc boff");
   uint64_t m_ll = 0, nb = 0;
-  check(cc_hip_split_count_device(d_row, d_col, n_local, v_start, v_end, n_global, d_lrow, d_boff,
-                                  &m_ll, &nb, NULL),
+  check(cc_hip_split_count_device(d_row, d_col, n_local, v_start, v_end, n_global, &m_ll, &nb,
+                                  NULL),
         comm, "split count");
   uint32_t *d_lcol = (uint32_t *)dalloc((size_t)m_ll * 4, comm, "alloc lcol");
   uint64_t *d_rec = (uint64_t *)dalloc((size_t)nb * 8, comm, "alloc boundary");

@@ -61,8 +61,8 @@ class RankShare:
         self.lrow = DeviceBuffer(8 * (nl + 1))
         boff = DeviceBuffer(8 * (nl + 1))
         mll, nb = ctypes.c_uint64(), ctypes.c_uint64()
-        nat.check(lib.cc_hip_split_count_device(d_row, d_col, nl, vs, ve, n, self.lrow.ptr, boff.ptr,
-                                                ctypes.byref(mll), ctypes.byref(nb), None), "split count")
+        nat.check(lib.cc_hip_split_count_device(d_row, d_col, nl, vs, ve, n, ctypes.byref(mll),
+                                                ctypes.byref(nb), None), "split count")
         self.m_ll, self.nb = int(mll.value), int(nb.value)
         self.lcol = DeviceBuffer(max(4, 4 * self.m_ll))
         rec = DeviceBuffer(max(8, 8 * self.nb))
@@ -139,9 +139,9 @@ class RankShare:
             lrow, boff = DeviceBuffer(8 * (nl + 1)), DeviceBuffer(8 * (nl + 1))
             mll, nb = ctypes.c_uint64(), ctypes.c_uint64()
             d_row = self.full.row_ptr.ptr + 8 * a
-            nat.check(lib.cc_hip_split_count_device(d_row, d_col, nl, a, b, self.full.n, lrow.ptr,
-                                                    boff.ptr, ctypes.byref(mll), ctypes.byref(nb),
-                                                    None), "split count")
+            nat.check(lib.cc_hip_split_count_device(d_row, d_col, nl, a, b, self.full.n,
+                                                    ctypes.byref(mll), ctypes.byref(nb), None),
+                      "split count")
             lcol, rec = DeviceBuffer(max(4, 4 * mll.value)), DeviceBuffer(max(8, 8 * nb.value))
             nat.check(lib.cc_hip_split_emit_device(d_row, d_col, nl, a, b, self.full.n, lrow.ptr,
                                                    boff.ptr, lcol.ptr, rec.ptr, None), "split emit")

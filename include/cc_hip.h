@@ -157,25 +157,23 @@ int cc_hip_exclusive_scan_u32_device(const uint32_t *d_in, uint64_t n, uint64_t 
 
 /*
  * Local-local split of an owned CSR with global column ids
- * (src/cc_mpi.c:252-304, 364-394).  Count phase: d_lrow[0..n_local] gets the
- * local CSR offsets (columns in [v_start, v_end)), d_boff[0..n_local] the
- * boundary-record offsets (other columns < n_global); the totals are
- * returned (blocking).  Emit phase: d_lcol gets the rebased local columns
- * and d_brec the records (remote << 32) | li, both in row order.
+ * (src/cc_mpi.c:252-304, 364-394).  d_row_ptr holds the n_local+1 offsets
+ * of the owned rows into d_col_idx (they need not start at 0).  Count
+ * phase: the number of local-local entries (columns in [v_start, v_end))
+ * and of boundary entries (other columns < n_global), blocking.  Emit
+ * phase: d_lrow[0..n_local] = local CSR offsets, d_lcol = rebased local
+ * columns, d_boff[0..n_local] = boundary-record offsets per row, d_brec =
+ * records (remote << 32) | li, all in row order (one order-preserving
+ * compaction over the entries).
  */
 int cc_hip_split_count_device(const uint64_t *d_row_ptr, const uint32_t *d_col_idx,
                               uint32_t n_local, uint32_t v_start, uint32_t v_end,
-                              uint32_t n_global, uint64_t *d_lrow, uint64_t *d_boff,
-                              uint64_t *m_local_local, uint64_t *n_boundary, void *stream);
+                              uint32_t n_global, uint64_t *m_local_local, uint64_t *n_boundary,
+                              void *stream);
 int cc_hip_split_emit_device(const uint64_t *d_row_ptr, const uint32_t *d_col_idx,
                              uint32_t n_local, uint32_t v_start, uint32_t v_end,
-                             uint32_t n_global, const uint64_t *d_lrow, const uint64_t *d_boff,
+                             uint32_t n_global, uint64_t *d_lrow, uint64_t *d_boff,
                              uint32_t *d_lcol, uint64_t *d_brec, void *stream);
-/* Both phases; d_boff is caller-provided scratch of n_local+1 entries. */
-int cc_hip_split_device(const uint64_t *d_row_ptr, const uint32_t *d_col_idx, uint32_t n_local,
-                        uint32_t v_start, uint32_t v_end, uint32_t n_global, uint64_t *d_lrow,
-                        uint32_t *d_lcol, uint64_t *d_boff, uint64_t *d_brec,
-                        uint64_t *m_local_local, uint64_t *n_boundary, void *stream);
 
 /* records (remote << 32) | li  ->  (remote << 32) | comp_of[li]. */
 int cc_hip_boundary_set_rep_device(uint64_t *d_records, uint64_t count,

@@ -19,6 +19,58 @@ def graphs():
             "grid": lambda: devgen.grid(200, 250, 0.7, seed=33)}
 
 
+@pytest.mark.parametrize("P,rank", [(1, 0), (4, 0), (4, 1), (7, 6)])
+def test_split_matches_oracle(gpu, P, rank):
+    import ctypes
+    from cc_mpi_amd import _native as nat
+    h = cm.rmat(13, 16, seed=41)
+    b = block_bounds(h.n, P)
+    vs, ve = int(b[rank]), int(b[rank + 1])
+    nl = ve - vs
+    # the owned window as the driver sees it (offsets not rebased)
+    d_rp = cm.DeviceBuffer.from_array(h.row_ptr)
+    d_ci = cm.DeviceBuffer.from_array(h.col_idx)
+    lib = nat.gpu_lib()
+    mll, nb = ctypes.c_uint64(), ctypes.c_uint64()
+    nat.check(lib.cc_hip_split_count_device(d_rp.ptr + 8 * vs, d_ci.ptr, nl, vs, ve, h.n,
+                                            ctypes.byref(mll), ctypes.byref(nb), None), "count")
+    lrow, boff = cm.DeviceBuffer(8 * (nl + 1)), cm.DeviceBuffer(8 * (nl + 1))
+    lcol, rec = cm.DeviceBuffer(max(4, 4 * mll.value)), cm.DeviceBuffer(max(8, 8 * nb.value))
+    nat.check(lib.cc_hip_split_emit_device(d_rp.ptr + 8 * vs, d_ci.ptr, nl, vs, ve, h.n, lrow.ptr,
+                                           boff.ptr, lcol.ptr, rec.ptr, None), "emit")
+    rp = (h.row_ptr[vs:ve + 1] - h.row_ptr[vs]).astype(np.uint64)
+    ci = h.col_idx[int(h.row_ptr[vs]):int(h.row_ptr[ve])].copy()
+    want_lrow = np.zeros(nl + 1, dtype=np.uint64)
+    want_lcol = np.zeros(max(1, ci.size), dtype=np.uint32)
+    braw = oracle.c_u64()
+    L = oracle.lib()
+    m = L.oracle_local_split(nl, oracle._p(rp, oracle.c_u64), oracle._p(ci, oracle.c_u32), vs, ve,
+                             h.n, oracle._p(want_lrow, oracle.c_u64),
+                             oracle._p(want_lcol, oracle.c_u32), oracle.ctypes.byref(braw))
+    assert (mll.value, nb.value) == (m, braw.value)
+    got_lrow = np.empty(nl + 1, dtype=np.uint64)
+    lrow.download(got_lrow)
+    assert np.array_equal(got_lrow, want_lrow)
+    got = np.empty(m, dtype=np.uint32)
+    lcol.download(got)
+    assert np.array_equal(got, want_lcol[:m])
+    # boundary records with li (the rep fill maps li -> comp_of[li] later)
+    ident = np.arange(nl, dtype=np.uint32)
+    want_rec = np.zeros(max(1, ci.size), dtype=np.uint64)
+    k = L.oracle_boundary_build(nl, oracle._p(rp, oracle.c_u64), oracle._p(ci, oracle.c_u32), vs,
+                                ve, h.n, oracle._p(ident, oracle.c_u32),
+                                oracle._p(want_rec, oracle.c_u64))
+    got_rec = np.empty(k, dtype=np.uint64)
+    rec.download(got_rec)
+    assert np.array_equal(got_rec, want_rec[:k])
+    got_boff = np.empty(nl + 1, dtype=np.uint64)
+    boff.download(got_boff)
+    src = np.repeat(np.arange(nl), np.diff(rp).astype(np.int64))
+    bmask = ((ci < vs) | (ci >= ve)) & (ci < h.n)
+    assert np.array_equal(got_boff, np.concatenate([[0], np.cumsum(np.bincount(
+        src[bmask], minlength=nl))]).astype(np.uint64))
+
+
 @pytest.mark.parametrize("name", ["chung_lu", "rmat", "grid"])
 @pytest.mark.parametrize("rank,interval", [(0, 1), (2, 3), (3, 2)])
 def test_rank_share(gpu, name, rank, interval):
