@@ -42,6 +42,8 @@
 
 #include <hip/hip_runtime.h>
 
+#include <string>
+
 #include "ccx_internal.h"
 #include "../../../include/cc_hip.h"
 #include "../../../include/cc.h"
@@ -197,6 +199,138 @@ __global__ __launch_bounds__(kBlock) void k_compress(uint32_t n, uint32_t *__res
   }
 }
 
+// ---- ILP variants: each thread carries ILP vertices through every
+// dependent-load step together (v, v + S, ..., S = grid size), so ILP
+// independent gathers are in flight per step instead of one.  The kernels
+// are latency-bound chains (offset -> neighbour -> parent -> parent's
+// parent), so this multiplies memory-level parallelism without adding
+// waves (more waves would add contention on hot roots).
+
+template <int ILP>
+__device__ __forceinline__ void roots_lockstep(uint32_t (&x)[ILP], const bool (&act)[ILP],
+                                               const uint32_t *__restrict__ p) {
+  bool more = true;
+  while (more) {
+    more = false;
+#pragma unroll
+    for (int k = 0; k < ILP; ++k) {
+      if (!act[k]) continue;
+      const uint32_t px = p[x[k]];
+      if (px != x[k]) {
+        x[k] = px;
+        more = true;
+      }
+    }
+  }
+}
+
+// hook() continued from two known (possibly stale) roots.
+template <int Mode>
+__device__ __forceinline__ void hook_roots(uint32_t ra, uint32_t rb, uint32_t *__restrict__ p) {
+  while (ra != rb) {
+    const uint32_t hi = ra > rb ? ra : rb;
+    const uint32_t lo = ra > rb ? rb : ra;
+    if (Mode != 0) {
+      const uint32_t cur = load_l2(&p[hi]);
+      if (cur != hi) {
+        ra = find_root_l2(cur, p);
+        rb = find_root_l2(lo, p);
+        continue;
+      }
+    }
+    const uint32_t old = atomicMin(&p[hi], lo);
+    if (old == hi || old == lo) return;
+    ra = Mode == 0 ? find_root(old, p) : find_root_l2(old, p);
+    rb = Mode == 0 ? find_root(lo, p) : find_root_l2(lo, p);
+  }
+}
+
+template <int ILP>
+__global__ __launch_bounds__(kBlock) void k_init_min_ilp(const uint64_t *__restrict__ row_ptr,
+                                                         const uint32_t *__restrict__ col,
+                                                         uint32_t n, uint32_t *__restrict__ p) {
+  const uint64_t S = (uint64_t)gridDim.x * kBlock;
+  for (uint64_t v0 = (uint64_t)blockIdx.x * kBlock + threadIdx.x; v0 < n; v0 += S * ILP) {
+    uint64_t b[ILP], e[ILP];
+#pragma unroll
+    for (int k = 0; k < ILP; ++k) {
+      const uint64_t v = v0 + k * S;
+      b[k] = (v < n) ? row_ptr[v] : 0;
+      e[k] = (v < n) ? row_ptr[v + 1] : 0;
+    }
+    uint32_t w[ILP];
+#pragma unroll
+    for (int k = 0; k < ILP; ++k) w[k] = (b[k] < e[k]) ? col[b[k]] : 0xFFFFFFFFu;
+#pragma unroll
+    for (int k = 0; k < ILP; ++k) {
+      const uint64_t v = v0 + k * S;
+      if (v < n) p[v] = w[k] < (uint32_t)v ? w[k] : (uint32_t)v;
+    }
+  }
+}
+
+template <int Mode, int ILP>
+__global__ __launch_bounds__(kBlock) void k_hook_neighbour_ilp(const uint64_t *__restrict__ row_ptr,
+                                                               const uint32_t *__restrict__ col,
+                                                               uint32_t n, uint32_t r,
+                                                               uint32_t *__restrict__ p) {
+  const uint64_t S = (uint64_t)gridDim.x * kBlock;
+  for (uint64_t v0 = (uint64_t)blockIdx.x * kBlock + threadIdx.x; v0 < n; v0 += S * ILP) {
+    uint64_t b[ILP], e[ILP];
+#pragma unroll
+    for (int k = 0; k < ILP; ++k) {
+      const uint64_t v = v0 + k * S;
+      b[k] = (v < n) ? row_ptr[v] + r : 1;
+      e[k] = (v < n) ? row_ptr[v + 1] : 0;
+    }
+    uint32_t ra[ILP], rb[ILP];
+    bool act[ILP];
+#pragma unroll
+    for (int k = 0; k < ILP; ++k) {
+      const uint32_t v = (uint32_t)(v0 + k * S);
+      act[k] = b[k] < e[k];
+      rb[k] = act[k] ? col[b[k]] : 0;
+      // neighbour 0 below v was linked by the init kernel already
+      act[k] = act[k] && rb[k] < n && !(r == 0 && rb[k] < v);
+      ra[k] = v;
+    }
+    // both endpoints' roots walked together: 2*ILP gathers per step
+    uint32_t x[2 * ILP];
+    bool a2[2 * ILP];
+#pragma unroll
+    for (int k = 0; k < ILP; ++k) {
+      x[k] = ra[k];
+      x[ILP + k] = rb[k];
+      a2[k] = a2[ILP + k] = act[k];
+    }
+    roots_lockstep<2 * ILP>(x, a2, p);
+#pragma unroll
+    for (int k = 0; k < ILP; ++k)
+      if (act[k] && x[k] != x[ILP + k]) hook_roots<Mode>(x[k], x[ILP + k], p);
+  }
+}
+
+template <int ILP>
+__global__ __launch_bounds__(kBlock) void k_compress_ilp(uint32_t n, uint32_t *__restrict__ p) {
+  const uint64_t S = (uint64_t)gridDim.x * kBlock;
+  for (uint64_t v0 = (uint64_t)blockIdx.x * kBlock + threadIdx.x; v0 < n; v0 += S * ILP) {
+    uint32_t x[ILP], first[ILP];
+    bool act[ILP];
+#pragma unroll
+    for (int k = 0; k < ILP; ++k) {
+      const uint64_t v = v0 + k * S;
+      act[k] = v < n;
+      x[k] = act[k] ? p[v] : 0;
+      first[k] = x[k];
+      act[k] = act[k] && x[k] != (uint32_t)v;
+    }
+    roots_lockstep<ILP>(x, act, p);
+#pragma unroll
+    for (int k = 0; k < ILP; ++k)
+      if (act[k] && x[k] != first[k]) p[v0 + k * S] = x[k];
+  }
+}
+
 __device__ __forceinline__ uint32_t sample_vertex(uint32_t i, uint32_t n) {
   // SplitMix64 finaliser of the sample index, reduced to [0, n).
   uint64_t z = (uint64_t)(i + 1) * 0x9E3779B97F4A7C15ull;
@@ -323,13 +457,38 @@ __global__ __launch_bounds__(kBlock) void k_hook_frontier(const uint64_t *__rest
   }
 }
 
-inline unsigned blocks_for(uint64_t n, int cu_count) {
+inline unsigned blocks_for(uint64_t n, int cu_count, int per_cu) {
   const uint64_t need = (n + kBlock - 1) / kBlock;
-  const uint64_t cap = (uint64_t)(cu_count > 0 ? cu_count : 256) * (uint64_t)g_blocks_per_cu;
+  const uint64_t cap = (uint64_t)(cu_count > 0 ? cu_count : 256) * (uint64_t)per_cu;
   return (unsigned)(need < cap ? need : cap);
 }
 
+int g_stream_blocks_per_cu = 8;  // grid cap of the contention-free kernels (init, compress)
+int g_ilp = 1;                   // vertices per thread per step in init/hook/compress (1, 2, 4)
+
 }  // namespace
+
+extern "C" int cc_hip_components_set(const char *key, int value) {
+  if (!key) return ccx::set_error(4, "cc_hip_components_set: null key");
+  const std::string k(key);
+  if (k == "sampled_neighbours" && value >= 1 && value <= 4) g_sampled_neighbours = value;
+  else if (k == "blocks_per_cu" && value >= 1 && value <= 64) g_blocks_per_cu = value;
+  else if (k == "stream_blocks_per_cu" && value >= 1 && value <= 64) g_stream_blocks_per_cu = value;
+  else if (k == "hook_mode" && value >= 0 && value <= 3) g_hook_mode = value;
+  else if (k == "ilp" && (value == 1 || value == 2 || value == 4)) g_ilp = value;
+  else return ccx::set_error(4, "cc_hip_components_set: bad key or value (%s=%d)", key, value);
+  return 0;
+}
+
+extern "C" int cc_hip_components_get(const char *key) {
+  const std::string k(key ? key : "");
+  if (k == "sampled_neighbours") return g_sampled_neighbours;
+  if (k == "blocks_per_cu") return g_blocks_per_cu;
+  if (k == "stream_blocks_per_cu") return g_stream_blocks_per_cu;
+  if (k == "hook_mode") return g_hook_mode;
+  if (k == "ilp") return g_ilp;
+  return -1;
+}
 
 extern "C" int cc_hip_components_tune(int sampled_neighbours, int blocks_per_cu, int hook_mode) {
   if (sampled_neighbours < 0 || sampled_neighbours > 4 || blocks_per_cu < 0 || blocks_per_cu > 64 ||
@@ -355,18 +514,36 @@ extern "C" int cc_hip_components_device(const uint64_t *d_row_ptr, const uint32_
   ccx::StreamState *st = ccx::stream_state(s);
   if (!st) return 3;
   uint32_t *giant = st->d_words;
-  const unsigned nb = blocks_for(n, c->cu_count);
+  const unsigned nb = blocks_for(n, c->cu_count, g_blocks_per_cu);
+  const unsigned ns = blocks_for(n, c->cu_count, g_stream_blocks_per_cu);
   const uint32_t rounds = (uint32_t)g_sampled_neighbours;
+  const int ilp = g_ilp;
 
   if (m == 0) {
-    CCX_LAUNCH("cc_init", s, k_init<<<nb, kBlock, 0, s>>>(n, d_labels));
+    CCX_LAUNCH("cc_init", s, k_init<<<ns, kBlock, 0, s>>>(n, d_labels));
     return 0;
   }
-  CCX_LAUNCH("cc_init_min", s, k_init_min<<<nb, kBlock, 0, s>>>(d_row_ptr, d_col_idx, n, d_labels));
+  auto compress = [&]() -> int {
+    if (ilp == 4) CCX_LAUNCH("cc_compress", s, k_compress_ilp<4><<<ns, kBlock, 0, s>>>(n, d_labels));
+    else if (ilp == 2) CCX_LAUNCH("cc_compress", s, k_compress_ilp<2><<<ns, kBlock, 0, s>>>(n, d_labels));
+    else CCX_LAUNCH("cc_compress", s, k_compress<<<ns, kBlock, 0, s>>>(n, d_labels));
+    return 0;
+  };
+  if (ilp == 4)
+    CCX_LAUNCH("cc_init_min", s, k_init_min_ilp<4><<<ns, kBlock, 0, s>>>(d_row_ptr, d_col_idx, n, d_labels));
+  else if (ilp == 2)
+    CCX_LAUNCH("cc_init_min", s, k_init_min_ilp<2><<<ns, kBlock, 0, s>>>(d_row_ptr, d_col_idx, n, d_labels));
+  else
+    CCX_LAUNCH("cc_init_min", s, k_init_min<<<ns, kBlock, 0, s>>>(d_row_ptr, d_col_idx, n, d_labels));
   const int mode = g_hook_mode;
   for (uint32_t r = 0; r < rounds; ++r) {
-    if (r > 0) CCX_LAUNCH("cc_compress", s, k_compress<<<nb, kBlock, 0, s>>>(n, d_labels));
-    if (mode == 0)
+    if (r > 0)
+      if (int rc = compress()) return rc;
+    if (mode == 1 && ilp == 4)
+      CCX_LAUNCH("cc_hook_sample", s, k_hook_neighbour_ilp<1, 4><<<nb, kBlock, 0, s>>>(d_row_ptr, d_col_idx, n, r, d_labels));
+    else if (mode == 1 && ilp == 2)
+      CCX_LAUNCH("cc_hook_sample", s, k_hook_neighbour_ilp<1, 2><<<nb, kBlock, 0, s>>>(d_row_ptr, d_col_idx, n, r, d_labels));
+    else if (mode == 0)
       CCX_LAUNCH("cc_hook_sample", s, k_hook_neighbour<0><<<nb, kBlock, 0, s>>>(d_row_ptr, d_col_idx, n, r, d_labels));
     else if (mode == 1)
       CCX_LAUNCH("cc_hook_sample", s, k_hook_neighbour<1><<<nb, kBlock, 0, s>>>(d_row_ptr, d_col_idx, n, r, d_labels));
@@ -388,8 +565,7 @@ extern "C" int cc_hip_components_device(const uint64_t *d_row_ptr, const uint32_
     CCX_LAUNCH("cc_hook_frontier", s, k_hook_frontier<2><<<nb, kBlock, 0, s>>>(d_row_ptr, d_col_idx, n, first, d_labels, giant));
   else
     CCX_LAUNCH("cc_hook_frontier", s, k_hook_frontier<3><<<nb, kBlock, 0, s>>>(d_row_ptr, d_col_idx, n, first, d_labels, giant));
-  CCX_LAUNCH("cc_compress", s, k_compress<<<nb, kBlock, 0, s>>>(n, d_labels));
-  return 0;
+  return compress();
 }
 
 extern "C" int cc_hip_components_host(const uint64_t *row_ptr, const uint32_t *col_idx, uint32_t n,

@@ -85,6 +85,13 @@ int cc_hip_components_device(const uint64_t *d_row_ptr, const uint32_t *d_col_id
  * sampling with every neighbour swept outside the giant tree).  Results
  * never depend on any of them. */
 int cc_hip_components_tune(int sampled_neighbours, int blocks_per_cu, int hook_mode);
+/* The same knobs by name, plus "stream_blocks_per_cu" (grid cap of the
+ * contention-free init/compress kernels, default 8) and "ilp" (vertices per
+ * thread carried through each dependent-load step: 1, 2 or 4).  Returns
+ * nonzero for an unknown key or out-of-range value; _get returns -1 for an
+ * unknown key. */
+int cc_hip_components_set(const char *key, int value);
+int cc_hip_components_get(const char *key);
 
 /* Same result through host buffers (H2D, solve, D2H; blocking). */
 int cc_hip_components_host(const uint64_t *row_ptr, const uint32_t *col_idx,
