@@ -58,7 +58,7 @@ constexpr int kGiantSamples = 1024;
 // result: sampling only decides how much of the work the cheap
 // per-vertex rounds do before the frontier sweep.
 int g_sampled_neighbours = 2;  // sampled hooking rounds (1..4)
-int g_blocks_per_cu = 4;       // grid cap for the vertex-parallel kernels (measured best)
+int g_blocks_per_cu = 0;       // grid cap of the sampled hook (0 = 8 with hook ILP 4, else 4)
 int g_hook_mode = 1;           // 0: plain loads, 1: L2-confirmed roots, 2: all loads from L2
 
 // Parent read that bypasses the CU's L1 (an agent-scope atomic load), so it
@@ -463,8 +463,18 @@ inline unsigned blocks_for(uint64_t n, int cu_count, int per_cu) {
   return (unsigned)(need < cap ? need : cap);
 }
 
-int g_stream_blocks_per_cu = 8;  // grid cap of the contention-free kernels (init, compress)
-int g_ilp = 1;                   // vertices per thread per step in init/hook/compress (1, 2, 4)
+// Measured on R-MAT 22, Chung-Lu 65.6M, ER 4M (degree 8) and a 2048^2
+// grid: the frontier sweep wants 8 blocks per CU everywhere; the sampled
+// hook gains from walking 4 roots per thread (with 8 blocks per CU) only on
+// dense power-law graphs, where hub rows keep many chains short, and loses
+// on sparse ones; init/compress are fastest one vertex per thread.
+int g_stream_blocks_per_cu = 8;    // grid cap of the contention-free kernels (init, compress)
+int g_frontier_blocks_per_cu = 8;  // grid cap of the frontier sweep
+int g_hook_ilp = 0;                // roots walked per thread in the sampled hook (0 = by degree)
+int g_stream_ilp = 1;              // vertices per thread per step in init/compress (1, 2, 4)
+constexpr uint64_t kIlpMinDegree = 16;  // auto: hook ILP 4 when m >= 16 n
+
+inline bool ilp_ok(int v) { return v == 1 || v == 2 || v == 4; }
 
 }  // namespace
 
@@ -472,10 +482,12 @@ extern "C" int cc_hip_components_set(const char *key, int value) {
   if (!key) return ccx::set_error(4, "cc_hip_components_set: null key");
   const std::string k(key);
   if (k == "sampled_neighbours" && value >= 1 && value <= 4) g_sampled_neighbours = value;
-  else if (k == "blocks_per_cu" && value >= 1 && value <= 64) g_blocks_per_cu = value;
+  else if (k == "blocks_per_cu" && value >= 0 && value <= 64) g_blocks_per_cu = value;
   else if (k == "stream_blocks_per_cu" && value >= 1 && value <= 64) g_stream_blocks_per_cu = value;
+  else if (k == "frontier_blocks_per_cu" && value >= 1 && value <= 64) g_frontier_blocks_per_cu = value;
   else if (k == "hook_mode" && value >= 0 && value <= 3) g_hook_mode = value;
-  else if (k == "ilp" && (value == 1 || value == 2 || value == 4)) g_ilp = value;
+  else if (k == "hook_ilp" && (value == 0 || ilp_ok(value))) g_hook_ilp = value;
+  else if (k == "stream_ilp" && ilp_ok(value)) g_stream_ilp = value;
   else return ccx::set_error(4, "cc_hip_components_set: bad key or value (%s=%d)", key, value);
   return 0;
 }
@@ -485,8 +497,10 @@ extern "C" int cc_hip_components_get(const char *key) {
   if (k == "sampled_neighbours") return g_sampled_neighbours;
   if (k == "blocks_per_cu") return g_blocks_per_cu;
   if (k == "stream_blocks_per_cu") return g_stream_blocks_per_cu;
+  if (k == "frontier_blocks_per_cu") return g_frontier_blocks_per_cu;
   if (k == "hook_mode") return g_hook_mode;
-  if (k == "ilp") return g_ilp;
+  if (k == "hook_ilp") return g_hook_ilp;
+  if (k == "stream_ilp") return g_stream_ilp;
   return -1;
 }
 
@@ -514,10 +528,13 @@ extern "C" int cc_hip_components_device(const uint64_t *d_row_ptr, const uint32_
   ccx::StreamState *st = ccx::stream_state(s);
   if (!st) return 3;
   uint32_t *giant = st->d_words;
-  const unsigned nb = blocks_for(n, c->cu_count, g_blocks_per_cu);
+  const int hook_ilp = g_hook_ilp ? g_hook_ilp : (m >= kIlpMinDegree * (uint64_t)n ? 4 : 1);
+  const int hook_bpc = g_blocks_per_cu ? g_blocks_per_cu : (hook_ilp == 4 ? 8 : 4);
+  const unsigned nb = blocks_for(n, c->cu_count, hook_bpc);
+  const unsigned nf = blocks_for(n, c->cu_count, g_frontier_blocks_per_cu);
   const unsigned ns = blocks_for(n, c->cu_count, g_stream_blocks_per_cu);
   const uint32_t rounds = (uint32_t)g_sampled_neighbours;
-  const int ilp = g_ilp;
+  const int ilp = g_stream_ilp;
 
   if (m == 0) {
     CCX_LAUNCH("cc_init", s, k_init<<<ns, kBlock, 0, s>>>(n, d_labels));
@@ -539,9 +556,9 @@ extern "C" int cc_hip_components_device(const uint64_t *d_row_ptr, const uint32_
   for (uint32_t r = 0; r < rounds; ++r) {
     if (r > 0)
       if (int rc = compress()) return rc;
-    if (mode == 1 && ilp == 4)
+    if (mode == 1 && hook_ilp == 4)
       CCX_LAUNCH("cc_hook_sample", s, k_hook_neighbour_ilp<1, 4><<<nb, kBlock, 0, s>>>(d_row_ptr, d_col_idx, n, r, d_labels));
-    else if (mode == 1 && ilp == 2)
+    else if (mode == 1 && hook_ilp == 2)
       CCX_LAUNCH("cc_hook_sample", s, k_hook_neighbour_ilp<1, 2><<<nb, kBlock, 0, s>>>(d_row_ptr, d_col_idx, n, r, d_labels));
     else if (mode == 0)
       CCX_LAUNCH("cc_hook_sample", s, k_hook_neighbour<0><<<nb, kBlock, 0, s>>>(d_row_ptr, d_col_idx, n, r, d_labels));
@@ -558,13 +575,13 @@ extern "C" int cc_hip_components_device(const uint64_t *d_row_ptr, const uint32_
   // for sure, so the sweep covers every neighbour.
   const uint32_t first = (mode == 3) ? 0u : rounds;
   if (mode == 0)
-    CCX_LAUNCH("cc_hook_frontier", s, k_hook_frontier<0><<<nb, kBlock, 0, s>>>(d_row_ptr, d_col_idx, n, first, d_labels, giant));
+    CCX_LAUNCH("cc_hook_frontier", s, k_hook_frontier<0><<<nf, kBlock, 0, s>>>(d_row_ptr, d_col_idx, n, first, d_labels, giant));
   else if (mode == 1)
-    CCX_LAUNCH("cc_hook_frontier", s, k_hook_frontier<1><<<nb, kBlock, 0, s>>>(d_row_ptr, d_col_idx, n, first, d_labels, giant));
+    CCX_LAUNCH("cc_hook_frontier", s, k_hook_frontier<1><<<nf, kBlock, 0, s>>>(d_row_ptr, d_col_idx, n, first, d_labels, giant));
   else if (mode == 2)
-    CCX_LAUNCH("cc_hook_frontier", s, k_hook_frontier<2><<<nb, kBlock, 0, s>>>(d_row_ptr, d_col_idx, n, first, d_labels, giant));
+    CCX_LAUNCH("cc_hook_frontier", s, k_hook_frontier<2><<<nf, kBlock, 0, s>>>(d_row_ptr, d_col_idx, n, first, d_labels, giant));
   else
-    CCX_LAUNCH("cc_hook_frontier", s, k_hook_frontier<3><<<nb, kBlock, 0, s>>>(d_row_ptr, d_col_idx, n, first, d_labels, giant));
+    CCX_LAUNCH("cc_hook_frontier", s, k_hook_frontier<3><<<nf, kBlock, 0, s>>>(d_row_ptr, d_col_idx, n, first, d_labels, giant));
   return compress();
 }
 

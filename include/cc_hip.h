@@ -79,17 +79,26 @@ int cc_hip_components_device(const uint64_t *d_row_ptr, const uint32_t *d_col_id
 
 /* Process-wide tuning of the solve: the number of sampled hooking rounds
  * (1..4, default 2; 0 keeps the current value), the grid cap of the
- * vertex-parallel kernels in workgroups per CU (1..64, default 4; 0 keeps),
- * and the hooking mode (-1 keeps; 0 plain loads, 1 roots confirmed with an
- * L2 read before each atomic (default), 2 all parent reads from L2, 3 lossy
+ * sampled-hook kernel in workgroups per CU (1..64; 0 keeps), and the
+ * hooking mode (-1 keeps; 0 plain loads, 1 roots confirmed with an L2 read
+ * before each atomic (default), 2 all parent reads from L2, 3 lossy
  * sampling with every neighbour swept outside the giant tree).  Results
  * never depend on any of them. */
 int cc_hip_components_tune(int sampled_neighbours, int blocks_per_cu, int hook_mode);
-/* The same knobs by name, plus "stream_blocks_per_cu" (grid cap of the
- * contention-free init/compress kernels, default 8) and "ilp" (vertices per
- * thread carried through each dependent-load step: 1, 2 or 4).  Returns
- * nonzero for an unknown key or out-of-range value; _get returns -1 for an
- * unknown key. */
+/* The same knobs by name:
+ *   "sampled_neighbours"      1..4 (default 2)
+ *   "hook_mode"               0..3 (default 1)
+ *   "blocks_per_cu"           sampled-hook grid cap; 0 (default) = 8 when the
+ *                             hook walks 4 roots per thread, else 4
+ *   "hook_ilp"                roots walked per thread in lockstep in the
+ *                             sampled hook (mode 1): 1, 2, 4, or 0 (default)
+ *                             = 4 when m >= 16 n, else 1
+ *   "frontier_blocks_per_cu"  grid cap of the frontier sweep (default 8)
+ *   "stream_blocks_per_cu"    grid cap of init/compress (default 8)
+ *   "stream_ilp"              vertices per thread per step in init/compress
+ *                             (1 default, 2, 4)
+ * Returns nonzero for an unknown key or out-of-range value; _get returns -1
+ * for an unknown key. */
 int cc_hip_components_set(const char *key, int value);
 int cc_hip_components_get(const char *key);
 
