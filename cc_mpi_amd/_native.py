@@ -139,6 +139,11 @@ def host_lib() -> ctypes.CDLL:
          [c_u32, c_u32, c_double, c_u64, c_int, ctypes.POINTER(CSRGraphC)])
     _sig(lib, "ccgen_chung_lu", c_int,
          [c_u32, c_u64, c_double, c_u64, c_int, ctypes.POINTER(CSRGraphC)])
+    _sig(lib, "load_csr_from_mat", c_int, [ctypes.c_char_p, ctypes.POINTER(CSRGraphC)])
+    _sig(lib, "load_csr_from_file", c_int,
+         [ctypes.c_char_p, c_int, c_int, ctypes.POINTER(CSRGraphC)])
+    _sig(lib, "cc_mat_write_sparse", c_int,
+         [ctypes.c_char_p, ctypes.c_char_p, ctypes.POINTER(CSRGraphC), c_int])
     _host = lib
     return lib
 

@@ -1,8 +1,9 @@
 /*
  * cc_mpi_run: MPI driver program for the distributed CC path.
  *
- * Rank 0 generates a seeded graph (libcchost), distributes it with the
- * reference's partitioning (dist_csr_from_global), every rank calls
+ * Rank 0 generates a seeded graph (libcchost) or reads a level 5 MAT file,
+ * distributes it with the reference's partitioning (dist_csr_from_global /
+ * load_dist_csr_from_file, as src/mpi_cc_benchmark.c:178), every rank calls
  * compute_connected_components_mpi_advanced, and rank 0 writes the gathered
  * labels (uint32 per vertex, little-endian) to --out.  Plays the role of the
  * reference's bench entry (src/mpi_cc_benchmark.c:31-251) for generated
@@ -11,9 +12,10 @@
  *   cc_mpi_run --gen rmat  --scale S [--ef F] [--seed X] [options]
  *   cc_mpi_run --gen er    --n N --deg D [--seed X] [options]
  *   cc_mpi_run --gen grid  --rows R --cols C [--keep P] [--seed X] [options]
+ *   cc_mpi_run --input GRAPH.mat [options]
  *   options: -e E (exchange interval, default 1), -c C (chunk, default 2048),
  *            --vertex-blocks (default: edge-balanced, as the reference bench),
- *            --out FILE, -r RUNS
+ *            --out FILE, -r RUNS, --write-mat FILE (save the generated graph)
  */
 #define _POSIX_C_SOURCE 200112L
 #include <stdint.h>
@@ -42,21 +44,13 @@ static int has_flag(int argc, char **argv, const char *name)
   return 0;
 }
 
-int main(int argc, char **argv)
+/* Rank 0 generates the graph and distributes it; 0 on success on every rank. */
+static int generate(int argc, char **argv, const char *gen, uint64_t seed, int use_edge,
+                    const char *write_mat, DistCSRGraph *Gd, MPI_Comm comm)
 {
-  MPI_Init(&argc, &argv);
   int rank, size;
-  MPI_Comm comm = MPI_COMM_WORLD;
   MPI_Comm_rank(comm, &rank);
   MPI_Comm_size(comm, &size);
-  const char *gen = arg_str(argc, argv, "--gen", "rmat");
-  const uint64_t seed = strtoull(arg_str(argc, argv, "--seed", "1002"), NULL, 10);
-  const int exch = atoi(arg_str(argc, argv, "-e", "1"));
-  const int chunk = atoi(arg_str(argc, argv, "-c", "2048"));
-  const int runs = atoi(arg_str(argc, argv, "-r", "1"));
-  const char *out = arg_str(argc, argv, "--out", NULL);
-  const int use_edge = !has_flag(argc, argv, "--vertex-blocks");
-
   CSRGraph full;
   memset(&full, 0, sizeof(full));
   int rc = 0;
@@ -82,20 +76,67 @@ int main(int argc, char **argv)
     else
       printf("graph %s: n=%u m=%llu ranks=%d\n", gen, full.n, (unsigned long long)full.m, size);
   }
+  if (rank == 0 && rc == 0 && write_mat)
+  {
+    rc = cc_mat_write_sparse(write_mat, gen, &full, 0);
+    if (rc)
+      fprintf(stderr, "cc_mpi_run: cannot write %s (%d)\n", write_mat, rc);
+  }
   MPI_Bcast(&rc, 1, MPI_INT, 0, comm);
   if (rc)
   {
-    MPI_Finalize();
+    if (rank == 0)
+      free_csr(&full);
     return 1;
   }
-  DistCSRGraph Gd;
-  if (dist_csr_from_global(&full, use_edge, &Gd, comm) != 0)
+  if (dist_csr_from_global(&full, use_edge, Gd, comm) != 0)
   {
     fprintf(stderr, "cc_mpi_run: distribution failed\n");
     MPI_Abort(comm, EXIT_FAILURE);
   }
   if (rank == 0)
     free_csr(&full);
+  return 0;
+}
+
+int main(int argc, char **argv)
+{
+  MPI_Init(&argc, &argv);
+  int rank, size;
+  MPI_Comm comm = MPI_COMM_WORLD;
+  MPI_Comm_rank(comm, &rank);
+  MPI_Comm_size(comm, &size);
+  const char *gen = arg_str(argc, argv, "--gen", "rmat");
+  const uint64_t seed = strtoull(arg_str(argc, argv, "--seed", "1002"), NULL, 10);
+  const int exch = atoi(arg_str(argc, argv, "-e", "1"));
+  const int chunk = atoi(arg_str(argc, argv, "-c", "2048"));
+  const int runs = atoi(arg_str(argc, argv, "-r", "1"));
+  const char *out = arg_str(argc, argv, "--out", NULL);
+  const int use_edge = !has_flag(argc, argv, "--vertex-blocks");
+
+  const char *input = arg_str(argc, argv, "--input", NULL);
+  const char *write_mat = arg_str(argc, argv, "--write-mat", NULL);
+
+  DistCSRGraph Gd;
+  if (input)
+  {
+    const int lrc = load_dist_csr_from_file(input, 1, 1, use_edge, &Gd, comm);
+    if (lrc != 0)
+    {
+      if (rank == 0)
+        fprintf(stderr, "cc_mpi_run: cannot load %s (%d)\n", input, lrc);
+      MPI_Finalize();
+      return 1;
+    }
+    if (rank == 0)
+      printf("graph %s: n=%u m=%llu ranks=%d\n", input, Gd.n_global,
+             (unsigned long long)Gd.m_global, size);
+  }
+  else if (generate(argc, argv, gen, seed, use_edge, write_mat, &Gd, comm) != 0)
+  {
+    MPI_Finalize();
+    return 1;
+  }
   uint32_t *labels = (uint32_t *)malloc((size_t)(Gd.n_global ? Gd.n_global : 1) * 4);
   if (!labels)
     MPI_Abort(comm, EXIT_FAILURE);

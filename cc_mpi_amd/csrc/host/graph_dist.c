@@ -10,6 +10,7 @@
 #include <stdio.h>
 #include <stdlib.h>
 #include <string.h>
+#include <strings.h>
 
 #include <mpi.h>
 
@@ -208,4 +209,30 @@ int dist_csr_from_global(const CSRGraph *full, int use_edge, DistCSRGraph *out, 
     }
   }
   return 0;
+}
+
+int load_dist_csr_from_file(const char *path, int symmetrize, int drop_self_loops, int use_edge,
+                            DistCSRGraph *out, MPI_Comm comm)
+{
+  const char *ext = strrchr(path, '.');
+  if (!ext || strcasecmp(ext, ".mat") != 0)
+    return -1;
+  int rank;
+  MPI_Comm_rank(comm, &rank);
+  CSRGraph full;
+  memset(&full, 0, sizeof(full));
+  int rc = 0;
+  if (rank == 0)
+    rc = load_csr_from_file(path, symmetrize, drop_self_loops, &full);
+  MPI_Bcast(&rc, 1, MPI_INT, 0, comm);
+  if (rc != 0)
+  {
+    if (rank == 0)
+      free_csr(&full);
+    return rc;
+  }
+  rc = dist_csr_from_global(&full, use_edge, out, comm);
+  if (rank == 0)
+    free_csr(&full);
+  return rc;
 }

@@ -104,3 +104,37 @@ def chung_lu(n: int, pairs: int, gamma: float = 2.1, seed: int = 1003,
     """Chung-Lu power-law graph with `pairs` sampled endpoint pairs."""
     return _gen(nat.host_lib().ccgen_chung_lu, n, pairs, gamma, seed, int(permute),
                 name=f"chunglu-{n}-g{gamma:g}")
+
+
+MAT_COMPRESS = 1   # CC_MAT_COMPRESS
+MAT_PROBLEM = 2    # CC_MAT_PROBLEM
+MAT_BIG_ENDIAN = 4  # CC_MAT_BIG_ENDIAN
+
+
+class MatFileError(RuntimeError):
+    """load_csr_from_file / load_csr_from_mat returned nonzero (``code``)."""
+
+    def __init__(self, path: str, code: int):
+        super().__init__(f"loading {path} failed (rc={code})")
+        self.code = code
+
+
+def load(path: str) -> Graph:
+    """Read a graph the way the reference's loader does (load_csr_from_file,
+    src/graph.c:14-128): ``.mat`` files only; rows of the stored sparse
+    matrix become vertices."""
+    g = nat.CSRGraphC()
+    rc = nat.host_lib().load_csr_from_file(str(path).encode(), 1, 1, ctypes.byref(g))
+    if rc != 0:
+        raise MatFileError(str(path), rc)
+    return _take(g, str(path))
+
+
+def save_mat(path: str, g: Graph, name: str = "A", flags: int = 0) -> None:
+    """Write ``g`` as a sparse logical matrix to a level 5 MAT file
+    (cc_mat_write_sparse); ``flags`` is a mix of MAT_COMPRESS, MAT_PROBLEM
+    and MAT_BIG_ENDIAN."""
+    rc = nat.host_lib().cc_mat_write_sparse(str(path).encode(), name.encode(),
+                                            ctypes.byref(g.as_c()), int(flags))
+    if rc != 0:
+        raise MatFileError(str(path), rc)

@@ -42,6 +42,14 @@ typedef struct
  */
 int dist_csr_from_global(const CSRGraph *full, int use_edge, DistCSRGraph *out, MPI_Comm comm);
 
+/* Replaces include/graph_dist.h:41-46 (src/graph_dist.c:344-357): a ".mat"
+ * path is read on rank 0 by load_csr_from_file, the return code is
+ * broadcast, and on success the graph is distributed as by
+ * dist_csr_from_global; any other extension returns -1.  symmetrize and
+ * drop_self_loops are ignored, as there.  Collective on comm. */
+int load_dist_csr_from_file(const char *path, int symmetrize, int drop_self_loops, int use_edge,
+                            DistCSRGraph *out, MPI_Comm comm);
+
 /* Free the local arrays (include/graph_dist.h:49, src/graph_dist.c:360-378). */
 void free_dist_csr(DistCSRGraph *graph);
 
