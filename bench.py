@@ -95,25 +95,41 @@ def frontier_stats(G):
     front = comp != giant
     rem = np.maximum(deg - 2, 0)
     active = front & (rem > 0)
+    has0 = np.flatnonzero(deg > 0)
+    up0 = int(np.count_nonzero(G.col_idx[G.row_ptr[has0].astype(np.int64)] > has0))
     return dict(frontier_vertices=int(active.sum()), frontier_edges=int(rem[active].sum()),
                 giant_fraction=float((~front).mean()),
-                deg_gt0=int((deg > 0).sum()), deg_gt1=int((deg > 1).sum()))
+                deg_gt0=int((deg > 0).sum()), deg_gt1=int((deg > 1).sum()), up0=up0)
 
 
-def kernel_bytes(name: str, n: int, st: dict, launches_per_solve: int) -> float | None:
+def kernel_bytes(name: str, n: int, st: dict, plan: dict) -> float | None:
     """Algorithmic HBM bytes of one launch of kernel `name` (model stated in
     the design notes): each array element the algorithm must touch, counted
-    once, 4 B per id, 8 B per offset."""
+    once, 4 B per id, 8 B per offset.  `plan` is the solve's configuration
+    (cc_hip_components_plan), with two sampled rounds.  d0/d1 = vertices
+    with at least 1/2 neighbours; up0 = vertices whose neighbour 0 is above
+    them (the only ones round 0 hooks; the others were linked by init)."""
+    d0, d1, up0 = st["deg_gt0"], st["deg_gt1"], st["up0"]
     if name == "cc_init":
         return 4.0 * n
     if name == "cc_init_min":
+        if plan["cache"]:
+            # row_ptr, the first two col entries of each row, own parent
+            # write, both sample arrays written
+            return 8.0 * (n + 1) + 4.0 * (d0 + d1) + 4.0 * n + 8.0 * n
         # row_ptr, first col entry of each non-isolated vertex, own parent write
-        return 8.0 * (n + 1) + 4.0 * st["deg_gt0"] + 4.0 * n
+        return 8.0 * (n + 1) + 4.0 * d0 + 4.0 * n
     if name == "cc_hook_sample":
-        # two launches (r=0,1): row_ptr, own parent, one col entry + one parent gather
-        # per vertex that has neighbour r; averaged over the two launches
-        a = (st["deg_gt0"] + st["deg_gt1"]) / 2.0
-        return 8.0 * (n + 1) + 4.0 * n + 8.0 * a
+        if plan["fused"]:
+            # both sample arrays, own parent, one parent gather per hooked neighbour
+            return 8.0 * n + 4.0 * n + 4.0 * (up0 + d1)
+        if plan["cache"]:
+            # per round: one sample array, own parent, one parent gather per
+            # hooked neighbour; averaged over the two launches
+            return 4.0 * n + 4.0 * n + 2.0 * (up0 + d1)
+        # per round: row_ptr, own parent, the col entry of each vertex that has
+        # neighbour r, one parent gather per hooked neighbour; averaged
+        return 8.0 * (n + 1) + 4.0 * n + 2.0 * (d0 + d1) + 2.0 * (up0 + d1)
     if name == "cc_compress":
         return 12.0 * n
     if name == "cc_hook_frontier":
@@ -122,6 +138,16 @@ def kernel_bytes(name: str, n: int, st: dict, launches_per_solve: int) -> float 
         # parent gather per remaining neighbour
         return 12.0 * n + 16.0 * st["frontier_vertices"] + 8.0 * st["frontier_edges"]
     return None
+
+
+def solve_plan(lib, n: int, m: int) -> dict:
+    """What the solve does on this graph (cc_hip_components_plan)."""
+    import ctypes
+    from cc_mpi_amd import _native as nat
+    vals = [ctypes.c_int() for _ in range(4)]
+    nat.check(lib.cc_hip_components_plan(n, m, *[ctypes.byref(v) for v in vals]), "plan")
+    return dict(zip(("cache", "fused", "hook_ilp", "hook_blocks_per_cu"),
+                    (int(v.value) for v in vals)))
 
 
 def cpu_baseline(G, budget_s: float, runs: int = 3, impl: str = "reference"):
@@ -379,9 +405,11 @@ def main() -> int:
 
     st = frontier_stats(G)
     log(f"frontier: {st}")
+    plan = solve_plan(lib, G.n, G.m)
+    log(f"solve plan: {plan}")
     roofline = None
     if dominant:
-        b = kernel_bytes(dominant, G.n, st, per[dominant]["launches"] // max(1, args.steps))
+        b = kernel_bytes(dominant, G.n, st, plan)
         avg_s = per[dominant]["avg_us"] / 1e6
         traffic = None
         tpath = os.path.join(ROOT, "profiles", f"traffic_{args.workload}_{dominant}.json")
@@ -430,6 +458,7 @@ def main() -> int:
         "kernels": {k: {"launches_per_solve": v["launches"] / args.steps,
                         "avg_us": round(v["avg_us"], 2)} for k, v in per.items()},
         "frontier": st,
+        "solve_plan": plan,
         "parity_bit_exact": parity,
     }
     if args.serving_requests > 0:

@@ -331,6 +331,94 @@ __global__ __launch_bounds__(kBlock) void k_compress_ilp(uint32_t n, uint32_t *_
   }
 }
 
+// ---- Sampled-neighbour cache.  Each sampled round gathers one column
+// entry per vertex: a random 4-byte read that moves a whole cache line from
+// HBM.  The init pass reads the first `Rounds` entries of every row once
+// (they share that line) into one coalesced array per round, so the rounds
+// stream 4 bytes per vertex instead of gathering a line plus an offset.
+constexpr uint32_t kNoNeighbour = 0xFFFFFFFFu;
+
+template <int Rounds>
+__global__ __launch_bounds__(kBlock) void k_init_sample(const uint64_t *__restrict__ row_ptr,
+                                                        const uint32_t *__restrict__ col,
+                                                        uint32_t n, uint32_t *__restrict__ p,
+                                                        uint32_t *__restrict__ s) {
+  CCX_GRID_STRIDE(v, n) {
+    const uint64_t b = row_ptr[v];
+    const uint64_t e = row_ptr[v + 1];
+    uint32_t w[Rounds];
+#pragma unroll
+    for (int r = 0; r < Rounds; ++r) w[r] = (b + r < e) ? col[b + r] : kNoNeighbour;
+    p[v] = w[0] < v ? w[0] : v;
+#pragma unroll
+    for (int r = 0; r < Rounds; ++r) s[(size_t)r * n + v] = w[r];
+  }
+}
+
+// Round r of the sampled hook from the cache (s = round r's array).
+template <int Mode, int ILP>
+__global__ __launch_bounds__(kBlock) void k_hook_cached(const uint32_t *__restrict__ s,
+                                                        uint32_t n, uint32_t r,
+                                                        uint32_t *__restrict__ p) {
+  const uint64_t S = (uint64_t)gridDim.x * kBlock;
+  for (uint64_t v0 = (uint64_t)blockIdx.x * kBlock + threadIdx.x; v0 < n; v0 += S * ILP) {
+    uint32_t x[2 * ILP];
+    bool act[2 * ILP];
+#pragma unroll
+    for (int k = 0; k < ILP; ++k) {
+      const uint64_t v = v0 + k * S;
+      const uint32_t w = v < n ? s[v] : kNoNeighbour;
+      // neighbour 0 below v was linked by the init kernel already
+      const bool a = w < n && !(r == 0 && w < (uint32_t)v);
+      x[k] = (uint32_t)v;
+      x[ILP + k] = w;
+      act[k] = act[ILP + k] = a;
+    }
+    roots_lockstep<2 * ILP>(x, act, p);
+#pragma unroll
+    for (int k = 0; k < ILP; ++k)
+      if (act[k] && x[k] != x[ILP + k]) hook_roots<Mode>(x[k], x[ILP + k], p);
+  }
+}
+
+// Both sampled rounds of a two-round solve in one pass: v's root is walked
+// once and hooked with the roots of neighbours 0 and 1 in turn, so the
+// pass between the rounds (and its pointer-jumping launch) disappears.
+template <int Mode, int ILP>
+__global__ __launch_bounds__(kBlock) void k_hook_pair(const uint32_t *__restrict__ s0,
+                                                      const uint32_t *__restrict__ s1,
+                                                      uint32_t n, uint32_t *__restrict__ p) {
+  const uint64_t S = (uint64_t)gridDim.x * kBlock;
+  for (uint64_t v0 = (uint64_t)blockIdx.x * kBlock + threadIdx.x; v0 < n; v0 += S * ILP) {
+    uint32_t x[3 * ILP];
+    bool act[3 * ILP];
+#pragma unroll
+    for (int k = 0; k < ILP; ++k) {
+      const uint64_t v = v0 + k * S;
+      const uint32_t w0 = v < n ? s0[v] : kNoNeighbour;
+      const uint32_t w1 = v < n ? s1[v] : kNoNeighbour;
+      // neighbour 0 below v was linked by the init kernel already
+      act[ILP + k] = w0 < n && w0 > (uint32_t)v;
+      act[2 * ILP + k] = w1 < n && w1 != (uint32_t)v;
+      act[k] = act[ILP + k] || act[2 * ILP + k];
+      x[k] = (uint32_t)v;
+      x[ILP + k] = w0;
+      x[2 * ILP + k] = w1;
+    }
+    roots_lockstep<3 * ILP>(x, act, p);
+#pragma unroll
+    for (int k = 0; k < ILP; ++k) {
+      uint32_t rv = x[k];
+      const uint32_t r0 = x[ILP + k], r1 = x[2 * ILP + k];
+      if (act[ILP + k] && rv != r0) {
+        hook_roots<Mode>(rv, r0, p);
+        rv = rv < r0 ? rv : r0;  // a member of the merged tree
+      }
+      if (act[2 * ILP + k] && rv != r1) hook_roots<Mode>(rv, r1, p);
+    }
+  }
+}
+
 __device__ __forceinline__ uint32_t sample_vertex(uint32_t i, uint32_t n) {
   // SplitMix64 finaliser of the sample index, reduced to [0, n).
   uint64_t z = (uint64_t)(i + 1) * 0x9E3779B97F4A7C15ull;
@@ -463,16 +551,46 @@ inline unsigned blocks_for(uint64_t n, int cu_count, int per_cu) {
   return (unsigned)(need < cap ? need : cap);
 }
 
-// Measured on R-MAT 22, Chung-Lu 65.6M, ER 4M (degree 8) and a 2048^2
-// grid: the frontier sweep wants 8 blocks per CU everywhere; the sampled
-// hook gains from walking 4 roots per thread (with 8 blocks per CU) only on
-// dense power-law graphs, where hub rows keep many chains short, and loses
-// on sparse ones; init/compress are fastest one vertex per thread.
+// Defaults measured on R-MAT scales 20-25, Chung-Lu 8.2M-65.6M vertices,
+// ER 4M/32M (degree 8) and 2048^2 / 8192^2 grids (70% of edges kept):
+//  * the frontier sweep and init/compress want 8 blocks per CU, one vertex
+//    per thread;
+//  * reading the sampled neighbours once into coalesced arrays (the cache)
+//    never loses much and wins up to 1.6x on the sampled rounds of large
+//    graphs, where each column gather is a cache line from HBM;
+//  * hooking both sampled neighbours in one pass wins everywhere except on
+//    dense graphs beyond about 2^30 CSR entries (Chung-Lu 32.8M and 65.6M),
+//    where the longer chains it leaves for the frontier sweep cost more than
+//    the pass it saves;
+//  * walking several roots per thread pays on dense graphs (m >= 16 n: 2 in
+//    the one-pass hook, 4 per round otherwise, with 8 blocks per CU) and
+//    loses on sparse ones (1, with 4 blocks per CU).
 int g_stream_blocks_per_cu = 8;    // grid cap of the contention-free kernels (init, compress)
 int g_frontier_blocks_per_cu = 8;  // grid cap of the frontier sweep
-int g_hook_ilp = 0;                // roots walked per thread in the sampled hook (0 = by degree)
+int g_hook_ilp = 0;                // roots walked per thread in the sampled hook (0 = auto)
 int g_stream_ilp = 1;              // vertices per thread per step in init/compress (1, 2, 4)
-constexpr uint64_t kIlpMinDegree = 16;  // auto: hook ILP 4 when m >= 16 n
+int g_sample_cache = 1;            // sampled neighbours read once into coalesced arrays
+int g_fused_pair = -1;             // both sampled rounds in one pass (-1 = auto)
+constexpr uint64_t kDenseDegree = 16;            // dense: m >= 16 n
+constexpr uint64_t kFusedMaxDenseEntries = 1ull << 30;
+
+struct Plan {
+  int cache;     // sampled-neighbour cache in use
+  int fused;     // one-pass hook of both sampled neighbours
+  int hook_ilp;  // roots per thread in the sampled hook
+  int hook_bpc;  // sampled-hook grid cap, workgroups per CU
+};
+
+inline Plan plan_for(uint32_t n, uint64_t m) {
+  Plan pl;
+  const bool dense = m >= kDenseDegree * (uint64_t)n;
+  pl.cache = g_sample_cache && (g_hook_mode == 0 || g_hook_mode == 1);
+  const int fused_auto = !dense || m < kFusedMaxDenseEntries;
+  pl.fused = pl.cache && g_sampled_neighbours == 2 && (g_fused_pair < 0 ? fused_auto : g_fused_pair);
+  pl.hook_ilp = g_hook_ilp ? g_hook_ilp : (!dense ? 1 : (pl.fused ? 2 : 4));
+  pl.hook_bpc = g_blocks_per_cu ? g_blocks_per_cu : (pl.hook_ilp >= 2 ? 8 : 4);
+  return pl;
+}
 
 inline bool ilp_ok(int v) { return v == 1 || v == 2 || v == 4; }
 
@@ -488,6 +606,8 @@ extern "C" int cc_hip_components_set(const char *key, int value) {
   else if (k == "hook_mode" && value >= 0 && value <= 3) g_hook_mode = value;
   else if (k == "hook_ilp" && (value == 0 || ilp_ok(value))) g_hook_ilp = value;
   else if (k == "stream_ilp" && ilp_ok(value)) g_stream_ilp = value;
+  else if (k == "sample_cache" && (value == 0 || value == 1)) g_sample_cache = value;
+  else if (k == "fused_pair" && value >= -1 && value <= 1) g_fused_pair = value;
   else return ccx::set_error(4, "cc_hip_components_set: bad key or value (%s=%d)", key, value);
   return 0;
 }
@@ -501,7 +621,19 @@ extern "C" int cc_hip_components_get(const char *key) {
   if (k == "hook_mode") return g_hook_mode;
   if (k == "hook_ilp") return g_hook_ilp;
   if (k == "stream_ilp") return g_stream_ilp;
+  if (k == "sample_cache") return g_sample_cache;
+  if (k == "fused_pair") return g_fused_pair;
   return -1;
+}
+
+extern "C" int cc_hip_components_plan(uint32_t n, uint64_t m, int *cache, int *fused,
+                                      int *hook_ilp, int *hook_blocks_per_cu) {
+  const Plan pl = plan_for(n, m);
+  if (cache) *cache = pl.cache;
+  if (fused) *fused = pl.fused;
+  if (hook_ilp) *hook_ilp = pl.hook_ilp;
+  if (hook_blocks_per_cu) *hook_blocks_per_cu = pl.hook_bpc;
+  return 0;
 }
 
 extern "C" int cc_hip_components_tune(int sampled_neighbours, int blocks_per_cu, int hook_mode) {
@@ -528,9 +660,9 @@ extern "C" int cc_hip_components_device(const uint64_t *d_row_ptr, const uint32_
   ccx::StreamState *st = ccx::stream_state(s);
   if (!st) return 3;
   uint32_t *giant = st->d_words;
-  const int hook_ilp = g_hook_ilp ? g_hook_ilp : (m >= kIlpMinDegree * (uint64_t)n ? 4 : 1);
-  const int hook_bpc = g_blocks_per_cu ? g_blocks_per_cu : (hook_ilp == 4 ? 8 : 4);
-  const unsigned nb = blocks_for(n, c->cu_count, hook_bpc);
+  const Plan plan = plan_for(n, m);
+  const int hook_ilp = plan.hook_ilp;
+  const unsigned nb = blocks_for(n, c->cu_count, plan.hook_bpc);
   const unsigned nf = blocks_for(n, c->cu_count, g_frontier_blocks_per_cu);
   const unsigned ns = blocks_for(n, c->cu_count, g_stream_blocks_per_cu);
   const uint32_t rounds = (uint32_t)g_sampled_neighbours;
@@ -546,28 +678,63 @@ extern "C" int cc_hip_components_device(const uint64_t *d_row_ptr, const uint32_
     else CCX_LAUNCH("cc_compress", s, k_compress<<<ns, kBlock, 0, s>>>(n, d_labels));
     return 0;
   };
-  if (ilp == 4)
-    CCX_LAUNCH("cc_init_min", s, k_init_min_ilp<4><<<ns, kBlock, 0, s>>>(d_row_ptr, d_col_idx, n, d_labels));
-  else if (ilp == 2)
-    CCX_LAUNCH("cc_init_min", s, k_init_min_ilp<2><<<ns, kBlock, 0, s>>>(d_row_ptr, d_col_idx, n, d_labels));
-  else
-    CCX_LAUNCH("cc_init_min", s, k_init_min<<<ns, kBlock, 0, s>>>(d_row_ptr, d_col_idx, n, d_labels));
   const int mode = g_hook_mode;
-  for (uint32_t r = 0; r < rounds; ++r) {
-    if (r > 0)
-      if (int rc = compress()) return rc;
-    if (mode == 1 && hook_ilp == 4)
-      CCX_LAUNCH("cc_hook_sample", s, k_hook_neighbour_ilp<1, 4><<<nb, kBlock, 0, s>>>(d_row_ptr, d_col_idx, n, r, d_labels));
-    else if (mode == 1 && hook_ilp == 2)
-      CCX_LAUNCH("cc_hook_sample", s, k_hook_neighbour_ilp<1, 2><<<nb, kBlock, 0, s>>>(d_row_ptr, d_col_idx, n, r, d_labels));
-    else if (mode == 0)
-      CCX_LAUNCH("cc_hook_sample", s, k_hook_neighbour<0><<<nb, kBlock, 0, s>>>(d_row_ptr, d_col_idx, n, r, d_labels));
-    else if (mode == 1)
-      CCX_LAUNCH("cc_hook_sample", s, k_hook_neighbour<1><<<nb, kBlock, 0, s>>>(d_row_ptr, d_col_idx, n, r, d_labels));
-    else if (mode == 2)
-      CCX_LAUNCH("cc_hook_sample", s, k_hook_neighbour<2><<<nb, kBlock, 0, s>>>(d_row_ptr, d_col_idx, n, r, d_labels));
+  // The cache holds `rounds` u32 per vertex in this stream's temp buffer;
+  // modes 0 and 1 read it (mode 2 walks with L2 reads, mode 3 is lossy).
+  const bool cache = plan.cache != 0;
+  uint32_t *smp = nullptr;
+  if (cache) {
+    smp = (uint32_t *)ccx::stream_tmp(st, (size_t)rounds * n * sizeof(uint32_t));
+    if (!smp) return 3;
+    switch (rounds) {
+      case 1: CCX_LAUNCH("cc_init_min", s, k_init_sample<1><<<ns, kBlock, 0, s>>>(d_row_ptr, d_col_idx, n, d_labels, smp)); break;
+      case 2: CCX_LAUNCH("cc_init_min", s, k_init_sample<2><<<ns, kBlock, 0, s>>>(d_row_ptr, d_col_idx, n, d_labels, smp)); break;
+      case 3: CCX_LAUNCH("cc_init_min", s, k_init_sample<3><<<ns, kBlock, 0, s>>>(d_row_ptr, d_col_idx, n, d_labels, smp)); break;
+      default: CCX_LAUNCH("cc_init_min", s, k_init_sample<4><<<ns, kBlock, 0, s>>>(d_row_ptr, d_col_idx, n, d_labels, smp)); break;
+    }
+  } else if (ilp == 4) {
+    CCX_LAUNCH("cc_init_min", s, k_init_min_ilp<4><<<ns, kBlock, 0, s>>>(d_row_ptr, d_col_idx, n, d_labels));
+  } else if (ilp == 2) {
+    CCX_LAUNCH("cc_init_min", s, k_init_min_ilp<2><<<ns, kBlock, 0, s>>>(d_row_ptr, d_col_idx, n, d_labels));
+  } else {
+    CCX_LAUNCH("cc_init_min", s, k_init_min<<<ns, kBlock, 0, s>>>(d_row_ptr, d_col_idx, n, d_labels));
+  }
+  if (plan.fused) {
+    const uint32_t *s1 = smp + n;
+    if (mode == 0)
+      CCX_LAUNCH("cc_hook_sample", s, k_hook_pair<0, 2><<<nb, kBlock, 0, s>>>(smp, s1, n, d_labels));
+    else if (hook_ilp >= 2)
+      CCX_LAUNCH("cc_hook_sample", s, k_hook_pair<1, 2><<<nb, kBlock, 0, s>>>(smp, s1, n, d_labels));
     else
-      CCX_LAUNCH("cc_hook_sample", s, k_hook_neighbour<3><<<nb, kBlock, 0, s>>>(d_row_ptr, d_col_idx, n, r, d_labels));
+      CCX_LAUNCH("cc_hook_sample", s, k_hook_pair<1, 1><<<nb, kBlock, 0, s>>>(smp, s1, n, d_labels));
+  } else {
+    for (uint32_t r = 0; r < rounds; ++r) {
+      if (r > 0)
+        if (int rc = compress()) return rc;
+      if (cache) {
+        const uint32_t *sr = smp + (size_t)r * n;
+        if (mode == 0)
+          CCX_LAUNCH("cc_hook_sample", s, k_hook_cached<0, 1><<<nb, kBlock, 0, s>>>(sr, n, r, d_labels));
+        else if (hook_ilp == 4)
+          CCX_LAUNCH("cc_hook_sample", s, k_hook_cached<1, 4><<<nb, kBlock, 0, s>>>(sr, n, r, d_labels));
+        else if (hook_ilp == 2)
+          CCX_LAUNCH("cc_hook_sample", s, k_hook_cached<1, 2><<<nb, kBlock, 0, s>>>(sr, n, r, d_labels));
+        else
+          CCX_LAUNCH("cc_hook_sample", s, k_hook_cached<1, 1><<<nb, kBlock, 0, s>>>(sr, n, r, d_labels));
+      } else if (mode == 1 && hook_ilp == 4) {
+        CCX_LAUNCH("cc_hook_sample", s, k_hook_neighbour_ilp<1, 4><<<nb, kBlock, 0, s>>>(d_row_ptr, d_col_idx, n, r, d_labels));
+      } else if (mode == 1 && hook_ilp == 2) {
+        CCX_LAUNCH("cc_hook_sample", s, k_hook_neighbour_ilp<1, 2><<<nb, kBlock, 0, s>>>(d_row_ptr, d_col_idx, n, r, d_labels));
+      } else if (mode == 0) {
+        CCX_LAUNCH("cc_hook_sample", s, k_hook_neighbour<0><<<nb, kBlock, 0, s>>>(d_row_ptr, d_col_idx, n, r, d_labels));
+      } else if (mode == 1) {
+        CCX_LAUNCH("cc_hook_sample", s, k_hook_neighbour<1><<<nb, kBlock, 0, s>>>(d_row_ptr, d_col_idx, n, r, d_labels));
+      } else if (mode == 2) {
+        CCX_LAUNCH("cc_hook_sample", s, k_hook_neighbour<2><<<nb, kBlock, 0, s>>>(d_row_ptr, d_col_idx, n, r, d_labels));
+      } else {
+        CCX_LAUNCH("cc_hook_sample", s, k_hook_neighbour<3><<<nb, kBlock, 0, s>>>(d_row_ptr, d_col_idx, n, r, d_labels));
+      }
+    }
   }
   CCX_LAUNCH("cc_sample_giant", s, k_sample_giant<<<1, kBlock, 0, s>>>(n, d_labels, giant));
   // Exact sampling (modes 0-2) links neighbours 0..rounds-1 of every vertex,

@@ -88,19 +88,35 @@ int cc_hip_components_tune(int sampled_neighbours, int blocks_per_cu, int hook_m
 /* The same knobs by name:
  *   "sampled_neighbours"      1..4 (default 2)
  *   "hook_mode"               0..3 (default 1)
+ *   "sample_cache"            1 (default): the init pass copies each vertex's
+ *                             sampled neighbours into coalesced arrays
+ *                             (rounds x n u32 in the stream's temp buffer)
+ *                             that the sampled rounds read; 0: the rounds
+ *                             gather from col_idx.  Modes 0 and 1 only.
+ *   "fused_pair"              1: hook both sampled neighbours in one pass
+ *                             (needs the cache and 2 rounds); 0: one pass
+ *                             per round; -1 (default): one pass unless the
+ *                             graph is dense (m >= 16 n) with m >= 2^30
  *   "blocks_per_cu"           sampled-hook grid cap; 0 (default) = 8 when the
- *                             hook walks 4 roots per thread, else 4
+ *                             hook walks several roots per thread, else 4
  *   "hook_ilp"                roots walked per thread in lockstep in the
- *                             sampled hook (mode 1): 1, 2, 4, or 0 (default)
- *                             = 4 when m >= 16 n, else 1
+ *                             sampled hook: 1, 2, 4, or 0 (default) = 1 when
+ *                             m < 16 n, else 2 in the one-pass hook, 4 per
+ *                             round
  *   "frontier_blocks_per_cu"  grid cap of the frontier sweep (default 8)
  *   "stream_blocks_per_cu"    grid cap of init/compress (default 8)
  *   "stream_ilp"              vertices per thread per step in init/compress
- *                             (1 default, 2, 4)
+ *                             without the cache (1 default, 2, 4)
  * Returns nonzero for an unknown key or out-of-range value; _get returns -1
  * for an unknown key. */
 int cc_hip_components_set(const char *key, int value);
 int cc_hip_components_get(const char *key);
+
+/* The choices a solve of an n-vertex, m-entry graph makes under the current
+ * knobs: cache in use, one-pass hook, roots per thread and grid cap of the
+ * sampled hook.  Any pointer may be NULL.  Returns 0. */
+int cc_hip_components_plan(uint32_t n, uint64_t m, int *cache, int *fused, int *hook_ilp,
+                           int *hook_blocks_per_cu);
 
 /* Same result through host buffers (H2D, solve, D2H; blocking). */
 int cc_hip_components_host(const uint64_t *row_ptr, const uint32_t *col_idx,

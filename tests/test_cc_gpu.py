@@ -114,3 +114,49 @@ def test_rmat22_full_size(gpu):
     assert np.count_nonzero(got == np.arange(G.n)) == roots.size
     assert np.array_equal(np.bincount(got, minlength=G.n)[roots],
                           np.bincount(want, minlength=G.n)[roots])
+
+
+PATHS = [  # every code path of the solve: (knobs) -> same labels
+    dict(sample_cache=0, fused_pair=0, hook_ilp=1, hook_mode=1),
+    dict(sample_cache=0, fused_pair=0, hook_ilp=2, hook_mode=1),
+    dict(sample_cache=0, fused_pair=0, hook_ilp=4, hook_mode=1),
+    dict(sample_cache=0, fused_pair=0, hook_ilp=1, hook_mode=0),
+    dict(sample_cache=0, fused_pair=0, hook_ilp=1, hook_mode=2),
+    dict(sample_cache=0, fused_pair=0, hook_ilp=1, hook_mode=3),
+    dict(sample_cache=1, fused_pair=0, hook_ilp=1, hook_mode=1),
+    dict(sample_cache=1, fused_pair=0, hook_ilp=2, hook_mode=1),
+    dict(sample_cache=1, fused_pair=0, hook_ilp=4, hook_mode=1),
+    dict(sample_cache=1, fused_pair=0, hook_ilp=1, hook_mode=0),
+    dict(sample_cache=1, fused_pair=1, hook_ilp=1, hook_mode=1),
+    dict(sample_cache=1, fused_pair=1, hook_ilp=2, hook_mode=1),
+    dict(sample_cache=1, fused_pair=1, hook_ilp=1, hook_mode=0),
+    dict(sample_cache=1, fused_pair=1, hook_ilp=2, hook_mode=1, sampled_neighbours=1),
+    dict(sample_cache=1, fused_pair=0, hook_ilp=4, hook_mode=1, sampled_neighbours=3),
+    dict(sample_cache=1, fused_pair=0, hook_ilp=1, hook_mode=1, sampled_neighbours=4),
+    dict(stream_ilp=2), dict(stream_ilp=4, sample_cache=0),
+]
+DEFAULTS = dict(sample_cache=1, fused_pair=-1, hook_ilp=0, hook_mode=1, sampled_neighbours=2,
+                stream_ilp=1, blocks_per_cu=0)
+
+
+@pytest.mark.parametrize("knobs", PATHS, ids=lambda k: ",".join(f"{a}={b}" for a, b in k.items()))
+def test_every_solve_path(gpu, knobs):
+    """Labels are a function of the graph alone, whichever kernels run."""
+    lib = nat.gpu_lib()
+    graphs = [cm.rmat(15, 16, seed=31), cm.erdos_renyi(50000, 3.0, seed=32),
+              cm.grid(120, 150, 0.6, seed=33), disjoint_cliques(200, 5, seed=34),
+              cm.chung_lu(1 << 16, 1 << 19, 2.1, seed=35)]
+    try:
+        for k, v in knobs.items():
+            nat.check(lib.cc_hip_components_set(k.encode(), v), k)
+        for G in graphs:
+            D = cm.DeviceGraph(G)
+            got = D.solve()
+            again = D.solve()
+            D.close()
+            check_labels(G, got)
+            assert np.array_equal(got, again)
+    finally:
+        for k, v in DEFAULTS.items():
+            nat.check(lib.cc_hip_components_set(k.encode(), v), k)
+

@@ -98,3 +98,17 @@ def test_grid_structure():
 def test_rmat_sizes():
     G = cm.rmat(10, 16, seed=1002)
     assert G.n == 1024 and G.m <= 2 * 16 * 1024
+
+
+def test_plan_defaults():
+    """The default plan: cache on; one-pass hook except on dense graphs
+    beyond 2^30 entries; several roots per thread only on dense graphs."""
+    lib = nat.gpu_lib()
+    def plan(n, m):
+        v = [ctypes.c_int() for _ in range(4)]
+        assert lib.cc_hip_components_plan(n, m, *[ctypes.byref(x) for x in v]) == 0
+        return tuple(x.value for x in v)
+    assert plan(4194304, 128313032) == (1, 1, 2, 8)
+    assert plan(65601024, 3377757790) == (1, 0, 4, 8)
+    assert plan(4194304, 33554406) == (1, 1, 1, 4)
+    assert plan(67108864, 187873588) == (1, 1, 1, 4)
