@@ -97,6 +97,10 @@ int cc_hip_components_tune(int sampled_neighbours, int blocks_per_cu, int hook_m
  *                             (needs the cache and 2 rounds); 0: one pass
  *                             per round; -1 (default): one pass unless the
  *                             graph is dense (m >= 16 n) with m >= 2^30
+ *   "giant_early"             1 (default): on the two-pass cached path,
+ *                             sample the giant root after round 0 and skip
+ *                             its members in round 1; 0: sample it once,
+ *                             after the last round
  *   "blocks_per_cu"           sampled-hook grid cap; 0 (default) = 8 when the
  *                             hook walks several roots per thread, else 4
  *   "hook_ilp"                roots walked per thread in lockstep in the

@@ -127,6 +127,9 @@ PATHS = [  # every code path of the solve: (knobs) -> same labels
     dict(sample_cache=1, fused_pair=0, hook_ilp=2, hook_mode=1),
     dict(sample_cache=1, fused_pair=0, hook_ilp=4, hook_mode=1),
     dict(sample_cache=1, fused_pair=0, hook_ilp=1, hook_mode=0),
+    dict(sample_cache=1, fused_pair=0, hook_ilp=4, hook_mode=1, giant_early=0),
+    dict(sample_cache=1, fused_pair=0, hook_ilp=1, hook_mode=0, giant_early=0),
+    dict(sample_cache=1, fused_pair=0, hook_ilp=2, hook_mode=1, giant_early=1, sampled_neighbours=4),
     dict(sample_cache=1, fused_pair=1, hook_ilp=1, hook_mode=1),
     dict(sample_cache=1, fused_pair=1, hook_ilp=2, hook_mode=1),
     dict(sample_cache=1, fused_pair=1, hook_ilp=1, hook_mode=0),
@@ -136,7 +139,7 @@ PATHS = [  # every code path of the solve: (knobs) -> same labels
     dict(stream_ilp=2), dict(stream_ilp=4, sample_cache=0),
 ]
 DEFAULTS = dict(sample_cache=1, fused_pair=-1, hook_ilp=0, hook_mode=1, sampled_neighbours=2,
-                stream_ilp=1, blocks_per_cu=0)
+                stream_ilp=1, blocks_per_cu=0, giant_early=1)
 
 
 @pytest.mark.parametrize("knobs", PATHS, ids=lambda k: ",".join(f"{a}={b}" for a, b in k.items()))
