@@ -96,6 +96,8 @@ def gpu_lib() -> ctypes.CDLL:
          [ctypes.POINTER(CSRGraphC), P_u32, c_int, c_int])
     _sig(lib, "cc_hip_sort_u64_device", c_int,
          [c_void_p, c_void_p, c_u64, ctypes.POINTER(c_int), c_void_p])
+    _sig(lib, "cc_hip_sort_set", c_int, [ctypes.c_char_p, c_int])
+    _sig(lib, "cc_hip_sort_get", c_int, [ctypes.c_char_p])
     _sig(lib, "cc_hip_dedup_u64_device", c_int, [c_void_p, c_u64, c_void_p, P_u64, c_void_p])
     _sig(lib, "cc_hip_ghost_extract_device", c_int,
          [c_void_p, c_u64, c_void_p, c_void_p, P_u64, c_void_p])
@@ -117,6 +119,7 @@ def gpu_lib() -> ctypes.CDLL:
     _sig(lib, "cc_hip_rank_labels_device", c_int, [V, V, c_u32, c_u32, V, P_u64, V])
     _sig(lib, "cc_hip_owner_keys_device", c_int, [V, c_u64, V, c_int, V, V])
     _sig(lib, "cc_hip_segments_device", c_int, [V, c_u64, c_int, c_u32, V, V, V])
+    _sig(lib, "cc_hip_send_lists_device", c_int, [V, c_u64, c_u32, V, c_int, c_u32, P_int, V, V])
     _sig(lib, "cc_hip_gather_u32_device", c_int, [V, V, c_u64, V, V])
     _sig(lib, "cc_hip_exclusive_scan_u32_device", c_int, [V, c_u64, V, V])
     _sig(lib, "cc_hip_delta_unpack_device", c_int, [V, c_u64, P_int, P_int, c_int, V, V, V])

@@ -60,6 +60,12 @@ int next_epoch(StreamState *st, uint32_t *epoch);
 // Grow the per-stream temp buffer to at least `bytes`.
 void *stream_tmp(StreamState *st, size_t bytes);
 
+// Store strategy of the split and dedup compactions (0: each thread stores
+// its own items; 1: staged in LDS, stored as contiguous runs).  Set through
+// cc_hip_sort_set.
+extern int g_split_staged;
+extern int g_dedup_staged;
+
 // Returns the context, initialising device 0 on first use.  nullptr on
 // failure (error recorded).
 Context *ctx();

@@ -127,9 +127,10 @@ __global__ __launch_bounds__(kThreads) void k_split_count(const uint32_t *__rest
   }
 }
 
-// First index r in [0, n] with row_ptr[r] >= x (row_ptr ascending).
-__device__ __forceinline__ uint32_t rows_lower_bound(const uint64_t *row_ptr, uint32_t n, uint64_t x) {
-  uint32_t lo = 0, hi = n + 1;
+// First index r in [lo, hi) with row_ptr[r] >= x, or hi if none
+// (row_ptr ascending).
+__device__ __forceinline__ uint32_t rows_lower_bound(const uint64_t *row_ptr, uint32_t lo,
+                                                     uint32_t hi, uint64_t x) {
   while (lo < hi) {
     const uint32_t mid = lo + (hi - lo) / 2;
     if (row_ptr[mid] < x) lo = mid + 1;
@@ -143,7 +144,13 @@ __device__ __forceinline__ uint32_t rows_lower_bound(const uint64_t *row_ptr, ui
 // publishes both of its counts through decoupled look-back (status words
 // 2*tile and 2*tile+1), and a per-thread cursor over the row starts gives
 // each row's local/boundary offsets and each boundary record's row li.
-// col_idx is read once, coalesced, whatever the degree distribution.
+// col_idx is read once, coalesced, whatever the degree distribution.  The
+// tile's local entries and records are staged in LDS at their tile-local
+// ranks and written out as two contiguous runs, so the stores coalesce
+// (Staged; 48 KiB of LDS, which costs occupancy), or stored straight from
+// each thread's cursor.  The rows overlapping the tile are found once per
+// workgroup, and each thread's search for its first row stays inside them.
+template <bool Staged>
 __global__ __launch_bounds__(kThreads) void k_split_emit(const uint64_t *__restrict__ row_ptr,
                                                          const uint32_t *__restrict__ col,
                                                          uint32_t n_local, uint32_t v_start,
@@ -157,6 +164,9 @@ __global__ __launch_bounds__(kThreads) void k_split_emit(const uint64_t *__restr
   __shared__ uint32_t s_tile;
   __shared__ uint64_t s_wsum[kWaves];
   __shared__ uint64_t s_base[2];
+  __shared__ uint32_t s_rows[2];
+  __shared__ uint32_t s_l[Staged ? kTile : 1];
+  __shared__ uint64_t s_b[Staged ? kTile : 1];
   const int t = threadIdx.x;
   const int lane = t & (kWave - 1);
   const int w = t / kWave;
@@ -166,7 +176,17 @@ __global__ __launch_bounds__(kThreads) void k_split_emit(const uint64_t *__restr
   __syncthreads();
   const uint64_t tile = s_tile;
   if (tile * kTile >= n) return;
-  const uint64_t first = e0 + tile * kTile + (uint64_t)t * kItems;
+  const uint64_t tile_first = e0 + tile * kTile;
+  const uint64_t tile_end = (tile_first + kTile < e1) ? tile_first + kTile : e1;
+  if (t == 0) {
+    // rows [r0, r1]: r0 = last row starting at or before tile_first (its
+    // entries may begin the tile), r1 = first row starting at or after
+    // tile_end (or n_local)
+    const uint32_t a = rows_lower_bound(row_ptr, 0, n_local + 1, tile_first);
+    s_rows[0] = (a <= n_local && row_ptr[a] == tile_first) ? a : a - 1;
+    s_rows[1] = rows_lower_bound(row_ptr, a, n_local + 1, tile_end);
+  }
+  const uint64_t first = tile_first + (uint64_t)t * kItems;
   uint32_t v[kItems];
   uint32_t lmask = 0, bmask = 0;
 #pragma unroll
@@ -187,8 +207,12 @@ __global__ __launch_bounds__(kThreads) void k_split_emit(const uint64_t *__restr
     if (ww < w) wprefix += s_wsum[ww];
     tile_total += s_wsum[ww];
   }
+  const uint32_t tile_l = (uint32_t)(tile_total & 0xFFFFFFFFull);
+  const uint32_t tile_b = (uint32_t)(tile_total >> 32);
+  (void)tile_l;
+  (void)tile_b;
   if (t < 2) {  // thread 0: local count, thread 1: boundary count
-    const uint64_t cnt = (t == 0) ? (tile_total & 0xFFFFFFFFull) : (tile_total >> 32);
+    const uint64_t cnt = (t == 0) ? tile_l : tile_b;
     uint64_t *my = status + 2 * tile + t;
     uint64_t excl = 0;
     if (tile == 0) {
@@ -200,16 +224,42 @@ __global__ __launch_bounds__(kThreads) void k_split_emit(const uint64_t *__restr
     }
     s_base[t] = excl;
   }
-  __syncthreads();
+  // tile-local ranks: stage the outputs while the look-back runs
   const uint64_t ex = wprefix + incl - mine;
-  uint64_t lr = s_base[0] + (ex & 0xFFFFFFFFull);
-  uint64_t br = s_base[1] + (ex >> 32);
-  if (first >= e1) return;
-  // rows starting inside [first, first + 16) get their offsets here
-  uint32_t r = rows_lower_bound(row_ptr, n_local, first);
-  // row of entry `first`: the last row starting at or before it
-  uint32_t cur = (r <= n_local && row_ptr[r] == first) ? r : r - 1;
+  uint32_t ll = (uint32_t)(ex & 0xFFFFFFFFull);
+  uint32_t bl = (uint32_t)(ex >> 32);
+  const uint32_t r_lo = s_rows[0], r_hi = s_rows[1];
+  uint32_t r = 0, cur = 0;
   const uint64_t stop = (first + kItems < e1) ? first + kItems : e1;
+  const bool has = first < e1;
+  if (has) {
+    // rows starting inside [first, first + 16) get their offsets below
+    r = rows_lower_bound(row_ptr, r_lo, r_hi + 1, first);
+    // row of entry `first`: the last row starting at or before it
+    cur = (r <= n_local && row_ptr[r] == first) ? r : r - 1;
+  }
+  if constexpr (Staged) {
+    if (has) {
+      uint32_t rr = r, cc = cur;
+#pragma unroll
+      for (int k = 0; k < kItems; ++k) {
+        const uint64_t j = first + k;
+        if (j >= stop) break;
+        while (rr <= n_local && row_ptr[rr] == j) cc = rr++;
+        if ((lmask >> k) & 1u) s_l[ll++] = v[k] - v_start;
+        else if ((bmask >> k) & 1u) s_b[bl++] = ((uint64_t)v[k] << 32) | cc;
+      }
+    }
+    __syncthreads();
+    for (uint32_t i = t; i < tile_l; i += kThreads) lcol[s_base[0] + i] = s_l[i];
+    for (uint32_t i = t; i < tile_b; i += kThreads) brec[s_base[1] + i] = s_b[i];
+  } else {
+    __syncthreads();
+  }
+  if (!has) return;
+  // row offsets: global ranks of each row's first local / boundary entry
+  uint64_t lr = s_base[0] + (uint32_t)(ex & 0xFFFFFFFFull);
+  uint64_t br = s_base[1] + (uint32_t)(ex >> 32);
 #pragma unroll
   for (int k = 0; k < kItems; ++k) {
     const uint64_t j = first + k;
@@ -220,8 +270,12 @@ __global__ __launch_bounds__(kThreads) void k_split_emit(const uint64_t *__restr
       cur = r;
       ++r;
     }
-    if ((lmask >> k) & 1u) lcol[lr++] = v[k] - v_start;
-    else if ((bmask >> k) & 1u) brec[br++] = ((uint64_t)v[k] << 32) | cur;
+    if constexpr (!Staged) {
+      if ((lmask >> k) & 1u) lcol[lr] = v[k] - v_start;
+      else if ((bmask >> k) & 1u) brec[br] = ((uint64_t)v[k] << 32) | cur;
+    }
+    lr += (lmask >> k) & 1u;
+    br += (bmask >> k) & 1u;
   }
   if (stop == e1) {  // trailing rows start at e1 (at least row n_local)
     while (r <= n_local) {
@@ -519,11 +573,16 @@ extern "C" int cc_hip_split_emit_device(const uint64_t *d_row_ptr, const uint32_
   if (int rc = ccx::next_epoch(st, &epoch)) return rc;
   uint32_t *ctr = st->d_words + 10;
   CCX_TRY(hipMemsetAsync(ctr, 0, sizeof(uint32_t), s));
-  CCX_LAUNCH("split_emit", s,
-             k_split_emit<<<(unsigned)tiles, kThreads, 0, s>>>(d_row_ptr, d_col_idx, n_local, v_start,
-                                                               v_end, n_global, d_lrow, d_boff,
-                                                               d_lcol, d_brec, ctr, st->d_status,
-                                                               epoch));
+  if (ccx::g_split_staged)
+    CCX_LAUNCH("split_emit", s,
+               k_split_emit<true><<<(unsigned)tiles, kThreads, 0, s>>>(
+                   d_row_ptr, d_col_idx, n_local, v_start, v_end, n_global, d_lrow, d_boff, d_lcol,
+                   d_brec, ctr, st->d_status, epoch));
+  else
+    CCX_LAUNCH("split_emit", s,
+               k_split_emit<false><<<(unsigned)tiles, kThreads, 0, s>>>(
+                   d_row_ptr, d_col_idx, n_local, v_start, v_end, n_global, d_lrow, d_boff, d_lcol,
+                   d_brec, ctr, st->d_status, epoch));
   return 0;
 }
 

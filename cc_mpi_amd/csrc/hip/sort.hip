@@ -85,32 +85,68 @@ struct DigitPlan {
 };
 
 // 256-bin histograms of every non-constant digit, one pass over the keys.
+// Each workgroup walks 4096-key tiles (16 keys per thread in flight, loaded
+// like the scatter passes load them) and counts into per-wave LDS
+// histograms.  A digit with at most 3 varying bits has at most 8 values, so
+// a wave's 64 lanes collide heavily on its bins: those digits are counted
+// by ballot matching (one add per group of equal digits); the others add
+// per lane, except that the lanes sharing lane 0's digit add once.
+constexpr int kHistWaveBins = 8 * kBins;
+
 __global__ __launch_bounds__(kThreads) void k_histograms(const uint64_t *__restrict__ keys, uint64_t n,
                                                          DigitPlan plan,
                                                          unsigned long long *__restrict__ hist) {
-  __shared__ uint32_t h[8][kBins];
-  for (int i = threadIdx.x; i < 8 * kBins; i += kThreads) (&h[0][0])[i] = 0;
+  __shared__ uint32_t h[kWaves][kHistWaveBins];
+  for (int i = threadIdx.x; i < kWaves * kHistWaveBins; i += kThreads) (&h[0][0])[i] = 0;
   __syncthreads();
   const int lane = threadIdx.x & (kWave - 1);
+  const int w = threadIdx.x / kWave;
   const uint64_t lt = lanemask_lt(lane);
-  const uint64_t stride = (uint64_t)gridDim.x * kThreads;
-  for (uint64_t base = (uint64_t)blockIdx.x * kThreads; base < n; base += stride) {
-    const uint64_t i = base + threadIdx.x;
-    const bool valid = i < n;
-    const uint64_t k = valid ? keys[i] : 0;
-    const uint64_t active = __ballot(valid);
+  uint32_t *hw = h[w];
+  const uint64_t tiles = (n + kTile - 1) / kTile;
+  for (uint64_t tile = blockIdx.x; tile < tiles; tile += gridDim.x) {
+    const uint64_t base = tile * kTile + (uint64_t)w * (kWave * kItems) + lane;
+    uint64_t k[kItems];
+#pragma unroll
+    for (int i = 0; i < kItems; ++i) {
+      const uint64_t pos = base + (uint64_t)i * kWave;
+      k[i] = pos < n ? keys[pos] : 0;
+    }
 #pragma unroll
     for (int d = 0; d < 8; ++d) {
-      if (!plan.vary[d]) continue;
-      const uint32_t dig = (uint32_t)(k >> (8 * d)) & 0xFFu;
-      const uint64_t peers = match_digit(dig, active, plan.vary[d]);
-      // the lowest peer adds the group's count
-      if (valid && (peers & lt) == 0) atomicAdd(&h[d][dig], (uint32_t)__popcll(peers));
+      const uint32_t vary = plan.vary[d];
+      if (!vary) continue;
+      if (__popc(vary) <= 3) {
+#pragma unroll
+        for (int i = 0; i < kItems; ++i) {
+          const bool valid = base + (uint64_t)i * kWave < n;
+          const uint32_t dig = (uint32_t)(k[i] >> (8 * d)) & 0xFFu;
+          const uint64_t peers = match_digit(dig, __ballot(valid), vary);
+          if (valid && (peers & lt) == 0) atomicAdd(&hw[d * kBins + dig], (uint32_t)__popcll(peers));
+        }
+      } else {
+#pragma unroll
+        for (int i = 0; i < kItems; ++i) {
+          // Wide digits can still be skewed (one value for most keys, e.g.
+          // the giant tree's rep): lanes equal to lane 0's digit are counted
+          // by lane 0 with one add, the rest add one each.  Valid lanes are
+          // a prefix of the wave, so lane 0 is valid whenever any lane is.
+          const bool valid = base + (uint64_t)i * kWave < n;
+          const uint32_t dig = (uint32_t)(k[i] >> (8 * d)) & 0xFFu;
+          const uint32_t first = __shfl(dig, 0, kWave);
+          const bool same = valid && dig == first;
+          const uint64_t bal = __ballot(same);
+          if (lane == 0 && valid) atomicAdd(&hw[d * kBins + first], (uint32_t)__popcll(bal));
+          else if (valid && !same) atomicAdd(&hw[d * kBins + dig], 1u);
+        }
+      }
     }
   }
   __syncthreads();
-  for (int i = threadIdx.x; i < 8 * kBins; i += kThreads) {
-    const uint32_t c = (&h[0][0])[i];
+  for (int i = threadIdx.x; i < kHistWaveBins; i += kThreads) {
+    uint32_t c = 0;
+#pragma unroll
+    for (int ww = 0; ww < kWaves; ++ww) c += h[ww][i];
     if (c) atomicAdd(&hist[i], (unsigned long long)c);
   }
 }
@@ -119,13 +155,15 @@ __global__ __launch_bounds__(kThreads) void k_histograms(const uint64_t *__restr
 // goff[d] = first output position of digit d (exclusive scan of the global
 // histogram).  tile_ctr hands out tiles in order; status holds per-tile,
 // per-digit look-back words (kBins per tile).
+template <int Items>
 __global__ __launch_bounds__(kThreads) void k_onesweep(const uint64_t *__restrict__ in,
                                                        uint64_t *__restrict__ out, uint64_t n,
                                                        int shift, uint32_t vary,
                                                        const uint64_t *__restrict__ goff,
                                                        uint32_t *__restrict__ tile_ctr,
                                                        uint64_t *__restrict__ status, uint32_t epoch) {
-  __shared__ uint64_t s_keys[kTile];
+  constexpr int TileK = kThreads * Items;
+  __shared__ uint64_t s_keys[TileK];
   __shared__ uint32_t s_wcnt[kWaves][kBins];  // per-wave digit counts, then exclusive offsets
   __shared__ uint32_t s_dstart[kBins];        // tile-local start of each digit
   __shared__ uint64_t s_gbase[kBins];         // global position of tile-local index 0 of digit d
@@ -141,20 +179,20 @@ __global__ __launch_bounds__(kThreads) void k_onesweep(const uint64_t *__restric
   const uint64_t tile = s_tile;
   // Tile ids come from a counter zeroed before the launch and run 0..tiles-1;
   // an id past the input (never expected) must not touch memory.
-  if (tile * kTile >= n) return;
-  const uint64_t tile_base = tile * kTile;
+  if (tile * TileK >= n) return;
+  const uint64_t tile_base = tile * TileK;
   const uint64_t lt = lanemask_lt(lane);
 
-  uint64_t key[kItems];
-  uint32_t rank[kItems];
+  uint64_t key[Items];
+  uint32_t rank[Items];
 #pragma unroll
-  for (int i = 0; i < kItems; ++i) {
-    const uint64_t pos = tile_base + (uint64_t)w * (kWave * kItems) + (uint64_t)i * kWave + lane;
+  for (int i = 0; i < Items; ++i) {
+    const uint64_t pos = tile_base + (uint64_t)w * (kWave * Items) + (uint64_t)i * kWave + lane;
     key[i] = (pos < n) ? in[pos] : ~0ull;
   }
 #pragma unroll
-  for (int i = 0; i < kItems; ++i) {
-    const uint64_t pos = tile_base + (uint64_t)w * (kWave * kItems) + (uint64_t)i * kWave + lane;
+  for (int i = 0; i < Items; ++i) {
+    const uint64_t pos = tile_base + (uint64_t)w * (kWave * Items) + (uint64_t)i * kWave + lane;
     const bool valid = pos < n;
     const uint32_t d = (uint32_t)(key[i] >> shift) & 0xFFu;
     const uint64_t active = __ballot(valid);
@@ -203,15 +241,15 @@ __global__ __launch_bounds__(kThreads) void k_onesweep(const uint64_t *__restric
 
   // Stable scatter into LDS in digit order.
 #pragma unroll
-  for (int i = 0; i < kItems; ++i) {
-    const uint64_t pos = tile_base + (uint64_t)w * (kWave * kItems) + (uint64_t)i * kWave + lane;
+  for (int i = 0; i < Items; ++i) {
+    const uint64_t pos = tile_base + (uint64_t)w * (kWave * Items) + (uint64_t)i * kWave + lane;
     if (pos < n) {
       const uint32_t d = (uint32_t)(key[i] >> shift) & 0xFFu;
       s_keys[s_dstart[d] + s_wcnt[w][d] + rank[i]] = key[i];
     }
   }
   __syncthreads();
-  const uint64_t valid_in_tile = (n - tile_base) < (uint64_t)kTile ? (n - tile_base) : kTile;
+  const uint64_t valid_in_tile = (n - tile_base) < (uint64_t)TileK ? (n - tile_base) : TileK;
   for (uint32_t j = t; j < valid_in_tile; j += kThreads) {
     const uint64_t k = s_keys[j];
     const uint32_t d = (uint32_t)(k >> shift) & 0xFFu;
@@ -222,9 +260,11 @@ __global__ __launch_bounds__(kThreads) void k_onesweep(const uint64_t *__restric
 // ---- ordered compaction --------------------------------------------------
 
 // Op interface: bool eval(uint64_t j, uint64_t &val) decides whether item j
-// is kept (val carries data to apply); void apply(uint64_t j, bool keep,
-// uint64_t rank, uint64_t val) is called for every item in order, rank =
-// number of kept items before j.  begin(first_item) runs once per thread.
+// is kept (val carries data to apply).  A staged op (kStaged) writes only
+// kept values: store(rank, val) is called for each, from a contiguous run
+// staged in LDS.  Otherwise void apply(uint64_t j, bool keep, uint64_t
+// rank, uint64_t val) is called for every item in order, rank = number of
+// kept items before j, after begin(first_item) once per thread.
 template <class Op>
 __global__ __launch_bounds__(kThreads) void k_compact(Op op, uint64_t n, uint32_t *__restrict__ tile_ctr,
                                                       uint64_t *__restrict__ status, uint32_t epoch,
@@ -272,38 +312,58 @@ __global__ __launch_bounds__(kThreads) void k_compact(Op op, uint64_t n, uint32_
     s_base = excl;
     if ((tile + 1) * kTile >= n) *total = excl + tile_total;
   }
-  __syncthreads();
-  uint64_t rank = s_base + wprefix + incl - cnt;
-  op.begin(first);
+  if constexpr (Op::kStaged) {
+    // kept values go to LDS at their tile-local rank, then out as one
+    // contiguous run (coalesced stores)
+    __shared__ uint64_t s_out[kTile];
+    uint32_t lr = wprefix + incl - cnt;
 #pragma unroll
-  for (int k = 0; k < kItems; ++k) {
-    const uint64_t j = first + k;
-    if (j < n) {
-      const bool kf = (keep >> k) & 1u;
-      op.apply(j, kf, rank, val[k]);
-      rank += kf;
+    for (int k = 0; k < kItems; ++k)
+      if ((keep >> k) & 1u) s_out[lr++] = val[k];
+    __syncthreads();
+    const uint64_t base = s_base;
+    for (uint32_t i = t; i < tile_total; i += kThreads) op.store(base + i, s_out[i]);
+  } else {
+    __syncthreads();
+    uint64_t rank = s_base + wprefix + incl - cnt;
+    op.begin(first);
+#pragma unroll
+    for (int k = 0; k < kItems; ++k) {
+      const uint64_t j = first + k;
+      if (j < n) {
+        const bool kf = (keep >> k) & 1u;
+        op.apply(j, kf, rank, val[k]);
+        rank += kf;
+      }
     }
   }
 }
 
 // Adjacent-unique (reference src/boundary_sort.c:100-119).
+template <bool Staged>
 struct DedupOp {
+  static constexpr bool kStaged = Staged;
   const uint64_t *in;
   uint64_t *out;
-  __device__ void begin(uint64_t) {}
   __device__ bool eval(uint64_t j, uint64_t &v) const {
     v = in[j];
     return j == 0 || v != in[j - 1];
   }
+  __device__ void store(uint64_t rank, uint64_t v) const { out[rank] = v; }
+  __device__ void begin(uint64_t) {}
   __device__ void apply(uint64_t, bool keep, uint64_t rank, uint64_t v) const {
     if (keep) out[rank] = v;
   }
 };
 
+int run_dedup(const uint64_t *in, uint64_t *out, uint64_t n, hipStream_t s, ccx::StreamState *st,
+              uint64_t *count_out);
+
 // Ghost extraction over sorted unique records (reference src/cc_mpi.c:431-457):
 // ghosts[g] = g-th distinct remote; each record's remote becomes its ghost
 // index g.  Records are (remote << 32) | rep.
 struct GhostOp {
+  static constexpr bool kStaged = false;
   const uint64_t *in;
   uint64_t *out;
   uint32_t *ghosts;
@@ -317,6 +377,65 @@ struct GhostOp {
     const uint64_t g = rank + (keep ? 1 : 0) - 1;
     out[j] = (g << 32) | (uint32_t)v;
   }
+};
+
+// ---- send lists without sorting ---------------------------------------------
+// For the exchange plan, rank q must receive every owned vertex li that has
+// a boundary record whose remote end q owns.  mask[li] collects those q as
+// bits: records are read coalesced, each wave OR-reduces the bits of equal
+// li over its 64 lanes (a segmented scan keyed on li; records of one row
+// are contiguous, so a row costs one atomic per wave it spans), and the
+// lane ending each run ORs the result into mask[li].
+__global__ __launch_bounds__(kThreads) void k_owner_mask(const uint64_t *__restrict__ rec, uint64_t nb,
+                                                         const uint32_t *__restrict__ bounds, int P,
+                                                         uint32_t n_local,
+                                                         unsigned long long *__restrict__ mask) {
+  const int lane = threadIdx.x & (kWave - 1);
+  const uint64_t nwaves = (uint64_t)gridDim.x * kWaves;
+  // the loop bound is uniform across the wave, so every lane takes part in
+  // the shuffles
+  for (uint64_t c = ((uint64_t)blockIdx.x * kThreads + threadIdx.x) / kWave; c * kWave < nb;
+       c += nwaves) {
+    const uint64_t i = c * kWave + lane;
+    const bool valid = i < nb;
+    uint32_t li = 0xFFFFFFFFu;
+    unsigned long long m = 0;
+    if (valid) {
+      const uint64_t r = rec[i];
+      const uint32_t v = (uint32_t)(r >> 32);
+      li = (uint32_t)r;
+      int lo = 0, hi = P - 1;
+      while (lo < hi) {  // last q with bounds[q] <= v
+        const int mid = (lo + hi + 1) >> 1;
+        if (bounds[mid] <= v) lo = mid;
+        else hi = mid - 1;
+      }
+      m = 1ull << lo;
+    }
+#pragma unroll
+    for (int d = 1; d < kWave; d <<= 1) {
+      const unsigned long long um = __shfl_up(m, d, kWave);
+      const uint32_t ul = __shfl_up(li, d, kWave);
+      if (lane >= d && ul == li) m |= um;
+    }
+    const uint32_t next = __shfl_down(li, 1, kWave);
+    const bool run_end = (lane == kWave - 1) || next != li;
+    if (valid && run_end && li < n_local) atomicOr(&mask[li], m);
+  }
+}
+
+// Ordered compaction of {j : bit q of mask[j]} into vals[] as j + base.
+struct MaskOp {
+  static constexpr bool kStaged = true;
+  const unsigned long long *mask;
+  uint32_t *vals;
+  uint32_t base;
+  int q;
+  __device__ bool eval(uint64_t j, uint64_t &v) const {
+    v = (uint32_t)j + base;
+    return (mask[j] >> q) & 1ull;
+  }
+  __device__ void store(uint64_t rank, uint64_t v) const { vals[rank] = (uint32_t)v; }
 };
 
 inline unsigned tiles_for(uint64_t n) { return (unsigned)((n + kTile - 1) / kTile); }
@@ -344,9 +463,49 @@ int run_compact(const char *name, Op op, uint64_t n, hipStream_t s, ccx::StreamS
   return 0;
 }
 
+int run_dedup(const uint64_t *in, uint64_t *out, uint64_t n, hipStream_t s, ccx::StreamState *st,
+              uint64_t *count_out) {
+  if (ccx::g_dedup_staged)
+    return run_compact("dedup_compact", DedupOp<true>{in, out}, n, s, st, count_out);
+  return run_compact("dedup_compact", DedupOp<false>{in, out}, n, s, st, count_out);
+}
+
 }  // namespace
 
 // ---- sort ------------------------------------------------------------------
+
+namespace ccx {
+int g_split_staged = 0;
+int g_dedup_staged = 0;
+}  // namespace ccx
+
+namespace {
+int g_sort_items = 32;  // keys per thread in a scatter pass (tile = 256 x items; 32 measured best)
+}  // namespace
+
+extern "C" int cc_hip_sort_set(const char *key, int value) {
+  if (key && strcmp(key, "items_per_thread") == 0 &&
+      (value == 8 || value == 16 || value == 24 || value == 32)) {
+    g_sort_items = value;
+    return 0;
+  }
+  if (key && strcmp(key, "split_staged") == 0 && (value == 0 || value == 1)) {
+    ccx::g_split_staged = value;
+    return 0;
+  }
+  if (key && strcmp(key, "dedup_staged") == 0 && (value == 0 || value == 1)) {
+    ccx::g_dedup_staged = value;
+    return 0;
+  }
+  return ccx::set_error(4, "cc_hip_sort_set: bad key or value (%s=%d)", key ? key : "", value);
+}
+
+extern "C" int cc_hip_sort_get(const char *key) {
+  if (key && strcmp(key, "items_per_thread") == 0) return g_sort_items;
+  if (key && strcmp(key, "split_staged") == 0) return ccx::g_split_staged;
+  if (key && strcmp(key, "dedup_staged") == 0) return ccx::g_dedup_staged;
+  return -1;
+}
 
 namespace ccx {
 // Sorts keys[0..n) ascending; alt is a buffer of n keys.  On return
@@ -383,7 +542,13 @@ int sort_u64(uint64_t *keys, uint64_t *alt, uint64_t n, int *in_alt, hipStream_t
   if (passes == 0) return 0;  // all keys equal
 
   CCX_TRY(hipMemsetAsync(d_hist, 0, 8 * kBins * sizeof(uint64_t), s));
-  CCX_LAUNCH("sort_histograms", s, k_histograms<<<red_blocks, kThreads, 0, s>>>(keys, n, plan, d_hist));
+  {
+    // 32 KiB of LDS per workgroup: 4 per CU resident
+    const uint64_t tiles = tiles_for(n);
+    const uint64_t cap = (uint64_t)(ctx()->cu_count > 0 ? ctx()->cu_count : 256) * 4;
+    const unsigned hb = (unsigned)(tiles < cap ? tiles : cap);
+    CCX_LAUNCH("sort_histograms", s, k_histograms<<<hb, kThreads, 0, s>>>(keys, n, plan, d_hist));
+  }
   uint64_t *h = st->h_stage;
   CCX_TRY(hipMemcpyAsync(h, d_hist, 8 * kBins * sizeof(uint64_t), hipMemcpyDeviceToHost, s));
   CCX_TRY(hipStreamSynchronize(s));
@@ -399,7 +564,9 @@ int sort_u64(uint64_t *keys, uint64_t *alt, uint64_t n, int *in_alt, hipStream_t
   // the staging buffer is reused by the next call: let the copy land first
   CCX_TRY(hipStreamSynchronize(s));
 
-  const uint64_t tiles = tiles_for(n);
+  const int items = g_sort_items;
+  const uint64_t tile_keys = (uint64_t)kThreads * items;
+  const uint64_t tiles = (n + tile_keys - 1) / tile_keys;
   if (tiles > 0xFFFFFFFFull) return set_error(4, "sort: input too large");
   if (int rc = ensure_status(st, tiles * kBins)) return rc;
   uint64_t *src = keys, *dst = alt;
@@ -410,10 +577,16 @@ int sort_u64(uint64_t *keys, uint64_t *alt, uint64_t n, int *in_alt, hipStream_t
     if (int rc = next_epoch(st, &epoch)) return rc;
     uint32_t *ctr = st->d_words + 8 + pass;
     CCX_TRY(hipMemsetAsync(ctr, 0, sizeof(uint32_t), s));
-    CCX_LAUNCH("sort_onesweep", s,
-               k_onesweep<<<(unsigned)tiles, kThreads, 0, s>>>(src, dst, n, 8 * d, plan.vary[d],
-                                                               d_goff + d * kBins, ctr,
-                                                               st->d_status, epoch));
+#define CCX_ONESWEEP(I)                                                                      \
+  CCX_LAUNCH("sort_onesweep", s,                                                             \
+             k_onesweep<I><<<(unsigned)tiles, kThreads, 0, s>>>(src, dst, n, 8 * d, plan.vary[d], \
+                                                                d_goff + d * kBins, ctr,       \
+                                                                st->d_status, epoch))
+    if (items == 8) CCX_ONESWEEP(8);
+    else if (items == 24) CCX_ONESWEEP(24);
+    else if (items == 32) CCX_ONESWEEP(32);
+    else CCX_ONESWEEP(16);
+#undef CCX_ONESWEEP
     uint64_t *tt = src;
     src = dst;
     dst = tt;
@@ -452,7 +625,7 @@ extern "C" int cc_hip_dedup_u64_device(const uint64_t *d_in, uint64_t count, uin
   hipStream_t s = ccx::pick_stream(stream);
   ccx::StreamState *st = ccx::stream_state(s);
   if (!st) return 3;
-  return run_compact("dedup_compact", DedupOp{d_in, d_out}, count, s, st, out_count);
+  return run_dedup(d_in, d_out, count, s, st, out_count);
 }
 
 extern "C" int cc_hip_ghost_extract_device(const uint64_t *d_records, uint64_t count,
@@ -469,6 +642,47 @@ extern "C" int cc_hip_ghost_extract_device(const uint64_t *d_records, uint64_t c
   if (!st) return 3;
   return run_compact("ghost_compact", GhostOp{d_records, d_out_records, d_ghost_vertices}, count,
                      s, st, ghost_count);
+}
+
+extern "C" int cc_hip_send_lists_device(const uint64_t *d_records, uint64_t count,
+                                        uint32_t n_local, const uint32_t *d_bounds, int comm_size,
+                                        uint32_t base, int *h_starts, uint32_t *d_vals,
+                                        void *stream) {
+  if (!ccx::ctx()) return 2;
+  if (comm_size < 1 || comm_size > 64)
+    return ccx::set_error(4, "cc_hip_send_lists_device: comm_size %d outside 1..64", comm_size);
+  if (!h_starts || !d_bounds || (count > 0 && (!d_records || !d_vals)))
+    return ccx::set_error(4, "cc_hip_send_lists_device: null pointer");
+  hipStream_t s = ccx::pick_stream(stream);
+  ccx::StreamState *st = ccx::stream_state(s);
+  if (!st) return 3;
+  h_starts[0] = 0;
+  if (count == 0 || n_local == 0) {
+    for (int q = 1; q <= comm_size; ++q) h_starts[q] = 0;
+    return 0;
+  }
+  unsigned long long *mask =
+      (unsigned long long *)ccx::stream_tmp(st, (size_t)n_local * sizeof(unsigned long long));
+  if (!mask) return 3;
+  CCX_TRY(hipMemsetAsync(mask, 0, (size_t)n_local * sizeof(unsigned long long), s));
+  const uint64_t waves = (count + kWave - 1) / kWave;
+  const uint64_t cap = (uint64_t)(ccx::ctx()->cu_count > 0 ? ccx::ctx()->cu_count : 256) * 8 * kWaves;
+  const unsigned grid = (unsigned)(((waves < cap ? waves : cap) + kWaves - 1) / kWaves);
+  CCX_LAUNCH("owner_mask", s,
+             k_owner_mask<<<grid, kThreads, 0, s>>>(d_records, count, d_bounds, comm_size, n_local,
+                                                   mask));
+  uint64_t total = 0;
+  for (int q = 0; q < comm_size; ++q) {
+    uint64_t cnt = 0;
+    if (int rc = run_compact("send_compact", MaskOp{mask, d_vals + total, base, q}, n_local, s, st,
+                             &cnt))
+      return rc;
+    total += cnt;
+    if (total > 0x7FFFFFFFull)
+      return ccx::set_error(4, "cc_hip_send_lists_device: lists exceed int offsets");
+    h_starts[q + 1] = (int)total;
+  }
+  return 0;
 }
 
 // ---- host drop-ins (include/boundary_sort.h) --------------------------------
@@ -502,7 +716,7 @@ static int host_dedup(BoundaryEdge *edges, uint64_t count, uint64_t *out) {
   CCX_TRY(hipMemcpyAsync(a, edges, bytes, hipMemcpyHostToDevice, s));
   ccx::StreamState *st = ccx::stream_state(s);
   if (!st) return 3;
-  if (int rc = run_compact("dedup_compact", DedupOp{a, b}, count, s, st, out)) return rc;
+  if (int rc = run_dedup(a, b, count, s, st, out)) return rc;
   CCX_TRY(hipMemcpyAsync(edges, b, *out * sizeof(uint64_t), hipMemcpyDeviceToHost, s));
   CCX_TRY(hipStreamSynchronize(s));
   return 0;

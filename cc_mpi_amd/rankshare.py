@@ -79,27 +79,23 @@ class RankShare:
         self.lrow.close()
         self.lcol.close()
 
-        # send lists: the owned vertices each other rank has as ghosts
+        # send lists: the owned vertices each other rank has as ghosts (one
+        # bitmask pass over the raw records, then one ordered compaction per
+        # destination)
         t0 = time.perf_counter()
         d_bounds = DeviceBuffer.from_array(self.bounds)
-        keys = DeviceBuffer(max(8, 8 * self.nb))
-        alt = DeviceBuffer(max(8, 8 * self.nb))
-        nat.check(lib.cc_hip_owner_keys_device(rec.ptr, self.nb, d_bounds.ptr, self.P, keys.ptr,
-                                               None), "owner keys")
-        which = sort_device(keys, alt, self.nb)
-        srt, oth = (alt, keys) if which else (keys, alt)
-        nsend = dedup_device(srt, self.nb, oth)
-        d_starts = DeviceBuffer(4 * (self.P + 1))
-        self.send_to_flat = DeviceBuffer(max(4, 4 * nsend))
-        nat.check(lib.cc_hip_segments_device(oth.ptr, nsend, self.P, vs, d_starts.ptr,
-                                             self.send_to_flat.ptr, None), "segments")
-        starts = np.empty(self.P + 1, dtype=np.int32)
-        d_starts.download(starts)
+        self.send_to_flat = DeviceBuffer(4 * max(1, min(self.nb, nl * self.P)))
+        starts = np.zeros(self.P + 1, dtype=np.int32)
+        nat.check(lib.cc_hip_send_lists_device(rec.ptr, self.nb, nl, d_bounds.ptr, self.P, vs,
+                                               nat.ptr(starts, ctypes.c_int),
+                                               self.send_to_flat.ptr, None), "send lists")
+        nsend = int(starts[self.P])
         self.total_send = nsend
         # destination segments in dests order (ranks with something to send)
         self.dests = [q for q in range(self.P) if starts[q + 1] > starts[q]]
         self.seg_start = np.array([starts[q] for q in self.dests] + [nsend], dtype=np.int32)
         self._t("send_lists", t0)
+        keys = DeviceBuffer(max(8, 8 * self.nb))
 
         # boundary records: rep fill, sort, dedup, ghost extraction
         t0 = time.perf_counter()
@@ -116,7 +112,7 @@ class RankShare:
         self.ng = ghost_extract_device(oth, self.nbd, srt, self.ghosts)
         self.bnd = srt
         self._t("ghost_extract", t0)
-        for b in (oth, alt, d_bounds, d_starts):
+        for b in (oth, d_bounds):
             if b is not srt:
                 b.close()
         self.stats.update(n_local=nl, m_local_local=self.m_ll, boundary_raw=self.nb,

@@ -139,6 +139,16 @@ int cc_hip_components_host(const uint64_t *row_ptr, const uint32_t *col_idx,
 int cc_hip_sort_u64_device(uint64_t *d_keys, uint64_t *d_alt, uint64_t count,
                            int *result_in_alt, void *stream);
 
+/* Tuning of the sort's scatter passes: "items_per_thread" (8, 16, 24 or 32
+ * (default) keys per thread; a workgroup's tile is 256 x that); and of the
+ * split and dedup compactions: "split_staged" / "dedup_staged" (0 default:
+ * each thread stores its own items; 1: outputs staged in LDS and stored as
+ * contiguous runs, which measured slower at 638M records because the LDS
+ * costs occupancy).  The result never depends on them.  _set returns nonzero for a bad key or value, _get -1
+ * for an unknown key. */
+int cc_hip_sort_set(const char *key, int value);
+int cc_hip_sort_get(const char *key);
+
 /* Adjacent-unique of d_in[0..count) into d_out (must not alias d_in);
  * *out_count receives the kept count (blocking).  src/boundary_sort.c:100-119. */
 int cc_hip_dedup_u64_device(const uint64_t *d_in, uint64_t count, uint64_t *d_out,
@@ -223,6 +233,18 @@ int cc_hip_iota_u32_device(uint32_t *d_out, uint32_t n, uint32_t base, void *str
  * d_bounds[0..comm_size] (src/cc_mpi.c:41-76). */
 int cc_hip_owner_keys_device(const uint64_t *d_records, uint64_t count, const uint32_t *d_bounds,
                              int comm_size, uint64_t *d_keys, void *stream);
+/* The send lists of the exchange plan in one step, without sorting: for
+ * each rank q in [0, comm_size) ascending, the ascending owned local ids li
+ * (as li + base) that have a raw record (remote << 32) | li whose remote end
+ * q owns under d_bounds[0..comm_size] (the same lists as owner keys ->
+ * sort -> dedup -> segments).  h_starts[0..comm_size] (host) receives the
+ * segment starts; d_vals needs room for every listed id (at most
+ * min(count, n_local * comm_size)).  Records must carry li < n_local; those
+ * of one li should be contiguous (as the split emits them) for speed, not
+ * for correctness.  comm_size <= 64.  Blocking. */
+int cc_hip_send_lists_device(const uint64_t *d_records, uint64_t count, uint32_t n_local,
+                             const uint32_t *d_bounds, int comm_size, uint32_t base,
+                             int *h_starts, uint32_t *d_vals, void *stream);
 /* Segments of sorted unique keys (q << 32) | x for q in [0, nseg):
  * d_starts[q] = first index of segment q (d_starts[nseg] = count) and
  * d_vals[i] = x + base. */

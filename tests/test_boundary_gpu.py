@@ -30,11 +30,18 @@ def test_dropin_sort_matches_oracle(gpu, count):
     assert np.array_equal(got, oracle.sort_u64(k))
 
 
+@pytest.mark.parametrize("staged", [0, 1])
 @pytest.mark.parametrize("count", SIZES)
-def test_dropin_dedup_matches_oracle(gpu, count):
+def test_dropin_dedup_matches_oracle(gpu, count, staged):
+    from cc_mpi_amd import _native as nat
+    lib = nat.gpu_lib()
     k = oracle.sort_u64(records(count, seed=count + 1, rep_bits=8, remote_bits=9))
     got = k.copy()
-    n = bd.boundary_edges_dedup(got)
+    try:
+        assert lib.cc_hip_sort_set(b"dedup_staged", staged) == 0
+        n = bd.boundary_edges_dedup(got)
+    finally:
+        lib.cc_hip_sort_set(b"dedup_staged", 0)
     want = oracle.dedup_u64(k)
     assert n == want.size
     assert np.array_equal(got[:n], want)
@@ -62,6 +69,32 @@ def test_sort_key_shapes(gpu, kind):
     got = k.copy()
     bd.boundary_edges_sort(got)
     assert np.array_equal(got, np.sort(k))
+
+
+@pytest.mark.parametrize("items", [8, 16, 24, 32])
+@pytest.mark.parametrize("kind", ["records", "skewed", "full64"])
+def test_sort_every_tile_size(gpu, items, kind):
+    """The scatter passes give the same bytes for every tile size, on tile
+    boundaries and ragged tails, with skewed and uniform digits."""
+    from cc_mpi_amd import _native as nat
+    lib = nat.gpu_lib()
+    rng = np.random.default_rng(items)
+    n = 256 * items * 37 + 123
+    if kind == "records":
+        k = records(n, seed=items)
+    elif kind == "skewed":
+        k = (rng.integers(0, 3, n, dtype=np.uint64) << np.uint64(56)) | rng.integers(
+            0, 1 << 12, n, dtype=np.uint64)
+    else:
+        k = rng.integers(0, np.iinfo(np.uint64).max, n, dtype=np.uint64, endpoint=True)
+    try:
+        assert lib.cc_hip_sort_set(b"items_per_thread", items) == 0
+        for m in (n, 256 * items, 256 * items + 1, 256 * items * 5):
+            got = k[:m].copy()
+            bd.boundary_edges_sort(got)
+            assert np.array_equal(got, np.sort(k[:m])), (items, kind, m)
+    finally:
+        lib.cc_hip_sort_set(b"items_per_thread", 32)
 
 
 def test_device_sort_ping_pong_and_repeat(gpu):

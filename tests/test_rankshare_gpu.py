@@ -19,8 +19,20 @@ def graphs():
             "grid": lambda: devgen.grid(200, 250, 0.7, seed=33)}
 
 
+@pytest.fixture(params=[0, 1], ids=["direct", "staged"])
+def store_strategy(request):
+    """Both store strategies of the split and dedup compactions."""
+    from cc_mpi_amd import _native as nat
+    lib = nat.gpu_lib()
+    for k in (b"split_staged", b"dedup_staged"):
+        assert lib.cc_hip_sort_set(k, request.param) == 0
+    yield request.param
+    for k in (b"split_staged", b"dedup_staged"):
+        lib.cc_hip_sort_set(k, 0)
+
+
 @pytest.mark.parametrize("P,rank", [(1, 0), (4, 0), (4, 1), (7, 6)])
-def test_split_matches_oracle(gpu, P, rank):
+def test_split_matches_oracle(gpu, store_strategy, P, rank):
     import ctypes
     from cc_mpi_amd import _native as nat
     h = cm.rmat(13, 16, seed=41)
@@ -116,3 +128,45 @@ def test_rank_share(gpu, name, rank, interval):
     assert packs[0] == rs.total_send  # the first exchange sends everything
     rs.close()
     full.close()
+
+
+@pytest.mark.parametrize("P,grouped", [(1, True), (2, True), (4, True), (7, False), (64, True),
+                                       (64, False)])
+def test_send_lists_match_their_definition(gpu, P, grouped):
+    """cc_hip_send_lists_device equals, list for list, the sorted unique
+    owned ids per destination rank (owner keys -> sort -> dedup ->
+    segments), with records grouped by row or in arbitrary order."""
+    import ctypes
+    from cc_mpi_amd import _native as nat
+    lib = nat.gpu_lib()
+    rng = np.random.default_rng(P * 2 + grouped)
+    n_global, nl, vs = 5_000_000, 300_000, 1_000_000
+    bounds = np.sort(rng.choice(np.arange(1, n_global), P - 1, replace=False)).astype(np.uint32)
+    bounds = np.concatenate([[0], bounds, [n_global]]).astype(np.uint32)
+    deg = rng.geometric(0.05, nl) - 1
+    deg[rng.integers(0, nl, 5)] = 200_000  # a few hub rows spanning many waves
+    li = np.repeat(np.arange(nl, dtype=np.uint64), deg)
+    remote = rng.integers(0, n_global, li.size, dtype=np.uint64)
+    rec = (remote << np.uint64(32)) | li
+    if not grouped:
+        rec = rec[rng.permutation(rec.size)]
+    owner = np.searchsorted(bounds, (rec >> np.uint64(32)).astype(np.int64), side="right") - 1
+    d_rec = cm.DeviceBuffer.from_array(rec)
+    d_b = cm.DeviceBuffer.from_array(bounds)
+    d_vals = cm.DeviceBuffer(4 * max(1, min(rec.size, nl * P)))
+    starts = np.zeros(P + 1, dtype=np.int32)
+    nat.check(lib.cc_hip_send_lists_device(d_rec.ptr, rec.size, nl, d_b.ptr, P, vs,
+                                           nat.ptr(starts, ctypes.c_int), d_vals.ptr, None), "sl")
+    vals = np.empty(int(starts[P]), dtype=np.uint32)
+    d_vals.download(vals)
+    low = (rec & np.uint64(0xFFFFFFFF)).astype(np.int64)
+    for q in range(P):
+        want = np.unique(low[owner == q]) + vs
+        assert np.array_equal(vals[starts[q]:starts[q + 1]], want.astype(np.uint32)), q
+    # nothing to send
+    z = np.zeros(P + 1, dtype=np.int32) - 1
+    nat.check(lib.cc_hip_send_lists_device(d_rec.ptr, 0, nl, d_b.ptr, P, vs,
+                                           nat.ptr(z, ctypes.c_int), d_vals.ptr, None), "sl0")
+    assert not z.any()
+    for b in (d_rec, d_b, d_vals):
+        b.close()
