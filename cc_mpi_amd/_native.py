@@ -96,6 +96,8 @@ def gpu_lib() -> ctypes.CDLL:
          [ctypes.POINTER(CSRGraphC), P_u32, c_int, c_int])
     _sig(lib, "cc_hip_sort_u64_device", c_int,
          [c_void_p, c_void_p, c_u64, ctypes.POINTER(c_int), c_void_p])
+    _sig(lib, "cc_hip_boundary_build_device", c_int,
+         [c_void_p, c_void_p, c_u64, c_void_p, c_u32, P_u64, P_int, c_void_p])
     _sig(lib, "cc_hip_sort_set", c_int, [ctypes.c_char_p, c_int])
     _sig(lib, "cc_hip_sort_get", c_int, [ctypes.c_char_p])
     _sig(lib, "cc_hip_dedup_u64_device", c_int, [c_void_p, c_u64, c_void_p, P_u64, c_void_p])

@@ -65,6 +65,7 @@ void *stream_tmp(StreamState *st, size_t bytes);
 // cc_hip_sort_set.
 extern int g_split_staged;
 extern int g_dedup_staged;
+extern int g_dedup_items;  // items per thread of the dedup compaction (16, 32)
 
 // Returns the context, initialising device 0 on first use.  nullptr on
 // failure (error recorded).
@@ -87,6 +88,17 @@ void prof_end(hipStream_t s);
     if (_e != hipSuccess)                                                          \
       return ccx::set_error(1, "%s failed: %s (%s:%d)", #expr, hipGetErrorString(_e), \
                             __FILE__, __LINE__);                                   \
+  } while (0)
+
+// CCX_LAUNCH for code that must clean up: sets `rc` instead of returning.
+#define CCX_LAUNCH_RC(rc, name, stream, ...)                                       \
+  do {                                                                             \
+    ccx::prof_begin(name, stream);                                                 \
+    __VA_ARGS__;                                                                   \
+    ccx::prof_end(stream);                                                         \
+    hipError_t _e = hipGetLastError();                                             \
+    if (_e != hipSuccess)                                                          \
+      rc = ccx::set_error(1, "launch of %s failed: %s", name, hipGetErrorString(_e)); \
   } while (0)
 
 #define CCX_LAUNCH(name, stream, ...)                                              \

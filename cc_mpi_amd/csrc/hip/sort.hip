@@ -272,6 +272,8 @@ __global__ __launch_bounds__(kThreads) void k_compact(Op op, uint64_t n, uint32_
   __shared__ uint32_t s_tile;
   __shared__ uint32_t s_wsum[kWaves];
   __shared__ uint64_t s_base;
+  constexpr int Items = Op::kItemsPerThread;
+  constexpr int TileK = kThreads * Items;
   const int t = threadIdx.x;
   const int lane = t & (kWave - 1);
   const int w = t / kWave;
@@ -280,12 +282,12 @@ __global__ __launch_bounds__(kThreads) void k_compact(Op op, uint64_t n, uint32_
   const uint64_t tile = s_tile;
   // Tile ids come from a counter zeroed before the launch and run 0..tiles-1;
   // an id past the input (never expected) must not touch memory.
-  if (tile * kTile >= n) return;
-  const uint64_t first = tile * kTile + (uint64_t)t * kItems;
+  if (tile * TileK >= n) return;
+  const uint64_t first = tile * TileK + (uint64_t)t * Items;
   uint32_t keep = 0;
-  uint64_t val[kItems];
+  uint64_t val[Items];
 #pragma unroll
-  for (int k = 0; k < kItems; ++k) {
+  for (int k = 0; k < Items; ++k) {
     const uint64_t j = first + k;
     val[k] = 0;
     if (j < n && op.eval(j, val[k])) keep |= 1u << k;
@@ -310,15 +312,15 @@ __global__ __launch_bounds__(kThreads) void k_compact(Op op, uint64_t n, uint32_
       ccx::status_store(my, ccx::tag_word(epoch, ccx::kFlagInclusive, excl + tile_total));
     }
     s_base = excl;
-    if ((tile + 1) * kTile >= n) *total = excl + tile_total;
+    if ((tile + 1) * TileK >= n) *total = excl + tile_total;
   }
   if constexpr (Op::kStaged) {
     // kept values go to LDS at their tile-local rank, then out as one
     // contiguous run (coalesced stores)
-    __shared__ uint64_t s_out[kTile];
+    __shared__ uint64_t s_out[TileK];
     uint32_t lr = wprefix + incl - cnt;
 #pragma unroll
-    for (int k = 0; k < kItems; ++k)
+    for (int k = 0; k < Items; ++k)
       if ((keep >> k) & 1u) s_out[lr++] = val[k];
     __syncthreads();
     const uint64_t base = s_base;
@@ -328,7 +330,7 @@ __global__ __launch_bounds__(kThreads) void k_compact(Op op, uint64_t n, uint32_
     uint64_t rank = s_base + wprefix + incl - cnt;
     op.begin(first);
 #pragma unroll
-    for (int k = 0; k < kItems; ++k) {
+    for (int k = 0; k < Items; ++k) {
       const uint64_t j = first + k;
       if (j < n) {
         const bool kf = (keep >> k) & 1u;
@@ -340,9 +342,10 @@ __global__ __launch_bounds__(kThreads) void k_compact(Op op, uint64_t n, uint32_
 }
 
 // Adjacent-unique (reference src/boundary_sort.c:100-119).
-template <bool Staged>
+template <bool Staged, int Items = kItems>
 struct DedupOp {
   static constexpr bool kStaged = Staged;
+  static constexpr int kItemsPerThread = Items;
   const uint64_t *in;
   uint64_t *out;
   __device__ bool eval(uint64_t j, uint64_t &v) const {
@@ -364,6 +367,7 @@ int run_dedup(const uint64_t *in, uint64_t *out, uint64_t n, hipStream_t s, ccx:
 // index g.  Records are (remote << 32) | rep.
 struct GhostOp {
   static constexpr bool kStaged = false;
+  static constexpr int kItemsPerThread = kItems;
   const uint64_t *in;
   uint64_t *out;
   uint32_t *ghosts;
@@ -427,6 +431,7 @@ __global__ __launch_bounds__(kThreads) void k_owner_mask(const uint64_t *__restr
 // Ordered compaction of {j : bit q of mask[j]} into vals[] as j + base.
 struct MaskOp {
   static constexpr bool kStaged = true;
+  static constexpr int kItemsPerThread = kItems;
   const unsigned long long *mask;
   uint32_t *vals;
   uint32_t base;
@@ -447,7 +452,8 @@ int run_compact(const char *name, Op op, uint64_t n, hipStream_t s, ccx::StreamS
     *count_out = 0;
     return 0;
   }
-  const uint64_t tiles = tiles_for(n);
+  const uint64_t tile_keys = (uint64_t)kThreads * Op::kItemsPerThread;
+  const uint64_t tiles = (n + tile_keys - 1) / tile_keys;
   if (tiles > 0xFFFFFFFFull) return ccx::set_error(4, "%s: input too large", name);
   if (int rc = ccx::ensure_status(st, tiles)) return rc;
   uint32_t epoch;
@@ -463,10 +469,70 @@ int run_compact(const char *name, Op op, uint64_t n, hipStream_t s, ccx::StreamS
   return 0;
 }
 
+// ---- boundary build with compressed reps ----------------------------------
+// The reps of a rank's boundary records are roots of its local forest, of
+// which there are usually few (one per local component touching the
+// boundary, often a few thousand against 2^24 ids).  Replacing each rep by
+// its rank among the roots keeps the key order and equality (the map is
+// increasing) while the key shrinks from 32 + 32 bits to 32 + ceil(log2
+// roots), which drops whole digit passes from the sort; the dedup writes
+// the records back with the original reps.
+__global__ __launch_bounds__(kThreads) void k_root_flags(const uint32_t *__restrict__ comp_of,
+                                                         uint32_t n, uint32_t *__restrict__ flags) {
+  for (uint64_t i = (uint64_t)blockIdx.x * kThreads + threadIdx.x; i < n;
+       i += (uint64_t)gridDim.x * kThreads)
+    flags[i] = comp_of[i] == (uint32_t)i;
+}
+
+__global__ __launch_bounds__(kThreads) void k_rep_tables(const uint32_t *__restrict__ comp_of,
+                                                         uint32_t n, const uint64_t *__restrict__ scan,
+                                                         uint32_t *__restrict__ packed,
+                                                         uint32_t *__restrict__ rep_of_rank) {
+  for (uint64_t i = (uint64_t)blockIdx.x * kThreads + threadIdx.x; i < n;
+       i += (uint64_t)gridDim.x * kThreads) {
+    const uint32_t c = comp_of[i];
+    packed[i] = (uint32_t)scan[c < n ? c : (uint32_t)i];
+    if (c == (uint32_t)i) rep_of_rank[scan[i]] = (uint32_t)i;
+  }
+}
+
+__global__ __launch_bounds__(kThreads) void k_set_rep_packed(uint64_t *__restrict__ rec, uint64_t nb,
+                                                             const uint32_t *__restrict__ packed,
+                                                             uint32_t n, int bits) {
+  for (uint64_t i = (uint64_t)blockIdx.x * kThreads + threadIdx.x; i < nb;
+       i += (uint64_t)gridDim.x * kThreads) {
+    const uint64_t r = rec[i];
+    const uint32_t li = (uint32_t)r;
+    rec[i] = ((r >> 32) << bits) | (li < n ? packed[li] : 0u);
+  }
+}
+
+// Adjacent-unique over packed keys, writing (remote << 32) | rep.
+template <int Items = kItems>
+struct DedupUnpackOp {
+  static constexpr bool kStaged = false;
+  static constexpr int kItemsPerThread = Items;
+  const uint64_t *in;
+  uint64_t *out;
+  const uint32_t *rep_of_rank;
+  int bits;
+  __device__ bool eval(uint64_t j, uint64_t &v) const {
+    v = in[j];
+    return j == 0 || v != in[j - 1];
+  }
+  __device__ void store(uint64_t, uint64_t) const {}
+  __device__ void begin(uint64_t) {}
+  __device__ void apply(uint64_t, bool keep, uint64_t rank, uint64_t v) const {
+    if (keep) out[rank] = ((v >> bits) << 32) | rep_of_rank[(uint32_t)(v & ((1ull << bits) - 1))];
+  }
+};
+
 int run_dedup(const uint64_t *in, uint64_t *out, uint64_t n, hipStream_t s, ccx::StreamState *st,
               uint64_t *count_out) {
   if (ccx::g_dedup_staged)
     return run_compact("dedup_compact", DedupOp<true>{in, out}, n, s, st, count_out);
+  if (ccx::g_dedup_items == 32)
+    return run_compact("dedup_compact", DedupOp<false, 32>{in, out}, n, s, st, count_out);
   return run_compact("dedup_compact", DedupOp<false>{in, out}, n, s, st, count_out);
 }
 
@@ -477,6 +543,7 @@ int run_dedup(const uint64_t *in, uint64_t *out, uint64_t n, hipStream_t s, ccx:
 namespace ccx {
 int g_split_staged = 0;
 int g_dedup_staged = 0;
+int g_dedup_items = 32;
 }  // namespace ccx
 
 namespace {
@@ -497,6 +564,10 @@ extern "C" int cc_hip_sort_set(const char *key, int value) {
     ccx::g_dedup_staged = value;
     return 0;
   }
+  if (key && strcmp(key, "dedup_items") == 0 && (value == 16 || value == 32)) {
+    ccx::g_dedup_items = value;
+    return 0;
+  }
   return ccx::set_error(4, "cc_hip_sort_set: bad key or value (%s=%d)", key ? key : "", value);
 }
 
@@ -504,6 +575,7 @@ extern "C" int cc_hip_sort_get(const char *key) {
   if (key && strcmp(key, "items_per_thread") == 0) return g_sort_items;
   if (key && strcmp(key, "split_staged") == 0) return ccx::g_split_staged;
   if (key && strcmp(key, "dedup_staged") == 0) return ccx::g_dedup_staged;
+  if (key && strcmp(key, "dedup_items") == 0) return ccx::g_dedup_items;
   return -1;
 }
 
@@ -683,6 +755,73 @@ extern "C" int cc_hip_send_lists_device(const uint64_t *d_records, uint64_t coun
     h_starts[q + 1] = (int)total;
   }
   return 0;
+}
+
+extern "C" int cc_hip_boundary_build_device(uint64_t *d_records, uint64_t *d_alt, uint64_t count,
+                                            const uint32_t *d_comp_of, uint32_t n_local,
+                                            uint64_t *out_count, int *result_in_alt,
+                                            void *stream) {
+  ccx::Context *c = ccx::ctx();
+  if (!c) return 2;
+  if (!out_count || !result_in_alt)
+    return ccx::set_error(4, "cc_hip_boundary_build_device: null output");
+  *out_count = 0;
+  *result_in_alt = 0;
+  if (count == 0) return 0;
+  if (!d_records || !d_alt || !d_comp_of || n_local == 0)
+    return ccx::set_error(4, "cc_hip_boundary_build_device: null pointer or no local vertices");
+  hipStream_t s = ccx::pick_stream(stream);
+  ccx::StreamState *st = ccx::stream_state(s);
+  if (!st) return 3;
+  // tables: flags/packed u32[n], scan u64[n + 1], rep_of_rank u32[n]
+  const size_t n = n_local;
+  char *tab = nullptr;
+  const size_t bytes = n * 4 + (n + 1) * 8 + n * 4 + 256;
+  if (hipMalloc(&tab, bytes) != hipSuccess)
+    return ccx::set_error(3, "cc_hip_boundary_build_device: hipMalloc of %zu bytes failed", bytes);
+  uint32_t *flags = (uint32_t *)tab;
+  uint64_t *scan = (uint64_t *)(tab + ((n * 4 + 7) & ~(size_t)7));
+  uint32_t *rep_of_rank = (uint32_t *)(scan + n + 1);
+  int rc = 0;
+  do {
+    const unsigned g = (unsigned)((n + kThreads - 1) / kThreads < 8192 ? (n + kThreads - 1) / kThreads : 8192);
+    CCX_LAUNCH_RC(rc, "rep_flags", s, k_root_flags<<<g, kThreads, 0, s>>>(d_comp_of, n_local, flags));
+    if (rc) break;
+    if ((rc = cc_hip_exclusive_scan_u32_device(flags, n, scan, s))) break;
+    // flags is rewritten in place as the packed rep of every local vertex
+    CCX_LAUNCH_RC(rc, "rep_tables", s,
+                  k_rep_tables<<<g, kThreads, 0, s>>>(d_comp_of, n_local, scan, flags, rep_of_rank));
+    if (rc) break;
+    if (hipMemcpyAsync(st->h_stage, scan + n, 8, hipMemcpyDeviceToHost, s) != hipSuccess ||
+        hipStreamSynchronize(s) != hipSuccess) {
+      rc = ccx::set_error(1, "cc_hip_boundary_build_device: reading the root count failed");
+      break;
+    }
+    const uint64_t roots = st->h_stage[0];
+    int bits = 1;
+    while (bits < 32 && (1ull << bits) < roots) ++bits;
+    const uint64_t gb = (count + kThreads - 1) / kThreads;
+    CCX_LAUNCH_RC(rc, "boundary_set_rep", s,
+                  k_set_rep_packed<<<(unsigned)(gb < 65536 ? gb : 65536), kThreads, 0, s>>>(
+                      d_records, count, flags, n_local, bits));
+    if (rc) break;
+    int in_alt = 0;
+    if ((rc = ccx::sort_u64(d_records, d_alt, count, &in_alt, s))) break;
+    uint64_t *sorted = in_alt ? d_alt : d_records, *other = in_alt ? d_records : d_alt;
+    uint64_t kept = 0;
+    if (ccx::g_dedup_items == 32)
+      rc = run_compact("dedup_compact", DedupUnpackOp<32>{sorted, other, rep_of_rank, bits}, count,
+                       s, st, &kept);
+    else
+      rc = run_compact("dedup_compact", DedupUnpackOp<>{sorted, other, rep_of_rank, bits}, count, s,
+                       st, &kept);
+    if (rc) break;
+    *out_count = kept;
+    *result_in_alt = (other == d_alt);
+  } while (0);
+  (void)hipStreamSynchronize(s);
+  (void)hipFree(tab);
+  return rc;
 }
 
 // ---- host drop-ins (include/boundary_sort.h) --------------------------------

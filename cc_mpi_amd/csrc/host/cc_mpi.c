@@ -134,18 +134,19 @@ void compute_connected_components_mpi_advanced(const DistCSRGraph *Gd, uint32_t 
   uint64_t nbd = 0;
   if (nb > 0)
   {
-    check(cc_hip_boundary_set_rep_device(d_rec, nb, d_comp_of, NULL), comm, "rep fill");
+    /* rep fill + sort + dedup in one call (src/cc_mpi.c:364-394,
+     * src/boundary_sort.c:93-119) */
     uint64_t *d_alt = (uint64_t *)dalloc((size_t)nb * 8, comm, "alloc sort buffer");
     int in_alt = 0;
-    check(cc_hip_sort_u64_device(d_rec, d_alt, nb, &in_alt, NULL), comm, "boundary sort");
-    uint64_t *sorted = in_alt ? d_alt : d_rec, *other = in_alt ? d_rec : d_alt;
-    check(cc_hip_dedup_u64_device(sorted, nb, other, &nbd, NULL), comm, "boundary dedup");
-    d_ghosts = (uint32_t *)dalloc((size_t)nbd * 4, comm, "alloc ghosts");
+    check(cc_hip_boundary_build_device(d_rec, d_alt, nb, d_comp_of, n_local, &nbd, &in_alt, NULL),
+          comm, "boundary build");
+    uint64_t *deduped = in_alt ? d_alt : d_rec, *other = in_alt ? d_rec : d_alt;
+    d_ghosts = (uint32_t *)dalloc((size_t)(nbd ? nbd : 1) * 4, comm, "alloc ghosts");
     uint64_t gc = 0;
-    check(cc_hip_ghost_extract_device(other, nbd, sorted, d_ghosts, &gc, NULL), comm, "ghosts");
+    check(cc_hip_ghost_extract_device(deduped, nbd, other, d_ghosts, &gc, NULL), comm, "ghosts");
     ghost_count = (uint32_t)gc;
-    d_bnd = sorted;
-    cc_hip_free(other);
+    d_bnd = other;
+    cc_hip_free(deduped);
     ghost_vertices = (uint32_t *)malloc((size_t)(ghost_count ? ghost_count : 1) * 4);
     if (!ghost_vertices)
       MPI_Abort(comm, EXIT_FAILURE);

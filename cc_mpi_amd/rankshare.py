@@ -25,7 +25,7 @@ import time
 import numpy as np
 
 from . import _native as nat
-from .boundary import PinnedBuffer, delta_pack_device, sort_device, dedup_device, ghost_extract_device
+from .boundary import PinnedBuffer, delta_pack_device, ghost_extract_device
 from .cc import DeviceBuffer, DeviceGraph
 
 
@@ -97,24 +97,23 @@ class RankShare:
         self._t("send_lists", t0)
         keys = DeviceBuffer(max(8, 8 * self.nb))
 
-        # boundary records: rep fill, sort, dedup, ghost extraction
+        # boundary records: rep fill, sort and dedup in one call (reps sorted
+        # by their rank among the local roots), then ghost extraction
         t0 = time.perf_counter()
-        nat.check(lib.cc_hip_boundary_set_rep_device(rec.ptr, self.nb, self.comp_of.ptr, None),
-                  "rep fill")
-        which = sort_device(rec, keys, self.nb)
-        srt, oth = (keys, rec) if which else (rec, keys)
-        self._t("boundary_sort", t0)
-        t0 = time.perf_counter()
-        self.nbd = dedup_device(srt, self.nb, oth)
-        self._t("boundary_dedup", t0)
+        kept, in_alt = ctypes.c_uint64(), ctypes.c_int()
+        nat.check(lib.cc_hip_boundary_build_device(rec.ptr, keys.ptr, self.nb, self.comp_of.ptr, nl,
+                                                   ctypes.byref(kept), ctypes.byref(in_alt), None),
+                  "boundary build")
+        self.nbd = int(kept.value)
+        srt, oth = (keys, rec) if in_alt.value else (rec, keys)
+        self._t("boundary_build", t0)
         t0 = time.perf_counter()
         self.ghosts = DeviceBuffer(4 * max(1, self.nbd))
-        self.ng = ghost_extract_device(oth, self.nbd, srt, self.ghosts)
-        self.bnd = srt
+        self.ng = ghost_extract_device(srt, self.nbd, oth, self.ghosts)
+        self.bnd = oth
         self._t("ghost_extract", t0)
-        for b in (oth, d_bounds):
-            if b is not srt:
-                b.close()
+        for b in (srt, d_bounds):
+            b.close()
         self.stats.update(n_local=nl, m_local_local=self.m_ll, boundary_raw=self.nb,
                           boundary_dedup=self.nbd, ghosts=self.ng, total_send_to=nsend,
                           outdegree=len(self.dests))

@@ -139,12 +139,28 @@ int cc_hip_components_host(const uint64_t *row_ptr, const uint32_t *col_idx,
 int cc_hip_sort_u64_device(uint64_t *d_keys, uint64_t *d_alt, uint64_t count,
                            int *result_in_alt, void *stream);
 
+/* Rep fill, sort and dedup of a rank's raw boundary records in one call:
+ * (remote << 32) | li  ->  ascending unique (remote << 32) | d_comp_of[li],
+ * the result of cc_hip_boundary_set_rep_device + cc_hip_sort_u64_device +
+ * cc_hip_dedup_u64_device (src/cc_mpi.c:364-394, src/boundary_sort.c:93-119).
+ * d_comp_of[0..n_local) must map every local vertex to a root r of its
+ * local forest (d_comp_of[r] == r), as cc_hip_components_device leaves it.
+ * Reps are sorted by their rank among the roots, which keeps the order and
+ * shortens the key to 32 + ceil(log2 roots) bits (fewer digit passes).
+ * *out_count receives the kept count; the records are in d_alt when
+ * *result_in_alt is 1, else in d_records; the other buffer is clobbered.
+ * Blocking. */
+int cc_hip_boundary_build_device(uint64_t *d_records, uint64_t *d_alt, uint64_t count,
+                                 const uint32_t *d_comp_of, uint32_t n_local, uint64_t *out_count,
+                                 int *result_in_alt, void *stream);
+
 /* Tuning of the sort's scatter passes: "items_per_thread" (8, 16, 24 or 32
  * (default) keys per thread; a workgroup's tile is 256 x that); and of the
  * split and dedup compactions: "split_staged" / "dedup_staged" (0 default:
  * each thread stores its own items; 1: outputs staged in LDS and stored as
  * contiguous runs, which measured slower at 638M records because the LDS
- * costs occupancy).  The result never depends on them.  _set returns nonzero for a bad key or value, _get -1
+ * costs occupancy); "dedup_items" (16 or 32 (default) records per thread
+ * in the dedup compaction).  The result never depends on them.  _set returns nonzero for a bad key or value, _get -1
  * for an unknown key. */
 int cc_hip_sort_set(const char *key, int value);
 int cc_hip_sort_get(const char *key);

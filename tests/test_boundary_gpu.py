@@ -30,18 +30,20 @@ def test_dropin_sort_matches_oracle(gpu, count):
     assert np.array_equal(got, oracle.sort_u64(k))
 
 
-@pytest.mark.parametrize("staged", [0, 1])
+@pytest.mark.parametrize("staged,items", [(0, 16), (1, 16), (0, 32)])
 @pytest.mark.parametrize("count", SIZES)
-def test_dropin_dedup_matches_oracle(gpu, count, staged):
+def test_dropin_dedup_matches_oracle(gpu, count, staged, items):
     from cc_mpi_amd import _native as nat
     lib = nat.gpu_lib()
     k = oracle.sort_u64(records(count, seed=count + 1, rep_bits=8, remote_bits=9))
     got = k.copy()
     try:
         assert lib.cc_hip_sort_set(b"dedup_staged", staged) == 0
+        assert lib.cc_hip_sort_set(b"dedup_items", items) == 0
         n = bd.boundary_edges_dedup(got)
     finally:
         lib.cc_hip_sort_set(b"dedup_staged", 0)
+        lib.cc_hip_sort_set(b"dedup_items", 32)
     want = oracle.dedup_u64(k)
     assert n == want.size
     assert np.array_equal(got[:n], want)
@@ -194,3 +196,41 @@ def test_delta_pack_nothing_to_send(gpu):
     sendbuf = bd.PinnedBuffer(stf.size)
     pairs, cnt, dis = bd.delta_pack_device(d[0], stf.size, seg, d[1], d[2], vs, ve, d[3], sendbuf)
     assert pairs.size == 0 and cnt.sum() == 0 and (dis == 0).all()
+
+
+@pytest.mark.parametrize("items", [16, 32])
+@pytest.mark.parametrize("roots", [1, 2, 3, 1000, 65537, "all"])
+def test_boundary_build_matches_three_steps(gpu, roots, items):
+    """cc_hip_boundary_build_device (reps packed by their rank among the
+    local roots) gives the bytes of rep fill -> sort -> dedup."""
+    import ctypes
+    from cc_mpi_amd import _native as nat
+    lib = nat.gpu_lib()
+    rng = np.random.default_rng(11 if roots == "all" else roots)
+    nl = 200_000
+    if roots == "all":
+        comp_of = np.arange(nl, dtype=np.uint32)
+    else:
+        rts = np.sort(rng.choice(nl, roots, replace=False)).astype(np.uint32)
+        comp_of = rts[rng.integers(0, roots, nl)]
+        comp_of[rts] = rts  # every root is its own rep
+    count = 3_000_017
+    li = rng.integers(0, nl, count, dtype=np.uint64)
+    remote = rng.integers(0, 1 << 26, count, dtype=np.uint64)
+    rec = (remote << np.uint64(32)) | li
+    want = oracle.dedup_u64(oracle.sort_u64((remote << np.uint64(32)) | comp_of[li].astype(np.uint64)))
+    d_rec, d_alt = DeviceBuffer.from_array(rec), DeviceBuffer(rec.nbytes)
+    d_comp = DeviceBuffer.from_array(comp_of)
+    kept, in_alt = ctypes.c_uint64(), ctypes.c_int()
+    assert lib.cc_hip_sort_set(b"dedup_items", items) == 0
+    try:
+        nat.check(lib.cc_hip_boundary_build_device(d_rec.ptr, d_alt.ptr, count, d_comp.ptr, nl,
+                                                   ctypes.byref(kept), ctypes.byref(in_alt), None),
+                  "bb")
+    finally:
+        lib.cc_hip_sort_set(b"dedup_items", 32)
+    got = np.empty(int(kept.value), dtype=np.uint64)
+    (d_alt if in_alt.value else d_rec).download(got)
+    assert np.array_equal(got, want)
+    for b in (d_rec, d_alt, d_comp):
+        b.close()
