@@ -78,6 +78,7 @@ def frontier_stats(G):
         has = np.flatnonzero(deg > r)
         src.append(has)
         dst.append(G.col_idx[G.row_ptr[has].astype(np.int64) + r].astype(np.int64))
+    src0, dst0 = src[0], dst[0]
     src = np.concatenate(src)
     dst = np.concatenate(dst)
     A = sp.coo_matrix((np.ones(src.size, dtype=np.int8), (src, dst)), shape=(n, n)).tocsr()
@@ -97,9 +98,16 @@ def frontier_stats(G):
     active = front & (rem > 0)
     has0 = np.flatnonzero(deg > 0)
     up0 = int(np.count_nonzero(G.col_idx[G.row_ptr[has0].astype(np.int64)] > has0))
+    # forest after round 0 alone (neighbour 0 of every vertex) and its
+    # sampled giant: round 1 of the two-pass plan skips that tree
+    A0 = sp.coo_matrix((np.ones(src0.size, dtype=np.int8), (src0, dst0)), shape=(n, n)).tocsr()
+    _, comp0 = connected_components(A0, directed=False)
+    v0, c0 = np.unique(comp0[pos], return_counts=True)
+    live1 = int(np.count_nonzero((deg > 1) & (comp0 != v0[np.argmax(c0)])))
     return dict(frontier_vertices=int(active.sum()), frontier_edges=int(rem[active].sum()),
                 giant_fraction=float((~front).mean()),
-                deg_gt0=int((deg > 0).sum()), deg_gt1=int((deg > 1).sum()), up0=up0)
+                deg_gt0=int((deg > 0).sum()), deg_gt1=int((deg > 1).sum()), up0=up0,
+                round1_outside_early_giant=live1)
 
 
 def kernel_bytes(name: str, n: int, st: dict, plan: dict) -> float | None:
@@ -123,6 +131,14 @@ def kernel_bytes(name: str, n: int, st: dict, plan: dict) -> float | None:
         if plan["fused"]:
             # both sample arrays, own parent, one parent gather per hooked neighbour
             return 8.0 * n + 4.0 * n + 4.0 * (up0 + d1)
+        if plan["cache"] and plan.get("giant_early"):
+            # round 0: sample array 0, own parent, one gather per hooked
+            # neighbour; round 1: own parent (the skip test), then sample 1
+            # and one gather for each vertex outside the early giant tree;
+            # averaged over the two launches
+            r0 = 4.0 * n + 4.0 * n + 4.0 * up0
+            r1 = 4.0 * n + 8.0 * st["round1_outside_early_giant"]
+            return (r0 + r1) / 2.0
         if plan["cache"]:
             # per round: one sample array, own parent, one parent gather per
             # hooked neighbour; averaged over the two launches
@@ -146,8 +162,11 @@ def solve_plan(lib, n: int, m: int) -> dict:
     from cc_mpi_amd import _native as nat
     vals = [ctypes.c_int() for _ in range(4)]
     nat.check(lib.cc_hip_components_plan(n, m, *[ctypes.byref(v) for v in vals]), "plan")
-    return dict(zip(("cache", "fused", "hook_ilp", "hook_blocks_per_cu"),
+    plan = dict(zip(("cache", "fused", "hook_ilp", "hook_blocks_per_cu"),
                     (int(v.value) for v in vals)))
+    plan["giant_early"] = int(plan["cache"] and not plan["fused"]
+                              and lib.cc_hip_components_get(b"giant_early") == 1)
+    return plan
 
 
 def cpu_baseline(G, budget_s: float, runs: int = 3, impl: str = "reference"):

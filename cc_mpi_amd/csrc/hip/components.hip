@@ -719,7 +719,9 @@ extern "C" int cc_hip_components_device(const uint64_t *d_row_ptr, const uint32_
     const uint32_t *s1 = smp + n;
     if (mode == 0)
       CCX_LAUNCH("cc_hook_sample", s, k_hook_pair<0, 2><<<nb, kBlock, 0, s>>>(smp, s1, n, d_labels));
-    else if (hook_ilp >= 2)
+    else if (hook_ilp == 4)
+      CCX_LAUNCH("cc_hook_sample", s, k_hook_pair<1, 4><<<nb, kBlock, 0, s>>>(smp, s1, n, d_labels));
+    else if (hook_ilp == 2)
       CCX_LAUNCH("cc_hook_sample", s, k_hook_pair<1, 2><<<nb, kBlock, 0, s>>>(smp, s1, n, d_labels));
     else
       CCX_LAUNCH("cc_hook_sample", s, k_hook_pair<1, 1><<<nb, kBlock, 0, s>>>(smp, s1, n, d_labels));
