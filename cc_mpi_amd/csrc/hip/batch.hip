@@ -7,7 +7,8 @@
 // graph and a thousand 1K-vertex graphs fills the chip with no per-graph
 // launches.  The stages are those of the single-graph solve
 // (components.hip): parent = min(v, first neighbour), exact sampled hooking
-// of neighbours 0 and 1 with L2-confirmed roots, pointer jumping, the most
+// of neighbours 0 and 1 with L2-confirmed roots (in one pass by default,
+// else a pass per neighbour with pointer jumping between), the most
 // frequent root of each graph (one workgroup per graph), the frontier sweep
 // of every vertex outside its graph's giant tree, pointer jumping.  Labels
 // are per graph: the minimum vertex id of the component inside that graph.
@@ -103,6 +104,23 @@ __device__ __forceinline__ void unite(uint32_t a, uint32_t b, uint32_t *p) {
   }
 }
 
+// unite() continued from two known (possibly stale) roots.
+__device__ __forceinline__ void unite_roots(uint32_t ra, uint32_t rb, uint32_t *p) {
+  while (ra != rb) {
+    const uint32_t hi = ra > rb ? ra : rb, lo = ra > rb ? rb : ra;
+    const uint32_t cur = load_l2(&p[hi]);
+    if (cur != hi) {
+      ra = root_of_l2(cur, p);
+      rb = root_of_l2(lo, p);
+      continue;
+    }
+    const uint32_t old = atomicMin(&p[hi], lo);
+    if (old == hi || old == lo) return;
+    ra = root_of_l2(old, p);
+    rb = root_of_l2(lo, p);
+  }
+}
+
 #define ITEM_OR_EXIT()                                   \
   if (blockIdx.x >= hdr->n_items) return;                \
   const WorkItem it = items[blockIdx.x];                 \
@@ -133,6 +151,44 @@ __global__ __launch_bounds__(kBlock) void kb_hook_neighbour(const Header *__rest
     if (b >= gd.row_ptr[v + 1]) continue;
     const uint32_t w = gd.col[b];
     if (w < gd.n && !(r == 0 && w < v)) unite(v, w, gd.labels);
+  }
+}
+
+// Both sampled neighbours in one pass: v's root and the roots of
+// neighbours 0 and 1 are walked in lockstep, then v's tree is hooked with
+// each (neighbour 0 only when it is above v; init linked it otherwise).
+// One pass and one pointer-jumping launch fewer than a round per neighbour.
+__global__ __launch_bounds__(kBlock) void kb_hook_pair(const Header *__restrict__ hdr,
+                                                       const GraphDesc *__restrict__ graphs,
+                                                       const WorkItem *__restrict__ items) {
+  ITEM_OR_EXIT();
+  uint32_t *p = gd.labels;
+  for (uint32_t v = it.v_begin + threadIdx.x; v < it.v_end; v += kBlock) {
+    const uint64_t b = gd.row_ptr[v], e = gd.row_ptr[v + 1];
+    const uint32_t w0 = b < e ? gd.col[b] : 0xFFFFFFFFu;
+    const uint32_t w1 = b + 1 < e ? gd.col[b + 1] : 0xFFFFFFFFu;
+    const bool a0 = w0 < gd.n && w0 > v;
+    const bool a1 = w1 < gd.n && w1 != v;
+    if (!a0 && !a1) continue;
+    uint32_t x[3] = {v, a0 ? w0 : v, a1 ? w1 : v};
+    bool more = true;
+    while (more) {  // the three root walks in lockstep
+      more = false;
+#pragma unroll
+      for (int k = 0; k < 3; ++k) {
+        const uint32_t px = p[x[k]];
+        if (px != x[k]) {
+          x[k] = px;
+          more = true;
+        }
+      }
+    }
+    uint32_t rv = x[0];
+    if (a0 && rv != x[1]) {
+      unite_roots(rv, x[1], p);
+      rv = rv < x[1] ? rv : x[1];  // a member of the merged tree
+    }
+    if (a1 && rv != x[2]) unite_roots(rv, x[2], p);
   }
 }
 
@@ -262,7 +318,17 @@ __global__ __launch_bounds__(kBlock) void kb_hook_frontier(const Header *__restr
   }
 }
 
+int g_batch_one_pass = 1;  // hook both sampled neighbours in one pass (read at create)
+
 }  // namespace
+
+extern "C" int cc_hip_batch_set(const char *key, int value) {
+  if (key && strcmp(key, "one_pass") == 0 && (value == 0 || value == 1)) {
+    g_batch_one_pass = value;
+    return 0;
+  }
+  return ccx::set_error(4, "cc_hip_batch_set: bad key or value (%s=%d)", key ? key : "", value);
+}
 
 struct cc_hip_batch_s {
   uint32_t max_items = 0, max_graphs = 0;
@@ -304,9 +370,13 @@ extern "C" int cc_hip_batch_create(uint32_t max_items, uint32_t max_graphs, cc_h
   hipError_t e = hipMemcpyAsync(b->d_tables, b->h_tables, b->table_bytes, hipMemcpyHostToDevice, s);
   if (e == hipSuccess) {
     kb_init_min<<<max_items, kBlock, 0, s>>>(hdr, graphs, items);
-    kb_hook_neighbour<<<max_items, kBlock, 0, s>>>(hdr, graphs, items, 0u);
-    kb_compress<<<max_items, kBlock, 0, s>>>(hdr, graphs, items);
-    kb_hook_neighbour<<<max_items, kBlock, 0, s>>>(hdr, graphs, items, 1u);
+    if (g_batch_one_pass) {
+      kb_hook_pair<<<max_items, kBlock, 0, s>>>(hdr, graphs, items);
+    } else {
+      kb_hook_neighbour<<<max_items, kBlock, 0, s>>>(hdr, graphs, items, 0u);
+      kb_compress<<<max_items, kBlock, 0, s>>>(hdr, graphs, items);
+      kb_hook_neighbour<<<max_items, kBlock, 0, s>>>(hdr, graphs, items, 1u);
+    }
     kb_sample_giant<<<max_graphs, kBlock, 0, s>>>(hdr, graphs);
     kb_hook_frontier<<<max_items, kBlock, 0, s>>>(hdr, graphs, items, 2u);
     kb_compress<<<max_items, kBlock, 0, s>>>(hdr, graphs, items);

@@ -38,6 +38,8 @@ def _lib():
         P = ctypes.c_void_p
         lib.cc_hip_batch_create.restype = ctypes.c_int
         lib.cc_hip_batch_create.argtypes = [ctypes.c_uint32, ctypes.c_uint32, ctypes.POINTER(P)]
+        lib.cc_hip_batch_set.restype = ctypes.c_int
+        lib.cc_hip_batch_set.argtypes = [ctypes.c_char_p, ctypes.c_int]
         lib.cc_hip_batch_destroy.restype = ctypes.c_int
         lib.cc_hip_batch_destroy.argtypes = [P]
         lib.cc_hip_batch_launch.restype = ctypes.c_int

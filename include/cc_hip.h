@@ -297,6 +297,10 @@ typedef struct cc_hip_batch_s *cc_hip_batch_t;
  * mirrors and an instantiated HIP graph of the whole solve. */
 int cc_hip_batch_create(uint32_t max_items, uint32_t max_graphs, cc_hip_batch_t *out);
 int cc_hip_batch_destroy(cc_hip_batch_t b);
+/* Solve shape captured by engines created afterwards: "one_pass" 1
+ * (default) hooks both sampled neighbours in one pass, 0 a pass per
+ * neighbour with pointer jumping between.  Results never depend on it. */
+int cc_hip_batch_set(const char *key, int value);
 /*
  * Solve n_graphs resident graphs (symmetric CSRs d_row_ptr[g], d_col_idx[g]
  * of n[g] vertices) into d_labels[g] (per-graph minimum ids) with one graph

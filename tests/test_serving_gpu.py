@@ -23,12 +23,18 @@ def oracle_of(g):
     return oracle.cc_min_label(h.row_ptr, h.col_idx)
 
 
+@pytest.mark.parametrize("one_pass", [1, 0])
 @pytest.mark.parametrize("item_vertices", [256, 1024, 65536])
-def test_batch_equals_alone_equals_oracle(gpu, item_vertices):
+def test_batch_equals_alone_equals_oracle(gpu, item_vertices, one_pass):
     gs = mixed_graphs()
     want = [oracle_of(g) for g in gs]
     alone = [g.solve() for g in gs]
-    eng = serving.BatchEngine(max_vertices=1 << 20, max_items=4096, max_graphs=64)
+    lib = serving._lib()
+    assert lib.cc_hip_batch_set(b"one_pass", one_pass) == 0
+    try:
+        eng = serving.BatchEngine(max_vertices=1 << 20, max_items=4096, max_graphs=64)
+    finally:
+        lib.cc_hip_batch_set(b"one_pass", 1)
     # several replays of the captured graph with different contents, with
     # eager work in between
     for order in ([0, 1, 2, 3, 4, 5, 6], [6, 5, 4], [2, 2, 0, 3], [1]):
