@@ -158,15 +158,9 @@ def kernel_bytes(name: str, n: int, st: dict, plan: dict) -> float | None:
 
 def solve_plan(lib, n: int, m: int) -> dict:
     """What the solve does on this graph (cc_hip_components_plan)."""
-    import ctypes
-    from cc_mpi_amd import _native as nat
-    vals = [ctypes.c_int() for _ in range(4)]
-    nat.check(lib.cc_hip_components_plan(n, m, *[ctypes.byref(v) for v in vals]), "plan")
-    plan = dict(zip(("cache", "fused", "hook_ilp", "hook_blocks_per_cu"),
-                    (int(v.value) for v in vals)))
-    plan["giant_early"] = int(plan["cache"] and not plan["fused"]
-                              and lib.cc_hip_components_get(b"giant_early") == 1)
-    return plan
+    keys = ("cache", "fused", "halve", "hook_ilp", "hook_blocks_per_cu", "giant_early",
+            "hook_mode")
+    return {k: int(lib.cc_hip_components_plan_get(n, m, k.encode())) for k in keys}
 
 
 def cpu_baseline(G, budget_s: float, runs: int = 3, impl: str = "reference"):
@@ -241,6 +235,7 @@ def serving_leg(args) -> dict:
     stats = srv.run(reqs)
     batched = serving.latency_summary(reqs, entries)
     batched["batches"] = stats["batches"]
+    batched["direct_batches"] = stats["direct_batches"]
     log(f"batched: {batched}")
     srv.close()
     base = serving.poisson_requests(args.serving_requests, args.serving_qps, keys, seed=7)
@@ -342,6 +337,8 @@ def main() -> int:
     ap.add_argument("--cpu-budget", type=float, default=30.0,
                     help="seconds of CPU work for the cpu_baseline leg")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--cpu-grid-side", type=int, default=2048,
+                    help="lattice side of the CPU leg's sample for the grid workload")
     ap.add_argument("--serving-requests", type=int, default=3000,
                     help="requests in the batched-serving leg (0 = skip; 10000 = the full stream)")
     ap.add_argument("--serving-qps", type=float, default=8000.0)
@@ -444,7 +441,22 @@ def main() -> int:
 
     cpu = None
     parity = None
-    if not args.no_cpu_baseline:
+    w = WORKLOADS[args.workload]
+    side = args.cpu_grid_side
+    if not args.no_cpu_baseline and w["kind"] == "grid" and side < w["rows"]:
+        # Label propagation needs on the order of the lattice's diameter in
+        # rounds (hours at 8192^2), so the CPU leg times a side x side
+        # lattice from the same generator (same keep fraction and seed) and
+        # the full graph's labels are checked against the oracle.
+        import oracle
+        sample = cm.grid(side, side, w["keep"], seed=w["seed"])
+        log(f"timing the CPU baseline on a {side}x{side} sample lattice ...")
+        _, cpu = cpu_baseline(sample, args.cpu_budget, args.cpu_runs, args.cpu_impl)
+        cpu["sample"] = (f"{side}x{side} lattice from the same generator (keep {w['keep']}, "
+                         f"seed {w['seed']}), n={sample.n} m={sample.m}: " + cpu["sample"])
+        cpu.pop("seconds")
+        parity = bool(np.array_equal(oracle.cc_min_label(G.row_ptr, G.col_idx), labels))
+    elif not args.no_cpu_baseline:
         log("timing the CPU baseline ...")
         ref_labels, cpu = cpu_baseline(G, args.cpu_budget, args.cpu_runs, args.cpu_impl)
         parity = bool(np.array_equal(ref_labels, labels))

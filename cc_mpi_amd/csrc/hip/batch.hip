@@ -66,15 +66,6 @@ __device__ __forceinline__ uint32_t root_of(uint32_t x, const uint32_t *p) {
   return x;
 }
 
-__device__ __forceinline__ uint32_t root_of_l2(uint32_t x, const uint32_t *p) {
-  uint32_t px = load_l2(&p[x]);
-  while (px != x) {
-    x = px;
-    px = load_l2(&p[x]);
-  }
-  return x;
-}
-
 __device__ __forceinline__ uint32_t root_via(uint32_t x, const uint32_t *p, uint32_t g, bool &hit) {
   bool h = (x == g);
   uint32_t px = p[x];
@@ -87,38 +78,28 @@ __device__ __forceinline__ uint32_t root_via(uint32_t x, const uint32_t *p, uint
   return x;
 }
 
-__device__ __forceinline__ void unite(uint32_t a, uint32_t b, uint32_t *p) {
-  uint32_t ra = root_of(a, p), rb = root_of(b, p);
-  while (ra != rb) {
-    const uint32_t hi = ra > rb ? ra : rb, lo = ra > rb ? rb : ra;
-    const uint32_t cur = load_l2(&p[hi]);
-    if (cur != hi) {
-      ra = root_of_l2(cur, p);
-      rb = root_of_l2(lo, p);
-      continue;
-    }
-    const uint32_t old = atomicMin(&p[hi], lo);
-    if (old == hi || old == lo) return;
-    ra = root_of_l2(old, p);
-    rb = root_of_l2(lo, p);
-  }
-}
-
-// unite() continued from two known (possibly stale) roots.
+// unite() continued from two known (possibly stale) roots: the larger root
+// is confirmed with an L2 read before the atomic, and a changed root is
+// re-walked with plain reads (the single-graph solve's default mode; every
+// walk still ends at or below its start, so max(ra, rb) keeps dropping).
 __device__ __forceinline__ void unite_roots(uint32_t ra, uint32_t rb, uint32_t *p) {
   while (ra != rb) {
     const uint32_t hi = ra > rb ? ra : rb, lo = ra > rb ? rb : ra;
     const uint32_t cur = load_l2(&p[hi]);
     if (cur != hi) {
-      ra = root_of_l2(cur, p);
-      rb = root_of_l2(lo, p);
+      ra = root_of(cur, p);
+      rb = root_of(lo, p);
       continue;
     }
     const uint32_t old = atomicMin(&p[hi], lo);
     if (old == hi || old == lo) return;
-    ra = root_of_l2(old, p);
-    rb = root_of_l2(lo, p);
+    ra = root_of(old, p);
+    rb = root_of(lo, p);
   }
+}
+
+__device__ __forceinline__ void unite(uint32_t a, uint32_t b, uint32_t *p) {
+  unite_roots(root_of(a, p), root_of(b, p), p);
 }
 
 #define ITEM_OR_EXIT()                                   \

@@ -15,9 +15,12 @@
 //      finds the hub's root at or below its own id and lowers its own entry
 //      instead of every such vertex hammering the hub's entry.
 //   2. Sampled hooking: every vertex hooks with its neighbour 0 (only where
-//      step 1 did not already link them), pointer jumping, then with its
-//      neighbour 1, pointer jumping.  After this the forest already holds
-//      the giant component of a power-law or random graph in one tree.
+//      step 1 did not already link them) and its neighbour 1, both in one
+//      pass that walks v's root once (or, as a tuning choice, a pass per
+//      neighbour with pointer jumping between).  On graphs of 2^24 vertices
+//      and more the pass halves the paths it walks.  After this the forest
+//      already holds the giant component of a power-law or random graph in
+//      one tree.
 //   3. The most frequent root among 1024 sampled vertices is taken as the
 //      giant root g.
 //   4. Frontier sweep: every vertex whose parent is not g hooks with all its
@@ -59,7 +62,9 @@ constexpr int kGiantSamples = 1024;
 // per-vertex rounds do before the frontier sweep.
 int g_sampled_neighbours = 2;  // sampled hooking rounds (1..4)
 int g_blocks_per_cu = 0;       // grid cap of the sampled hook (0 = 8 with hook ILP 4, else 4)
-int g_hook_mode = 1;           // 0: plain loads, 1: L2-confirmed roots, 2: all loads from L2
+// 0: plain loads; 1: L2-confirmed roots, L2 re-walks; 2: all loads from L2;
+// 3: lossy sampling; 4: L2-confirmed roots, plain re-walks
+int g_hook_mode = 4;
 
 // Parent read that bypasses the CU's L1 (an agent-scope atomic load), so it
 // sees every hook that has reached L2.
@@ -105,12 +110,22 @@ __device__ __forceinline__ uint32_t find_root_via(uint32_t x, const uint32_t *__
   return x;
 }
 
+// Root walk after a hook found a root changed: L2 reads in modes 1 and 2,
+// plain (possibly stale) reads otherwise.  Either way the walk ends at or
+// below the ids it starts from, which is all the hook loop needs.
+template <int Mode>
+__device__ __forceinline__ uint32_t rewalk(uint32_t x, const uint32_t *__restrict__ p) {
+  return (Mode == 1 || Mode == 2) ? find_root_l2(x, p) : find_root(x, p);
+}
+
 // Union of the trees of a and b.  Many members of one tree often hook
 // towards the same other tree in the same launch; with plain loads their
 // CUs keep seeing the tree's root as unhooked (stale L1 lines) and every one
-// of them pays a same-address atomic.  Mode 1 confirms, with one L2 read,
-// that the larger root is still a root before the atomic, and follows any
-// change with L2 reads; mode 2 walks with L2 reads throughout.
+// of them pays a same-address atomic.  Modes 1 and 4 confirm, with one L2
+// read, that the larger root is still a root before the atomic; mode 1
+// follows any change with L2 reads, mode 4 (the default) with plain reads,
+// which measured faster on every graph tried; mode 2 walks with L2 reads
+// throughout.
 template <int Mode>
 __device__ __forceinline__ void hook(uint32_t a, uint32_t b, uint32_t *__restrict__ p) {
   uint32_t ra = Mode == 2 ? find_root_l2(a, p) : find_root(a, p);
@@ -121,15 +136,15 @@ __device__ __forceinline__ void hook(uint32_t a, uint32_t b, uint32_t *__restric
     if (Mode != 0) {
       const uint32_t cur = load_l2(&p[hi]);
       if (cur != hi) {  // hooked meanwhile: cur < hi, so max(ra, rb) still drops
-        ra = find_root_l2(cur, p);
-        rb = find_root_l2(lo, p);
+        ra = rewalk<Mode>(cur, p);
+        rb = rewalk<Mode>(lo, p);
         continue;
       }
     }
     const uint32_t old = atomicMin(&p[hi], lo);
     if (old == hi || old == lo) return;
-    ra = Mode == 0 ? find_root(old, p) : find_root_l2(old, p);
-    rb = Mode == 0 ? find_root(lo, p) : find_root_l2(lo, p);
+    ra = rewalk<Mode>(old, p);
+    rb = rewalk<Mode>(lo, p);
   }
 }
 
@@ -233,15 +248,45 @@ __device__ __forceinline__ void hook_roots(uint32_t ra, uint32_t rb, uint32_t *_
     if (Mode != 0) {
       const uint32_t cur = load_l2(&p[hi]);
       if (cur != hi) {
-        ra = find_root_l2(cur, p);
-        rb = find_root_l2(lo, p);
+        ra = rewalk<Mode>(cur, p);
+        rb = rewalk<Mode>(lo, p);
         continue;
       }
     }
     const uint32_t old = atomicMin(&p[hi], lo);
     if (old == hi || old == lo) return;
-    ra = Mode == 0 ? find_root(old, p) : find_root_l2(old, p);
-    rb = Mode == 0 ? find_root(lo, p) : find_root_l2(lo, p);
+    ra = rewalk<Mode>(old, p);
+    rb = rewalk<Mode>(lo, p);
+  }
+}
+
+// roots_lockstep with path halving: each step moves x to its grandparent
+// and lowers p[x] to that grandparent.  The lowering is an atomic min (with
+// no return value, so the wave does not wait for it): p[x] only ever moves
+// to a smaller ancestor, which keeps the forest valid whatever hooks land
+// in between.  A plain store could raise an entry that a concurrent hook
+// had just lowered, which lost links in practice.  Halving pays where the
+// parent array is far larger than the L2s (n >= 2^24): each walk then costs
+// HBM lines, and flatter trees make every later walk, including the
+// frontier sweep's, shorter.
+template <int ILP>
+__device__ __forceinline__ void roots_lockstep_halve(uint32_t (&x)[ILP], const bool (&act)[ILP],
+                                                     uint32_t *__restrict__ p) {
+  bool more = true;
+  while (more) {
+    more = false;
+#pragma unroll
+    for (int k = 0; k < ILP; ++k) {
+      if (!act[k]) continue;
+      const uint32_t px = p[x[k]];
+      if (px != x[k]) {
+        const uint32_t ppx = p[px];
+        if (ppx != px)
+          (void)__hip_atomic_fetch_min(&p[x[k]], ppx, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        x[k] = ppx;
+        more = true;
+      }
+    }
   }
 }
 
@@ -394,7 +439,8 @@ __global__ __launch_bounds__(kBlock) void k_hook_cached(const uint32_t *__restri
 // Both sampled rounds of a two-round solve in one pass: v's root is walked
 // once and hooked with the roots of neighbours 0 and 1 in turn, so the
 // pass between the rounds (and its pointer-jumping launch) disappears.
-template <int Mode, int ILP>
+// Halve: the three root walks halve the paths they follow.
+template <int Mode, int ILP, bool Halve = false>
 __global__ __launch_bounds__(kBlock) void k_hook_pair(const uint32_t *__restrict__ s0,
                                                       const uint32_t *__restrict__ s1,
                                                       uint32_t n, uint32_t *__restrict__ p) {
@@ -415,7 +461,8 @@ __global__ __launch_bounds__(kBlock) void k_hook_pair(const uint32_t *__restrict
       x[ILP + k] = w0;
       x[2 * ILP + k] = w1;
     }
-    roots_lockstep<3 * ILP>(x, act, p);
+    if (Halve) roots_lockstep_halve<3 * ILP>(x, act, p);
+    else roots_lockstep<3 * ILP>(x, act, p);
 #pragma unroll
     for (int k = 0; k < ILP; ++k) {
       uint32_t rv = x[k];
@@ -440,22 +487,23 @@ __device__ __forceinline__ uint32_t sample_vertex(uint32_t i, uint32_t n) {
 
 // One workgroup: the most frequent root among kGiantSamples sampled
 // vertices, counted in an LDS open-addressing table; ties go to the smaller
-// root so the choice is a function of the forest alone.
-__global__ __launch_bounds__(kBlock) void k_sample_giant(uint32_t n, const uint32_t *__restrict__ p,
+// root so the choice is a function of the forest alone.  One thread per
+// sample, so the root walks (chains of dependent gathers) all run at once.
+__global__ __launch_bounds__(kGiantSamples) void k_sample_giant(uint32_t n, const uint32_t *__restrict__ p,
                                                          uint32_t *__restrict__ giant) {
   constexpr uint32_t kSlots = 2 * kGiantSamples;
   constexpr uint32_t kEmpty = 0xFFFFFFFFu;
   __shared__ uint32_t keys[kSlots];
   __shared__ uint32_t cnts[kSlots];
-  __shared__ uint64_t best[kBlock / kWave];
+  __shared__ uint64_t best[kGiantSamples / kWave];
   const int t = threadIdx.x;
-  for (uint32_t i = t; i < kSlots; i += kBlock) {
+  for (uint32_t i = t; i < kSlots; i += kGiantSamples) {
     keys[i] = kEmpty;
     cnts[i] = 0;
   }
   __syncthreads();
-  for (int i = t; i < kGiantSamples; i += kBlock) {
-    const uint32_t x = find_root(sample_vertex(i, n), p);
+  {
+    const uint32_t x = find_root(sample_vertex(t, n), p);
     uint32_t h = (x * 0x9E3779B1u) >> 21;  // 11-bit slot
     for (;;) {
       const uint32_t prev = atomicCAS(&keys[h], kEmpty, x);
@@ -470,7 +518,7 @@ __global__ __launch_bounds__(kBlock) void k_sample_giant(uint32_t n, const uint3
   // (count, ~root) packed so the max picks the largest count, then the
   // smallest root.
   uint64_t mine = 0;
-  for (uint32_t i = t; i < kSlots; i += kBlock)
+  for (uint32_t i = t; i < kSlots; i += kGiantSamples)
     if (cnts[i]) {
       const uint64_t cand = ((uint64_t)cnts[i] << 32) | (uint32_t)~keys[i];
       if (cand > mine) mine = cand;
@@ -483,7 +531,7 @@ __global__ __launch_bounds__(kBlock) void k_sample_giant(uint32_t n, const uint3
   __syncthreads();
   if (t == 0) {
     uint64_t b = best[0];
-    for (int w = 1; w < kBlock / kWave; ++w)
+    for (int w = 1; w < kGiantSamples / kWave; ++w)
       if (best[w] > b) b = best[w];
     giant[0] = ~(uint32_t)b;
   }
@@ -568,13 +616,18 @@ inline unsigned blocks_for(uint64_t n, int cu_count, int per_cu) {
 //  * reading the sampled neighbours once into coalesced arrays (the cache)
 //    never loses much and wins up to 1.6x on the sampled rounds of large
 //    graphs, where each column gather is a cache line from HBM;
-//  * hooking both sampled neighbours in one pass wins everywhere except on
-//    dense graphs beyond about 2^30 CSR entries (Chung-Lu 32.8M and 65.6M),
-//    where the longer chains it leaves for the frontier sweep cost more than
-//    the pass it saves;
-//  * walking several roots per thread pays on dense graphs (m >= 16 n: 2 in
-//    the one-pass hook, 4 per round otherwise, with 8 blocks per CU) and
-//    loses on sparse ones (1, with 4 blocks per CU);
+//  * hooking both sampled neighbours in one pass, with path halving in its
+//    root walks once n >= 2^24, beats two passes (with pointer jumping
+//    between them) on every graph; without halving the two-pass plan had
+//    won on dense graphs beyond 2^30 entries (Chung-Lu 65.6M: 3.70 ms two
+//    passes, 3.49 ms one halving pass);
+//  * halving loses below 2^24 vertices (R-MAT 22: 235 vs 217 us), where the
+//    parent array lives in the L2s and MALL and the extra atomics on hub
+//    entries cost more than the shorter walks save;
+//  * walking several roots per thread pays on dense graphs (m >= 16 n: 2
+//    per thread in the one-pass hook with 8 blocks per CU; 4 per round on
+//    the two-pass path), except past 2^31 entries, and loses on sparse ones
+//    (1, with 4 blocks per CU);
 //  * on the two-pass path, sampling the giant root after round 0 and
 //    skipping its members in round 1 saves 5-6% on Chung-Lu 16.4M/65.6M.
 int g_stream_blocks_per_cu = 8;    // grid cap of the contention-free kernels (init, compress)
@@ -583,13 +636,16 @@ int g_hook_ilp = 0;                // roots walked per thread in the sampled hoo
 int g_stream_ilp = 1;              // vertices per thread per step in init/compress (1, 2, 4)
 int g_sample_cache = 1;            // sampled neighbours read once into coalesced arrays
 int g_fused_pair = -1;             // both sampled rounds in one pass (-1 = auto)
+int g_halve = -1;                  // path halving in the one-pass hook's walks (-1 = auto)
 int g_giant_early = 1;             // two-pass path: sample the giant after round 0, skip it in round 1
-constexpr uint64_t kDenseDegree = 16;            // dense: m >= 16 n
-constexpr uint64_t kFusedMaxDenseEntries = 1ull << 30;
+constexpr uint64_t kDenseDegree = 16;              // dense: m >= 16 n
+constexpr uint64_t kHalveMinVertices = 1ull << 24;  // halving from this many vertices
+constexpr uint64_t kWideIlpMaxEntries = 1ull << 31; // dense one-pass ILP 2 below this many entries
 
 struct Plan {
   int cache;     // sampled-neighbour cache in use
   int fused;     // one-pass hook of both sampled neighbours
+  int halve;     // path halving in the one-pass hook
   int hook_ilp;  // roots per thread in the sampled hook
   int hook_bpc;  // sampled-hook grid cap, workgroups per CU
 };
@@ -597,10 +653,11 @@ struct Plan {
 inline Plan plan_for(uint32_t n, uint64_t m) {
   Plan pl;
   const bool dense = m >= kDenseDegree * (uint64_t)n;
-  pl.cache = g_sample_cache && (g_hook_mode == 0 || g_hook_mode == 1);
-  const int fused_auto = !dense || m < kFusedMaxDenseEntries;
-  pl.fused = pl.cache && g_sampled_neighbours == 2 && (g_fused_pair < 0 ? fused_auto : g_fused_pair);
-  pl.hook_ilp = g_hook_ilp ? g_hook_ilp : (!dense ? 1 : (pl.fused ? 2 : 4));
+  pl.cache = g_sample_cache && (g_hook_mode == 0 || g_hook_mode == 1 || g_hook_mode == 4);
+  pl.fused = pl.cache && g_sampled_neighbours == 2 && (g_fused_pair < 0 ? 1 : g_fused_pair);
+  pl.halve = pl.fused && g_hook_mode != 0 && (g_halve < 0 ? n >= kHalveMinVertices : g_halve);
+  const int wide = pl.fused ? (m < kWideIlpMaxEntries ? 2 : 1) : 4;
+  pl.hook_ilp = g_hook_ilp ? g_hook_ilp : (dense ? wide : 1);
   pl.hook_bpc = g_blocks_per_cu ? g_blocks_per_cu : (pl.hook_ilp >= 2 ? 8 : 4);
   return pl;
 }
@@ -616,7 +673,8 @@ extern "C" int cc_hip_components_set(const char *key, int value) {
   else if (k == "blocks_per_cu" && value >= 0 && value <= 64) g_blocks_per_cu = value;
   else if (k == "stream_blocks_per_cu" && value >= 1 && value <= 64) g_stream_blocks_per_cu = value;
   else if (k == "frontier_blocks_per_cu" && value >= 1 && value <= 64) g_frontier_blocks_per_cu = value;
-  else if (k == "hook_mode" && value >= 0 && value <= 3) g_hook_mode = value;
+  else if (k == "hook_mode" && value >= 0 && value <= 4) g_hook_mode = value;
+  else if (k == "halve" && value >= -1 && value <= 1) g_halve = value;
   else if (k == "hook_ilp" && (value == 0 || ilp_ok(value))) g_hook_ilp = value;
   else if (k == "stream_ilp" && ilp_ok(value)) g_stream_ilp = value;
   else if (k == "sample_cache" && (value == 0 || value == 1)) g_sample_cache = value;
@@ -638,6 +696,20 @@ extern "C" int cc_hip_components_get(const char *key) {
   if (k == "sample_cache") return g_sample_cache;
   if (k == "fused_pair") return g_fused_pair;
   if (k == "giant_early") return g_giant_early;
+  if (k == "halve") return g_halve;
+  return -1;
+}
+
+extern "C" int cc_hip_components_plan_get(uint32_t n, uint64_t m, const char *key) {
+  const Plan pl = plan_for(n, m);
+  const std::string k(key ? key : "");
+  if (k == "cache") return pl.cache;
+  if (k == "fused") return pl.fused;
+  if (k == "halve") return pl.halve;
+  if (k == "hook_ilp") return pl.hook_ilp;
+  if (k == "hook_blocks_per_cu") return pl.hook_bpc;
+  if (k == "giant_early") return pl.cache && !pl.fused && g_giant_early;
+  if (k == "hook_mode") return g_hook_mode;
   return -1;
 }
 
@@ -653,7 +725,7 @@ extern "C" int cc_hip_components_plan(uint32_t n, uint64_t m, int *cache, int *f
 
 extern "C" int cc_hip_components_tune(int sampled_neighbours, int blocks_per_cu, int hook_mode) {
   if (sampled_neighbours < 0 || sampled_neighbours > 4 || blocks_per_cu < 0 || blocks_per_cu > 64 ||
-      hook_mode < -1 || hook_mode > 3)
+      hook_mode < -1 || hook_mode > 4)
     return ccx::set_error(4, "cc_hip_components_tune: out of range (%d, %d)", sampled_neighbours,
                           blocks_per_cu);
   if (sampled_neighbours > 0) g_sampled_neighbours = sampled_neighbours;
@@ -695,7 +767,7 @@ extern "C" int cc_hip_components_device(const uint64_t *d_row_ptr, const uint32_
   };
   const int mode = g_hook_mode;
   // The cache holds `rounds` u32 per vertex in this stream's temp buffer;
-  // modes 0 and 1 read it (mode 2 walks with L2 reads, mode 3 is lossy).
+  // modes 0, 1 and 4 read it (mode 2 walks with L2 reads, mode 3 is lossy).
   const bool cache = plan.cache != 0;
   const bool early_giant = cache && !plan.fused && g_giant_early;
   uint32_t *smp = nullptr;
@@ -717,14 +789,22 @@ extern "C" int cc_hip_components_device(const uint64_t *d_row_ptr, const uint32_
   }
   if (plan.fused) {
     const uint32_t *s1 = smp + n;
+#define CCX_HOOK_PAIR(M, H)                                                                        \
+  do {                                                                                             \
+    if (hook_ilp == 4)                                                                             \
+      CCX_LAUNCH("cc_hook_sample", s, k_hook_pair<M, 4, H><<<nb, kBlock, 0, s>>>(smp, s1, n, d_labels)); \
+    else if (hook_ilp == 2)                                                                        \
+      CCX_LAUNCH("cc_hook_sample", s, k_hook_pair<M, 2, H><<<nb, kBlock, 0, s>>>(smp, s1, n, d_labels)); \
+    else                                                                                           \
+      CCX_LAUNCH("cc_hook_sample", s, k_hook_pair<M, 1, H><<<nb, kBlock, 0, s>>>(smp, s1, n, d_labels)); \
+  } while (0)
     if (mode == 0)
       CCX_LAUNCH("cc_hook_sample", s, k_hook_pair<0, 2><<<nb, kBlock, 0, s>>>(smp, s1, n, d_labels));
-    else if (hook_ilp == 4)
-      CCX_LAUNCH("cc_hook_sample", s, k_hook_pair<1, 4><<<nb, kBlock, 0, s>>>(smp, s1, n, d_labels));
-    else if (hook_ilp == 2)
-      CCX_LAUNCH("cc_hook_sample", s, k_hook_pair<1, 2><<<nb, kBlock, 0, s>>>(smp, s1, n, d_labels));
-    else
-      CCX_LAUNCH("cc_hook_sample", s, k_hook_pair<1, 1><<<nb, kBlock, 0, s>>>(smp, s1, n, d_labels));
+    else if (mode == 1 && plan.halve) CCX_HOOK_PAIR(1, true);
+    else if (mode == 1) CCX_HOOK_PAIR(1, false);
+    else if (plan.halve) CCX_HOOK_PAIR(4, true);
+    else CCX_HOOK_PAIR(4, false);
+#undef CCX_HOOK_PAIR
   } else {
     for (uint32_t r = 0; r < rounds; ++r) {
       if (r > 0)
@@ -733,7 +813,7 @@ extern "C" int cc_hip_components_device(const uint64_t *d_row_ptr, const uint32_
         const uint32_t *sr = smp + (size_t)r * n;
         const bool skip = early_giant && r > 0;
         if (r == 1 && early_giant)
-          CCX_LAUNCH("cc_sample_giant", s, k_sample_giant<<<1, kBlock, 0, s>>>(n, d_labels, giant));
+          CCX_LAUNCH("cc_sample_giant", s, k_sample_giant<<<1, kGiantSamples, 0, s>>>(n, d_labels, giant));
 #define CCX_HOOK_CACHED(M, I)                                                                      \
   do {                                                                                             \
     if (skip)                                                                                      \
@@ -742,18 +822,25 @@ extern "C" int cc_hip_components_device(const uint64_t *d_row_ptr, const uint32_
       CCX_LAUNCH("cc_hook_sample", s, k_hook_cached<M, I, false><<<nb, kBlock, 0, s>>>(sr, n, r, d_labels, giant)); \
   } while (0)
         if (mode == 0) CCX_HOOK_CACHED(0, 1);
-        else if (hook_ilp == 4) CCX_HOOK_CACHED(1, 4);
-        else if (hook_ilp == 2) CCX_HOOK_CACHED(1, 2);
-        else CCX_HOOK_CACHED(1, 1);
+        else if (mode == 1 && hook_ilp == 4) CCX_HOOK_CACHED(1, 4);
+        else if (mode == 1 && hook_ilp == 2) CCX_HOOK_CACHED(1, 2);
+        else if (mode == 1) CCX_HOOK_CACHED(1, 1);
+        else if (hook_ilp == 4) CCX_HOOK_CACHED(4, 4);
+        else if (hook_ilp == 2) CCX_HOOK_CACHED(4, 2);
+        else CCX_HOOK_CACHED(4, 1);
 #undef CCX_HOOK_CACHED
-      } else if (mode == 1 && hook_ilp == 4) {
-        CCX_LAUNCH("cc_hook_sample", s, k_hook_neighbour_ilp<1, 4><<<nb, kBlock, 0, s>>>(d_row_ptr, d_col_idx, n, r, d_labels));
-      } else if (mode == 1 && hook_ilp == 2) {
-        CCX_LAUNCH("cc_hook_sample", s, k_hook_neighbour_ilp<1, 2><<<nb, kBlock, 0, s>>>(d_row_ptr, d_col_idx, n, r, d_labels));
+      } else if ((mode == 1 || mode == 4) && hook_ilp == 4) {
+        if (mode == 1) CCX_LAUNCH("cc_hook_sample", s, k_hook_neighbour_ilp<1, 4><<<nb, kBlock, 0, s>>>(d_row_ptr, d_col_idx, n, r, d_labels));
+        else CCX_LAUNCH("cc_hook_sample", s, k_hook_neighbour_ilp<4, 4><<<nb, kBlock, 0, s>>>(d_row_ptr, d_col_idx, n, r, d_labels));
+      } else if ((mode == 1 || mode == 4) && hook_ilp == 2) {
+        if (mode == 1) CCX_LAUNCH("cc_hook_sample", s, k_hook_neighbour_ilp<1, 2><<<nb, kBlock, 0, s>>>(d_row_ptr, d_col_idx, n, r, d_labels));
+        else CCX_LAUNCH("cc_hook_sample", s, k_hook_neighbour_ilp<4, 2><<<nb, kBlock, 0, s>>>(d_row_ptr, d_col_idx, n, r, d_labels));
       } else if (mode == 0) {
         CCX_LAUNCH("cc_hook_sample", s, k_hook_neighbour<0><<<nb, kBlock, 0, s>>>(d_row_ptr, d_col_idx, n, r, d_labels));
       } else if (mode == 1) {
         CCX_LAUNCH("cc_hook_sample", s, k_hook_neighbour<1><<<nb, kBlock, 0, s>>>(d_row_ptr, d_col_idx, n, r, d_labels));
+      } else if (mode == 4) {
+        CCX_LAUNCH("cc_hook_sample", s, k_hook_neighbour<4><<<nb, kBlock, 0, s>>>(d_row_ptr, d_col_idx, n, r, d_labels));
       } else if (mode == 2) {
         CCX_LAUNCH("cc_hook_sample", s, k_hook_neighbour<2><<<nb, kBlock, 0, s>>>(d_row_ptr, d_col_idx, n, r, d_labels));
       } else {
@@ -764,7 +851,7 @@ extern "C" int cc_hip_components_device(const uint64_t *d_row_ptr, const uint32_
   // (sampled again after early skipping: the giant tree has since absorbed
   // other trees under smaller roots, whose members would not pass through
   // the early root)
-  CCX_LAUNCH("cc_sample_giant", s, k_sample_giant<<<1, kBlock, 0, s>>>(n, d_labels, giant));
+  CCX_LAUNCH("cc_sample_giant", s, k_sample_giant<<<1, kGiantSamples, 0, s>>>(n, d_labels, giant));
   // Exact sampling (modes 0-2) links neighbours 0..rounds-1 of every vertex,
   // so the sweep starts after them; lossy sampling (mode 3) links nothing
   // for sure, so the sweep covers every neighbour.
@@ -775,6 +862,8 @@ extern "C" int cc_hip_components_device(const uint64_t *d_row_ptr, const uint32_
     CCX_LAUNCH("cc_hook_frontier", s, k_hook_frontier<1><<<nf, kBlock, 0, s>>>(d_row_ptr, d_col_idx, n, first, d_labels, giant));
   else if (mode == 2)
     CCX_LAUNCH("cc_hook_frontier", s, k_hook_frontier<2><<<nf, kBlock, 0, s>>>(d_row_ptr, d_col_idx, n, first, d_labels, giant));
+  else if (mode == 4)
+    CCX_LAUNCH("cc_hook_frontier", s, k_hook_frontier<4><<<nf, kBlock, 0, s>>>(d_row_ptr, d_col_idx, n, first, d_labels, giant));
   else
     CCX_LAUNCH("cc_hook_frontier", s, k_hook_frontier<3><<<nf, kBlock, 0, s>>>(d_row_ptr, d_col_idx, n, first, d_labels, giant));
   return compress();

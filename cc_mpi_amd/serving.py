@@ -8,8 +8,10 @@ the whole solve each) forms a batch from every request that has arrived as
 soon as a lane is free, within the lane's vertex/item capacity, launches it
 as one graph replay, and queues the label copies to pinned host memory on
 the same stream; requests that arrive meanwhile go into the next lane's
-batch, so their work overlaps on the device.  Latency = completion of the
-label copy - arrival.
+batch, so their work overlaps on the device.  A batch of one request (more
+generally, of at most `direct_max`) skips the replay and runs the
+single-graph solve on the lane's stream, so light load pays no replay
+overhead.  Latency = completion of the label copy - arrival.
 
 The one-graph-per-launch baseline serves the same arrival schedule FIFO:
 one `cc_hip_components_device` solve and one label copy per request.
@@ -88,10 +90,16 @@ class BatchEngine:
         self.max_vertices, self.max_items, self.max_graphs = max_vertices, max_items, max_graphs
         self.labels = DeviceBuffer(4 * max_vertices)
         self.out = PinnedU32(max_vertices)
+        self.direct_launches = 0
 
-    def launch(self, graphs: list[DeviceGraph], item_vertices: int) -> list[tuple[int, int]]:
-        """Solve `graphs` in one replay; returns each graph's (offset, n) in
-        the label arenas.  Label copies to pinned memory are queued behind it."""
+    def launch(self, graphs: list[DeviceGraph], item_vertices: int,
+               direct_max: int = 0) -> list[tuple[int, int]]:
+        """Solve `graphs` on this lane's stream; returns each graph's (offset,
+        n) in the label arenas.  Label copies to pinned memory are queued
+        behind the solve.  A batch of at most `direct_max` graphs runs as
+        that many single-graph solves (`cc_hip_components_device`) instead
+        of one replay: a replay carries fixed work (table upload, every
+        stage of the schedule) that a lone small request does not amortise."""
         lib = _lib()
         k = len(graphs)
         offs, off = [], 0
@@ -100,14 +108,21 @@ class BatchEngine:
             off += g.n
         if off > self.max_vertices:
             raise ValueError("batch exceeds the lane's vertex capacity")
-        P = ctypes.c_void_p
-        rp = (P * k)(*[g.row_ptr.ptr for g in graphs])
-        ci = (P * k)(*[g.col_idx.ptr for g in graphs])
-        nn = (ctypes.c_uint32 * k)(*[g.n for g in graphs])
-        lb = (P * k)(*[self.labels.ptr + 4 * o for o, _ in offs])
-        used = ctypes.c_uint32(0)
-        nat.check(lib.cc_hip_batch_launch(self.handle, k, rp, ci, nn, lb, item_vertices,
-                                          ctypes.byref(used)), "cc_hip_batch_launch")
+        if k <= direct_max:
+            for g, (o, _) in zip(graphs, offs):
+                nat.check(lib.cc_hip_components_device(g.row_ptr.ptr, g.col_idx.ptr, g.n, g.m,
+                                                       self.labels.ptr + 4 * o, self.stream),
+                          "cc_hip_components_device")
+            self.direct_launches += 1
+        else:
+            P = ctypes.c_void_p
+            rp = (P * k)(*[g.row_ptr.ptr for g in graphs])
+            ci = (P * k)(*[g.col_idx.ptr for g in graphs])
+            nn = (ctypes.c_uint32 * k)(*[g.n for g in graphs])
+            lb = (P * k)(*[self.labels.ptr + 4 * o for o, _ in offs])
+            used = ctypes.c_uint32(0)
+            nat.check(lib.cc_hip_batch_launch(self.handle, k, rp, ci, nn, lb, item_vertices,
+                                              ctypes.byref(used)), "cc_hip_batch_launch")
         if off:
             nat.check(lib.cc_hip_memcpy_d2h_async(self.out.ptr, self.labels.ptr, 4 * off,
                                                   self.stream), "label copy")
@@ -221,11 +236,12 @@ class Server:
 
     def __init__(self, store: GraphStore, lanes: int = 2, max_vertices: int = 1 << 25,
                  max_items: int = 1 << 15, max_graphs: int = 1024, item_vertices: int = 4096,
-                 keep_labels: bool = False):
+                 keep_labels: bool = False, direct_max: int = 1):
         self.store = store
         self.engines = [BatchEngine(max_vertices, max_items, max_graphs) for _ in range(lanes)]
         self.item_vertices = item_vertices
         self.keep_labels = keep_labels
+        self.direct_max = direct_max  # batches this small run as single-graph solves
 
     def close(self):
         for e in self.engines:
@@ -235,7 +251,7 @@ class Server:
         pending = collections.deque(sorted(requests, key=lambda r: r.arrival))
         queue: collections.deque = collections.deque()
         busy: dict[int, list] = {}
-        batches = 0
+        batches = direct = 0
         t0 = time.perf_counter()
         total = len(pending)
         finished = 0
@@ -277,9 +293,10 @@ class Server:
                     verts += g.n
                     items += it
                 if chosen:
-                    offs = eng.launch(graphs, self.item_vertices)
+                    offs = eng.launch(graphs, self.item_vertices, self.direct_max)
                     busy[li] = list(zip(chosen, graphs, offs))
                     batches += 1
+                    direct += len(graphs) <= self.direct_max
             if queue and not busy:
                 raise MemoryError(f"request {queue[0].rid} does not fit the graph store budget")
             if not busy and not queue and pending:
@@ -287,7 +304,7 @@ class Server:
                 if wait > 0:
                     time.sleep(min(wait, 0.002))
         wall = time.perf_counter() - t0
-        return {"batches": batches, "wall_s": wall}
+        return {"batches": batches, "direct_batches": direct, "wall_s": wall}
 
 
 def serve_one_by_one(store: GraphStore, requests: list[Request], keep_labels: bool = False) -> dict:

@@ -81,22 +81,26 @@ int cc_hip_components_device(const uint64_t *d_row_ptr, const uint32_t *d_col_id
  * (1..4, default 2; 0 keeps the current value), the grid cap of the
  * sampled-hook kernel in workgroups per CU (1..64; 0 keeps), and the
  * hooking mode (-1 keeps; 0 plain loads, 1 roots confirmed with an L2 read
- * before each atomic (default), 2 all parent reads from L2, 3 lossy
- * sampling with every neighbour swept outside the giant tree).  Results
- * never depend on any of them. */
+ * before each atomic and re-walked with L2 reads, 2 all parent reads from
+ * L2, 3 lossy sampling with every neighbour swept outside the giant tree,
+ * 4 roots confirmed with an L2 read and re-walked with plain reads
+ * (default)).  Results never depend on any of them. */
 int cc_hip_components_tune(int sampled_neighbours, int blocks_per_cu, int hook_mode);
 /* The same knobs by name:
  *   "sampled_neighbours"      1..4 (default 2)
- *   "hook_mode"               0..3 (default 1)
+ *   "hook_mode"               0..4 (default 4)
  *   "sample_cache"            1 (default): the init pass copies each vertex's
  *                             sampled neighbours into coalesced arrays
  *                             (rounds x n u32 in the stream's temp buffer)
  *                             that the sampled rounds read; 0: the rounds
- *                             gather from col_idx.  Modes 0 and 1 only.
- *   "fused_pair"              1: hook both sampled neighbours in one pass
- *                             (needs the cache and 2 rounds); 0: one pass
- *                             per round; -1 (default): one pass unless the
- *                             graph is dense (m >= 16 n) with m >= 2^30
+ *                             gather from col_idx.  Modes 0, 1 and 4 only.
+ *   "fused_pair"              1 or -1 (default): hook both sampled
+ *                             neighbours in one pass (needs the cache and 2
+ *                             rounds); 0: one pass per round
+ *   "halve"                   path halving (atomic min of each visited
+ *                             entry to its grandparent) in the one-pass
+ *                             hook's root walks: 1, 0, or -1 (default) =
+ *                             on when n >= 2^24; not in mode 0
  *   "giant_early"             1 (default): on the two-pass cached path,
  *                             sample the giant root after round 0 and skip
  *                             its members in round 1; 0: sample it once,
@@ -105,8 +109,8 @@ int cc_hip_components_tune(int sampled_neighbours, int blocks_per_cu, int hook_m
  *                             hook walks several roots per thread, else 4
  *   "hook_ilp"                roots walked per thread in lockstep in the
  *                             sampled hook: 1, 2, 4, or 0 (default) = 1 when
- *                             m < 16 n, else 2 in the one-pass hook, 4 per
- *                             round
+ *                             m < 16 n, else 2 in the one-pass hook (1 from
+ *                             m >= 2^31), 4 per round
  *   "frontier_blocks_per_cu"  grid cap of the frontier sweep (default 8)
  *   "stream_blocks_per_cu"    grid cap of init/compress (default 8)
  *   "stream_ilp"              vertices per thread per step in init/compress
@@ -121,6 +125,10 @@ int cc_hip_components_get(const char *key);
  * sampled hook.  Any pointer may be NULL.  Returns 0. */
 int cc_hip_components_plan(uint32_t n, uint64_t m, int *cache, int *fused, int *hook_ilp,
                            int *hook_blocks_per_cu);
+/* One field of that plan by name: "cache", "fused", "halve", "hook_ilp",
+ * "hook_blocks_per_cu", "giant_early" (in effect for this graph),
+ * "hook_mode".  Returns -1 for an unknown key. */
+int cc_hip_components_plan_get(uint32_t n, uint64_t m, const char *key);
 
 /* Same result through host buffers (H2D, solve, D2H; blocking). */
 int cc_hip_components_host(const uint64_t *row_ptr, const uint32_t *col_idx,

@@ -91,6 +91,26 @@ def test_chung_lu(gpu):
     check_labels(G, dropin(G))
 
 
+@pytest.mark.parametrize("kind", ["er", "grid"])
+def test_halving_plan_at_scale(gpu, kind):
+    """Graphs of 2^24 vertices take the one-pass hook with path halving by
+    default; labels, component count and size histogram equal the oracle's."""
+    from cc_mpi_amd import devgen
+    lib = nat.gpu_lib()
+    D = (devgen.erdos_renyi(1 << 24, 3.0, seed=41) if kind == "er"
+         else devgen.grid(4096, 4096, 0.7, seed=42))
+    assert lib.cc_hip_components_plan_get(D.n, D.m, b"halve") == 1
+    assert lib.cc_hip_components_plan_get(D.n, D.m, b"fused") == 1
+    got = D.solve()
+    G = D.to_host()
+    D.close()
+    want = check_labels(G, got)
+    roots = np.flatnonzero(want == np.arange(G.n))
+    assert np.count_nonzero(got == np.arange(G.n)) == roots.size
+    assert np.array_equal(np.bincount(got, minlength=G.n)[roots],
+                          np.bincount(want, minlength=G.n)[roots])
+
+
 def test_device_resident_repeat(gpu):
     # same labels on repeated solves of a resident graph, and equal to the oracle
     G = cm.rmat(16, 16, seed=5)
@@ -136,10 +156,26 @@ PATHS = [  # every code path of the solve: (knobs) -> same labels
     dict(sample_cache=1, fused_pair=1, hook_ilp=2, hook_mode=1, sampled_neighbours=1),
     dict(sample_cache=1, fused_pair=0, hook_ilp=4, hook_mode=1, sampled_neighbours=3),
     dict(sample_cache=1, fused_pair=0, hook_ilp=1, hook_mode=1, sampled_neighbours=4),
+    dict(sample_cache=0, fused_pair=0, hook_ilp=1, hook_mode=4),
+    dict(sample_cache=0, fused_pair=0, hook_ilp=2, hook_mode=4),
+    dict(sample_cache=0, fused_pair=0, hook_ilp=4, hook_mode=4),
+    dict(sample_cache=1, fused_pair=0, hook_ilp=1, hook_mode=4),
+    dict(sample_cache=1, fused_pair=0, hook_ilp=2, hook_mode=4, giant_early=0),
+    dict(sample_cache=1, fused_pair=0, hook_ilp=4, hook_mode=4),
+    dict(sample_cache=1, fused_pair=1, hook_ilp=1, hook_mode=4, halve=0),
+    dict(sample_cache=1, fused_pair=1, hook_ilp=2, hook_mode=4, halve=0),
+    dict(sample_cache=1, fused_pair=1, hook_ilp=4, hook_mode=4, halve=0),
+    dict(sample_cache=1, fused_pair=1, hook_ilp=1, hook_mode=4, halve=1),
+    dict(sample_cache=1, fused_pair=1, hook_ilp=2, hook_mode=4, halve=1),
+    dict(sample_cache=1, fused_pair=1, hook_ilp=4, hook_mode=4, halve=1),
+    dict(sample_cache=1, fused_pair=1, hook_ilp=1, hook_mode=1, halve=1),
+    dict(sample_cache=1, fused_pair=1, hook_ilp=2, hook_mode=1, halve=1),
+    dict(sample_cache=1, fused_pair=1, hook_ilp=4, hook_mode=1, halve=1),
+    dict(sample_cache=1, fused_pair=1, hook_ilp=4, hook_mode=4, halve=1, blocks_per_cu=1),
     dict(stream_ilp=2), dict(stream_ilp=4, sample_cache=0),
 ]
-DEFAULTS = dict(sample_cache=1, fused_pair=-1, hook_ilp=0, hook_mode=1, sampled_neighbours=2,
-                stream_ilp=1, blocks_per_cu=0, giant_early=1)
+DEFAULTS = dict(sample_cache=1, fused_pair=-1, hook_ilp=0, hook_mode=4, sampled_neighbours=2,
+                stream_ilp=1, blocks_per_cu=0, giant_early=1, halve=-1)
 
 
 @pytest.mark.parametrize("knobs", PATHS, ids=lambda k: ",".join(f"{a}={b}" for a, b in k.items()))

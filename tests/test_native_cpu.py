@@ -111,14 +111,20 @@ def test_rmat_sizes():
 
 
 def test_plan_defaults():
-    """The default plan: cache on; one-pass hook except on dense graphs
-    beyond 2^30 entries; several roots per thread only on dense graphs."""
+    """The default plan: cache on; one-pass hook; path halving from 2^24
+    vertices; two roots per thread only on dense graphs below 2^31 entries."""
     lib = nat.gpu_lib()
     def plan(n, m):
         v = [ctypes.c_int() for _ in range(4)]
         assert lib.cc_hip_components_plan(n, m, *[ctypes.byref(x) for x in v]) == 0
         return tuple(x.value for x in v)
-    assert plan(4194304, 128313032) == (1, 1, 2, 8)
-    assert plan(65601024, 3377757790) == (1, 0, 4, 8)
-    assert plan(4194304, 33554406) == (1, 1, 1, 4)
-    assert plan(67108864, 187873588) == (1, 1, 1, 4)
+
+    def halve(n, m):
+        return lib.cc_hip_components_plan_get(n, m, b"halve")
+    assert plan(4194304, 128313032) == (1, 1, 2, 8) and halve(4194304, 128313032) == 0
+    assert plan(65601024, 3377757790) == (1, 1, 1, 4) and halve(65601024, 3377757790) == 1
+    assert plan(4194304, 33554406) == (1, 1, 1, 4) and halve(4194304, 33554406) == 0
+    assert plan(67108864, 187873588) == (1, 1, 1, 4) and halve(67108864, 187873588) == 1
+    assert plan(16777216, 520769534) == (1, 1, 2, 8) and halve(16777216, 520769534) == 1
+    assert lib.cc_hip_components_plan_get(1, 1, b"hook_mode") == 4
+    assert lib.cc_hip_components_plan_get(1, 1, b"nonsense") == -1

@@ -49,6 +49,21 @@ def test_batch_equals_alone_equals_oracle(gpu, item_vertices, one_pass):
     eng.close()
 
 
+def test_direct_launch_equals_oracle(gpu):
+    """Small batches solved as single-graph solves on the lane's stream,
+    interleaved with replays on the same lane."""
+    gs = mixed_graphs()
+    want = [oracle_of(g) for g in gs]
+    eng = serving.BatchEngine(max_vertices=1 << 20, max_items=4096, max_graphs=64)
+    for order, direct_max in (([3], 1), ([6], 1), ([0, 5, 1], 3), ([2, 4], 0), ([5], 1)):
+        offs = eng.launch([gs[i] for i in order], 1024, direct_max)
+        eng.sync()
+        for i, (off, n) in zip(order, offs):
+            assert np.array_equal(eng.out.array[off:off + n], want[i]), f"graph {i} in {order}"
+    assert eng.direct_launches == 4
+    eng.close()
+
+
 def test_batch_capacity_errors(gpu):
     eng = serving.BatchEngine(max_vertices=1 << 16, max_items=8, max_graphs=2)
     g = devgen.erdos_renyi(5000, 2.0, seed=1)
@@ -59,7 +74,8 @@ def test_batch_capacity_errors(gpu):
     eng.close()
 
 
-def test_server_requests_match_oracle(gpu):
+@pytest.mark.parametrize("direct_max", [0, 1, 3])
+def test_server_requests_match_oracle(gpu, direct_max):
     keys, loader = serving.make_pool(per_type=3, seed=11, n_min=1024, n_max=200000)
     want = {}
     store = serving.GraphStore(budget_bytes=1 << 32, loader=loader)
@@ -69,10 +85,13 @@ def test_server_requests_match_oracle(gpu):
         store.release(k)
     reqs = serving.poisson_requests(300, qps=3000.0, keys=keys, seed=5)
     srv = serving.Server(store, lanes=2, max_vertices=1 << 22, max_items=1 << 13,
-                         max_graphs=256, item_vertices=1024, keep_labels=True)
+                         max_graphs=256, item_vertices=1024, keep_labels=True,
+                         direct_max=direct_max)
     stats = srv.run(reqs)
     srv.close()
     assert stats["batches"] >= 1
+    # small batches run as single-graph solves on the lane's stream
+    assert (stats["direct_batches"] > 0) == (direct_max > 0)
     for r in reqs:
         assert np.isfinite(r.done) and r.done >= r.arrival
         assert np.array_equal(r.labels, want[r.key]), f"request {r.rid} ({r.key})"
