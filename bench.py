@@ -158,8 +158,8 @@ def kernel_bytes(name: str, n: int, st: dict, plan: dict) -> float | None:
 
 def solve_plan(lib, n: int, m: int) -> dict:
     """What the solve does on this graph (cc_hip_components_plan)."""
-    keys = ("cache", "fused", "halve", "hook_ilp", "hook_blocks_per_cu", "giant_early",
-            "hook_mode")
+    keys = ("cache", "fused", "rem", "rem_splice", "halve", "hook_ilp", "hook_blocks_per_cu",
+            "giant_early", "hook_mode")
     return {k: int(lib.cc_hip_components_plan_get(n, m, k.encode())) for k in keys}
 
 
@@ -439,6 +439,17 @@ def main() -> int:
                             traffic=traffic, algorithmic_bytes=b,
                             avg_us=round(per[dominant]["avg_us"], 2))
 
+    # every solve kernel against the same HBM roofline, in algorithmic bytes
+    kernel_table = {}
+    for k, v in per.items():
+        row = {"launches_per_solve": v["launches"] / args.steps, "avg_us": round(v["avg_us"], 2)}
+        kb = kernel_bytes(k, G.n, st, plan)
+        if kb is not None and v["avg_us"] > 0:
+            gbs = kb / (v["avg_us"] * 1e3)
+            row.update(algorithmic_bytes=kb, algorithmic_gbs=round(gbs, 1),
+                       frac=round(gbs / HBM_PEAK_GBS, 4))
+        kernel_table[k] = row
+
     cpu = None
     parity = None
     w = WORKLOADS[args.workload]
@@ -486,8 +497,7 @@ def main() -> int:
                    "h2d_inclusive_gteps": round(G.m / (h2d_s + ms / 1e3) / 1e9, 4)},
         "roofline": roofline,
         "cpu_baseline": cpu,
-        "kernels": {k: {"launches_per_solve": v["launches"] / args.steps,
-                        "avg_us": round(v["avg_us"], 2)} for k, v in per.items()},
+        "kernels": kernel_table,
         "frontier": st,
         "solve_plan": plan,
         "parity_bit_exact": parity,

@@ -102,23 +102,29 @@ __device__ __forceinline__ void unite(uint32_t a, uint32_t b, uint32_t *p) {
   unite_roots(root_of(a, p), root_of(b, p), p);
 }
 
-#define ITEM_OR_EXIT()                                   \
-  if (blockIdx.x >= hdr->n_items) return;                \
-  const WorkItem it = items[blockIdx.x];                 \
+// Work items are taken grid-stride: a launch has a few workgroups per CU
+// whatever the capacity of the engine, so a small batch does not pay for
+// thousands of workgroups that find no item and exit.
+#define FOR_EACH_ITEM \
+  for (uint32_t item_ = blockIdx.x; item_ < hdr->n_items; item_ += gridDim.x)
+#define ITEM_VARS                        \
+  const WorkItem it = items[item_];      \
   const GraphDesc gd = graphs[it.graph];
 
 __global__ __launch_bounds__(kBlock) void kb_init_min(const Header *__restrict__ hdr,
                                                       const GraphDesc *__restrict__ graphs,
                                                       const WorkItem *__restrict__ items) {
-  ITEM_OR_EXIT();
-  for (uint32_t v = it.v_begin + threadIdx.x; v < it.v_end; v += kBlock) {
-    uint32_t x = v;
-    const uint64_t b = gd.row_ptr[v];
-    if (b < gd.row_ptr[v + 1]) {
-      const uint32_t w = gd.col[b];
-      if (w < x) x = w;
+  FOR_EACH_ITEM {
+    ITEM_VARS
+    for (uint32_t v = it.v_begin + threadIdx.x; v < it.v_end; v += kBlock) {
+      uint32_t x = v;
+      const uint64_t b = gd.row_ptr[v];
+      if (b < gd.row_ptr[v + 1]) {
+        const uint32_t w = gd.col[b];
+        if (w < x) x = w;
+      }
+      gd.labels[v] = x;
     }
-    gd.labels[v] = x;
   }
 }
 
@@ -126,63 +132,110 @@ __global__ __launch_bounds__(kBlock) void kb_hook_neighbour(const Header *__rest
                                                             const GraphDesc *__restrict__ graphs,
                                                             const WorkItem *__restrict__ items,
                                                             uint32_t r) {
-  ITEM_OR_EXIT();
-  for (uint32_t v = it.v_begin + threadIdx.x; v < it.v_end; v += kBlock) {
-    const uint64_t b = gd.row_ptr[v] + r;
-    if (b >= gd.row_ptr[v + 1]) continue;
-    const uint32_t w = gd.col[b];
-    if (w < gd.n && !(r == 0 && w < v)) unite(v, w, gd.labels);
+  FOR_EACH_ITEM {
+    ITEM_VARS
+    for (uint32_t v = it.v_begin + threadIdx.x; v < it.v_end; v += kBlock) {
+      const uint64_t b = gd.row_ptr[v] + r;
+      if (b >= gd.row_ptr[v + 1]) continue;
+      const uint32_t w = gd.col[b];
+      if (w < gd.n && !(r == 0 && w < v)) unite(v, w, gd.labels);
+    }
   }
 }
 
-// Both sampled neighbours in one pass: v's root and the roots of
-// neighbours 0 and 1 are walked in lockstep, then v's tree is hooked with
-// each (neighbour 0 only when it is above v; init linked it otherwise).
-// One pass and one pointer-jumping launch fewer than a round per neighbour.
+// Union by Rem's algorithm (the single-graph solve's rem_unite without
+// splicing): both paths are climbed together, the side with the larger
+// parent first, until the parents meet or the larger side is a root, which
+// is then linked under the other side's parent by a CAS (after one L2 read
+// confirms it is still a root).  Links only ever point a root at a smaller
+// id of its component, so roots stay tree minima; each iteration returns or
+// moves to a smaller id, so it ends.
+__device__ __forceinline__ void rem_unite(uint32_t rx, uint32_t ry, uint32_t *p) {
+  uint32_t px = p[rx], py = p[ry];
+  while (px != py) {
+    if (px < py) {
+      uint32_t t = rx;
+      rx = ry;
+      ry = t;
+      t = px;
+      px = py;
+      py = t;
+    }
+    if (rx == px) {
+      const uint32_t cur = load_l2(&p[rx]);
+      if (cur != rx) {
+        px = cur;
+        continue;
+      }
+      const uint32_t old = atomicCAS(&p[rx], rx, py);
+      if (old == rx) return;
+      px = old;
+      continue;
+    }
+    rx = px;
+    px = p[rx];
+  }
+}
+
+// Both sampled neighbours in one pass (neighbour 0 only when it is above v;
+// init linked it otherwise).  Rem: each by rem_unite.  Otherwise v's root
+// and the roots of neighbours 0 and 1 are walked in lockstep, then v's tree
+// is hooked with each.  One pass and one pointer-jumping launch fewer than
+// a round per neighbour.
+template <bool Rem>
 __global__ __launch_bounds__(kBlock) void kb_hook_pair(const Header *__restrict__ hdr,
                                                        const GraphDesc *__restrict__ graphs,
                                                        const WorkItem *__restrict__ items) {
-  ITEM_OR_EXIT();
-  uint32_t *p = gd.labels;
-  for (uint32_t v = it.v_begin + threadIdx.x; v < it.v_end; v += kBlock) {
-    const uint64_t b = gd.row_ptr[v], e = gd.row_ptr[v + 1];
-    const uint32_t w0 = b < e ? gd.col[b] : 0xFFFFFFFFu;
-    const uint32_t w1 = b + 1 < e ? gd.col[b + 1] : 0xFFFFFFFFu;
-    const bool a0 = w0 < gd.n && w0 > v;
-    const bool a1 = w1 < gd.n && w1 != v;
-    if (!a0 && !a1) continue;
-    uint32_t x[3] = {v, a0 ? w0 : v, a1 ? w1 : v};
-    bool more = true;
-    while (more) {  // the three root walks in lockstep
-      more = false;
+  FOR_EACH_ITEM {
+    ITEM_VARS
+    uint32_t *p = gd.labels;
+    for (uint32_t v = it.v_begin + threadIdx.x; v < it.v_end; v += kBlock) {
+      const uint64_t b = gd.row_ptr[v], e = gd.row_ptr[v + 1];
+      const uint32_t w0 = b < e ? gd.col[b] : 0xFFFFFFFFu;
+      const uint32_t w1 = b + 1 < e ? gd.col[b + 1] : 0xFFFFFFFFu;
+      const bool a0 = w0 < gd.n && w0 > v;
+      const bool a1 = w1 < gd.n && w1 != v;
+      if (!a0 && !a1) continue;
+      if (Rem) {
+        if (a0) rem_unite(v, w0, p);
+        if (a1) rem_unite(v, w1, p);
+        continue;
+      }
+      uint32_t x[3] = {v, a0 ? w0 : v, a1 ? w1 : v};
+      bool more = true;
+      while (more) {  // the three root walks in lockstep
+        more = false;
 #pragma unroll
-      for (int k = 0; k < 3; ++k) {
-        const uint32_t px = p[x[k]];
-        if (px != x[k]) {
-          x[k] = px;
-          more = true;
+        for (int k = 0; k < 3; ++k) {
+          const uint32_t px = p[x[k]];
+          if (px != x[k]) {
+            x[k] = px;
+            more = true;
+          }
         }
       }
+      uint32_t rv = x[0];
+      if (a0 && rv != x[1]) {
+        unite_roots(rv, x[1], p);
+        rv = rv < x[1] ? rv : x[1];  // a member of the merged tree
+      }
+      if (a1 && rv != x[2]) unite_roots(rv, x[2], p);
     }
-    uint32_t rv = x[0];
-    if (a0 && rv != x[1]) {
-      unite_roots(rv, x[1], p);
-      rv = rv < x[1] ? rv : x[1];  // a member of the merged tree
-    }
-    if (a1 && rv != x[2]) unite_roots(rv, x[2], p);
   }
 }
 
 __global__ __launch_bounds__(kBlock) void kb_compress(const Header *__restrict__ hdr,
                                                       const GraphDesc *__restrict__ graphs,
                                                       const WorkItem *__restrict__ items) {
-  ITEM_OR_EXIT();
-  uint32_t *p = gd.labels;
-  for (uint32_t v = it.v_begin + threadIdx.x; v < it.v_end; v += kBlock) {
-    const uint32_t pv = p[v];
-    if (pv == v) continue;
-    const uint32_t r = root_of(pv, p);
-    if (r != pv) p[v] = r;
+  FOR_EACH_ITEM {
+    ITEM_VARS
+    uint32_t *p = gd.labels;
+    for (uint32_t v = it.v_begin + threadIdx.x; v < it.v_end; v += kBlock) {
+      const uint32_t pv = p[v];
+      if (pv == v) continue;
+      const uint32_t r = root_of(pv, p);
+      if (r != pv) p[v] = r;
+    }
   }
 }
 
@@ -194,7 +247,8 @@ __device__ __forceinline__ uint32_t sample_index(uint32_t i, uint32_t n) {
   return (uint32_t)(((z >> 32) * (uint64_t)n) >> 32);
 }
 
-// One workgroup per graph: its most frequent root (ties: smaller root).
+// Each graph's most frequent root (ties: smaller root), one workgroup at a
+// time per graph.
 __global__ __launch_bounds__(kBlock) void kb_sample_giant(const Header *__restrict__ hdr,
                                                           GraphDesc *__restrict__ graphs) {
   constexpr uint32_t kSlots = 2 * kSamples;
@@ -202,46 +256,49 @@ __global__ __launch_bounds__(kBlock) void kb_sample_giant(const Header *__restri
   __shared__ uint32_t keys[kSlots];
   __shared__ uint32_t cnts[kSlots];
   __shared__ uint64_t best[kBlock / kWave];
-  if (blockIdx.x >= hdr->n_graphs) return;
-  GraphDesc &gd = graphs[blockIdx.x];
-  const uint32_t n = gd.n;
-  if (n == 0) return;
-  const int t = threadIdx.x;
-  for (uint32_t i = t; i < kSlots; i += kBlock) {
-    keys[i] = kEmpty;
-    cnts[i] = 0;
-  }
-  __syncthreads();
-  for (int i = t; i < kSamples; i += kBlock) {
-    const uint32_t x = root_of(sample_index(i, n), gd.labels);
-    uint32_t h = (x * 0x9E3779B1u) >> 21;
-    for (;;) {
-      const uint32_t prev = atomicCAS(&keys[h], kEmpty, x);
-      if (prev == kEmpty || prev == x) {
-        atomicAdd(&cnts[h], 1u);
-        break;
+  // grid-stride over the graphs; every barrier below is reached by the
+  // whole workgroup (g is uniform across it)
+  for (uint32_t g = blockIdx.x; g < hdr->n_graphs; g += gridDim.x) {
+    GraphDesc &gd = graphs[g];
+    const uint32_t n = gd.n;
+    if (n == 0) continue;
+    const int t = threadIdx.x;
+    for (uint32_t i = t; i < kSlots; i += kBlock) {
+      keys[i] = kEmpty;
+      cnts[i] = 0;
+    }
+    __syncthreads();
+    for (int i = t; i < kSamples; i += kBlock) {
+      const uint32_t x = root_of(sample_index(i, n), gd.labels);
+      uint32_t h = (x * 0x9E3779B1u) >> 21;
+      for (;;) {
+        const uint32_t prev = atomicCAS(&keys[h], kEmpty, x);
+        if (prev == kEmpty || prev == x) {
+          atomicAdd(&cnts[h], 1u);
+          break;
+        }
+        h = (h + 1) & (kSlots - 1);
       }
-      h = (h + 1) & (kSlots - 1);
     }
-  }
-  __syncthreads();
-  uint64_t mine = 0;
-  for (uint32_t i = t; i < kSlots; i += kBlock)
-    if (cnts[i]) {
-      const uint64_t cand = ((uint64_t)cnts[i] << 32) | (uint32_t)~keys[i];
-      if (cand > mine) mine = cand;
+    __syncthreads();
+    uint64_t mine = 0;
+    for (uint32_t i = t; i < kSlots; i += kBlock)
+      if (cnts[i]) {
+        const uint64_t cand = ((uint64_t)cnts[i] << 32) | (uint32_t)~keys[i];
+        if (cand > mine) mine = cand;
+      }
+    for (int d = kWave / 2; d > 0; d >>= 1) {
+      const uint64_t o = __shfl_xor(mine, d, kWave);
+      if (o > mine) mine = o;
     }
-  for (int d = kWave / 2; d > 0; d >>= 1) {
-    const uint64_t o = __shfl_xor(mine, d, kWave);
-    if (o > mine) mine = o;
-  }
-  if ((t & (kWave - 1)) == 0) best[t / kWave] = mine;
-  __syncthreads();
-  if (t == 0) {
-    uint64_t b = best[0];
-    for (int w = 1; w < kBlock / kWave; ++w)
-      if (best[w] > b) b = best[w];
-    gd.giant = ~(uint32_t)b;
+    if ((t & (kWave - 1)) == 0) best[t / kWave] = mine;
+    __syncthreads();
+    if (t == 0) {
+      uint64_t b = best[0];
+      for (int w = 1; w < kBlock / kWave; ++w)
+        if (best[w] > b) b = best[w];
+      gd.giant = ~(uint32_t)b;
+    }
   }
 }
 
@@ -251,64 +308,70 @@ __global__ __launch_bounds__(kBlock) void kb_hook_frontier(const Header *__restr
                                                            uint32_t k_first) {
   __shared__ uint64_t s_excl[kBlock / kWave][kWave];
   __shared__ uint64_t s_begin[kBlock / kWave][kWave];
-  ITEM_OR_EXIT();
-  uint32_t *p = gd.labels;
-  const int lane = threadIdx.x & (kWave - 1);
-  const int wid = threadIdx.x / kWave;
-  for (uint32_t base = it.v_begin + wid * kWave; base < it.v_end; base += kBlock) {
-    const uint32_t v = base + lane;
-    uint64_t work = 0, begin = 0;
-    if (v < it.v_end) {
-      const uint32_t pv = p[v];
-      bool in_giant = false;
-      const uint32_t r = root_via(v, p, gd.giant, in_giant);
-      if (r != pv) p[v] = r;
-      if (!in_giant) {
-        const uint64_t b = gd.row_ptr[v] + k_first, e = gd.row_ptr[v + 1];
-        if (e > b) {
-          work = e - b;
-          begin = b;
+  FOR_EACH_ITEM {
+    ITEM_VARS
+    uint32_t *p = gd.labels;
+    const int lane = threadIdx.x & (kWave - 1);
+    const int wid = threadIdx.x / kWave;
+    for (uint32_t base = it.v_begin + wid * kWave; base < it.v_end; base += kBlock) {
+      const uint32_t v = base + lane;
+      uint64_t work = 0, begin = 0;
+      if (v < it.v_end) {
+        const uint32_t pv = p[v];
+        bool in_giant = false;
+        const uint32_t r = root_via(v, p, gd.giant, in_giant);
+        if (r != pv) p[v] = r;
+        if (!in_giant) {
+          const uint64_t b = gd.row_ptr[v] + k_first, e = gd.row_ptr[v + 1];
+          if (e > b) {
+            work = e - b;
+            begin = b;
+          }
         }
       }
-    }
-    uint64_t incl = work;
+      uint64_t incl = work;
 #pragma unroll
-    for (int d = 1; d < kWave; d <<= 1) {
-      const uint64_t up = __shfl_up(incl, d, kWave);
-      if (lane >= d) incl += up;
-    }
-    const uint64_t total = __shfl(incl, kWave - 1, kWave);
-    if (total == 0) continue;
-    s_excl[wid][lane] = incl - work;
-    s_begin[wid][lane] = begin;
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-    __builtin_amdgcn_wave_barrier();
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-    const uint64_t *ex = s_excl[wid];
-    for (uint64_t e = lane; e < total; e += kWave) {
-      int j = 0;
+      for (int d = 1; d < kWave; d <<= 1) {
+        const uint64_t up = __shfl_up(incl, d, kWave);
+        if (lane >= d) incl += up;
+      }
+      const uint64_t total = __shfl(incl, kWave - 1, kWave);
+      if (total == 0) continue;
+      s_excl[wid][lane] = incl - work;
+      s_begin[wid][lane] = begin;
+      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+      __builtin_amdgcn_wave_barrier();
+      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+      const uint64_t *ex = s_excl[wid];
+      for (uint64_t e = lane; e < total; e += kWave) {
+        int j = 0;
 #pragma unroll
-      for (int step = kWave / 2; step > 0; step >>= 1)
-        if (ex[j + step] <= e) j += step;
-      const uint32_t w = gd.col[s_begin[wid][j] + (e - ex[j])];
-      if (w < gd.n) unite(base + j, w, p);
+        for (int step = kWave / 2; step > 0; step >>= 1)
+          if (ex[j + step] <= e) j += step;
+        const uint32_t w = gd.col[s_begin[wid][j] + (e - ex[j])];
+        if (w < gd.n) unite(base + j, w, p);
+      }
+      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+      __builtin_amdgcn_wave_barrier();
+      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
     }
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-    __builtin_amdgcn_wave_barrier();
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
   }
 }
 
-int g_batch_one_pass = 1;  // hook both sampled neighbours in one pass (read at create)
+// Engine knobs, read when an engine is created (its graph is captured then).
+int g_batch_one_pass = 1;    // hook both sampled neighbours in one pass
+int g_batch_rem = 1;         // ... by Rem's unions (else lockstep root walks)
+int g_batch_grid_per_cu = 8;  // workgroups per CU of each launch (0: one per item)
 
 }  // namespace
 
 extern "C" int cc_hip_batch_set(const char *key, int value) {
-  if (key && strcmp(key, "one_pass") == 0 && (value == 0 || value == 1)) {
-    g_batch_one_pass = value;
-    return 0;
-  }
-  return ccx::set_error(4, "cc_hip_batch_set: bad key or value (%s=%d)", key ? key : "", value);
+  const char *k = key ? key : "";
+  if (strcmp(k, "one_pass") == 0 && (value == 0 || value == 1)) g_batch_one_pass = value;
+  else if (strcmp(k, "rem") == 0 && (value == 0 || value == 1)) g_batch_rem = value;
+  else if (strcmp(k, "grid_per_cu") == 0 && value >= 0 && value <= 64) g_batch_grid_per_cu = value;
+  else return ccx::set_error(4, "cc_hip_batch_set: bad key or value (%s=%d)", k, value);
+  return 0;
 }
 
 struct cc_hip_batch_s {
@@ -347,20 +410,27 @@ extern "C" int cc_hip_batch_create(uint32_t max_items, uint32_t max_graphs, cc_h
   GraphDesc *graphs = (GraphDesc *)(b->d_tables + b->off_graphs);
   const WorkItem *items = (const WorkItem *)(b->d_tables + b->off_items);
   hipStream_t s = b->stream;
+  // launch widths: the kernels loop over items (graphs) grid-stride
+  const uint32_t cus = (uint32_t)(ccx::ctx()->cu_count > 0 ? ccx::ctx()->cu_count : 256);
+  const uint32_t cap = g_batch_grid_per_cu ? cus * (uint32_t)g_batch_grid_per_cu : max_items;
+  const uint32_t gi = max_items < cap ? max_items : cap;
+  const uint32_t gg = max_graphs < cap ? max_graphs : cap;
   CCX_TRY(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
   hipError_t e = hipMemcpyAsync(b->d_tables, b->h_tables, b->table_bytes, hipMemcpyHostToDevice, s);
   if (e == hipSuccess) {
-    kb_init_min<<<max_items, kBlock, 0, s>>>(hdr, graphs, items);
-    if (g_batch_one_pass) {
-      kb_hook_pair<<<max_items, kBlock, 0, s>>>(hdr, graphs, items);
+    kb_init_min<<<gi, kBlock, 0, s>>>(hdr, graphs, items);
+    if (g_batch_one_pass && g_batch_rem) {
+      kb_hook_pair<true><<<gi, kBlock, 0, s>>>(hdr, graphs, items);
+    } else if (g_batch_one_pass) {
+      kb_hook_pair<false><<<gi, kBlock, 0, s>>>(hdr, graphs, items);
     } else {
-      kb_hook_neighbour<<<max_items, kBlock, 0, s>>>(hdr, graphs, items, 0u);
-      kb_compress<<<max_items, kBlock, 0, s>>>(hdr, graphs, items);
-      kb_hook_neighbour<<<max_items, kBlock, 0, s>>>(hdr, graphs, items, 1u);
+      kb_hook_neighbour<<<gi, kBlock, 0, s>>>(hdr, graphs, items, 0u);
+      kb_compress<<<gi, kBlock, 0, s>>>(hdr, graphs, items);
+      kb_hook_neighbour<<<gi, kBlock, 0, s>>>(hdr, graphs, items, 1u);
     }
-    kb_sample_giant<<<max_graphs, kBlock, 0, s>>>(hdr, graphs);
-    kb_hook_frontier<<<max_items, kBlock, 0, s>>>(hdr, graphs, items, 2u);
-    kb_compress<<<max_items, kBlock, 0, s>>>(hdr, graphs, items);
+    kb_sample_giant<<<gg, kBlock, 0, s>>>(hdr, graphs);
+    kb_hook_frontier<<<gi, kBlock, 0, s>>>(hdr, graphs, items, 2u);
+    kb_compress<<<gi, kBlock, 0, s>>>(hdr, graphs, items);
   }
   hipError_t e2 = hipStreamEndCapture(s, &b->graph);
   if (e != hipSuccess || e2 != hipSuccess) {

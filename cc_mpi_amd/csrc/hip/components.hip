@@ -476,6 +476,66 @@ __global__ __launch_bounds__(kBlock) void k_hook_pair(const uint32_t *__restrict
   }
 }
 
+// Union of the trees of x and y by climbing both paths together (Rem's
+// algorithm, min-root form): always step the side whose parent is larger;
+// stop as soon as both sides have the same parent (same tree: no need to
+// reach the root) or when the larger side is a root, which is then linked
+// under the other side's parent by a CAS from itself (after one L2 read
+// confirms it is still a root).  With Splice, each step first lowers the
+// entry it leaves to the other side's parent (an atomic min, so an entry
+// never rises): the walk the next union makes through it is shorter.
+// Every link and splice points an entry at a smaller id of the same
+// component once this union completes, and the union completes in this
+// call, so the forest keeps p[v] <= v and each root is its tree's minimum.
+// Each iteration returns, or moves the larger side to its parent, or
+// learns a smaller parent for it from a failed CAS, so the loop ends.
+template <bool Splice>
+__device__ __forceinline__ void rem_unite(uint32_t rx, uint32_t ry, uint32_t *__restrict__ p) {
+  uint32_t px = p[rx], py = p[ry];
+  while (px != py) {
+    if (px < py) {
+      uint32_t t = rx;
+      rx = ry;
+      ry = t;
+      t = px;
+      px = py;
+      py = t;
+    }
+    if (rx == px) {  // rx looks like a root; py < rx
+      const uint32_t cur = load_l2(&p[rx]);
+      if (cur != rx) {
+        px = cur;
+        continue;
+      }
+      const uint32_t old = atomicCAS(&p[rx], rx, py);
+      if (old == rx) return;
+      px = old;
+      continue;
+    }
+    if (Splice)
+      (void)__hip_atomic_fetch_min(&p[rx], py, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    rx = px;
+    px = p[rx];
+  }
+}
+
+// Both sampled neighbours of every vertex united with it by rem_unite (one
+// pass, no lockstep: each union ends as soon as the two paths meet, which
+// on graphs whose trees are shallow and hub-centred is well before the
+// roots).
+template <bool Splice>
+__global__ __launch_bounds__(kBlock) void k_hook_rem(const uint32_t *__restrict__ s0,
+                                                     const uint32_t *__restrict__ s1, uint32_t n,
+                                                     uint32_t *__restrict__ p) {
+  CCX_GRID_STRIDE(v, n) {
+    const uint32_t w0 = s0[v];
+    const uint32_t w1 = s1[v];
+    // neighbour 0 below v was linked by the init kernel already
+    if (w0 < n && w0 > v) rem_unite<Splice>(v, w0, p);
+    if (w1 < n && w1 != v) rem_unite<Splice>(v, w1, p);
+  }
+}
+
 __device__ __forceinline__ uint32_t sample_vertex(uint32_t i, uint32_t n) {
   // SplitMix64 finaliser of the sample index, reduced to [0, n).
   uint64_t z = (uint64_t)(i + 1) * 0x9E3779B97F4A7C15ull;
@@ -637,6 +697,8 @@ int g_stream_ilp = 1;              // vertices per thread per step in init/compr
 int g_sample_cache = 1;            // sampled neighbours read once into coalesced arrays
 int g_fused_pair = -1;             // both sampled rounds in one pass (-1 = auto)
 int g_halve = -1;                  // path halving in the one-pass hook's walks (-1 = auto)
+int g_rem = -1;                    // one-pass hook by Rem's unions (-1 = auto)
+int g_rem_splice = -1;             // splicing in Rem's unions (-1 = auto)
 int g_giant_early = 1;             // two-pass path: sample the giant after round 0, skip it in round 1
 constexpr uint64_t kDenseDegree = 16;              // dense: m >= 16 n
 constexpr uint64_t kHalveMinVertices = 1ull << 24;  // halving from this many vertices
@@ -645,20 +707,26 @@ constexpr uint64_t kWideIlpMaxEntries = 1ull << 31; // dense one-pass ILP 2 belo
 struct Plan {
   int cache;     // sampled-neighbour cache in use
   int fused;     // one-pass hook of both sampled neighbours
-  int halve;     // path halving in the one-pass hook
-  int hook_ilp;  // roots per thread in the sampled hook
+  int rem;       // the one pass unites by Rem's algorithm (k_hook_rem)
+  int splice;    // ... with splicing
+  int halve;     // path halving in the lockstep one-pass hook (k_hook_pair)
+  int hook_ilp;  // roots per thread in the lockstep sampled hook
   int hook_bpc;  // sampled-hook grid cap, workgroups per CU
 };
 
 inline Plan plan_for(uint32_t n, uint64_t m) {
   Plan pl;
   const bool dense = m >= kDenseDegree * (uint64_t)n;
+  const bool big = n >= kHalveMinVertices;
   pl.cache = g_sample_cache && (g_hook_mode == 0 || g_hook_mode == 1 || g_hook_mode == 4);
   pl.fused = pl.cache && g_sampled_neighbours == 2 && (g_fused_pair < 0 ? 1 : g_fused_pair);
-  pl.halve = pl.fused && g_hook_mode != 0 && (g_halve < 0 ? n >= kHalveMinVertices : g_halve);
+  pl.rem = pl.fused && g_hook_mode != 0 && (g_rem < 0 ? (!big || !dense) : g_rem);
+  pl.splice = pl.rem && (g_rem_splice < 0 ? big : g_rem_splice);
+  pl.halve = pl.fused && !pl.rem && g_hook_mode != 0 && (g_halve < 0 ? big : g_halve);
   const int wide = pl.fused ? (m < kWideIlpMaxEntries ? 2 : 1) : 4;
-  pl.hook_ilp = g_hook_ilp ? g_hook_ilp : (dense ? wide : 1);
-  pl.hook_bpc = g_blocks_per_cu ? g_blocks_per_cu : (pl.hook_ilp >= 2 ? 8 : 4);
+  pl.hook_ilp = pl.rem ? 1 : (g_hook_ilp ? g_hook_ilp : (dense ? wide : 1));
+  const int bpc_auto = pl.rem ? (pl.splice ? 4 : 8) : (pl.hook_ilp >= 2 ? 8 : 4);
+  pl.hook_bpc = g_blocks_per_cu ? g_blocks_per_cu : bpc_auto;
   return pl;
 }
 
@@ -675,6 +743,8 @@ extern "C" int cc_hip_components_set(const char *key, int value) {
   else if (k == "frontier_blocks_per_cu" && value >= 1 && value <= 64) g_frontier_blocks_per_cu = value;
   else if (k == "hook_mode" && value >= 0 && value <= 4) g_hook_mode = value;
   else if (k == "halve" && value >= -1 && value <= 1) g_halve = value;
+  else if (k == "rem" && value >= -1 && value <= 1) g_rem = value;
+  else if (k == "rem_splice" && value >= -1 && value <= 1) g_rem_splice = value;
   else if (k == "hook_ilp" && (value == 0 || ilp_ok(value))) g_hook_ilp = value;
   else if (k == "stream_ilp" && ilp_ok(value)) g_stream_ilp = value;
   else if (k == "sample_cache" && (value == 0 || value == 1)) g_sample_cache = value;
@@ -697,6 +767,8 @@ extern "C" int cc_hip_components_get(const char *key) {
   if (k == "fused_pair") return g_fused_pair;
   if (k == "giant_early") return g_giant_early;
   if (k == "halve") return g_halve;
+  if (k == "rem") return g_rem;
+  if (k == "rem_splice") return g_rem_splice;
   return -1;
 }
 
@@ -706,6 +778,8 @@ extern "C" int cc_hip_components_plan_get(uint32_t n, uint64_t m, const char *ke
   if (k == "cache") return pl.cache;
   if (k == "fused") return pl.fused;
   if (k == "halve") return pl.halve;
+  if (k == "rem") return pl.rem;
+  if (k == "rem_splice") return pl.splice;
   if (k == "hook_ilp") return pl.hook_ilp;
   if (k == "hook_blocks_per_cu") return pl.hook_bpc;
   if (k == "giant_early") return pl.cache && !pl.fused && g_giant_early;
@@ -800,6 +874,10 @@ extern "C" int cc_hip_components_device(const uint64_t *d_row_ptr, const uint32_
   } while (0)
     if (mode == 0)
       CCX_LAUNCH("cc_hook_sample", s, k_hook_pair<0, 2><<<nb, kBlock, 0, s>>>(smp, s1, n, d_labels));
+    else if (plan.rem && plan.splice)
+      CCX_LAUNCH("cc_hook_sample", s, k_hook_rem<true><<<nb, kBlock, 0, s>>>(smp, s1, n, d_labels));
+    else if (plan.rem)
+      CCX_LAUNCH("cc_hook_sample", s, k_hook_rem<false><<<nb, kBlock, 0, s>>>(smp, s1, n, d_labels));
     else if (mode == 1 && plan.halve) CCX_HOOK_PAIR(1, true);
     else if (mode == 1) CCX_HOOK_PAIR(1, false);
     else if (plan.halve) CCX_HOOK_PAIR(4, true);

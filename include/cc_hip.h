@@ -97,16 +97,27 @@ int cc_hip_components_tune(int sampled_neighbours, int blocks_per_cu, int hook_m
  *   "fused_pair"              1 or -1 (default): hook both sampled
  *                             neighbours in one pass (needs the cache and 2
  *                             rounds); 0: one pass per round
+ *   "rem"                     the one-pass hook unites each vertex with its
+ *                             sampled neighbours by Rem's algorithm (both
+ *                             paths climbed together, stopping where they
+ *                             meet) instead of walking all three roots in
+ *                             lockstep: 1, 0, or -1 (default) = on when
+ *                             n < 2^24 or m < 16 n; not in mode 0
+ *   "rem_splice"              splicing in those unions (each step lowers the
+ *                             entry it leaves to the other side's parent):
+ *                             1, 0, or -1 (default) = on when n >= 2^24
  *   "halve"                   path halving (atomic min of each visited
- *                             entry to its grandparent) in the one-pass
- *                             hook's root walks: 1, 0, or -1 (default) =
- *                             on when n >= 2^24; not in mode 0
+ *                             entry to its grandparent) in the lockstep
+ *                             one-pass hook's root walks: 1, 0, or -1
+ *                             (default) = on when n >= 2^24; not in mode 0
  *   "giant_early"             1 (default): on the two-pass cached path,
  *                             sample the giant root after round 0 and skip
  *                             its members in round 1; 0: sample it once,
  *                             after the last round
- *   "blocks_per_cu"           sampled-hook grid cap; 0 (default) = 8 when the
- *                             hook walks several roots per thread, else 4
+ *   "blocks_per_cu"           sampled-hook grid cap; 0 (default) = 8 for
+ *                             Rem's unions without splicing and when the
+ *                             lockstep hook walks several roots per thread,
+ *                             else 4
  *   "hook_ilp"                roots walked per thread in lockstep in the
  *                             sampled hook: 1, 2, 4, or 0 (default) = 1 when
  *                             m < 16 n, else 2 in the one-pass hook (1 from
@@ -125,9 +136,9 @@ int cc_hip_components_get(const char *key);
  * sampled hook.  Any pointer may be NULL.  Returns 0. */
 int cc_hip_components_plan(uint32_t n, uint64_t m, int *cache, int *fused, int *hook_ilp,
                            int *hook_blocks_per_cu);
-/* One field of that plan by name: "cache", "fused", "halve", "hook_ilp",
- * "hook_blocks_per_cu", "giant_early" (in effect for this graph),
- * "hook_mode".  Returns -1 for an unknown key. */
+/* One field of that plan by name: "cache", "fused", "rem", "rem_splice",
+ * "halve", "hook_ilp", "hook_blocks_per_cu", "giant_early" (in effect for
+ * this graph), "hook_mode".  Returns -1 for an unknown key. */
 int cc_hip_components_plan_get(uint32_t n, uint64_t m, const char *key);
 
 /* Same result through host buffers (H2D, solve, D2H; blocking). */
@@ -300,14 +311,18 @@ int cc_hip_rank_labels_device(const uint32_t *d_comp_label, const uint32_t *d_co
 
 typedef struct cc_hip_batch_s *cc_hip_batch_t;
 
-/* A batch engine with room for max_items work items (workgroups) and
- * max_graphs graphs per launch: its own stream, device tables, pinned
- * mirrors and an instantiated HIP graph of the whole solve. */
+/* A batch engine with room for max_items work items (vertex ranges of one
+ * graph each) and max_graphs graphs per launch: its own stream, device
+ * tables, pinned mirrors and an instantiated HIP graph of the whole solve. */
 int cc_hip_batch_create(uint32_t max_items, uint32_t max_graphs, cc_hip_batch_t *out);
 int cc_hip_batch_destroy(cc_hip_batch_t b);
 /* Solve shape captured by engines created afterwards: "one_pass" 1
  * (default) hooks both sampled neighbours in one pass, 0 a pass per
- * neighbour with pointer jumping between.  Results never depend on it. */
+ * neighbour with pointer jumping between; "rem" 1 (default) makes that one
+ * pass unite by Rem's algorithm, 0 by lockstep root walks; "grid_per_cu"
+ * (default 8, 0..64) workgroups per CU of each launch, which loops over
+ * the work items, 0 = one workgroup per item of the engine's capacity.
+ * Results never depend on them. */
 int cc_hip_batch_set(const char *key, int value);
 /*
  * Solve n_graphs resident graphs (symmetric CSRs d_row_ptr[g], d_col_idx[g]

@@ -91,16 +91,20 @@ def test_chung_lu(gpu):
     check_labels(G, dropin(G))
 
 
-@pytest.mark.parametrize("kind", ["er", "grid"])
-def test_halving_plan_at_scale(gpu, kind):
-    """Graphs of 2^24 vertices take the one-pass hook with path halving by
-    default; labels, component count and size histogram equal the oracle's."""
+@pytest.mark.parametrize("kind,rem,splice,halve", [
+    ("er", 1, 1, 0), ("grid", 1, 1, 0), ("rmat24", 0, 0, 1)])
+def test_default_plan_at_scale(gpu, kind, rem, splice, halve):
+    """Graphs of 2^24 vertices: sparse ones take Rem's unions with splicing,
+    dense ones the lockstep hook with path halving, by default; labels,
+    component count and size histogram equal the oracle's."""
     from cc_mpi_amd import devgen
     lib = nat.gpu_lib()
     D = (devgen.erdos_renyi(1 << 24, 3.0, seed=41) if kind == "er"
-         else devgen.grid(4096, 4096, 0.7, seed=42))
-    assert lib.cc_hip_components_plan_get(D.n, D.m, b"halve") == 1
-    assert lib.cc_hip_components_plan_get(D.n, D.m, b"fused") == 1
+         else devgen.grid(4096, 4096, 0.7, seed=42) if kind == "grid"
+         else devgen.rmat(24, 16, seed=43))
+    plan = {k: lib.cc_hip_components_plan_get(D.n, D.m, k.encode())
+            for k in ("fused", "rem", "rem_splice", "halve")}
+    assert plan == dict(fused=1, rem=rem, rem_splice=splice, halve=halve)
     got = D.solve()
     G = D.to_host()
     D.close()
@@ -150,8 +154,8 @@ PATHS = [  # every code path of the solve: (knobs) -> same labels
     dict(sample_cache=1, fused_pair=0, hook_ilp=4, hook_mode=1, giant_early=0),
     dict(sample_cache=1, fused_pair=0, hook_ilp=1, hook_mode=0, giant_early=0),
     dict(sample_cache=1, fused_pair=0, hook_ilp=2, hook_mode=1, giant_early=1, sampled_neighbours=4),
-    dict(sample_cache=1, fused_pair=1, hook_ilp=1, hook_mode=1),
-    dict(sample_cache=1, fused_pair=1, hook_ilp=2, hook_mode=1),
+    dict(sample_cache=1, fused_pair=1, hook_ilp=1, hook_mode=1, rem=0),
+    dict(sample_cache=1, fused_pair=1, hook_ilp=2, hook_mode=1, rem=0),
     dict(sample_cache=1, fused_pair=1, hook_ilp=1, hook_mode=0),
     dict(sample_cache=1, fused_pair=1, hook_ilp=2, hook_mode=1, sampled_neighbours=1),
     dict(sample_cache=1, fused_pair=0, hook_ilp=4, hook_mode=1, sampled_neighbours=3),
@@ -162,20 +166,25 @@ PATHS = [  # every code path of the solve: (knobs) -> same labels
     dict(sample_cache=1, fused_pair=0, hook_ilp=1, hook_mode=4),
     dict(sample_cache=1, fused_pair=0, hook_ilp=2, hook_mode=4, giant_early=0),
     dict(sample_cache=1, fused_pair=0, hook_ilp=4, hook_mode=4),
-    dict(sample_cache=1, fused_pair=1, hook_ilp=1, hook_mode=4, halve=0),
-    dict(sample_cache=1, fused_pair=1, hook_ilp=2, hook_mode=4, halve=0),
-    dict(sample_cache=1, fused_pair=1, hook_ilp=4, hook_mode=4, halve=0),
-    dict(sample_cache=1, fused_pair=1, hook_ilp=1, hook_mode=4, halve=1),
-    dict(sample_cache=1, fused_pair=1, hook_ilp=2, hook_mode=4, halve=1),
-    dict(sample_cache=1, fused_pair=1, hook_ilp=4, hook_mode=4, halve=1),
-    dict(sample_cache=1, fused_pair=1, hook_ilp=1, hook_mode=1, halve=1),
-    dict(sample_cache=1, fused_pair=1, hook_ilp=2, hook_mode=1, halve=1),
-    dict(sample_cache=1, fused_pair=1, hook_ilp=4, hook_mode=1, halve=1),
-    dict(sample_cache=1, fused_pair=1, hook_ilp=4, hook_mode=4, halve=1, blocks_per_cu=1),
+    dict(sample_cache=1, fused_pair=1, hook_ilp=1, hook_mode=4, halve=0, rem=0),
+    dict(sample_cache=1, fused_pair=1, hook_ilp=2, hook_mode=4, halve=0, rem=0),
+    dict(sample_cache=1, fused_pair=1, hook_ilp=4, hook_mode=4, halve=0, rem=0),
+    dict(sample_cache=1, fused_pair=1, hook_ilp=1, hook_mode=4, halve=1, rem=0),
+    dict(sample_cache=1, fused_pair=1, hook_ilp=2, hook_mode=4, halve=1, rem=0),
+    dict(sample_cache=1, fused_pair=1, hook_ilp=4, hook_mode=4, halve=1, rem=0),
+    dict(sample_cache=1, fused_pair=1, hook_ilp=1, hook_mode=1, halve=1, rem=0),
+    dict(sample_cache=1, fused_pair=1, hook_ilp=2, hook_mode=1, halve=1, rem=0),
+    dict(sample_cache=1, fused_pair=1, hook_ilp=4, hook_mode=1, halve=1, rem=0),
+    dict(sample_cache=1, fused_pair=1, hook_ilp=4, hook_mode=4, halve=1, rem=0, blocks_per_cu=1),
+    dict(sample_cache=1, fused_pair=1, hook_mode=4, rem=1, rem_splice=0),
+    dict(sample_cache=1, fused_pair=1, hook_mode=4, rem=1, rem_splice=1),
+    dict(sample_cache=1, fused_pair=1, hook_mode=1, rem=1, rem_splice=1),
+    dict(sample_cache=1, fused_pair=1, hook_mode=4, rem=1, rem_splice=1, blocks_per_cu=1),
+    dict(sample_cache=1, fused_pair=1, hook_mode=4, rem=0, halve=0),
     dict(stream_ilp=2), dict(stream_ilp=4, sample_cache=0),
 ]
 DEFAULTS = dict(sample_cache=1, fused_pair=-1, hook_ilp=0, hook_mode=4, sampled_neighbours=2,
-                stream_ilp=1, blocks_per_cu=0, giant_early=1, halve=-1)
+                stream_ilp=1, blocks_per_cu=0, giant_early=1, halve=-1, rem=-1, rem_splice=-1)
 
 
 @pytest.mark.parametrize("knobs", PATHS, ids=lambda k: ",".join(f"{a}={b}" for a, b in k.items()))

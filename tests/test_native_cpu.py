@@ -111,20 +111,24 @@ def test_rmat_sizes():
 
 
 def test_plan_defaults():
-    """The default plan: cache on; one-pass hook; path halving from 2^24
-    vertices; two roots per thread only on dense graphs below 2^31 entries."""
+    """The default plan: cache on; one-pass hook; Rem's unions below 2^24
+    vertices (8 workgroups per CU) and on sparse graphs above (with
+    splicing, 4 per CU); otherwise the lockstep hook with path halving, two
+    roots per thread below 2^31 entries."""
     lib = nat.gpu_lib()
     def plan(n, m):
         v = [ctypes.c_int() for _ in range(4)]
         assert lib.cc_hip_components_plan(n, m, *[ctypes.byref(x) for x in v]) == 0
         return tuple(x.value for x in v)
 
-    def halve(n, m):
-        return lib.cc_hip_components_plan_get(n, m, b"halve")
-    assert plan(4194304, 128313032) == (1, 1, 2, 8) and halve(4194304, 128313032) == 0
-    assert plan(65601024, 3377757790) == (1, 1, 1, 4) and halve(65601024, 3377757790) == 1
-    assert plan(4194304, 33554406) == (1, 1, 1, 4) and halve(4194304, 33554406) == 0
-    assert plan(67108864, 187873588) == (1, 1, 1, 4) and halve(67108864, 187873588) == 1
-    assert plan(16777216, 520769534) == (1, 1, 2, 8) and halve(16777216, 520769534) == 1
+    def get(n, m):
+        return tuple(lib.cc_hip_components_plan_get(n, m, k) for k in (b"rem", b"rem_splice",
+                                                                         b"halve"))
+    # R-MAT 22, Chung-Lu 65.6M, ER 4M degree 8, 8192^2 grid, R-MAT 24
+    assert plan(4194304, 128313032) == (1, 1, 1, 8) and get(4194304, 128313032) == (1, 0, 0)
+    assert plan(65601024, 3377757790) == (1, 1, 1, 4) and get(65601024, 3377757790) == (0, 0, 1)
+    assert plan(4194304, 33554406) == (1, 1, 1, 8) and get(4194304, 33554406) == (1, 0, 0)
+    assert plan(67108864, 187873588) == (1, 1, 1, 4) and get(67108864, 187873588) == (1, 1, 0)
+    assert plan(16777216, 520769534) == (1, 1, 2, 8) and get(16777216, 520769534) == (0, 0, 1)
     assert lib.cc_hip_components_plan_get(1, 1, b"hook_mode") == 4
     assert lib.cc_hip_components_plan_get(1, 1, b"nonsense") == -1

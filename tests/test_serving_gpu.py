@@ -23,18 +23,24 @@ def oracle_of(g):
     return oracle.cc_min_label(h.row_ptr, h.col_idx)
 
 
-@pytest.mark.parametrize("one_pass", [1, 0])
+ENGINE_SHAPES = [  # (one_pass, rem, grid_per_cu)
+    (1, 1, 8), (1, 0, 8), (0, 1, 8), (1, 1, 0), (1, 0, 1), (0, 0, 1)]
+
+
+@pytest.mark.parametrize("shape", ENGINE_SHAPES, ids=lambda s: "one_pass=%d,rem=%d,grid=%d" % s)
 @pytest.mark.parametrize("item_vertices", [256, 1024, 65536])
-def test_batch_equals_alone_equals_oracle(gpu, item_vertices, one_pass):
+def test_batch_equals_alone_equals_oracle(gpu, item_vertices, shape):
     gs = mixed_graphs()
     want = [oracle_of(g) for g in gs]
     alone = [g.solve() for g in gs]
     lib = serving._lib()
-    assert lib.cc_hip_batch_set(b"one_pass", one_pass) == 0
+    for k, v in zip((b"one_pass", b"rem", b"grid_per_cu"), shape):
+        assert lib.cc_hip_batch_set(k, v) == 0
     try:
         eng = serving.BatchEngine(max_vertices=1 << 20, max_items=4096, max_graphs=64)
     finally:
-        lib.cc_hip_batch_set(b"one_pass", 1)
+        for k, v in ((b"one_pass", 1), (b"rem", 1), (b"grid_per_cu", 8)):
+            lib.cc_hip_batch_set(k, v)
     # several replays of the captured graph with different contents, with
     # eager work in between
     for order in ([0, 1, 2, 3, 4, 5, 6], [6, 5, 4], [2, 2, 0, 3], [1]):
