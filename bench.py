@@ -147,7 +147,14 @@ def kernel_bytes(name: str, n: int, st: dict, plan: dict) -> float | None:
         # neighbour r, one parent gather per hooked neighbour; averaged
         return 8.0 * (n + 1) + 4.0 * n + 2.0 * (d0 + d1) + 2.0 * (up0 + d1)
     if name == "cc_compress":
-        return 12.0 * n
+        # The last pass follows the frontier sweep, which already pointed
+        # every vertex it resolved at its root: it reads every entry and,
+        # for each non-root, the parent's entry to see that it is a root
+        # (`components` = roots of the final forest); nothing need be
+        # written.  On the two-pass path a full pass (read, gather, write
+        # back: 12n) runs between the rounds as well; averaged.
+        last = 4.0 * n + 4.0 * (n - st["components"])
+        return last if plan["fused"] else (12.0 * n + last) / 2.0
     if name == "cc_hook_frontier":
         # every vertex resolves its root (own entry + one gather, written back);
         # frontier vertices read their row offsets, then one col entry and one
@@ -420,6 +427,7 @@ def main() -> int:
     D.close()
 
     st = frontier_stats(G)
+    st["components"] = int(np.count_nonzero(labels == np.arange(G.n, dtype=np.uint32)))
     log(f"frontier: {st}")
     plan = solve_plan(lib, G.n, G.m)
     log(f"solve plan: {plan}")
