@@ -476,21 +476,25 @@ __global__ __launch_bounds__(kBlock) void k_hook_pair(const uint32_t *__restrict
   }
 }
 
-// Union of the trees of x and y by climbing both paths together (Rem's
+// Union of the trees of v and w by climbing both paths together (Rem's
 // algorithm, min-root form): always step the side whose parent is larger;
 // stop as soon as both sides have the same parent (same tree: no need to
 // reach the root) or when the larger side is a root, which is then linked
 // under the other side's parent by a CAS from itself (after one L2 read
-// confirms it is still a root).  With Splice, each step first lowers the
-// entry it leaves to the other side's parent (an atomic min, so an entry
-// never rises): the walk the next union makes through it is shorter.
-// Every link and splice points an entry at a smaller id of the same
-// component once this union completes, and the union completes in this
-// call, so the forest keeps p[v] <= v and each root is its tree's minimum.
-// Each iteration returns, or moves the larger side to its parent, or
-// learns a smaller parent for it from a failed CAS, so the loop ends.
-template <bool Splice>
-__device__ __forceinline__ void rem_unite(uint32_t rx, uint32_t ry, uint32_t *__restrict__ p) {
+// confirms it is still a root).  Splicing lowers the entry a step leaves
+// to the other side's parent (an atomic min, so an entry never rises), so
+// the next walk through it is shorter: Splice 1 on every step, Splice 2
+// only on v's own entry (a neighbour's entry may be a hub that many threads
+// pass through, where the atomics would queue).  Every link and splice
+// points an entry at a smaller id of the same component once this union
+// completes, and the union completes in this call, so the forest keeps
+// p[v] <= v and each root is its tree's minimum.  Each iteration returns,
+// or moves the larger side to its parent, or learns a smaller parent for it
+// from a failed CAS, so the loop ends.
+template <int Splice>
+__device__ __forceinline__ void rem_unite(const uint32_t v, const uint32_t w,
+                                          uint32_t *__restrict__ p) {
+  uint32_t rx = v, ry = w;
   uint32_t px = p[rx], py = p[ry];
   while (px != py) {
     if (px < py) {
@@ -512,7 +516,7 @@ __device__ __forceinline__ void rem_unite(uint32_t rx, uint32_t ry, uint32_t *__
       px = old;
       continue;
     }
-    if (Splice)
+    if (Splice == 1 || (Splice == 2 && rx == v))
       (void)__hip_atomic_fetch_min(&p[rx], py, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     rx = px;
     px = p[rx];
@@ -523,7 +527,7 @@ __device__ __forceinline__ void rem_unite(uint32_t rx, uint32_t ry, uint32_t *__
 // pass, no lockstep: each union ends as soon as the two paths meet, which
 // on graphs whose trees are shallow and hub-centred is well before the
 // roots).
-template <bool Splice>
+template <int Splice>
 __global__ __launch_bounds__(kBlock) void k_hook_rem(const uint32_t *__restrict__ s0,
                                                      const uint32_t *__restrict__ s1, uint32_t n,
                                                      uint32_t *__restrict__ p) {
@@ -708,7 +712,7 @@ struct Plan {
   int cache;     // sampled-neighbour cache in use
   int fused;     // one-pass hook of both sampled neighbours
   int rem;       // the one pass unites by Rem's algorithm (k_hook_rem)
-  int splice;    // ... with splicing
+  int splice;    // ... splicing: 0 none, 1 every step, 2 v's own entry only
   int halve;     // path halving in the lockstep one-pass hook (k_hook_pair)
   int hook_ilp;  // roots per thread in the lockstep sampled hook
   int hook_bpc;  // sampled-hook grid cap, workgroups per CU
@@ -721,11 +725,12 @@ inline Plan plan_for(uint32_t n, uint64_t m) {
   pl.cache = g_sample_cache && (g_hook_mode == 0 || g_hook_mode == 1 || g_hook_mode == 4);
   pl.fused = pl.cache && g_sampled_neighbours == 2 && (g_fused_pair < 0 ? 1 : g_fused_pair);
   pl.rem = pl.fused && g_hook_mode != 0 && (g_rem < 0 ? (!big || !dense) : g_rem);
-  pl.splice = pl.rem && (g_rem_splice < 0 ? big : g_rem_splice);
+  const int splice_auto = big ? 1 : (dense ? 2 : 0);
+  pl.splice = pl.rem ? (g_rem_splice < 0 ? splice_auto : g_rem_splice) : 0;
   pl.halve = pl.fused && !pl.rem && g_hook_mode != 0 && (g_halve < 0 ? big : g_halve);
   const int wide = pl.fused ? (m < kWideIlpMaxEntries ? 2 : 1) : 4;
   pl.hook_ilp = pl.rem ? 1 : (g_hook_ilp ? g_hook_ilp : (dense ? wide : 1));
-  const int bpc_auto = pl.rem ? (pl.splice ? 4 : 8) : (pl.hook_ilp >= 2 ? 8 : 4);
+  const int bpc_auto = pl.rem ? (pl.splice == 1 ? 4 : 8) : (pl.hook_ilp >= 2 ? 8 : 4);
   pl.hook_bpc = g_blocks_per_cu ? g_blocks_per_cu : bpc_auto;
   return pl;
 }
@@ -744,7 +749,7 @@ extern "C" int cc_hip_components_set(const char *key, int value) {
   else if (k == "hook_mode" && value >= 0 && value <= 4) g_hook_mode = value;
   else if (k == "halve" && value >= -1 && value <= 1) g_halve = value;
   else if (k == "rem" && value >= -1 && value <= 1) g_rem = value;
-  else if (k == "rem_splice" && value >= -1 && value <= 1) g_rem_splice = value;
+  else if (k == "rem_splice" && value >= -1 && value <= 2) g_rem_splice = value;
   else if (k == "hook_ilp" && (value == 0 || ilp_ok(value))) g_hook_ilp = value;
   else if (k == "stream_ilp" && ilp_ok(value)) g_stream_ilp = value;
   else if (k == "sample_cache" && (value == 0 || value == 1)) g_sample_cache = value;
@@ -874,10 +879,12 @@ extern "C" int cc_hip_components_device(const uint64_t *d_row_ptr, const uint32_
   } while (0)
     if (mode == 0)
       CCX_LAUNCH("cc_hook_sample", s, k_hook_pair<0, 2><<<nb, kBlock, 0, s>>>(smp, s1, n, d_labels));
-    else if (plan.rem && plan.splice)
-      CCX_LAUNCH("cc_hook_sample", s, k_hook_rem<true><<<nb, kBlock, 0, s>>>(smp, s1, n, d_labels));
+    else if (plan.rem && plan.splice == 1)
+      CCX_LAUNCH("cc_hook_sample", s, k_hook_rem<1><<<nb, kBlock, 0, s>>>(smp, s1, n, d_labels));
+    else if (plan.rem && plan.splice == 2)
+      CCX_LAUNCH("cc_hook_sample", s, k_hook_rem<2><<<nb, kBlock, 0, s>>>(smp, s1, n, d_labels));
     else if (plan.rem)
-      CCX_LAUNCH("cc_hook_sample", s, k_hook_rem<false><<<nb, kBlock, 0, s>>>(smp, s1, n, d_labels));
+      CCX_LAUNCH("cc_hook_sample", s, k_hook_rem<0><<<nb, kBlock, 0, s>>>(smp, s1, n, d_labels));
     else if (mode == 1 && plan.halve) CCX_HOOK_PAIR(1, true);
     else if (mode == 1) CCX_HOOK_PAIR(1, false);
     else if (plan.halve) CCX_HOOK_PAIR(4, true);

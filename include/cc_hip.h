@@ -103,9 +103,11 @@ int cc_hip_components_tune(int sampled_neighbours, int blocks_per_cu, int hook_m
  *                             meet) instead of walking all three roots in
  *                             lockstep: 1, 0, or -1 (default) = on when
  *                             n < 2^24 or m < 16 n; not in mode 0
- *   "rem_splice"              splicing in those unions (each step lowers the
+ *   "rem_splice"              splicing in those unions (a step lowers the
  *                             entry it leaves to the other side's parent):
- *                             1, 0, or -1 (default) = on when n >= 2^24
+ *                             1 on every step, 2 on the vertex's own entry
+ *                             only, 0 never, or -1 (default) = 1 when
+ *                             n >= 2^24, else 2 when m >= 16 n, else 0
  *   "halve"                   path halving (atomic min of each visited
  *                             entry to its grandparent) in the lockstep
  *                             one-pass hook's root walks: 1, 0, or -1

@@ -180,6 +180,8 @@ PATHS = [  # every code path of the solve: (knobs) -> same labels
     dict(sample_cache=1, fused_pair=1, hook_mode=4, rem=1, rem_splice=1),
     dict(sample_cache=1, fused_pair=1, hook_mode=1, rem=1, rem_splice=1),
     dict(sample_cache=1, fused_pair=1, hook_mode=4, rem=1, rem_splice=1, blocks_per_cu=1),
+    dict(sample_cache=1, fused_pair=1, hook_mode=4, rem=1, rem_splice=2),
+    dict(sample_cache=1, fused_pair=1, hook_mode=1, rem=1, rem_splice=2, blocks_per_cu=2),
     dict(sample_cache=1, fused_pair=1, hook_mode=4, rem=0, halve=0),
     dict(stream_ilp=2), dict(stream_ilp=4, sample_cache=0),
 ]

@@ -112,9 +112,10 @@ def test_rmat_sizes():
 
 def test_plan_defaults():
     """The default plan: cache on; one-pass hook; Rem's unions below 2^24
-    vertices (8 workgroups per CU) and on sparse graphs above (with
-    splicing, 4 per CU); otherwise the lockstep hook with path halving, two
-    roots per thread below 2^31 entries."""
+    vertices (8 workgroups per CU; dense graphs splice each vertex's own
+    entry) and on sparse graphs above (splicing every step, 4 per CU);
+    otherwise the lockstep hook with path halving, two roots per thread
+    below 2^31 entries."""
     lib = nat.gpu_lib()
     def plan(n, m):
         v = [ctypes.c_int() for _ in range(4)]
@@ -125,7 +126,7 @@ def test_plan_defaults():
         return tuple(lib.cc_hip_components_plan_get(n, m, k) for k in (b"rem", b"rem_splice",
                                                                          b"halve"))
     # R-MAT 22, Chung-Lu 65.6M, ER 4M degree 8, 8192^2 grid, R-MAT 24
-    assert plan(4194304, 128313032) == (1, 1, 1, 8) and get(4194304, 128313032) == (1, 0, 0)
+    assert plan(4194304, 128313032) == (1, 1, 1, 8) and get(4194304, 128313032) == (1, 2, 0)
     assert plan(65601024, 3377757790) == (1, 1, 1, 4) and get(65601024, 3377757790) == (0, 0, 1)
     assert plan(4194304, 33554406) == (1, 1, 1, 8) and get(4194304, 33554406) == (1, 0, 0)
     assert plan(67108864, 187873588) == (1, 1, 1, 4) and get(67108864, 187873588) == (1, 1, 0)

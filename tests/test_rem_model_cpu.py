@@ -28,7 +28,9 @@ class Mem:
 
 
 def rem_unite(mem, seen, rx, ry, splice, rng, planted=False):
-    """Generator: one yield per memory operation of the device loop."""
+    """Generator: one yield per memory operation of the device loop.
+    splice: 0 never, 1 every step, 2 only the first vertex's own entry."""
+    v = rx
     def plain(a):
         h = mem.hist[a]
         i = rng.randint(seen.get(a, 0), len(h) - 1)
@@ -58,7 +60,7 @@ def rem_unite(mem, seen, rx, ry, splice, rng, planted=False):
                 return
             px = old
             continue
-        if splice:  # atomic min of p[rx] to py
+        if splice == 1 or (splice == 2 and rx == v):  # atomic min of p[rx] to py
             if py < mem.latest(rx):
                 mem.write(rx, py)
             yield
@@ -129,7 +131,7 @@ def random_graph(rng):
     return n, [(a, b) for a, b in edges if a != b]
 
 
-@pytest.mark.parametrize("splice", [0, 1])
+@pytest.mark.parametrize("splice", [0, 1, 2])
 def test_rem_unite_model(splice):
     for trial in range(250):
         rng = random.Random(1000 * splice + trial)
