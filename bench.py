@@ -424,6 +424,9 @@ def main() -> int:
     dominant = max(per, key=lambda k: per[k]["total_ms"]) if per else None
 
     labels = D.solve()
+    ran = ctypes.c_int(0)
+    nat.check(lib.cc_hip_components_last_pass_ran(None, ctypes.byref(ran)), "last pass")
+    last_pass_ran = bool(ran.value)
     D.close()
 
     st = frontier_stats(G)
@@ -452,6 +455,11 @@ def main() -> int:
     for k, v in per.items():
         row = {"launches_per_solve": v["launches"] / args.steps, "avg_us": round(v["avg_us"], 2)}
         kb = kernel_bytes(k, G.n, st, plan)
+        if k == "cc_compress" and plan["fused"] and not last_pass_ran:
+            # the frontier sweep linked nothing, so the last pass only read
+            # its gate word and returned
+            row["gated_off"] = True
+            kb = None
         if kb is not None and v["avg_us"] > 0:
             gbs = kb / (v["avg_us"] * 1e3)
             row.update(algorithmic_bytes=kb, algorithmic_gbs=round(gbs, 1),

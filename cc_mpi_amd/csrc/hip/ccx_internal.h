@@ -36,7 +36,9 @@ struct Context {
 // shares scratch memory.
 struct StreamState {
   hipStream_t stream = nullptr;
-  uint32_t *d_words = nullptr;  // 256 device words: [0] sampled root, [8..] tile counters
+  // 256 device words: [0] sampled root, [1] frontier link flag, [8..] tile
+  // counters and flags of the other passes
+  uint32_t *d_words = nullptr;
   uint64_t *d_status = nullptr;  // look-back status words (epoch-tagged)
   size_t status_words = 0;
   uint32_t epoch = 0;           // last epoch handed out; 0 is never valid

@@ -14,13 +14,14 @@
 //      smallest neighbour, so a low-id vertex that later hooks onto a hub
 //      finds the hub's root at or below its own id and lowers its own entry
 //      instead of every such vertex hammering the hub's entry.
-//   2. Sampled hooking: every vertex hooks with its neighbour 0 (only where
-//      step 1 did not already link them) and its neighbour 1, both in one
-//      pass that walks v's root once (or, as a tuning choice, a pass per
-//      neighbour with pointer jumping between).  On graphs of 2^24 vertices
-//      and more the pass halves the paths it walks.  After this the forest
-//      already holds the giant component of a power-law or random graph in
-//      one tree.
+//   2. Sampled hooking: every vertex is united with its neighbour 0 (only
+//      where step 1 did not already link them) and its neighbour 1, in one
+//      pass: by Rem's unions (both paths climbed together, stopping where
+//      they meet) below 2^24 vertices and on sparse graphs, else by walking
+//      v's root and both neighbours' roots in lockstep with path halving
+//      (or, as a tuning choice, a pass per neighbour with pointer jumping
+//      between).  After this the forest already holds the giant component
+//      of a power-law or random graph in one tree.
 //   3. The most frequent root among 1024 sampled vertices is taken as the
 //      giant root g.
 //   4. Frontier sweep: every vertex whose parent is not g hooks with all its
@@ -32,7 +33,8 @@
 //      sweeps the concatenated neighbour lists 64 entries at a time, so the
 //      col_idx reads of a wave are contiguous runs regardless of how the
 //      degrees are spread over its lanes.
-//   5. Pointer jumping to the root: labels[v] = root(v).
+//   5. Pointer jumping to the root: labels[v] = root(v) (skipped when the
+//      frontier sweep linked no root: it then wrote every final root).
 //
 // Hooking: for an edge (a, b) with roots ra != rb, let hi = max, lo = min;
 // old = atomicMin(&p[hi], lo).  If old == hi, hi was still a root and is now
@@ -125,9 +127,10 @@ __device__ __forceinline__ uint32_t rewalk(uint32_t x, const uint32_t *__restric
 // read, that the larger root is still a root before the atomic; mode 1
 // follows any change with L2 reads, mode 4 (the default) with plain reads,
 // which measured faster on every graph tried; mode 2 walks with L2 reads
-// throughout.
+// throughout.  Returns true if this call linked a root (the forest's root
+// set changed).
 template <int Mode>
-__device__ __forceinline__ void hook(uint32_t a, uint32_t b, uint32_t *__restrict__ p) {
+__device__ __forceinline__ bool hook(uint32_t a, uint32_t b, uint32_t *__restrict__ p) {
   uint32_t ra = Mode == 2 ? find_root_l2(a, p) : find_root(a, p);
   uint32_t rb = Mode == 2 ? find_root_l2(b, p) : find_root(b, p);
   while (ra != rb) {
@@ -142,10 +145,12 @@ __device__ __forceinline__ void hook(uint32_t a, uint32_t b, uint32_t *__restric
       }
     }
     const uint32_t old = atomicMin(&p[hi], lo);
-    if (old == hi || old == lo) return;
+    if (old == hi) return true;
+    if (old == lo) return false;
     ra = rewalk<Mode>(old, p);
     rb = rewalk<Mode>(lo, p);
   }
+  return false;
 }
 
 // Lossy union used only by the sampling rounds of mode 3: one
@@ -204,8 +209,13 @@ __global__ __launch_bounds__(kBlock) void k_hook_neighbour(const uint64_t *__res
   }
 }
 
-// Pointer jumping: p[v] = root(v).  Run with no hooks in flight.
-__global__ __launch_bounds__(kBlock) void k_compress(uint32_t n, uint32_t *__restrict__ p) {
+// Pointer jumping: p[v] = root(v).  Run with no hooks in flight.  With a
+// gate, the pass is skipped when the gate word is 0: the frontier sweep
+// sets it only if it linked a root, and otherwise has already written every
+// vertex's final root.
+__global__ __launch_bounds__(kBlock) void k_compress(uint32_t n, uint32_t *__restrict__ p,
+                                                     const uint32_t *__restrict__ gate) {
+  if (gate && *gate == 0) return;
   CCX_GRID_STRIDE(v, n) {
     const uint32_t pv = p[v];
     if (pv == v) continue;
@@ -356,7 +366,9 @@ __global__ __launch_bounds__(kBlock) void k_hook_neighbour_ilp(const uint64_t *_
 }
 
 template <int ILP>
-__global__ __launch_bounds__(kBlock) void k_compress_ilp(uint32_t n, uint32_t *__restrict__ p) {
+__global__ __launch_bounds__(kBlock) void k_compress_ilp(uint32_t n, uint32_t *__restrict__ p,
+                                                         const uint32_t *__restrict__ gate) {
+  if (gate && *gate == 0) return;
   const uint64_t S = (uint64_t)gridDim.x * kBlock;
   for (uint64_t v0 = (uint64_t)blockIdx.x * kBlock + threadIdx.x; v0 < n; v0 += S * ILP) {
     uint32_t x[ILP], first[ILP];
@@ -598,6 +610,7 @@ __global__ __launch_bounds__(kGiantSamples) void k_sample_giant(uint32_t n, cons
     for (int w = 1; w < kGiantSamples / kWave; ++w)
       if (best[w] > b) b = best[w];
     giant[0] = ~(uint32_t)b;
+    giant[1] = 0;  // the frontier sweep's link flag (see k_hook_frontier)
   }
 }
 
@@ -605,17 +618,21 @@ __global__ __launch_bounds__(kGiantSamples) void k_sample_giant(uint32_t n, cons
 // vertex first resolves its root (and writes it back to its own entry,
 // which does the pointer jumping the sampled rounds left); a walk that
 // passes through g marks the vertex as a giant member and skips it.
+// slots[0] = g; slots[1] (cleared by k_sample_giant) is set if any hook
+// here linked a root: only then can a written-back root be stale, so the
+// last pointer-jumping pass is gated on it.
 template <int Mode>
 __global__ __launch_bounds__(kBlock) void k_hook_frontier(const uint64_t *__restrict__ row_ptr,
                                                           const uint32_t *__restrict__ col,
                                                           uint32_t n, uint32_t k_first,
                                                           uint32_t *__restrict__ p,
-                                                          const uint32_t *__restrict__ giant_slot) {
+                                                          uint32_t *__restrict__ slots) {
   __shared__ uint64_t s_excl[kBlock / kWave][kWave];
   __shared__ uint64_t s_begin[kBlock / kWave][kWave];
   const int lane = threadIdx.x & (kWave - 1);
   const int wid = threadIdx.x / kWave;
-  const uint32_t giant = giant_slot[0];
+  const uint32_t giant = slots[0];
+  bool linked = false;
   const uint64_t waves = (uint64_t)gridDim.x * (kBlock / kWave);
   for (uint64_t chunk = (uint64_t)blockIdx.x * (kBlock / kWave) + wid; chunk * kWave < n;
        chunk += waves) {
@@ -658,13 +675,14 @@ __global__ __launch_bounds__(kBlock) void k_hook_frontier(const uint64_t *__rest
         if (ex[j + step] <= e) j += step;
       const uint64_t idx = s_begin[wid][j] + (e - ex[j]);
       const uint32_t w = col[idx];
-      if (w < n) hook<(Mode == 3 ? 1 : Mode)>((uint32_t)(base + j), w, p);
+      if (w < n && hook<(Mode == 3 ? 1 : Mode)>((uint32_t)(base + j), w, p)) linked = true;
     }
     // the LDS rows are rewritten by the next chunk of this wave
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
     __builtin_amdgcn_wave_barrier();
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
   }
+  if (linked) slots[1] = 1;
 }
 
 inline unsigned blocks_for(uint64_t n, int cu_count, int per_cu) {
@@ -777,6 +795,19 @@ extern "C" int cc_hip_components_get(const char *key) {
   return -1;
 }
 
+extern "C" int cc_hip_components_last_pass_ran(void *stream, int *ran) {
+  if (!ccx::ctx()) return 2;
+  if (!ran) return ccx::set_error(4, "cc_hip_components_last_pass_ran: null pointer");
+  hipStream_t s = ccx::pick_stream(stream);
+  ccx::StreamState *st = ccx::stream_state(s);
+  if (!st) return 3;
+  uint32_t flag = 0;
+  CCX_TRY(hipMemcpyAsync(&flag, st->d_words + 1, sizeof(flag), hipMemcpyDeviceToHost, s));
+  CCX_TRY(hipStreamSynchronize(s));
+  *ran = flag != 0;
+  return 0;
+}
+
 extern "C" int cc_hip_components_plan_get(uint32_t n, uint64_t m, const char *key) {
   const Plan pl = plan_for(n, m);
   const std::string k(key ? key : "");
@@ -838,10 +869,12 @@ extern "C" int cc_hip_components_device(const uint64_t *d_row_ptr, const uint32_
     CCX_LAUNCH("cc_init", s, k_init<<<ns, kBlock, 0, s>>>(n, d_labels));
     return 0;
   }
-  auto compress = [&]() -> int {
-    if (ilp == 4) CCX_LAUNCH("cc_compress", s, k_compress_ilp<4><<<ns, kBlock, 0, s>>>(n, d_labels));
-    else if (ilp == 2) CCX_LAUNCH("cc_compress", s, k_compress_ilp<2><<<ns, kBlock, 0, s>>>(n, d_labels));
-    else CCX_LAUNCH("cc_compress", s, k_compress<<<ns, kBlock, 0, s>>>(n, d_labels));
+  // gate: nullptr for a pass between hook rounds, the frontier's link flag
+  // for the last pass
+  auto compress = [&](const uint32_t *gate) -> int {
+    if (ilp == 4) CCX_LAUNCH("cc_compress", s, k_compress_ilp<4><<<ns, kBlock, 0, s>>>(n, d_labels, gate));
+    else if (ilp == 2) CCX_LAUNCH("cc_compress", s, k_compress_ilp<2><<<ns, kBlock, 0, s>>>(n, d_labels, gate));
+    else CCX_LAUNCH("cc_compress", s, k_compress<<<ns, kBlock, 0, s>>>(n, d_labels, gate));
     return 0;
   };
   const int mode = g_hook_mode;
@@ -893,7 +926,7 @@ extern "C" int cc_hip_components_device(const uint64_t *d_row_ptr, const uint32_
   } else {
     for (uint32_t r = 0; r < rounds; ++r) {
       if (r > 0)
-        if (int rc = compress()) return rc;
+        if (int rc = compress(nullptr)) return rc;
       if (cache) {
         const uint32_t *sr = smp + (size_t)r * n;
         const bool skip = early_giant && r > 0;
@@ -951,7 +984,7 @@ extern "C" int cc_hip_components_device(const uint64_t *d_row_ptr, const uint32_
     CCX_LAUNCH("cc_hook_frontier", s, k_hook_frontier<4><<<nf, kBlock, 0, s>>>(d_row_ptr, d_col_idx, n, first, d_labels, giant));
   else
     CCX_LAUNCH("cc_hook_frontier", s, k_hook_frontier<3><<<nf, kBlock, 0, s>>>(d_row_ptr, d_col_idx, n, first, d_labels, giant));
-  return compress();
+  return compress(giant + 1);
 }
 
 extern "C" int cc_hip_components_host(const uint64_t *row_ptr, const uint32_t *col_idx, uint32_t n,

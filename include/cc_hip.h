@@ -142,6 +142,11 @@ int cc_hip_components_plan(uint32_t n, uint64_t m, int *cache, int *fused, int *
  * "halve", "hook_ilp", "hook_blocks_per_cu", "giant_early" (in effect for
  * this graph), "hook_mode".  Returns -1 for an unknown key. */
 int cc_hip_components_plan_get(uint32_t n, uint64_t m, const char *key);
+/* Whether the last solve on `stream` (NULL = the library stream) ran its
+ * final pointer-jumping pass: *ran = 1 if its frontier sweep linked a root,
+ * 0 if the pass was skipped because every vertex already held its final
+ * root.  Blocking (waits for the stream). */
+int cc_hip_components_last_pass_ran(void *stream, int *ran);
 
 /* Same result through host buffers (H2D, solve, D2H; blocking). */
 int cc_hip_components_host(const uint64_t *row_ptr, const uint32_t *col_idx,

@@ -115,6 +115,24 @@ def test_default_plan_at_scale(gpu, kind, rem, splice, halve):
                           np.bincount(want, minlength=G.n)[roots])
 
 
+def test_last_pass_gate(gpu):
+    """The last pointer-jumping pass is skipped when the frontier sweep links
+    no root; labels are exact either way, and both cases occur here."""
+    lib = nat.gpu_lib()
+    seen = set()
+    for G in (disjoint_cliques(200, 5, seed=34), cm.grid(120, 150, 0.6, seed=33),
+              cm.rmat(14, 16, seed=31), cm.erdos_renyi(50000, 1.2, seed=32),
+              cm.chung_lu(1 << 16, 1 << 19, 2.1, seed=35), path(3000, perm_seed=2)):
+        D = cm.DeviceGraph(G)
+        got = D.solve()
+        ran = ctypes.c_int(-1)
+        nat.check(lib.cc_hip_components_last_pass_ran(None, ctypes.byref(ran)), "last pass")
+        D.close()
+        check_labels(G, got)
+        seen.add(ran.value)
+    assert seen == {0, 1}
+
+
 def test_device_resident_repeat(gpu):
     # same labels on repeated solves of a resident graph, and equal to the oracle
     G = cm.rmat(16, 16, seed=5)
