@@ -9,7 +9,10 @@
 // in ascending idx order, i.e. exactly the reference's send buffer, plus
 // per-destination counts and displacements.  On the device this is one
 // order-preserving stream compaction (decoupled look-back); the pairs are
-// written straight into a pinned host buffer, which is what MPI sends.
+// staged in HBM and moved to the caller's buffer (the pinned buffer MPI
+// sends from) by one DMA copy of exactly the emitted pairs.  Kernel stores
+// straight into pinned host memory measured 8 GB/s on a pack that emits
+// every entry (46.4M pairs: 45 ms); the copy engine moves them far faster.
 
 #include <hip/hip_runtime.h>
 
@@ -168,10 +171,12 @@ extern "C" int cc_hip_delta_pack_device(const uint32_t *d_send_to_flat, uint64_t
   if (int rc = ccx::ensure_status(st, tiles)) return rc;
   const size_t seg_bytes = (size_t)(nseg + 1) * sizeof(int);
   const size_t rank_off = (seg_bytes + 255) & ~(size_t)255;
-  char *tmp = (char *)ccx::stream_tmp(st, rank_off + (size_t)(nseg + 1) * sizeof(uint64_t));
+  const size_t stage_off = (rank_off + (size_t)(nseg + 1) * sizeof(uint64_t) + 255) & ~(size_t)255;
+  char *tmp = (char *)ccx::stream_tmp(st, stage_off + total_send_to * sizeof(uint64_t));
   if (!tmp) return 3;
   int *d_seg = (int *)tmp;
   unsigned long long *d_rank = (unsigned long long *)(tmp + rank_off);
+  uint64_t *d_stage = (uint64_t *)(tmp + stage_off);  // room for every entry
   memcpy(st->h_stage, seg_start, seg_bytes);
   CCX_TRY(hipMemcpyAsync(d_seg, st->h_stage, seg_bytes, hipMemcpyHostToDevice, s));
   uint32_t epoch;
@@ -179,7 +184,7 @@ extern "C" int cc_hip_delta_pack_device(const uint32_t *d_send_to_flat, uint64_t
   uint32_t *ctr = st->d_words + 8;
   unsigned long long *tot = (unsigned long long *)(st->d_words + 16);
   CCX_TRY(hipMemsetAsync(ctr, 0, sizeof(uint32_t), s));
-  DeltaOp op{d_send_to_flat, d_comp_label, d_comp_of, v_start, v_end, d_prev_sent, h_sendbuf,
+  DeltaOp op{d_send_to_flat, d_comp_label, d_comp_of, v_start, v_end, d_prev_sent, d_stage,
              d_seg, nseg, d_rank, 0, 0};
   CCX_LAUNCH("delta_pack", s,
              k_delta<<<(unsigned)tiles, kThreads, 0, s>>>(op, total_send_to, ctr, st->d_status,
@@ -190,6 +195,15 @@ extern "C" int cc_hip_delta_pack_device(const uint32_t *d_send_to_flat, uint64_t
                          hipMemcpyDeviceToHost, s));
   CCX_TRY(hipStreamSynchronize(s));
   const uint64_t total = h[0];
+  if (total > total_send_to)
+    return ccx::set_error(1, "cc_hip_delta_pack_device: %llu pairs from %llu entries",
+                          (unsigned long long)total, (unsigned long long)total_send_to);
+  if (total > 0) {
+    ccx::prof_begin("delta_copy", s);
+    CCX_TRY(hipMemcpyAsync(h_sendbuf, d_stage, total * sizeof(uint64_t), hipMemcpyDefault, s));
+    ccx::prof_end(s);
+    CCX_TRY(hipStreamSynchronize(s));
+  }
   uint64_t *rank = h + 1;
   for (int k = 0; k <= nseg; ++k)
     if ((uint64_t)seg_start[k] >= total_send_to) rank[k] = total;

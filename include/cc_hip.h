@@ -218,8 +218,10 @@ int cc_hip_ghost_extract_device(const uint64_t *d_records, uint64_t count,
  * [v_start, v_end), else v) is below d_prev_sent, d_prev_sent is lowered and
  * the pair ((idx - segment start) << 32) | label is written to h_sendbuf
  * (pinned host memory from cc_hip_host_alloc, or device memory; room for
- * total_send_to pairs) in ascending idx order.  sendcounts/sdispls (nseg
- * ints each) and *total_pairs receive the layout.  Blocking.
+ * total_send_to pairs) in ascending idx order: compacted in the stream's
+ * HBM temp buffer (which grows to total_send_to pairs), then copied by one
+ * DMA of the emitted pairs.  sendcounts/sdispls (nseg ints each) and
+ * *total_pairs receive the layout.  Blocking.
  */
 int cc_hip_delta_pack_device(const uint32_t *d_send_to_flat, uint64_t total_send_to,
                              const int *seg_start, int nseg,

@@ -103,8 +103,8 @@ void exchangeplan_device_free(ExchangeDevice *D);
 
 /*
  * exchangeplan_exchange_delta with comp_label, comp_of, prev_sent and
- * ghost_labels resident on the device: the pack writes the changed pairs
- * straight into the pinned send buffer, MPI exchanges them, and the unpack
+ * ghost_labels resident on the device: the pack compacts the changed pairs
+ * in HBM and copies them into the pinned send buffer, MPI exchanges them, and the unpack
  * kernel scatters the received pairs into d_ghost_labels.  Returns the
  * number of pairs this rank sent.
  */
