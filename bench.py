@@ -299,7 +299,13 @@ def rankshare_leg(args) -> dict:
         got = rs.merge(truth, exchange_interval=interval, seed=11,
                        check_pack=check if interval == 1 else None)
         ok = bool(np.array_equal(got, want))
+        # `seconds` includes the host-side simulation of the other ranks
+        # (the seeded delta schedule over every ghost, uploaded per
+        # exchange) and, at interval 1, the oracle check of every pack;
+        # the device side of the loop is merge_steps_s + delta_packs_s
         runs[str(interval)] = {"seconds": round(time.perf_counter() - t0, 3), "labels_exact": ok,
+                               "merge_steps_s": round(rs.times["merge_steps"], 4),
+                               "delta_packs_s": round(rs.times["delta_packs"], 4),
                                **{k: rs.stats[k] for k in ("merge_iterations", "exchanges",
                                                            "packs", "pairs_emitted")}}
         log(f"rankshare interval {interval}: {runs[str(interval)]}")

@@ -210,12 +210,12 @@ class RankShare:
                 break
             steps += 1
             if steps >= interval or not changed.value:
-                t0 = time.perf_counter()
                 if self.dests:
                     before = None
                     if check_pack is not None:
                         before = np.empty(self.total_send, dtype=np.uint32)
                         prev_sent.download(before)
+                    t0 = time.perf_counter()
                     pairs, counts, displs = delta_pack_device(
                         self.send_to_flat, self.total_send, self.seg_start, comp_label,
                         self.comp_of, vs, ve, prev_sent, sendbuf)
