@@ -150,6 +150,54 @@ class RankShare:
                 buf.close()
         return out
 
+    def prepare_schedule(self, truth_labels: DeviceBuffer, seed: int) -> dict:
+        """The injected deltas, built once per seed and kept on the device:
+        ghost g receives min(g's id, its owner's local-CC label) at exchange
+        mid[g] (when that comes before due[g]) and its true global label at
+        exchange due[g] = geometric(1/2) - 1, mid = due // 2; both values
+        are ids of vertices in g's component, as every real delta is.  The
+        pairs (ghost index << 32) | label are sorted by exchange, so
+        exchange k applies the slice start[k]..start[k+1]."""
+        cache = getattr(self, "_sched", None)
+        if cache is not None and cache["seed"] == seed:
+            return cache
+        if cache is not None:
+            for b in ("pairs", "iota"):
+                cache[b].close()
+        lib = nat.gpu_lib()
+        ng = self.ng
+        ghost_ids = np.empty(ng, dtype=np.uint32)
+        self.ghosts.download(ghost_ids)
+        truth = np.empty(ng, dtype=np.uint32)
+        tmp = DeviceBuffer(4 * max(1, ng))
+        nat.check(lib.cc_hip_gather_u32_device(truth_labels.ptr, self.ghosts.ptr, ng, tmp.ptr, None),
+                  "gather")
+        tmp.download(truth)
+        tmp.close()
+        rng = np.random.default_rng(seed)
+        due = rng.geometric(0.5, ng).astype(np.int64) - 1
+        mid = due // 2
+        if not hasattr(self, "_owner_local"):
+            self._owner_local = self.owner_local_labels()
+        half = np.flatnonzero(mid < due)
+        gidx = np.concatenate([np.arange(ng, dtype=np.uint64), half.astype(np.uint64)])
+        val = np.concatenate([truth, np.minimum(ghost_ids[half],
+                                                self._owner_local[ghost_ids[half]])])
+        # exchange indices are small (geometric(1/2) draws), so a stable
+        # sort of them as bytes is a linear radix sort
+        assert ng == 0 or int(due.max()) < 256
+        when = np.concatenate([due, mid[half]]).astype(np.uint8)
+        order = np.argsort(when, kind="stable")
+        pairs = (gidx[order] << np.uint64(32)) | val[order].astype(np.uint64)
+        last = int(due.max()) if ng else -1
+        start = np.concatenate([[0], np.cumsum(np.bincount(when, minlength=last + 1))]).astype(np.int64)
+        d_pairs = DeviceBuffer.from_array(pairs) if pairs.size else DeviceBuffer(8)
+        iota = DeviceBuffer(4 * max(1, ng))
+        nat.check(lib.cc_hip_iota_u32_device(iota.ptr, ng, 0, None), "iota")
+        self._sched = dict(seed=seed, pairs=d_pairs, start=start, last=last, iota=iota,
+                           zero=np.zeros(1, dtype=np.int32))
+        return self._sched
+
     def merge(self, truth_labels: DeviceBuffer, exchange_interval: int = 1, seed: int = 1,
               check_pack=None, max_iters: int = 100000) -> np.ndarray:
         """Run the merge loop with injected ghost deltas; returns the owned
@@ -164,25 +212,8 @@ class RankShare:
         nat.check(lib.cc_hip_iota_u32_device(comp_label.ptr, nl, vs, None), "iota")
         ghost_labels = DeviceBuffer(4 * max(1, ng))
         nat.check(lib.cc_hip_memcpy_d2d(ghost_labels.ptr, self.ghosts.ptr, 4 * ng, None), "copy")
-        # the injection schedule: ghost g gets its owner's local-CC label at
-        # exchange mid[g] and its true global label at exchange due[g]; both
-        # are ids of vertices in g's component, as every real delta is
-        ghost_ids = np.empty(ng, dtype=np.uint32)
-        self.ghosts.download(ghost_ids)
-        truth = np.empty(ng, dtype=np.uint32)
-        tmp = DeviceBuffer(4 * max(1, ng))
-        nat.check(lib.cc_hip_gather_u32_device(truth_labels.ptr, self.ghosts.ptr, ng, tmp.ptr, None),
-                  "gather")
-        tmp.download(truth)
-        tmp.close()
-        rng = np.random.default_rng(seed)
-        due = rng.geometric(0.5, ng).astype(np.int64) - 1
-        mid = due // 2
-        if not hasattr(self, "_owner_local"):
-            self._owner_local = self.owner_local_labels()
-        midval = self._owner_local[ghost_ids]
-        cur = ghost_ids.copy()
-        last_exchange = int(due.max()) if ng else -1
+        sched = self.prepare_schedule(truth_labels, seed)
+        last_exchange = sched["last"]
         prev_sent = DeviceBuffer(4 * max(1, self.total_send))
         ones = np.full(max(1, self.total_send), 0xFFFFFFFF, dtype=np.uint32)
         prev_sent.upload(ones[: max(1, self.total_send)])
@@ -229,13 +260,16 @@ class RankShare:
                         self.send_to_flat.download(stf)
                         check_pack(stf, self.seg_start, cl, comp_of_h, vs, ve, before,
                                    pairs.copy(), counts, displs)
-                # deltas from the other ranks for this exchange
-                upd = (due == k)
-                cur[upd] = truth[upd]
-                half = (mid == k) & (due > k)
-                cur[half] = np.minimum(cur[half], midval[half])
-                if upd.any() or half.any():
-                    ghost_labels.upload(cur)
+                # deltas from the other ranks for this exchange, applied by
+                # the receive-side unpack kernel as received pairs would be
+                lo = hi = 0
+                if k <= last_exchange:
+                    lo, hi = int(sched["start"][k]), int(sched["start"][k + 1])
+                if hi > lo:
+                    nat.check(lib.cc_hip_delta_unpack_device(
+                        sched["pairs"].ptr + 8 * lo, hi - lo, nat.ptr(sched["zero"], ctypes.c_int),
+                        nat.ptr(sched["zero"], ctypes.c_int), 1, sched["iota"].ptr,
+                        ghost_labels.ptr, None), "delta unpack")
                 k += 1
                 pending_since = 0
                 steps = 0
@@ -255,6 +289,11 @@ class RankShare:
         return out
 
     def close(self) -> None:
+        sched = getattr(self, "_sched", None)
+        if sched is not None:
+            sched["pairs"].close()
+            sched["iota"].close()
+            self._sched = None
         for name in ("comp_of", "send_to_flat", "ghosts", "bnd"):
             b = getattr(self, name, None)
             if b is not None:
