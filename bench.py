@@ -355,8 +355,9 @@ def main() -> int:
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-grid-side", type=int, default=2048,
                     help="lattice side of the CPU leg's sample for the grid workload")
-    ap.add_argument("--serving-requests", type=int, default=3000,
-                    help="requests in the batched-serving leg (0 = skip; 10000 = the full stream)")
+    ap.add_argument("--serving-requests", type=int, default=10000,
+                    help="requests in the batched-serving leg (0 = skip; default: the stream of "
+                         "10,000 requests)")
     ap.add_argument("--serving-qps", type=float, default=8000.0)
     ap.add_argument("--rankshare", action="store_true",
                     help="also run one rank's share of the 65.6M-vertex graph (4-rank blocks)")
