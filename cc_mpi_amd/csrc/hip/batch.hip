@@ -451,6 +451,7 @@ extern "C" int cc_hip_batch_destroy(cc_hip_batch_t b) {
   if (b->graph) (void)hipGraphDestroy(b->graph);
   (void)hipFree(b->d_tables);
   (void)hipHostFree(b->h_tables);
+  ccx::release_stream_state(b->stream);  // direct solves on this stream made one
   (void)hipStreamDestroy(b->stream);
   delete b;
   return 0;
@@ -468,8 +469,15 @@ extern "C" int cc_hip_batch_launch(cc_hip_batch_t b, uint32_t n_graphs,
   if (item_vertices == 0 || item_vertices % kBlock)
     return ccx::set_error(4, "cc_hip_batch_launch: item_vertices must be a positive multiple of %d",
                           kBlock);
+  if (n_graphs > 0 && (!d_row_ptr || !d_col_idx || !n || !d_labels))
+    return ccx::set_error(4, "cc_hip_batch_launch: null pointer");
   uint64_t need = 0;
-  for (uint32_t g = 0; g < n_graphs; ++g) need += (n[g] + (uint64_t)item_vertices - 1) / item_vertices;
+  for (uint32_t g = 0; g < n_graphs; ++g) {
+    if (n[g] > ccx::kMaxVertices)
+      return ccx::set_error(4, "cc_hip_batch_launch: graph %u has %u vertices (limit %u)", g, n[g],
+                            ccx::kMaxVertices);
+    need += (n[g] + (uint64_t)item_vertices - 1) / item_vertices;
+  }
   if (need > b->max_items)
     return ccx::set_error(4, "cc_hip_batch_launch: %llu work items > capacity %u",
                           (unsigned long long)need, b->max_items);

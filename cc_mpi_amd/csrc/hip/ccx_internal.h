@@ -47,6 +47,11 @@ struct StreamState {
   size_t tmp_bytes = 0;
 };
 
+// Largest vertex count a solve accepts: every id stays below the
+// 0xFFFFFFFF "no neighbour" sentinel, and a 32-bit grid-stride loop (stride
+// below 2^24 threads) cannot wrap past n.
+constexpr uint32_t kMaxVertices = 0xFF000000u;
+
 // Look-back status word layout: [63:42] epoch, [41:40] flag, [39:0] value.
 constexpr int kEpochShift = 42;
 constexpr uint32_t kEpochMax = (1u << 22) - 1;
@@ -55,6 +60,8 @@ constexpr uint64_t kFlagInclusive = 2ull << 40;
 constexpr uint64_t kValueMask = (1ull << 40) - 1;
 
 StreamState *stream_state(hipStream_t s);
+// Wait for stream s and free its state (call before destroying s).
+void release_stream_state(hipStream_t s);
 // Grow the status array to `words` (new memory zeroed, epochs restart).
 int ensure_status(StreamState *st, size_t words);
 // Fresh epoch for one look-back pass; recycles the array on wrap-around.

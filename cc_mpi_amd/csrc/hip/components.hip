@@ -9,7 +9,8 @@
 // one is a parent-pointer forest that keeps the invariant p[v] <= v:
 //
 //   1. p[v] = min(v, first listed neighbour of v): an own-entry write with
-//      no atomics.  Rows are sorted, so this is v's smallest neighbour; the
+//      no atomics.  With sorted rows (the generators' and the split's) this
+//      is v's smallest neighbour (the result needs no row order); the
 //      point is that afterwards every vertex already hangs at or below its
 //      smallest neighbour, so a low-id vertex that later hooks onto a hub
 //      finds the hub's root at or below its own id and lowers its own entry
@@ -851,6 +852,9 @@ extern "C" int cc_hip_components_device(const uint64_t *d_row_ptr, const uint32_
   if (n == 0) return 0;
   if (!d_row_ptr || !d_labels || (m > 0 && !d_col_idx))
     return ccx::set_error(4, "cc_hip_components_device: null pointer");
+  if (n > ccx::kMaxVertices)
+    return ccx::set_error(4, "cc_hip_components_device: n=%u exceeds the limit of %u vertices", n,
+                          ccx::kMaxVertices);
   hipStream_t s = ccx::pick_stream(stream);
   // The sampled giant root lives in this stream's own scratch word, so
   // solves on different streams never share it.

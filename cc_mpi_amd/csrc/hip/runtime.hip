@@ -4,6 +4,7 @@
 
 #include <hip/hip_runtime.h>
 
+#include <atomic>
 #include <cstdarg>
 #include <cstdio>
 #include <cstring>
@@ -31,7 +32,7 @@ void clear_error() { g_err[0] = '\0'; }
 
 static std::mutex g_ctx_mu;
 static Context g_ctx;
-static bool g_ctx_ok = false;
+static std::atomic<bool> g_ctx_ok{false};  // set once, after g_ctx is complete
 
 static int init_locked(int device) {
   if (g_ctx_ok) {
@@ -58,12 +59,12 @@ static int init_locked(int device) {
   CCX_TRY(hipMalloc(&g_ctx.d_small, 4096));
   CCX_TRY(hipMemset(g_ctx.d_small, 0, 4096));
   CCX_TRY(hipHostMalloc(&g_ctx.h_small, 4096, hipHostMallocDefault));
-  g_ctx_ok = true;
+  g_ctx_ok.store(true, std::memory_order_release);
   return 0;
 }
 
 Context *ctx() {
-  if (g_ctx_ok) {
+  if (g_ctx_ok.load(std::memory_order_acquire)) {
     // Every call runs on the context's device, whatever the caller set.
     int cur = -1;
     if (hipGetDevice(&cur) == hipSuccess && cur != g_ctx.device) (void)hipSetDevice(g_ctx.device);
@@ -123,6 +124,26 @@ StreamState *stream_state(hipStream_t s) {
   }
   g_ss[s] = st;
   return st;
+}
+
+void release_stream_state(hipStream_t s) {
+  StreamState *st = nullptr;
+  {
+    std::lock_guard<std::mutex> lk(g_ss_mu);
+    auto it = g_ss.find(s);
+    if (it != g_ss.end()) {
+      st = it->second;
+      g_ss.erase(it);
+    }
+  }
+  (void)hipStreamSynchronize(s);
+  if (g_ctx.ws_stream == s) g_ctx.ws_stream = nullptr;
+  if (!st) return;
+  (void)hipFree(st->d_words);
+  (void)hipFree(st->d_status);
+  (void)hipFree(st->tmp);
+  (void)hipHostFree(st->h_stage);
+  delete st;
 }
 
 int ensure_status(StreamState *st, size_t words) {
@@ -323,6 +344,7 @@ int cc_hip_memcpy_d2d(void *d_dst, const void *d_src, size_t bytes, void *stream
 
 int cc_hip_stream_create(void **stream) {
   if (!ctx()) return 2;
+  if (!stream) return set_error(4, "cc_hip_stream_create: null out pointer");
   hipStream_t s = nullptr;
   CCX_TRY(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
   *stream = (void *)s;
@@ -332,6 +354,7 @@ int cc_hip_stream_create(void **stream) {
 int cc_hip_stream_destroy(void *stream) {
   if (!stream) return 0;
   if (!ctx()) return 2;
+  release_stream_state((hipStream_t)stream);
   CCX_TRY(hipStreamDestroy((hipStream_t)stream));
   return 0;
 }

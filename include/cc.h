@@ -24,7 +24,10 @@ extern "C" {
  * labels[v] = smallest vertex id in v's connected component, for every
  * v in [0, G->n).  Same contract as the reference (src/cc_pthreads.c:152-229:
  * labels start at the vertex id and are only ever lowered to a neighbour's
- * label, so the fixed point is the per-component minimum).
+ * label, so the fixed point is the per-component minimum).  G must be
+ * symmetric (both directions of every edge stored), as the reference's
+ * inputs and its local-local CSR are; duplicates, self-loops and unsorted
+ * rows are fine, and column ids >= n are ignored.
  *
  * G is read-only host memory; labels is caller-owned host memory of G->n
  * entries and is fully written.  Blocking: returns when the labels are back

@@ -69,10 +69,12 @@ const char *cc_hip_prof_names(void);
  * d_labels[v] = smallest vertex id in v's component of the undirected graph
  * (d_row_ptr[n+1], d_col_idx[m]) resident in device memory.  The graph must
  * be symmetric (both directions stored), as the reference's local CSR is
- * (src/cc_mpi.c:252-304).  Self-loops and duplicate entries are allowed.
- * Algorithm: sampled hooking + full hooking of the non-giant frontier with
- * atomicMin on parent pointers, then pointer jumping; deterministic result.
- * Asynchronous on `stream`.
+ * (src/cc_mpi.c:252-304).  Self-loops, duplicate entries and unsorted rows
+ * are allowed; column ids >= n are ignored.  n is at most 0xFF000000 (error
+ * 4 above that).  Algorithm: each vertex united with two sampled neighbours
+ * on a parent-pointer forest, then every remaining neighbour of the vertices
+ * outside the giant tree, then pointer jumping; the result does not depend
+ * on the order the atomics land in.  Asynchronous on `stream`.
  */
 int cc_hip_components_device(const uint64_t *d_row_ptr, const uint32_t *d_col_idx,
                              uint32_t n, uint64_t m, uint32_t *d_labels, void *stream);
@@ -335,8 +337,8 @@ int cc_hip_batch_destroy(cc_hip_batch_t b);
 int cc_hip_batch_set(const char *key, int value);
 /*
  * Solve n_graphs resident graphs (symmetric CSRs d_row_ptr[g], d_col_idx[g]
- * of n[g] vertices) into d_labels[g] (per-graph minimum ids) with one graph
- * replay; each workgroup gets at most item_vertices (multiple of 256)
+ * of n[g] <= 0xFF000000 vertices) into d_labels[g] (per-graph minimum ids)
+ * with one graph replay; each work item holds at most item_vertices (multiple of 256)
  * vertices of one graph.  Asynchronous on the batch's stream; waits for the
  * previous launch of the same batch first.  *items_used = work items.
  */

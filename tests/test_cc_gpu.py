@@ -60,6 +60,16 @@ def test_empty_graph(gpu):
     assert cm.compute_connected_components_pthreads(G).size == 0
 
 
+def test_vertex_limit_rejected(gpu):
+    """More than 0xFF000000 vertices is refused on the host (error 4) before
+    any device memory is touched; null pointers likewise."""
+    lib = nat.gpu_lib()
+    dummy = ctypes.c_void_p(0x1000)
+    assert lib.cc_hip_components_device(dummy, dummy, 0xFF000001, 1, dummy, None) == 4
+    assert b"limit" in lib.cc_hip_last_error()
+    assert lib.cc_hip_components_device(None, dummy, 10, 1, dummy, None) == 4
+
+
 @pytest.mark.parametrize("seed", [1, 2, 3])
 def test_er_64k(gpu, seed):
     # Erdos-Renyi 65,536 vertices, average degree 8 (the reference's CPU case)
