@@ -143,6 +143,25 @@ def test_last_pass_gate(gpu):
     assert seen == {0, 1}
 
 
+def test_solve_on_caller_streams(gpu):
+    """Solves on streams the caller creates and destroys in turn: each
+    stream's scratch state is made on first use and freed with the stream,
+    and the labels are exact every time."""
+    lib = nat.gpu_lib()
+    G = cm.rmat(14, 16, seed=9)
+    D = cm.DeviceGraph(G)
+    for _ in range(3):
+        s = ctypes.c_void_p()
+        nat.check(lib.cc_hip_stream_create(ctypes.byref(s)), "stream create")
+        try:
+            got = D.solve(s)
+        finally:
+            nat.check(lib.cc_hip_stream_destroy(s), "stream destroy")
+        check_labels(G, got)
+    assert np.array_equal(D.solve(), got)
+    D.close()
+
+
 def test_device_resident_repeat(gpu):
     # same labels on repeated solves of a resident graph, and equal to the oracle
     G = cm.rmat(16, 16, seed=5)
