@@ -73,6 +73,9 @@ void *stream_tmp(StreamState *st, size_t bytes);
 // its own items; 1: staged in LDS, stored as contiguous runs).  Set through
 // cc_hip_sort_set.
 extern int g_split_staged;
+// Ordered compactions (dedup, ghost extraction, send lists, delta pack) in
+// row-striped tiles (1) or with a run of consecutive items per thread (0).
+extern int g_compact_rows;
 extern int g_dedup_staged;
 extern int g_dedup_items;  // items per thread of the dedup compaction (16, 32)
 

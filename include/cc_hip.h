@@ -183,13 +183,17 @@ int cc_hip_boundary_build_device(uint64_t *d_records, uint64_t *d_alt, uint64_t 
                                  int *result_in_alt, void *stream);
 
 /* Tuning of the sort's scatter passes: "items_per_thread" (8, 16, 24 or 32
- * (default) keys per thread; a workgroup's tile is 256 x that); and of the
- * split and dedup compactions: "split_staged" / "dedup_staged" (0 default:
- * each thread stores its own items; 1: outputs staged in LDS and stored as
- * contiguous runs, which measured slower at 638M records because the LDS
- * costs occupancy); "dedup_items" (16 or 32 (default) records per thread
- * in the dedup compaction).  The result never depends on them.  _set returns nonzero for a bad key or value, _get -1
- * for an unknown key. */
+ * (default) keys per thread; a workgroup's tile is 256 x that); of the
+ * ordered compactions (dedup, ghost extraction, send lists, delta pack):
+ * "compact_rows" (1 default: row-striped tiles, each wave reading 64
+ * consecutive items per row; 0: a run of consecutive items per thread,
+ * where the next two knobs apply); "dedup_staged" (0 default: each thread
+ * stores its own items; 1: outputs staged in LDS and stored as contiguous
+ * runs); "dedup_items" (16 or 32 (default) records per thread); and of the
+ * split: "split_staged" (as dedup_staged; staging measured slower at 854M
+ * entries because the LDS costs occupancy).  The result never depends on
+ * them.  _set returns nonzero for a bad key or value, _get -1 for an
+ * unknown key. */
 int cc_hip_sort_set(const char *key, int value);
 int cc_hip_sort_get(const char *key);
 

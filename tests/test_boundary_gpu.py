@@ -1,13 +1,30 @@
 """Parity of the device boundary-record path (sort, dedup, ghost extraction)
 and the delta pack with the oracle, bit-exact."""
+import contextlib
+
 import numpy as np
 import pytest
 
 import oracle
+from cc_mpi_amd import _native as nat
 from cc_mpi_amd import boundary as bd
 from cc_mpi_amd.cc import DeviceBuffer
 
 pytestmark = pytest.mark.gpu
+
+
+@contextlib.contextmanager
+def sort_knobs(**knobs):
+    """Set cc_hip_sort_set knobs for the block, restoring the old values."""
+    lib = nat.gpu_lib()
+    old = {k: lib.cc_hip_sort_get(k.encode()) for k in knobs}
+    try:
+        for k, v in knobs.items():
+            assert lib.cc_hip_sort_set(k.encode(), v) == 0, k
+        yield
+    finally:
+        for k, v in old.items():
+            lib.cc_hip_sort_set(k.encode(), v)
 
 
 def records(count, seed, rep_bits=24, remote_bits=26):
@@ -19,7 +36,7 @@ def records(count, seed, rep_bits=24, remote_bits=26):
     return (remote << np.uint64(32)) | rep
 
 
-SIZES = [0, 1, 2, 3, 64, 4095, 4096, 4097, 100_000, 1_000_003]
+SIZES = [0, 1, 2, 3, 64, 4095, 4096, 4097, 8191, 8192, 8193, 100_000, 1_000_003]
 
 
 @pytest.mark.parametrize("count", SIZES)
@@ -30,20 +47,17 @@ def test_dropin_sort_matches_oracle(gpu, count):
     assert np.array_equal(got, oracle.sort_u64(k))
 
 
-@pytest.mark.parametrize("staged,items", [(0, 16), (1, 16), (0, 32)])
+# (row-striped tiles) or (a run of items per thread: staged stores, items)
+LAYOUTS = [(1, 0, 32), (0, 0, 16), (0, 1, 16), (0, 0, 32)]
+
+
+@pytest.mark.parametrize("rows,staged,items", LAYOUTS)
 @pytest.mark.parametrize("count", SIZES)
-def test_dropin_dedup_matches_oracle(gpu, count, staged, items):
-    from cc_mpi_amd import _native as nat
-    lib = nat.gpu_lib()
+def test_dropin_dedup_matches_oracle(gpu, count, rows, staged, items):
     k = oracle.sort_u64(records(count, seed=count + 1, rep_bits=8, remote_bits=9))
     got = k.copy()
-    try:
-        assert lib.cc_hip_sort_set(b"dedup_staged", staged) == 0
-        assert lib.cc_hip_sort_set(b"dedup_items", items) == 0
+    with sort_knobs(compact_rows=rows, dedup_staged=staged, dedup_items=items):
         n = bd.boundary_edges_dedup(got)
-    finally:
-        lib.cc_hip_sort_set(b"dedup_staged", 0)
-        lib.cc_hip_sort_set(b"dedup_items", 32)
     want = oracle.dedup_u64(k)
     assert n == want.size
     assert np.array_equal(got[:n], want)
@@ -111,17 +125,21 @@ def test_device_sort_ping_pong_and_repeat(gpu):
         assert np.array_equal(out, want)
 
 
-def test_device_dedup_and_ghosts(gpu):
-    k = oracle.sort_u64(records(1_500_000, seed=5, rep_bits=12, remote_bits=14))
+@pytest.mark.parametrize("rows", [1, 0])
+@pytest.mark.parametrize("count", [1_500_000, 4097, 1])
+def test_device_dedup_and_ghosts(gpu, rows, count):
+    k = oracle.sort_u64(records(count, seed=5, rep_bits=12, remote_bits=14))
     a, b, c = DeviceBuffer.from_array(k), DeviceBuffer(k.nbytes), DeviceBuffer(k.nbytes)
-    n = bd.dedup_device(a, k.size, b)
+    with sort_knobs(compact_rows=rows):
+        n = bd.dedup_device(a, k.size, b)
     uniq = oracle.dedup_u64(k)
     assert n == uniq.size
     got = np.empty(n, dtype=np.uint64)
     b.download(got)
     assert np.array_equal(got, uniq)
     ghosts_d = DeviceBuffer(4 * n)
-    g = bd.ghost_extract_device(b, n, c, ghosts_d)
+    with sort_knobs(compact_rows=rows):
+        g = bd.ghost_extract_device(b, n, c, ghosts_d)
     want_rec, want_ghosts = oracle.ghost_extract(uniq)
     assert g == want_ghosts.size
     gg = np.empty(g, dtype=np.uint32)
@@ -164,9 +182,10 @@ def delta_case(seed, n_send=50_000, nseg=4, empty=()):
     return stf, seg, comp_label, comp_of, vs, ve, prev
 
 
+@pytest.mark.parametrize("rows", [1, 0])
 @pytest.mark.parametrize("seed,nseg,empty", [(0, 1, ()), (1, 4, ()), (2, 5, (0, 2)),
-                                             (3, 3, (2,)), (4, 200, ())])
-def test_delta_pack_rounds(gpu, seed, nseg, empty):
+                                             (3, 3, (2,)), (4, 200, ()), (5, 6, (5,))])
+def test_delta_pack_rounds(gpu, seed, nseg, empty, rows):
     stf, seg, cl, co, vs, ve, prev = delta_case(seed, nseg=nseg, empty=empty)
     d_stf, d_cl, d_co = (DeviceBuffer.from_array(x) for x in (stf, cl, co))
     d_prev = DeviceBuffer.from_array(prev)
@@ -174,8 +193,9 @@ def test_delta_pack_rounds(gpu, seed, nseg, empty):
     rng = np.random.default_rng(seed + 100)
     prev_o = prev.copy()
     for rnd in range(4):
-        pairs, cnt, dis = bd.delta_pack_device(d_stf, stf.size, seg, d_cl, d_co, vs, ve, d_prev,
-                                               sendbuf)
+        with sort_knobs(compact_rows=rows):
+            pairs, cnt, dis = bd.delta_pack_device(d_stf, stf.size, seg, d_cl, d_co, vs, ve,
+                                                   d_prev, sendbuf)
         wp, wc, wd = oracle.delta_pack(stf, seg, cl, co, vs, ve, prev_o)
         assert np.array_equal(pairs, wp), f"round {rnd}"
         assert np.array_equal(cnt, wc) and np.array_equal(dis, wd)
@@ -189,22 +209,41 @@ def test_delta_pack_rounds(gpu, seed, nseg, empty):
     sendbuf.close()
 
 
-def test_delta_pack_nothing_to_send(gpu):
+@pytest.mark.parametrize("rows", [1, 0])
+def test_delta_pack_empty_trailing_segments(gpu, rows):
+    """Destinations at the end with nothing asked: their segment starts equal
+    total_send_to, and their counts and displacements still come out right."""
+    stf, seg, cl, co, vs, ve, prev = delta_case(12, nseg=5)
+    seg[3:] = stf.size
+    d = [DeviceBuffer.from_array(x) for x in (stf, cl, co, prev)]
+    sendbuf = bd.PinnedBuffer(stf.size)
+    with sort_knobs(compact_rows=rows):
+        pairs, cnt, dis = bd.delta_pack_device(d[0], stf.size, seg, d[1], d[2], vs, ve, d[3],
+                                               sendbuf)
+    wp, wc, wd = oracle.delta_pack(stf, seg, cl, co, vs, ve, prev.copy())
+    assert np.array_equal(pairs, wp) and np.array_equal(cnt, wc) and np.array_equal(dis, wd)
+    assert cnt[3] == 0 and cnt[4] == 0
+    sendbuf.close()
+
+
+@pytest.mark.parametrize("rows", [1, 0])
+def test_delta_pack_nothing_to_send(gpu, rows):
     stf, seg, cl, co, vs, ve, prev = delta_case(9)
     prev[:] = 0  # everything already sent at label 0
     d = [DeviceBuffer.from_array(x) for x in (stf, cl, co, prev)]
     sendbuf = bd.PinnedBuffer(stf.size)
-    pairs, cnt, dis = bd.delta_pack_device(d[0], stf.size, seg, d[1], d[2], vs, ve, d[3], sendbuf)
+    with sort_knobs(compact_rows=rows):
+        pairs, cnt, dis = bd.delta_pack_device(d[0], stf.size, seg, d[1], d[2], vs, ve, d[3],
+                                               sendbuf)
     assert pairs.size == 0 and cnt.sum() == 0 and (dis == 0).all()
 
 
-@pytest.mark.parametrize("items", [16, 32])
+@pytest.mark.parametrize("rows,items", [(1, 32), (0, 16), (0, 32)])
 @pytest.mark.parametrize("roots", [1, 2, 3, 1000, 65537, "all"])
-def test_boundary_build_matches_three_steps(gpu, roots, items):
+def test_boundary_build_matches_three_steps(gpu, roots, rows, items):
     """cc_hip_boundary_build_device (reps packed by their rank among the
     local roots) gives the bytes of rep fill -> sort -> dedup."""
     import ctypes
-    from cc_mpi_amd import _native as nat
     lib = nat.gpu_lib()
     rng = np.random.default_rng(11 if roots == "all" else roots)
     nl = 200_000
@@ -222,13 +261,10 @@ def test_boundary_build_matches_three_steps(gpu, roots, items):
     d_rec, d_alt = DeviceBuffer.from_array(rec), DeviceBuffer(rec.nbytes)
     d_comp = DeviceBuffer.from_array(comp_of)
     kept, in_alt = ctypes.c_uint64(), ctypes.c_int()
-    assert lib.cc_hip_sort_set(b"dedup_items", items) == 0
-    try:
+    with sort_knobs(compact_rows=rows, dedup_items=items):
         nat.check(lib.cc_hip_boundary_build_device(d_rec.ptr, d_alt.ptr, count, d_comp.ptr, nl,
                                                    ctypes.byref(kept), ctypes.byref(in_alt), None),
                   "bb")
-    finally:
-        lib.cc_hip_sort_set(b"dedup_items", 32)
     got = np.empty(int(kept.value), dtype=np.uint64)
     (d_alt if in_alt.value else d_rec).download(got)
     assert np.array_equal(got, want)

@@ -21,14 +21,18 @@ def graphs():
 
 @pytest.fixture(params=[0, 1], ids=["direct", "staged"])
 def store_strategy(request):
-    """Both store strategies of the split and dedup compactions."""
+    """Both store strategies of the split, and both tile layouts of the
+    other compactions (row-striped; a run of items per thread with staged
+    stores)."""
     from cc_mpi_amd import _native as nat
     lib = nat.gpu_lib()
-    for k in (b"split_staged", b"dedup_staged"):
-        assert lib.cc_hip_sort_set(k, request.param) == 0
-    yield request.param
-    for k in (b"split_staged", b"dedup_staged"):
-        lib.cc_hip_sort_set(k, 0)
+    p = request.param
+    knobs = {b"split_staged": p, b"dedup_staged": p, b"compact_rows": 1 - p}
+    for k, v in knobs.items():
+        assert lib.cc_hip_sort_set(k, v) == 0
+    yield p
+    for k, v in {b"split_staged": 0, b"dedup_staged": 0, b"compact_rows": 1}.items():
+        lib.cc_hip_sort_set(k, v)
 
 
 @pytest.mark.parametrize("P,rank", [(1, 0), (4, 0), (4, 1), (7, 6)])
@@ -130,9 +134,10 @@ def test_rank_share(gpu, name, rank, interval):
     full.close()
 
 
+@pytest.mark.parametrize("rows", [1, 0])
 @pytest.mark.parametrize("P,grouped", [(1, True), (2, True), (4, True), (7, False), (64, True),
                                        (64, False)])
-def test_send_lists_match_their_definition(gpu, P, grouped):
+def test_send_lists_match_their_definition(gpu, P, grouped, rows):
     """cc_hip_send_lists_device equals, list for list, the sorted unique
     owned ids per destination rank (owner keys -> sort -> dedup ->
     segments), with records grouped by row or in arbitrary order."""
@@ -155,8 +160,13 @@ def test_send_lists_match_their_definition(gpu, P, grouped):
     d_b = cm.DeviceBuffer.from_array(bounds)
     d_vals = cm.DeviceBuffer(4 * max(1, min(rec.size, nl * P)))
     starts = np.zeros(P + 1, dtype=np.int32)
-    nat.check(lib.cc_hip_send_lists_device(d_rec.ptr, rec.size, nl, d_b.ptr, P, vs,
-                                           nat.ptr(starts, ctypes.c_int), d_vals.ptr, None), "sl")
+    assert lib.cc_hip_sort_set(b"compact_rows", rows) == 0
+    try:
+        nat.check(lib.cc_hip_send_lists_device(d_rec.ptr, rec.size, nl, d_b.ptr, P, vs,
+                                               nat.ptr(starts, ctypes.c_int), d_vals.ptr, None),
+                  "sl")
+    finally:
+        lib.cc_hip_sort_set(b"compact_rows", 1)
     vals = np.empty(int(starts[P]), dtype=np.uint32)
     d_vals.download(vals)
     low = (rec & np.uint64(0xFFFFFFFF)).astype(np.int64)
