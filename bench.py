@@ -331,9 +331,9 @@ def rankshare_leg(args) -> dict:
         emitted = runs["1"]["pairs_emitted"] / max(1, runs["1"]["packs"])
         out["delta_pack_algorithmic_gbs"] = round((b + 12.0 * emitted) /
                                                   (kern["delta_pack"]["avg_us"] * 1e3), 1)
-    if "sort_onesweep" in kern:
-        out["boundary_sort_onesweep_gbs"] = round(16.0 * rs.nb / (kern["sort_onesweep"]["avg_us"]
-                                                                  * 1e3), 1)
+    # raw boundary records turned into sorted unique pairs per second (rep
+    # fill, the dominant rep's bitmap, sort of the rest, dedup, merge)
+    out["boundary_build_grecords_per_s"] = round(rs.nb / rs.times["boundary_build"] / 1e9, 2)
     rs.close()
     full.close()
     return out

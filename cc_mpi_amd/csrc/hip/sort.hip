@@ -664,6 +664,199 @@ __global__ __launch_bounds__(kThreads) void k_set_rep_packed(uint64_t *__restric
   }
 }
 
+// ---- boundary build, dominant rep split off through a bitmap ---------------
+// Most boundary records of a rank carry one rep, the root of its giant local
+// component (at the 4-rank share of the 65.6M-vertex graph, 46.5M of the
+// 52.8M distinct pairs, from 638M raw records).  Their distinct pairs are
+// just the distinct remotes, so those records only set one bit per remote;
+// the rest go through pack -> sort -> dedup as before, and a merge of the
+// two sorted lists gives the bytes of the full sort + dedup.
+constexpr int kModeSamples = 1024;
+
+// Most frequent packed rep (rank among the roots) among kModeSamples records
+// spread evenly over the input; ties go to the smaller rank.  One workgroup.
+__global__ __launch_bounds__(kModeSamples) void k_mode_rank(const uint64_t *__restrict__ rec,
+                                                            uint64_t count,
+                                                            const uint32_t *__restrict__ packed,
+                                                            uint32_t n, uint32_t *__restrict__ out) {
+  constexpr uint32_t kSlots = 2 * kModeSamples;
+  constexpr uint32_t kEmpty = 0xFFFFFFFFu;
+  __shared__ uint32_t keys[kSlots];
+  __shared__ uint32_t cnts[kSlots];
+  __shared__ uint64_t best[kModeSamples / kWave];
+  const int t = threadIdx.x;
+  for (uint32_t i = t; i < kSlots; i += kModeSamples) {
+    keys[i] = kEmpty;
+    cnts[i] = 0;
+  }
+  __syncthreads();
+  {
+    const uint32_t li = (uint32_t)rec[(uint64_t)t * count / kModeSamples];
+    const uint32_t x = li < n ? packed[li] : 0u;
+    uint32_t h = (x * 0x9E3779B1u) >> 21;  // 11-bit slot
+    for (;;) {
+      const uint32_t prev = atomicCAS(&keys[h], kEmpty, x);
+      if (prev == kEmpty || prev == x) {
+        atomicAdd(&cnts[h], 1u);
+        break;
+      }
+      h = (h + 1) & (kSlots - 1);
+    }
+  }
+  __syncthreads();
+  uint64_t mine = 0;  // (count, ~rank): the max is the most frequent, then the smallest
+  for (uint32_t i = t; i < kSlots; i += kModeSamples)
+    if (cnts[i]) {
+      const uint64_t cand = ((uint64_t)cnts[i] << 32) | (uint32_t)~keys[i];
+      if (cand > mine) mine = cand;
+    }
+  for (int d = kWave / 2; d > 0; d >>= 1) {
+    const uint64_t o = __shfl_xor(mine, d, kWave);
+    if (o > mine) mine = o;
+  }
+  if ((t & (kWave - 1)) == 0) best[t / kWave] = mine;
+  __syncthreads();
+  if (t == 0) {
+    uint64_t b = best[0];
+    for (int w = 1; w < kModeSamples / kWave; ++w)
+      if (best[w] > b) b = best[w];
+    out[0] = ~(uint32_t)b;
+  }
+}
+
+// The split as an ordered compaction (k_compact_rows): records whose
+// packed rep is *g_rank set bit `remote` of the bitmap; the others are kept,
+// packed as (remote << bits) | rank, in input order.
+struct SplitRows {
+  using Val = uint64_t;
+  static constexpr int kRows = 16;
+  const uint64_t *rec;
+  const uint32_t *packed;
+  uint32_t n;
+  int bits;
+  const uint32_t *g_rank;
+  uint32_t *bitmap;
+  uint64_t *others;
+  template <int Rows>
+  __device__ void rows(uint64_t wbase, int lane, uint64_t count, uint64_t (&v)[Rows],
+                       uint64_t (&bal)[Rows]) const {
+    const uint32_t g = *g_rank;
+    uint32_t rank[Rows];
+#pragma unroll
+    for (int i = 0; i < Rows; ++i) {
+      const uint64_t j = wbase + (uint64_t)i * kWave + lane;
+      v[i] = j < count ? rec[j] : 0;
+    }
+#pragma unroll
+    for (int i = 0; i < Rows; ++i) {
+      const uint64_t j = wbase + (uint64_t)i * kWave + lane;
+      const uint32_t li = (uint32_t)v[i];
+      rank[i] = (j < count && li < n) ? packed[li] : 0u;
+    }
+#pragma unroll
+    for (int i = 0; i < Rows; ++i) {
+      const uint64_t j = wbase + (uint64_t)i * kWave + lane;
+      const uint32_t remote = (uint32_t)(v[i] >> 32);
+      const bool dominant = j < count && rank[i] == g;
+      if (dominant) {
+        uint32_t *word = &bitmap[remote >> 5];
+        const uint32_t bit = 1u << (remote & 31);
+        // Read first: most bits are already set, and a plain read is far
+        // cheaper than an atomic (an atomic on every record measured 3x
+        // slower on 638M records); a stale read only costs a redundant atomic.
+        if (!(*word & bit)) atomicOr(word, bit);
+      }
+      bal[i] = __ballot(j < count && !dominant);
+      v[i] = ((uint64_t)remote << bits) | rank[i];
+    }
+  }
+  __device__ void apply(uint64_t, bool keep, uint64_t rank, uint64_t v) const {
+    if (keep) others[rank] = v;
+  }
+};
+
+__global__ __launch_bounds__(kThreads) void k_word_counts(const uint32_t *__restrict__ bitmap,
+                                                          uint64_t words, uint32_t *__restrict__ cnt) {
+  for (uint64_t w = (uint64_t)blockIdx.x * kThreads + threadIdx.x; w < words;
+       w += (uint64_t)gridDim.x * kThreads)
+    cnt[w] = (uint32_t)__popc(bitmap[w]);
+}
+
+// Set bits in ascending order as records (remote << 32) | rep of *g_rank,
+// word w's bits starting at off[w].
+__global__ __launch_bounds__(kThreads) void k_emit_bits(const uint32_t *__restrict__ bitmap,
+                                                        uint64_t words,
+                                                        const uint64_t *__restrict__ off,
+                                                        const uint32_t *__restrict__ rep_of_rank,
+                                                        const uint32_t *__restrict__ g_rank,
+                                                        uint64_t *__restrict__ out) {
+  const uint64_t rep = rep_of_rank[*g_rank];
+  for (uint64_t w = (uint64_t)blockIdx.x * kThreads + threadIdx.x; w < words;
+       w += (uint64_t)gridDim.x * kThreads) {
+    uint32_t b = bitmap[w];
+    uint64_t o = off[w];
+    while (b) {
+      const uint32_t k = (uint32_t)__ffs(b) - 1;
+      b &= b - 1;
+      out[o++] = ((w * 32 + k) << 32) | rep;
+    }
+  }
+}
+
+// Merge of two sorted arrays with no key in common (merge path), one tile
+// of kMergeTile outputs per workgroup: the tile's first and last diagonals
+// are found by binary search in HBM, the two input runs they bound are
+// staged in LDS (coalesced), each thread merges its 16 outputs after a
+// binary search of its own diagonal in LDS, and the tile is written out
+// from LDS in one coalesced pass.
+constexpr int kMergePer = 16;
+constexpr int kMergeTile = kThreads * kMergePer;
+
+// How many of the first d outputs come from a: the smallest i with
+// a[i] > b[d - i - 1] (the arrays share no key).
+template <class A, class B>
+__device__ __forceinline__ uint64_t merge_split(const A &a, uint64_t na, const B &b, uint64_t nb,
+                                                uint64_t d) {
+  uint64_t lo = d > nb ? d - nb : 0, hi = d < na ? d : na;
+  while (lo < hi) {
+    const uint64_t mid = (lo + hi) >> 1;
+    if (a[mid] < b[d - mid - 1]) lo = mid + 1;
+    else hi = mid;
+  }
+  return lo;
+}
+
+__global__ __launch_bounds__(kThreads) void k_merge_tiles(const uint64_t *__restrict__ a, uint64_t na,
+                                                          const uint64_t *__restrict__ b, uint64_t nb,
+                                                          uint64_t *__restrict__ out) {
+  __shared__ uint64_t s_in[kMergeTile];
+  __shared__ uint64_t s_out[kMergeTile];
+  __shared__ uint64_t s_split[2];
+  const uint64_t total = na + nb;
+  const int t = threadIdx.x;
+  for (uint64_t tile = blockIdx.x; tile * kMergeTile < total; tile += gridDim.x) {
+    const uint64_t d0 = tile * kMergeTile;
+    const uint64_t d1 = d0 + kMergeTile < total ? d0 + kMergeTile : total;
+    if (t < 2) s_split[t] = merge_split(a, na, b, nb, t == 0 ? d0 : d1);
+    __syncthreads();
+    const uint64_t i0 = s_split[0], i1 = s_split[1];
+    const uint32_t ta = (uint32_t)(i1 - i0), tn = (uint32_t)(d1 - d0), tb = tn - ta;
+    const uint64_t j0 = d0 - i0;
+    for (uint32_t x = t; x < tn; x += kThreads) s_in[x] = x < ta ? a[i0 + x] : b[j0 + (x - ta)];
+    __syncthreads();
+    const uint64_t *la = s_in, *lb = s_in + ta;
+    const uint32_t dk = (uint32_t)t * kMergePer;
+    if (dk < tn) {
+      uint32_t i = (uint32_t)merge_split(la, ta, lb, tb, dk), j = dk - i;
+      const uint32_t end = dk + kMergePer < tn ? dk + kMergePer : tn;
+      for (uint32_t o = dk; o < end; ++o) s_out[o] = (j >= tb || (i < ta && la[i] < lb[j])) ? la[i++] : lb[j++];
+    }
+    __syncthreads();
+    for (uint32_t x = t; x < tn; x += kThreads) out[d0 + x] = s_out[x];
+    __syncthreads();  // s_split and the LDS runs are rewritten by the next tile
+  }
+}
+
 // Adjacent-unique over packed keys, writing (remote << 32) | rep.
 template <int Items = kItems>
 struct DedupUnpackOp {
@@ -751,6 +944,7 @@ int g_dedup_items = 32;
 
 namespace {
 int g_sort_items = 32;  // keys per thread in a scatter pass (tile = 256 x items; 32 measured best)
+int g_boundary_split = 1;  // boundary build: dominant rep through a bitmap (0: sort every record)
 }  // namespace
 
 extern "C" int cc_hip_sort_set(const char *key, int value) {
@@ -765,6 +959,10 @@ extern "C" int cc_hip_sort_set(const char *key, int value) {
   }
   if (key && strcmp(key, "compact_rows") == 0 && (value == 0 || value == 1)) {
     ccx::g_compact_rows = value;
+    return 0;
+  }
+  if (key && strcmp(key, "boundary_split") == 0 && (value == 0 || value == 1)) {
+    g_boundary_split = value;
     return 0;
   }
   if (key && strcmp(key, "dedup_staged") == 0 && (value == 0 || value == 1)) {
@@ -782,6 +980,7 @@ extern "C" int cc_hip_sort_get(const char *key) {
   if (key && strcmp(key, "items_per_thread") == 0) return g_sort_items;
   if (key && strcmp(key, "split_staged") == 0) return ccx::g_split_staged;
   if (key && strcmp(key, "compact_rows") == 0) return ccx::g_compact_rows;
+  if (key && strcmp(key, "boundary_split") == 0) return g_boundary_split;
   if (key && strcmp(key, "dedup_staged") == 0) return ccx::g_dedup_staged;
   if (key && strcmp(key, "dedup_items") == 0) return ccx::g_dedup_items;
   return -1;
@@ -971,6 +1170,128 @@ extern "C" int cc_hip_send_lists_device(const uint64_t *d_records, uint64_t coun
   return 0;
 }
 
+namespace {
+// Adjacent-unique over packed keys (remote << bits) | rank into
+// (remote << 32) | rep, in the layout the knobs select.
+int dedup_unpack(const uint64_t *in, uint64_t *out, uint64_t count, const uint32_t *rep_of_rank,
+                 int bits, hipStream_t s, ccx::StreamState *st, uint64_t *kept) {
+  if (ccx::g_compact_rows)
+    return run_compact_rows("dedup_compact", DedupUnpackRows{in, out, rep_of_rank, bits}, count, s,
+                            st, kept);
+  if (ccx::g_dedup_items == 32)
+    return run_compact("dedup_compact", DedupUnpackOp<32>{in, out, rep_of_rank, bits}, count, s, st,
+                       kept);
+  return run_compact("dedup_compact", DedupUnpackOp<>{in, out, rep_of_rank, bits}, count, s, st,
+                     kept);
+}
+
+unsigned grid_cap(uint64_t threads, int per_cu) {
+  const uint64_t need = (threads + kThreads - 1) / kThreads;
+  const ccx::Context *c = ccx::ctx();
+  const uint64_t cap = (uint64_t)(c && c->cu_count > 0 ? c->cu_count : 256) * per_cu;
+  return (unsigned)(need == 0 ? 1 : (need < cap ? need : cap));
+}
+
+// Boundary build with the dominant rep split off (see SplitRows):
+// records of the most frequent rep set bits of a bitmap over the remotes,
+// the others are packed, sorted and deduplicated, and the set bits, read
+// in order, are merged with them.  On return the result is in d_records or
+// d_alt (*result_in_alt), *out_count records.  `packed` holds each local
+// vertex's rep as its rank among the roots, rep_of_rank the inverse.
+int build_split_dominant(uint64_t *d_records, uint64_t *d_alt, uint64_t count,
+                         const uint32_t *packed, const uint32_t *rep_of_rank, uint32_t n_local,
+                         int bits, hipStream_t s, ccx::StreamState *st, uint64_t *out_count,
+                         int *result_in_alt) {
+  unsigned long long *d_ao = (unsigned long long *)(st->d_words + 32);  // AND, OR
+  uint32_t *d_g = st->d_words + 38;                                      // dominant rank
+  const uint64_t init[2] = {~0ull, 0ull};
+  memcpy(st->h_stage, init, sizeof(init));
+  CCX_TRY(hipMemcpyAsync(d_ao, st->h_stage, sizeof(init), hipMemcpyHostToDevice, s));
+  CCX_LAUNCH("sort_and_or", s, k_and_or<<<grid_cap(count, 8), kThreads, 0, s>>>(d_records, count, d_ao));
+  CCX_LAUNCH("mode_rep", s,
+             k_mode_rank<<<1, kModeSamples, 0, s>>>(d_records, count, packed, n_local, d_g));
+  CCX_TRY(hipMemcpyAsync(st->h_stage, d_ao, sizeof(init), hipMemcpyDeviceToHost, s));
+  CCX_TRY(hipStreamSynchronize(s));
+  // the OR's high half bounds every remote
+  const uint64_t words = (st->h_stage[1] >> 37) + 1;
+  const size_t map_bytes = (words * 4 + 7) & ~(size_t)7;
+  char *bm = nullptr;
+  const size_t bm_bytes = 2 * map_bytes + (words + 1) * 8;
+  if (hipMalloc(&bm, bm_bytes) != hipSuccess)
+    return ccx::set_error(3, "cc_hip_boundary_build_device: hipMalloc of %zu bytes failed", bm_bytes);
+  uint32_t *bitmap = (uint32_t *)bm;
+  uint32_t *wcnt = (uint32_t *)(bm + map_bytes);
+  uint64_t *woff = (uint64_t *)(bm + 2 * map_bytes);
+  int rc = 0;
+  do {
+    if (hipMemsetAsync(bitmap, 0, words * 4, s) != hipSuccess) {
+      rc = ccx::set_error(1, "cc_hip_boundary_build_device: bitmap clear failed");
+      break;
+    }
+    uint64_t n_other = 0;
+    if ((rc = run_compact_rows("split_dominant",
+                               SplitRows{d_records, packed, n_local, bits, d_g, bitmap, d_alt},
+                               count, s, st, &n_other)))
+      break;
+    if (n_other > count) {
+      rc = ccx::set_error(1, "cc_hip_boundary_build_device: %llu others from %llu records",
+                          (unsigned long long)n_other, (unsigned long long)count);
+      break;
+    }
+    // the others: sort (d_alt <-> d_records; every raw record has been read)
+    // and dedup from S into T
+    uint64_t *S = d_alt, *T = d_records;
+    if (n_other > 1) {
+      int in_alt = 0;
+      if ((rc = ccx::sort_u64(d_alt, d_records, n_other, &in_alt, s))) break;
+      if (in_alt) {
+        S = d_records;
+        T = d_alt;
+      }
+    }
+    uint64_t n_o = 0;
+    if ((rc = dedup_unpack(S, T, n_other, rep_of_rank, bits, s, st, &n_o))) break;
+    // the dominant rep's pairs, in remote order, after the others in T (they
+    // fit: n_o <= n_other and the set bits are at most count - n_other)
+    CCX_LAUNCH_RC(rc, "bitmap_counts", s,
+                  k_word_counts<<<grid_cap(words, 8), kThreads, 0, s>>>(bitmap, words, wcnt));
+    if (rc) break;
+    if ((rc = cc_hip_exclusive_scan_u32_device(wcnt, words, woff, s))) break;
+    if (hipMemcpyAsync(st->h_stage, woff + words, 8, hipMemcpyDeviceToHost, s) != hipSuccess ||
+        hipStreamSynchronize(s) != hipSuccess) {
+      rc = ccx::set_error(1, "cc_hip_boundary_build_device: reading the bitmap count failed");
+      break;
+    }
+    const uint64_t n_g = st->h_stage[0];
+    if (n_o + n_g > count) {
+      rc = ccx::set_error(1, "cc_hip_boundary_build_device: %llu + %llu pairs from %llu records",
+                          (unsigned long long)n_o, (unsigned long long)n_g,
+                          (unsigned long long)count);
+      break;
+    }
+    CCX_LAUNCH_RC(rc, "bitmap_emit", s,
+                  k_emit_bits<<<grid_cap(words, 8), kThreads, 0, s>>>(bitmap, words, woff, rep_of_rank,
+                                                                   d_g, T + n_o));
+    if (rc) break;
+    uint64_t *result = T;
+    if (n_o > 0 && n_g > 0) {
+      const uint64_t tiles = (n_o + n_g + kMergeTile - 1) / kMergeTile;
+      const uint64_t cap = (uint64_t)(ccx::ctx()->cu_count > 0 ? ccx::ctx()->cu_count : 256) * 2;
+      CCX_LAUNCH_RC(rc, "bitmap_merge", s,
+                    k_merge_tiles<<<(unsigned)(tiles < cap ? tiles : cap), kThreads, 0, s>>>(
+                        T + n_o, n_g, T, n_o, S));
+      if (rc) break;
+      result = S;
+    }
+    *out_count = n_o + n_g;
+    *result_in_alt = (result == d_alt);
+  } while (0);
+  (void)hipStreamSynchronize(s);
+  (void)hipFree(bm);
+  return rc;
+}
+}  // namespace
+
 extern "C" int cc_hip_boundary_build_device(uint64_t *d_records, uint64_t *d_alt, uint64_t count,
                                             const uint32_t *d_comp_of, uint32_t n_local,
                                             uint64_t *out_count, int *result_in_alt,
@@ -1014,6 +1335,11 @@ extern "C" int cc_hip_boundary_build_device(uint64_t *d_records, uint64_t *d_alt
     const uint64_t roots = st->h_stage[0];
     int bits = 1;
     while (bits < 32 && (1ull << bits) < roots) ++bits;
+    if (g_boundary_split) {
+      rc = build_split_dominant(d_records, d_alt, count, flags, rep_of_rank, n_local, bits, s, st,
+                                out_count, result_in_alt);
+      break;
+    }
     const uint64_t gb = (count + kThreads - 1) / kThreads;
     CCX_LAUNCH_RC(rc, "boundary_set_rep", s,
                   k_set_rep_packed<<<(unsigned)(gb < 65536 ? gb : 65536), kThreads, 0, s>>>(
@@ -1023,16 +1349,7 @@ extern "C" int cc_hip_boundary_build_device(uint64_t *d_records, uint64_t *d_alt
     if ((rc = ccx::sort_u64(d_records, d_alt, count, &in_alt, s))) break;
     uint64_t *sorted = in_alt ? d_alt : d_records, *other = in_alt ? d_records : d_alt;
     uint64_t kept = 0;
-    if (ccx::g_compact_rows)
-      rc = run_compact_rows("dedup_compact", DedupUnpackRows{sorted, other, rep_of_rank, bits},
-                            count, s, st, &kept);
-    else if (ccx::g_dedup_items == 32)
-      rc = run_compact("dedup_compact", DedupUnpackOp<32>{sorted, other, rep_of_rank, bits}, count,
-                       s, st, &kept);
-    else
-      rc = run_compact("dedup_compact", DedupUnpackOp<>{sorted, other, rep_of_rank, bits}, count, s,
-                       st, &kept);
-    if (rc) break;
+    if ((rc = dedup_unpack(sorted, other, count, rep_of_rank, bits, s, st, &kept))) break;
     *out_count = kept;
     *result_in_alt = (other == d_alt);
   } while (0);

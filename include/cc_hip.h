@@ -174,8 +174,12 @@ int cc_hip_sort_u64_device(uint64_t *d_keys, uint64_t *d_alt, uint64_t count,
  * d_comp_of[0..n_local) must map every local vertex to a root r of its
  * local forest (d_comp_of[r] == r), as cc_hip_components_device leaves it.
  * Reps are sorted by their rank among the roots, which keeps the order and
- * shortens the key to 32 + ceil(log2 roots) bits (fewer digit passes).
- * *out_count receives the kept count; the records are in d_alt when
+ * shortens the key to 32 + ceil(log2 roots) bits (fewer digit passes); the
+ * records of the most frequent rep (sampled) only set bits of a bitmap over
+ * the remotes, and the other records alone are sorted, then merged with the
+ * set bits read in order ("boundary_split" of cc_hip_sort_set).  Temporary
+ * device memory: 12 bytes per local vertex and about 2 bytes per possible
+ * remote id.  *out_count receives the kept count; the records are in d_alt when
  * *result_in_alt is 1, else in d_records; the other buffer is clobbered.
  * Blocking. */
 int cc_hip_boundary_build_device(uint64_t *d_records, uint64_t *d_alt, uint64_t count,
@@ -189,11 +193,14 @@ int cc_hip_boundary_build_device(uint64_t *d_records, uint64_t *d_alt, uint64_t 
  * consecutive items per row; 0: a run of consecutive items per thread,
  * where the next two knobs apply); "dedup_staged" (0 default: each thread
  * stores its own items; 1: outputs staged in LDS and stored as contiguous
- * runs); "dedup_items" (16 or 32 (default) records per thread); and of the
- * split: "split_staged" (as dedup_staged; staging measured slower at 854M
- * entries because the LDS costs occupancy).  The result never depends on
- * them.  _set returns nonzero for a bad key or value, _get -1 for an
- * unknown key. */
+ * runs); "dedup_items" (16 or 32 (default) records per thread); of the
+ * boundary build: "boundary_split" (1 default: the records of the most
+ * frequent rep go through a bitmap over the remotes and only the rest are
+ * sorted, then the two sorted lists are merged; 0: every record sorted);
+ * and of the split: "split_staged" (as dedup_staged; staging measured
+ * slower at 854M entries because the LDS costs occupancy).  The result
+ * never depends on them.  _set returns nonzero for a bad key or value, _get
+ * -1 for an unknown key. */
 int cc_hip_sort_set(const char *key, int value);
 int cc_hip_sort_get(const char *key);
 

@@ -238,11 +238,13 @@ def test_delta_pack_nothing_to_send(gpu, rows):
     assert pairs.size == 0 and cnt.sum() == 0 and (dis == 0).all()
 
 
-@pytest.mark.parametrize("rows,items", [(1, 32), (0, 16), (0, 32)])
+@pytest.mark.parametrize("split,rows,items", [(1, 1, 32), (0, 1, 32), (1, 0, 16), (0, 0, 32)])
 @pytest.mark.parametrize("roots", [1, 2, 3, 1000, 65537, "all"])
-def test_boundary_build_matches_three_steps(gpu, roots, rows, items):
+def test_boundary_build_matches_three_steps(gpu, roots, split, rows, items):
     """cc_hip_boundary_build_device (reps packed by their rank among the
-    local roots) gives the bytes of rep fill -> sort -> dedup."""
+    local roots; with split, the most frequent rep's records through a
+    bitmap over the remotes, merged back) gives the bytes of rep fill ->
+    sort -> dedup."""
     import ctypes
     lib = nat.gpu_lib()
     rng = np.random.default_rng(11 if roots == "all" else roots)
@@ -261,7 +263,41 @@ def test_boundary_build_matches_three_steps(gpu, roots, rows, items):
     d_rec, d_alt = DeviceBuffer.from_array(rec), DeviceBuffer(rec.nbytes)
     d_comp = DeviceBuffer.from_array(comp_of)
     kept, in_alt = ctypes.c_uint64(), ctypes.c_int()
-    with sort_knobs(compact_rows=rows, dedup_items=items):
+    with sort_knobs(boundary_split=split, compact_rows=rows, dedup_items=items):
+        nat.check(lib.cc_hip_boundary_build_device(d_rec.ptr, d_alt.ptr, count, d_comp.ptr, nl,
+                                                   ctypes.byref(kept), ctypes.byref(in_alt), None),
+                  "bb")
+    got = np.empty(int(kept.value), dtype=np.uint64)
+    (d_alt if in_alt.value else d_rec).download(got)
+    assert np.array_equal(got, want)
+    for b in (d_rec, d_alt, d_comp):
+        b.close()
+
+
+@pytest.mark.parametrize("split", [1, 0])
+@pytest.mark.parametrize("count,remote_bits", [(1, 26), (7, 26), (1000, 32), (500_000, 32),
+                                               (2_000_003, 26)])
+def test_boundary_build_giant_component(gpu, split, count, remote_bits):
+    """The shape of a real rank's boundary: one local component holds most
+    vertices (so most records carry its root), the rest are small; remotes
+    up to 2^32 - 1.  Same bytes as rep fill -> sort -> dedup."""
+    import ctypes
+    lib = nat.gpu_lib()
+    rng = np.random.default_rng(count + remote_bits)
+    nl = 100_000
+    comp_of = np.full(nl, 17, dtype=np.uint32)  # the giant component's root is vertex 17
+    comp_of[:17] = np.arange(17, dtype=np.uint32)
+    small = rng.choice(np.arange(18, nl), 3000, replace=False).astype(np.uint32)
+    comp_of[small] = small  # 3000 singleton components
+    li = rng.integers(0, nl, count, dtype=np.uint64)
+    remote = rng.integers(0, 1 << remote_bits, count, dtype=np.uint64)
+    remote[: count // 3] = remote[count // 3: 2 * (count // 3)]  # repeated remotes
+    rec = (remote << np.uint64(32)) | li
+    want = oracle.dedup_u64(oracle.sort_u64((remote << np.uint64(32)) | comp_of[li].astype(np.uint64)))
+    d_rec, d_alt = DeviceBuffer.from_array(rec), DeviceBuffer(rec.nbytes)
+    d_comp = DeviceBuffer.from_array(comp_of)
+    kept, in_alt = ctypes.c_uint64(), ctypes.c_int()
+    with sort_knobs(boundary_split=split):
         nat.check(lib.cc_hip_boundary_build_device(d_rec.ptr, d_alt.ptr, count, d_comp.ptr, nl,
                                                    ctypes.byref(kept), ctypes.byref(in_alt), None),
                   "bb")
