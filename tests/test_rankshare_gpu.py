@@ -19,15 +19,15 @@ def graphs():
             "grid": lambda: devgen.grid(200, 250, 0.7, seed=33)}
 
 
-@pytest.fixture(params=[0, 1], ids=["direct", "staged"])
+@pytest.fixture(params=[0, 1, 2], ids=["rows", "runs_staged", "runs_direct"])
 def store_strategy(request):
-    """Both store strategies of the split, and both tile layouts of the
-    other compactions (row-striped; a run of items per thread with staged
-    stores)."""
+    """Every layout of the split and the other compactions: row-striped
+    tiles, or a run of items per thread with staged or direct stores."""
     from cc_mpi_amd import _native as nat
     lib = nat.gpu_lib()
     p = request.param
-    knobs = {b"split_staged": p, b"dedup_staged": p, b"compact_rows": 1 - p}
+    knobs = {b"split_staged": int(p == 1), b"dedup_staged": int(p == 1),
+             b"compact_rows": int(p == 0)}
     for k, v in knobs.items():
         assert lib.cc_hip_sort_set(k, v) == 0
     yield p
@@ -35,11 +35,20 @@ def store_strategy(request):
         lib.cc_hip_sort_set(k, v)
 
 
-@pytest.mark.parametrize("P,rank", [(1, 0), (4, 0), (4, 1), (7, 6)])
-def test_split_matches_oracle(gpu, store_strategy, P, rank):
+SPLIT_GRAPHS = {"rmat13": lambda: cm.rmat(13, 16, seed=41),
+                "rmat17": lambda: cm.rmat(17, 16, seed=42),
+                # long runs of empty rows: most vertices isolated
+                "er_sparse": lambda: cm.erdos_renyi(300_000, 0.2, seed=43),
+                "grid": lambda: cm.grid(300, 500, 0.7, seed=44)}
+
+
+@pytest.mark.parametrize("graph,P,rank", [("rmat13", 1, 0), ("rmat13", 4, 0), ("rmat13", 4, 1),
+                                          ("rmat13", 7, 6), ("rmat17", 4, 2), ("rmat17", 1, 0),
+                                          ("er_sparse", 3, 1), ("grid", 5, 4)])
+def test_split_matches_oracle(gpu, store_strategy, graph, P, rank):
     import ctypes
     from cc_mpi_amd import _native as nat
-    h = cm.rmat(13, 16, seed=41)
+    h = SPLIT_GRAPHS[graph]()
     b = block_bounds(h.n, P)
     vs, ve = int(b[rank]), int(b[rank + 1])
     nl = ve - vs
