@@ -302,11 +302,14 @@ def rankshare_leg(args) -> dict:
         got = rs.merge(truth, exchange_interval=interval, seed=11,
                        check_pack=check if interval == 1 else None)
         ok = bool(np.array_equal(got, want))
-        # `seconds` is the whole loop (merge steps, packs, the other ranks'
-        # deltas applied by the unpack kernel) and, at interval 1, the
-        # oracle check of every pack; merge_steps_s + delta_packs_s is the
-        # merge and pack part
+        # `seconds` is the whole call: the merge state's set-up (label
+        # arrays, prev_sent, the pinned send buffer), the loop and, at
+        # interval 1, the oracle check of every pack; `loop_s` is the loop
+        # alone (merge steps, packs, the other ranks' deltas applied by the
+        # unpack kernel, and at interval 1 the checks); merge_steps_s +
+        # delta_packs_s is its merge and pack part
         runs[str(interval)] = {"seconds": round(time.perf_counter() - t0, 3), "labels_exact": ok,
+                               "loop_s": round(rs.times["merge_loop"], 4),
                                "merge_steps_s": round(rs.times["merge_steps"], 4),
                                "delta_packs_s": round(rs.times["delta_packs"], 4),
                                **{k: rs.stats[k] for k in ("merge_iterations", "exchanges",
@@ -324,7 +327,8 @@ def rankshare_leg(args) -> dict:
     out = {"graph": "Chung-Lu n=65601024 pairs=1.8e9 gamma 2.1 seed 1003, rank 0 of 4 "
                     "(vertex blocks)",
            "stats": rs.stats, "preprocess_seconds": {k: round(v, 4) for k, v in rs.times.items()
-                                                      if k not in ("merge_steps", "delta_packs")},
+                                                      if k not in ("merge_steps", "delta_packs",
+                                                                   "merge_loop")},
            "intervals": runs, "packs_checked_against_oracle": checked[0], "kernels": kern}
     if "delta_pack" in kern:
         b = 16.0 * rs.total_send  # + 12 per emitted pair, averaged below

@@ -225,6 +225,7 @@ class RankShare:
         pending_since = 1 if (ng and self.nbd and nl) else 0
         steps, k, iters, packs, emitted_total = 0, 0, 0, 0, 0
         t_merge = t_pack = 0.0
+        t_loop0 = time.perf_counter()  # the loop alone, after the buffers above are set up
         changed = ctypes.c_int(0)
         while True:
             t0 = time.perf_counter()
@@ -273,6 +274,7 @@ class RankShare:
                 k += 1
                 pending_since = 0
                 steps = 0
+        t_loop = time.perf_counter() - t_loop0
         labels = DeviceBuffer(4 * max(1, nl))
         roots = ctypes.c_uint64()
         nat.check(lib.cc_hip_rank_labels_device(comp_label.ptr, self.comp_of.ptr, nl, vs,
@@ -283,6 +285,7 @@ class RankShare:
             b.close()
         sendbuf.close()
         self.times["merge_steps"] = t_merge
+        self.times["merge_loop"] = t_loop
         self.times["delta_packs"] = t_pack
         self.stats.update(merge_iterations=iters, exchanges=k, packs=packs,
                           pairs_emitted=emitted_total, local_roots=int(roots.value))
