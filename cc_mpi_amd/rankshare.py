@@ -97,8 +97,10 @@ class RankShare:
         self._t("send_lists", t0)
         keys = DeviceBuffer(max(8, 8 * self.nb))
 
-        # boundary records: rep fill, sort and dedup in one call (reps sorted
-        # by their rank among the local roots), then ghost extraction
+        # boundary records: rep fill, sort and dedup in one call (the most
+        # frequent rep's records through a bitmap over the remotes, the rest
+        # sorted by rank among the local roots, the two merged), then ghost
+        # extraction
         t0 = time.perf_counter()
         kept, in_alt = ctypes.c_uint64(), ctypes.c_int()
         nat.check(lib.cc_hip_boundary_build_device(rec.ptr, keys.ptr, self.nb, self.comp_of.ptr, nl,
