@@ -729,7 +729,7 @@ __global__ __launch_bounds__(kModeSamples) void k_mode_rank(const uint64_t *__re
 // packed as (remote << bits) | rank, in input order.
 struct SplitRows {
   using Val = uint64_t;
-  static constexpr int kRows = 16;
+  static constexpr int kRows = 32;  // 4% faster than 16 on 638M records
   const uint64_t *rec;
   const uint32_t *packed;
   uint32_t n;
