@@ -28,6 +28,10 @@ CASES = [
      False),
     (4, ["--gen", "grid", "--rows", "64", "--cols", "64", "--keep", "0.9", "--seed", "26"], 1,
      True),
+    # larger shares: many split tiles per rank, a giant local component
+    # (the boundary build's bitmap path), many small components
+    (4, ["--gen", "rmat", "--scale", "18", "--ef", "16", "--seed", "27"], 2, True),
+    (2, ["--gen", "er", "--n", "200000", "--deg", "0.8", "--seed", "28"], 3, False),
 ]
 
 
