@@ -170,15 +170,16 @@ def solve_plan(lib, n: int, m: int) -> dict:
     return {k: int(lib.cc_hip_components_plan_get(n, m, k.encode())) for k in keys}
 
 
-def cpu_baseline(G, budget_s: float, runs: int = 3, impl: str = "reference"):
+def cpu_baseline(G, budget_s: float, runs: int = 3, impl: str = "reference", chunk: int = 8192):
     """The reference's pthreads CC on this host (oracle/_ref), or the oracle
     restatement if the reference build is absent.  Median of up to `runs`
     timed runs after one warm-up, bounded by budget_s; runs=1 times a single
-    cold run (for graphs where one solve already fills the budget)."""
+    cold run (for graphs where one solve already fills the budget).  chunk:
+    the dynamic chunk size (8192 as the reference's run script,
+    run_benchmark.sh:21; 2048 is its bench's default)."""
     import oracle
     aff = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else os.cpu_count()
     threads = max(1, min(aff or 1, int(os.environ.get("OMP_NUM_THREADS", "16") or 16), 16))
-    chunk = 8192
     ref = oracle.ref_lib() if impl == "reference" else None
     kind = "reference" if ref is not None else "port"
     fn = oracle.ref_cc_pthreads if ref is not None else oracle.cc_lp_pthreads
@@ -503,6 +504,13 @@ def main() -> int:
         parity = bool(np.array_equal(ref_labels, labels))
         log(f"cpu baseline: {cpu['value']:.4f} GTEPS ({cpu['seconds']:.2f} s/solve, "
             f"{cpu['cores']} threads); GPU labels equal to the reference's: {parity}")
+        if cpu["seconds"] * (args.cpu_runs + 1) < args.cpu_budget / 2:
+            # also time the reference bench's default chunk, 2048
+            # (src/mpi_cc_benchmark.c:40)
+            alt_labels, alt = cpu_baseline(G, args.cpu_budget / 2, args.cpu_runs, args.cpu_impl,
+                                           chunk=2048)
+            parity = parity and bool(np.array_equal(alt_labels, labels))
+            cpu["chunk_2048"] = dict(value=alt["value"], sample=alt["sample"])
         cpu.pop("seconds")
     else:
         import oracle
