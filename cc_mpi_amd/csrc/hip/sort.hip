@@ -11,7 +11,7 @@
 //     ballot matching, so skewed digits do not serialise LDS atomics);
 //     digits that are constant over the input are skipped;
 //   * each digit pass: a workgroup takes the next 8192-key tile (atomic tile
-//     counter; 32 keys per thread), ranks its keys stably with wave64 ballot matching into
+//     counter; 512 threads, 16 keys each), ranks its keys stably with wave64 ballot matching into
 //     per-wave LDS counters, publishes its 256 digit counts, resolves the
 //     global offsets by decoupled look-back over the preceding tiles, and
 //     scatters through LDS so that each digit's run of the tile is written
@@ -60,15 +60,34 @@ __device__ __forceinline__ uint64_t lanemask_lt(int lane) {
   return (lane == 0) ? 0ull : (~0ull >> (64 - lane));
 }
 
-// AND and OR of all keys: a bit that is equal in both is constant.
+// AND and OR of all keys: a bit that is equal in both is constant.  Each
+// thread keeps 16 coalesced loads in flight per chunk (one load at a time
+// held the pass to 4 TB/s).
+constexpr int kAndOrItems = 16;
+
 __global__ __launch_bounds__(kThreads) void k_and_or(const uint64_t *__restrict__ keys, uint64_t n,
                                                      unsigned long long *__restrict__ out) {
   uint64_t a = ~0ull, o = 0;
-  for (uint64_t i = (uint64_t)blockIdx.x * kThreads + threadIdx.x; i < n;
-       i += (uint64_t)gridDim.x * kThreads) {
-    const uint64_t k = keys[i];
-    a &= k;
-    o |= k;
+  const uint64_t step = (uint64_t)gridDim.x * kThreads * kAndOrItems;
+  uint64_t i = (uint64_t)blockIdx.x * kThreads * kAndOrItems + threadIdx.x;
+  for (; i + (uint64_t)(kAndOrItems - 1) * kThreads < n; i += step) {
+    uint64_t k[kAndOrItems];
+#pragma unroll
+    for (int j = 0; j < kAndOrItems; ++j) k[j] = keys[i + (uint64_t)j * kThreads];
+#pragma unroll
+    for (int j = 0; j < kAndOrItems; ++j) {
+      a &= k[j];
+      o |= k[j];
+    }
+  }
+  // at most one partial chunk per thread: the next chunk starts past n
+#pragma unroll
+  for (int j = 0; j < kAndOrItems; ++j) {
+    const uint64_t pos = i + (uint64_t)j * kThreads;
+    if (pos < n) {
+      a &= keys[pos];
+      o |= keys[pos];
+    }
   }
 #pragma unroll
   for (int d = kWave / 2; d > 0; d >>= 1) {
@@ -152,30 +171,53 @@ __global__ __launch_bounds__(kThreads) void k_histograms(const uint64_t *__restr
   }
 }
 
+// goff[d][b] = exclusive scan over b of hist[d][b]: workgroup d scans
+// digit d's 256 bins (one per thread).
+__global__ __launch_bounds__(kBins) void k_digit_offsets(const unsigned long long *__restrict__ hist,
+                                                         uint64_t *__restrict__ goff) {
+  __shared__ uint64_t s_wave[kBins / kWave];
+  const int b = threadIdx.x, lane = b & (kWave - 1), w = b / kWave;
+  const uint64_t v = hist[blockIdx.x * kBins + b];
+  uint64_t inc = v;
+#pragma unroll
+  for (int d = 1; d < kWave; d <<= 1) {
+    const uint64_t u = __shfl_up(inc, d, kWave);
+    if (lane >= d) inc += u;
+  }
+  if (lane == kWave - 1) s_wave[w] = inc;
+  __syncthreads();
+  uint64_t before = 0;
+  for (int ww = 0; ww < w; ++ww) before += s_wave[ww];
+  goff[blockIdx.x * kBins + b] = before + inc - v;
+}
+
 // One LSD pass on digit `shift/8`: stable scatter of in[] into out[].
 // goff[d] = first output position of digit d (exclusive scan of the global
 // histogram).  tile_ctr hands out tiles in order; status holds per-tile,
 // per-digit look-back words (kBins per tile).
-template <int Items>
-__global__ __launch_bounds__(kThreads) void k_onesweep(const uint64_t *__restrict__ in,
-                                                       uint64_t *__restrict__ out, uint64_t n,
-                                                       int shift, uint32_t vary,
-                                                       const uint64_t *__restrict__ goff,
-                                                       uint32_t *__restrict__ tile_ctr,
-                                                       uint64_t *__restrict__ status, uint32_t epoch) {
-  constexpr int TileK = kThreads * Items;
+template <int Items, int Threads = kThreads>
+__global__ __launch_bounds__(Threads) void k_onesweep(const uint64_t *__restrict__ in,
+                                                      uint64_t *__restrict__ out, uint64_t n,
+                                                      int shift, uint32_t vary,
+                                                      const uint64_t *__restrict__ goff,
+                                                      uint32_t *__restrict__ tile_ctr,
+                                                      uint64_t *__restrict__ status, uint32_t epoch) {
+  static_assert(Threads >= kBins && Threads % kWave == 0, "a thread per digit");
+  constexpr int TileK = Threads * Items;
+  constexpr int Waves = Threads / kWave;
   __shared__ uint64_t s_keys[TileK];
-  __shared__ uint32_t s_wcnt[kWaves][kBins];  // per-wave digit counts, then exclusive offsets
-  __shared__ uint32_t s_dstart[kBins];        // tile-local start of each digit
-  __shared__ uint64_t s_gbase[kBins];         // global position of tile-local index 0 of digit d
-  __shared__ uint32_t s_wsum[kWaves];
+  __shared__ uint32_t s_wcnt[Waves][kBins];  // per-wave digit counts, then exclusive offsets
+  __shared__ uint32_t s_dstart[kBins];       // tile-local start of each digit
+  __shared__ uint64_t s_gbase[kBins];        // global position of tile-local index 0 of digit d
+  __shared__ uint32_t s_wsum[kBins / kWave];
   __shared__ uint32_t s_tile;
 
   const int t = threadIdx.x;
   const int lane = t & (kWave - 1);
   const int w = t / kWave;
+  const bool owner = t < kBins;  // threads 0..255 each own one digit
   if (t == 0) s_tile = atomicAdd(tile_ctr, 1u);
-  for (int i = t; i < kWaves * kBins; i += kThreads) (&s_wcnt[0][0])[i] = 0;
+  for (int i = t; i < Waves * kBins; i += Threads) (&s_wcnt[0][0])[i] = 0;
   __syncthreads();
   const uint64_t tile = s_tile;
   // Tile ids come from a counter zeroed before the launch and run 0..tiles-1;
@@ -208,36 +250,39 @@ __global__ __launch_bounds__(kThreads) void k_onesweep(const uint64_t *__restric
   }
   __syncthreads();
 
-  // Thread t owns digit t: per-wave exclusive offsets and the tile count.
+  // Thread t < 256 owns digit t: per-wave exclusive offsets and the tile count.
   uint32_t cnt = 0;
+  uint64_t *my_status = status + tile * kBins + (owner ? t : 0);
+  if (owner) {
 #pragma unroll
-  for (int ww = 0; ww < kWaves; ++ww) {
-    const uint32_t c = s_wcnt[ww][t];
-    s_wcnt[ww][t] = cnt;
-    cnt += c;
+    for (int ww = 0; ww < Waves; ++ww) {
+      const uint32_t c = s_wcnt[ww][t];
+      s_wcnt[ww][t] = cnt;
+      cnt += c;
+    }
+    // publish this tile's count for digit t as early as possible
+    if (tile == 0)
+      ccx::status_store(my_status, ccx::tag_word(epoch, ccx::kFlagInclusive, cnt));
+    else
+      ccx::status_store(my_status, ccx::tag_word(epoch, ccx::kFlagAggregate, cnt));
   }
-  // publish this tile's count for digit t as early as possible
-  uint64_t *my_status = status + tile * kBins + t;
-  if (tile == 0)
-    ccx::status_store(my_status, ccx::tag_word(epoch, ccx::kFlagInclusive, cnt));
-  else
-    ccx::status_store(my_status, ccx::tag_word(epoch, ccx::kFlagAggregate, cnt));
 
-  // tile-local exclusive scan over digits
+  // tile-local exclusive scan over digits (the owner waves)
   const uint32_t incl = ccx::wave_incl_scan<uint32_t>(cnt, lane);
-  if (lane == kWave - 1) s_wsum[w] = incl;
+  if (owner && lane == kWave - 1) s_wsum[w] = incl;
   __syncthreads();
-  uint32_t wprefix = 0;
-  for (int ww = 0; ww < w; ++ww) wprefix += s_wsum[ww];
-  const uint32_t dstart = wprefix + incl - cnt;
-  s_dstart[t] = dstart;
-
-  uint64_t excl = 0;
-  if (tile > 0) {
-    excl = ccx::lookback(status, tile, kBins, t, epoch);
-    ccx::status_store(my_status, ccx::tag_word(epoch, ccx::kFlagInclusive, excl + cnt));
+  if (owner) {
+    uint32_t wprefix = 0;
+    for (int ww = 0; ww < w; ++ww) wprefix += s_wsum[ww];
+    const uint32_t dstart = wprefix + incl - cnt;
+    s_dstart[t] = dstart;
+    uint64_t excl = 0;
+    if (tile > 0) {
+      excl = ccx::lookback(status, tile, kBins, t, epoch);
+      ccx::status_store(my_status, ccx::tag_word(epoch, ccx::kFlagInclusive, excl + cnt));
+    }
+    s_gbase[t] = goff[t] + excl - dstart;
   }
-  s_gbase[t] = goff[t] + excl - dstart;
   __syncthreads();
 
   // Stable scatter into LDS in digit order.
@@ -251,7 +296,7 @@ __global__ __launch_bounds__(kThreads) void k_onesweep(const uint64_t *__restric
   }
   __syncthreads();
   const uint64_t valid_in_tile = (n - tile_base) < (uint64_t)TileK ? (n - tile_base) : TileK;
-  for (uint32_t j = t; j < valid_in_tile; j += kThreads) {
+  for (uint32_t j = t; j < valid_in_tile; j += Threads) {
     const uint64_t k = s_keys[j];
     const uint32_t d = (uint32_t)(k >> shift) & 0xFFu;
     out[s_gbase[d] + j] = k;
@@ -943,7 +988,11 @@ int g_dedup_items = 32;
 }  // namespace ccx
 
 namespace {
-int g_sort_items = 32;  // keys per thread in a scatter pass (tile = 256 x items; 32 measured best)
+// Scatter-pass tile: threads x items keys.  512 x 16 (8192 keys, 16 waves
+// per CU at 2 workgroups) measured 1.44 ms per pass over 268M keys, against
+// 1.65 ms for 256 x 32 and 1.89 ms for 1024 x 8.
+int g_sort_items = 16;     // keys per thread (8, 16, 24, 32)
+int g_sort_threads = 512;  // threads per workgroup (256; 512 with 8 or 16 items; 1024 with 8)
 int g_boundary_split = 1;  // boundary build: dominant rep through a bitmap (0: sort every record)
 }  // namespace
 
@@ -955,6 +1004,10 @@ extern "C" int cc_hip_sort_set(const char *key, int value) {
   }
   if (key && strcmp(key, "split_staged") == 0 && (value == 0 || value == 1)) {
     ccx::g_split_staged = value;
+    return 0;
+  }
+  if (key && strcmp(key, "sort_threads") == 0 && (value == 256 || value == 512 || value == 1024)) {
+    g_sort_threads = value;
     return 0;
   }
   if (key && strcmp(key, "compact_rows") == 0 && (value == 0 || value == 1)) {
@@ -978,6 +1031,7 @@ extern "C" int cc_hip_sort_set(const char *key, int value) {
 
 extern "C" int cc_hip_sort_get(const char *key) {
   if (key && strcmp(key, "items_per_thread") == 0) return g_sort_items;
+  if (key && strcmp(key, "sort_threads") == 0) return g_sort_threads;
   if (key && strcmp(key, "split_staged") == 0) return ccx::g_split_staged;
   if (key && strcmp(key, "compact_rows") == 0) return ccx::g_compact_rows;
   if (key && strcmp(key, "boundary_split") == 0) return g_boundary_split;
@@ -1007,7 +1061,8 @@ int sort_u64(uint64_t *keys, uint64_t *alt, uint64_t n, int *in_alt, hipStream_t
   uint64_t init_ao[2] = {~0ull, 0ull};
   memcpy(st->h_stage, init_ao, sizeof(init_ao));
   CCX_TRY(hipMemcpyAsync(d_ao, st->h_stage, sizeof(init_ao), hipMemcpyHostToDevice, s));
-  const unsigned red_blocks = (unsigned)((n + kThreads - 1) / kThreads < 2048 ? (n + kThreads - 1) / kThreads : 2048);
+  const uint64_t chunks = (n + kThreads * kAndOrItems - 1) / (kThreads * kAndOrItems);
+  const unsigned red_blocks = (unsigned)(chunks < 2048 ? chunks : 2048);
   CCX_LAUNCH("sort_and_or", s, k_and_or<<<red_blocks, kThreads, 0, s>>>(keys, n, d_ao));
   CCX_TRY(hipMemcpyAsync(st->h_stage, d_ao, 2 * sizeof(uint64_t), hipMemcpyDeviceToHost, s));
   CCX_TRY(hipStreamSynchronize(s));
@@ -1028,43 +1083,38 @@ int sort_u64(uint64_t *keys, uint64_t *alt, uint64_t n, int *in_alt, hipStream_t
     const unsigned hb = (unsigned)(tiles < cap ? tiles : cap);
     CCX_LAUNCH("sort_histograms", s, k_histograms<<<hb, kThreads, 0, s>>>(keys, n, plan, d_hist));
   }
-  uint64_t *h = st->h_stage;
-  CCX_TRY(hipMemcpyAsync(h, d_hist, 8 * kBins * sizeof(uint64_t), hipMemcpyDeviceToHost, s));
-  CCX_TRY(hipStreamSynchronize(s));
-  uint64_t *offs = st->h_stage + 8 * kBins;
-  for (int d = 0; d < 8; ++d) {
-    uint64_t acc = 0;
-    for (int b = 0; b < kBins; ++b) {
-      offs[d * kBins + b] = acc;
-      acc += h[d * kBins + b];
-    }
-  }
-  CCX_TRY(hipMemcpyAsync(d_goff, offs, 8 * kBins * sizeof(uint64_t), hipMemcpyHostToDevice, s));
-  // the staging buffer is reused by the next call: let the copy land first
-  CCX_TRY(hipStreamSynchronize(s));
-
+  CCX_LAUNCH("sort_digit_offsets", s, k_digit_offsets<<<8, kBins, 0, s>>>(d_hist, d_goff));
   const int items = g_sort_items;
-  const uint64_t tile_keys = (uint64_t)kThreads * items;
+  // larger workgroups only with tiles that fit their LDS (512 threads: 8 or
+  // 16 keys each; 1024 threads: 8)
+  const int threads = (g_sort_threads == 1024 && items == 8)  ? 1024
+                      : (g_sort_threads >= 512 && items <= 16) ? 512
+                                                               : kThreads;
+  const uint64_t tile_keys = (uint64_t)threads * items;
   const uint64_t tiles = (n + tile_keys - 1) / tile_keys;
   if (tiles > 0xFFFFFFFFull) return set_error(4, "sort: input too large");
   if (int rc = ensure_status(st, tiles * kBins)) return rc;
   uint64_t *src = keys, *dst = alt;
   int pass = 0;
+  // one tile counter per pass (at most 8), zeroed together
+  CCX_TRY(hipMemsetAsync(st->d_words + 8, 0, 8 * sizeof(uint32_t), s));
   for (int d = 0; d < 8; ++d) {
     if (!plan.vary[d]) continue;
     uint32_t epoch;
     if (int rc = next_epoch(st, &epoch)) return rc;
     uint32_t *ctr = st->d_words + 8 + pass;
-    CCX_TRY(hipMemsetAsync(ctr, 0, sizeof(uint32_t), s));
-#define CCX_ONESWEEP(I)                                                                      \
+#define CCX_ONESWEEP(I, T)                                                                   \
   CCX_LAUNCH("sort_onesweep", s,                                                             \
-             k_onesweep<I><<<(unsigned)tiles, kThreads, 0, s>>>(src, dst, n, 8 * d, plan.vary[d], \
-                                                                d_goff + d * kBins, ctr,       \
-                                                                st->d_status, epoch))
-    if (items == 8) CCX_ONESWEEP(8);
-    else if (items == 24) CCX_ONESWEEP(24);
-    else if (items == 32) CCX_ONESWEEP(32);
-    else CCX_ONESWEEP(16);
+             (k_onesweep<I, T><<<(unsigned)tiles, T, 0, s>>>(src, dst, n, 8 * d, plan.vary[d], \
+                                                             d_goff + d * kBins, ctr,          \
+                                                             st->d_status, epoch)))
+    if (threads == 1024) CCX_ONESWEEP(8, 1024);
+    else if (threads == 512 && items == 8) CCX_ONESWEEP(8, 512);
+    else if (threads == 512) CCX_ONESWEEP(16, 512);
+    else if (items == 8) CCX_ONESWEEP(8, 256);
+    else if (items == 24) CCX_ONESWEEP(24, 256);
+    else if (items == 32) CCX_ONESWEEP(32, 256);
+    else CCX_ONESWEEP(16, 256);
 #undef CCX_ONESWEEP
     uint64_t *tt = src;
     src = dst;
@@ -1207,7 +1257,7 @@ int build_split_dominant(uint64_t *d_records, uint64_t *d_alt, uint64_t count,
   const uint64_t init[2] = {~0ull, 0ull};
   memcpy(st->h_stage, init, sizeof(init));
   CCX_TRY(hipMemcpyAsync(d_ao, st->h_stage, sizeof(init), hipMemcpyHostToDevice, s));
-  CCX_LAUNCH("sort_and_or", s, k_and_or<<<grid_cap(count, 8), kThreads, 0, s>>>(d_records, count, d_ao));
+  CCX_LAUNCH("sort_and_or", s, k_and_or<<<grid_cap((count + kAndOrItems - 1) / kAndOrItems, 8), kThreads, 0, s>>>(d_records, count, d_ao));
   CCX_LAUNCH("mode_rep", s,
              k_mode_rank<<<1, kModeSamples, 0, s>>>(d_records, count, packed, n_local, d_g));
   CCX_TRY(hipMemcpyAsync(st->h_stage, d_ao, sizeof(init), hipMemcpyDeviceToHost, s));

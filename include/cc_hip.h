@@ -186,8 +186,10 @@ int cc_hip_boundary_build_device(uint64_t *d_records, uint64_t *d_alt, uint64_t 
                                  const uint32_t *d_comp_of, uint32_t n_local, uint64_t *out_count,
                                  int *result_in_alt, void *stream);
 
-/* Tuning of the sort's scatter passes: "items_per_thread" (8, 16, 24 or 32
- * (default) keys per thread; a workgroup's tile is 256 x that); of the
+/* Tuning of the sort's scatter passes: "items_per_thread" (8, 16 (default),
+ * 24 or 32 keys per thread) and "sort_threads" (256, 512 (default) with 8 or
+ * 16 keys, 1024 with 8; other pairs run 256 threads): a workgroup's tile is
+ * threads x keys; of the
  * ordered compactions (dedup, ghost extraction, send lists, delta pack):
  * "compact_rows" (1 default: row-striped tiles, each wave reading 64
  * consecutive items per row; 0: a run of consecutive items per thread,

@@ -87,15 +87,16 @@ def test_sort_key_shapes(gpu, kind):
     assert np.array_equal(got, np.sort(k))
 
 
-@pytest.mark.parametrize("items", [8, 16, 24, 32])
+@pytest.mark.parametrize("threads,items", [(256, 8), (256, 16), (256, 24), (256, 32), (512, 8),
+                                           (512, 16), (1024, 8)])
 @pytest.mark.parametrize("kind", ["records", "skewed", "full64"])
-def test_sort_every_tile_size(gpu, items, kind):
-    """The scatter passes give the same bytes for every tile size, on tile
+def test_sort_every_tile_size(gpu, threads, items, kind):
+    """The scatter passes give the same bytes for every tile shape, on tile
     boundaries and ragged tails, with skewed and uniform digits."""
-    from cc_mpi_amd import _native as nat
     lib = nat.gpu_lib()
     rng = np.random.default_rng(items)
-    n = 256 * items * 37 + 123
+    tile = threads * items
+    n = tile * 37 + 123
     if kind == "records":
         k = records(n, seed=items)
     elif kind == "skewed":
@@ -103,14 +104,11 @@ def test_sort_every_tile_size(gpu, items, kind):
             0, 1 << 12, n, dtype=np.uint64)
     else:
         k = rng.integers(0, np.iinfo(np.uint64).max, n, dtype=np.uint64, endpoint=True)
-    try:
-        assert lib.cc_hip_sort_set(b"items_per_thread", items) == 0
-        for m in (n, 256 * items, 256 * items + 1, 256 * items * 5):
+    with sort_knobs(items_per_thread=items, sort_threads=threads):
+        for m in (n, tile, tile + 1, tile * 5):
             got = k[:m].copy()
             bd.boundary_edges_sort(got)
-            assert np.array_equal(got, np.sort(k[:m])), (items, kind, m)
-    finally:
-        lib.cc_hip_sort_set(b"items_per_thread", 32)
+            assert np.array_equal(got, np.sort(k[:m])), (threads, items, kind, m)
 
 
 def test_device_sort_ping_pong_and_repeat(gpu):
