@@ -8,8 +8,9 @@
 // sort gives the same bytes.  Here it is an LSD radix sort on the raw
 // little-endian records (8-bit digits), one "onesweep" pass per digit:
 //   * one pass over the keys builds all eight 256-bin histograms (wave64
-//     ballot matching, so skewed digits do not serialise LDS atomics);
-//     digits that are constant over the input are skipped;
+//     ballot matching, so skewed digits do not serialise LDS atomics) and
+//     the AND/OR of the keys; digits that are constant over the input get
+//     no scatter pass;
 //   * each digit pass: a workgroup takes the next 8192-key tile (atomic tile
 //     counter; 512 threads, 16 keys each), ranks its keys stably with wave64 ballot matching into
 //     per-wave LDS counters, publishes its 256 digit counts, resolves the
@@ -60,6 +61,13 @@ __device__ __forceinline__ uint64_t lanemask_lt(int lane) {
   return (lane == 0) ? 0ull : (~0ull >> (64 - lane));
 }
 
+// A value that is equal in every lane of the wave, as a scalar.
+__device__ __forceinline__ uint64_t uniform64(uint64_t x) {
+  const uint32_t lo = __builtin_amdgcn_readfirstlane((uint32_t)x);
+  const uint32_t hi = __builtin_amdgcn_readfirstlane((uint32_t)(x >> 32));
+  return ((uint64_t)hi << 32) | lo;
+}
+
 // AND and OR of all keys: a bit that is equal in both is constant.  Each
 // thread keeps 16 coalesced loads in flight per chunk (one load at a time
 // held the pass to 4 TB/s).
@@ -104,18 +112,20 @@ struct DigitPlan {
   uint32_t vary[8];  // varying bits of each digit (0 = constant digit)
 };
 
-// 256-bin histograms of every non-constant digit, one pass over the keys.
-// Each workgroup walks 4096-key tiles (16 keys per thread in flight, loaded
-// like the scatter passes load them) and counts into per-wave LDS
-// histograms.  A digit with at most 3 varying bits has at most 8 values, so
-// a wave's 64 lanes collide heavily on its bins: those digits are counted
-// by ballot matching (one add per group of equal digits); the others add
-// per lane, except that the lanes sharing lane 0's digit add once.
+// 256-bin histograms of all eight digits and the AND/OR of all keys, one
+// pass over the keys.  Each workgroup walks 4096-key tiles (16 keys per
+// thread in flight, loaded like the scatter passes load them) and counts
+// into per-wave LDS histograms.  A wave first reduces the AND and OR of its
+// 1024 keys: a digit constant over them is counted with one add; a digit
+// with at most 3 varying bits has at most 8 values, so the 64 lanes would
+// collide heavily on its bins: it is counted by ballot matching (one add
+// per group of equal digits); the others add per lane, except that the
+// lanes sharing lane 0's digit add once.
 constexpr int kHistWaveBins = 8 * kBins;
 
 __global__ __launch_bounds__(kThreads) void k_histograms(const uint64_t *__restrict__ keys, uint64_t n,
-                                                         DigitPlan plan,
-                                                         unsigned long long *__restrict__ hist) {
+                                                         unsigned long long *__restrict__ hist,
+                                                         unsigned long long *__restrict__ and_or) {
   __shared__ uint32_t h[kWaves][kHistWaveBins];
   for (int i = threadIdx.x; i < kWaves * kHistWaveBins; i += kThreads) (&h[0][0])[i] = 0;
   __syncthreads();
@@ -123,20 +133,46 @@ __global__ __launch_bounds__(kThreads) void k_histograms(const uint64_t *__restr
   const int w = threadIdx.x / kWave;
   const uint64_t lt = lanemask_lt(lane);
   uint32_t *hw = h[w];
+  uint64_t all_and = ~0ull, all_or = 0;  // wave-uniform
   const uint64_t tiles = (n + kTile - 1) / kTile;
   for (uint64_t tile = blockIdx.x; tile < tiles; tile += gridDim.x) {
-    const uint64_t base = tile * kTile + (uint64_t)w * (kWave * kItems) + lane;
+    const uint64_t wbase = tile * kTile + (uint64_t)w * (kWave * kItems);
+    if (wbase >= n) continue;  // wave-uniform: this wave's keys are all past n
+    const uint64_t base = wbase + lane;
     uint64_t k[kItems];
 #pragma unroll
     for (int i = 0; i < kItems; ++i) {
       const uint64_t pos = base + (uint64_t)i * kWave;
       k[i] = pos < n ? keys[pos] : 0;
     }
+    uint64_t a = ~0ull, o = 0;
+#pragma unroll
+    for (int i = 0; i < kItems; ++i) {
+      if (base + (uint64_t)i * kWave < n) {
+        a &= k[i];
+        o |= k[i];
+      }
+    }
+#pragma unroll
+    for (int d = kWave / 2; d > 0; d >>= 1) {
+      a &= __shfl_xor(a, d, kWave);
+      o |= __shfl_xor(o, d, kWave);
+    }
+    // equal in every lane after the butterfly: keep them in scalar registers
+    // so the per-digit branches below are uniform
+    a = uniform64(a);
+    o = uniform64(o);
+    all_and &= a;
+    all_or |= o;
+    const uint64_t left = n - wbase;
+    const uint32_t nvalid = left < (uint64_t)(kWave * kItems) ? (uint32_t)left : kWave * kItems;
+    const uint64_t diff = a ^ o;
 #pragma unroll
     for (int d = 0; d < 8; ++d) {
-      const uint32_t vary = plan.vary[d];
-      if (!vary) continue;
-      if (__popc(vary) <= 3) {
+      const uint32_t vary = (uint32_t)(diff >> (8 * d)) & 0xFFu;
+      if (vary == 0) {
+        if (lane == 0) atomicAdd(&hw[d * kBins + ((uint32_t)(a >> (8 * d)) & 0xFFu)], nvalid);
+      } else if (__popc(vary) <= 3) {
 #pragma unroll
         for (int i = 0; i < kItems; ++i) {
           const bool valid = base + (uint64_t)i * kWave < n;
@@ -162,6 +198,8 @@ __global__ __launch_bounds__(kThreads) void k_histograms(const uint64_t *__restr
       }
     }
   }
+  if (lane == 0 && all_or != 0) atomicOr(&and_or[1], (unsigned long long)all_or);
+  if (lane == 0 && all_and != ~0ull) atomicAnd(&and_or[0], (unsigned long long)all_and);
   __syncthreads();
   for (int i = threadIdx.x; i < kHistWaveBins; i += kThreads) {
     uint32_t c = 0;
@@ -1061,9 +1099,14 @@ int sort_u64(uint64_t *keys, uint64_t *alt, uint64_t n, int *in_alt, hipStream_t
   uint64_t init_ao[2] = {~0ull, 0ull};
   memcpy(st->h_stage, init_ao, sizeof(init_ao));
   CCX_TRY(hipMemcpyAsync(d_ao, st->h_stage, sizeof(init_ao), hipMemcpyHostToDevice, s));
-  const uint64_t chunks = (n + kThreads * kAndOrItems - 1) / (kThreads * kAndOrItems);
-  const unsigned red_blocks = (unsigned)(chunks < 2048 ? chunks : 2048);
-  CCX_LAUNCH("sort_and_or", s, k_and_or<<<red_blocks, kThreads, 0, s>>>(keys, n, d_ao));
+  CCX_TRY(hipMemsetAsync(d_hist, 0, 8 * kBins * sizeof(uint64_t), s));
+  {
+    // 32 KiB of LDS per workgroup: 4 per CU resident
+    const uint64_t tiles = tiles_for(n);
+    const uint64_t cap = (uint64_t)(ctx()->cu_count > 0 ? ctx()->cu_count : 256) * 4;
+    const unsigned hb = (unsigned)(tiles < cap ? tiles : cap);
+    CCX_LAUNCH("sort_histograms", s, k_histograms<<<hb, kThreads, 0, s>>>(keys, n, d_hist, d_ao));
+  }
   CCX_TRY(hipMemcpyAsync(st->h_stage, d_ao, 2 * sizeof(uint64_t), hipMemcpyDeviceToHost, s));
   CCX_TRY(hipStreamSynchronize(s));
   const uint64_t diff = st->h_stage[0] ^ st->h_stage[1];
@@ -1074,15 +1117,6 @@ int sort_u64(uint64_t *keys, uint64_t *alt, uint64_t n, int *in_alt, hipStream_t
     passes += plan.vary[d] != 0;
   }
   if (passes == 0) return 0;  // all keys equal
-
-  CCX_TRY(hipMemsetAsync(d_hist, 0, 8 * kBins * sizeof(uint64_t), s));
-  {
-    // 32 KiB of LDS per workgroup: 4 per CU resident
-    const uint64_t tiles = tiles_for(n);
-    const uint64_t cap = (uint64_t)(ctx()->cu_count > 0 ? ctx()->cu_count : 256) * 4;
-    const unsigned hb = (unsigned)(tiles < cap ? tiles : cap);
-    CCX_LAUNCH("sort_histograms", s, k_histograms<<<hb, kThreads, 0, s>>>(keys, n, plan, d_hist));
-  }
   CCX_LAUNCH("sort_digit_offsets", s, k_digit_offsets<<<8, kBins, 0, s>>>(d_hist, d_goff));
   const int items = g_sort_items;
   // larger workgroups only with tiles that fit their LDS (512 threads: 8 or
