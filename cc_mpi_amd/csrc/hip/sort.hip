@@ -607,41 +607,55 @@ struct GhostRows {
 // li over its 64 lanes (a segmented scan keyed on li; records of one row
 // are contiguous, so a row costs one atomic per wave it spans), and the
 // lane ending each run ORs the result into mask[li].
+constexpr int kMaskWindows = 8;
+
 __global__ __launch_bounds__(kThreads) void k_owner_mask(const uint64_t *__restrict__ rec, uint64_t nb,
                                                          const uint32_t *__restrict__ bounds, int P,
                                                          uint32_t n_local,
                                                          unsigned long long *__restrict__ mask) {
+  __shared__ uint32_t s_bounds[64];
+  if (threadIdx.x < P) s_bounds[threadIdx.x] = bounds[threadIdx.x];
+  __syncthreads();
   const int lane = threadIdx.x & (kWave - 1);
   const uint64_t nwaves = (uint64_t)gridDim.x * kWaves;
-  // the loop bound is uniform across the wave, so every lane takes part in
-  // the shuffles
-  for (uint64_t c = ((uint64_t)blockIdx.x * kThreads + threadIdx.x) / kWave; c * kWave < nb;
-       c += nwaves) {
-    const uint64_t i = c * kWave + lane;
-    const bool valid = i < nb;
-    uint32_t li = 0xFFFFFFFFu;
-    unsigned long long m = 0;
-    if (valid) {
-      const uint64_t r = rec[i];
-      const uint32_t v = (uint32_t)(r >> 32);
-      li = (uint32_t)r;
-      int lo = 0, hi = P - 1;
-      while (lo < hi) {  // last q with bounds[q] <= v
-        const int mid = (lo + hi + 1) >> 1;
-        if (bounds[mid] <= v) lo = mid;
-        else hi = mid - 1;
-      }
-      m = 1ull << lo;
+  // A wave takes kMaskWindows consecutive windows of 64 records per step
+  // and issues all their loads before using any (one window at a time kept
+  // too few bytes in flight: 2 TB/s).  The loop bound is uniform across the
+  // wave, so every lane takes part in the shuffles.
+  for (uint64_t c = ((uint64_t)blockIdx.x * kThreads + threadIdx.x) / kWave;
+       c * (kWave * kMaskWindows) < nb; c += nwaves) {
+    uint64_t r[kMaskWindows];
+#pragma unroll
+    for (int u = 0; u < kMaskWindows; ++u) {
+      const uint64_t i = (c * kMaskWindows + u) * kWave + lane;
+      r[u] = i < nb ? rec[i] : ~0ull;
     }
 #pragma unroll
-    for (int d = 1; d < kWave; d <<= 1) {
-      const unsigned long long um = __shfl_up(m, d, kWave);
-      const uint32_t ul = __shfl_up(li, d, kWave);
-      if (lane >= d && ul == li) m |= um;
+    for (int u = 0; u < kMaskWindows; ++u) {
+      const bool valid = (c * kMaskWindows + u) * kWave + lane < nb;
+      uint32_t li = 0xFFFFFFFFu;
+      unsigned long long m = 0;
+      if (valid) {
+        const uint32_t v = (uint32_t)(r[u] >> 32);
+        li = (uint32_t)r[u];
+        int lo = 0, hi = P - 1;
+        while (lo < hi) {  // last q with bounds[q] <= v
+          const int mid = (lo + hi + 1) >> 1;
+          if (s_bounds[mid] <= v) lo = mid;
+          else hi = mid - 1;
+        }
+        m = 1ull << lo;
+      }
+#pragma unroll
+      for (int d = 1; d < kWave; d <<= 1) {
+        const unsigned long long um = __shfl_up(m, d, kWave);
+        const uint32_t ul = __shfl_up(li, d, kWave);
+        if (lane >= d && ul == li) m |= um;
+      }
+      const uint32_t next = __shfl_down(li, 1, kWave);
+      const bool run_end = (lane == kWave - 1) || next != li;
+      if (valid && run_end && li < n_local) atomicOr(&mask[li], m);
     }
-    const uint32_t next = __shfl_down(li, 1, kWave);
-    const bool run_end = (lane == kWave - 1) || next != li;
-    if (valid && run_end && li < n_local) atomicOr(&mask[li], m);
   }
 }
 
@@ -1231,7 +1245,7 @@ extern "C" int cc_hip_send_lists_device(const uint64_t *d_records, uint64_t coun
       (unsigned long long *)ccx::stream_tmp(st, (size_t)n_local * sizeof(unsigned long long));
   if (!mask) return 3;
   CCX_TRY(hipMemsetAsync(mask, 0, (size_t)n_local * sizeof(unsigned long long), s));
-  const uint64_t waves = (count + kWave - 1) / kWave;
+  const uint64_t waves = (count + kWave * kMaskWindows - 1) / (kWave * kMaskWindows);
   const uint64_t cap = (uint64_t)(ccx::ctx()->cu_count > 0 ? ccx::ctx()->cu_count : 256) * 8 * kWaves;
   const unsigned grid = (unsigned)(((waves < cap ? waves : cap) + kWaves - 1) / kWaves);
   CCX_LAUNCH("owner_mask", s,
