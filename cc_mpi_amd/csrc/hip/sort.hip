@@ -824,9 +824,10 @@ __global__ __launch_bounds__(kModeSamples) void k_mode_rank(const uint64_t *__re
 // The split as an ordered compaction (k_compact_rows): records whose
 // packed rep is *g_rank set bit `remote` of the bitmap; the others are kept,
 // packed as (remote << bits) | rank, in input order.
+template <int R>
 struct SplitRows {
   using Val = uint64_t;
-  static constexpr int kRows = 32;  // 4% faster than 16 on 638M records
+  static constexpr int kRows = R;
   const uint64_t *rec;
   const uint32_t *packed;
   uint32_t n;
@@ -842,7 +843,7 @@ struct SplitRows {
 #pragma unroll
     for (int i = 0; i < Rows; ++i) {
       const uint64_t j = wbase + (uint64_t)i * kWave + lane;
-      v[i] = j < count ? rec[j] : 0;
+      v[i] = j < count ? __builtin_nontemporal_load(&rec[j]) : 0;
     }
 #pragma unroll
     for (int i = 0; i < Rows; ++i) {
@@ -850,19 +851,25 @@ struct SplitRows {
       const uint32_t li = (uint32_t)v[i];
       rank[i] = (j < count && li < n) ? packed[li] : 0u;
     }
+    // Read first: most bits are already set, and a plain read is far
+    // cheaper than an atomic (an atomic on every record measured 3x slower
+    // on 638M records).  All rows' words are read before any atomic, so the
+    // reads are in flight together (a read after an atomic on the same array
+    // waits for it); a stale read only costs a redundant atomic.
+    uint32_t word[Rows];
+#pragma unroll
+    for (int i = 0; i < Rows; ++i) {
+      const uint64_t j = wbase + (uint64_t)i * kWave + lane;
+      const uint32_t remote = (uint32_t)(v[i] >> 32);
+      word[i] = (j < count && rank[i] == g) ? bitmap[remote >> 5] : ~0u;
+    }
 #pragma unroll
     for (int i = 0; i < Rows; ++i) {
       const uint64_t j = wbase + (uint64_t)i * kWave + lane;
       const uint32_t remote = (uint32_t)(v[i] >> 32);
       const bool dominant = j < count && rank[i] == g;
-      if (dominant) {
-        uint32_t *word = &bitmap[remote >> 5];
-        const uint32_t bit = 1u << (remote & 31);
-        // Read first: most bits are already set, and a plain read is far
-        // cheaper than an atomic (an atomic on every record measured 3x
-        // slower on 638M records); a stale read only costs a redundant atomic.
-        if (!(*word & bit)) atomicOr(word, bit);
-      }
+      const uint32_t bit = 1u << (remote & 31);
+      if (dominant && !(word[i] & bit)) atomicOr(&bitmap[remote >> 5], bit);
       bal[i] = __ballot(j < count && !dominant);
       v[i] = ((uint64_t)remote << bits) | rank[i];
     }
@@ -1046,6 +1053,7 @@ namespace {
 int g_sort_items = 16;     // keys per thread (8, 16, 24, 32)
 int g_sort_threads = 512;  // threads per workgroup (256; 512 with 8 or 16 items; 1024 with 8)
 int g_boundary_split = 1;  // boundary build: dominant rep through a bitmap (0: sort every record)
+int g_split_dominant_rows = 32;  // rows of 64 records per wave step in the bitmap split (16, 32)
 }  // namespace
 
 extern "C" int cc_hip_sort_set(const char *key, int value) {
@@ -1070,6 +1078,10 @@ extern "C" int cc_hip_sort_set(const char *key, int value) {
     g_boundary_split = value;
     return 0;
   }
+  if (key && strcmp(key, "split_dominant_rows") == 0 && (value == 16 || value == 32)) {
+    g_split_dominant_rows = value;
+    return 0;
+  }
   if (key && strcmp(key, "dedup_staged") == 0 && (value == 0 || value == 1)) {
     ccx::g_dedup_staged = value;
     return 0;
@@ -1087,6 +1099,7 @@ extern "C" int cc_hip_sort_get(const char *key) {
   if (key && strcmp(key, "split_staged") == 0) return ccx::g_split_staged;
   if (key && strcmp(key, "compact_rows") == 0) return ccx::g_compact_rows;
   if (key && strcmp(key, "boundary_split") == 0) return g_boundary_split;
+  if (key && strcmp(key, "split_dominant_rows") == 0) return g_split_dominant_rows;
   if (key && strcmp(key, "dedup_staged") == 0) return ccx::g_dedup_staged;
   if (key && strcmp(key, "dedup_items") == 0) return ccx::g_dedup_items;
   return -1;
@@ -1327,10 +1340,15 @@ int build_split_dominant(uint64_t *d_records, uint64_t *d_alt, uint64_t count,
       break;
     }
     uint64_t n_other = 0;
-    if ((rc = run_compact_rows("split_dominant",
-                               SplitRows{d_records, packed, n_local, bits, d_g, bitmap, d_alt},
-                               count, s, st, &n_other)))
-      break;
+    if (g_split_dominant_rows == 16)
+      rc = run_compact_rows("split_dominant",
+                            SplitRows<16>{d_records, packed, n_local, bits, d_g, bitmap, d_alt},
+                            count, s, st, &n_other);
+    else
+      rc = run_compact_rows("split_dominant",
+                            SplitRows<32>{d_records, packed, n_local, bits, d_g, bitmap, d_alt},
+                            count, s, st, &n_other);
+    if (rc) break;
     if (n_other > count) {
       rc = ccx::set_error(1, "cc_hip_boundary_build_device: %llu others from %llu records",
                           (unsigned long long)n_other, (unsigned long long)count);

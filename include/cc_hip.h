@@ -198,8 +198,9 @@ int cc_hip_boundary_build_device(uint64_t *d_records, uint64_t *d_alt, uint64_t 
  * runs); "dedup_items" (16 or 32 (default) records per thread); of the
  * boundary build: "boundary_split" (1 default: the records of the most
  * frequent rep go through a bitmap over the remotes and only the rest are
- * sorted, then the two sorted lists are merged; 0: every record sorted);
- * and of the split: "split_staged" (as dedup_staged; staging measured
+ * sorted, then the two sorted lists are merged; 0: every record sorted)
+ * and "split_dominant_rows" (16 or 32 (default) rows of 64 records per
+ * wave step of that bitmap pass); and of the split: "split_staged" (as dedup_staged; staging measured
  * slower at 854M entries because the LDS costs occupancy).  The result
  * never depends on them.  _set returns nonzero for a bad key or value, _get
  * -1 for an unknown key. */

@@ -272,10 +272,10 @@ def test_boundary_build_matches_three_steps(gpu, roots, split, rows, items):
         b.close()
 
 
-@pytest.mark.parametrize("split", [1, 0])
+@pytest.mark.parametrize("split,split_rows", [(1, 32), (1, 16), (0, 32)])
 @pytest.mark.parametrize("count,remote_bits", [(1, 26), (7, 26), (1000, 32), (500_000, 32),
                                                (2_000_003, 26)])
-def test_boundary_build_giant_component(gpu, split, count, remote_bits):
+def test_boundary_build_giant_component(gpu, split, split_rows, count, remote_bits):
     """The shape of a real rank's boundary: one local component holds most
     vertices (so most records carry its root), the rest are small; remotes
     up to 2^32 - 1.  Same bytes as rep fill -> sort -> dedup."""
@@ -295,7 +295,7 @@ def test_boundary_build_giant_component(gpu, split, count, remote_bits):
     d_rec, d_alt = DeviceBuffer.from_array(rec), DeviceBuffer(rec.nbytes)
     d_comp = DeviceBuffer.from_array(comp_of)
     kept, in_alt = ctypes.c_uint64(), ctypes.c_int()
-    with sort_knobs(boundary_split=split):
+    with sort_knobs(boundary_split=split, split_dominant_rows=split_rows):
         nat.check(lib.cc_hip_boundary_build_device(d_rec.ptr, d_alt.ptr, count, d_comp.ptr, nl,
                                                    ctypes.byref(kept), ctypes.byref(in_alt), None),
                   "bb")
