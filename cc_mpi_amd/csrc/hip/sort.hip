@@ -603,10 +603,10 @@ struct GhostRows {
 // ---- send lists without sorting ---------------------------------------------
 // For the exchange plan, rank q must receive every owned vertex li that has
 // a boundary record whose remote end q owns.  mask[li] collects those q as
-// bits: records are read coalesced, each wave OR-reduces the bits of equal
-// li over its 64 lanes (a segmented scan keyed on li; records of one row
-// are contiguous, so a row costs one atomic per wave it spans), and the
-// lane ending each run ORs the result into mask[li].
+// bits: records are read coalesced, each window of 64 records OR-reduces
+// the bits of each run of equal li (records of one row are contiguous, so a
+// row costs one atomic per window it spans), and the lane ending each run
+// ORs the result into mask[li].
 constexpr int kMaskWindows = 8;
 
 __global__ __launch_bounds__(kThreads) void k_owner_mask(const uint64_t *__restrict__ rec, uint64_t nb,
@@ -632,29 +632,41 @@ __global__ __launch_bounds__(kThreads) void k_owner_mask(const uint64_t *__restr
     }
 #pragma unroll
     for (int u = 0; u < kMaskWindows; ++u) {
-      const bool valid = (c * kMaskWindows + u) * kWave + lane < nb;
+      const uint64_t i = (c * kMaskWindows + u) * kWave + lane;
+      const bool valid = i < nb;
       uint32_t li = 0xFFFFFFFFu;
-      unsigned long long m = 0;
+      int q = 0;
       if (valid) {
         const uint32_t v = (uint32_t)(r[u] >> 32);
         li = (uint32_t)r[u];
-        int lo = 0, hi = P - 1;
-        while (lo < hi) {  // last q with bounds[q] <= v
-          const int mid = (lo + hi + 1) >> 1;
-          if (s_bounds[mid] <= v) lo = mid;
+        int hi = P - 1;
+        while (q < hi) {  // last q with bounds[q] <= v
+          const int mid = (q + hi + 1) >> 1;
+          if (s_bounds[mid] <= v) q = mid;
           else hi = mid - 1;
         }
-        m = 1ull << lo;
       }
-#pragma unroll
-      for (int d = 1; d < kWave; d <<= 1) {
-        const unsigned long long um = __shfl_up(m, d, kWave);
-        const uint32_t ul = __shfl_up(li, d, kWave);
-        if (lane >= d && ul == li) m |= um;
+      // Runs of equal li: a run starts where li differs from the lane below.
+      // This lane's run is [first start at or below it, lane]; its OR of
+      // owner bits takes one ballot per owner present in the window (valid
+      // lanes are a prefix, so lane 0 starts a run whenever any lane is valid).
+      const uint32_t prev = __shfl_up(li, 1, kWave);
+      const uint64_t starts = __ballot(valid && (lane == 0 || prev != li));
+      const uint64_t upto = lane == kWave - 1 ? ~0ull : (2ull << lane) - 1;
+      const uint64_t below = starts & upto;
+      const int first = below ? 63 - __clzll((long long)below) : 0;
+      const uint64_t run = upto & (~0ull << first);
+      unsigned long long m = 0;
+      uint64_t pending = __ballot(valid);
+      while (pending) {
+        const int src = __ffsll((long long)pending) - 1;
+        const int owner = __builtin_amdgcn_readlane(q, src);
+        const uint64_t same = __ballot(valid && q == owner);
+        pending &= ~same;
+        if (same & run) m |= 1ull << owner;
       }
-      const uint32_t next = __shfl_down(li, 1, kWave);
-      const bool run_end = (lane == kWave - 1) || next != li;
-      if (valid && run_end && li < n_local) atomicOr(&mask[li], m);
+      const bool next_starts = lane == kWave - 1 || ((starts >> (lane + 1)) & 1ull) || i + 1 >= nb;
+      if (valid && next_starts && li < n_local) atomicOr(&mask[li], m);
     }
   }
 }

@@ -296,8 +296,8 @@ int cc_hip_owner_keys_device(const uint64_t *d_records, uint64_t count, const ui
 /* The send lists of the exchange plan in one step, without sorting: for
  * each rank q in [0, comm_size) ascending, the ascending owned local ids li
  * (as li + base) that have a raw record (remote << 32) | li whose remote end
- * q owns under d_bounds[0..comm_size] (the same lists as owner keys ->
- * sort -> dedup -> segments).  h_starts[0..comm_size] (host) receives the
+ * q owns under the ascending partition bounds d_bounds[0..comm_size] (the
+ * same lists as owner keys -> sort -> dedup -> segments).  h_starts[0..comm_size] (host) receives the
  * segment starts; d_vals needs room for every listed id (at most
  * min(count, n_local * comm_size)).  Records must carry li < n_local; those
  * of one li should be contiguous (as the split emits them) for speed, not
