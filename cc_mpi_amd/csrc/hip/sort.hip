@@ -187,13 +187,14 @@ __global__ __launch_bounds__(kThreads) void k_histograms(const uint64_t *__restr
           // the giant tree's rep): lanes equal to lane 0's digit are counted
           // by lane 0 with one add, the rest add one each.  Valid lanes are
           // a prefix of the wave, so lane 0 is valid whenever any lane is.
+          // (one atomic instruction: lane 0's digit is `first`)
           const bool valid = base + (uint64_t)i * kWave < n;
           const uint32_t dig = (uint32_t)(k[i] >> (8 * d)) & 0xFFu;
           const uint32_t first = __shfl(dig, 0, kWave);
           const bool same = valid && dig == first;
           const uint64_t bal = __ballot(same);
-          if (lane == 0 && valid) atomicAdd(&hw[d * kBins + first], (uint32_t)__popcll(bal));
-          else if (valid && !same) atomicAdd(&hw[d * kBins + dig], 1u);
+          if (valid && (lane == 0 || !same))
+            atomicAdd(&hw[d * kBins + dig], lane == 0 ? (uint32_t)__popcll(bal) : 1u);
         }
       }
     }
