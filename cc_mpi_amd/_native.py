@@ -120,6 +120,9 @@ def gpu_lib() -> ctypes.CDLL:
     _sig(lib, "cc_hip_iota_u32_device", c_int, [V, c_u32, c_u32, V])
     _sig(lib, "cc_hip_merge_step_device", c_int,
          [V, c_u64, V, V, c_u32, V, V, V, c_u32, c_u32, c_u32, P_int, V])
+    _sig(lib, "cc_hip_merge_step_reps_device", c_int,
+         [V, c_u64, V, V, c_u32, V, V, V, c_u32, V, c_u32, c_u32, c_u32, P_int, V])
+    _sig(lib, "cc_hip_local_reps_device", c_int, [V, c_u32, V, P_u32, V])
     _sig(lib, "cc_hip_rank_labels_device", c_int, [V, V, c_u32, c_u32, V, P_u64, V])
     _sig(lib, "cc_hip_owner_keys_device", c_int, [V, c_u64, V, c_int, V, V])
     _sig(lib, "cc_hip_segments_device", c_int, [V, c_u64, c_int, c_u32, V, V, V])

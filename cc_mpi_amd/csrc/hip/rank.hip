@@ -499,11 +499,21 @@ __global__ __launch_bounds__(kThreads) void k_iota(uint32_t *__restrict__ a, uin
   GRID_STRIDE(i, n) a[i] = base + (uint32_t)i;
 }
 
-// comp_min[r] = comp_label[r] for every rep r (comp_of[r] == r).
+// comp_min[r] = comp_label[r] for every rep r: the rep_count listed in
+// reps, or, with reps null, every r with comp_of[r] == r.
 __global__ __launch_bounds__(kThreads) void k_merge_init(const uint32_t *__restrict__ comp_of,
                                                          const uint32_t *__restrict__ comp_label,
                                                          uint32_t *__restrict__ comp_min,
-                                                         uint32_t n_local) {
+                                                         uint32_t n_local,
+                                                         const uint32_t *__restrict__ reps,
+                                                         uint32_t rep_count) {
+  if (reps) {
+    GRID_STRIDE(k, rep_count) {
+      const uint32_t r = reps[k];
+      comp_min[r] = comp_label[r];
+    }
+    return;
+  }
   GRID_STRIDE(i, n_local) {
     if (comp_of[i] == (uint32_t)i) comp_min[i] = comp_label[i];
   }
@@ -543,31 +553,53 @@ __device__ __forceinline__ bool label_of(uint32_t x, const uint32_t *comp_label,
   return false;
 }
 
-// Label hops and the comp_label update for every rep (src/cc_mpi.c:588-621).
+// Label hops and the comp_label update of rep r (src/cc_mpi.c:588-621);
+// true if comp_label[r] dropped.
+__device__ __forceinline__ bool merge_update_rep(uint32_t r, const uint32_t *comp_of,
+                                                 uint32_t *comp_label, uint32_t *comp_min,
+                                                 uint32_t v_start, uint32_t v_end,
+                                                 const uint32_t *ghosts, uint32_t ghost_count,
+                                                 const uint32_t *ghost_labels) {
+  uint32_t x = comp_min[r];
+  for (int t = 0; t < 4; ++t) {
+    uint32_t y;
+    if (!label_of(x, comp_label, comp_of, v_start, v_end, ghosts, ghost_count, ghost_labels, y))
+      break;
+    if (y >= x) break;
+    x = y;
+  }
+  if (x < comp_min[r]) comp_min[r] = x;
+  if (comp_min[r] < comp_label[r]) {
+    comp_label[r] = comp_min[r];
+    return true;
+  }
+  return false;
+}
+
+// The update for every rep: the rep_count listed in reps, or, with reps
+// null, every r with comp_of[r] == r.
 __global__ __launch_bounds__(kThreads) void k_merge_update(const uint32_t *__restrict__ comp_of,
                                                            uint32_t *__restrict__ comp_label,
                                                            uint32_t *__restrict__ comp_min,
-                                                           uint32_t n_local, uint32_t v_start,
+                                                           uint32_t n_local,
+                                                           const uint32_t *__restrict__ reps,
+                                                           uint32_t rep_count, uint32_t v_start,
                                                            uint32_t v_end,
                                                            const uint32_t *__restrict__ ghosts,
                                                            uint32_t ghost_count,
                                                            const uint32_t *__restrict__ ghost_labels,
                                                            uint32_t *__restrict__ changed) {
   bool any = false;
-  GRID_STRIDE(i, n_local) {
-    if (comp_of[i] != (uint32_t)i) continue;
-    uint32_t x = comp_min[i];
-    for (int t = 0; t < 4; ++t) {
-      uint32_t y;
-      if (!label_of(x, comp_label, comp_of, v_start, v_end, ghosts, ghost_count, ghost_labels, y))
-        break;
-      if (y >= x) break;
-      x = y;
+  if (reps) {
+    GRID_STRIDE(k, rep_count) {
+      any |= merge_update_rep(reps[k], comp_of, comp_label, comp_min, v_start, v_end, ghosts,
+                              ghost_count, ghost_labels);
     }
-    if (x < comp_min[i]) comp_min[i] = x;
-    if (comp_min[i] < comp_label[i]) {
-      comp_label[i] = comp_min[i];
-      any = true;
+  } else {
+    GRID_STRIDE(i, n_local) {
+      if (comp_of[i] != (uint32_t)i) continue;
+      any |= merge_update_rep((uint32_t)i, comp_of, comp_label, comp_min, v_start, v_end, ghosts,
+                              ghost_count, ghost_labels);
     }
   }
   if (__ballot(any) != 0 && (threadIdx.x & (kWave - 1)) == 0) atomicOr(changed, 1u);
@@ -813,12 +845,13 @@ extern "C" int cc_hip_iota_u32_device(uint32_t *d_out, uint32_t n, uint32_t base
   return 0;
 }
 
-extern "C" int cc_hip_merge_step_device(const uint64_t *d_records, uint64_t count,
-                                        const uint32_t *d_ghost_vertices,
-                                        const uint32_t *d_ghost_labels, uint32_t ghost_count,
-                                        const uint32_t *d_comp_of, uint32_t *d_comp_label,
-                                        uint32_t *d_comp_min, uint32_t n_local, uint32_t v_start,
-                                        uint32_t v_end, int *changed, void *stream) {
+namespace {
+// One merge round; reps null: the reps are found by scanning comp_of.
+int merge_step(const uint64_t *d_records, uint64_t count, const uint32_t *d_ghost_vertices,
+               const uint32_t *d_ghost_labels, uint32_t ghost_count, const uint32_t *d_comp_of,
+               uint32_t *d_comp_label, uint32_t *d_comp_min, uint32_t n_local,
+               const uint32_t *d_reps, uint32_t rep_count, uint32_t v_start, uint32_t v_end,
+               int *changed, void *stream) {
   if (!ccx::ctx()) return 2;
   if (changed) *changed = 0;
   if (n_local == 0) return 0;
@@ -827,22 +860,50 @@ extern "C" int cc_hip_merge_step_device(const uint64_t *d_records, uint64_t coun
   if (!st) return 3;
   uint32_t *d_changed = st->d_words + 24;
   CCX_TRY(hipMemsetAsync(d_changed, 0, sizeof(uint32_t), s));
-  const unsigned g = grid_for(n_local);
+  const unsigned g = grid_for(d_reps ? rep_count : n_local);
   CCX_LAUNCH("merge_init", s,
-             k_merge_init<<<g, kThreads, 0, s>>>(d_comp_of, d_comp_label, d_comp_min, n_local));
+             k_merge_init<<<g, kThreads, 0, s>>>(d_comp_of, d_comp_label, d_comp_min, n_local,
+                                                 d_reps, rep_count));
   if (count > 0)
     CCX_LAUNCH("merge_hook", s,
                k_merge_hook<<<grid_for(count), kThreads, 0, s>>>(d_records, count, d_ghost_labels,
                                                                  d_comp_min));
   CCX_LAUNCH("merge_update", s,
              k_merge_update<<<g, kThreads, 0, s>>>(d_comp_of, d_comp_label, d_comp_min, n_local,
-                                                   v_start, v_end, d_ghost_vertices, ghost_count,
-                                                   d_ghost_labels, d_changed));
+                                                   d_reps, rep_count, v_start, v_end,
+                                                   d_ghost_vertices, ghost_count, d_ghost_labels,
+                                                   d_changed));
   uint32_t *h = (uint32_t *)st->h_stage;
   CCX_TRY(hipMemcpyAsync(h, d_changed, sizeof(uint32_t), hipMemcpyDeviceToHost, s));
   CCX_TRY(hipStreamSynchronize(s));
   if (changed) *changed = (int)h[0];
   return 0;
+}
+}  // namespace
+
+extern "C" int cc_hip_merge_step_device(const uint64_t *d_records, uint64_t count,
+                                        const uint32_t *d_ghost_vertices,
+                                        const uint32_t *d_ghost_labels, uint32_t ghost_count,
+                                        const uint32_t *d_comp_of, uint32_t *d_comp_label,
+                                        uint32_t *d_comp_min, uint32_t n_local, uint32_t v_start,
+                                        uint32_t v_end, int *changed, void *stream) {
+  return merge_step(d_records, count, d_ghost_vertices, d_ghost_labels, ghost_count, d_comp_of,
+                    d_comp_label, d_comp_min, n_local, nullptr, 0, v_start, v_end, changed, stream);
+}
+
+extern "C" int cc_hip_merge_step_reps_device(const uint64_t *d_records, uint64_t count,
+                                             const uint32_t *d_ghost_vertices,
+                                             const uint32_t *d_ghost_labels, uint32_t ghost_count,
+                                             const uint32_t *d_comp_of, uint32_t *d_comp_label,
+                                             uint32_t *d_comp_min, uint32_t n_local,
+                                             const uint32_t *d_reps, uint32_t rep_count,
+                                             uint32_t v_start, uint32_t v_end, int *changed,
+                                             void *stream) {
+  if (n_local > 0 && (!d_reps || rep_count == 0 || rep_count > n_local))
+    return ccx::set_error(4, "cc_hip_merge_step_reps_device: rep list missing or larger than n_local");
+  return merge_step(d_records, count, d_ghost_vertices, d_ghost_labels, ghost_count, d_comp_of,
+                    d_comp_label, d_comp_min, n_local, d_reps, rep_count, v_start, v_end, changed,
+                    stream);
 }
 
 extern "C" int cc_hip_rank_labels_device(const uint32_t *d_comp_label, const uint32_t *d_comp_of,

@@ -712,6 +712,34 @@ struct MaskRows {
   }
 };
 
+// The local reps {r : comp_of[r] == r}, ascending (the rep list of
+// src/cc_mpi.c:328-362), row-striped.
+struct RepRows {
+  using Val = uint32_t;
+  static constexpr int kRows = 32;
+  const uint32_t *comp_of;
+  uint32_t *reps;
+  template <int Rows>
+  __device__ void rows(uint64_t wbase, int lane, uint64_t n, uint32_t (&v)[Rows],
+                       uint64_t (&bal)[Rows]) const {
+    uint32_t c[Rows];
+#pragma unroll
+    for (int i = 0; i < Rows; ++i) {
+      const uint64_t j = wbase + (uint64_t)i * kWave + lane;
+      c[i] = j < n ? comp_of[j] : 0xFFFFFFFFu;
+    }
+#pragma unroll
+    for (int i = 0; i < Rows; ++i) {
+      const uint64_t j = wbase + (uint64_t)i * kWave + lane;
+      v[i] = (uint32_t)j;
+      bal[i] = __ballot(j < n && c[i] == (uint32_t)j);
+    }
+  }
+  __device__ void apply(uint64_t, bool keep, uint64_t rank, uint32_t v) const {
+    if (keep) reps[rank] = v;
+  }
+};
+
 inline unsigned tiles_for(uint64_t n) { return (unsigned)((n + kTile - 1) / kTile); }
 
 template <class Op>
@@ -1250,6 +1278,23 @@ extern "C" int cc_hip_ghost_extract_device(const uint64_t *d_records, uint64_t c
                             count, s, st, ghost_count);
   return run_compact("ghost_compact", GhostOp{d_records, d_out_records, d_ghost_vertices}, count,
                      s, st, ghost_count);
+}
+
+extern "C" int cc_hip_local_reps_device(const uint32_t *d_comp_of, uint32_t n_local,
+                                        uint32_t *d_reps, uint32_t *rep_count, void *stream) {
+  if (!ccx::ctx()) return 2;
+  if (!rep_count) return ccx::set_error(4, "cc_hip_local_reps_device: null count");
+  *rep_count = 0;
+  if (n_local == 0) return 0;
+  if (!d_comp_of || !d_reps) return ccx::set_error(4, "cc_hip_local_reps_device: null pointer");
+  hipStream_t s = ccx::pick_stream(stream);
+  ccx::StreamState *st = ccx::stream_state(s);
+  if (!st) return 3;
+  uint64_t cnt = 0;
+  if (int rc = run_compact_rows("local_reps", RepRows{d_comp_of, d_reps}, n_local, s, st, &cnt))
+    return rc;
+  *rep_count = (uint32_t)cnt;
+  return 0;
 }
 
 extern "C" int cc_hip_send_lists_device(const uint64_t *d_records, uint64_t count,

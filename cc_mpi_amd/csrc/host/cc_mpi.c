@@ -184,6 +184,11 @@ void compute_connected_components_mpi_advanced(const DistCSRGraph *Gd, uint32_t 
   uint32_t *d_comp_label = (uint32_t *)dalloc((size_t)n_local * 4, comm, "alloc comp_label");
   uint32_t *d_comp_min = (uint32_t *)dalloc((size_t)n_local * 4, comm, "alloc comp_min");
   check(cc_hip_iota_u32_device(d_comp_label, n_local, v_start, NULL), comm, "comp_label init");
+  /* the rep list (cc_mpi.c:328-362 of the reference): the merge round's
+     per-rep passes walk it instead of scanning comp_of */
+  uint32_t *d_reps = (uint32_t *)dalloc((size_t)n_local * 4, comm, "alloc reps");
+  uint32_t rep_count = 0;
+  check(cc_hip_local_reps_device(d_comp_of, n_local, d_reps, &rep_count, NULL), comm, "local reps");
 
   /* 5. merge loop */
   MPI_Barrier(comm);
@@ -195,9 +200,9 @@ void compute_connected_components_mpi_advanced(const DistCSRGraph *Gd, uint32_t 
   while (global_pending)
   {
     int changed = 0;
-    check(cc_hip_merge_step_device(d_bnd, nbd, d_ghosts, d_ghost_labels, ghost_count, d_comp_of,
-                                   d_comp_label, d_comp_min, n_local, v_start, v_end, &changed,
-                                   NULL),
+    check(cc_hip_merge_step_reps_device(d_bnd, nbd, d_ghosts, d_ghost_labels, ghost_count,
+                                        d_comp_of, d_comp_label, d_comp_min, n_local, d_reps,
+                                        rep_count, v_start, v_end, &changed, NULL),
           comm, "merge step");
     if (changed)
       pending_since_exchange = 1;
@@ -259,6 +264,7 @@ void compute_connected_components_mpi_advanced(const DistCSRGraph *Gd, uint32_t 
   cc_hip_free(d_labels);
   cc_hip_free(d_comp_label);
   cc_hip_free(d_comp_min);
+  cc_hip_free(d_reps);
   cc_hip_free(d_prev);
   exchangeplan_device_free(&xd);
   exchangeplan_free(&plan);

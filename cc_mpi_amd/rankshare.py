@@ -212,6 +212,12 @@ class RankShare:
         comp_label = DeviceBuffer(4 * max(1, nl))
         comp_min = DeviceBuffer(4 * max(1, nl))
         nat.check(lib.cc_hip_iota_u32_device(comp_label.ptr, nl, vs, None), "iota")
+        # the rep list: the merge round's per-rep passes walk it instead of
+        # scanning comp_of (src/cc_mpi.c:328-362)
+        reps = DeviceBuffer(4 * max(1, nl))
+        rep_count = ctypes.c_uint32(0)
+        nat.check(lib.cc_hip_local_reps_device(self.comp_of.ptr, nl, reps.ptr,
+                                               ctypes.byref(rep_count), None), "local reps")
         ghost_labels = DeviceBuffer(4 * max(1, ng))
         nat.check(lib.cc_hip_memcpy_d2d(ghost_labels.ptr, self.ghosts.ptr, 4 * ng, None), "copy")
         sched = self.prepare_schedule(truth_labels, seed)
@@ -231,10 +237,11 @@ class RankShare:
         changed = ctypes.c_int(0)
         while True:
             t0 = time.perf_counter()
-            nat.check(lib.cc_hip_merge_step_device(self.bnd.ptr, self.nbd, self.ghosts.ptr,
-                                                   ghost_labels.ptr, ng, self.comp_of.ptr,
-                                                   comp_label.ptr, comp_min.ptr, nl, vs, ve,
-                                                   ctypes.byref(changed), None), "merge step")
+            nat.check(lib.cc_hip_merge_step_reps_device(self.bnd.ptr, self.nbd, self.ghosts.ptr,
+                                                        ghost_labels.ptr, ng, self.comp_of.ptr,
+                                                        comp_label.ptr, comp_min.ptr, nl, reps.ptr,
+                                                        rep_count.value, vs, ve,
+                                                        ctypes.byref(changed), None), "merge step")
             t_merge += time.perf_counter() - t0
             iters += 1
             if changed.value:
@@ -283,7 +290,7 @@ class RankShare:
                                                 labels.ptr, ctypes.byref(roots), None), "labels")
         out = np.empty(nl, dtype=np.uint32)
         labels.download(out)
-        for b in (labels, comp_label, comp_min, ghost_labels, prev_sent):
+        for b in (labels, comp_label, comp_min, ghost_labels, prev_sent, reps):
             b.close()
         sendbuf.close()
         self.times["merge_steps"] = t_merge

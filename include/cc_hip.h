@@ -326,6 +326,24 @@ int cc_hip_merge_step_device(const uint64_t *d_records, uint64_t count,
                              uint32_t *d_comp_label, uint32_t *d_comp_min, uint32_t n_local,
                              uint32_t v_start, uint32_t v_end, int *changed, void *stream);
 
+/* The same round with the reps given as a list: d_reps[0..rep_count) holds
+ * every r with comp_of[r] == r (cc_hip_local_reps_device), so the per-rep
+ * passes touch only those entries instead of scanning comp_of (the merge
+ * loop of src/cc_mpi.c:541-710 walks the rep list built at :328-362).
+ * A forest over n_local > 0 vertices has at least one rep. */
+int cc_hip_merge_step_reps_device(const uint64_t *d_records, uint64_t count,
+                                  const uint32_t *d_ghost_vertices, const uint32_t *d_ghost_labels,
+                                  uint32_t ghost_count, const uint32_t *d_comp_of,
+                                  uint32_t *d_comp_label, uint32_t *d_comp_min, uint32_t n_local,
+                                  const uint32_t *d_reps, uint32_t rep_count, uint32_t v_start,
+                                  uint32_t v_end, int *changed, void *stream);
+
+/* The local reps {r < n_local : comp_of[r] == r}, ascending, into d_reps
+ * (room for n_local); *rep_count receives their number (blocking).
+ * src/cc_mpi.c:328-362. */
+int cc_hip_local_reps_device(const uint32_t *d_comp_of, uint32_t n_local, uint32_t *d_reps,
+                             uint32_t *rep_count, void *stream);
+
 /* d_labels[li] = comp_label[comp_of[li]] (if d_labels non-NULL) and *roots =
  * #{li : that label == v_start + li} (src/cc_mpi.c:730-736; blocking). */
 int cc_hip_rank_labels_device(const uint32_t *d_comp_label, const uint32_t *d_comp_of,

@@ -189,3 +189,66 @@ def test_send_lists_match_their_definition(gpu, P, grouped, rows):
     assert not z.any()
     for b in (d_rec, d_b, d_vals):
         b.close()
+
+
+@pytest.mark.parametrize("nl,roots", [(1, 1), (5000, 37), (70_000, 1)])
+def test_merge_step_over_rep_list(gpu, nl, roots):
+    """cc_hip_local_reps_device lists exactly {r : comp_of[r] == r} in
+    ascending order, and a merge round over that list leaves comp_label,
+    comp_min and the changed flag as the round that scans comp_of."""
+    import ctypes
+    from cc_mpi_amd import _native as nat
+    lib = nat.gpu_lib()
+    rng = np.random.default_rng(nl + roots)
+    vs, ve, n_global = 1000, 1000 + nl, 1000 + nl + 50_000
+    root_ids = np.sort(rng.choice(nl, roots, replace=False)).astype(np.uint32)
+    comp_of = root_ids[rng.integers(0, roots, nl)]
+    comp_of[root_ids] = root_ids
+    ghosts = np.unique(rng.integers(ve, n_global, 3000)).astype(np.uint32)
+    g_lab = rng.integers(0, n_global, ghosts.size).astype(np.uint32)
+    nb = 20_000
+    rec = np.unique((rng.integers(0, ghosts.size, nb).astype(np.uint64) << np.uint64(32))
+                    | comp_of[rng.integers(0, nl, nb)].astype(np.uint64))
+    d_comp = cm.DeviceBuffer.from_array(comp_of)
+    d_reps = cm.DeviceBuffer(4 * nl)
+    cnt = ctypes.c_uint32()
+    nat.check(lib.cc_hip_local_reps_device(d_comp.ptr, nl, d_reps.ptr, ctypes.byref(cnt), None),
+              "reps")
+    reps = np.empty(cnt.value, dtype=np.uint32)
+    d_reps.download(reps)
+    np.testing.assert_array_equal(reps, np.flatnonzero(comp_of == np.arange(nl)).astype(np.uint32))
+    d_rec = cm.DeviceBuffer.from_array(rec)
+    d_gh = cm.DeviceBuffer.from_array(ghosts)
+    outs = []
+    for use_list in (False, True):
+        d_gl = cm.DeviceBuffer.from_array(g_lab)
+        d_cl = cm.DeviceBuffer.from_array(np.arange(vs, ve, dtype=np.uint32))
+        d_cm = cm.DeviceBuffer(4 * nl)
+        changed = ctypes.c_int(-1)
+        for _ in range(2):  # two rounds: the second starts from lowered labels
+            if use_list:
+                rc = lib.cc_hip_merge_step_reps_device(d_rec.ptr, rec.size, d_gh.ptr, d_gl.ptr,
+                                                       ghosts.size, d_comp.ptr, d_cl.ptr, d_cm.ptr,
+                                                       nl, d_reps.ptr, cnt.value, vs, ve,
+                                                       ctypes.byref(changed), None)
+            else:
+                rc = lib.cc_hip_merge_step_device(d_rec.ptr, rec.size, d_gh.ptr, d_gl.ptr,
+                                                  ghosts.size, d_comp.ptr, d_cl.ptr, d_cm.ptr, nl,
+                                                  vs, ve, ctypes.byref(changed), None)
+            nat.check(rc, "merge step")
+        cl = np.empty(nl, dtype=np.uint32)
+        cmin = np.empty(nl, dtype=np.uint32)
+        d_cl.download(cl)
+        d_cm.download(cmin)
+        outs.append((cl, cmin[root_ids], changed.value))
+        for b in (d_gl, d_cl, d_cm):
+            b.close()
+    np.testing.assert_array_equal(outs[0][0], outs[1][0])
+    np.testing.assert_array_equal(outs[0][1], outs[1][1])
+    assert outs[0][2] == outs[1][2]
+    # a missing list is an error, not a silent scan
+    assert lib.cc_hip_merge_step_reps_device(d_rec.ptr, rec.size, d_gh.ptr, d_gh.ptr, ghosts.size,
+                                             d_comp.ptr, d_comp.ptr, d_comp.ptr, nl, None, 0, vs,
+                                             ve, ctypes.byref(changed), None) == 4
+    for b in (d_comp, d_reps, d_rec, d_gh):
+        b.close()
