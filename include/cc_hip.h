@@ -119,9 +119,9 @@ int cc_hip_components_tune(int sampled_neighbours, int blocks_per_cu, int hook_m
  *                             its members in round 1; 0: sample it once,
  *                             after the last round
  *   "blocks_per_cu"           sampled-hook grid cap; 0 (default) = 8 for
- *                             Rem's unions without splicing and when the
- *                             lockstep hook walks several roots per thread,
- *                             else 4
+ *                             Rem's unions unless they splice on every step
+ *                             (rem_splice 1), and when the lockstep hook
+ *                             walks several roots per thread, else 4
  *   "hook_ilp"                roots walked per thread in lockstep in the
  *                             sampled hook: 1, 2, 4, or 0 (default) = 1 when
  *                             m < 16 n, else 2 in the one-pass hook (1 from
