@@ -5,11 +5,15 @@ A step is one full local-CC solve (labels = minimum vertex id per component)
 of the workload graph, CSR already resident in HBM.  TEPS = m / t where m is
 the directed CSR entry count row_ptr[n] (the reference's m).
 
+The default workload is the full 65.6M-vertex Chung-Lu power-law graph
+(1.80e9 undirected edges after dedup, 3.60e9 CSR entries) on one GPU.
+
 Prints one JSON line (rank 0) with the metric, a `roofline` object for the
 dominant kernel (algorithmic bytes / HIP-event-timed average duration, vs the
-8 TB/s HBM peak) and a `cpu_baseline` object: the reference's pthreads label
-propagation, compiled from its own sources into oracle/_ref, timed on this
-host's cores on the same graph.
+8 TB/s HBM peak) and a `cpu_baseline` object: the oracle's C restatement of
+the reference's pthreads label propagation (src/cc_pthreads.c:14-229),
+timed on every online core of this host (the reference's default thread
+count, src/runtime_utils.c:16-24) on the same graph.
 """
 from __future__ import annotations
 
@@ -31,6 +35,9 @@ def log(msg: str) -> None:
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
 
+CHUNG_LU_N = 65601024
+CHUNG_LU_PAIRS = 1921400000
+
 WORKLOADS = {
     # R-MAT scale 22, edge factor 16, (a,b,c,d) = (0.57,0.19,0.19,0.05),
     # symmetrised, deduplicated, ids permuted from the seed.
@@ -39,8 +46,10 @@ WORKLOADS = {
     "er64k": dict(kind="er", n=65536, deg=8.0, seed=1001),
     # 8192 x 8192 lattice with 30% of edges deleted.
     "grid8k": dict(kind="grid", rows=8192, cols=8192, keep=0.7, seed=1004),
-    # Chung-Lu power law (gamma 2.1), 65,601,024 vertices, 1.8e9 pairs.
-    "chunglu65m": dict(kind="chunglu", n=65601024, pairs=1800000000, gamma=2.1, seed=1003),
+    # Chung-Lu power law (gamma 2.1), 65,601,024 vertices; 1,921,400,000
+    # sampled pairs leave 1.80e9 undirected edges (3.60e9 CSR entries) once
+    # self-loops and duplicates are dropped.
+    "chunglu65m": dict(kind="chunglu", n=CHUNG_LU_N, pairs=CHUNG_LU_PAIRS, gamma=2.1, seed=1003),
 }
 
 
@@ -64,11 +73,12 @@ def make_device_graph(name: str):
         f"Chung-Lu n={w['n']} pairs={w['pairs']} gamma {w['gamma']}, permuted, seed {w['seed']}")
 
 
-def frontier_stats(G):
+def frontier_stats(G, two_pass: bool = True):
     """Size of the frontier the hooking sweep visits: vertices not in the
     sampled giant tree after hooking each vertex with neighbours 0 and 1.
     The trees of that sampled forest are the components of the sampled edge
-    set, so they are computed here independently with scipy."""
+    set, so they are computed here independently with scipy.  two_pass: also
+    count what round 1 of the two-pass plan visits (another scipy pass)."""
     import scipy.sparse as sp
     from scipy.sparse.csgraph import connected_components
     n = G.n
@@ -98,12 +108,15 @@ def frontier_stats(G):
     active = front & (rem > 0)
     has0 = np.flatnonzero(deg > 0)
     up0 = int(np.count_nonzero(G.col_idx[G.row_ptr[has0].astype(np.int64)] > has0))
-    # forest after round 0 alone (neighbour 0 of every vertex) and its
-    # sampled giant: round 1 of the two-pass plan skips that tree
-    A0 = sp.coo_matrix((np.ones(src0.size, dtype=np.int8), (src0, dst0)), shape=(n, n)).tocsr()
-    _, comp0 = connected_components(A0, directed=False)
-    v0, c0 = np.unique(comp0[pos], return_counts=True)
-    live1 = int(np.count_nonzero((deg > 1) & (comp0 != v0[np.argmax(c0)])))
+    live1 = None
+    if two_pass:
+        # forest after round 0 alone (neighbour 0 of every vertex) and its
+        # sampled giant: round 1 of the two-pass plan skips that tree
+        A0 = sp.coo_matrix((np.ones(src0.size, dtype=np.int8), (src0, dst0)),
+                           shape=(n, n)).tocsr()
+        _, comp0 = connected_components(A0, directed=False)
+        v0, c0 = np.unique(comp0[pos], return_counts=True)
+        live1 = int(np.count_nonzero((deg > 1) & (comp0 != v0[np.argmax(c0)])))
     return dict(frontier_vertices=int(active.sum()), frontier_edges=int(rem[active].sum()),
                 giant_fraction=float((~front).mean()),
                 deg_gt0=int((deg > 0).sum()), deg_gt1=int((deg > 1).sum()), up0=up0,
@@ -170,66 +183,123 @@ def solve_plan(lib, n: int, m: int) -> dict:
     return {k: int(lib.cc_hip_components_plan_get(n, m, k.encode())) for k in keys}
 
 
-def cpu_baseline(G, budget_s: float, runs: int = 3, impl: str = "reference", chunk: int = 8192):
-    """The reference's pthreads CC on this host (oracle/_ref), or the oracle
-    restatement if the reference build is absent.  Median of up to `runs`
-    timed runs after one warm-up, bounded by budget_s; runs=1 times a single
-    cold run (for graphs where one solve already fills the budget).  chunk:
-    the dynamic chunk size (8192 as the reference's run script,
-    run_benchmark.sh:21; 2048 is its bench's default)."""
+def host_cpu_info() -> dict:
+    """What the CPU leg runs on: online cores (the reference's default
+    thread count, src/runtime_utils.c:16-24), the cores this process may use
+    (affinity, cgroup quota), SMT state and the CPU model."""
+    info = {"online": os.sysconf("SC_NPROCESSORS_ONLN"),
+            "affinity": len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity")
+            else os.cpu_count()}
+    try:
+        quota, period = open("/sys/fs/cgroup/cpu.max").read().split()
+        info["cgroup_cpus"] = None if quota == "max" else round(int(quota) / int(period), 2)
+    except (OSError, ValueError):
+        info["cgroup_cpus"] = None
+    try:
+        info["smt_active"] = open("/sys/devices/system/cpu/smt/active").read().strip() == "1"
+    except OSError:
+        info["smt_active"] = None
+    info["model"] = ""
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                info["model"] = line.split(":", 1)[1].strip()
+                break
+    except OSError:
+        pass
+    return info
+
+
+def cpu_baseline(G, budget_s: float, runs: int = 3, impl: str = "port", chunk: int = 8192):
+    """The reference's pthreads CC on this host's online cores: the oracle's
+    C restatement (impl "port", pinned against the reference compiled from
+    its sources), or that compiled reference itself (impl "reference", only
+    where oracle/_ref was built).  One warm-up run, then `runs` timed runs
+    (median), stopping early once the budget is spent; runs=1 times a single
+    cold run.  chunk: the dynamic chunk size (8192 as the reference's run
+    script, run_benchmark.sh:21; 2048 is its bench's default,
+    src/mpi_cc_benchmark.c:40)."""
     import oracle
-    aff = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else os.cpu_count()
-    threads = max(1, min(aff or 1, int(os.environ.get("OMP_NUM_THREADS", "16") or 16), 16))
+    cpu = host_cpu_info()
+    threads = max(1, int(cpu["online"]))
     ref = oracle.ref_lib() if impl == "reference" else None
+    if impl == "reference" and ref is None:
+        raise RuntimeError("--cpu-impl reference needs oracle/_ref (built only where the "
+                           "reference sources are present)")
     kind = "reference" if ref is not None else "port"
     fn = oracle.ref_cc_pthreads if ref is not None else oracle.cc_lp_pthreads
-    if ref is None:
-        oracle.lib().oracle_set_verbose(1)  # one progress line per round
     times = []
     t_start = time.perf_counter()
     warm = 0.0
     if runs > 1:
         labels = fn(G.row_ptr, G.col_idx, threads, chunk)  # warm-up
         warm = time.perf_counter() - t_start
+        log(f"cpu warm-up run: {warm:.2f} s")
     for _ in range(max(1, runs)):
         if times and time.perf_counter() - t_start + max(warm, times[-1]) > budget_s:
             break
         t0 = time.perf_counter()
         labels = fn(G.row_ptr, G.col_idx, threads, chunk)
         times.append(time.perf_counter() - t0)
+        log(f"cpu timed run {len(times)}: {times[-1]:.2f} s")
     t = float(np.median(times))
-    cpu_model = ""
-    try:
-        for line in open("/proc/cpuinfo"):
-            if line.startswith("model name"):
-                cpu_model = line.split(":", 1)[1].strip()
-                break
-    except OSError:
-        pass
     return labels, dict(value=G.m / t / 1e9, unit="GTEPS", cores=threads, kind=kind,
                         sample=(f"full workload graph, {len(times)} timed run(s)"
                                 f"{' after 1 warm-up' if runs > 1 else ' (cold)'}, "
-                                f"median {t:.3f} s, chunk {chunk}, {threads} threads on "
-                                f"{cpu_model or 'host CPU'}"),
-                        seconds=t)
+                                f"median {t:.3f} s (runs {', '.join(f'{x:.3f}' for x in times)}), "
+                                f"chunk {chunk}, {threads} threads = online cores"),
+                        host=cpu, seconds=t, undirected_gteps=G.m / 2 / t / 1e9)
+
+
+def oracle_labels(G, kind: str) -> np.ndarray:
+    """The oracle's labels for G: its union-find for lattices (label
+    propagation needs rounds on the order of the diameter there), else its
+    restatement of the reference's label propagation on every online core."""
+    import oracle
+    if kind == "grid":
+        return oracle.cc_min_label(G.row_ptr, G.col_idx)
+    return oracle.cc_lp_pthreads(G.row_ptr, G.col_idx, max(1, os.sysconf("SC_NPROCESSORS_ONLN")),
+                                 8192)
+
+
+def component_summary(labels: np.ndarray) -> dict:
+    """Component count and size histogram (size -> number of components of
+    that size, for the smallest sizes and the largest components)."""
+    sizes = np.bincount(labels, minlength=labels.size)
+    sizes = sizes[sizes > 0]
+    vals, counts = np.unique(sizes, return_counts=True)
+    small = {str(int(v)): int(c) for v, c in zip(vals[:8], counts[:8])}
+    return {"components": int(sizes.size), "largest": [int(x) for x in np.sort(sizes)[-3:][::-1]],
+            "size_histogram_smallest": small}
 
 
 def serving_leg(args) -> dict:
     """Batched serving vs one-graph-per-launch on the same Poisson stream:
     requests name graphs from a pool (Erdos-Renyi d=8, R-MAT ef 16, 2-D grid
-    with 30% deletion; n log-uniform in [1K, 4M]) resident in HBM."""
+    with 30% deletion; n log-uniform in [1K, 4M]) resident in HBM.  Runs:
+    the stream at --serving-qps in both modes (timed); both modes saturated
+    (every request arrives within a few ms); an untimed verification pass of
+    the same stream in both modes checking every request's labels against
+    the oracle's for its graph; and a memory-pressure run whose store budget
+    holds only part of the pool, so graphs are evicted and rebuilt and
+    requests wait for HBM (labels checked there too)."""
     from cc_mpi_amd import serving
+    import oracle
     t0 = time.perf_counter()
-    keys, loader = serving.make_pool(per_type=args.serving_pool, seed=2024)
-    store = serving.GraphStore(budget_bytes=64 << 30, loader=loader)
-    entries, nmax = {}, 0
+    keys, loader, estimate = serving.make_pool(per_type=args.serving_pool, seed=2024)
+    store = serving.GraphStore(budget_bytes=64 << 30, loader=loader, estimate=estimate)
+    entries, want, nmax, kinds = {}, {}, 0, {}
     for k in keys:  # admit every pool graph once (built in HBM)
         g = store.acquire(k)
         entries[k] = g.m
         nmax = max(nmax, g.n)
+        h = g.to_host()
+        want[k] = oracle_labels(h, "grid" if k[0] == "grid" else "other")
+        kinds[k] = (g.n, g.m)
         store.release(k)
     log(f"serving pool: {len(keys)} graphs, {sum(entries.values()) / 1e9:.2f}e9 entries, "
-        f"built in {time.perf_counter() - t0:.1f}s")
+        f"{store.used / 2**30:.2f} GiB resident, built and oracle-labelled in "
+        f"{time.perf_counter() - t0:.1f}s")
     lanes, max_vertices = 2, max(1 << 24, nmax)
     srv = serving.Server(store, lanes=lanes, max_vertices=max_vertices, max_items=20000,
                          max_graphs=2048)
@@ -237,26 +307,91 @@ def serving_leg(args) -> dict:
     tune = serving.autotune_item_vertices(srv.engines[0], tune_set)
     srv.item_vertices = tune["item_vertices"]
     log(f"autotuned work-item size: {tune}")
-    reqs = serving.poisson_requests(args.serving_requests, args.serving_qps, keys, seed=7)
-    srv.run(reqs[: min(200, len(reqs))])  # warm-up on a prefix of the stream
-    reqs = serving.poisson_requests(args.serving_requests, args.serving_qps, keys, seed=7)
+
+    def stream(count, qps, seed=7):
+        return serving.poisson_requests(count, qps, keys, seed=seed)
+
+    bad = {"checked": 0, "wrong": 0}
+
+    def check(req, labels):
+        bad["checked"] += 1
+        if not np.array_equal(labels, want[req.key]):
+            bad["wrong"] += 1
+
+    srv.run(stream(min(200, args.serving_requests), args.serving_qps))  # warm-up
+    reqs = stream(args.serving_requests, args.serving_qps)
     stats = srv.run(reqs)
     batched = serving.latency_summary(reqs, entries)
-    batched["batches"] = stats["batches"]
-    batched["direct_batches"] = stats["direct_batches"]
+    batched.update(batches=stats["batches"], direct_batches=stats["direct_batches"])
     log(f"batched: {batched}")
-    srv.close()
-    base = serving.poisson_requests(args.serving_requests, args.serving_qps, keys, seed=7)
-    serving.serve_one_by_one(store, base[: min(200, len(base))])
-    base = serving.poisson_requests(args.serving_requests, args.serving_qps, keys, seed=7)
+    serving.serve_one_by_one(store, stream(min(200, args.serving_requests), args.serving_qps))
+    base = stream(args.serving_requests, args.serving_qps)
     serving.serve_one_by_one(store, base)
     single = serving.latency_summary(base, entries)
     log(f"one-graph-per-launch: {single}")
+
+    # saturation: the whole stream arrives within a few milliseconds
+    sat_qps = 1e7
+    reqs = stream(args.serving_requests, sat_qps, seed=8)
+    srv.run(reqs)
+    sat_b = serving.latency_summary(reqs, entries)
+    base = stream(args.serving_requests, sat_qps, seed=8)
+    serving.serve_one_by_one(store, base)
+    sat_s = serving.latency_summary(base, entries)
+    log(f"saturated: batched {sat_b['throughput_rps']} req/s, one-per-launch "
+        f"{sat_s['throughput_rps']} req/s")
+
+    # verification pass (not timed): every request of the stream, both modes
+    srv.run(stream(args.serving_requests, args.serving_qps), on_result=check)
+    serving.serve_one_by_one(store, stream(args.serving_requests, args.serving_qps),
+                             on_result=check)
+    verified = dict(bad)
+    log(f"verification: {verified}")
+
+    # memory pressure: a wider pool of moderate graphs (64 per type, n
+    # log-uniform in [1K, 512K]) under a budget of a third of its resident
+    # bytes (still above the largest graph's build peak, so every request
+    # can be admitted once others are evicted)
+    bad.update(checked=0, wrong=0)
+    pkeys, ploader, pestimate = serving.make_pool(per_type=64, seed=99, n_max=1 << 19)
+    pbig = serving.GraphStore(budget_bytes=64 << 30, loader=ploader, estimate=pestimate)
+    pwant, pentries = {}, {}
+    for k in pkeys:
+        g = pbig.acquire(k)
+        pentries[k] = g.m
+        pwant[k] = oracle_labels(g.to_host(), "grid" if k[0] == "grid" else "other")
+        pbig.release(k)
+    ppool_bytes = pbig.used
+    pbig.close()
+    want.update(pwant)
+    pbudget = max(ppool_bytes // 3, max(pestimate(k)[1] for k in pkeys))
+    pstore = serving.GraphStore(budget_bytes=pbudget, loader=ploader, estimate=pestimate)
+    srv.store = pstore
+    preqs = serving.poisson_requests(args.serving_pressure_requests, args.serving_pressure_qps,
+                                     pkeys, seed=9)
+    pst = srv.run(preqs, on_result=check)
+    pressure = serving.latency_summary(preqs, pentries)
+    pressure.update(pstore.stats(), admission_waits=pst["admission_waits"],
+                    pool=f"{len(pkeys)} graphs, n log-uniform 1K-512K",
+                    pool_resident_bytes=ppool_bytes, labels_checked=bad["checked"],
+                    labels_wrong=bad["wrong"])
+    log(f"memory pressure: {pressure}")
+    srv.store = store
+    pstore.close()
+    srv.close()
     store.close()
+    if verified["wrong"] or pressure["labels_wrong"]:
+        raise AssertionError(f"serving labels differ from the oracle: {verified}, {pressure}")
     return {"qps": args.serving_qps, "requests": args.serving_requests,
             "pool": f"{len(keys)} graphs: ER d8 / R-MAT ef16 / grid 30% deleted, n log-uniform 1K-4M",
             "lanes": lanes, "item_vertices": tune["item_vertices"], "autotune": tune,
-            "batched": batched, "one_graph_per_launch": single}
+            "batched": batched, "one_graph_per_launch": single,
+            "saturated": {"offered_qps": sat_qps, "batched": sat_b, "one_graph_per_launch": sat_s},
+            "labels_verified": {"requests_checked": verified["checked"],
+                                "wrong": verified["wrong"],
+                                "note": "untimed second pass of the same stream in both modes, "
+                                        "each request's labels against the oracle's"},
+            "memory_pressure": pressure}
 
 
 def rankshare_leg(args) -> dict:
@@ -272,7 +407,8 @@ def rankshare_leg(args) -> dict:
     from cc_mpi_amd.rankshare import RankShare
     lib = nat.gpu_lib()
     t0 = time.perf_counter()
-    full = devgen.chung_lu(65601024, 1800000000, 2.1, seed=1003)
+    w = WORKLOADS["chunglu65m"]
+    full = devgen.chung_lu(w["n"], w["pairs"], w["gamma"], seed=w["seed"])
     full.solve_async()
     nat.check(lib.cc_hip_device_sync(), "sync")
     log(f"rankshare: full graph n={full.n} m={full.m} built+solved in "
@@ -325,8 +461,8 @@ def rankshare_leg(args) -> dict:
         lib.cc_hip_prof_query(nm.encode(), ctypes.byref(c), ctypes.byref(t))
         kern[nm] = {"launches": int(c.value), "avg_us": round(1e3 * t.value / max(1, c.value), 2)}
     lib.cc_hip_prof_reset()
-    out = {"graph": "Chung-Lu n=65601024 pairs=1.8e9 gamma 2.1 seed 1003, rank 0 of 4 "
-                    "(vertex blocks)",
+    out = {"graph": f"Chung-Lu n={full.n} pairs={w['pairs']} gamma {w['gamma']} seed "
+                    f"{w['seed']} (m={full.m}), rank 0 of 4 (vertex blocks)",
            "stats": rs.stats, "preprocess_seconds": {k: round(v, 4) for k, v in rs.times.items()
                                                       if k not in ("merge_steps", "delta_packs",
                                                                    "merge_loop")},
@@ -349,14 +485,14 @@ def main() -> int:
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=50)
     ap.add_argument("--warmup", type=int, default=5)
-    ap.add_argument("--workload", default="rmat22", choices=sorted(WORKLOADS))
-    ap.add_argument("--cpu-impl", choices=["reference", "port"], default="reference",
-                    help="time the reference compiled from its sources, or the oracle's "
-                         "restatement (prints a line per round)")
+    ap.add_argument("--workload", default="chunglu65m", choices=sorted(WORKLOADS))
+    ap.add_argument("--cpu-impl", choices=["reference", "port"], default="port",
+                    help="time the oracle's C restatement of the reference (default), or the "
+                         "reference compiled from its sources (only where oracle/_ref exists)")
     ap.add_argument("--cpu-runs", type=int, default=3,
                     help="timed CPU runs (after a warm-up when > 1)")
-    ap.add_argument("--cpu-budget", type=float, default=30.0,
-                    help="seconds of CPU work for the cpu_baseline leg")
+    ap.add_argument("--cpu-budget", type=float, default=45.0,
+                    help="seconds of CPU work for the cpu_baseline leg (warm-up included)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-grid-side", type=int, default=2048,
                     help="lattice side of the CPU leg's sample for the grid workload")
@@ -364,6 +500,9 @@ def main() -> int:
                     help="requests in the batched-serving leg (0 = skip; default: the stream of "
                          "10,000 requests)")
     ap.add_argument("--serving-qps", type=float, default=8000.0)
+    ap.add_argument("--serving-pressure-requests", type=int, default=2000,
+                    help="requests in the memory-pressure serving run")
+    ap.add_argument("--serving-pressure-qps", type=float, default=2000.0)
     ap.add_argument("--rankshare", action="store_true",
                     help="also run one rank's share of the 65.6M-vertex graph (4-rank blocks)")
     ap.add_argument("--serving-pool", type=int, default=16,
@@ -444,11 +583,12 @@ def main() -> int:
     last_pass_ran = bool(ran.value)
     D.close()
 
-    st = frontier_stats(G)
-    st["components"] = int(np.count_nonzero(labels == np.arange(G.n, dtype=np.uint32)))
-    log(f"frontier: {st}")
     plan = solve_plan(lib, G.n, G.m)
     log(f"solve plan: {plan}")
+    t0 = time.perf_counter()
+    st = frontier_stats(G, two_pass=not plan["fused"])
+    st["components"] = int(np.count_nonzero(labels == np.arange(G.n, dtype=np.uint32)))
+    log(f"frontier: {st} ({time.perf_counter() - t0:.1f}s)")
     roofline = None
     if dominant:
         b = kernel_bytes(dominant, G.n, st, plan)
@@ -497,7 +637,7 @@ def main() -> int:
         cpu["sample"] = (f"{side}x{side} lattice from the same generator (keep {w['keep']}, "
                          f"seed {w['seed']}), n={sample.n} m={sample.m}: " + cpu["sample"])
         cpu.pop("seconds")
-        parity = bool(np.array_equal(oracle.cc_min_label(G.row_ptr, G.col_idx), labels))
+        parity = bool(np.array_equal(oracle_labels(G, w["kind"]), labels))
     elif not args.no_cpu_baseline:
         log("timing the CPU baseline ...")
         ref_labels, cpu = cpu_baseline(G, args.cpu_budget, args.cpu_runs, args.cpu_impl)
@@ -513,8 +653,9 @@ def main() -> int:
             cpu["chunk_2048"] = dict(value=alt["value"], sample=alt["sample"])
         cpu.pop("seconds")
     else:
-        import oracle
-        parity = bool(np.array_equal(oracle.cc_min_label(G.row_ptr, G.col_idx), labels))
+        parity = bool(np.array_equal(oracle_labels(G, w["kind"]), labels))
+    comp = component_summary(labels)
+    log(f"components: {comp}")
 
     out = {
         "metric": "CC GTEPS on 1 MI355X vs host CPU",
@@ -539,6 +680,7 @@ def main() -> int:
         "frontier": st,
         "solve_plan": plan,
         "parity_bit_exact": parity,
+        "components": comp,
     }
     if args.serving_requests > 0:
         out["serving"] = serving_leg(args)

@@ -16,10 +16,6 @@ void clear_error();
 struct Context {
   int device = -1;
   hipStream_t stream = nullptr;  // library stream, default priority
-  // Grow-only device workspace reused across calls (never shrinks).
-  void *ws = nullptr;
-  size_t ws_bytes = 0;
-  hipStream_t ws_stream = nullptr;  // stream of the workspace's last user
   // Small device scratch: counters, sampled giant root, flags.
   uint32_t *d_small = nullptr;
   // Host-pinned mirror of a few counters read back after a launch.
@@ -82,11 +78,34 @@ extern int g_dedup_items;  // items per thread of the dedup compaction (16, 32)
 // Returns the context, initialising device 0 on first use.  nullptr on
 // failure (error recorded).
 Context *ctx();
-// Grow the workspace to at least `bytes` for use on stream `s`; if its last
-// user was another stream, waits for that stream first.  Returns nullptr on
-// failure.
-void *workspace(size_t bytes, hipStream_t s);
 hipStream_t pick_stream(void *stream);
+
+// A host slot: a stream of its own plus a grow-only device workspace, held
+// by one call of a host-buffer entry point (the reference-prototype
+// drop-ins) at a time.  Concurrent callers get different slots, so they
+// share no device memory and no stream state; slots are created on demand
+// and kept until the process ends.
+struct HostSlot {
+  hipStream_t stream = nullptr;
+  void *ws = nullptr;
+  size_t ws_bytes = 0;
+  bool busy = false;
+};
+// A free slot (created if every slot is busy); nullptr on failure.
+HostSlot *acquire_host_slot();
+void release_host_slot(HostSlot *slot);
+// Grow the slot's workspace to at least `bytes` (waits for the slot's
+// stream before freeing the old buffer); nullptr on failure.
+void *slot_workspace(HostSlot *slot, size_t bytes);
+struct SlotGuard {
+  HostSlot *slot;
+  SlotGuard() : slot(acquire_host_slot()) {}
+  ~SlotGuard() {
+    if (slot) release_host_slot(slot);
+  }
+  SlotGuard(const SlotGuard &) = delete;
+  SlotGuard &operator=(const SlotGuard &) = delete;
+};
 
 // Kernel timing: call before/after a launch; no-ops unless enabled.
 void prof_begin(const char *name, hipStream_t s);

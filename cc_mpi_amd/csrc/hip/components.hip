@@ -1006,8 +1006,12 @@ extern "C" int cc_hip_components_host(const uint64_t *row_ptr, const uint32_t *c
   const size_t lab_bytes = (size_t)n * sizeof(uint32_t);
   const size_t off_col = (rp_bytes + 255) & ~(size_t)255;
   const size_t off_lab = (off_col + col_bytes + 255) & ~(size_t)255;
-  hipStream_t s = c->stream;
-  char *ws = (char *)ccx::workspace(off_lab + lab_bytes, s);
+  // a host slot of this call's own (stream and workspace): concurrent
+  // callers share no device memory
+  ccx::SlotGuard g;
+  if (!g.slot) return 3;
+  hipStream_t s = g.slot->stream;
+  char *ws = (char *)ccx::slot_workspace(g.slot, off_lab + lab_bytes);
   if (!ws) return 3;
   uint64_t *d_rp = (uint64_t *)ws;
   uint32_t *d_col = (uint32_t *)(ws + off_col);

@@ -75,35 +75,53 @@ Context *ctx() {
   return &g_ctx;
 }
 
-void *workspace(size_t bytes, hipStream_t s) {
-  Context *c = ctx();
-  if (!c) return nullptr;
-  if (c->ws_stream && c->ws_stream != s) {
-    if (hipStreamSynchronize(c->ws_stream) != hipSuccess) {
-      set_error(1, "workspace: synchronize of previous stream failed");
-      return nullptr;
+static std::mutex g_slot_mu;
+static std::vector<HostSlot *> g_slots;
+
+HostSlot *acquire_host_slot() {
+  if (!ctx()) return nullptr;
+  std::lock_guard<std::mutex> lk(g_slot_mu);
+  for (HostSlot *s : g_slots)
+    if (!s->busy) {
+      s->busy = true;
+      return s;
     }
-  }
-  c->ws_stream = s;
-  if (bytes <= c->ws_bytes) return c->ws;
-  if (c->ws) {
-    // Pending work may still use the old buffer.
-    if (hipDeviceSynchronize() != hipSuccess) {
-      set_error(1, "workspace: synchronize failed");
-      return nullptr;
-    }
-    (void)hipFree(c->ws);
-    c->ws = nullptr;
-    c->ws_bytes = 0;
-  }
-  size_t want = bytes + bytes / 8 + 4096;
-  if (hipMalloc(&c->ws, want) != hipSuccess) {
-    c->ws = nullptr;
-    set_error(3, "workspace: hipMalloc of %zu bytes failed", want);
+  HostSlot *s = new HostSlot();
+  if (hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking) != hipSuccess) {
+    delete s;
+    set_error(1, "host slot: stream creation failed");
     return nullptr;
   }
-  c->ws_bytes = want;
-  return c->ws;
+  s->busy = true;
+  g_slots.push_back(s);
+  return s;
+}
+
+void release_host_slot(HostSlot *slot) {
+  std::lock_guard<std::mutex> lk(g_slot_mu);
+  slot->busy = false;
+}
+
+void *slot_workspace(HostSlot *slot, size_t bytes) {
+  if (bytes <= slot->ws_bytes) return slot->ws;
+  if (slot->ws) {
+    // work still queued on the slot's stream may use the old buffer
+    if (hipStreamSynchronize(slot->stream) != hipSuccess) {
+      set_error(1, "slot workspace: synchronize failed");
+      return nullptr;
+    }
+    (void)hipFree(slot->ws);
+    slot->ws = nullptr;
+    slot->ws_bytes = 0;
+  }
+  const size_t want = bytes + bytes / 8 + 4096;
+  if (hipMalloc(&slot->ws, want) != hipSuccess) {
+    slot->ws = nullptr;
+    set_error(3, "slot workspace: hipMalloc of %zu bytes failed", want);
+    return nullptr;
+  }
+  slot->ws_bytes = want;
+  return slot->ws;
 }
 
 static std::mutex g_ss_mu;
@@ -137,7 +155,6 @@ void release_stream_state(hipStream_t s) {
     }
   }
   (void)hipStreamSynchronize(s);
-  if (g_ctx.ws_stream == s) g_ctx.ws_stream = nullptr;
   if (!st) return;
   (void)hipFree(st->d_words);
   (void)hipFree(st->d_status);

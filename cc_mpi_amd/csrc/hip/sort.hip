@@ -1536,12 +1536,15 @@ extern "C" int cc_hip_boundary_build_device(uint64_t *d_records, uint64_t *d_alt
 
 // ---- host drop-ins (include/boundary_sort.h) --------------------------------
 
+// Both run on a host slot of their own (stream and workspace), so calls
+// from several host threads at once share no device memory.
 static int host_sort(BoundaryEdge *edges, uint64_t count) {
-  ccx::Context *c = ccx::ctx();
-  if (!c) return 2;
-  hipStream_t s = c->stream;
+  if (!ccx::ctx()) return 2;
+  ccx::SlotGuard g;
+  if (!g.slot) return 3;
+  hipStream_t s = g.slot->stream;
   const size_t bytes = count * sizeof(uint64_t);
-  char *ws = (char *)ccx::workspace(2 * bytes + 256, s);
+  char *ws = (char *)ccx::slot_workspace(g.slot, 2 * bytes + 256);
   if (!ws) return 3;
   uint64_t *a = (uint64_t *)ws;
   uint64_t *b = (uint64_t *)(ws + ((bytes + 255) & ~(size_t)255));
@@ -1554,11 +1557,12 @@ static int host_sort(BoundaryEdge *edges, uint64_t count) {
 }
 
 static int host_dedup(BoundaryEdge *edges, uint64_t count, uint64_t *out) {
-  ccx::Context *c = ccx::ctx();
-  if (!c) return 2;
-  hipStream_t s = c->stream;
+  if (!ccx::ctx()) return 2;
+  ccx::SlotGuard g;
+  if (!g.slot) return 3;
+  hipStream_t s = g.slot->stream;
   const size_t bytes = count * sizeof(uint64_t);
-  char *ws = (char *)ccx::workspace(2 * bytes + 256, s);
+  char *ws = (char *)ccx::slot_workspace(g.slot, 2 * bytes + 256);
   if (!ws) return 3;
   uint64_t *a = (uint64_t *)ws;
   uint64_t *b = (uint64_t *)(ws + ((bytes + 255) & ~(size_t)255));

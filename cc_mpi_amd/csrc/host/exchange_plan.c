@@ -12,6 +12,7 @@
 #include <string.h>
 
 #include <mpi.h>
+#include <pthread.h>
 
 #include "../../../include/cc_hip.h"
 #include "../../../include/exchange_plan.h"
@@ -46,10 +47,13 @@ static void hip_or_die(int rc, MPI_Comm comm, const char *what)
   }
 }
 
+static void delta_cache_drop(const ExchangePlan *P);
+
 void exchangeplan_free(ExchangePlan *P)
 {
   if (!P)
     return;
+  delta_cache_drop(P);
   free(P->sources);
   free(P->dests);
   free(P->need_from_counts);
@@ -247,6 +251,101 @@ uint64_t exchangeplan_exchange_delta_device(const ExchangePlan *P, ExchangeDevic
 
 /* ---- host-array drop-in (include/exchange_plan.h:75-83) ---------------- */
 
+/*
+ * Device state of the host-array exchange, kept per plan so that only the
+ * first call on a plan allocates: the plan's index lists on the device and
+ * the pinned MPI buffers (ExchangeDevice), plus device copies of the
+ * caller's comp_label, comp_of and prev_sent, which each call uploads.  An
+ * entry is dropped by exchangeplan_free (which exchangeplan_build also
+ * calls first), and replaced if a plan at the same address no longer
+ * matches it.  The list is guarded by a mutex; calls on one plan from
+ * several threads at once are not supported (as in the reference).
+ */
+typedef struct DeltaCache
+{
+  const ExchangePlan *plan;
+  const uint32_t *send_to_flat; /* identity of the plan it was built for */
+  MPI_Comm comm_graph;
+  int total_send_to;
+  ExchangeDevice D;
+  uint32_t *d_cl, *d_co, *d_prev;
+  size_t nl_cap; /* entries of d_cl and d_co */
+  struct DeltaCache *next;
+} DeltaCache;
+
+static DeltaCache *g_delta_cache;
+static pthread_mutex_t g_delta_mu = PTHREAD_MUTEX_INITIALIZER;
+
+static void delta_cache_release(DeltaCache *c)
+{
+  cc_hip_free(c->d_cl);
+  cc_hip_free(c->d_co);
+  cc_hip_free(c->d_prev);
+  exchangeplan_device_free(&c->D);
+  free(c);
+}
+
+static void delta_cache_drop(const ExchangePlan *P)
+{
+  pthread_mutex_lock(&g_delta_mu);
+  for (DeltaCache **pp = &g_delta_cache; *pp;)
+  {
+    DeltaCache *c = *pp;
+    if (c->plan == P)
+    {
+      *pp = c->next;
+      delta_cache_release(c);
+    }
+    else
+      pp = &c->next;
+  }
+  pthread_mutex_unlock(&g_delta_mu);
+}
+
+/* The cache entry of P with d_cl/d_co holding at least n_local entries. */
+static DeltaCache *delta_cache_get(const ExchangePlan *P, size_t n_local, MPI_Comm comm)
+{
+  pthread_mutex_lock(&g_delta_mu);
+  DeltaCache *c = g_delta_cache;
+  while (c && c->plan != P)
+    c = c->next;
+  pthread_mutex_unlock(&g_delta_mu);
+  if (c && (c->send_to_flat != P->send_to_flat || c->comm_graph != P->comm_graph ||
+            c->total_send_to != P->total_send_to))
+  {
+    delta_cache_drop(P);
+    c = NULL;
+  }
+  if (!c)
+  {
+    c = (DeltaCache *)calloc(1, sizeof(DeltaCache));
+    if (!c)
+      die(comm, "exchangeplan_exchange_delta: OOM cache");
+    c->plan = P;
+    c->send_to_flat = P->send_to_flat;
+    c->comm_graph = P->comm_graph;
+    c->total_send_to = P->total_send_to;
+    hip_or_die(exchangeplan_device_init(P, &c->D), comm, "exchange delta: device init");
+    hip_or_die(cc_hip_malloc((void **)&c->d_prev, (size_t)P->total_send_to * sizeof(uint32_t)),
+               comm, "exchange delta: device alloc");
+    pthread_mutex_lock(&g_delta_mu);
+    c->next = g_delta_cache;
+    g_delta_cache = c;
+    pthread_mutex_unlock(&g_delta_mu);
+  }
+  if (c->nl_cap < n_local)
+  {
+    cc_hip_free(c->d_cl);
+    cc_hip_free(c->d_co);
+    c->d_cl = c->d_co = NULL;
+    hip_or_die(cc_hip_malloc((void **)&c->d_cl, n_local * sizeof(uint32_t)) ||
+                   cc_hip_malloc((void **)&c->d_co, n_local * sizeof(uint32_t)),
+               comm, "exchange delta: device alloc");
+    c->nl_cap = n_local;
+  }
+  return c;
+}
+
 void exchangeplan_exchange_delta(const ExchangePlan *P, const uint32_t *comp_label,
                                  const uint32_t *comp_of, uint32_t v_start, uint32_t v_end,
                                  uint32_t *prev_sent, uint32_t *ghost_labels, MPI_Comm comm,
@@ -268,12 +367,10 @@ void exchangeplan_exchange_delta(const ExchangePlan *P, const uint32_t *comp_lab
     *recvbuf_io = (uint64_t *)xmalloc(sizeof(uint64_t), comm, "OOM: delta recvbuf init");
     *recvcap_io = 1;
   }
-  int *sendcounts = (int *)calloc((size_t)(outd ? outd : 1), sizeof(int));
-  int *sdispls = (int *)calloc((size_t)(outd ? outd : 1), sizeof(int));
-  int *recvcounts = (int *)calloc((size_t)(ind ? ind : 1), sizeof(int));
-  int *rdispls = (int *)calloc((size_t)(ind ? ind : 1), sizeof(int));
-  if (!sendcounts || !sdispls || !recvcounts || !rdispls)
-    die(comm, "exchangeplan_exchange_delta: OOM counts");
+  DeltaCache *c = delta_cache_get(P, n_local, comm);
+  ExchangeDevice *D = &c->D;
+  for (int k = 0; k < outd; ++k)
+    D->sendcounts[k] = D->sdispls[k] = 0;
   if (outd > 0)
   {
     if (*sendcap_io < P->total_send_to)
@@ -284,37 +381,28 @@ void exchangeplan_exchange_delta(const ExchangePlan *P, const uint32_t *comp_lab
       *sendbuf_io = nb;
       *sendcap_io = P->total_send_to;
     }
-    /* Stage the plan and the labels on the device, pack there. */
-    ExchangeDevice D;
-    hip_or_die(exchangeplan_device_init(P, &D), comm, "exchange delta: device init");
-    uint32_t *d_cl = NULL, *d_co = NULL, *d_prev = NULL;
+    /* Stage the caller's labels on the device and pack there. */
     const size_t nl = (size_t)n_local * sizeof(uint32_t);
     const size_t ps = (size_t)P->total_send_to * sizeof(uint32_t);
-    hip_or_die(cc_hip_malloc((void **)&d_cl, nl) || cc_hip_malloc((void **)&d_co, nl) ||
-                   cc_hip_malloc((void **)&d_prev, ps),
-               comm, "exchange delta: device alloc");
-    hip_or_die(cc_hip_memcpy_h2d(d_cl, comp_label, nl, NULL) ||
-                   cc_hip_memcpy_h2d(d_co, comp_of, nl, NULL) ||
-                   cc_hip_memcpy_h2d(d_prev, prev_sent, ps, NULL),
+    hip_or_die(cc_hip_memcpy_h2d(c->d_cl, comp_label, nl, NULL) ||
+                   cc_hip_memcpy_h2d(c->d_co, comp_of, nl, NULL) ||
+                   cc_hip_memcpy_h2d(c->d_prev, prev_sent, ps, NULL),
                comm, "exchange delta: upload");
     uint64_t sent = 0;
-    hip_or_die(cc_hip_delta_pack_device(D.d_send_to_flat, (uint64_t)P->total_send_to, D.seg_start,
-                                        outd, d_cl, d_co, v_start, v_end, d_prev, D.h_sendbuf,
-                                        sendcounts, sdispls, &sent, NULL),
+    hip_or_die(cc_hip_delta_pack_device(D->d_send_to_flat, (uint64_t)P->total_send_to,
+                                        D->seg_start, outd, c->d_cl, c->d_co, v_start, v_end,
+                                        c->d_prev, D->h_sendbuf, D->sendcounts, D->sdispls, &sent,
+                                        NULL),
                comm, "exchange delta pack");
-    memcpy(*sendbuf_io, D.h_sendbuf, (size_t)sent * sizeof(uint64_t));
-    hip_or_die(cc_hip_memcpy_d2h(prev_sent, d_prev, ps, NULL), comm, "exchange delta: download");
-    cc_hip_free(d_cl);
-    cc_hip_free(d_co);
-    cc_hip_free(d_prev);
-    exchangeplan_device_free(&D);
+    memcpy(*sendbuf_io, D->h_sendbuf, (size_t)sent * sizeof(uint64_t));
+    hip_or_die(cc_hip_memcpy_d2h(prev_sent, c->d_prev, ps, NULL), comm, "exchange delta: download");
   }
-  MPI_Neighbor_alltoall(sendcounts, 1, MPI_INT, recvcounts, 1, MPI_INT, P->comm_graph);
+  MPI_Neighbor_alltoall(D->sendcounts, 1, MPI_INT, D->recvcounts, 1, MPI_INT, P->comm_graph);
   int total_recv = 0;
   for (int k = 0; k < ind; ++k)
   {
-    rdispls[k] = total_recv;
-    total_recv += recvcounts[k];
+    D->rdispls[k] = total_recv;
+    total_recv += D->recvcounts[k];
   }
   if (total_recv > 0 && *recvcap_io < total_recv)
   {
@@ -324,22 +412,18 @@ void exchangeplan_exchange_delta(const ExchangePlan *P, const uint32_t *comp_lab
     *recvbuf_io = nb;
     *recvcap_io = total_recv;
   }
-  MPI_Neighbor_alltoallv(*sendbuf_io, sendcounts, sdispls, MPI_UINT64_T, *recvbuf_io, recvcounts,
-                         rdispls, MPI_UINT64_T, P->comm_graph);
+  MPI_Neighbor_alltoallv(*sendbuf_io, D->sendcounts, D->sdispls, MPI_UINT64_T, *recvbuf_io,
+                         D->recvcounts, D->rdispls, MPI_UINT64_T, P->comm_graph);
   /* Unpack: a scatter of at most total_need_from labels (src/exchange_plan.c:390-413). */
   for (int k = 0; k < ind; ++k)
   {
-    const int base = P->need_from_displs[P->sources[k]];
-    for (int i = 0; i < recvcounts[k]; ++i)
+    const int base = D->need_base[k];
+    for (int i = 0; i < D->recvcounts[k]; ++i)
     {
-      const uint64_t pr = (*recvbuf_io)[rdispls[k] + i];
+      const uint64_t pr = (*recvbuf_io)[D->rdispls[k] + i];
       ghost_labels[P->need_from_gidx_flat[base + (int)(pr >> 32)]] = (uint32_t)pr;
     }
   }
-  free(sendcounts);
-  free(sdispls);
-  free(recvcounts);
-  free(rdispls);
 }
 
 /* ---- legacy full-label exchange (include/exchange_plan.h:90-107) --------- */

@@ -48,12 +48,30 @@ def _host():
     return lib
 
 
+def max_pairs_of(kind: str, params: list[int]) -> int:
+    """Upper bound on the undirected pairs the generator of `kind` emits."""
+    return {"er": params[1], "rmat": params[1] << params[0],
+            "grid": 2 * params[0] * params[1], "chung_lu": params[1]}[kind]
+
+
+def footprint(kind: str, n: int, params: list[int], permute: bool = True) -> tuple[int, int]:
+    """(resident, peak) device bytes of `build(kind, n, params, ...)`: what
+    the finished graph holds (row offsets, the column buffer, which is the
+    pair buffer reused, and the label array) and the most the build holds at
+    once (pair buffer, two key buffers of both directions, row offsets,
+    permutation, Chung-Lu CDF)."""
+    P = max_pairs_of(kind, params)
+    resident = 8 * (n + 1) + max(1, 8 * P) + max(4, 4 * n)
+    peak = (8 * P + 2 * 16 * P + 8 * (n + 1) + (4 * n if permute else 0)
+            + (8 * n if kind == "chung_lu" else 0))
+    return resident, max(resident, peak)
+
+
 def build(kind: str, n: int, params: list[int], seed: int, permute: bool,
           cdf: np.ndarray | None = None) -> DeviceGraph:
     """Generate pairs of `kind` on the device and build the CSR there."""
     gpu = _gpu()
-    max_pairs = {"er": params[1], "rmat": params[1] << params[0],
-                 "grid": 2 * params[0] * params[1], "chung_lu": params[1]}[kind]
+    max_pairs = max_pairs_of(kind, params)
     pairbuf = DeviceBuffer(8 * max_pairs)  # src | dst, later reused for col_idx
     d_src = pairbuf.ptr
     d_dst = pairbuf.ptr + 4 * max_pairs
@@ -88,19 +106,35 @@ def build(kind: str, n: int, params: list[int], seed: int, permute: bool,
     return DeviceGraph.adopt(n, int(m.value), row_ptr, pairbuf)
 
 
+def rmat_params(scale: int, edge_factor: int = 16, a: float = 0.57, b: float = 0.19,
+                c: float = 0.19) -> tuple[int, list[int]]:
+    ta, tab, tabc = int(a * 65536.0), int((a + b) * 65536.0), int((a + b + c) * 65536.0)
+    return 1 << scale, [scale, edge_factor, ta, tab, tabc]
+
+
+def er_params(n: int, avg_degree: float) -> tuple[int, list[int]]:
+    return n, [n, int(round(n * avg_degree / 2))]
+
+
+def grid_params(rows: int, cols: int, keep: float = 0.7) -> tuple[int, list[int]]:
+    return rows * cols, [rows, cols, int(keep * 65536.0)]
+
+
 def rmat(scale: int, edge_factor: int = 16, a: float = 0.57, b: float = 0.19, c: float = 0.19,
          seed: int = 1002, permute: bool = True) -> DeviceGraph:
-    ta, tab, tabc = int(a * 65536.0), int((a + b) * 65536.0), int((a + b + c) * 65536.0)
-    return build("rmat", 1 << scale, [scale, edge_factor, ta, tab, tabc], seed, permute)
+    n, params = rmat_params(scale, edge_factor, a, b, c)
+    return build("rmat", n, params, seed, permute)
 
 
 def erdos_renyi(n: int, avg_degree: float, seed: int) -> DeviceGraph:
-    return build("er", n, [n, int(round(n * avg_degree / 2))], seed, False)
+    n, params = er_params(n, avg_degree)
+    return build("er", n, params, seed, False)
 
 
 def grid(rows: int, cols: int, keep: float = 0.7, seed: int = 1004,
          permute: bool = True) -> DeviceGraph:
-    return build("grid", rows * cols, [rows, cols, int(keep * 65536.0)], seed, permute)
+    n, params = grid_params(rows, cols, keep)
+    return build("grid", n, params, seed, permute)
 
 
 def chung_lu(n: int, pairs: int, gamma: float = 2.1, seed: int = 1003,

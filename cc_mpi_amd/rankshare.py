@@ -250,7 +250,11 @@ class RankShare:
             if not (pending_since or others_pending) or iters >= max_iters:
                 break
             steps += 1
-            if steps >= interval or not changed.value:
+            # src/cc_mpi.c:669 exchanges when global_changed_any, the OR of
+            # every rank's changed flag, is 0; the other ranks keep changing
+            # while deltas from them remain scheduled
+            global_changed_any = bool(changed.value) or others_pending
+            if steps >= interval or not global_changed_any:
                 if self.dests:
                     before = None
                     if check_pack is not None:

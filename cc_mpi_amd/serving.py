@@ -148,21 +148,47 @@ class BatchEngine:
             self.out.close()
 
 
-def footprint_bytes(n: int, m: int) -> int:
-    """HBM a resident graph holds: row offsets and columns."""
-    return 8 * (n + 1) + 4 * m
+def resident_bytes(g: DeviceGraph) -> int:
+    """HBM a resident graph holds: its row offsets, column and label buffers."""
+    return g.row_ptr.nbytes + g.col_idx.nbytes + g.labels.nbytes
 
 
 class GraphStore:
-    """Resident graphs in HBM under a byte budget (LRU eviction of graphs
-    no in-flight request holds)."""
+    """Resident graphs in HBM under a byte budget.
 
-    def __init__(self, budget_bytes: int, loader):
+    A graph is admitted only if its cost fits: `estimate(key)` gives
+    (resident, peak) bytes from the key's spec before anything is built,
+    where peak covers the loader's transient buffers; least-recently-used
+    graphs that no in-flight request holds are evicted to make room, and a
+    key that still does not fit is refused without being built and stays
+    refused until some graph is released.  Without an estimate, the graph is
+    loaded first and priced by what it holds (the transient build is then
+    not bounded)."""
+
+    def __init__(self, budget_bytes: int, loader, estimate=None):
         self.budget = budget_bytes
         self.loader = loader  # key -> DeviceGraph (builds/uploads it)
+        self.estimate = estimate  # key -> (resident, peak) bytes, or None
         self.graphs: "collections.OrderedDict[object, DeviceGraph]" = collections.OrderedDict()
+        self.cost: dict = {}
         self.pins: collections.Counter = collections.Counter()
+        self.refused: set = set()
         self.used = 0
+        self.peak_used = 0
+        self.loads = self.evictions = self.refusals = 0
+
+    def _make_room(self, need: int) -> bool:
+        """Evict unpinned graphs (LRU first) until `need` more bytes fit;
+        evicts nothing if even evicting every unpinned graph would not do."""
+        if need > self.budget:
+            return False
+        free_able = sum(self.cost[k] for k in self.graphs if self.pins[k] == 0)
+        if self.used - free_able + need > self.budget:
+            return False
+        while self.used + need > self.budget:
+            victim = next(k for k in self.graphs if self.pins[k] == 0)
+            self._evict(victim)
+        return True
 
     def acquire(self, key) -> DeviceGraph | None:
         """The resident graph for key (loading it if there is room), or None
@@ -171,26 +197,51 @@ class GraphStore:
             self.graphs.move_to_end(key)
             self.pins[key] += 1
             return self.graphs[key]
-        g = self.loader(key)
-        need = footprint_bytes(g.n, g.m)
-        while self.used + need > self.budget:
-            victim = next((k for k in self.graphs if self.pins[k] == 0), None)
-            if victim is None:
-                g.close()
+        if key in self.refused:
+            return None
+        if self.estimate is not None:
+            resident, peak = self.estimate(key)
+            if not self._make_room(max(resident, peak)):
+                self.refused.add(key)
+                self.refusals += 1
                 return None
-            self._evict(victim)
+            g = self.loader(key)
+            cost = resident_bytes(g)
+            if cost > resident:
+                g.close()
+                raise RuntimeError(f"graph {key!r} holds {cost} bytes, above its estimate "
+                                   f"{resident}")
+        else:
+            g = self.loader(key)
+            cost = resident_bytes(g)
+            if not self._make_room(cost):
+                g.close()
+                self.refused.add(key)
+                self.refusals += 1
+                return None
+        self.loads += 1
         self.graphs[key] = g
+        self.cost[key] = cost
         self.pins[key] += 1
-        self.used += need
+        self.used += cost
+        self.peak_used = max(self.peak_used, self.used)
         return g
 
     def release(self, key) -> None:
         self.pins[key] -= 1
+        self.refused.clear()
 
     def _evict(self, key) -> None:
         g = self.graphs.pop(key)
-        self.used -= footprint_bytes(g.n, g.m)
+        self.used -= self.cost.pop(key)
+        self.evictions += 1
         g.close()
+
+    def stats(self) -> dict:
+        return {"budget_bytes": self.budget, "resident_bytes": self.used,
+                "peak_resident_bytes": self.peak_used, "loads": self.loads,
+                "evictions": self.evictions, "refusals": self.refusals,
+                "resident_graphs": len(self.graphs)}
 
     def close(self) -> None:
         for k in list(self.graphs):
@@ -247,11 +298,14 @@ class Server:
         for e in self.engines:
             e.close()
 
-    def run(self, requests: list[Request]) -> dict:
+    def run(self, requests: list[Request], on_result=None) -> dict:
+        """Serve `requests` on their arrival schedule.  on_result(req, labels)
+        is called for every request as its batch retires, with a view of its
+        labels in the lane's pinned buffer (valid only during the call)."""
         pending = collections.deque(sorted(requests, key=lambda r: r.arrival))
         queue: collections.deque = collections.deque()
         busy: dict[int, list] = {}
-        batches = direct = 0
+        batches = direct = waits = 0
         t0 = time.perf_counter()
         total = len(pending)
         finished = 0
@@ -268,6 +322,8 @@ class Server:
                         req.done = tdone
                         if self.keep_labels:
                             req.labels = eng.out.array[off:off + n].copy()
+                        if on_result is not None:
+                            on_result(req, eng.out.array[off:off + n])
                         self.store.release(req.key)
                         finished += 1
             # fill free lanes with everything that has arrived
@@ -278,7 +334,8 @@ class Server:
                 while queue and len(chosen) < eng.max_graphs:
                     req = queue[0]
                     g = self.store.acquire(req.key)
-                    if g is None:
+                    if g is None:  # waits for HBM until a lane retires
+                        waits += 1
                         break
                     it = math.ceil(g.n / self.item_vertices)
                     if chosen and (verts + g.n > eng.max_vertices or items + it > eng.max_items):
@@ -304,11 +361,14 @@ class Server:
                 if wait > 0:
                     time.sleep(min(wait, 0.002))
         wall = time.perf_counter() - t0
-        return {"batches": batches, "direct_batches": direct, "wall_s": wall}
+        return {"batches": batches, "direct_batches": direct, "admission_waits": waits,
+                "wall_s": wall}
 
 
-def serve_one_by_one(store: GraphStore, requests: list[Request], keep_labels: bool = False) -> dict:
-    """Baseline: FIFO, one solve launch sequence and one label copy per request."""
+def serve_one_by_one(store: GraphStore, requests: list[Request], keep_labels: bool = False,
+                     on_result=None) -> dict:
+    """Baseline: FIFO, one solve launch sequence and one label copy per
+    request.  on_result(req, labels) as in Server.run."""
     lib = nat.gpu_lib()
     pending = collections.deque(sorted(requests, key=lambda r: r.arrival))
     out = None
@@ -331,6 +391,8 @@ def serve_one_by_one(store: GraphStore, requests: list[Request], keep_labels: bo
         req.done = time.perf_counter() - t0
         if keep_labels:
             req.labels = out.array[:g.n].copy()
+        if on_result is not None:
+            on_result(req, out.array[:g.n])
         store.release(req.key)
     if out is not None:
         out.close()
@@ -352,7 +414,8 @@ def make_pool(per_type: int, seed: int, n_min: int = 1024, n_max: int = 4194304)
     """The request graphs: `per_type` graphs of each type (Erdos-Renyi with
     average degree 8, R-MAT edge factor 16, 2-D grid with 30% of edges
     deleted), vertex counts log-uniform in [n_min, n_max], built in HBM.
-    Returns (keys, loader) for a GraphStore."""
+    Returns (keys, loader, estimate) for a GraphStore: estimate(key) prices
+    a graph from its spec, before it is built."""
     from . import devgen
     rng = np.random.default_rng(seed)
     specs = {}
@@ -361,17 +424,33 @@ def make_pool(per_type: int, seed: int, n_min: int = 1024, n_max: int = 4194304)
             n = int(round(math.exp(rng.uniform(math.log(n_min), math.log(n_max)))))
             specs[(t, i)] = (n, seed * 1000 + len(specs))
 
-    def loader(key):
+    def shape(key):
         t, _ = key
-        n, s = specs[key]
+        n, _ = specs[key]
         if t == "er":
-            return devgen.erdos_renyi(n, 8.0, seed=s)
+            return ("er", n) + devgen.er_params(n, 8.0)
         if t == "rmat":
-            return devgen.rmat(max(10, int(round(math.log2(n)))), 16, seed=s)
+            return ("rmat", max(10, int(round(math.log2(n))))) + devgen.rmat_params(
+                max(10, int(round(math.log2(n)))), 16)
         side = max(2, int(round(math.sqrt(n))))
-        return devgen.grid(side, side, 0.7, seed=s)
+        return ("grid", side) + devgen.grid_params(side, side, 0.7)
 
-    return list(specs), loader
+    def loader(key):
+        kind, arg, _, _ = shape(key)
+        s = specs[key][1]
+        if kind == "er":
+            return devgen.erdos_renyi(arg, 8.0, seed=s)
+        if kind == "rmat":
+            return devgen.rmat(arg, 16, seed=s)
+        return devgen.grid(arg, arg, 0.7, seed=s)
+
+    def estimate(key):
+        kind, _, n, params = shape(key)
+        resident, peak = devgen.footprint(kind, n, params, permute=kind != "er")
+        # the library's per-stream scratch and allocation rounding
+        return resident + (1 << 20), peak + peak // 16 + (16 << 20)
+
+    return list(specs), loader, estimate
 
 
 def poisson_requests(count: int, qps: float, keys: list, seed: int) -> list[Request]:

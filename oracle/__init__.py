@@ -3,8 +3,11 @@
 Python bindings to ``libccoracle.so`` (this project's C restatement of the
 reference path, ``cc_oracle.c``) and, when built, ``_ref/libccref.so`` (the
 reference's own ``src/cc_pthreads.c`` and ``src/boundary_sort.c`` compiled
-from its sources by ``oracle/Makefile``).  Only tests, ``smoke()`` and the
-``cpu_baseline`` leg of ``bench.py`` may import this package.
+from its sources by ``oracle/Makefile``; ``_ref/libccref_mpi.so``, its
+``src/exchange_plan.c``, is driven by ``tests/mpi/delta_check.c``).  The
+``_ref`` builds exist only in the build container.  Only tests, ``smoke()``
+and ``bench.py`` (its CPU baseline and its parity checks) may import this
+package.
 """
 from __future__ import annotations
 
@@ -69,6 +72,9 @@ def lib():
         L.oracle_boundary_build.argtypes = [c_u32, P_u64, P_u32, c_u32, c_u32, c_u32, P_u32,
                                             P_u64]
         L.oracle_boundary_build.restype = c_u64
+        L.oracle_merge_step.argtypes = [P_u64, c_u64, P_u32, P_u32, c_u32, P_u32, P_u32, P_u32,
+                                        P_u32, c_u32, c_u32, c_u32]
+        L.oracle_merge_step.restype = c_int
         _lib = L
     return _lib
 
@@ -157,3 +163,46 @@ def delta_pack(send_to_flat, seg_start, comp_label, comp_of, v_start, v_end, pre
                                   _p(co, c_u32), v_start, v_end, _p(prev_sent, c_u32),
                                   _p(buf, c_u64), _p(cnt, c_int), _p(dis, c_int))
     return buf[:tot].copy(), cnt[:nseg].copy(), dis[:nseg].copy()
+
+
+def delta_unpack(recvbuf, recvcounts, rdispls, need_base, need_from_gidx_flat, ghost_labels):
+    """oracle_delta_unpack; ghost_labels (uint32) is updated in place."""
+    rb = np.ascontiguousarray(recvbuf, dtype=np.uint64)
+    rc = np.ascontiguousarray(recvcounts, dtype=np.int32)
+    rd = np.ascontiguousarray(rdispls, dtype=np.int32)
+    nb = np.ascontiguousarray(need_base, dtype=np.int32)
+    gx = np.ascontiguousarray(need_from_gidx_flat, dtype=np.uint32)
+    if rc.size != rd.size or rc.size != nb.size:
+        raise ValueError("recvcounts, rdispls and need_base need one entry per source")
+    for k in range(rc.size):  # bounds of every read and write, before the C loop
+        if rc[k] == 0:
+            continue
+        if rd[k] < 0 or rd[k] + rc[k] > rb.size:
+            raise ValueError(f"source {k}: pairs outside recvbuf")
+        pos = (rb[rd[k]:rd[k] + rc[k]] >> np.uint64(32)).astype(np.int64)
+        if nb[k] < 0 or nb[k] + int(pos.max()) >= gx.size:
+            raise ValueError(f"source {k}: positions outside need_from_gidx_flat")
+        if int(gx[nb[k] + pos].max()) >= ghost_labels.size:
+            raise ValueError(f"source {k}: ghost index outside ghost_labels")
+    if rb.size == 0:
+        rb = np.zeros(1, dtype=np.uint64)
+    if gx.size == 0:
+        gx = np.zeros(1, dtype=np.uint32)
+    lib().oracle_delta_unpack(_p(rb, c_u64), _p(rc, c_int), _p(rd, c_int), _p(nb, c_int),
+                              rc.size, _p(gx, c_u32), _p(ghost_labels, c_u32))
+
+
+def merge_step(records, ghost_vertices, ghost_labels, comp_of, comp_label, comp_min, rep_list,
+               v_start, v_end) -> int:
+    """oracle_merge_step; comp_label and comp_min (uint32) are updated in
+    place.  Returns 1 if any comp_label dropped."""
+    rec = np.ascontiguousarray(records, dtype=np.uint64)
+    gv = np.ascontiguousarray(ghost_vertices, dtype=np.uint32)
+    gl = np.ascontiguousarray(ghost_labels, dtype=np.uint32)
+    co = np.ascontiguousarray(comp_of, dtype=np.uint32)
+    rl = np.ascontiguousarray(rep_list, dtype=np.uint32)
+    z64, z32 = np.zeros(1, np.uint64), np.zeros(1, np.uint32)
+    return int(lib().oracle_merge_step(
+        _p(rec if rec.size else z64, c_u64), rec.size, _p(gv if gv.size else z32, c_u32),
+        _p(gl if gl.size else z32, c_u32), gv.size, _p(co, c_u32), _p(comp_label, c_u32),
+        _p(comp_min, c_u32), _p(rl, c_u32), rl.size, v_start, v_end))

@@ -383,6 +383,86 @@ void oracle_delta_unpack(const uint64_t *recvbuf, const int *recvcounts, const i
 }
 
 /* ------------------------------------------------------------------ */
+/* Merge round (src/cc_mpi.c:547-622, single-thread path)              */
+/* ------------------------------------------------------------------ */
+
+/* get_vertex_label_global (src/cc_mpi.c:78-115): the label of vertex x as
+ * this rank sees it; *missing = 1 if x is neither owned nor a ghost. */
+static uint32_t merge_label_of(uint32_t x, const uint32_t *comp_label, const uint32_t *comp_of,
+                               uint32_t v_start, uint32_t v_end, const uint32_t *ghost_vertices,
+                               uint32_t ghost_count, const uint32_t *ghost_labels, int *missing)
+{
+  if (x >= v_start && x < v_end)
+  {
+    *missing = 0;
+    return comp_label[comp_of[x - v_start]];
+  }
+  uint32_t lo = 0, hi = ghost_count;
+  while (lo < hi)
+  {
+    const uint32_t mid = lo + (hi - lo) / 2;
+    if (ghost_vertices[mid] < x)
+      lo = mid + 1;
+    else
+      hi = mid;
+  }
+  if (lo < ghost_count && ghost_vertices[lo] == x)
+  {
+    *missing = 0;
+    return ghost_labels[lo];
+  }
+  *missing = 1;
+  return x;
+}
+
+/*
+ * One merge iteration's init, hook and update phases (src/cc_mpi.c:547-622,
+ * the nthreads == 1 branches) over records (gidx << 32) | rep (BoundaryEdge
+ * {rep, remote} after ghost extraction) and the rep list: comp_min[r] =
+ * comp_label[r]; comp_min[rep] lowered by every record's ghost label; then,
+ * rep by rep in list order, up to 4 label hops from comp_min[r], and
+ * comp_label[r] lowered to the result.  Returns 1 if any comp_label dropped.
+ */
+int oracle_merge_step(const uint64_t *records, uint64_t count, const uint32_t *ghost_vertices,
+                      const uint32_t *ghost_labels, uint32_t ghost_count, const uint32_t *comp_of,
+                      uint32_t *comp_label, uint32_t *comp_min, const uint32_t *rep_list,
+                      uint32_t rep_count, uint32_t v_start, uint32_t v_end)
+{
+  for (uint32_t i = 0; i < rep_count; ++i)
+    comp_min[rep_list[i]] = comp_label[rep_list[i]];
+  for (uint64_t e = 0; e < count; ++e)
+  {
+    const uint32_t rep = (uint32_t)records[e];
+    const uint32_t lbl = ghost_labels[(uint32_t)(records[e] >> 32)];
+    if (lbl < comp_min[rep])
+      comp_min[rep] = lbl;
+  }
+  int any = 0;
+  for (uint32_t i = 0; i < rep_count; ++i)
+  {
+    const uint32_t r = rep_list[i];
+    uint32_t x = comp_min[r];
+    for (int t = 0; t < 4; ++t)
+    {
+      int missing = 0;
+      const uint32_t y = merge_label_of(x, comp_label, comp_of, v_start, v_end, ghost_vertices,
+                                        ghost_count, ghost_labels, &missing);
+      if (missing || y >= x)
+        break;
+      x = y;
+    }
+    if (x < comp_min[r])
+      comp_min[r] = x;
+    if (comp_min[r] < comp_label[r])
+    {
+      comp_label[r] = comp_min[r];
+      any = 1;
+    }
+  }
+  return any;
+}
+
+/* ------------------------------------------------------------------ */
 /* Partition math (src/graph_dist.c:22-92, src/cc_mpi.c:41-76)          */
 /* ------------------------------------------------------------------ */
 

@@ -236,6 +236,57 @@ def test_delta_pack_nothing_to_send(gpu, rows):
     assert pairs.size == 0 and cnt.sum() == 0 and (dis == 0).all()
 
 
+@pytest.mark.parametrize("seed,nsrc", [(0, 1), (1, 3), (2, 7), (3, 64)])
+def test_delta_unpack_equals_oracle(gpu, seed, nsrc):
+    """cc_hip_delta_unpack_device against oracle_delta_unpack (pinned against
+    the reference's src/exchange_plan.c:390-413): received pairs grouped per
+    source, some sources sending nothing, positions relative to each
+    source's segment of need_from_gidx_flat; over three exchanges, the pairs
+    coming from pinned host memory and from device memory."""
+    import ctypes
+    from cc_mpi_amd import _native as nat
+    lib = nat.gpu_lib()
+    rng = np.random.default_rng(seed)
+    need = rng.integers(0, 3000, nsrc)
+    need[rng.random(nsrc) < 0.25] = 0
+    tn = int(need.sum())
+    ng = tn + 20_000  # ghosts this rank holds; tn of them are needed from the sources
+    base = np.concatenate([[0], np.cumsum(need)[:-1]]).astype(np.int32)
+    gidx = rng.permutation(ng)[:tn].astype(np.uint32)  # each ghost has one owner
+    d_gidx = DeviceBuffer.from_array(gidx if tn else np.zeros(1, np.uint32))
+    ghost = rng.integers(0, 1 << 30, ng).astype(np.uint32)
+    d_ghost = DeviceBuffer.from_array(ghost)
+    want = ghost.copy()
+    for rnd in range(3):
+        parts, counts = [], []
+        for k in range(nsrc):
+            take = np.sort(rng.choice(int(need[k]), int(rng.integers(0, need[k] + 1)),
+                                      replace=False)) if need[k] else np.zeros(0, np.int64)
+            lbl = rng.integers(0, 1 << 30, take.size).astype(np.uint64)
+            parts.append((take.astype(np.uint64) << np.uint64(32)) | lbl)
+            counts.append(take.size)
+        rb = np.concatenate(parts).astype(np.uint64) if parts else np.zeros(0, np.uint64)
+        counts = np.array(counts, dtype=np.int32)
+        rd = np.concatenate([[0], np.cumsum(counts)[:-1]]).astype(np.int32)
+        oracle.delta_unpack(rb, counts, rd, base, gidx, want)
+        if rnd == 1:  # pairs in device memory
+            src = DeviceBuffer.from_array(rb if rb.size else np.zeros(1, np.uint64))
+            ptr = src.ptr
+        else:  # pairs in pinned host memory, as MPI received them
+            src = bd.PinnedBuffer(max(1, rb.size))
+            src.array[:rb.size] = rb
+            ptr = src.ptr
+        nat.check(lib.cc_hip_delta_unpack_device(ptr, rb.size, nat.ptr(rd, ctypes.c_int),
+                                                 nat.ptr(base, ctypes.c_int), nsrc, d_gidx.ptr,
+                                                 d_ghost.ptr, None), "unpack")
+        got = np.empty(ng, dtype=np.uint32)
+        d_ghost.download(got)
+        assert np.array_equal(got, want), f"exchange {rnd}"
+        src.close()
+    d_gidx.close()
+    d_ghost.close()
+
+
 @pytest.mark.parametrize("split,rows,items", [(1, 1, 32), (0, 1, 32), (1, 0, 16), (0, 0, 32)])
 @pytest.mark.parametrize("roots", [1, 2, 3, 1000, 65537, "all"])
 def test_boundary_build_matches_three_steps(gpu, roots, split, rows, items):

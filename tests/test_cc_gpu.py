@@ -257,3 +257,48 @@ def test_every_solve_path(gpu, knobs):
         for k, v in DEFAULTS.items():
             nat.check(lib.cc_hip_components_set(k.encode(), v), k)
 
+
+
+def test_drop_ins_from_four_host_threads(gpu):
+    """compute_connected_components_pthreads, boundary_edges_sort and
+    boundary_edges_dedup called from 4 host threads at once (ctypes releases
+    the interpreter lock during the calls), as the reference's stateless
+    functions may be (src/cc_pthreads.c:152-229): every result equals the
+    oracle's."""
+    import threading
+    from cc_mpi_amd import boundary as bd
+    rng = np.random.default_rng(77)
+    graphs = [cm.rmat(14, 16, seed=60), cm.grid(300, 300, 0.7, seed=61),
+              cm.erdos_renyi(200_000, 3.0, seed=62), cm.rmat(15, 8, seed=63)]
+    want = [oracle.cc_min_label(G.row_ptr, G.col_idx) for G in graphs]
+    keys = [rng.integers(0, 1 << 50, int(n), dtype=np.uint64) // np.uint64(7 ** i)
+            for i, n in enumerate((1_500_000, 700_000, 2_100_000, 300_000))]
+    want_sorted = [oracle.sort_u64(k) for k in keys]
+    want_dedup = [oracle.dedup_u64(k) for k in want_sorted]
+    errors = []
+    start = threading.Barrier(4)
+
+    def worker(i):
+        try:
+            start.wait()
+            for rep in range(5):
+                got = cm.compute_connected_components_pthreads(graphs[i])
+                if not np.array_equal(got, want[i]):
+                    errors.append(f"thread {i} rep {rep}: labels differ")
+                k = keys[i].copy()
+                bd.boundary_edges_sort(k)
+                if not np.array_equal(k, want_sorted[i]):
+                    errors.append(f"thread {i} rep {rep}: sort differs")
+                cnt = bd.boundary_edges_dedup(k)
+                if not np.array_equal(k[:cnt], want_dedup[i]):
+                    errors.append(f"thread {i} rep {rep}: dedup differs")
+        except Exception as e:  # noqa: BLE001
+            errors.append(f"thread {i}: {e!r}")
+
+    threads = [threading.Thread(target=worker, args=(i,)) for i in range(4)]
+    for t in threads:
+        t.start()
+    for t in threads:
+        t.join(timeout=120)
+    assert not any(t.is_alive() for t in threads)
+    assert not errors, errors

@@ -94,3 +94,27 @@ def test_driver_rejects_unreadable_input(tmp_path):
     r = subprocess.run(cmd, capture_output=True, text=True, timeout=120, cwd=str(tmp_path))
     assert r.returncode != 0
     assert "cannot load" in r.stderr
+
+
+import sys  # noqa: E402
+
+sys.path.insert(0, os.path.dirname(os.path.abspath(__file__)))
+import delta_harness as dh  # noqa: E402
+
+
+@pytest.fixture(scope="module")
+def delta_gpu(tmp_path_factory):
+    return dh.build_harness(str(tmp_path_factory.mktemp("dgpu") / "delta_check_gpu"), "gpu")
+
+
+@pytest.mark.parametrize("case", dh.CASES, ids=lambda c: "P%d-n%d" % c[:2])
+def test_host_array_delta_exchange(delta_gpu, case):
+    """The reference-prototype exchangeplan_exchange_delta (host arrays; pack
+    on the GPU into pinned memory, device state cached per plan after the
+    first call) with 2-4 ranks sharing the GPU, over several rounds: every
+    round equals the oracle's pack, transport and unpack (checked inside the
+    harness), and the plan and per-round digests equal the ones the
+    reference's own build produced (tests/golden/delta_exchange_digests.txt)."""
+    print("running delta_check", case, flush=True)
+    lines = dh.run_harness(delta_gpu, *case)
+    assert lines == dh.golden()[case]

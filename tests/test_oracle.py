@@ -236,3 +236,116 @@ def test_local_split_and_boundary_build():
                                  oracle._p(rec, oracle.c_u64))
     want = (ci[~loc].astype(np.uint64) << np.uint64(32)) | comp_of[src[~loc]].astype(np.uint64)
     assert np.array_equal(rec[:nb], want)
+
+
+# ---- merge round (src/cc_mpi.c:547-622) ------------------------------------
+
+def _rec(gidx, rep):
+    return (np.uint64(gidx) << np.uint64(32)) | np.uint64(rep)
+
+
+def test_merge_step_hand_cases():
+    """Worked by hand from src/cc_mpi.c:547-622.  Block [50, 60), reps 0 and
+    5; ghosts 30, 40 with labels 40, 52.  Rep 0 takes 40 from its record
+    and stops (label of 40 is 52).  Rep 5, after it, takes 52 from its
+    record, hops to label_of(52) = comp_label[comp_of[2]] = 40 (already
+    lowered this round), then stops.  In the other list order rep 5 first
+    hops to 50 only."""
+    vs, ve = 50, 60
+    comp_of = np.array([0] * 5 + [5] * 5, dtype=np.uint32)
+    gv = np.array([30, 40], dtype=np.uint32)
+    gl = np.array([40, 52], dtype=np.uint32)
+    rec = np.array([_rec(0, 0), _rec(1, 5)], dtype=np.uint64)
+    for order, want in (([0, 5], (40, 40)), ([5, 0], (40, 50))):
+        cl = np.arange(vs, ve, dtype=np.uint32)
+        cm = np.zeros(10, dtype=np.uint32)
+        assert oracle.merge_step(rec, gv, gl, comp_of, cl, cm, np.array(order), vs, ve) == 1
+        assert (int(cl[0]), int(cl[5])) == want
+        assert oracle.merge_step(rec, gv, gl, comp_of, cl, cm, np.array(order), vs, ve) == \
+            int(want[1] != 40)
+        assert (int(cl[0]), int(cl[5])) == (40, 40)  # the next round agrees
+    # four hops at most: rep 0's record gives 44 (ghost 45's label); the
+    # ghost chain 44 -> 43 -> 42 -> 41 -> 40 is four hops, and with one more
+    # link (40 -> 39) the fifth hop is not taken
+    for gv, gl, want in (([40, 41, 42, 43, 44, 45], [40, 40, 41, 42, 43, 44], 40),
+                         ([39, 40, 41, 42, 43, 44, 45], [39, 39, 40, 41, 42, 43, 44], 40)):
+        gv, gl = np.array(gv, dtype=np.uint32), np.array(gl, dtype=np.uint32)
+        rec = np.array([_rec(gv.size - 1, 0)], dtype=np.uint64)
+        cl = np.arange(vs, ve, dtype=np.uint32)
+        cm = np.zeros(10, dtype=np.uint32)
+        oracle.merge_step(rec, gv, gl, comp_of, cl, cm, np.array([0, 5]), vs, ve)
+        assert int(cl[0]) == want
+    # a vertex that is neither owned nor a ghost ends the hops ("missing")
+    gv3 = np.array([45], dtype=np.uint32)
+    gl3 = np.array([44], dtype=np.uint32)
+    rec = np.array([_rec(0, 5)], dtype=np.uint64)
+    cl = np.arange(vs, ve, dtype=np.uint32)
+    assert oracle.merge_step(rec, gv3, gl3, comp_of, cl, cm, np.array([0, 5]), vs, ve) == 1
+    assert int(cl[5]) == 44 and int(cl[0]) == 50
+
+
+def _share(G, P, rank):
+    """Rank `rank` of a P-way block partition: comp_of (local roots),
+    boundary records (gidx << 32) | rep sorted unique, ghosts, reps."""
+    n = G.n
+    base, rem = divmod(n, P)
+    sizes = [base + 1 if r < rem else base for r in range(P)]
+    b = np.concatenate([[0], np.cumsum(sizes)])
+    vs, ve = int(b[rank]), int(b[rank + 1])
+    rows = np.repeat(np.arange(n, dtype=np.int64), np.diff(G.row_ptr).astype(np.int64))
+    own = (rows >= vs) & (rows < ve)
+    src, dst = rows[own] - vs, G.col_idx[own].astype(np.int64)
+    loc = (dst >= vs) & (dst < ve)
+    nl = ve - vs
+    lr = np.concatenate([[0], np.cumsum(np.bincount(src[loc], minlength=nl))]).astype(np.uint64)
+    lc = (dst[loc] - vs).astype(np.uint32)
+    comp_of = oracle.cc_min_label(lr, lc)
+    ghosts, gidx = np.unique(dst[~loc], return_inverse=True)
+    rec = np.unique((gidx.astype(np.uint64) << np.uint64(32)) | comp_of[src[~loc]].astype(np.uint64))
+    reps = np.flatnonzero(comp_of == np.arange(nl)).astype(np.uint32)
+    return vs, ve, comp_of, rec, ghosts.astype(np.uint32), reps
+
+
+@pytest.mark.parametrize("P,rank", [(2, 0), (4, 1), (4, 3), (7, 5)])
+def test_merge_step_closed_form(P, rank):
+    """With every ghost holding its true global label (the component
+    minimum), one merge round gives each owned vertex its global label
+    comp_label[comp_of[li]] = min id of its global component."""
+    import cc_mpi_amd as cm
+    G = cm.rmat(12, 4, seed=P * 10 + rank)
+    truth = oracle.cc_min_label(G.row_ptr, G.col_idx)
+    vs, ve, comp_of, rec, ghosts, reps = _share(G, P, rank)
+    cl = np.arange(vs, ve, dtype=np.uint32)
+    cmin = np.zeros(ve - vs, dtype=np.uint32)
+    oracle.merge_step(rec, ghosts, truth[ghosts], comp_of, cl, cmin, reps, vs, ve)
+    assert np.array_equal(cl[comp_of], truth[vs:ve])
+    assert oracle.merge_step(rec, ghosts, truth[ghosts], comp_of, cl, cmin, reps, vs, ve) == 0
+
+
+def test_merge_step_converges_whatever_the_order():
+    """Rounds repeated until nothing changes reach the same labels with the
+    rep list forwards and backwards, from intermediate ghost labels (each
+    ghost's label on its own rank after local CC): the order only changes
+    how many rounds it takes."""
+    import cc_mpi_amd as cm
+    for seed in range(6):
+        G = cm.rmat(11, 3, seed=100 + seed)
+        P, rank = 4, seed % 4
+        vs, ve, comp_of, rec, ghosts, reps = _share(G, P, rank)
+        # a ghost's intermediate label: the smallest id it reaches within its
+        # own block (what its owner sends first)
+        inter = ghosts.copy()
+        for q in range(P):
+            qs, qe, qco, _, _, _ = _share(G, P, q)
+            m = (ghosts >= qs) & (ghosts < qe)
+            inter[m] = qco[ghosts[m] - qs] + qs
+        ends = []
+        for order in (reps, reps[::-1].copy()):
+            cl = np.arange(vs, ve, dtype=np.uint32)
+            cmin = np.zeros(ve - vs, dtype=np.uint32)
+            rounds = 0
+            while oracle.merge_step(rec, ghosts, inter, comp_of, cl, cmin, order, vs, ve):
+                rounds += 1
+                assert rounds < 100
+            ends.append(cl[comp_of].copy())
+        assert np.array_equal(ends[0], ends[1]), seed

@@ -191,11 +191,16 @@ def test_send_lists_match_their_definition(gpu, P, grouped, rows):
         b.close()
 
 
-@pytest.mark.parametrize("nl,roots", [(1, 1), (5000, 37), (70_000, 1)])
+@pytest.mark.parametrize("nl,roots", [(1, 1), (5000, 37), (70_000, 1), (5000, 500),
+                                      (20000, 3000)])
 def test_merge_step_over_rep_list(gpu, nl, roots):
     """cc_hip_local_reps_device lists exactly {r : comp_of[r] == r} in
-    ascending order, and a merge round over that list leaves comp_label,
-    comp_min and the changed flag as the round that scans comp_of."""
+    ascending order; merge rounds over that list and rounds that scan
+    comp_of both equal the oracle's merge round (src/cc_mpi.c:547-622):
+    round for round, with the changed flag, when there is one rep (no other
+    rep's update can be seen mid-round), and at convergence otherwise (the
+    reps update in parallel, as the reference's threads do, so one round's
+    labels depend on the order; the fixed point does not)."""
     import ctypes
     from cc_mpi_amd import _native as nat
     lib = nat.gpu_lib()
@@ -219,33 +224,55 @@ def test_merge_step_over_rep_list(gpu, nl, roots):
     np.testing.assert_array_equal(reps, np.flatnonzero(comp_of == np.arange(nl)).astype(np.uint32))
     d_rec = cm.DeviceBuffer.from_array(rec)
     d_gh = cm.DeviceBuffer.from_array(ghosts)
-    outs = []
+
+    def gpu_round(use_list, d_gl, d_cl, d_cm, changed):
+        if use_list:
+            rc = lib.cc_hip_merge_step_reps_device(d_rec.ptr, rec.size, d_gh.ptr, d_gl.ptr,
+                                                   ghosts.size, d_comp.ptr, d_cl.ptr, d_cm.ptr, nl,
+                                                   d_reps.ptr, cnt.value, vs, ve,
+                                                   ctypes.byref(changed), None)
+        else:
+            rc = lib.cc_hip_merge_step_device(d_rec.ptr, rec.size, d_gh.ptr, d_gl.ptr,
+                                              ghosts.size, d_comp.ptr, d_cl.ptr, d_cm.ptr, nl, vs,
+                                              ve, ctypes.byref(changed), None)
+        nat.check(rc, "merge step")
+
+    o_cl = np.arange(vs, ve, dtype=np.uint32)
+    o_cm = np.zeros(nl, dtype=np.uint32)
+    o_rounds, o_flags = [], []
+    while True:
+        o_flags.append(oracle.merge_step(rec, ghosts, g_lab, comp_of, o_cl, o_cm, reps, vs, ve))
+        o_rounds.append(o_cl[root_ids].copy())
+        if not o_flags[-1]:
+            break
+        assert len(o_rounds) < 100
     for use_list in (False, True):
         d_gl = cm.DeviceBuffer.from_array(g_lab)
         d_cl = cm.DeviceBuffer.from_array(np.arange(vs, ve, dtype=np.uint32))
         d_cm = cm.DeviceBuffer(4 * nl)
         changed = ctypes.c_int(-1)
-        for _ in range(2):  # two rounds: the second starts from lowered labels
-            if use_list:
-                rc = lib.cc_hip_merge_step_reps_device(d_rec.ptr, rec.size, d_gh.ptr, d_gl.ptr,
-                                                       ghosts.size, d_comp.ptr, d_cl.ptr, d_cm.ptr,
-                                                       nl, d_reps.ptr, cnt.value, vs, ve,
-                                                       ctypes.byref(changed), None)
-            else:
-                rc = lib.cc_hip_merge_step_device(d_rec.ptr, rec.size, d_gh.ptr, d_gl.ptr,
-                                                  ghosts.size, d_comp.ptr, d_cl.ptr, d_cm.ptr, nl,
-                                                  vs, ve, ctypes.byref(changed), None)
-            nat.check(rc, "merge step")
-        cl = np.empty(nl, dtype=np.uint32)
+        for rnd in range(100):
+            gpu_round(use_list, d_gl, d_cl, d_cm, changed)
+            cl = np.empty(nl, dtype=np.uint32)
+            d_cl.download(cl)
+            if roots == 1:  # one rep: the round is the oracle's, exactly
+                np.testing.assert_array_equal(cl[root_ids], o_rounds[min(rnd, len(o_rounds) - 1)],
+                                              err_msg=f"round {rnd}")
+                assert changed.value == (o_flags[rnd] if rnd < len(o_flags) else 0)
+            if not changed.value:
+                break
+        else:
+            raise AssertionError("merge rounds did not converge")
         cmin = np.empty(nl, dtype=np.uint32)
-        d_cl.download(cl)
         d_cm.download(cmin)
-        outs.append((cl, cmin[root_ids], changed.value))
+        # the fixed point: the oracle's, with comp_min equal to comp_label on the reps
+        np.testing.assert_array_equal(cl[root_ids], o_cl[root_ids])
+        np.testing.assert_array_equal(cmin[root_ids], cl[root_ids])
+        # non-reps are never touched
+        rest = np.setdiff1d(np.arange(nl), root_ids)
+        np.testing.assert_array_equal(cl[rest], np.arange(vs, ve, dtype=np.uint32)[rest])
         for b in (d_gl, d_cl, d_cm):
             b.close()
-    np.testing.assert_array_equal(outs[0][0], outs[1][0])
-    np.testing.assert_array_equal(outs[0][1], outs[1][1])
-    assert outs[0][2] == outs[1][2]
     # a missing list is an error, not a silent scan
     assert lib.cc_hip_merge_step_reps_device(d_rec.ptr, rec.size, d_gh.ptr, d_gh.ptr, ghosts.size,
                                              d_comp.ptr, d_comp.ptr, d_comp.ptr, nl, None, 0, vs,

@@ -1,6 +1,8 @@
 """Batched solves and the serving loop: every request's labels equal the
 oracle's for its graph, whether solved alone, in a batch with others, or
 through the continuous-batching server (exact: integer labels)."""
+import os
+
 import numpy as np
 import pytest
 
@@ -82,9 +84,9 @@ def test_batch_capacity_errors(gpu):
 
 @pytest.mark.parametrize("direct_max", [0, 1, 3])
 def test_server_requests_match_oracle(gpu, direct_max):
-    keys, loader = serving.make_pool(per_type=3, seed=11, n_min=1024, n_max=200000)
+    keys, loader, estimate = serving.make_pool(per_type=3, seed=11, n_min=1024, n_max=200000)
     want = {}
-    store = serving.GraphStore(budget_bytes=1 << 32, loader=loader)
+    store = serving.GraphStore(budget_bytes=1 << 32, loader=loader, estimate=estimate)
     for k in keys:
         g = store.acquire(k)
         want[k] = oracle_of(g)
@@ -110,16 +112,100 @@ def test_server_requests_match_oracle(gpu, direct_max):
 
 
 def test_store_with_device_graphs(gpu):
-    keys, loader = serving.make_pool(per_type=2, seed=3, n_min=50000, n_max=60000)
+    keys, loader, estimate = serving.make_pool(per_type=2, seed=3, n_min=50000, n_max=60000)
     sizes = {}
     for k in keys:
         g = loader(k)
-        sizes[k] = serving.footprint_bytes(g.n, g.m)
+        sizes[k] = serving.resident_bytes(g)
+        assert sizes[k] <= estimate(k)[0]  # the spec's price bounds what the graph holds
         g.close()
-    store = serving.GraphStore(budget_bytes=2 * max(sizes.values()), loader=loader)
+    # room for one build's peak plus a little: every admission after the
+    # first must evict
+    budget = int(1.2 * max(estimate(k)[1] for k in keys))
+    store = serving.GraphStore(budget_bytes=budget, loader=loader, estimate=estimate)
     for k in keys:  # each admitted in turn, evicting older ones as needed
         g = store.acquire(k)
         assert g is not None and store.used <= store.budget
         assert np.array_equal(g.solve(), oracle_of(g))
         store.release(k)
+    assert store.evictions > 0
+    store.close()
+
+
+def pool_oracle(keys, store):
+    """Oracle labels of every pool graph (union-find on lattices, the
+    restated label propagation on all cores elsewhere)."""
+    want, entries = {}, {}
+    for k in keys:
+        g = store.acquire(k)
+        h = g.to_host()
+        entries[k] = g.m
+        if k[0] == "grid":
+            want[k] = oracle.cc_min_label(h.row_ptr, h.col_idx)
+        else:
+            want[k] = oracle.cc_lp_pthreads(h.row_ptr, h.col_idx, os.cpu_count() or 1, 8192)
+        store.release(k)
+    return want, entries
+
+
+def test_serving_stream_10000_requests(gpu):
+    """The serving workload at full size: 10,000 Poisson requests over a pool
+    of 48 graphs (ER d8 / R-MAT ef16 / grid 30% deleted, n log-uniform
+    1K-4M), through the continuous-batching server and the one-per-launch
+    baseline; every request's labels are checked as its batch retires."""
+    keys, loader, estimate = serving.make_pool(per_type=16, seed=2024)
+    store = serving.GraphStore(budget_bytes=64 << 30, loader=loader, estimate=estimate)
+    want, entries = pool_oracle(keys, store)
+    ns = [store.graphs[k].n for k in keys]
+    assert min(ns) < 4096 and max(ns) > 1 << 21  # the pool spans the configured range
+    seen = {"ok": 0, "bad": []}
+
+    def check(req, labels):
+        if np.array_equal(labels, want[req.key]):
+            seen["ok"] += 1
+        else:
+            seen["bad"].append(req.rid)
+    srv = serving.Server(store, lanes=2, max_vertices=max(1 << 24, max(ns)), max_items=20000,
+                         max_graphs=2048, item_vertices=1024)
+    reqs = serving.poisson_requests(10000, qps=8000.0, keys=keys, seed=7)
+    stats = srv.run(reqs, on_result=check)
+    srv.close()
+    assert not seen["bad"], f"requests with wrong labels: {seen['bad'][:10]}"
+    assert seen["ok"] == 10000
+    assert stats["batches"] > stats["direct_batches"]  # some requests were batched together
+    assert all(np.isfinite(r.done) and r.done >= r.arrival for r in reqs)
+    seen["ok"] = 0
+    base = serving.poisson_requests(2000, qps=8000.0, keys=keys, seed=8)
+    serving.serve_one_by_one(store, base, on_result=check)
+    assert not seen["bad"] and seen["ok"] == 2000
+    store.close()
+
+
+def test_serving_under_memory_pressure(gpu):
+    """A store budget of a third of the pool: graphs are evicted and rebuilt
+    by the loader, requests wait for HBM, and every label is still exact."""
+    keys, loader, estimate = serving.make_pool(per_type=64, seed=99, n_min=1024, n_max=1 << 19)
+    big = serving.GraphStore(budget_bytes=64 << 30, loader=loader, estimate=estimate)
+    want, _ = pool_oracle(keys, big)
+    pool_bytes = big.used
+    big.close()
+    budget = pool_bytes // 3
+    assert budget >= max(estimate(k)[1] for k in keys)  # every graph can be admitted alone
+    store = serving.GraphStore(budget_bytes=budget, loader=loader, estimate=estimate)
+    seen = {"ok": 0, "bad": 0}
+
+    def check(req, labels):
+        if np.array_equal(labels, want[req.key]):
+            seen["ok"] += 1
+        else:
+            seen["bad"] += 1
+    srv = serving.Server(store, lanes=2, max_vertices=1 << 22, max_items=1 << 14,
+                         max_graphs=256, item_vertices=1024)
+    reqs = serving.poisson_requests(600, qps=2000.0, keys=keys, seed=4)
+    srv.run(reqs, on_result=check)
+    srv.close()
+    st = store.stats()
+    assert seen == {"ok": 600, "bad": 0}
+    assert st["evictions"] > 0 and st["loads"] > len(keys)
+    assert st["peak_resident_bytes"] <= budget
     store.close()
