@@ -110,6 +110,20 @@ struct SlotGuard {
 // Kernel timing: call before/after a launch; no-ops unless enabled.
 void prof_begin(const char *name, hipStream_t s);
 void prof_end(hipStream_t s);
+// While one is alive, this thread's launches record no timing events
+// (which may create events), whatever cc_hip_prof_enable says.
+void prof_mute(bool on);
+struct ProfMute {
+  ProfMute() { prof_mute(true); }
+  ~ProfMute() { prof_mute(false); }
+};
+
+// The context if the library is initialised, else nullptr (never
+// initialises it, so never allocates).
+const Context *ctx_if_ready();
+// Sets up the scratch array of the allocation-free solve on `device`
+// (current at the call).
+int components_noalloc_setup(int device);
 
 }  // namespace ccx
 

@@ -60,14 +60,30 @@ constexpr int kBlock = 256;
 constexpr int kWave = 64;
 constexpr int kGiantSamples = 1024;
 
-// Tuning knobs (process-wide; cc_hip_components_tune).  Neither changes the
-// result: sampling only decides how much of the work the cheap
-// per-vertex rounds do before the frontier sweep.
-int g_sampled_neighbours = 2;  // sampled hooking rounds (1..4)
-int g_blocks_per_cu = 0;       // grid cap of the sampled hook (0 = 8 with hook ILP 4, else 4)
-// 0: plain loads; 1: L2-confirmed roots, L2 re-walks; 2: all loads from L2;
-// 3: lossy sampling; 4: L2-confirmed roots, plain re-walks
-int g_hook_mode = 4;
+// Tuning knobs.  None changes the result: sampling only decides how much of
+// the work the cheap per-vertex rounds do before the frontier sweep.  The
+// defaults are the measured plan (notes at plan_for below).
+struct Knobs {
+  int sampled_neighbours = 2;  // sampled hooking rounds (1..4)
+  int blocks_per_cu = 0;       // grid cap of the sampled hook (0 = auto)
+  // 0: plain loads; 1: L2-confirmed roots, L2 re-walks; 2: all loads from
+  // L2; 3: lossy sampling; 4: L2-confirmed roots, plain re-walks
+  int hook_mode = 4;
+  int stream_blocks_per_cu = 8;    // grid cap of the contention-free kernels (init, compress)
+  int frontier_blocks_per_cu = 8;  // grid cap of the frontier sweep
+  int hook_ilp = 0;                // roots walked per thread in the sampled hook (0 = auto)
+  int stream_ilp = 1;              // vertices per thread per step in init/compress (1, 2, 4)
+  int sample_cache = 1;            // sampled neighbours read once into coalesced arrays
+  int fused_pair = -1;             // both sampled rounds in one pass (-1 = auto)
+  int halve = -1;                  // path halving in the one-pass hook's walks (-1 = auto)
+  int rem = -1;                    // one-pass hook by Rem's unions (-1 = auto)
+  int rem_splice = -1;             // splicing in Rem's unions (-1 = auto)
+  int giant_early = 1;             // two-pass path: sample the giant after round 0, skip it in round 1
+};
+// Process-wide settings (cc_hip_components_set / _tune); the entry point
+// without allocations always runs with the defaults.
+Knobs g_knobs;
+const Knobs kDefaultKnobs{};
 
 // Parent read that bypasses the CU's L1 (an agent-scope atomic load), so it
 // sees every hook that has reached L2.
@@ -713,16 +729,6 @@ inline unsigned blocks_for(uint64_t n, int cu_count, int per_cu) {
 //    (1, with 4 blocks per CU);
 //  * on the two-pass path, sampling the giant root after round 0 and
 //    skipping its members in round 1 saves 5-6% on Chung-Lu 16.4M/65.6M.
-int g_stream_blocks_per_cu = 8;    // grid cap of the contention-free kernels (init, compress)
-int g_frontier_blocks_per_cu = 8;  // grid cap of the frontier sweep
-int g_hook_ilp = 0;                // roots walked per thread in the sampled hook (0 = auto)
-int g_stream_ilp = 1;              // vertices per thread per step in init/compress (1, 2, 4)
-int g_sample_cache = 1;            // sampled neighbours read once into coalesced arrays
-int g_fused_pair = -1;             // both sampled rounds in one pass (-1 = auto)
-int g_halve = -1;                  // path halving in the one-pass hook's walks (-1 = auto)
-int g_rem = -1;                    // one-pass hook by Rem's unions (-1 = auto)
-int g_rem_splice = -1;             // splicing in Rem's unions (-1 = auto)
-int g_giant_early = 1;             // two-pass path: sample the giant after round 0, skip it in round 1
 constexpr uint64_t kDenseDegree = 16;              // dense: m >= 16 n
 constexpr uint64_t kHalveMinVertices = 1ull << 24;  // halving from this many vertices
 constexpr uint64_t kWideIlpMaxEntries = 1ull << 31; // dense one-pass ILP 2 below this many entries
@@ -737,20 +743,20 @@ struct Plan {
   int hook_bpc;  // sampled-hook grid cap, workgroups per CU
 };
 
-inline Plan plan_for(uint32_t n, uint64_t m) {
+inline Plan plan_for(uint32_t n, uint64_t m, const Knobs &K = g_knobs) {
   Plan pl;
   const bool dense = m >= kDenseDegree * (uint64_t)n;
   const bool big = n >= kHalveMinVertices;
-  pl.cache = g_sample_cache && (g_hook_mode == 0 || g_hook_mode == 1 || g_hook_mode == 4);
-  pl.fused = pl.cache && g_sampled_neighbours == 2 && (g_fused_pair < 0 ? 1 : g_fused_pair);
-  pl.rem = pl.fused && g_hook_mode != 0 && (g_rem < 0 ? (!big || !dense) : g_rem);
+  pl.cache = K.sample_cache && (K.hook_mode == 0 || K.hook_mode == 1 || K.hook_mode == 4);
+  pl.fused = pl.cache && K.sampled_neighbours == 2 && (K.fused_pair < 0 ? 1 : K.fused_pair);
+  pl.rem = pl.fused && K.hook_mode != 0 && (K.rem < 0 ? (!big || !dense) : K.rem);
   const int splice_auto = big ? 1 : (dense ? 2 : 0);
-  pl.splice = pl.rem ? (g_rem_splice < 0 ? splice_auto : g_rem_splice) : 0;
-  pl.halve = pl.fused && !pl.rem && g_hook_mode != 0 && (g_halve < 0 ? big : g_halve);
+  pl.splice = pl.rem ? (K.rem_splice < 0 ? splice_auto : K.rem_splice) : 0;
+  pl.halve = pl.fused && !pl.rem && K.hook_mode != 0 && (K.halve < 0 ? big : K.halve);
   const int wide = pl.fused ? (m < kWideIlpMaxEntries ? 2 : 1) : 4;
-  pl.hook_ilp = pl.rem ? 1 : (g_hook_ilp ? g_hook_ilp : (dense ? wide : 1));
+  pl.hook_ilp = pl.rem ? 1 : (K.hook_ilp ? K.hook_ilp : (dense ? wide : 1));
   const int bpc_auto = pl.rem ? (pl.splice == 1 ? 4 : 8) : (pl.hook_ilp >= 2 ? 8 : 4);
-  pl.hook_bpc = g_blocks_per_cu ? g_blocks_per_cu : bpc_auto;
+  pl.hook_bpc = K.blocks_per_cu ? K.blocks_per_cu : bpc_auto;
   return pl;
 }
 
@@ -761,38 +767,38 @@ inline bool ilp_ok(int v) { return v == 1 || v == 2 || v == 4; }
 extern "C" int cc_hip_components_set(const char *key, int value) {
   if (!key) return ccx::set_error(4, "cc_hip_components_set: null key");
   const std::string k(key);
-  if (k == "sampled_neighbours" && value >= 1 && value <= 4) g_sampled_neighbours = value;
-  else if (k == "blocks_per_cu" && value >= 0 && value <= 64) g_blocks_per_cu = value;
-  else if (k == "stream_blocks_per_cu" && value >= 1 && value <= 64) g_stream_blocks_per_cu = value;
-  else if (k == "frontier_blocks_per_cu" && value >= 1 && value <= 64) g_frontier_blocks_per_cu = value;
-  else if (k == "hook_mode" && value >= 0 && value <= 4) g_hook_mode = value;
-  else if (k == "halve" && value >= -1 && value <= 1) g_halve = value;
-  else if (k == "rem" && value >= -1 && value <= 1) g_rem = value;
-  else if (k == "rem_splice" && value >= -1 && value <= 2) g_rem_splice = value;
-  else if (k == "hook_ilp" && (value == 0 || ilp_ok(value))) g_hook_ilp = value;
-  else if (k == "stream_ilp" && ilp_ok(value)) g_stream_ilp = value;
-  else if (k == "sample_cache" && (value == 0 || value == 1)) g_sample_cache = value;
-  else if (k == "fused_pair" && value >= -1 && value <= 1) g_fused_pair = value;
-  else if (k == "giant_early" && (value == 0 || value == 1)) g_giant_early = value;
+  if (k == "sampled_neighbours" && value >= 1 && value <= 4) g_knobs.sampled_neighbours = value;
+  else if (k == "blocks_per_cu" && value >= 0 && value <= 64) g_knobs.blocks_per_cu = value;
+  else if (k == "stream_blocks_per_cu" && value >= 1 && value <= 64) g_knobs.stream_blocks_per_cu = value;
+  else if (k == "frontier_blocks_per_cu" && value >= 1 && value <= 64) g_knobs.frontier_blocks_per_cu = value;
+  else if (k == "hook_mode" && value >= 0 && value <= 4) g_knobs.hook_mode = value;
+  else if (k == "halve" && value >= -1 && value <= 1) g_knobs.halve = value;
+  else if (k == "rem" && value >= -1 && value <= 1) g_knobs.rem = value;
+  else if (k == "rem_splice" && value >= -1 && value <= 2) g_knobs.rem_splice = value;
+  else if (k == "hook_ilp" && (value == 0 || ilp_ok(value))) g_knobs.hook_ilp = value;
+  else if (k == "stream_ilp" && ilp_ok(value)) g_knobs.stream_ilp = value;
+  else if (k == "sample_cache" && (value == 0 || value == 1)) g_knobs.sample_cache = value;
+  else if (k == "fused_pair" && value >= -1 && value <= 1) g_knobs.fused_pair = value;
+  else if (k == "giant_early" && (value == 0 || value == 1)) g_knobs.giant_early = value;
   else return ccx::set_error(4, "cc_hip_components_set: bad key or value (%s=%d)", key, value);
   return 0;
 }
 
 extern "C" int cc_hip_components_get(const char *key) {
   const std::string k(key ? key : "");
-  if (k == "sampled_neighbours") return g_sampled_neighbours;
-  if (k == "blocks_per_cu") return g_blocks_per_cu;
-  if (k == "stream_blocks_per_cu") return g_stream_blocks_per_cu;
-  if (k == "frontier_blocks_per_cu") return g_frontier_blocks_per_cu;
-  if (k == "hook_mode") return g_hook_mode;
-  if (k == "hook_ilp") return g_hook_ilp;
-  if (k == "stream_ilp") return g_stream_ilp;
-  if (k == "sample_cache") return g_sample_cache;
-  if (k == "fused_pair") return g_fused_pair;
-  if (k == "giant_early") return g_giant_early;
-  if (k == "halve") return g_halve;
-  if (k == "rem") return g_rem;
-  if (k == "rem_splice") return g_rem_splice;
+  if (k == "sampled_neighbours") return g_knobs.sampled_neighbours;
+  if (k == "blocks_per_cu") return g_knobs.blocks_per_cu;
+  if (k == "stream_blocks_per_cu") return g_knobs.stream_blocks_per_cu;
+  if (k == "frontier_blocks_per_cu") return g_knobs.frontier_blocks_per_cu;
+  if (k == "hook_mode") return g_knobs.hook_mode;
+  if (k == "hook_ilp") return g_knobs.hook_ilp;
+  if (k == "stream_ilp") return g_knobs.stream_ilp;
+  if (k == "sample_cache") return g_knobs.sample_cache;
+  if (k == "fused_pair") return g_knobs.fused_pair;
+  if (k == "giant_early") return g_knobs.giant_early;
+  if (k == "halve") return g_knobs.halve;
+  if (k == "rem") return g_knobs.rem;
+  if (k == "rem_splice") return g_knobs.rem_splice;
   return -1;
 }
 
@@ -819,8 +825,8 @@ extern "C" int cc_hip_components_plan_get(uint32_t n, uint64_t m, const char *ke
   if (k == "rem_splice") return pl.splice;
   if (k == "hook_ilp") return pl.hook_ilp;
   if (k == "hook_blocks_per_cu") return pl.hook_bpc;
-  if (k == "giant_early") return pl.cache && !pl.fused && g_giant_early;
-  if (k == "hook_mode") return g_hook_mode;
+  if (k == "giant_early") return pl.cache && !pl.fused && g_knobs.giant_early;
+  if (k == "hook_mode") return g_knobs.hook_mode;
   return -1;
 }
 
@@ -839,35 +845,30 @@ extern "C" int cc_hip_components_tune(int sampled_neighbours, int blocks_per_cu,
       hook_mode < -1 || hook_mode > 4)
     return ccx::set_error(4, "cc_hip_components_tune: out of range (%d, %d)", sampled_neighbours,
                           blocks_per_cu);
-  if (sampled_neighbours > 0) g_sampled_neighbours = sampled_neighbours;
-  if (blocks_per_cu > 0) g_blocks_per_cu = blocks_per_cu;
-  if (hook_mode >= 0) g_hook_mode = hook_mode;
+  if (sampled_neighbours > 0) g_knobs.sampled_neighbours = sampled_neighbours;
+  if (blocks_per_cu > 0) g_knobs.blocks_per_cu = blocks_per_cu;
+  if (hook_mode >= 0) g_knobs.hook_mode = hook_mode;
   return 0;
 }
 
-extern "C" int cc_hip_components_device(const uint64_t *d_row_ptr, const uint32_t *d_col_idx,
-                                        uint32_t n, uint64_t m, uint32_t *d_labels, void *stream) {
-  ccx::Context *c = ccx::ctx();
-  if (!c) return 2;
-  if (n == 0) return 0;
-  if (!d_row_ptr || !d_labels || (m > 0 && !d_col_idx))
-    return ccx::set_error(4, "cc_hip_components_device: null pointer");
-  if (n > ccx::kMaxVertices)
-    return ccx::set_error(4, "cc_hip_components_device: n=%u exceeds the limit of %u vertices", n,
-                          ccx::kMaxVertices);
-  hipStream_t s = ccx::pick_stream(stream);
-  // The sampled giant root lives in this stream's own scratch word, so
-  // solves on different streams never share it.
-  ccx::StreamState *st = ccx::stream_state(s);
-  if (!st) return 3;
-  uint32_t *giant = st->d_words;
-  const Plan plan = plan_for(n, m);
+namespace {
+
+// The whole solve on stream s with knobs K.  words[0..1]: the sampled giant
+// root and the frontier sweep's link flag (written before they are read);
+// smp: rounds * n words for the sampled-neighbour cache when the plan uses
+// it (written by the init pass before the hooks read it).  Allocates
+// nothing itself.
+int solve_with(const Knobs &K, const ccx::Context *c, const uint64_t *d_row_ptr,
+               const uint32_t *d_col_idx, uint32_t n, uint64_t m, uint32_t *d_labels,
+               hipStream_t s, uint32_t *words, uint32_t *smp) {
+  uint32_t *giant = words;
+  const Plan plan = plan_for(n, m, K);
   const int hook_ilp = plan.hook_ilp;
   const unsigned nb = blocks_for(n, c->cu_count, plan.hook_bpc);
-  const unsigned nf = blocks_for(n, c->cu_count, g_frontier_blocks_per_cu);
-  const unsigned ns = blocks_for(n, c->cu_count, g_stream_blocks_per_cu);
-  const uint32_t rounds = (uint32_t)g_sampled_neighbours;
-  const int ilp = g_stream_ilp;
+  const unsigned nf = blocks_for(n, c->cu_count, K.frontier_blocks_per_cu);
+  const unsigned ns = blocks_for(n, c->cu_count, K.stream_blocks_per_cu);
+  const uint32_t rounds = (uint32_t)K.sampled_neighbours;
+  const int ilp = K.stream_ilp;
 
   if (m == 0) {
     CCX_LAUNCH("cc_init", s, k_init<<<ns, kBlock, 0, s>>>(n, d_labels));
@@ -881,15 +882,13 @@ extern "C" int cc_hip_components_device(const uint64_t *d_row_ptr, const uint32_
     else CCX_LAUNCH("cc_compress", s, k_compress<<<ns, kBlock, 0, s>>>(n, d_labels, gate));
     return 0;
   };
-  const int mode = g_hook_mode;
+  const int mode = K.hook_mode;
   // The cache holds `rounds` u32 per vertex in this stream's temp buffer;
   // modes 0, 1 and 4 read it (mode 2 walks with L2 reads, mode 3 is lossy).
   const bool cache = plan.cache != 0;
-  const bool early_giant = cache && !plan.fused && g_giant_early;
-  uint32_t *smp = nullptr;
+  const bool early_giant = cache && !plan.fused && K.giant_early;
   if (cache) {
-    smp = (uint32_t *)ccx::stream_tmp(st, (size_t)rounds * n * sizeof(uint32_t));
-    if (!smp) return 3;
+    if (!smp) return ccx::set_error(4, "solve: no sample buffer");
     switch (rounds) {
       case 1: CCX_LAUNCH("cc_init_min", s, k_init_sample<1><<<ns, kBlock, 0, s>>>(d_row_ptr, d_col_idx, n, d_labels, smp)); break;
       case 2: CCX_LAUNCH("cc_init_min", s, k_init_sample<2><<<ns, kBlock, 0, s>>>(d_row_ptr, d_col_idx, n, d_labels, smp)); break;
@@ -989,6 +988,94 @@ extern "C" int cc_hip_components_device(const uint64_t *d_row_ptr, const uint32_
   else
     CCX_LAUNCH("cc_hook_frontier", s, k_hook_frontier<3><<<nf, kBlock, 0, s>>>(d_row_ptr, d_col_idx, n, first, d_labels, giant));
   return compress(giant + 1);
+}
+
+}  // namespace
+
+extern "C" int cc_hip_components_device(const uint64_t *d_row_ptr, const uint32_t *d_col_idx,
+                                        uint32_t n, uint64_t m, uint32_t *d_labels, void *stream) {
+  ccx::Context *c = ccx::ctx();
+  if (!c) return 2;
+  if (n == 0) return 0;
+  if (!d_row_ptr || !d_labels || (m > 0 && !d_col_idx))
+    return ccx::set_error(4, "cc_hip_components_device: null pointer");
+  if (n > ccx::kMaxVertices)
+    return ccx::set_error(4, "cc_hip_components_device: n=%u exceeds the limit of %u vertices", n,
+                          ccx::kMaxVertices);
+  hipStream_t s = ccx::pick_stream(stream);
+  // The sampled giant root and the sample cache live in this stream's own
+  // state, so solves on different streams never share them.
+  ccx::StreamState *st = ccx::stream_state(s);
+  if (!st) return 3;
+  const Knobs K = g_knobs;
+  uint32_t *smp = nullptr;
+  if (m > 0 && plan_for(n, m, K).cache) {
+    smp = (uint32_t *)ccx::stream_tmp(st, (size_t)K.sampled_neighbours * n * sizeof(uint32_t));
+    if (!smp) return 3;
+  }
+  return solve_with(K, c, d_row_ptr, d_col_idx, n, m, d_labels, s, st->d_words, smp);
+}
+
+// ---- the solve without allocations ------------------------------------------
+//
+// cc_hip_components_device_noalloc runs the default plan with its scratch in
+// the file-scope array `scratch` below instead of the stream's state: 64
+// control words (the sampled giant root and the frontier's link flag; both
+// written before they are read) followed by the sampled-neighbour cache, 2
+// words per vertex (written by the init pass before the hook reads it).  So
+// nothing needs clearing between calls.  Every call on one GPU shares that
+// GPU's copy of the array: calls that overlap on one GPU race.
+
+namespace {
+constexpr uint32_t kNoAllocMaxVertices = 1u << 22;  // 4,194,304 vertices
+constexpr size_t kNoAllocControlWords = 64;
+constexpr size_t kScratchWords = kNoAllocControlWords + 2 * (size_t)kNoAllocMaxVertices;
+constexpr int kMaxDevices = 64;
+// Device address of `scratch` on each device, from hipGetSymbolAddress at
+// initialisation (the array's name is not a device address in host code).
+uint32_t *g_scratch_dev[kMaxDevices];
+}  // namespace
+
+// 32 MiB + 256 B, held on every GPU the library runs on until the process ends.
+static __device__ __align__(256) uint32_t scratch[kScratchWords];
+
+namespace ccx {
+// Called by the library's initialisation once per device, with that device
+// current.
+int components_noalloc_setup(int device) {
+  if (device < 0 || device >= kMaxDevices)
+    return set_error(2, "scratch setup: device %d out of range", device);
+  void *p = nullptr;
+  const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+  if (e != hipSuccess || !p)
+    return set_error(1, "scratch setup: hipGetSymbolAddress failed on device %d: %s", device,
+                     hipGetErrorString(e));
+  g_scratch_dev[device] = (uint32_t *)p;
+  return 0;
+}
+}  // namespace ccx
+
+extern "C" int cc_hip_components_device_noalloc(const uint64_t *d_row_ptr,
+                                                const uint32_t *d_col_idx, uint32_t n,
+                                                uint64_t m, uint32_t *d_labels, void *stream) {
+  const ccx::Context *c = ccx::ctx_if_ready();
+  if (!c)
+    return ccx::set_error(2, "cc_hip_components_device_noalloc: the library is not initialised "
+                             "(call cc_hip_init first; this entry point never initialises it)");
+  if (n == 0) return 0;
+  if (!d_row_ptr || !d_labels || (m > 0 && !d_col_idx))
+    return ccx::set_error(4, "cc_hip_components_device_noalloc: null pointer");
+  if (n > kNoAllocMaxVertices)
+    return ccx::set_error(4, "cc_hip_components_device_noalloc: n=%u exceeds the %u vertices its "
+                             "scratch array holds (use cc_hip_components_device)",
+                          n, kNoAllocMaxVertices);
+  uint32_t *ptr = g_scratch_dev[c->device];
+  if (!ptr)
+    return ccx::set_error(2, "cc_hip_components_device_noalloc: scratch array not set up");
+  const hipStream_t s = stream ? (hipStream_t)stream : c->stream;
+  const ccx::ProfMute mute;  // no timing events either
+  return solve_with(kDefaultKnobs, c, d_row_ptr, d_col_idx, n, m, d_labels, s, ptr,
+                    ptr + kNoAllocControlWords);
 }
 
 extern "C" int cc_hip_components_host(const uint64_t *row_ptr, const uint32_t *col_idx, uint32_t n,

@@ -59,6 +59,7 @@ static int init_locked(int device) {
   CCX_TRY(hipMalloc(&g_ctx.d_small, 4096));
   CCX_TRY(hipMemset(g_ctx.d_small, 0, 4096));
   CCX_TRY(hipHostMalloc(&g_ctx.h_small, 4096, hipHostMallocDefault));
+  if (int rc = components_noalloc_setup(device)) return rc;
   g_ctx_ok.store(true, std::memory_order_release);
   return 0;
 }
@@ -72,6 +73,13 @@ Context *ctx() {
   }
   std::lock_guard<std::mutex> lk(g_ctx_mu);
   if (init_locked(-1) != 0) return nullptr;
+  return &g_ctx;
+}
+
+const Context *ctx_if_ready() {
+  if (!g_ctx_ok.load(std::memory_order_acquire)) return nullptr;
+  int cur = -1;
+  if (hipGetDevice(&cur) == hipSuccess && cur != g_ctx.device) (void)hipSetDevice(g_ctx.device);
   return &g_ctx;
 }
 
@@ -223,6 +231,7 @@ static std::vector<PendingPair> g_pending;
 static std::vector<hipEvent_t> g_event_pool;
 static std::map<std::string, std::pair<uint64_t, double>> g_prof_acc;
 static thread_local PendingPair *g_open = nullptr;
+static thread_local bool g_prof_muted = false;
 static thread_local PendingPair g_open_slot;
 static std::string g_names;
 
@@ -237,8 +246,10 @@ static hipEvent_t take_event() {
   return e;
 }
 
+void prof_mute(bool on) { g_prof_muted = on; }
+
 void prof_begin(const char *name, hipStream_t s) {
-  if (!g_prof_on) return;
+  if (!g_prof_on || g_prof_muted) return;
   std::lock_guard<std::mutex> lk(g_prof_mu);
   g_open_slot.name = name;
   g_open_slot.a = take_event();

@@ -79,6 +79,24 @@ const char *cc_hip_prof_names(void);
 int cc_hip_components_device(const uint64_t *d_row_ptr, const uint32_t *d_col_idx,
                              uint32_t n, uint64_t m, uint32_t *d_labels, void *stream);
 
+/*
+ * cc_hip_components_device without any allocation.  No counterpart in the
+ * reference: it is for callers that cannot allocate device memory (code
+ * under HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc).  Runs the default plan (whatever cc_hip_components_set has
+ * tuned) with its scratch in a fixed device array of the library (32 MiB +
+ * 256 B per GPU, held until the process ends) instead of the stream's state;
+ * calls no hipMalloc, hipFree or other allocating routine, records no timing
+ * events, and never initialises the library: cc_hip_init must have run
+ * (return 2 otherwise).  n <= 4,194,304 (return 4 beyond).  Same labels as
+ * cc_hip_components_device.  Every call on one GPU shares that GPU's scratch
+ * array, so calls that overlap on one GPU (on different streams, or from
+ * different threads) race; calls queued one after another on one stream
+ * are fine.  Asynchronous on `stream` (NULL: the library stream).
+ */
+int cc_hip_components_device_noalloc(const uint64_t *d_row_ptr, const uint32_t *d_col_idx,
+                                     uint32_t n, uint64_t m, uint32_t *d_labels, void *stream);
+
 /* Process-wide tuning of the solve: the number of sampled hooking rounds
  * (1..4, default 2; 0 keeps the current value), the grid cap of the
  * sampled-hook kernel in workgroups per CU (1..64; 0 keeps), and the

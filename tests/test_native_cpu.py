@@ -75,6 +75,24 @@ def test_no_device_reports_error_not_fallback():
         cm.compute_connected_components_pthreads(G)
 
 
+def test_noalloc_solve_never_initialises(tmp_path):
+    """The allocation-free solve refuses to run before cc_hip_init (which
+    sets up its scratch array) instead of initialising the library, which
+    would allocate; checked in a fresh process that never initialises."""
+    code = (
+        "import ctypes, sys\n"
+        f"lib = ctypes.CDLL({os.path.join(nat.LIBDIR, 'libccmi355x.so')!r})\n"
+        "lib.cc_hip_last_error.restype = ctypes.c_char_p\n"
+        "f = lib.cc_hip_components_device_noalloc\n"
+        "f.argtypes = [ctypes.c_void_p] * 2 + [ctypes.c_uint32, ctypes.c_uint64]"
+        " + [ctypes.c_void_p] * 2\n"
+        "rc = f(None, None, 10, 0, None, None)\n"
+        "print(rc, lib.cc_hip_last_error().decode())\n")
+    r = subprocess.run(["python3", "-c", code], capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr
+    assert r.stdout.startswith("2 ") and "cc_hip_init" in r.stdout
+
+
 def well_formed(G):
     rp, ci = G.row_ptr, G.col_idx
     assert rp[0] == 0 and (np.diff(rp.astype(np.int64)) >= 0).all() and rp[-1] == ci.size
