@@ -468,10 +468,21 @@ def rankshare_leg(args) -> dict:
                                                                    "merge_loop")},
            "intervals": runs, "packs_checked_against_oracle": checked[0], "kernels": kern}
     if "delta_pack" in kern:
-        b = 16.0 * rs.total_send  # + 12 per emitted pair, averaged below
-        emitted = runs["1"]["pairs_emitted"] / max(1, runs["1"]["packs"])
-        out["delta_pack_algorithmic_gbs"] = round((b + 12.0 * emitted) /
-                                                  (kern["delta_pack"]["avg_us"] * 1e3), 1)
+        # one pack = its count, scan and emit kernels (delta_pack, delta_scan,
+        # delta_emit), averaged over every pack of the three intervals
+        packs = kern["delta_pack"]["launches"]
+        pack_us = sum(kern[k]["avg_us"] * kern[k]["launches"]
+                      for k in ("delta_pack", "delta_scan", "delta_emit") if k in kern) / packs
+        emitted = sum(r["pairs_emitted"] for r in runs.values()) / max(1, packs)
+        # algorithmic bytes of the reference's pack: send_to_flat, comp_of,
+        # comp_label and prev_sent per entry, 12 per emitted pair (prev_sent
+        # write, 8-byte pair); with the reps gathered once the pack itself
+        # streams the rep list and prev_sent (8 per entry)
+        out["delta_pack_us"] = round(pack_us, 1)
+        out["delta_pack_algorithmic_gbs"] = round((16.0 * rs.total_send + 12.0 * emitted) /
+                                                  (pack_us * 1e3), 1)
+        out["delta_pack_streamed_gbs"] = round((8.0 * rs.total_send + 12.0 * emitted) /
+                                               (pack_us * 1e3), 1)
     # raw boundary records turned into sorted unique pairs per second (rep
     # fill, the dominant rep's bitmap, sort of the rest, dedup, merge)
     out["boundary_build_grecords_per_s"] = round(rs.nb / rs.times["boundary_build"] / 1e9, 2)

@@ -130,6 +130,10 @@ def gpu_lib() -> ctypes.CDLL:
     _sig(lib, "cc_hip_gather_u32_device", c_int, [V, V, c_u64, V, V])
     _sig(lib, "cc_hip_exclusive_scan_u32_device", c_int, [V, c_u64, V, V])
     _sig(lib, "cc_hip_delta_unpack_device", c_int, [V, c_u64, P_int, P_int, c_int, V, V, V])
+    _sig(lib, "cc_hip_delta_prepare_device", c_int, [V, c_u64, V, c_u32, c_u32, V, V])
+    _sig(lib, "cc_hip_delta_pack_reps_device", c_int,
+         [V, V, c_u64, P_int, c_int, V, V, V, P_int, P_int, P_u64, V])
+    _sig(lib, "cc_hip_components_device_noalloc", c_int, [V, V, c_u32, c_u64, V, V])
     _gpu = lib
     return lib
 

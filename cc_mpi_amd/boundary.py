@@ -97,8 +97,11 @@ class PinnedBuffer:
 
 def delta_pack_device(d_send_to_flat: DeviceBuffer, total_send_to: int, seg_start,
                       d_comp_label: DeviceBuffer, d_comp_of: DeviceBuffer, v_start: int,
-                      v_end: int, d_prev_sent: DeviceBuffer, sendbuf: PinnedBuffer, stream=None):
+                      v_end: int, d_prev_sent: DeviceBuffer, sendbuf: PinnedBuffer, stream=None,
+                      d_rep_of_send: DeviceBuffer | None = None):
     """Delta pack (src/exchange_plan.c:325-357) on device-resident arrays.
+    With d_rep_of_send (delta_prepare_device), the send entries' reps are
+    read from it instead of gathered from d_comp_of.
 
     Returns (pairs view into sendbuf, sendcounts, sdispls)."""
     seg = np.ascontiguousarray(seg_start, dtype=np.int32)
@@ -106,8 +109,25 @@ def delta_pack_device(d_send_to_flat: DeviceBuffer, total_send_to: int, seg_star
     cnt = np.zeros(max(1, nseg), dtype=np.int32)
     dis = np.zeros(max(1, nseg), dtype=np.int32)
     tot = ctypes.c_uint64(0)
-    nat.check(nat.gpu_lib().cc_hip_delta_pack_device(
-        d_send_to_flat.ptr, total_send_to, nat.ptr(seg, ctypes.c_int), nseg, d_comp_label.ptr,
-        d_comp_of.ptr, v_start, v_end, d_prev_sent.ptr, sendbuf.ptr, nat.ptr(cnt, ctypes.c_int),
-        nat.ptr(dis, ctypes.c_int), ctypes.byref(tot), stream), "cc_hip_delta_pack_device")
+    lib = nat.gpu_lib()
+    if d_rep_of_send is not None:
+        nat.check(lib.cc_hip_delta_pack_reps_device(
+            d_send_to_flat.ptr, d_rep_of_send.ptr, total_send_to, nat.ptr(seg, ctypes.c_int), nseg,
+            d_comp_label.ptr, d_prev_sent.ptr, sendbuf.ptr, nat.ptr(cnt, ctypes.c_int),
+            nat.ptr(dis, ctypes.c_int), ctypes.byref(tot), stream), "cc_hip_delta_pack_reps_device")
+    else:
+        nat.check(lib.cc_hip_delta_pack_device(
+            d_send_to_flat.ptr, total_send_to, nat.ptr(seg, ctypes.c_int), nseg, d_comp_label.ptr,
+            d_comp_of.ptr, v_start, v_end, d_prev_sent.ptr, sendbuf.ptr,
+            nat.ptr(cnt, ctypes.c_int), nat.ptr(dis, ctypes.c_int), ctypes.byref(tot), stream),
+            "cc_hip_delta_pack_device")
     return sendbuf.array[:int(tot.value)], cnt[:nseg], dis[:nseg]
+
+
+def delta_prepare_device(d_send_to_flat: DeviceBuffer, total_send_to: int,
+                         d_comp_of: DeviceBuffer, v_start: int, v_end: int,
+                         d_rep_of_send: DeviceBuffer, stream=None) -> None:
+    """The send entries' reps gathered once (cc_hip_delta_prepare_device)."""
+    nat.check(nat.gpu_lib().cc_hip_delta_prepare_device(
+        d_send_to_flat.ptr, total_send_to, d_comp_of.ptr, v_start, v_end, d_rep_of_send.ptr,
+        stream), "cc_hip_delta_prepare_device")

@@ -74,6 +74,9 @@ extern int g_split_staged;
 extern int g_compact_rows;
 extern int g_dedup_staged;
 extern int g_dedup_items;  // items per thread of the dedup compaction (16, 32)
+// Passes of the delta pack: 3 (count, scan, emit) or 1 (decoupled look-back).
+int delta_passes_set(int v);
+int delta_passes_get();
 
 // Returns the context, initialising device 0 on first use.  nullptr on
 // failure (error recorded).

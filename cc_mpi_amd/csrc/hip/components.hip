@@ -79,6 +79,13 @@ struct Knobs {
   int rem = -1;                    // one-pass hook by Rem's unions (-1 = auto)
   int rem_splice = -1;             // splicing in Rem's unions (-1 = auto)
   int giant_early = 1;             // two-pass path: sample the giant after round 0, skip it in round 1
+  // non-temporal streaming (bitmask; -1 = auto): 1 the init pass stores the
+  // sample arrays non-temporally, 2 it loads the column ids
+  // non-temporally, 4 the one-pass hook loads the sample arrays
+  // non-temporally
+  int nt = -1;
+  int init_ilp = 0;  // rows per thread per step of the sampling init pass (0 = auto; 1, 2, 4)
+  int frontier_chunks = 0;  // 64-vertex chunks per wave step of the frontier sweep (0 = auto; 1, 2, 4)
 };
 // Process-wide settings (cc_hip_components_set / _tune); the entry point
 // without allocations always runs with the defaults.
@@ -112,20 +119,6 @@ __device__ __forceinline__ uint32_t find_root_l2(uint32_t x, const uint32_t *__r
     x = px;
     px = load_l2(&p[x]);
   }
-  return x;
-}
-
-// Root of x, and whether the walk passed through (or ended at) g.
-__device__ __forceinline__ uint32_t find_root_via(uint32_t x, const uint32_t *__restrict__ p,
-                                                  uint32_t g, bool &via_g) {
-  bool hit = (x == g);
-  uint32_t px = p[x];
-  while (px != x) {
-    x = px;
-    hit |= (x == g);
-    px = p[x];
-  }
-  via_g = hit;
   return x;
 }
 
@@ -412,7 +405,10 @@ __global__ __launch_bounds__(kBlock) void k_compress_ilp(uint32_t n, uint32_t *_
 // stream 4 bytes per vertex instead of gathering a line plus an offset.
 constexpr uint32_t kNoNeighbour = 0xFFFFFFFFu;
 
-template <int Rounds>
+// Nt: bit 1 stores the sample arrays non-temporally, bit 2 loads the
+// column ids non-temporally (both are streamed once; the parent array
+// written here is what the hook gathers from next).
+template <int Rounds, int Nt = 0>
 __global__ __launch_bounds__(kBlock) void k_init_sample(const uint64_t *__restrict__ row_ptr,
                                                         const uint32_t *__restrict__ col,
                                                         uint32_t n, uint32_t *__restrict__ p,
@@ -422,10 +418,57 @@ __global__ __launch_bounds__(kBlock) void k_init_sample(const uint64_t *__restri
     const uint64_t e = row_ptr[v + 1];
     uint32_t w[Rounds];
 #pragma unroll
-    for (int r = 0; r < Rounds; ++r) w[r] = (b + r < e) ? col[b + r] : kNoNeighbour;
+    for (int r = 0; r < Rounds; ++r)
+      w[r] = (b + r < e) ? ((Nt & 2) ? __builtin_nontemporal_load(&col[b + r]) : col[b + r])
+                         : kNoNeighbour;
     p[v] = w[0] < v ? w[0] : v;
 #pragma unroll
-    for (int r = 0; r < Rounds; ++r) s[(size_t)r * n + v] = w[r];
+    for (int r = 0; r < Rounds; ++r) {
+      if (Nt & 1) __builtin_nontemporal_store(w[r], &s[(size_t)r * n + v]);
+      else s[(size_t)r * n + v] = w[r];
+    }
+  }
+}
+
+// k_init_sample<2, Nt> with ILP rows per thread per step (v, v + S, ...,
+// S = grid size): every row's offsets, then every row's two column ids,
+// are loaded before any is used, so ILP dependent chains overlap.
+template <int Nt, int ILP>
+__global__ __launch_bounds__(kBlock) void k_init_sample2_ilp(const uint64_t *__restrict__ row_ptr,
+                                                             const uint32_t *__restrict__ col,
+                                                             uint32_t n, uint32_t *__restrict__ p,
+                                                             uint32_t *__restrict__ s) {
+  const uint64_t S = (uint64_t)gridDim.x * kBlock;
+  for (uint64_t v0 = (uint64_t)blockIdx.x * kBlock + threadIdx.x; v0 < n; v0 += S * ILP) {
+    uint64_t b[ILP], e[ILP];
+#pragma unroll
+    for (int k = 0; k < ILP; ++k) {
+      const uint64_t v = v0 + k * S;
+      b[k] = v < n ? row_ptr[v] : 0;
+      e[k] = v < n ? row_ptr[v + 1] : 0;
+    }
+    uint32_t w0[ILP], w1[ILP];
+#pragma unroll
+    for (int k = 0; k < ILP; ++k) {
+      w0[k] = b[k] < e[k] ? ((Nt & 2) ? __builtin_nontemporal_load(&col[b[k]]) : col[b[k]])
+                          : kNoNeighbour;
+      w1[k] = b[k] + 1 < e[k]
+                  ? ((Nt & 2) ? __builtin_nontemporal_load(&col[b[k] + 1]) : col[b[k] + 1])
+                  : kNoNeighbour;
+    }
+#pragma unroll
+    for (int k = 0; k < ILP; ++k) {
+      const uint64_t v = v0 + k * S;
+      if (v >= n) continue;
+      p[v] = w0[k] < (uint32_t)v ? w0[k] : (uint32_t)v;
+      if (Nt & 1) {
+        __builtin_nontemporal_store(w0[k], &s[v]);
+        __builtin_nontemporal_store(w1[k], &s[(size_t)n + v]);
+      } else {
+        s[v] = w0[k];
+        s[(size_t)n + v] = w1[k];
+      }
+    }
   }
 }
 
@@ -469,7 +512,8 @@ __global__ __launch_bounds__(kBlock) void k_hook_cached(const uint32_t *__restri
 // once and hooked with the roots of neighbours 0 and 1 in turn, so the
 // pass between the rounds (and its pointer-jumping launch) disappears.
 // Halve: the three root walks halve the paths they follow.
-template <int Mode, int ILP, bool Halve = false>
+// Nt: the sample arrays are loaded non-temporally (streamed once).
+template <int Mode, int ILP, bool Halve = false, bool Nt = false>
 __global__ __launch_bounds__(kBlock) void k_hook_pair(const uint32_t *__restrict__ s0,
                                                       const uint32_t *__restrict__ s1,
                                                       uint32_t n, uint32_t *__restrict__ p) {
@@ -480,8 +524,8 @@ __global__ __launch_bounds__(kBlock) void k_hook_pair(const uint32_t *__restrict
 #pragma unroll
     for (int k = 0; k < ILP; ++k) {
       const uint64_t v = v0 + k * S;
-      const uint32_t w0 = v < n ? s0[v] : kNoNeighbour;
-      const uint32_t w1 = v < n ? s1[v] : kNoNeighbour;
+      const uint32_t w0 = v < n ? (Nt ? __builtin_nontemporal_load(&s0[v]) : s0[v]) : kNoNeighbour;
+      const uint32_t w1 = v < n ? (Nt ? __builtin_nontemporal_load(&s1[v]) : s1[v]) : kNoNeighbour;
       // neighbour 0 below v was linked by the init kernel already
       act[ILP + k] = w0 < n && w0 > (uint32_t)v;
       act[2 * ILP + k] = w1 < n && w1 != (uint32_t)v;
@@ -556,13 +600,13 @@ __device__ __forceinline__ void rem_unite(const uint32_t v, const uint32_t w,
 // pass, no lockstep: each union ends as soon as the two paths meet, which
 // on graphs whose trees are shallow and hub-centred is well before the
 // roots).
-template <int Splice>
+template <int Splice, bool Nt = false>
 __global__ __launch_bounds__(kBlock) void k_hook_rem(const uint32_t *__restrict__ s0,
                                                      const uint32_t *__restrict__ s1, uint32_t n,
                                                      uint32_t *__restrict__ p) {
   CCX_GRID_STRIDE(v, n) {
-    const uint32_t w0 = s0[v];
-    const uint32_t w1 = s1[v];
+    const uint32_t w0 = Nt ? __builtin_nontemporal_load(&s0[v]) : s0[v];
+    const uint32_t w1 = Nt ? __builtin_nontemporal_load(&s1[v]) : s1[v];
     // neighbour 0 below v was linked by the init kernel already
     if (w0 < n && w0 > v) rem_unite<Splice>(v, w0, p);
     if (w1 < n && w1 != v) rem_unite<Splice>(v, w1, p);
@@ -638,7 +682,13 @@ __global__ __launch_bounds__(kGiantSamples) void k_sample_giant(uint32_t n, cons
 // slots[0] = g; slots[1] (cleared by k_sample_giant) is set if any hook
 // here linked a root: only then can a written-back root be stale, so the
 // last pointer-jumping pass is gated on it.
-template <int Mode>
+// Chunks: each wave takes Chunks consecutive chunks of 64 vertices per
+// step and walks all their roots in lockstep (Chunks dependent chains per
+// lane in flight: on a graph whose sampled forest already holds nearly
+// every vertex under g, the sweep is nothing but these walks); then each
+// chunk with an active lane sweeps its lanes' remaining neighbours, and a
+// chunk with none skips the prefix sums.
+template <int Mode, int Chunks>
 __global__ __launch_bounds__(kBlock) void k_hook_frontier(const uint64_t *__restrict__ row_ptr,
                                                           const uint32_t *__restrict__ col,
                                                           uint32_t n, uint32_t k_first,
@@ -651,53 +701,77 @@ __global__ __launch_bounds__(kBlock) void k_hook_frontier(const uint64_t *__rest
   const uint32_t giant = slots[0];
   bool linked = false;
   const uint64_t waves = (uint64_t)gridDim.x * (kBlock / kWave);
-  for (uint64_t chunk = (uint64_t)blockIdx.x * (kBlock / kWave) + wid; chunk * kWave < n;
-       chunk += waves) {
-    const uint64_t base = chunk * kWave;
-    const uint64_t v = base + lane;
-    uint64_t work = 0, begin = 0;
-    if (v < n) {
-      const uint32_t pv = p[v];
-      bool in_giant = false;
-      const uint32_t r = find_root_via((uint32_t)v, p, giant, in_giant);
-      if (r != pv) p[v] = r;
-      if (!in_giant) {
-        const uint64_t b = row_ptr[v] + k_first;
-        const uint64_t e = row_ptr[v + 1];
-        if (e > b) {
-          work = e - b;
-          begin = b;
-        }
+  for (uint64_t c0 = ((uint64_t)blockIdx.x * (kBlock / kWave) + wid) * Chunks; c0 * kWave < n;
+       c0 += waves * Chunks) {
+    // root walks of every chunk of this step, in lockstep
+    uint32_t cur[Chunks], nxt[Chunks], pv[Chunks];
+    bool hit[Chunks];
+#pragma unroll
+    for (int j = 0; j < Chunks; ++j) {
+      const uint64_t v = (c0 + j) * kWave + lane;
+      cur[j] = (uint32_t)v;
+      pv[j] = v < n ? p[v] : (uint32_t)v;
+      nxt[j] = pv[j];
+      hit[j] = (v == giant);
+    }
+    bool more = true;
+    while (more) {
+      more = false;
+#pragma unroll
+      for (int j = 0; j < Chunks; ++j) {
+        if (nxt[j] == cur[j]) continue;
+        cur[j] = nxt[j];
+        hit[j] |= (cur[j] == giant);
+        nxt[j] = p[cur[j]];
+        more = true;
       }
     }
-    // Wave-inclusive prefix sum of the remaining degrees.
-    uint64_t incl = work;
 #pragma unroll
-    for (int d = 1; d < kWave; d <<= 1) {
-      const uint64_t up = __shfl_up(incl, d, kWave);
-      if (lane >= d) incl += up;
-    }
-    const uint64_t total = __shfl(incl, kWave - 1, kWave);
-    if (total == 0) continue;
-    s_excl[wid][lane] = incl - work;
-    s_begin[wid][lane] = begin;
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-    __builtin_amdgcn_wave_barrier();
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-    const uint64_t *ex = s_excl[wid];
-    for (uint64_t e = lane; e < total; e += kWave) {
-      int j = 0;
+    for (int j = 0; j < Chunks; ++j) {
+      const uint64_t base = (c0 + j) * kWave;
+      const uint64_t v = base + lane;
+      if (base >= n) break;
+      uint64_t work = 0, begin = 0;
+      if (v < n) {
+        if (cur[j] != pv[j]) p[v] = cur[j];
+        if (!hit[j]) {
+          const uint64_t b = row_ptr[v] + k_first;
+          const uint64_t e = row_ptr[v + 1];
+          if (e > b) {
+            work = e - b;
+            begin = b;
+          }
+        }
+      }
+      if (__ballot(work != 0) == 0) continue;  // wave-uniform
+      // Wave-inclusive prefix sum of the remaining degrees.
+      uint64_t incl = work;
 #pragma unroll
-      for (int step = kWave / 2; step > 0; step >>= 1)
-        if (ex[j + step] <= e) j += step;
-      const uint64_t idx = s_begin[wid][j] + (e - ex[j]);
-      const uint32_t w = col[idx];
-      if (w < n && hook<(Mode == 3 ? 1 : Mode)>((uint32_t)(base + j), w, p)) linked = true;
+      for (int d = 1; d < kWave; d <<= 1) {
+        const uint64_t up = __shfl_up(incl, d, kWave);
+        if (lane >= d) incl += up;
+      }
+      const uint64_t total = __shfl(incl, kWave - 1, kWave);
+      s_excl[wid][lane] = incl - work;
+      s_begin[wid][lane] = begin;
+      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+      __builtin_amdgcn_wave_barrier();
+      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+      const uint64_t *ex = s_excl[wid];
+      for (uint64_t e = lane; e < total; e += kWave) {
+        int jj = 0;
+#pragma unroll
+        for (int step = kWave / 2; step > 0; step >>= 1)
+          if (ex[jj + step] <= e) jj += step;
+        const uint64_t idx = s_begin[wid][jj] + (e - ex[jj]);
+        const uint32_t w = col[idx];
+        if (w < n && hook<(Mode == 3 ? 1 : Mode)>((uint32_t)(base + jj), w, p)) linked = true;
+      }
+      // the LDS rows are rewritten by the next chunk of this wave
+      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+      __builtin_amdgcn_wave_barrier();
+      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
     }
-    // the LDS rows are rewritten by the next chunk of this wave
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-    __builtin_amdgcn_wave_barrier();
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
   }
   if (linked) slots[1] = 1;
 }
@@ -741,6 +815,9 @@ struct Plan {
   int halve;     // path halving in the lockstep one-pass hook (k_hook_pair)
   int hook_ilp;  // roots per thread in the lockstep sampled hook
   int hook_bpc;  // sampled-hook grid cap, workgroups per CU
+  int nt;        // non-temporal streaming bits (Knobs::nt)
+  int init_ilp;  // rows per thread per step of the sampling init pass
+  int frontier_chunks;  // chunks per wave step of the frontier sweep
 };
 
 inline Plan plan_for(uint32_t n, uint64_t m, const Knobs &K = g_knobs) {
@@ -757,6 +834,19 @@ inline Plan plan_for(uint32_t n, uint64_t m, const Knobs &K = g_knobs) {
   pl.hook_ilp = pl.rem ? 1 : (K.hook_ilp ? K.hook_ilp : (dense ? wide : 1));
   const int bpc_auto = pl.rem ? (pl.splice == 1 ? 4 : 8) : (pl.hook_ilp >= 2 ? 8 : 4);
   pl.hook_bpc = K.blocks_per_cu ? K.blocks_per_cu : bpc_auto;
+  // From 2^24 vertices the parent array no longer fits the L2s, and the
+  // sample arrays streamed through the MALL would evict it: both are
+  // stored and read non-temporally (grid 8192^2 5.59 -> 5.15 ms per solve,
+  // Chung-Lu 65.6M 3.41 -> 3.36 ms; below that they cost, R-MAT 22 hook
+  // 91 -> 96 us).  Dense graphs of that size also take 4 rows per thread
+  // in the init pass (Chung-Lu 65.6M 3.36 -> 3.28 ms; the grid gains
+  // nothing).
+  pl.nt = K.nt >= 0 ? K.nt : (big ? 5 : 0);
+  pl.init_ilp = K.init_ilp ? K.init_ilp : (big && dense ? 4 : 1);
+  // two 64-vertex chunks per wave step of the frontier sweep: Chung-Lu
+  // 65.6M 195 -> 146 us, R-MAT 22 33 -> 29 us, R-MAT 24 80 -> 71 us, ER 32M
+  // 103 -> 99 us; four lose on every graph (register pressure)
+  pl.frontier_chunks = K.frontier_chunks ? K.frontier_chunks : 2;
   return pl;
 }
 
@@ -780,6 +870,9 @@ extern "C" int cc_hip_components_set(const char *key, int value) {
   else if (k == "sample_cache" && (value == 0 || value == 1)) g_knobs.sample_cache = value;
   else if (k == "fused_pair" && value >= -1 && value <= 1) g_knobs.fused_pair = value;
   else if (k == "giant_early" && (value == 0 || value == 1)) g_knobs.giant_early = value;
+  else if (k == "nt" && value >= -1 && value <= 7) g_knobs.nt = value;
+  else if (k == "init_ilp" && (value == 0 || ilp_ok(value))) g_knobs.init_ilp = value;
+  else if (k == "frontier_chunks" && (value == 0 || ilp_ok(value))) g_knobs.frontier_chunks = value;
   else return ccx::set_error(4, "cc_hip_components_set: bad key or value (%s=%d)", key, value);
   return 0;
 }
@@ -799,6 +892,9 @@ extern "C" int cc_hip_components_get(const char *key) {
   if (k == "halve") return g_knobs.halve;
   if (k == "rem") return g_knobs.rem;
   if (k == "rem_splice") return g_knobs.rem_splice;
+  if (k == "nt") return g_knobs.nt;
+  if (k == "init_ilp") return g_knobs.init_ilp;
+  if (k == "frontier_chunks") return g_knobs.frontier_chunks;
   return -1;
 }
 
@@ -824,6 +920,9 @@ extern "C" int cc_hip_components_plan_get(uint32_t n, uint64_t m, const char *ke
   if (k == "rem") return pl.rem;
   if (k == "rem_splice") return pl.splice;
   if (k == "hook_ilp") return pl.hook_ilp;
+  if (k == "nt") return pl.nt;
+  if (k == "init_ilp") return pl.init_ilp;
+  if (k == "frontier_chunks") return pl.frontier_chunks;
   if (k == "hook_blocks_per_cu") return pl.hook_bpc;
   if (k == "giant_early") return pl.cache && !pl.fused && g_knobs.giant_early;
   if (k == "hook_mode") return g_knobs.hook_mode;
@@ -891,7 +990,31 @@ int solve_with(const Knobs &K, const ccx::Context *c, const uint64_t *d_row_ptr,
     if (!smp) return ccx::set_error(4, "solve: no sample buffer");
     switch (rounds) {
       case 1: CCX_LAUNCH("cc_init_min", s, k_init_sample<1><<<ns, kBlock, 0, s>>>(d_row_ptr, d_col_idx, n, d_labels, smp)); break;
-      case 2: CCX_LAUNCH("cc_init_min", s, k_init_sample<2><<<ns, kBlock, 0, s>>>(d_row_ptr, d_col_idx, n, d_labels, smp)); break;
+      case 2:
+        if (plan.init_ilp == 2 || plan.init_ilp == 4) {
+#define CCX_INIT2(NT)                                                                              \
+  do {                                                                                             \
+    if (plan.init_ilp == 4)                                                                        \
+      CCX_LAUNCH("cc_init_min", s, k_init_sample2_ilp<NT, 4><<<ns, kBlock, 0, s>>>(d_row_ptr, d_col_idx, n, d_labels, smp)); \
+    else                                                                                           \
+      CCX_LAUNCH("cc_init_min", s, k_init_sample2_ilp<NT, 2><<<ns, kBlock, 0, s>>>(d_row_ptr, d_col_idx, n, d_labels, smp)); \
+  } while (0)
+          switch (plan.nt & 3) {
+            case 1: CCX_INIT2(1); break;
+            case 2: CCX_INIT2(2); break;
+            case 3: CCX_INIT2(3); break;
+            default: CCX_INIT2(0); break;
+          }
+#undef CCX_INIT2
+          break;
+        }
+        switch (plan.nt & 3) {
+          case 1: CCX_LAUNCH("cc_init_min", s, k_init_sample<2, 1><<<ns, kBlock, 0, s>>>(d_row_ptr, d_col_idx, n, d_labels, smp)); break;
+          case 2: CCX_LAUNCH("cc_init_min", s, k_init_sample<2, 2><<<ns, kBlock, 0, s>>>(d_row_ptr, d_col_idx, n, d_labels, smp)); break;
+          case 3: CCX_LAUNCH("cc_init_min", s, k_init_sample<2, 3><<<ns, kBlock, 0, s>>>(d_row_ptr, d_col_idx, n, d_labels, smp)); break;
+          default: CCX_LAUNCH("cc_init_min", s, k_init_sample<2><<<ns, kBlock, 0, s>>>(d_row_ptr, d_col_idx, n, d_labels, smp)); break;
+        }
+        break;
       case 3: CCX_LAUNCH("cc_init_min", s, k_init_sample<3><<<ns, kBlock, 0, s>>>(d_row_ptr, d_col_idx, n, d_labels, smp)); break;
       default: CCX_LAUNCH("cc_init_min", s, k_init_sample<4><<<ns, kBlock, 0, s>>>(d_row_ptr, d_col_idx, n, d_labels, smp)); break;
     }
@@ -915,12 +1038,22 @@ int solve_with(const Knobs &K, const ccx::Context *c, const uint64_t *d_row_ptr,
   } while (0)
     if (mode == 0)
       CCX_LAUNCH("cc_hook_sample", s, k_hook_pair<0, 2><<<nb, kBlock, 0, s>>>(smp, s1, n, d_labels));
+    else if (plan.rem && plan.splice == 1 && (plan.nt & 4))
+      CCX_LAUNCH("cc_hook_sample", s, k_hook_rem<1, true><<<nb, kBlock, 0, s>>>(smp, s1, n, d_labels));
     else if (plan.rem && plan.splice == 1)
       CCX_LAUNCH("cc_hook_sample", s, k_hook_rem<1><<<nb, kBlock, 0, s>>>(smp, s1, n, d_labels));
+    else if (plan.rem && plan.splice == 2 && (plan.nt & 4))
+      CCX_LAUNCH("cc_hook_sample", s, k_hook_rem<2, true><<<nb, kBlock, 0, s>>>(smp, s1, n, d_labels));
     else if (plan.rem && plan.splice == 2)
       CCX_LAUNCH("cc_hook_sample", s, k_hook_rem<2><<<nb, kBlock, 0, s>>>(smp, s1, n, d_labels));
     else if (plan.rem)
       CCX_LAUNCH("cc_hook_sample", s, k_hook_rem<0><<<nb, kBlock, 0, s>>>(smp, s1, n, d_labels));
+    else if (mode == 4 && plan.halve && (plan.nt & 4) && hook_ilp == 4)
+      CCX_LAUNCH("cc_hook_sample", s, k_hook_pair<4, 4, true, true><<<nb, kBlock, 0, s>>>(smp, s1, n, d_labels));
+    else if (mode == 4 && plan.halve && (plan.nt & 4) && hook_ilp == 2)
+      CCX_LAUNCH("cc_hook_sample", s, k_hook_pair<4, 2, true, true><<<nb, kBlock, 0, s>>>(smp, s1, n, d_labels));
+    else if (mode == 4 && plan.halve && (plan.nt & 4))
+      CCX_LAUNCH("cc_hook_sample", s, k_hook_pair<4, 1, true, true><<<nb, kBlock, 0, s>>>(smp, s1, n, d_labels));
     else if (mode == 1 && plan.halve) CCX_HOOK_PAIR(1, true);
     else if (mode == 1) CCX_HOOK_PAIR(1, false);
     else if (plan.halve) CCX_HOOK_PAIR(4, true);
@@ -977,16 +1110,21 @@ int solve_with(const Knobs &K, const ccx::Context *c, const uint64_t *d_row_ptr,
   // so the sweep starts after them; lossy sampling (mode 3) links nothing
   // for sure, so the sweep covers every neighbour.
   const uint32_t first = (mode == 3) ? 0u : rounds;
-  if (mode == 0)
-    CCX_LAUNCH("cc_hook_frontier", s, k_hook_frontier<0><<<nf, kBlock, 0, s>>>(d_row_ptr, d_col_idx, n, first, d_labels, giant));
-  else if (mode == 1)
-    CCX_LAUNCH("cc_hook_frontier", s, k_hook_frontier<1><<<nf, kBlock, 0, s>>>(d_row_ptr, d_col_idx, n, first, d_labels, giant));
-  else if (mode == 2)
-    CCX_LAUNCH("cc_hook_frontier", s, k_hook_frontier<2><<<nf, kBlock, 0, s>>>(d_row_ptr, d_col_idx, n, first, d_labels, giant));
-  else if (mode == 4)
-    CCX_LAUNCH("cc_hook_frontier", s, k_hook_frontier<4><<<nf, kBlock, 0, s>>>(d_row_ptr, d_col_idx, n, first, d_labels, giant));
-  else
-    CCX_LAUNCH("cc_hook_frontier", s, k_hook_frontier<3><<<nf, kBlock, 0, s>>>(d_row_ptr, d_col_idx, n, first, d_labels, giant));
+#define CCX_FRONTIER(M)                                                                            \
+  do {                                                                                             \
+    if (plan.frontier_chunks == 4)                                                                 \
+      CCX_LAUNCH("cc_hook_frontier", s, k_hook_frontier<M, 4><<<nf, kBlock, 0, s>>>(d_row_ptr, d_col_idx, n, first, d_labels, giant)); \
+    else if (plan.frontier_chunks == 2)                                                            \
+      CCX_LAUNCH("cc_hook_frontier", s, k_hook_frontier<M, 2><<<nf, kBlock, 0, s>>>(d_row_ptr, d_col_idx, n, first, d_labels, giant)); \
+    else                                                                                           \
+      CCX_LAUNCH("cc_hook_frontier", s, k_hook_frontier<M, 1><<<nf, kBlock, 0, s>>>(d_row_ptr, d_col_idx, n, first, d_labels, giant)); \
+  } while (0)
+  if (mode == 0) CCX_FRONTIER(0);
+  else if (mode == 1) CCX_FRONTIER(1);
+  else if (mode == 2) CCX_FRONTIER(2);
+  else if (mode == 4) CCX_FRONTIER(4);
+  else CCX_FRONTIER(3);
+#undef CCX_FRONTIER
   return compress(giant + 1);
 }
 

@@ -1100,6 +1100,11 @@ int g_split_dominant_rows = 32;  // rows of 64 records per wave step in the bitm
 }  // namespace
 
 extern "C" int cc_hip_sort_set(const char *key, int value) {
+  if (key && strcmp(key, "delta_passes") == 0) {
+    if (ccx::delta_passes_set(value) == 0) return 0;
+    return ccx::set_error(4, "cc_hip_sort_set: bad key or value (%s=%d)", key, value);
+  }
+
   if (key && strcmp(key, "items_per_thread") == 0 &&
       (value == 8 || value == 16 || value == 24 || value == 32)) {
     g_sort_items = value;
@@ -1138,6 +1143,8 @@ extern "C" int cc_hip_sort_set(const char *key, int value) {
 
 extern "C" int cc_hip_sort_get(const char *key) {
   if (key && strcmp(key, "items_per_thread") == 0) return g_sort_items;
+  if (key && strcmp(key, "delta_passes") == 0) return ccx::delta_passes_get();
+
   if (key && strcmp(key, "sort_threads") == 0) return g_sort_threads;
   if (key && strcmp(key, "split_staged") == 0) return ccx::g_split_staged;
   if (key && strcmp(key, "compact_rows") == 0) return ccx::g_compact_rows;

@@ -189,6 +189,9 @@ void compute_connected_components_mpi_advanced(const DistCSRGraph *Gd, uint32_t 
   uint32_t *d_reps = (uint32_t *)dalloc((size_t)n_local * 4, comm, "alloc reps");
   uint32_t rep_count = 0;
   check(cc_hip_local_reps_device(d_comp_of, n_local, d_reps, &rep_count, NULL), comm, "local reps");
+  /* comp_of is fixed from here on: the packs gather the send entries' reps once */
+  check(exchangeplan_device_prepare(&plan, &xd, d_comp_of, v_start, v_end), comm,
+        "exchange device prepare");
 
   /* 5. merge loop */
   MPI_Barrier(comm);

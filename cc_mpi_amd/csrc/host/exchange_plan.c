@@ -199,10 +199,21 @@ int exchangeplan_device_init(const ExchangePlan *P, ExchangeDevice *D)
   return 0;
 }
 
+int exchangeplan_device_prepare(const ExchangePlan *P, ExchangeDevice *D,
+                                const uint32_t *d_comp_of, uint32_t v_start, uint32_t v_end)
+{
+  const size_t ts = (size_t)P->total_send_to;
+  if (!D->d_rep_of_send && cc_hip_malloc((void **)&D->d_rep_of_send, ts * sizeof(uint32_t)))
+    return 1;
+  return cc_hip_delta_prepare_device(D->d_send_to_flat, ts, d_comp_of, v_start, v_end,
+                                     D->d_rep_of_send, NULL);
+}
+
 void exchangeplan_device_free(ExchangeDevice *D)
 {
   if (!D)
     return;
+  cc_hip_free(D->d_rep_of_send);
   cc_hip_free(D->d_send_to_flat);
   cc_hip_free(D->d_need_from_gidx_flat);
   cc_hip_host_free(D->h_sendbuf);
@@ -226,7 +237,13 @@ uint64_t exchangeplan_exchange_delta_device(const ExchangePlan *P, ExchangeDevic
     return 0;
   const int outd = P->outdegree, ind = P->indegree;
   uint64_t sent = 0;
-  if (outd > 0)
+  if (outd > 0 && D->d_rep_of_send)
+    hip_or_die(cc_hip_delta_pack_reps_device(D->d_send_to_flat, D->d_rep_of_send,
+                                             (uint64_t)P->total_send_to, D->seg_start, outd,
+                                             d_comp_label, d_prev_sent, D->h_sendbuf,
+                                             D->sendcounts, D->sdispls, &sent, NULL),
+               P->comm_graph, "exchange delta pack");
+  else if (outd > 0)
     hip_or_die(cc_hip_delta_pack_device(D->d_send_to_flat, (uint64_t)P->total_send_to,
                                         D->seg_start, outd, d_comp_label, d_comp_of, v_start,
                                         v_end, d_prev_sent, D->h_sendbuf, D->sendcounts,

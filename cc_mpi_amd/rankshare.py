@@ -25,7 +25,7 @@ import time
 import numpy as np
 
 from . import _native as nat
-from .boundary import PinnedBuffer, delta_pack_device, ghost_extract_device
+from .boundary import PinnedBuffer, delta_pack_device, delta_prepare_device, ghost_extract_device
 from .cc import DeviceBuffer, DeviceGraph
 
 
@@ -116,6 +116,12 @@ class RankShare:
         self._t("ghost_extract", t0)
         for b in (srt, d_bounds):
             b.close()
+        # comp_of is fixed from here on: the packs read the send entries'
+        # reps from a list gathered once
+        t0 = time.perf_counter()
+        self.rep_of_send = DeviceBuffer(4 * max(1, nsend))
+        delta_prepare_device(self.send_to_flat, nsend, self.comp_of, vs, ve, self.rep_of_send)
+        self._t("delta_prepare", t0)
         self.stats.update(n_local=nl, m_local_local=self.m_ll, boundary_raw=self.nb,
                           boundary_dedup=self.nbd, ghosts=self.ng, total_send_to=nsend,
                           outdegree=len(self.dests))
@@ -201,11 +207,12 @@ class RankShare:
         return self._sched
 
     def merge(self, truth_labels: DeviceBuffer, exchange_interval: int = 1, seed: int = 1,
-              check_pack=None, max_iters: int = 100000) -> np.ndarray:
+              check_pack=None, max_iters: int = 100000, use_reps: bool = True) -> np.ndarray:
         """Run the merge loop with injected ghost deltas; returns the owned
         labels (comp_label[comp_of[li]]).  check_pack(seg_start, comp_label,
         comp_of, prev_before, pairs, counts, displs) is called per exchange
-        if given."""
+        if given.  use_reps: the packs read the reps gathered once after
+        local CC (else each pack gathers them from comp_of)."""
         lib = nat.gpu_lib()
         nl, vs, ve, ng = self.nl, self.vs, self.ve, self.ng
         interval = max(1, exchange_interval)
@@ -263,7 +270,8 @@ class RankShare:
                     t0 = time.perf_counter()
                     pairs, counts, displs = delta_pack_device(
                         self.send_to_flat, self.total_send, self.seg_start, comp_label,
-                        self.comp_of, vs, ve, prev_sent, sendbuf)
+                        self.comp_of, vs, ve, prev_sent, sendbuf,
+                        d_rep_of_send=self.rep_of_send if use_reps else None)
                     emitted_total += int(pairs.size)
                     packs += 1
                     t_pack += time.perf_counter() - t0
@@ -310,7 +318,7 @@ class RankShare:
             sched["pairs"].close()
             sched["iota"].close()
             self._sched = None
-        for name in ("comp_of", "send_to_flat", "ghosts", "bnd"):
+        for name in ("comp_of", "send_to_flat", "ghosts", "bnd", "rep_of_send"):
             b = getattr(self, name, None)
             if b is not None:
                 b.close()

@@ -148,6 +148,19 @@ int cc_hip_components_tune(int sampled_neighbours, int blocks_per_cu, int hook_m
  *   "stream_blocks_per_cu"    grid cap of init/compress (default 8)
  *   "stream_ilp"              vertices per thread per step in init/compress
  *                             without the cache (1 default, 2, 4)
+ *   "nt"                      non-temporal streaming, a bitmask: 1 the init
+ *                             pass stores the sample arrays non-temporally,
+ *                             2 it loads the column ids non-temporally, 4 the
+ *                             one-pass hook loads the sample arrays
+ *                             non-temporally; -1 (default) = 5 when
+ *                             n >= 2^24, else 0
+ *   "init_ilp"                rows per thread per step of the sampling init
+ *                             pass (two sampled neighbours): 1, 2, 4, or 0
+ *                             (default) = 4 when n >= 2^24 and m >= 16 n,
+ *                             else 1
+ *   "frontier_chunks"         64-vertex chunks per wave step of the frontier
+ *                             sweep, their root walks in lockstep: 1, 2, 4, or
+ *                             0 (default) = 2
  * Returns nonzero for an unknown key or out-of-range value; _get returns -1
  * for an unknown key. */
 int cc_hip_components_set(const char *key, int value);
@@ -160,7 +173,8 @@ int cc_hip_components_plan(uint32_t n, uint64_t m, int *cache, int *fused, int *
                            int *hook_blocks_per_cu);
 /* One field of that plan by name: "cache", "fused", "rem", "rem_splice",
  * "halve", "hook_ilp", "hook_blocks_per_cu", "giant_early" (in effect for
- * this graph), "hook_mode".  Returns -1 for an unknown key. */
+ * this graph), "hook_mode", "nt", "init_ilp", "frontier_chunks".  Returns -1
+ * for an unknown key. */
 int cc_hip_components_plan_get(uint32_t n, uint64_t m, const char *key);
 /* Whether the last solve on `stream` (NULL = the library stream) ran its
  * final pointer-jumping pass: *ran = 1 if its frontier sweep linked a root,
@@ -263,6 +277,25 @@ int cc_hip_delta_pack_device(const uint32_t *d_send_to_flat, uint64_t total_send
                              uint32_t v_start, uint32_t v_end, uint32_t *d_prev_sent,
                              uint64_t *h_sendbuf, int *sendcounts, int *sdispls,
                              uint64_t *total_pairs, void *stream);
+
+/* The owned entries' reps gathered once: d_rep_of_send[i] =
+ * d_comp_of[d_send_to_flat[i] - v_start] for every i with the id in
+ * [v_start, v_end), 0xFFFFFFFF otherwise.  comp_of does not change during
+ * the merge loop (src/cc_mpi.c:541-710), so the list serves every exchange.
+ * Asynchronous on `stream`. */
+int cc_hip_delta_prepare_device(const uint32_t *d_send_to_flat, uint64_t total_send_to,
+                                const uint32_t *d_comp_of, uint32_t v_start, uint32_t v_end,
+                                uint32_t *d_rep_of_send, void *stream);
+
+/* cc_hip_delta_pack_device with the reps from cc_hip_delta_prepare_device:
+ * lbl = comp_label[rep_of_send[idx]] (the id itself where the rep list holds
+ * 0xFFFFFFFF), so a pack streams the rep list and prev_sent and gathers
+ * only comp_label at the reps.  Same output, same contract. */
+int cc_hip_delta_pack_reps_device(const uint32_t *d_send_to_flat, const uint32_t *d_rep_of_send,
+                                  uint64_t total_send_to, const int *seg_start, int nseg,
+                                  const uint32_t *d_comp_label, uint32_t *d_prev_sent,
+                                  uint64_t *h_sendbuf, int *sendcounts, int *sdispls,
+                                  uint64_t *total_pairs, void *stream);
 
 /* Received-pair unpack (src/exchange_plan.c:390-413): pairs (pos << 32) |
  * lbl grouped per source k from rdispls[k] (h_pairs: pinned host or device

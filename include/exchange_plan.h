@@ -96,16 +96,27 @@ typedef struct
   int *seg_start;      /* outdegree+1 segment starts of send_to_flat in dests order */
   int *need_base;      /* need_from_displs of each source, in sources order */
   int *sendcounts, *sdispls, *recvcounts, *rdispls;
+  uint32_t *d_rep_of_send; /* send entries' reps (exchangeplan_device_prepare), or NULL */
 } ExchangeDevice;
 
 int exchangeplan_device_init(const ExchangePlan *P, ExchangeDevice *D);
 void exchangeplan_device_free(ExchangeDevice *D);
 
 /*
+ * Gather the send entries' reps once (cc_hip_delta_prepare_device) for a
+ * comp_of that stays fixed from here on, as it does through the merge loop:
+ * the packs of exchangeplan_exchange_delta_device then gather only
+ * comp_label at the reps.  Returns 0, or nonzero on a device failure.
+ */
+int exchangeplan_device_prepare(const ExchangePlan *P, ExchangeDevice *D,
+                                const uint32_t *d_comp_of, uint32_t v_start, uint32_t v_end);
+
+/*
  * exchangeplan_exchange_delta with comp_label, comp_of, prev_sent and
  * ghost_labels resident on the device: the pack compacts the changed pairs
  * in HBM and copies them into the pinned send buffer, MPI exchanges them, and the unpack
- * kernel scatters the received pairs into d_ghost_labels.  Returns the
+ * kernel scatters the received pairs into d_ghost_labels.  With a prepared
+ * rep list (exchangeplan_device_prepare) d_comp_of is not read.  Returns the
  * number of pairs this rank sent.
  */
 uint64_t exchangeplan_exchange_delta_device(const ExchangePlan *P, ExchangeDevice *D,

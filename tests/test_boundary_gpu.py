@@ -180,20 +180,40 @@ def delta_case(seed, n_send=50_000, nseg=4, empty=()):
     return stf, seg, comp_label, comp_of, vs, ve, prev
 
 
-@pytest.mark.parametrize("rows", [1, 0])
+# pack designs: three passes (default), or one look-back pass over
+# row-striped tiles or runs of items
+PACKS = [dict(delta_passes=3), dict(delta_passes=1, compact_rows=1),
+         dict(delta_passes=1, compact_rows=0)]
+
+
+@pytest.mark.parametrize("prepared", [False, True], ids=["gather", "reps"])
+@pytest.mark.parametrize("design", PACKS, ids=["three_pass", "lookback_rows", "lookback_runs"])
 @pytest.mark.parametrize("seed,nseg,empty", [(0, 1, ()), (1, 4, ()), (2, 5, (0, 2)),
                                              (3, 3, (2,)), (4, 200, ()), (5, 6, (5,))])
-def test_delta_pack_rounds(gpu, seed, nseg, empty, rows):
+def test_delta_pack_rounds(gpu, seed, nseg, empty, design, prepared):
+    """Four packs of lowered labels against the oracle's, gathering the reps
+    from comp_of each time or reading them from the list gathered once
+    (cc_hip_delta_prepare_device); send_to_flat also holds ids outside the
+    owned block, whose label is the id itself."""
     stf, seg, cl, co, vs, ve, prev = delta_case(seed, nseg=nseg, empty=empty)
     d_stf, d_cl, d_co = (DeviceBuffer.from_array(x) for x in (stf, cl, co))
     d_prev = DeviceBuffer.from_array(prev)
     sendbuf = bd.PinnedBuffer(stf.size)
+    d_rep = None
+    if prepared:
+        d_rep = DeviceBuffer(4 * stf.size)
+        bd.delta_prepare_device(d_stf, stf.size, d_co, vs, ve, d_rep)
+        rep = np.empty(stf.size, dtype=np.uint32)
+        d_rep.download(rep)
+        owned = (stf >= vs) & (stf < ve)
+        assert np.array_equal(rep[owned], co[stf[owned] - vs])
+        assert (rep[~owned] == 0xFFFFFFFF).all() and (~owned).any()
     rng = np.random.default_rng(seed + 100)
     prev_o = prev.copy()
     for rnd in range(4):
-        with sort_knobs(compact_rows=rows):
+        with sort_knobs(**design):
             pairs, cnt, dis = bd.delta_pack_device(d_stf, stf.size, seg, d_cl, d_co, vs, ve,
-                                                   d_prev, sendbuf)
+                                                   d_prev, sendbuf, d_rep_of_send=d_rep)
         wp, wc, wd = oracle.delta_pack(stf, seg, cl, co, vs, ve, prev_o)
         assert np.array_equal(pairs, wp), f"round {rnd}"
         assert np.array_equal(cnt, wc) and np.array_equal(dis, wd)
@@ -207,15 +227,15 @@ def test_delta_pack_rounds(gpu, seed, nseg, empty, rows):
     sendbuf.close()
 
 
-@pytest.mark.parametrize("rows", [1, 0])
-def test_delta_pack_empty_trailing_segments(gpu, rows):
+@pytest.mark.parametrize("design", PACKS, ids=["three_pass", "lookback_rows", "lookback_runs"])
+def test_delta_pack_empty_trailing_segments(gpu, design):
     """Destinations at the end with nothing asked: their segment starts equal
     total_send_to, and their counts and displacements still come out right."""
     stf, seg, cl, co, vs, ve, prev = delta_case(12, nseg=5)
     seg[3:] = stf.size
     d = [DeviceBuffer.from_array(x) for x in (stf, cl, co, prev)]
     sendbuf = bd.PinnedBuffer(stf.size)
-    with sort_knobs(compact_rows=rows):
+    with sort_knobs(**design):
         pairs, cnt, dis = bd.delta_pack_device(d[0], stf.size, seg, d[1], d[2], vs, ve, d[3],
                                                sendbuf)
     wp, wc, wd = oracle.delta_pack(stf, seg, cl, co, vs, ve, prev.copy())
@@ -224,13 +244,13 @@ def test_delta_pack_empty_trailing_segments(gpu, rows):
     sendbuf.close()
 
 
-@pytest.mark.parametrize("rows", [1, 0])
-def test_delta_pack_nothing_to_send(gpu, rows):
+@pytest.mark.parametrize("design", PACKS, ids=["three_pass", "lookback_rows", "lookback_runs"])
+def test_delta_pack_nothing_to_send(gpu, design):
     stf, seg, cl, co, vs, ve, prev = delta_case(9)
     prev[:] = 0  # everything already sent at label 0
     d = [DeviceBuffer.from_array(x) for x in (stf, cl, co, prev)]
     sendbuf = bd.PinnedBuffer(stf.size)
-    with sort_knobs(compact_rows=rows):
+    with sort_knobs(**design):
         pairs, cnt, dis = bd.delta_pack_device(d[0], stf.size, seg, d[1], d[2], vs, ve, d[3],
                                                sendbuf)
     assert pairs.size == 0 and cnt.sum() == 0 and (dis == 0).all()

@@ -231,9 +231,20 @@ PATHS = [  # every code path of the solve: (knobs) -> same labels
     dict(sample_cache=1, fused_pair=1, hook_mode=1, rem=1, rem_splice=2, blocks_per_cu=2),
     dict(sample_cache=1, fused_pair=1, hook_mode=4, rem=0, halve=0),
     dict(stream_ilp=2), dict(stream_ilp=4, sample_cache=0),
+    # non-temporal streaming and the init pass's rows per thread
+    dict(nt=1), dict(nt=2), dict(nt=3), dict(nt=7, init_ilp=2), dict(nt=5, init_ilp=4),
+    dict(nt=4, rem=1, rem_splice=1), dict(nt=4, rem=1, rem_splice=2),
+    dict(nt=5, rem=0, halve=1, hook_ilp=1), dict(nt=5, rem=0, halve=1, hook_ilp=2),
+    dict(nt=5, rem=0, halve=1, hook_ilp=4, init_ilp=4), dict(nt=0, init_ilp=4),
+    # frontier sweep: chunks per wave step, with every hook mode
+    dict(frontier_chunks=1), dict(frontier_chunks=2), dict(frontier_chunks=4),
+    dict(frontier_chunks=4, hook_mode=0), dict(frontier_chunks=2, hook_mode=1),
+    dict(frontier_chunks=4, hook_mode=2, sample_cache=0, fused_pair=0),
+    dict(frontier_chunks=4, hook_mode=3, sample_cache=0, fused_pair=0),
 ]
 DEFAULTS = dict(sample_cache=1, fused_pair=-1, hook_ilp=0, hook_mode=4, sampled_neighbours=2,
-                stream_ilp=1, blocks_per_cu=0, giant_early=1, halve=-1, rem=-1, rem_splice=-1)
+                stream_ilp=1, blocks_per_cu=0, giant_early=1, halve=-1, rem=-1, rem_splice=-1,
+                nt=-1, init_ilp=0, frontier_chunks=0)
 
 
 @pytest.mark.parametrize("knobs", PATHS, ids=lambda k: ",".join(f"{a}={b}" for a, b in k.items()))
