@@ -179,7 +179,7 @@ def kernel_bytes(name: str, n: int, st: dict, plan: dict) -> float | None:
 def solve_plan(lib, n: int, m: int) -> dict:
     """What the solve does on this graph (cc_hip_components_plan)."""
     keys = ("cache", "fused", "rem", "rem_splice", "halve", "hook_ilp", "hook_blocks_per_cu",
-            "giant_early", "hook_mode")
+            "giant_early", "hook_mode", "nt", "init_ilp", "frontier_chunks")
     return {k: int(lib.cc_hip_components_plan_get(n, m, k.encode())) for k in keys}
 
 
