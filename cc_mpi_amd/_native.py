@@ -122,6 +122,11 @@ def gpu_lib() -> ctypes.CDLL:
          [V, c_u64, V, V, c_u32, V, V, V, c_u32, c_u32, c_u32, P_int, V])
     _sig(lib, "cc_hip_merge_step_reps_device", c_int,
          [V, c_u64, V, V, c_u32, V, V, V, c_u32, V, c_u32, c_u32, c_u32, P_int, V])
+    _sig(lib, "cc_hip_merge_prepare_device", c_int,
+         [V, c_u64, c_u32, V, c_u32, V, V, P_u64, P_u32, V])
+    _sig(lib, "cc_hip_merge_step_prepared_device", c_int,
+         [V, c_u32, V, c_u64, V, V, c_u32, V, V, V, V, c_u32, V, c_u32, c_u32, c_u32, c_int,
+          P_int, V])
     _sig(lib, "cc_hip_local_reps_device", c_int, [V, c_u32, V, P_u32, V])
     _sig(lib, "cc_hip_rank_labels_device", c_int, [V, V, c_u32, c_u32, V, P_u64, V])
     _sig(lib, "cc_hip_owner_keys_device", c_int, [V, c_u64, V, c_int, V, V])

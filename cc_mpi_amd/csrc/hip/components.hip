@@ -86,6 +86,10 @@ struct Knobs {
   int nt = -1;
   int init_ilp = 0;  // rows per thread per step of the sampling init pass (0 = auto; 1, 2, 4)
   int frontier_chunks = 0;  // 64-vertex chunks per wave step of the frontier sweep (0 = auto; 1, 2, 4)
+  // the one-pass hook reads the first two entries of every row from the
+  // CSR itself, after an identity init, instead of from sample arrays
+  // written by a sampling init pass (-1 = auto)
+  int csr_hook = -1;
 };
 // Process-wide settings (cc_hip_components_set / _tune); the entry point
 // without allocations always runs with the defaults.
@@ -613,6 +617,84 @@ __global__ __launch_bounds__(kBlock) void k_hook_rem(const uint32_t *__restrict_
   }
 }
 
+// The one-pass hooks straight from the CSR, with no sample arrays: p starts
+// as the identity (k_init), and each thread reads the first two entries of
+// its row itself.  Neighbour 0 below v is linked by a CAS of v's own entry
+// from v to it, which succeeds while v is still a root; that is the link
+// the sampling init pass makes with a plain store when nothing else runs.
+// (Linking a root v under any w0 < v is a union: w0 cannot be in v's tree,
+// whose members are all >= v.)  If another thread hooked v first, the CAS
+// fails and v is united with neighbour 0 by the walks like neighbour 1.
+// Against the init pass + hook from the sample arrays this saves writing
+// and reading the arrays (8 bytes per vertex each way) and lets the row
+// reads of one vertex overlap the walks of another.
+__device__ __forceinline__ void csr_first_two(const uint64_t *__restrict__ row_ptr,
+                                              const uint32_t *__restrict__ col, uint64_t v,
+                                              uint32_t n, uint32_t &w0, uint32_t &w1) {
+  const uint64_t b = v < n ? row_ptr[v] : 0;
+  const uint64_t e = v < n ? row_ptr[v + 1] : 0;
+  w0 = b < e ? col[b] : kNoNeighbour;
+  w1 = b + 1 < e ? col[b + 1] : kNoNeighbour;
+}
+
+// True if neighbour 0 still needs a union with v (the CAS link failed, or
+// neighbour 0 is above v).
+__device__ __forceinline__ bool link_first(uint32_t v, uint32_t w0, uint32_t n,
+                                           uint32_t *__restrict__ p) {
+  if (w0 >= n || w0 == v) return false;
+  if (w0 > v) return true;
+  return atomicCAS(&p[v], v, w0) != v;
+}
+
+template <int Mode, int ILP, bool Halve>
+__global__ __launch_bounds__(kBlock) void k_hook_pair_csr(const uint64_t *__restrict__ row_ptr,
+                                                          const uint32_t *__restrict__ col,
+                                                          uint32_t n, uint32_t *__restrict__ p) {
+  const uint64_t S = (uint64_t)gridDim.x * kBlock;
+  for (uint64_t v0 = (uint64_t)blockIdx.x * kBlock + threadIdx.x; v0 < n; v0 += S * ILP) {
+    uint32_t w0[ILP], w1[ILP];
+#pragma unroll
+    for (int k = 0; k < ILP; ++k) csr_first_two(row_ptr, col, v0 + k * S, n, w0[k], w1[k]);
+    uint32_t x[3 * ILP];
+    bool act[3 * ILP];
+#pragma unroll
+    for (int k = 0; k < ILP; ++k) {
+      const uint64_t v = v0 + k * S;
+      const bool live = v < n;
+      act[ILP + k] = live && link_first((uint32_t)v, w0[k], n, p);
+      act[2 * ILP + k] = live && w1[k] < n && w1[k] != (uint32_t)v;
+      act[k] = act[ILP + k] || act[2 * ILP + k];
+      x[k] = (uint32_t)v;
+      x[ILP + k] = w0[k];
+      x[2 * ILP + k] = w1[k];
+    }
+    if (Halve) roots_lockstep_halve<3 * ILP>(x, act, p);
+    else roots_lockstep<3 * ILP>(x, act, p);
+#pragma unroll
+    for (int k = 0; k < ILP; ++k) {
+      uint32_t rv = x[k];
+      const uint32_t r0 = x[ILP + k], r1 = x[2 * ILP + k];
+      if (act[ILP + k] && rv != r0) {
+        hook_roots<Mode>(rv, r0, p);
+        rv = rv < r0 ? rv : r0;  // a member of the merged tree
+      }
+      if (act[2 * ILP + k] && rv != r1) hook_roots<Mode>(rv, r1, p);
+    }
+  }
+}
+
+template <int Splice>
+__global__ __launch_bounds__(kBlock) void k_hook_rem_csr(const uint64_t *__restrict__ row_ptr,
+                                                         const uint32_t *__restrict__ col,
+                                                         uint32_t n, uint32_t *__restrict__ p) {
+  CCX_GRID_STRIDE(v, n) {
+    uint32_t w0, w1;
+    csr_first_two(row_ptr, col, v, n, w0, w1);
+    if (link_first((uint32_t)v, w0, n, p)) rem_unite<Splice>((uint32_t)v, w0, p);
+    if (w1 < n && w1 != (uint32_t)v) rem_unite<Splice>((uint32_t)v, w1, p);
+  }
+}
+
 __device__ __forceinline__ uint32_t sample_vertex(uint32_t i, uint32_t n) {
   // SplitMix64 finaliser of the sample index, reduced to [0, n).
   uint64_t z = (uint64_t)(i + 1) * 0x9E3779B97F4A7C15ull;
@@ -818,6 +900,7 @@ struct Plan {
   int nt;        // non-temporal streaming bits (Knobs::nt)
   int init_ilp;  // rows per thread per step of the sampling init pass
   int frontier_chunks;  // chunks per wave step of the frontier sweep
+  int csr;       // one-pass hook from the CSR (k_hook_pair_csr / k_hook_rem_csr), no sample arrays
 };
 
 inline Plan plan_for(uint32_t n, uint64_t m, const Knobs &K = g_knobs) {
@@ -847,6 +930,7 @@ inline Plan plan_for(uint32_t n, uint64_t m, const Knobs &K = g_knobs) {
   // 65.6M 195 -> 146 us, R-MAT 22 33 -> 29 us, R-MAT 24 80 -> 71 us, ER 32M
   // 103 -> 99 us; four lose on every graph (register pressure)
   pl.frontier_chunks = K.frontier_chunks ? K.frontier_chunks : 2;
+  pl.csr = pl.fused && (pl.rem || K.hook_mode == 4) && (K.csr_hook < 0 ? 0 : K.csr_hook);
   return pl;
 }
 
@@ -873,6 +957,7 @@ extern "C" int cc_hip_components_set(const char *key, int value) {
   else if (k == "nt" && value >= -1 && value <= 7) g_knobs.nt = value;
   else if (k == "init_ilp" && (value == 0 || ilp_ok(value))) g_knobs.init_ilp = value;
   else if (k == "frontier_chunks" && (value == 0 || ilp_ok(value))) g_knobs.frontier_chunks = value;
+  else if (k == "csr_hook" && value >= -1 && value <= 1) g_knobs.csr_hook = value;
   else return ccx::set_error(4, "cc_hip_components_set: bad key or value (%s=%d)", key, value);
   return 0;
 }
@@ -895,6 +980,7 @@ extern "C" int cc_hip_components_get(const char *key) {
   if (k == "nt") return g_knobs.nt;
   if (k == "init_ilp") return g_knobs.init_ilp;
   if (k == "frontier_chunks") return g_knobs.frontier_chunks;
+  if (k == "csr_hook") return g_knobs.csr_hook;
   return -1;
 }
 
@@ -923,6 +1009,7 @@ extern "C" int cc_hip_components_plan_get(uint32_t n, uint64_t m, const char *ke
   if (k == "nt") return pl.nt;
   if (k == "init_ilp") return pl.init_ilp;
   if (k == "frontier_chunks") return pl.frontier_chunks;
+  if (k == "csr_hook") return pl.csr;
   if (k == "hook_blocks_per_cu") return pl.hook_bpc;
   if (k == "giant_early") return pl.cache && !pl.fused && g_knobs.giant_early;
   if (k == "hook_mode") return g_knobs.hook_mode;
@@ -986,7 +1073,28 @@ int solve_with(const Knobs &K, const ccx::Context *c, const uint64_t *d_row_ptr,
   // modes 0, 1 and 4 read it (mode 2 walks with L2 reads, mode 3 is lossy).
   const bool cache = plan.cache != 0;
   const bool early_giant = cache && !plan.fused && K.giant_early;
-  if (cache) {
+  if (plan.csr) {
+    // identity, then the one-pass hook reading the rows itself
+    CCX_LAUNCH("cc_init", s, k_init<<<ns, kBlock, 0, s>>>(n, d_labels));
+    if (plan.rem && plan.splice == 1)
+      CCX_LAUNCH("cc_hook_sample", s, k_hook_rem_csr<1><<<nb, kBlock, 0, s>>>(d_row_ptr, d_col_idx, n, d_labels));
+    else if (plan.rem && plan.splice == 2)
+      CCX_LAUNCH("cc_hook_sample", s, k_hook_rem_csr<2><<<nb, kBlock, 0, s>>>(d_row_ptr, d_col_idx, n, d_labels));
+    else if (plan.rem)
+      CCX_LAUNCH("cc_hook_sample", s, k_hook_rem_csr<0><<<nb, kBlock, 0, s>>>(d_row_ptr, d_col_idx, n, d_labels));
+#define CCX_HOOK_CSR(H)                                                                            \
+  do {                                                                                             \
+    if (hook_ilp == 4)                                                                             \
+      CCX_LAUNCH("cc_hook_sample", s, k_hook_pair_csr<4, 4, H><<<nb, kBlock, 0, s>>>(d_row_ptr, d_col_idx, n, d_labels)); \
+    else if (hook_ilp == 2)                                                                        \
+      CCX_LAUNCH("cc_hook_sample", s, k_hook_pair_csr<4, 2, H><<<nb, kBlock, 0, s>>>(d_row_ptr, d_col_idx, n, d_labels)); \
+    else                                                                                           \
+      CCX_LAUNCH("cc_hook_sample", s, k_hook_pair_csr<4, 1, H><<<nb, kBlock, 0, s>>>(d_row_ptr, d_col_idx, n, d_labels)); \
+  } while (0)
+    else if (plan.halve) CCX_HOOK_CSR(true);
+    else CCX_HOOK_CSR(false);
+#undef CCX_HOOK_CSR
+  } else if (cache) {
     if (!smp) return ccx::set_error(4, "solve: no sample buffer");
     switch (rounds) {
       case 1: CCX_LAUNCH("cc_init_min", s, k_init_sample<1><<<ns, kBlock, 0, s>>>(d_row_ptr, d_col_idx, n, d_labels, smp)); break;
@@ -1025,7 +1133,9 @@ int solve_with(const Knobs &K, const ccx::Context *c, const uint64_t *d_row_ptr,
   } else {
     CCX_LAUNCH("cc_init_min", s, k_init_min<<<ns, kBlock, 0, s>>>(d_row_ptr, d_col_idx, n, d_labels));
   }
-  if (plan.fused) {
+  if (plan.csr) {
+    // hooked above
+  } else if (plan.fused) {
     const uint32_t *s1 = smp + n;
 #define CCX_HOOK_PAIR(M, H)                                                                        \
   do {                                                                                             \
@@ -1147,7 +1257,8 @@ extern "C" int cc_hip_components_device(const uint64_t *d_row_ptr, const uint32_
   if (!st) return 3;
   const Knobs K = g_knobs;
   uint32_t *smp = nullptr;
-  if (m > 0 && plan_for(n, m, K).cache) {
+  const Plan pl = plan_for(n, m, K);
+  if (m > 0 && pl.cache && !pl.csr) {
     smp = (uint32_t *)ccx::stream_tmp(st, (size_t)K.sampled_neighbours * n * sizeof(uint32_t));
     if (!smp) return 3;
   }

@@ -532,6 +532,99 @@ __global__ __launch_bounds__(kThreads) void k_merge_hook(const uint64_t *__restr
   }
 }
 
+// ghost_min[k] = comp_label[reps[k]] for every rep, before the hooks lower
+// it by the ghost labels (ghost_min is indexed by position in the rep
+// list).  Starting from the label rather than from ~0 lets the hooks'
+// plain read skip the atomic for nearly every record, and leaves the
+// round's result unchanged: comp_label only decreases, so
+// min(comp_label now, ghost_min) is the same either way.
+__global__ __launch_bounds__(kThreads) void k_ghost_min_init(uint32_t *__restrict__ ghost_min,
+                                                             const uint32_t *__restrict__ comp_label,
+                                                             const uint32_t *__restrict__ reps,
+                                                             uint32_t rep_count) {
+  GRID_STRIDE(k, rep_count) ghost_min[k] = comp_label[reps[k]];
+}
+
+// ghost_min[dominant] lowered to the minimum ghost label over the set bits
+// of the dominant rep's bitmap over the ghosts: one streaming pass over the
+// bitmap and ghost_labels, where the record list costs 8 bytes per record
+// plus the label.  Reduced per workgroup, one atomic each: all of them
+// target one address, which serves atomics about 11 ns apart.
+__global__ __launch_bounds__(kThreads) void k_merge_hook_dom(const uint32_t *__restrict__ bits,
+                                                             uint32_t ghost_count,
+                                                             const uint32_t *__restrict__ ghost_labels,
+                                                             uint32_t dominant,
+                                                             uint32_t *__restrict__ ghost_min) {
+  constexpr int kIlp = 4;  // loads in flight per thread
+  __shared__ uint32_t s_min[kThreads / kWave];
+  uint32_t m = 0xFFFFFFFFu;
+  const uint64_t stride = (uint64_t)gridDim.x * kThreads;
+  uint64_t g = (uint64_t)blockIdx.x * kThreads + threadIdx.x;
+  for (; g + (kIlp - 1) * stride < ghost_count; g += kIlp * stride) {
+    uint32_t w[kIlp], l[kIlp];
+#pragma unroll
+    for (int u = 0; u < kIlp; ++u) {
+      const uint64_t x = g + u * stride;
+      w[u] = bits[x >> 5];
+      l[u] = ghost_labels[x];
+    }
+#pragma unroll
+    for (int u = 0; u < kIlp; ++u) {
+      const uint64_t x = g + u * stride;
+      const uint32_t v = ((w[u] >> (x & 31)) & 1u) ? l[u] : 0xFFFFFFFFu;
+      m = v < m ? v : m;
+    }
+  }
+  for (; g < ghost_count; g += stride) {
+    if ((bits[g >> 5] >> (g & 31)) & 1u) {
+      const uint32_t l = ghost_labels[g];
+      m = l < m ? l : m;
+    }
+  }
+  for (int d = kWave / 2; d > 0; d >>= 1) {
+    const uint32_t o = __shfl_xor(m, d, kWave);
+    m = o < m ? o : m;
+  }
+  if ((threadIdx.x & (kWave - 1)) == 0) s_min[threadIdx.x / kWave] = m;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    for (int w = 1; w < kThreads / kWave; ++w) m = s_min[w] < m ? s_min[w] : m;
+    if (m < ghost_min[dominant]) atomicMin(&ghost_min[dominant], m);
+  }
+}
+
+// The side records' hook of a prepared round: ghost_min[k] lowered by
+// every record (gidx << 32) | k, k the rep's index in the rep list (a
+// compact array, where indexing by local id spread the reps' entries over
+// a 4n-byte array: 125 us for 5.7M records at one rank's share).
+// record -> label -> ghost_min is a chain of three dependent loads, so each
+// thread takes four records per step with every load of the step issued
+// before the first use.  Atomics are rare (ghost_min only moves when a
+// label drops below it), so they go lane by lane.
+__global__ __launch_bounds__(kThreads) void k_merge_hook_side(const uint64_t *__restrict__ rec,
+                                                              uint64_t nb,
+                                                              const uint32_t *__restrict__ ghost_labels,
+                                                              uint32_t *__restrict__ ghost_min) {
+  constexpr int kIlp = 4;
+  const uint64_t stride = (uint64_t)gridDim.x * kThreads;
+  for (uint64_t i0 = (uint64_t)blockIdx.x * kThreads + threadIdx.x; i0 < nb; i0 += kIlp * stride) {
+    uint64_t r[kIlp];
+    uint32_t l[kIlp], g[kIlp];
+#pragma unroll
+    for (int u = 0; u < kIlp; ++u) {
+      const uint64_t i = i0 + u * stride;
+      r[u] = i < nb ? rec[i] : 0;
+    }
+#pragma unroll
+    for (int u = 0; u < kIlp; ++u) l[u] = i0 + u * stride < nb ? ghost_labels[r[u] >> 32] : 0xFFFFFFFFu;
+#pragma unroll
+    for (int u = 0; u < kIlp; ++u) g[u] = i0 + u * stride < nb ? ghost_min[(uint32_t)r[u]] : 0u;
+#pragma unroll
+    for (int u = 0; u < kIlp; ++u)
+      if (l[u] < g[u]) atomicMin(&ghost_min[(uint32_t)r[u]], l[u]);
+  }
+}
+
 __device__ __forceinline__ bool label_of(uint32_t x, const uint32_t *comp_label,
                                          const uint32_t *comp_of, uint32_t v_start, uint32_t v_end,
                                          const uint32_t *ghosts, uint32_t ghost_count,
@@ -559,7 +652,12 @@ __device__ __forceinline__ bool merge_update_rep(uint32_t r, const uint32_t *com
                                                  uint32_t *comp_label, uint32_t *comp_min,
                                                  uint32_t v_start, uint32_t v_end,
                                                  const uint32_t *ghosts, uint32_t ghost_count,
-                                                 const uint32_t *ghost_labels) {
+                                                 const uint32_t *ghost_labels,
+                                                 const uint32_t *ghost_min) {
+  if (ghost_min) {  // the init of this round, fused: comp_min = min(comp_label, this rep's ghost_min)
+    const uint32_t l = comp_label[r], g = *ghost_min;
+    comp_min[r] = g < l ? g : l;
+  }
   uint32_t x = comp_min[r];
   for (int t = 0; t < 4; ++t) {
     uint32_t y;
@@ -588,18 +686,19 @@ __global__ __launch_bounds__(kThreads) void k_merge_update(const uint32_t *__res
                                                            const uint32_t *__restrict__ ghosts,
                                                            uint32_t ghost_count,
                                                            const uint32_t *__restrict__ ghost_labels,
-                                                           uint32_t *__restrict__ changed) {
+                                                           uint32_t *__restrict__ changed,
+                                                           const uint32_t *__restrict__ ghost_min) {
   bool any = false;
   if (reps) {
     GRID_STRIDE(k, rep_count) {
       any |= merge_update_rep(reps[k], comp_of, comp_label, comp_min, v_start, v_end, ghosts,
-                              ghost_count, ghost_labels);
+                              ghost_count, ghost_labels, ghost_min ? ghost_min + k : nullptr);
     }
   } else {
     GRID_STRIDE(i, n_local) {
       if (comp_of[i] != (uint32_t)i) continue;
       any |= merge_update_rep((uint32_t)i, comp_of, comp_label, comp_min, v_start, v_end, ghosts,
-                              ghost_count, ghost_labels);
+                              ghost_count, ghost_labels, nullptr);
     }
   }
   if (__ballot(any) != 0 && (threadIdx.x & (kWave - 1)) == 0) atomicOr(changed, 1u);
@@ -846,12 +945,24 @@ extern "C" int cc_hip_iota_u32_device(uint32_t *d_out, uint32_t n, uint32_t base
 }
 
 namespace {
-// One merge round; reps null: the reps are found by scanning comp_of.
+// The dominant rep's part of a prepared merge round (null bits: none).
+struct DomPart {
+  const uint32_t *bits;
+  uint32_t dominant;
+  uint32_t *ghost_min;
+  int ghosts_changed;
+};
+
+// One merge round; reps null: the reps are found by scanning comp_of.  With
+// dom.bits the records are the side records of cc_hip_merge_prepare_device
+// and the per-rep ghost minimum lives in dom.ghost_min, recomputed only when
+// the ghost labels changed; the update then starts each rep from
+// min(comp_label, ghost_min), which is what the init and hook give.
 int merge_step(const uint64_t *d_records, uint64_t count, const uint32_t *d_ghost_vertices,
                const uint32_t *d_ghost_labels, uint32_t ghost_count, const uint32_t *d_comp_of,
                uint32_t *d_comp_label, uint32_t *d_comp_min, uint32_t n_local,
                const uint32_t *d_reps, uint32_t rep_count, uint32_t v_start, uint32_t v_end,
-               int *changed, void *stream) {
+               int *changed, void *stream, const DomPart &dom = DomPart{nullptr, 0, nullptr, 0}) {
   if (!ccx::ctx()) return 2;
   if (changed) *changed = 0;
   if (n_local == 0) return 0;
@@ -861,18 +972,41 @@ int merge_step(const uint64_t *d_records, uint64_t count, const uint32_t *d_ghos
   uint32_t *d_changed = st->d_words + 24;
   CCX_TRY(hipMemsetAsync(d_changed, 0, sizeof(uint32_t), s));
   const unsigned g = grid_for(d_reps ? rep_count : n_local);
-  CCX_LAUNCH("merge_init", s,
-             k_merge_init<<<g, kThreads, 0, s>>>(d_comp_of, d_comp_label, d_comp_min, n_local,
-                                                 d_reps, rep_count));
-  if (count > 0)
-    CCX_LAUNCH("merge_hook", s,
-               k_merge_hook<<<grid_for(count), kThreads, 0, s>>>(d_records, count, d_ghost_labels,
-                                                                 d_comp_min));
+  if (dom.bits) {
+    // 2: the first round (ghost_min seeded from comp_label); 1: ghost labels
+    // lowered since the last round, so the hooks lower ghost_min in place:
+    // min over the records of the lowered labels is at most the old one,
+    // and only records whose label dropped below it pass the hooks' filter
+    if (dom.ghosts_changed >= 2)
+      CCX_LAUNCH("merge_ghost_min_init", s,
+                 k_ghost_min_init<<<g, kThreads, 0, s>>>(dom.ghost_min, d_comp_label, d_reps,
+                                                         rep_count));
+    if (dom.ghosts_changed) {
+      if (ghost_count > 0)
+        CCX_LAUNCH("merge_hook_dom", s,
+                   k_merge_hook_dom<<<grid_for(ghost_count), kThreads, 0, s>>>(
+                       dom.bits, ghost_count, d_ghost_labels, dom.dominant, dom.ghost_min));
+      if (count > 0)
+        CCX_LAUNCH("merge_hook_side", s,
+                   k_merge_hook_side<<<grid_for(count), kThreads, 0, s>>>(d_records, count,
+                                                                          d_ghost_labels,
+                                                                          dom.ghost_min));
+    }
+    // the round's init is fused into the update (comp_min from ghost_min)
+  } else {
+    CCX_LAUNCH("merge_init", s,
+               k_merge_init<<<g, kThreads, 0, s>>>(d_comp_of, d_comp_label, d_comp_min, n_local,
+                                                   d_reps, rep_count));
+    if (count > 0)
+      CCX_LAUNCH("merge_hook", s,
+                 k_merge_hook<<<grid_for(count), kThreads, 0, s>>>(d_records, count, d_ghost_labels,
+                                                                   d_comp_min));
+  }
   CCX_LAUNCH("merge_update", s,
              k_merge_update<<<g, kThreads, 0, s>>>(d_comp_of, d_comp_label, d_comp_min, n_local,
                                                    d_reps, rep_count, v_start, v_end,
                                                    d_ghost_vertices, ghost_count, d_ghost_labels,
-                                                   d_changed));
+                                                   d_changed, dom.bits ? dom.ghost_min : nullptr));
   uint32_t *h = (uint32_t *)st->h_stage;
   CCX_TRY(hipMemcpyAsync(h, d_changed, sizeof(uint32_t), hipMemcpyDeviceToHost, s));
   CCX_TRY(hipStreamSynchronize(s));
@@ -904,6 +1038,27 @@ extern "C" int cc_hip_merge_step_reps_device(const uint64_t *d_records, uint64_t
   return merge_step(d_records, count, d_ghost_vertices, d_ghost_labels, ghost_count, d_comp_of,
                     d_comp_label, d_comp_min, n_local, d_reps, rep_count, v_start, v_end, changed,
                     stream);
+}
+
+extern "C" int cc_hip_merge_step_prepared_device(
+    const uint32_t *d_dom_bits, uint32_t dominant, const uint64_t *d_side, uint64_t side_count,
+    const uint32_t *d_ghost_vertices, const uint32_t *d_ghost_labels, uint32_t ghost_count,
+    const uint32_t *d_comp_of, uint32_t *d_comp_label, uint32_t *d_comp_min,
+    uint32_t *d_ghost_min, uint32_t n_local, const uint32_t *d_reps, uint32_t rep_count,
+    uint32_t v_start, uint32_t v_end, int ghosts_changed, int *changed, void *stream) {
+  if (n_local > 0 && (!d_reps || rep_count == 0 || rep_count > n_local))
+    return ccx::set_error(4, "cc_hip_merge_step_prepared_device: rep list missing or larger than n_local");
+  if (n_local > 0 && (!d_ghost_min || (ghost_count > 0 && !d_dom_bits) ||
+                      (side_count > 0 && !d_side)))
+    return ccx::set_error(4, "cc_hip_merge_step_prepared_device: null pointer");
+  if (n_local > 0 && dominant >= rep_count)
+    return ccx::set_error(4, "cc_hip_merge_step_prepared_device: dominant index %u not below rep_count %u",
+                          dominant, rep_count);
+  // a valid address for the dominant part even without ghosts (no bit is read then)
+  const uint32_t *bits = d_dom_bits ? d_dom_bits : d_ghost_min;
+  return merge_step(d_side, side_count, d_ghost_vertices, d_ghost_labels, ghost_count, d_comp_of,
+                    d_comp_label, d_comp_min, n_local, d_reps, rep_count, v_start, v_end, changed,
+                    stream, DomPart{bits, dominant, d_ghost_min, ghosts_changed});
 }
 
 extern "C" int cc_hip_rank_labels_device(const uint32_t *d_comp_label, const uint32_t *d_comp_of,

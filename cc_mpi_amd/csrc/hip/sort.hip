@@ -815,6 +815,7 @@ constexpr int kModeSamples = 1024;
 
 // Most frequent packed rep (rank among the roots) among kModeSamples records
 // spread evenly over the input; ties go to the smaller rank.  One workgroup.
+// With packed null the records' reps are taken as they are.
 __global__ __launch_bounds__(kModeSamples) void k_mode_rank(const uint64_t *__restrict__ rec,
                                                             uint64_t count,
                                                             const uint32_t *__restrict__ packed,
@@ -832,7 +833,7 @@ __global__ __launch_bounds__(kModeSamples) void k_mode_rank(const uint64_t *__re
   __syncthreads();
   {
     const uint32_t li = (uint32_t)rec[(uint64_t)t * count / kModeSamples];
-    const uint32_t x = li < n ? packed[li] : 0u;
+    const uint32_t x = packed ? (li < n ? packed[li] : 0u) : li;
     uint32_t h = (x * 0x9E3779B1u) >> 21;  // 11-bit slot
     for (;;) {
       const uint32_t prev = atomicCAS(&keys[h], kEmpty, x);
@@ -921,6 +922,87 @@ struct SplitRows {
     if (keep) others[rank] = v;
   }
 };
+
+// Index of rep r in the ascending rep list (rep_count if absent).
+__device__ __forceinline__ uint32_t rep_index(const uint32_t *reps, uint32_t rep_count, uint32_t r) {
+  uint32_t lo = 0, hi = rep_count;
+  while (lo < hi) {
+    const uint32_t mid = lo + (hi - lo) / 2;
+    if (reps[mid] < r) lo = mid + 1;
+    else hi = mid;
+  }
+  return (lo < rep_count && reps[lo] == r) ? lo : rep_count;
+}
+
+// The merge loop's records split by rep (cc_hip_merge_prepare_device):
+// records (gidx << 32) | rep whose rep is *g set bit gidx of a bitmap over
+// the ghosts; the others are kept in input order.  The records are sorted,
+// so the lanes' bitmap words are nondecreasing along a row: each run of
+// lanes on one word ORs its bits together (a segmented reduction across
+// the wave) and the run's first lane sets them with one atomic.  The
+// records are unique, so every bit is new (no read-before-atomic filter).
+// A kept record's rep is replaced by its index in the rep list, so that the
+// rounds keep their per-rep state in an array of rep_count entries.
+template <int R>
+struct DomRows {
+  using Val = uint64_t;
+  static constexpr int kRows = R;
+  const uint64_t *rec;
+  const uint32_t *g;
+  uint32_t nbits;  // ghost count: a record with gidx >= nbits is kept as a side record
+  uint32_t *bitmap;
+  uint64_t *side;
+  const uint32_t *reps;  // ascending; a side record's rep is replaced by its index here
+  uint32_t rep_count;
+  uint32_t *bad;  // set when a record's rep is not in the list
+  template <int Rows>
+  __device__ void rows(uint64_t wbase, int lane, uint64_t count, uint64_t (&v)[Rows],
+                       uint64_t (&bal)[Rows]) const {
+    const uint32_t d = *g;
+#pragma unroll
+    for (int i = 0; i < Rows; ++i) {
+      const uint64_t j = wbase + (uint64_t)i * kWave + lane;
+      v[i] = j < count ? __builtin_nontemporal_load(&rec[j]) : 0;
+    }
+#pragma unroll
+    for (int i = 0; i < Rows; ++i) {
+      const uint64_t j = wbase + (uint64_t)i * kWave + lane;
+      const bool live = j < count;
+      const uint32_t gidx = (uint32_t)(v[i] >> 32);
+      const bool dom = live && (uint32_t)v[i] == d && gidx < nbits;
+      const uint32_t word = live ? (gidx >> 5) : 0xFFFFFFFFu;
+      uint32_t bits = dom ? (1u << (gidx & 31)) : 0u;
+#pragma unroll
+      for (int dd = 1; dd < kWave; dd <<= 1) {
+        const uint32_t ob = __shfl_down(bits, dd, kWave);
+        const uint32_t ow = __shfl_down(word, dd, kWave);
+        if (lane + dd < kWave && ow == word) bits |= ob;
+      }
+      const uint32_t up = __shfl_up(word, 1, kWave);
+      if (bits && (lane == 0 || up != word)) atomicOr(&bitmap[word], bits);
+      bal[i] = __ballot(live && !dom);
+    }
+  }
+  __device__ void apply(uint64_t, bool keep, uint64_t rank, uint64_t v) const {
+    if (!keep) return;
+    const uint32_t k = rep_index(reps, rep_count, (uint32_t)v);
+    if (k == rep_count) *bad = 1u;
+    side[rank] = (v & 0xFFFFFFFF00000000ull) | k;
+  }
+};
+
+// out[0] = index of *rep in the rep list (rep_count if absent).  One thread.
+__global__ void k_rep_index(const uint32_t *__restrict__ reps, uint32_t rep_count,
+                            const uint32_t *__restrict__ rep, uint32_t *__restrict__ out) {
+  out[0] = rep_index(reps, rep_count, rep[0]);
+}
+
+__global__ __launch_bounds__(kThreads) void k_fill_u32(uint32_t *__restrict__ a, uint64_t n,
+                                                       uint32_t v) {
+  for (uint64_t i = (uint64_t)blockIdx.x * kThreads + threadIdx.x; i < n;
+       i += (uint64_t)gridDim.x * kThreads)
+    a[i] = v;
+}
 
 __global__ __launch_bounds__(kThreads) void k_word_counts(const uint32_t *__restrict__ bitmap,
                                                           uint64_t words, uint32_t *__restrict__ cnt) {
@@ -1097,6 +1179,8 @@ int g_sort_items = 16;     // keys per thread (8, 16, 24, 32)
 int g_sort_threads = 512;  // threads per workgroup (256; 512 with 8 or 16 items; 1024 with 8)
 int g_boundary_split = 1;  // boundary build: dominant rep through a bitmap (0: sort every record)
 int g_split_dominant_rows = 32;  // rows of 64 records per wave step in the bitmap split (16, 32)
+// workgroups for `threads` threads, at most per_cu per CU
+unsigned grid_cap(uint64_t threads, int per_cu);
 }  // namespace
 
 extern "C" int cc_hip_sort_set(const char *key, int value) {
@@ -1285,6 +1369,49 @@ extern "C" int cc_hip_ghost_extract_device(const uint64_t *d_records, uint64_t c
                             count, s, st, ghost_count);
   return run_compact("ghost_compact", GhostOp{d_records, d_out_records, d_ghost_vertices}, count,
                      s, st, ghost_count);
+}
+
+extern "C" int cc_hip_merge_prepare_device(const uint64_t *d_records, uint64_t count,
+                                           uint32_t ghost_count, const uint32_t *d_reps,
+                                           uint32_t rep_count, uint32_t *d_dom_bits,
+                                           uint64_t *d_side, uint64_t *side_count,
+                                           uint32_t *dominant, void *stream) {
+  if (!ccx::ctx()) return 2;
+  if (!side_count || !dominant) return ccx::set_error(4, "cc_hip_merge_prepare_device: null output");
+  *side_count = 0;
+  *dominant = 0;
+  const uint64_t words = ((uint64_t)ghost_count + 31) / 32;
+  if ((words > 0 && !d_dom_bits) || (count > 0 && (!d_records || !d_side || !d_reps)))
+    return ccx::set_error(4, "cc_hip_merge_prepare_device: null pointer");
+  if (count > 0 && rep_count == 0)
+    return ccx::set_error(4, "cc_hip_merge_prepare_device: records but no reps");
+  hipStream_t s = ccx::pick_stream(stream);
+  ccx::StreamState *st = ccx::stream_state(s);
+  if (!st) return 3;
+  if (words > 0)
+    CCX_LAUNCH("merge_prepare_clear", s,
+               k_fill_u32<<<grid_cap(words, 8), kThreads, 0, s>>>(d_dom_bits, words, 0u));
+  if (count == 0) return 0;
+  uint32_t *d_g = st->d_words + 40;  // the dominant rep, its index, the bad-rep flag
+  CCX_TRY(hipMemsetAsync(d_g + 2, 0, sizeof(uint32_t), s));
+  CCX_LAUNCH("merge_prepare_mode", s,
+             k_mode_rank<<<1, kModeSamples, 0, s>>>(d_records, count, nullptr, 0, d_g));
+  uint64_t n_side = 0;
+  if (int rc = run_compact_rows(
+          "merge_prepare",
+          DomRows<32>{d_records, d_g, ghost_count, d_dom_bits, d_side, d_reps, rep_count, d_g + 2},
+          count,
+          s, st, &n_side))
+    return rc;
+  CCX_LAUNCH("merge_prepare_index", s, k_rep_index<<<1, 1, 0, s>>>(d_reps, rep_count, d_g, d_g + 1));
+  uint32_t *h = (uint32_t *)st->h_stage;
+  CCX_TRY(hipMemcpyAsync(h, d_g + 1, 2 * sizeof(uint32_t), hipMemcpyDeviceToHost, s));
+  CCX_TRY(hipStreamSynchronize(s));
+  if (h[0] >= rep_count || h[1] != 0)
+    return ccx::set_error(4, "cc_hip_merge_prepare_device: a record's rep is not in the rep list");
+  *dominant = h[0];
+  *side_count = n_side;
+  return 0;
 }
 
 extern "C" int cc_hip_local_reps_device(const uint32_t *d_comp_of, uint32_t n_local,

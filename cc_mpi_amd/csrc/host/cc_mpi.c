@@ -192,6 +192,19 @@ void compute_connected_components_mpi_advanced(const DistCSRGraph *Gd, uint32_t 
   /* comp_of is fixed from here on: the packs gather the send entries' reps once */
   check(exchangeplan_device_prepare(&plan, &xd, d_comp_of, v_start, v_end), comm,
         "exchange device prepare");
+  /* the merge rounds' records, split once: the most frequent rep's as a
+     bitmap over the ghosts, the others as a side list; each rep's minimum
+     ghost label is kept until the ghost labels change */
+  uint32_t *d_dom_bits = (uint32_t *)dalloc((size_t)((ghost_count + 31) / 32) * 4, comm,
+                                            "alloc dominant bitmap");
+  uint64_t *d_side = (uint64_t *)dalloc((size_t)nbd * 8, comm, "alloc side records");
+  uint32_t *d_ghost_min = (uint32_t *)dalloc((size_t)rep_count * 4, comm, "alloc ghost minimum");
+  uint64_t n_side = 0;
+  uint32_t dominant = 0;
+  check(cc_hip_merge_prepare_device(d_bnd, nbd, ghost_count, d_reps, rep_count, d_dom_bits, d_side,
+                                    &n_side, &dominant, NULL),
+        comm, "merge prepare");
+  int ghosts_changed = 2; /* first round: seed the per-rep ghost minimum */
 
   /* 5. merge loop */
   MPI_Barrier(comm);
@@ -203,10 +216,12 @@ void compute_connected_components_mpi_advanced(const DistCSRGraph *Gd, uint32_t 
   while (global_pending)
   {
     int changed = 0;
-    check(cc_hip_merge_step_reps_device(d_bnd, nbd, d_ghosts, d_ghost_labels, ghost_count,
-                                        d_comp_of, d_comp_label, d_comp_min, n_local, d_reps,
-                                        rep_count, v_start, v_end, &changed, NULL),
+    check(cc_hip_merge_step_prepared_device(d_dom_bits, dominant, d_side, n_side, d_ghosts,
+                                            d_ghost_labels, ghost_count, d_comp_of, d_comp_label,
+                                            d_comp_min, d_ghost_min, n_local, d_reps, rep_count,
+                                            v_start, v_end, ghosts_changed, &changed, NULL),
           comm, "merge step");
+    ghosts_changed = 0;
     if (changed)
       pending_since_exchange = 1;
     int global_changed = 0;
@@ -220,6 +235,7 @@ void compute_connected_components_mpi_advanced(const DistCSRGraph *Gd, uint32_t 
     {
       exchangeplan_exchange_delta_device(&plan, &xd, d_comp_label, d_comp_of, v_start, v_end,
                                          d_prev, d_ghost_labels);
+      ghosts_changed = xd.last_received > 0;
       pending_since_exchange = 0;
       steps_since_exchange = 0;
     }
@@ -269,6 +285,9 @@ void compute_connected_components_mpi_advanced(const DistCSRGraph *Gd, uint32_t 
   cc_hip_free(d_comp_min);
   cc_hip_free(d_reps);
   cc_hip_free(d_prev);
+  cc_hip_free(d_dom_bits);
+  cc_hip_free(d_side);
+  cc_hip_free(d_ghost_min);
   exchangeplan_device_free(&xd);
   exchangeplan_free(&plan);
   cc_hip_free(d_bnd);

@@ -263,6 +263,7 @@ uint64_t exchangeplan_exchange_delta_device(const ExchangePlan *P, ExchangeDevic
                                           D->need_base, ind, D->d_need_from_gidx_flat,
                                           d_ghost_labels, NULL),
                P->comm_graph, "exchange delta unpack");
+  D->last_received = (uint64_t)total_recv;
   return sent;
 }
 

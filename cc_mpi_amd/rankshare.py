@@ -122,9 +122,29 @@ class RankShare:
         self.rep_of_send = DeviceBuffer(4 * max(1, nsend))
         delta_prepare_device(self.send_to_flat, nsend, self.comp_of, vs, ve, self.rep_of_send)
         self._t("delta_prepare", t0)
+        # the rep list: the merge round's per-rep passes walk it instead of
+        # scanning comp_of (src/cc_mpi.c:328-362)
+        t0 = time.perf_counter()
+        self.reps = DeviceBuffer(4 * max(1, nl))
+        rep_count = ctypes.c_uint32(0)
+        nat.check(lib.cc_hip_local_reps_device(self.comp_of.ptr, nl, self.reps.ptr,
+                                               ctypes.byref(rep_count), None), "local reps")
+        self.rep_count = int(rep_count.value)
+        # the merge rounds' records, split once: the most frequent rep's as
+        # a bitmap over the ghosts, the rest as a side list
+        self.dom_bits = DeviceBuffer(4 * max(1, (self.ng + 31) // 32))
+        self.side = DeviceBuffer(8 * max(1, self.nbd))
+        n_side, dom = ctypes.c_uint64(), ctypes.c_uint32()
+        nat.check(lib.cc_hip_merge_prepare_device(self.bnd.ptr, self.nbd, self.ng, self.reps.ptr,
+                                                  self.rep_count, self.dom_bits.ptr,
+                                                  self.side.ptr, ctypes.byref(n_side),
+                                                  ctypes.byref(dom), None),
+                  "merge prepare")
+        self.n_side, self.dominant = int(n_side.value), int(dom.value)
+        self._t("merge_prepare", t0)
         self.stats.update(n_local=nl, m_local_local=self.m_ll, boundary_raw=self.nb,
                           boundary_dedup=self.nbd, ghosts=self.ng, total_send_to=nsend,
-                          outdegree=len(self.dests))
+                          outdegree=len(self.dests), side_records=self.n_side)
 
     def owner_local_labels(self) -> np.ndarray:
         """For every vertex, the label its owning rank holds right after its
@@ -207,25 +227,26 @@ class RankShare:
         return self._sched
 
     def merge(self, truth_labels: DeviceBuffer, exchange_interval: int = 1, seed: int = 1,
-              check_pack=None, max_iters: int = 100000, use_reps: bool = True) -> np.ndarray:
+              check_pack=None, max_iters: int = 100000, use_reps: bool = True,
+              prepared: bool = True) -> np.ndarray:
         """Run the merge loop with injected ghost deltas; returns the owned
         labels (comp_label[comp_of[li]]).  check_pack(seg_start, comp_label,
         comp_of, prev_before, pairs, counts, displs) is called per exchange
         if given.  use_reps: the packs read the reps gathered once after
-        local CC (else each pack gathers them from comp_of)."""
+        local CC (else each pack gathers them from comp_of).  prepared: the
+        rounds run over the split records and keep each rep's minimum ghost
+        label until the ghost labels change (else every round reads the
+        whole record list)."""
         lib = nat.gpu_lib()
         nl, vs, ve, ng = self.nl, self.vs, self.ve, self.ng
         interval = max(1, exchange_interval)
         comp_label = DeviceBuffer(4 * max(1, nl))
         comp_min = DeviceBuffer(4 * max(1, nl))
         nat.check(lib.cc_hip_iota_u32_device(comp_label.ptr, nl, vs, None), "iota")
-        # the rep list: the merge round's per-rep passes walk it instead of
-        # scanning comp_of (src/cc_mpi.c:328-362)
-        reps = DeviceBuffer(4 * max(1, nl))
-        rep_count = ctypes.c_uint32(0)
-        nat.check(lib.cc_hip_local_reps_device(self.comp_of.ptr, nl, reps.ptr,
-                                               ctypes.byref(rep_count), None), "local reps")
+        reps, rep_count = self.reps, ctypes.c_uint32(self.rep_count)
         ghost_labels = DeviceBuffer(4 * max(1, ng))
+        ghost_min = DeviceBuffer(4 * max(1, self.rep_count))
+        ghosts_changed = 2  # first round: seed the per-rep ghost minimum
         nat.check(lib.cc_hip_memcpy_d2d(ghost_labels.ptr, self.ghosts.ptr, 4 * ng, None), "copy")
         sched = self.prepare_schedule(truth_labels, seed)
         last_exchange = sched["last"]
@@ -244,11 +265,18 @@ class RankShare:
         changed = ctypes.c_int(0)
         while True:
             t0 = time.perf_counter()
-            nat.check(lib.cc_hip_merge_step_reps_device(self.bnd.ptr, self.nbd, self.ghosts.ptr,
-                                                        ghost_labels.ptr, ng, self.comp_of.ptr,
-                                                        comp_label.ptr, comp_min.ptr, nl, reps.ptr,
-                                                        rep_count.value, vs, ve,
-                                                        ctypes.byref(changed), None), "merge step")
+            if prepared:
+                nat.check(lib.cc_hip_merge_step_prepared_device(
+                    self.dom_bits.ptr, self.dominant, self.side.ptr, self.n_side, self.ghosts.ptr,
+                    ghost_labels.ptr, ng, self.comp_of.ptr, comp_label.ptr, comp_min.ptr,
+                    ghost_min.ptr, nl, reps.ptr, rep_count.value, vs, ve, ghosts_changed,
+                    ctypes.byref(changed), None), "merge step")
+                ghosts_changed = 0
+            else:
+                nat.check(lib.cc_hip_merge_step_reps_device(
+                    self.bnd.ptr, self.nbd, self.ghosts.ptr, ghost_labels.ptr, ng,
+                    self.comp_of.ptr, comp_label.ptr, comp_min.ptr, nl, reps.ptr,
+                    rep_count.value, vs, ve, ctypes.byref(changed), None), "merge step")
             t_merge += time.perf_counter() - t0
             iters += 1
             if changed.value:
@@ -292,6 +320,7 @@ class RankShare:
                         sched["pairs"].ptr + 8 * lo, hi - lo, nat.ptr(sched["zero"], ctypes.c_int),
                         nat.ptr(sched["zero"], ctypes.c_int), 1, sched["iota"].ptr,
                         ghost_labels.ptr, None), "delta unpack")
+                    ghosts_changed = 1
                 k += 1
                 pending_since = 0
                 steps = 0
@@ -302,7 +331,7 @@ class RankShare:
                                                 labels.ptr, ctypes.byref(roots), None), "labels")
         out = np.empty(nl, dtype=np.uint32)
         labels.download(out)
-        for b in (labels, comp_label, comp_min, ghost_labels, prev_sent, reps):
+        for b in (labels, comp_label, comp_min, ghost_labels, ghost_min, prev_sent):
             b.close()
         sendbuf.close()
         self.times["merge_steps"] = t_merge
@@ -318,7 +347,8 @@ class RankShare:
             sched["pairs"].close()
             sched["iota"].close()
             self._sched = None
-        for name in ("comp_of", "send_to_flat", "ghosts", "bnd", "rep_of_send"):
+        for name in ("comp_of", "send_to_flat", "ghosts", "bnd", "rep_of_send", "dom_bits",
+                     "side", "reps"):
             b = getattr(self, name, None)
             if b is not None:
                 b.close()

@@ -389,6 +389,42 @@ int cc_hip_merge_step_reps_device(const uint64_t *d_records, uint64_t count,
                                   const uint32_t *d_reps, uint32_t rep_count, uint32_t v_start,
                                   uint32_t v_end, int *changed, void *stream);
 
+/* Once per merge loop, over the records (gidx << 32) | rep of the merge
+ * round (sorted, unique; every rep in the ascending rep list d_reps of
+ * cc_hip_local_reps_device): the most frequent rep among 1024 records
+ * spread evenly over them (the choice affects speed only) is the dominant
+ * one, and *dominant receives its index in d_reps; d_dom_bits
+ * (ceil(ghost_count / 32) words) gets bit gidx set for every record of the
+ * dominant rep; every other record goes, in order, to d_side (room for
+ * count) as (gidx << 32) | index of its rep in d_reps, *side_count of them
+ * (blocking; error 4 if a record's rep is not in the list).  The merge
+ * loop of src/cc_mpi.c:541-710 reads the same records every round; at one
+ * rank's share nearly all carry the rep of the rank's giant component. */
+int cc_hip_merge_prepare_device(const uint64_t *d_records, uint64_t count, uint32_t ghost_count,
+                                const uint32_t *d_reps, uint32_t rep_count, uint32_t *d_dom_bits,
+                                uint64_t *d_side, uint64_t *side_count, uint32_t *dominant,
+                                void *stream);
+
+/* cc_hip_merge_step_reps_device over the prepared records: the same round
+ * (src/cc_mpi.c:547-622), with the hook's per-rep minimum ghost label kept
+ * in d_ghost_min (rep_count entries, in rep-list order) across rounds;
+ * dominant, d_dom_bits, d_side and side_count as the prepare call left
+ * them for the same d_reps.
+ * ghosts_changed: 2 on the first round (the minimum is seeded and computed
+ * from the dominant rep's bitmap and the side records); 1 when
+ * d_ghost_labels has been lowered since the last round (after an exchange
+ * that delivered pairs: the minimum is lowered by the records in place);
+ * 0 when it is unchanged (the minimum is reused).  Exchanged ghost labels
+ * only ever decrease (a rank sends a label only below the one it sent
+ * before), which is what makes the in-place update exact; a caller whose
+ * ghost labels can rise passes 2 instead of 1. */
+int cc_hip_merge_step_prepared_device(
+    const uint32_t *d_dom_bits, uint32_t dominant, const uint64_t *d_side, uint64_t side_count,
+    const uint32_t *d_ghost_vertices, const uint32_t *d_ghost_labels, uint32_t ghost_count,
+    const uint32_t *d_comp_of, uint32_t *d_comp_label, uint32_t *d_comp_min,
+    uint32_t *d_ghost_min, uint32_t n_local, const uint32_t *d_reps, uint32_t rep_count,
+    uint32_t v_start, uint32_t v_end, int ghosts_changed, int *changed, void *stream);
+
 /* The local reps {r < n_local : comp_of[r] == r}, ascending, into d_reps
  * (room for n_local); *rep_count receives their number (blocking).
  * src/cc_mpi.c:328-362. */

@@ -97,6 +97,7 @@ typedef struct
   int *need_base;      /* need_from_displs of each source, in sources order */
   int *sendcounts, *sdispls, *recvcounts, *rdispls;
   uint32_t *d_rep_of_send; /* send entries' reps (exchangeplan_device_prepare), or NULL */
+  uint64_t last_received;  /* pairs the last exchange delivered into d_ghost_labels */
 } ExchangeDevice;
 
 int exchangeplan_device_init(const ExchangePlan *P, ExchangeDevice *D);

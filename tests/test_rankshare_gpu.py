@@ -224,9 +224,18 @@ def test_merge_step_over_rep_list(gpu, nl, roots):
     np.testing.assert_array_equal(reps, np.flatnonzero(comp_of == np.arange(nl)).astype(np.uint32))
     d_rec = cm.DeviceBuffer.from_array(rec)
     d_gh = cm.DeviceBuffer.from_array(ghosts)
+    prep = _prepare(lib, d_rec, rec, ghosts.size, d_reps, reps)
+    d_gmin = cm.DeviceBuffer(4 * reps.size)
+    first = [True]
 
     def gpu_round(use_list, d_gl, d_cl, d_cm, changed):
-        if use_list:
+        if use_list == "prepared":
+            rc = lib.cc_hip_merge_step_prepared_device(
+                prep["bits"].ptr, prep["dominant"], prep["side"].ptr, prep["n_side"], d_gh.ptr,
+                d_gl.ptr, ghosts.size, d_comp.ptr, d_cl.ptr, d_cm.ptr, d_gmin.ptr, nl, d_reps.ptr,
+                cnt.value, vs, ve, 2 if first[0] else 0, ctypes.byref(changed), None)
+            first[0] = False
+        elif use_list:
             rc = lib.cc_hip_merge_step_reps_device(d_rec.ptr, rec.size, d_gh.ptr, d_gl.ptr,
                                                    ghosts.size, d_comp.ptr, d_cl.ptr, d_cm.ptr, nl,
                                                    d_reps.ptr, cnt.value, vs, ve,
@@ -246,7 +255,8 @@ def test_merge_step_over_rep_list(gpu, nl, roots):
         if not o_flags[-1]:
             break
         assert len(o_rounds) < 100
-    for use_list in (False, True):
+    for use_list in (False, True, "prepared"):
+        first[0] = True
         d_gl = cm.DeviceBuffer.from_array(g_lab)
         d_cl = cm.DeviceBuffer.from_array(np.arange(vs, ve, dtype=np.uint32))
         d_cm = cm.DeviceBuffer(4 * nl)
@@ -277,5 +287,135 @@ def test_merge_step_over_rep_list(gpu, nl, roots):
     assert lib.cc_hip_merge_step_reps_device(d_rec.ptr, rec.size, d_gh.ptr, d_gh.ptr, ghosts.size,
                                              d_comp.ptr, d_comp.ptr, d_comp.ptr, nl, None, 0, vs,
                                              ve, ctypes.byref(changed), None) == 4
-    for b in (d_comp, d_reps, d_rec, d_gh):
+    for b in (d_comp, d_reps, d_rec, d_gh, d_gmin, prep["bits"], prep["side"]):
+        b.close()
+
+
+def _prepare(lib, d_rec, rec, ghost_count, d_reps, reps):
+    """cc_hip_merge_prepare_device over the records, checked against its
+    definition: the dominant rep is a rep of the records, its records are
+    exactly the set bits, and the other records are the side list in
+    input order, each with its rep replaced by the rep's index in the list."""
+    import ctypes
+    from cc_mpi_amd import _native as nat
+    words = (ghost_count + 31) // 32
+    d_bits = cm.DeviceBuffer(4 * max(1, words))
+    d_side = cm.DeviceBuffer(8 * max(1, rec.size))
+    n_side, dom = ctypes.c_uint64(), ctypes.c_uint32()
+    nat.check(lib.cc_hip_merge_prepare_device(d_rec.ptr, rec.size, ghost_count, d_reps.ptr,
+                                              reps.size, d_bits.ptr, d_side.ptr,
+                                              ctypes.byref(n_side), ctypes.byref(dom), None),
+              "merge prepare")
+    reps_of = (rec & np.uint64(0xFFFFFFFF)).astype(np.uint32)
+    gidx = (rec >> np.uint64(32)).astype(np.uint32)
+    assert dom.value < reps.size
+    is_dom = reps_of == reps[dom.value]
+    if rec.size:
+        assert is_dom.any()
+    bits = np.empty(max(1, words), dtype=np.uint32)
+    d_bits.download(bits)
+    want_bits = np.zeros(max(1, words), dtype=np.uint32)
+    np.bitwise_or.at(want_bits, gidx[is_dom] >> 5, (np.uint32(1) << (gidx[is_dom] & 31)))
+    np.testing.assert_array_equal(bits[:words], want_bits[:words])
+    side = np.empty(max(1, rec.size), dtype=np.uint64)
+    d_side.download(side)
+    assert n_side.value == int((~is_dom).sum())
+    index_of = np.searchsorted(reps, reps_of[~is_dom]).astype(np.uint64)
+    np.testing.assert_array_equal(reps[index_of.astype(np.int64)], reps_of[~is_dom])
+    np.testing.assert_array_equal(side[:n_side.value],
+                                  (rec[~is_dom] & np.uint64(0xFFFFFFFF00000000)) | index_of)
+    return {"bits": d_bits, "side": d_side, "n_side": n_side.value, "dominant": dom.value}
+
+
+@pytest.mark.parametrize("nl,roots,skew", [(5000, 37, 0.95), (20000, 3000, 0.5), (70_000, 1, 1.0),
+                                           (3000, 12, 0.0)])
+def test_prepared_rounds_follow_ghost_changes(gpu, nl, roots, skew):
+    """Merge rounds over the prepared records keep each rep's minimum ghost
+    label until told the ghost labels changed.  A loop that lowers a third
+    of the ghost labels after it converges (as an exchange would) and runs
+    on to convergence again ends with the labels of the same loop over the
+    full record list, and with the oracle's fixed point; with one rep it
+    matches the full-list rounds round for round, changed flags included.
+    `skew` is the share of records on one rep, as the rank's giant local
+    component holds most of them."""
+    import ctypes
+    from cc_mpi_amd import _native as nat
+    lib = nat.gpu_lib()
+    rng = np.random.default_rng(nl * 7 + roots)
+    vs, ve, n_global = 2000, 2000 + nl, 2000 + nl + 80_000
+    root_ids = np.sort(rng.choice(nl, roots, replace=False)).astype(np.uint32)
+    comp_of = root_ids[rng.integers(0, roots, nl)]
+    comp_of[root_ids] = root_ids
+    ghosts = np.unique(rng.integers(ve, n_global, 6000)).astype(np.uint32)
+    g_lab = rng.integers(0, n_global, ghosts.size).astype(np.uint32)
+    nb = 40_000
+    rep_pick = np.where(rng.random(nb) < skew, root_ids[0], comp_of[rng.integers(0, nl, nb)])
+    rec = np.unique((rng.integers(0, ghosts.size, nb).astype(np.uint64) << np.uint64(32))
+                    | rep_pick.astype(np.uint64))
+    lowered = g_lab.copy()
+    pick = rng.random(ghosts.size) < 1 / 3
+    lowered[pick] = np.minimum(lowered[pick],
+                               rng.integers(0, vs, int(pick.sum())).astype(np.uint32))
+    d_comp = cm.DeviceBuffer.from_array(comp_of)
+    d_reps = cm.DeviceBuffer(4 * nl)
+    cnt = ctypes.c_uint32()
+    nat.check(lib.cc_hip_local_reps_device(d_comp.ptr, nl, d_reps.ptr, ctypes.byref(cnt), None),
+              "reps")
+    d_rec = cm.DeviceBuffer.from_array(rec)
+    d_gh = cm.DeviceBuffer.from_array(ghosts)
+    reps = np.flatnonzero(comp_of == np.arange(nl)).astype(np.uint32)
+    prep = _prepare(lib, d_rec, rec, ghosts.size, d_reps, reps)
+
+    def run(prepared):
+        d_gl = cm.DeviceBuffer.from_array(g_lab)
+        d_cl = cm.DeviceBuffer.from_array(np.arange(vs, ve, dtype=np.uint32))
+        d_cm = cm.DeviceBuffer(4 * nl)
+        d_gmin = cm.DeviceBuffer(4 * reps.size)
+        changed = ctypes.c_int(-1)
+        trace, ghosts_changed = [], 2
+        for phase in range(2):
+            if phase == 1:
+                d_gl.upload(lowered)
+                ghosts_changed = 1
+            for _ in range(200):
+                if prepared:
+                    rc = lib.cc_hip_merge_step_prepared_device(
+                        prep["bits"].ptr, prep["dominant"], prep["side"].ptr, prep["n_side"],
+                        d_gh.ptr, d_gl.ptr, ghosts.size, d_comp.ptr, d_cl.ptr, d_cm.ptr,
+                        d_gmin.ptr, nl, d_reps.ptr, cnt.value, vs, ve, ghosts_changed,
+                        ctypes.byref(changed), None)
+                    ghosts_changed = 0
+                else:
+                    rc = lib.cc_hip_merge_step_reps_device(
+                        d_rec.ptr, rec.size, d_gh.ptr, d_gl.ptr, ghosts.size, d_comp.ptr,
+                        d_cl.ptr, d_cm.ptr, nl, d_reps.ptr, cnt.value, vs, ve,
+                        ctypes.byref(changed), None)
+                nat.check(rc, "merge step")
+                cl = np.empty(nl, dtype=np.uint32)
+                d_cl.download(cl)
+                trace.append((cl[root_ids].copy(), changed.value))
+                if not changed.value:
+                    break
+            else:
+                raise AssertionError("merge rounds did not converge")
+        for b in (d_gl, d_cl, d_cm, d_gmin):
+            b.close()
+        return trace
+
+    full, prepd = run(False), run(True)
+    np.testing.assert_array_equal(prepd[-1][0], full[-1][0])
+    if roots == 1:
+        assert len(prepd) == len(full)
+        for (a, fa), (b, fb) in zip(full, prepd):
+            np.testing.assert_array_equal(a, b)
+            assert fa == fb
+    # the oracle's fixed point under the lowered labels
+    o_cl = np.arange(vs, ve, dtype=np.uint32)
+    o_cm = np.zeros(nl, dtype=np.uint32)
+    for labels in (g_lab, lowered):
+        for _ in range(200):
+            if not oracle.merge_step(rec, ghosts, labels, comp_of, o_cl, o_cm, reps, vs, ve):
+                break
+    np.testing.assert_array_equal(prepd[-1][0], o_cl[root_ids])
+    for b in (d_comp, d_reps, d_rec, d_gh, prep["bits"], prep["side"]):
         b.close()
