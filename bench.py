@@ -141,6 +141,12 @@ def kernel_bytes(name: str, n: int, st: dict, plan: dict) -> float | None:
         # row_ptr, first col entry of each non-isolated vertex, own parent write
         return 8.0 * (n + 1) + 4.0 * d0 + 4.0 * n
     if name == "cc_hook_sample":
+        if plan.get("csr_hook"):
+            # the one pass from the CSR: row_ptr, the first two col entries
+            # of each row, own parent read and written (the link under
+            # neighbour 0 is a CAS of it), one parent gather per neighbour
+            # the walks unite
+            return 8.0 * (n + 1) + 4.0 * (d0 + d1) + 8.0 * n + 4.0 * (up0 + d1)
         if plan["fused"]:
             # both sample arrays, own parent, one parent gather per hooked neighbour
             return 8.0 * n + 4.0 * n + 4.0 * (up0 + d1)
@@ -179,7 +185,7 @@ def kernel_bytes(name: str, n: int, st: dict, plan: dict) -> float | None:
 def solve_plan(lib, n: int, m: int) -> dict:
     """What the solve does on this graph (cc_hip_components_plan)."""
     keys = ("cache", "fused", "rem", "rem_splice", "halve", "hook_ilp", "hook_blocks_per_cu",
-            "giant_early", "hook_mode", "nt", "init_ilp", "frontier_chunks")
+            "giant_early", "hook_mode", "nt", "init_ilp", "frontier_chunks", "csr_hook")
     return {k: int(lib.cc_hip_components_plan_get(n, m, k.encode())) for k in keys}
 
 

@@ -930,7 +930,17 @@ inline Plan plan_for(uint32_t n, uint64_t m, const Knobs &K = g_knobs) {
   // 65.6M 195 -> 146 us, R-MAT 22 33 -> 29 us, R-MAT 24 80 -> 71 us, ER 32M
   // 103 -> 99 us; four lose on every graph (register pressure)
   pl.frontier_chunks = K.frontier_chunks ? K.frontier_chunks : 2;
-  pl.csr = pl.fused && (pl.rem || K.hook_mode == 4) && (K.csr_hook < 0 ? 0 : K.csr_hook);
+  // The lockstep hook with one root per thread (dense graphs of 2^24+
+  // vertices past 2^31 entries) reads the rows itself instead of sample
+  // arrays: Chung-Lu 65.6M (3.60e9 entries) 3.30 -> 3.22 ms per solve,
+  // Chung-Lu 40M (2.31e9) 2.04 -> 1.98 ms, Chung-Lu 65.6M at gamma 2.3
+  // (3.82e9) 3.76 -> 3.60 ms.  The other plans keep the sampling init pass:
+  // with two roots per thread it is mixed (R-MAT 24 0.67 -> 0.63 ms, R-MAT
+  // 25 1.07 -> 1.14, R-MAT 26 2.27 -> 2.30), and Rem's unions lose badly
+  // (R-MAT 22 hook 93 -> 181 us, 8192^2 grid 3.76 -> 5.67 ms), since the row
+  // reads sit in front of every latency-bound union.
+  const int csr_auto = !pl.rem && K.hook_mode == 4 && big && dense && pl.hook_ilp == 1;
+  pl.csr = pl.fused && (pl.rem || K.hook_mode == 4) && (K.csr_hook < 0 ? csr_auto : K.csr_hook);
   return pl;
 }
 

@@ -241,10 +241,16 @@ PATHS = [  # every code path of the solve: (knobs) -> same labels
     dict(frontier_chunks=4, hook_mode=0), dict(frontier_chunks=2, hook_mode=1),
     dict(frontier_chunks=4, hook_mode=2, sample_cache=0, fused_pair=0),
     dict(frontier_chunks=4, hook_mode=3, sample_cache=0, fused_pair=0),
+    # the one-pass hooks that read the rows themselves after an identity init
+    dict(csr_hook=1, rem=0, halve=1, hook_ilp=1), dict(csr_hook=1, rem=0, halve=1, hook_ilp=2),
+    dict(csr_hook=1, rem=0, halve=0, hook_ilp=4), dict(csr_hook=1, rem=0, halve=1, hook_ilp=4),
+    dict(csr_hook=1, rem=1, rem_splice=0), dict(csr_hook=1, rem=1, rem_splice=1),
+    dict(csr_hook=1, rem=1, rem_splice=2), dict(csr_hook=0, rem=0, halve=1, hook_ilp=1),
+    dict(csr_hook=1, rem=0, halve=1, hook_ilp=1, blocks_per_cu=1),
 ]
 DEFAULTS = dict(sample_cache=1, fused_pair=-1, hook_ilp=0, hook_mode=4, sampled_neighbours=2,
                 stream_ilp=1, blocks_per_cu=0, giant_early=1, halve=-1, rem=-1, rem_splice=-1,
-                nt=-1, init_ilp=0, frontier_chunks=0)
+                nt=-1, init_ilp=0, frontier_chunks=0, csr_hook=-1)
 
 
 @pytest.mark.parametrize("knobs", PATHS, ids=lambda k: ",".join(f"{a}={b}" for a, b in k.items()))
