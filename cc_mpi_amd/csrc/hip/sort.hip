@@ -941,8 +941,9 @@ __device__ __forceinline__ uint32_t rep_index(const uint32_t *reps, uint32_t rep
 // lanes on one word ORs its bits together (a segmented reduction across
 // the wave) and the run's first lane sets them with one atomic.  The
 // records are unique, so every bit is new (no read-before-atomic filter).
-// A kept record's rep is replaced by its index in the rep list, so that the
-// rounds keep their per-rep state in an array of rep_count entries.
+// The kept records' reps become indices in the rep list afterwards
+// (k_side_rep_index), so that the rounds keep their per-rep state in an
+// array of rep_count entries.
 template <int R>
 struct DomRows {
   using Val = uint64_t;
@@ -952,9 +953,7 @@ struct DomRows {
   uint32_t nbits;  // ghost count: a record with gidx >= nbits is kept as a side record
   uint32_t *bitmap;
   uint64_t *side;
-  const uint32_t *reps;  // ascending; a side record's rep is replaced by its index here
-  uint32_t rep_count;
-  uint32_t *bad;  // set when a record's rep is not in the list
+
   template <int Rows>
   __device__ void rows(uint64_t wbase, int lane, uint64_t count, uint64_t (&v)[Rows],
                        uint64_t (&bal)[Rows]) const {
@@ -984,12 +983,37 @@ struct DomRows {
     }
   }
   __device__ void apply(uint64_t, bool keep, uint64_t rank, uint64_t v) const {
-    if (!keep) return;
-    const uint32_t k = rep_index(reps, rep_count, (uint32_t)v);
-    if (k == rep_count) *bad = 1u;
-    side[rank] = (v & 0xFFFFFFFF00000000ull) | k;
+    if (keep) side[rank] = v;
   }
 };
+
+// side[i] = (gidx << 32) | index of its rep in the rep list; *bad = 1 if
+// some rep is not in it.  A kernel of its own over the side records: the
+// same binary searches inside the split pass, row after row in a wave,
+// made that pass 4-7x slower.
+// Up to kLdsReps reps the list is staged in LDS first, so the 11 or so
+// dependent probes of each search are LDS reads rather than L1 ones.
+constexpr uint32_t kLdsReps = 8192;
+
+__global__ __launch_bounds__(kThreads) void k_side_rep_index(uint64_t *__restrict__ side, uint64_t n,
+                                                             const uint32_t *__restrict__ reps,
+                                                             uint32_t rep_count,
+                                                             uint32_t *__restrict__ bad) {
+  __shared__ uint32_t s_reps[kLdsReps];
+  const bool staged = rep_count <= kLdsReps;
+  if (staged) {
+    for (uint32_t k = threadIdx.x; k < rep_count; k += kThreads) s_reps[k] = reps[k];
+    __syncthreads();
+  }
+  for (uint64_t i = (uint64_t)blockIdx.x * kThreads + threadIdx.x; i < n;
+       i += (uint64_t)gridDim.x * kThreads) {
+    const uint64_t v = side[i];
+    const uint32_t k = staged ? rep_index(s_reps, rep_count, (uint32_t)v)
+                              : rep_index(reps, rep_count, (uint32_t)v);
+    if (k == rep_count) *bad = 1u;
+    side[i] = (v & 0xFFFFFFFF00000000ull) | k;
+  }
+}
 
 // out[0] = index of *rep in the rep list (rep_count if absent).  One thread.
 __global__ void k_rep_index(const uint32_t *__restrict__ reps, uint32_t rep_count,
@@ -1399,10 +1423,13 @@ extern "C" int cc_hip_merge_prepare_device(const uint64_t *d_records, uint64_t c
   uint64_t n_side = 0;
   if (int rc = run_compact_rows(
           "merge_prepare",
-          DomRows<32>{d_records, d_g, ghost_count, d_dom_bits, d_side, d_reps, rep_count, d_g + 2},
-          count,
+          DomRows<32>{d_records, d_g, ghost_count, d_dom_bits, d_side}, count,
           s, st, &n_side))
     return rc;
+  if (n_side > 0)
+    CCX_LAUNCH("merge_prepare_side", s,
+               k_side_rep_index<<<grid_cap(n_side, 8), kThreads, 0, s>>>(d_side, n_side, d_reps,
+                                                                         rep_count, d_g + 2));
   CCX_LAUNCH("merge_prepare_index", s, k_rep_index<<<1, 1, 0, s>>>(d_reps, rep_count, d_g, d_g + 1));
   uint32_t *h = (uint32_t *)st->h_stage;
   CCX_TRY(hipMemcpyAsync(h, d_g + 1, 2 * sizeof(uint32_t), hipMemcpyDeviceToHost, s));
