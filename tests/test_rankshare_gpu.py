@@ -419,3 +419,38 @@ def test_prepared_rounds_follow_ghost_changes(gpu, nl, roots, skew):
     np.testing.assert_array_equal(prepd[-1][0], o_cl[root_ids])
     for b in (d_comp, d_reps, d_rec, d_gh, prep["bits"], prep["side"]):
         b.close()
+
+
+def test_merge_prepare_edge_cases(gpu):
+    """No records: the bitmap is cleared (garbage in it beforehand) and the
+    side list is empty.  A record whose rep is not in the rep list, and a
+    dominant index past the rep list, are errors (code 4), not silent
+    out-of-range state."""
+    import ctypes
+    from cc_mpi_amd import _native as nat
+    lib = nat.gpu_lib()
+    reps = np.array([0, 5, 9], dtype=np.uint32)
+    d_reps = cm.DeviceBuffer.from_array(reps)
+    d_bits = cm.DeviceBuffer.from_array(np.full(4, 0xDEADBEEF, dtype=np.uint32))
+    d_side = cm.DeviceBuffer(8 * 8)
+    n_side, dom = ctypes.c_uint64(7), ctypes.c_uint32(7)
+    nat.check(lib.cc_hip_merge_prepare_device(None, 0, 100, d_reps.ptr, reps.size, d_bits.ptr,
+                                              d_side.ptr, ctypes.byref(n_side), ctypes.byref(dom),
+                                              None), "prepare, no records")
+    bits = np.empty(4, dtype=np.uint32)
+    d_bits.download(bits)
+    assert n_side.value == 0 and dom.value == 0 and not bits.any()
+    rec = np.array([(1 << 32) | 5, (2 << 32) | 5, (3 << 32) | 6], dtype=np.uint64)  # 6 is no rep
+    d_rec = cm.DeviceBuffer.from_array(rec)
+    rc = lib.cc_hip_merge_prepare_device(d_rec.ptr, rec.size, 100, d_reps.ptr, reps.size,
+                                         d_bits.ptr, d_side.ptr, ctypes.byref(n_side),
+                                         ctypes.byref(dom), None)
+    assert rc == 4 and b"not in the rep list" in lib.cc_hip_last_error()
+    d_one = cm.DeviceBuffer.from_array(np.zeros(16, dtype=np.uint32))
+    changed = ctypes.c_int()
+    rc = lib.cc_hip_merge_step_prepared_device(
+        d_bits.ptr, 3, d_side.ptr, 0, d_one.ptr, d_one.ptr, 1, d_one.ptr, d_one.ptr, d_one.ptr,
+        d_one.ptr, 10, d_reps.ptr, reps.size, 0, 10, 2, ctypes.byref(changed), None)
+    assert rc == 4 and b"dominant index" in lib.cc_hip_last_error()
+    for b in (d_reps, d_bits, d_side, d_rec, d_one):
+        b.close()
