@@ -191,26 +191,40 @@ def test_send_lists_match_their_definition(gpu, P, grouped, rows):
         b.close()
 
 
-@pytest.mark.parametrize("nl,roots", [(1, 1), (5000, 37), (70_000, 1), (5000, 500),
-                                      (20000, 3000)])
-def test_merge_step_over_rep_list(gpu, nl, roots):
+@pytest.mark.parametrize("nl,roots,nghost,layout",
+                         [(1, 1, 3000, "even"), (5000, 37, 3000, "even"), (70_000, 1, 3000, "even"),
+                          (5000, 500, 3000, "even"), (20000, 3000, 3000, "even"),
+                          (5000, 37, 60000, "even"), (5000, 37, 60000, "clustered"),
+                          (5000, 1, 60000, "clustered")])
+def test_merge_step_over_rep_list(gpu, nl, roots, nghost, layout):
     """cc_hip_local_reps_device lists exactly {r : comp_of[r] == r} in
     ascending order; merge rounds over that list and rounds that scan
     comp_of both equal the oracle's merge round (src/cc_mpi.c:547-622):
     round for round, with the changed flag, when there is one rep (no other
     rep's update can be seen mid-round), and at convergence otherwise (the
     reps update in parallel, as the reference's threads do, so one round's
-    labels depend on the order; the fixed point does not)."""
+    labels depend on the order; the fixed point does not).  With more than
+    16K ghosts the hops' ghost lookups bracket the target by interpolation
+    first: an even spread of ids takes that path, a clustered one (most ids
+    packed at the low end, a sparse tail) falls back to the full search."""
     import ctypes
     from cc_mpi_amd import _native as nat
     lib = nat.gpu_lib()
-    rng = np.random.default_rng(nl + roots)
-    vs, ve, n_global = 1000, 1000 + nl, 1000 + nl + 50_000
+    rng = np.random.default_rng(nl + roots + nghost)
+    vs, ve = 1000, 1000 + nl
+    n_global = ve + max(50_000, 3 * nghost)
     root_ids = np.sort(rng.choice(nl, roots, replace=False)).astype(np.uint32)
     comp_of = root_ids[rng.integers(0, roots, nl)]
     comp_of[root_ids] = root_ids
-    ghosts = np.unique(rng.integers(ve, n_global, 3000)).astype(np.uint32)
-    g_lab = rng.integers(0, n_global, ghosts.size).astype(np.uint32)
+    if layout == "even":
+        ghosts = np.unique(rng.integers(ve, n_global, nghost)).astype(np.uint32)
+    else:
+        dense = rng.integers(ve, ve + nghost, nghost * 9 // 10)
+        tail = rng.integers(ve + nghost, n_global + 40 * nghost, nghost // 10)
+        ghosts = np.unique(np.concatenate([dense, tail])).astype(np.uint32)
+    # labels: other ghosts' ids half the time, so the hops look ghosts up
+    g_lab = np.where(rng.random(ghosts.size) < 0.5, rng.choice(ghosts, ghosts.size),
+                     rng.integers(0, n_global, ghosts.size)).astype(np.uint32)
     nb = 20_000
     rec = np.unique((rng.integers(0, ghosts.size, nb).astype(np.uint64) << np.uint64(32))
                     | comp_of[rng.integers(0, nl, nb)].astype(np.uint64))
