@@ -422,14 +422,15 @@ def rankshare_leg(args) -> dict:
     # a first share, preprocessed and dropped, loads every kernel of the
     # path (each code object loads on its first launch, which a kernel
     # launched once per share would otherwise be timed with)
-    rs = RankShare(full, P=4, rank=0)
+    rank = args.rankshare_rank
+    rs = RankShare(full, P=4, rank=rank)
     rs.preprocess()
     cold = {k: round(v, 4) for k, v in rs.times.items()}
     rs.close()
     log(f"rankshare: cold preprocess {cold}")
     lib.cc_hip_prof_reset()
     lib.cc_hip_prof_enable(1)
-    rs = RankShare(full, P=4, rank=0)
+    rs = RankShare(full, P=4, rank=rank)
     rs.preprocess()
     log(f"rankshare: preprocess {rs.times} {rs.stats}")
     checked = [0]
@@ -443,8 +444,10 @@ def rankshare_leg(args) -> dict:
 
     runs = {}
     truth = full.labels
-    want = np.empty(rs.nl, dtype=np.uint32)
-    truth.download(want)  # rank 0's block is vertices [0, n_local)
+    everything = np.empty(full.n, dtype=np.uint32)
+    truth.download(everything)
+    want = everything[rs.vs:rs.ve].copy()  # the rank's block of vertices
+    del everything
     t0 = time.perf_counter()
     rs.prepare_schedule(truth, seed=11)  # the other ranks' deltas, built once
     log(f"rankshare: delta schedule built in {time.perf_counter() - t0:.1f}s")
@@ -476,7 +479,7 @@ def rankshare_leg(args) -> dict:
         kern[nm] = {"launches": int(c.value), "avg_us": round(1e3 * t.value / max(1, c.value), 2)}
     lib.cc_hip_prof_reset()
     out = {"graph": f"Chung-Lu n={full.n} pairs={w['pairs']} gamma {w['gamma']} seed "
-                    f"{w['seed']} (m={full.m}), rank 0 of 4 (vertex blocks)",
+                    f"{w['seed']} (m={full.m}), rank {rank} of 4 (vertex blocks)",
            "stats": rs.stats, "preprocess_seconds": {k: round(v, 4) for k, v in rs.times.items()
                                                       if k not in ("merge_steps", "delta_packs",
                                                                    "merge_loop")},
@@ -529,6 +532,8 @@ def main() -> int:
     ap.add_argument("--serving-pressure-requests", type=int, default=2000,
                     help="requests in the memory-pressure serving run")
     ap.add_argument("--serving-pressure-qps", type=float, default=2000.0)
+    ap.add_argument("--rankshare-rank", type=int, default=0, choices=range(4),
+                    help="which rank's share of the 4-rank partition the rank-share leg runs")
     ap.add_argument("--rankshare", action="store_true",
                     help="also run one rank's share of the 65.6M-vertex graph (4-rank blocks)")
     ap.add_argument("--serving-pool", type=int, default=16,
