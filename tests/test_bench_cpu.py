@@ -1,0 +1,98 @@
+"""Host logic of bench.py that needs no GPU: the frontier statistics that
+the algorithmic-byte model of the solve's kernels is built from, checked
+against a union-find restatement written here, and the one-GPU rule of
+the command line."""
+import os
+import subprocess
+import sys
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+sys.path.insert(0, ROOT)
+
+import bench  # noqa: E402
+import cc_mpi_amd as cm  # noqa: E402
+
+
+def _brute_stats(G):
+    """deg_gt0/deg_gt1/up0 by a loop over the vertices; the sampled forest
+    (every vertex joined with its neighbours 0 and 1) by a union-find; the
+    giant tree is the largest one (the graphs below make it hold most
+    vertices, so the sampled mode is it); frontier = vertices outside it
+    with more than two neighbours, and their entries past the first two."""
+    n = G.n
+    rp = G.row_ptr.astype(np.int64)
+    ci = G.col_idx.astype(np.int64)
+    parent = list(range(n))
+
+    def find(x):
+        while parent[x] != x:
+            parent[x] = parent[parent[x]]
+            x = parent[x]
+        return x
+
+    d0 = d1 = up0 = 0
+    for v in range(n):
+        deg = rp[v + 1] - rp[v]
+        d0 += deg > 0
+        d1 += deg > 1
+        if deg > 0 and ci[rp[v]] > v:
+            up0 += 1
+        for r in range(min(2, deg)):
+            a, b = find(v), find(int(ci[rp[v] + r]))
+            if a != b:
+                parent[max(a, b)] = min(a, b)
+    roots = [find(v) for v in range(n)]
+    vals, counts = np.unique(roots, return_counts=True)
+    giant = vals[np.argmax(counts)]
+    fv = fe = 0
+    for v in range(n):
+        deg = rp[v + 1] - rp[v]
+        if roots[v] != giant and deg > 2:
+            fv += 1
+            fe += deg - 2
+    return dict(deg_gt0=d0, deg_gt1=d1, up0=up0, frontier_vertices=fv, frontier_edges=fe)
+
+
+def _two_part_graph(seed):
+    """An R-MAT graph (its sampled forest has one dominant tree) plus a
+    disjoint clique block and isolated vertices: vertices that stay outside
+    the giant tree, some with more than two neighbours."""
+    G = cm.rmat(12, 8, seed=seed)
+    n0 = G.n
+    k = 9  # a 9-clique: degree 8, so 6 entries past the first two each
+    rows = [list(G.col_idx[G.row_ptr[v]:G.row_ptr[v + 1]]) for v in range(n0)]
+    for i in range(k):
+        rows.append([n0 + j for j in range(k) if j != i])
+    rows += [[], []]
+    row_ptr = np.zeros(len(rows) + 1, dtype=np.uint64)
+    row_ptr[1:] = np.cumsum([len(r) for r in rows])
+    col_idx = np.array([c for r in rows for c in r], dtype=np.uint32)
+    return _graph(len(rows), row_ptr, col_idx)
+
+
+def _graph(n, row_ptr, col_idx):
+    class G:  # the fields bench.frontier_stats reads
+        pass
+    g = G()
+    g.n, g.row_ptr, g.col_idx = n, row_ptr, col_idx
+    return g
+
+
+def test_frontier_stats_match_a_union_find_restatement():
+    for seed in (3, 4):
+        G = _two_part_graph(seed)
+        got = bench.frontier_stats(G, two_pass=False)
+        want = _brute_stats(G)
+        for k, v in want.items():
+            assert got[k] == v, (seed, k, got[k], v)
+        assert want["frontier_vertices"] >= 9  # the clique block is outside the giant
+        assert 0.5 < got["giant_fraction"] < 1.0
+
+
+def test_bench_refuses_more_than_one_gpu():
+    r = subprocess.run([sys.executable, os.path.join(ROOT, "bench.py"), "--gpus", "2"],
+                       capture_output=True, text=True, timeout=120)
+    assert r.returncode == 2
+    assert "one GPU only" in r.stdout + r.stderr
