@@ -73,19 +73,28 @@ def make_device_graph(name: str):
         f"Chung-Lu n={w['n']} pairs={w['pairs']} gamma {w['gamma']}, permuted, seed {w['seed']}")
 
 
-def frontier_stats(G, two_pass: bool = True):
+def frontier_stats(G, two_pass: bool = True, skip: bool = False):
     """Size of the frontier the hooking sweep visits: vertices not in the
     sampled giant tree after hooking each vertex with neighbours 0 and 1.
     The trees of that sampled forest are the components of the sampled edge
     set, so they are computed here independently with scipy.  two_pass: also
-    count what round 1 of the two-pass plan visits (another scipy pass)."""
+    count what round 1 of the two-pass plan visits (another scipy pass).
+    skip: the hook unites neighbour 1 only for local minima (neighbour 0
+    above the vertex), and the sweep starts at entry 1.  (A vertex whose
+    own link fails because another thread linked it first also unites
+    neighbour 1 on the device; that depends on timing and is rare, so the
+    model leaves it out.)"""
     import scipy.sparse as sp
     from scipy.sparse.csgraph import connected_components
     n = G.n
     deg = np.diff(G.row_ptr).astype(np.int64)
+    has0 = np.flatnonzero(deg > 0)
+    first = G.col_idx[G.row_ptr[has0].astype(np.int64)]
+    minimum = np.zeros(n, dtype=bool)
+    minimum[has0[first > has0]] = True
     src, dst = [], []
     for r in (0, 1):
-        has = np.flatnonzero(deg > r)
+        has = np.flatnonzero((deg > r) & (minimum if (skip and r == 1) else True))
         src.append(has)
         dst.append(G.col_idx[G.row_ptr[has].astype(np.int64) + r].astype(np.int64))
     src0, dst0 = src[0], dst[0]
@@ -104,10 +113,10 @@ def frontier_stats(G, two_pass: bool = True):
     vals, counts = np.unique(comp[pos], return_counts=True)
     giant = vals[np.argmax(counts)]
     front = comp != giant
-    rem = np.maximum(deg - 2, 0)
+    rem = np.maximum(deg - (1 if skip else 2), 0)
     active = front & (rem > 0)
-    has0 = np.flatnonzero(deg > 0)
-    up0 = int(np.count_nonzero(G.col_idx[G.row_ptr[has0].astype(np.int64)] > has0))
+    up0 = int(np.count_nonzero(minimum))
+    minima_gt1 = int(np.count_nonzero(minimum & (deg > 1)))
     live1 = None
     if two_pass:
         # forest after round 0 alone (neighbour 0 of every vertex) and its
@@ -120,7 +129,7 @@ def frontier_stats(G, two_pass: bool = True):
     return dict(frontier_vertices=int(active.sum()), frontier_edges=int(rem[active].sum()),
                 giant_fraction=float((~front).mean()),
                 deg_gt0=int((deg > 0).sum()), deg_gt1=int((deg > 1).sum()), up0=up0,
-                round1_outside_early_giant=live1)
+                minima_gt1=minima_gt1, round1_outside_early_giant=live1)
 
 
 def kernel_bytes(name: str, n: int, st: dict, plan: dict) -> float | None:
@@ -141,6 +150,23 @@ def kernel_bytes(name: str, n: int, st: dict, plan: dict) -> float | None:
         # row_ptr, first col entry of each non-isolated vertex, own parent write
         return 8.0 * (n + 1) + 4.0 * d0 + 4.0 * n
     if name == "cc_hook_sample":
+        skip = plan.get("skip_linked", 0)
+        m1 = st.get("minima_gt1", 0)
+        if skip and plan.get("csr_hook"):
+            # the one pass from the CSR with linked vertices skipping
+            # neighbour 1: row_ptr, the first col entry of each row and the
+            # second of each local minimum, own parent read and written,
+            # neighbour 0's parent for each linked vertex (one hop; with
+            # skip_linked 1 none), one parent gather per neighbour the
+            # minima's walks unite
+            hop = 4.0 * (d0 - up0) if skip == 2 else 0.0
+            return 8.0 * (n + 1) + 4.0 * (d0 + m1) + 8.0 * n + hop + 4.0 * (up0 + m1)
+        if skip and plan["fused"]:
+            # both sample arrays, own parent, neighbour 0's parent for each
+            # linked vertex (skip_linked 2), one parent gather per neighbour
+            # the minima unite
+            hop = 4.0 * (d0 - up0) if skip == 2 else 0.0
+            return 8.0 * n + 4.0 * n + hop + 4.0 * (up0 + m1)
         if plan.get("csr_hook"):
             # the one pass from the CSR: row_ptr, the first two col entries
             # of each row, own parent read and written (the link under
@@ -185,7 +211,8 @@ def kernel_bytes(name: str, n: int, st: dict, plan: dict) -> float | None:
 def solve_plan(lib, n: int, m: int) -> dict:
     """What the solve does on this graph (cc_hip_components_plan)."""
     keys = ("cache", "fused", "rem", "rem_splice", "halve", "hook_ilp", "hook_blocks_per_cu",
-            "giant_early", "hook_mode", "nt", "init_ilp", "frontier_chunks", "csr_hook")
+            "giant_early", "hook_mode", "nt", "init_ilp", "frontier_chunks", "csr_hook",
+            "skip_linked", "minima_entries", "compress_ilp")
     return {k: int(lib.cc_hip_components_plan_get(n, m, k.encode())) for k in keys}
 
 
@@ -617,7 +644,7 @@ def main() -> int:
     plan = solve_plan(lib, G.n, G.m)
     log(f"solve plan: {plan}")
     t0 = time.perf_counter()
-    st = frontier_stats(G, two_pass=not plan["fused"])
+    st = frontier_stats(G, two_pass=not plan["fused"], skip=bool(plan["skip_linked"]))
     st["components"] = int(np.count_nonzero(labels == np.arange(G.n, dtype=np.uint32)))
     log(f"frontier: {st} ({time.perf_counter() - t0:.1f}s)")
     roofline = None

@@ -22,18 +22,25 @@
 //      v's root and both neighbours' roots in lockstep with path halving
 //      (or, as a tuning choice, a pass per neighbour with pointer jumping
 //      between).  After this the forest already holds the giant component
-//      of a power-law or random graph in one tree.
+//      of a power-law or random graph in one tree.  On the largest dense
+//      graphs and on sparse graphs of degree 6 and more, a vertex linked
+//      under neighbour 0 skips neighbour 1 and takes neighbour 0's parent
+//      as its own (one hop): the trees then hang from the local minima
+//      (vertices below all their neighbours), which unite with both
+//      sampled neighbours, and those links join nearly every tree into the
+//      giant one while only the minima walk.
 //   3. The most frequent root among 1024 sampled vertices is taken as the
 //      giant root g.
 //   4. Frontier sweep: every vertex whose parent is not g hooks with all its
-//      remaining neighbours (indices 2..deg-1).  Edges between two vertices
-//      already under g are skipped; an edge between a g-vertex and another
-//      vertex is still seen from the other end because the graph is
-//      symmetric.  Each wave64 takes 64 consecutive vertices, compacts the
-//      active ones with a wave prefix sum of their remaining degrees and
-//      sweeps the concatenated neighbour lists 64 entries at a time, so the
-//      col_idx reads of a wave are contiguous runs regardless of how the
-//      degrees are spread over its lanes.
+//      remaining neighbours (indices 2..deg-1, or 1.. after skipping).
+//      Edges between two vertices already under g are skipped; an edge
+//      between a g-vertex and another vertex is still seen from the other
+//      end because the graph is symmetric.  Each wave64 takes 64
+//      consecutive vertices, compacts the active ones with a wave prefix
+//      sum of their remaining degrees and sweeps the concatenated neighbour
+//      lists 64 entries at a time, so the col_idx reads of a wave are
+//      contiguous runs regardless of how the degrees are spread over its
+//      lanes.
 //   5. Pointer jumping to the root: labels[v] = root(v) (skipped when the
 //      frontier sweep linked no root: it then wrote every final root).
 //
@@ -90,6 +97,14 @@ struct Knobs {
   // CSR itself, after an identity init, instead of from sample arrays
   // written by a sampling init pass (-1 = auto)
   int csr_hook = -1;
+  // a vertex linked under its neighbour 0 by its own entry skips the union
+  // with neighbour 1 (only local minima and failed links take both); the
+  // frontier sweep then starts at entry 1.  2: the link goes to neighbour
+  // 0's parent instead (one hop), which keeps the trees flat (-1 = auto)
+  int skip_linked = -1;
+  // entries of its row a local minimum (neighbour 0 above it) unites with
+  // when skipping is on (2..8; 0 = auto)
+  int minima_entries = 0;
 };
 // Process-wide settings (cc_hip_components_set / _tune); the entry point
 // without allocations always runs with the defaults.
@@ -517,7 +532,11 @@ __global__ __launch_bounds__(kBlock) void k_hook_cached(const uint32_t *__restri
 // pass between the rounds (and its pointer-jumping launch) disappears.
 // Halve: the three root walks halve the paths they follow.
 // Nt: the sample arrays are loaded non-temporally (streamed once).
-template <int Mode, int ILP, bool Halve = false, bool Nt = false>
+//
+// Skip1: a vertex the init kernel linked under neighbour 0 skips neighbour
+// 1 (2: and its entry is lowered to neighbour 0's parent, one hop); only
+// local minima unite with both (the reasoning at k_hook_rem).
+template <int Mode, int ILP, bool Halve = false, bool Nt = false, int Skip1 = 0>
 __global__ __launch_bounds__(kBlock) void k_hook_pair(const uint32_t *__restrict__ s0,
                                                       const uint32_t *__restrict__ s1,
                                                       uint32_t n, uint32_t *__restrict__ p) {
@@ -532,7 +551,11 @@ __global__ __launch_bounds__(kBlock) void k_hook_pair(const uint32_t *__restrict
       const uint32_t w1 = v < n ? (Nt ? __builtin_nontemporal_load(&s1[v]) : s1[v]) : kNoNeighbour;
       // neighbour 0 below v was linked by the init kernel already
       act[ILP + k] = w0 < n && w0 > (uint32_t)v;
-      act[2 * ILP + k] = w1 < n && w1 != (uint32_t)v;
+      if (Skip1 == 2 && w0 < (uint32_t)v) {
+        const uint32_t q = p[w0];
+        if (q < w0) (void)__hip_atomic_fetch_min(&p[v], q, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      }
+      act[2 * ILP + k] = w1 < n && w1 != (uint32_t)v && (!Skip1 || act[ILP + k]);
       act[k] = act[ILP + k] || act[2 * ILP + k];
       x[k] = (uint32_t)v;
       x[ILP + k] = w0;
@@ -604,16 +627,44 @@ __device__ __forceinline__ void rem_unite(const uint32_t v, const uint32_t w,
 // pass, no lockstep: each union ends as soon as the two paths meet, which
 // on graphs whose trees are shallow and hub-centred is well before the
 // roots).
-template <int Splice, bool Nt = false>
+//
+// Skip1: a vertex that the init kernel linked under neighbour 0 skips
+// neighbour 1, and a local minimum (neighbour 0 above it) unites with the
+// first `minima_entries` entries of its row (read from the CSR).  Every
+// vertex still hangs under neighbour 0's tree; the trees are joined at
+// their roots, the local minima, each with up to `minima_entries` links
+// leaving it, which on power-law and random graphs joins nearly all of them
+// into the giant tree while only the minima pay for random walks.  The
+// frontier sweep then starts at entry 1.
+template <int Splice, bool Nt = false, int Skip1 = 0>
 __global__ __launch_bounds__(kBlock) void k_hook_rem(const uint32_t *__restrict__ s0,
                                                      const uint32_t *__restrict__ s1, uint32_t n,
-                                                     uint32_t *__restrict__ p) {
+                                                     uint32_t *__restrict__ p,
+                                                     const uint64_t *__restrict__ row_ptr = nullptr,
+                                                     const uint32_t *__restrict__ col = nullptr,
+                                                     uint32_t minima_entries = 2) {
   CCX_GRID_STRIDE(v, n) {
     const uint32_t w0 = Nt ? __builtin_nontemporal_load(&s0[v]) : s0[v];
     const uint32_t w1 = Nt ? __builtin_nontemporal_load(&s1[v]) : s1[v];
     // neighbour 0 below v was linked by the init kernel already
-    if (w0 < n && w0 > v) rem_unite<Splice>(v, w0, p);
-    if (w1 < n && w1 != v) rem_unite<Splice>(v, w1, p);
+    const bool up0 = w0 < n && w0 > v;
+    if (up0) rem_unite<Splice>(v, w0, p);
+    if (Skip1 == 2 && w0 < v) {
+      // one hop: v's entry (w0, set by the init kernel) lowered to w0's
+      // parent, an ancestor of w0 and so a vertex of v's component below v
+      const uint32_t q = p[w0];
+      if (q < w0) (void)__hip_atomic_fetch_min(&p[v], q, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
+    if (w1 < n && w1 != v && (!Skip1 || up0)) rem_unite<Splice>(v, w1, p);
+    if (Skip1 && up0 && minima_entries > 2) {
+      const uint64_t b = row_ptr[v] + 2;
+      const uint64_t e = row_ptr[v + 1];
+      const uint64_t stop = b + (minima_entries - 2) < e ? b + (minima_entries - 2) : e;
+      for (uint64_t i = b; i < stop; ++i) {
+        const uint32_t w = col[i];
+        if (w < n && w != v) rem_unite<Splice>(v, w, p);
+      }
+    }
   }
 }
 
@@ -646,24 +697,46 @@ __device__ __forceinline__ bool link_first(uint32_t v, uint32_t w0, uint32_t n,
   return atomicCAS(&p[v], v, w0) != v;
 }
 
-template <int Mode, int ILP, bool Halve>
+//
+// Skip1: a vertex whose own-entry link to neighbour 0 succeeds skips
+// neighbour 1; a local minimum (neighbour 0 above it) or a vertex whose
+// link failed unites with both, and a local minimum also with entries
+// 2 .. minima_entries-1 of its row (the reasoning at k_hook_rem).
+template <int Mode, int ILP, bool Halve, int Skip1 = 0>
 __global__ __launch_bounds__(kBlock) void k_hook_pair_csr(const uint64_t *__restrict__ row_ptr,
                                                           const uint32_t *__restrict__ col,
-                                                          uint32_t n, uint32_t *__restrict__ p) {
+                                                          uint32_t n, uint32_t *__restrict__ p,
+                                                          uint32_t minima_entries = 2) {
   const uint64_t S = (uint64_t)gridDim.x * kBlock;
   for (uint64_t v0 = (uint64_t)blockIdx.x * kBlock + threadIdx.x; v0 < n; v0 += S * ILP) {
     uint32_t w0[ILP], w1[ILP];
+    uint64_t rb[ILP], re[ILP];
 #pragma unroll
-    for (int k = 0; k < ILP; ++k) csr_first_two(row_ptr, col, v0 + k * S, n, w0[k], w1[k]);
+    for (int k = 0; k < ILP; ++k) {
+      const uint64_t v = v0 + k * S;
+      rb[k] = v < n ? row_ptr[v] : 0;
+      re[k] = v < n ? row_ptr[v + 1] : 0;
+      w0[k] = rb[k] < re[k] ? col[rb[k]] : kNoNeighbour;
+      w1[k] = rb[k] + 1 < re[k] ? col[rb[k] + 1] : kNoNeighbour;
+    }
     uint32_t x[3 * ILP];
     bool act[3 * ILP];
 #pragma unroll
     for (int k = 0; k < ILP; ++k) {
       const uint64_t v = v0 + k * S;
       const bool live = v < n;
-      act[ILP + k] = live && link_first((uint32_t)v, w0[k], n, p);
-      act[2 * ILP + k] = live && w1[k] < n && w1[k] != (uint32_t)v;
-      act[k] = act[ILP + k] || act[2 * ILP + k];
+      if (Skip1 == 2 && live && w0[k] < (uint32_t)v) {
+        // one hop: link v under neighbour 0's parent (an ancestor of
+        // neighbour 0, below v) by the CAS of its own entry
+        const uint32_t q = p[w0[k]];
+        act[ILP + k] = atomicCAS(&p[v], (uint32_t)v, q < w0[k] ? q : w0[k]) != (uint32_t)v;
+      } else {
+        act[ILP + k] = live && link_first((uint32_t)v, w0[k], n, p);
+      }
+      act[2 * ILP + k] = live && w1[k] < n && w1[k] != (uint32_t)v && (!Skip1 || act[ILP + k]);
+      // Skip1 == 3: a linked vertex still walks (and, with Halve, halves)
+      // its own path, which keeps the trees flat for the frontier sweep
+      act[k] = act[ILP + k] || act[2 * ILP + k] || (Skip1 == 3 && live && w0[k] < (uint32_t)v);
       x[k] = (uint32_t)v;
       x[ILP + k] = w0[k];
       x[2 * ILP + k] = w1[k];
@@ -679,6 +752,19 @@ __global__ __launch_bounds__(kBlock) void k_hook_pair_csr(const uint64_t *__rest
         rv = rv < r0 ? rv : r0;  // a member of the merged tree
       }
       if (act[2 * ILP + k] && rv != r1) hook_roots<Mode>(rv, r1, p);
+    }
+    if (Skip1 && minima_entries > 2) {
+#pragma unroll
+      for (int k = 0; k < ILP; ++k) {
+        const uint64_t v = v0 + k * S;
+        if (v >= n || w0[k] >= n || w0[k] <= (uint32_t)v) continue;
+        const uint64_t b = rb[k] + 2;
+        const uint64_t stop = b + (minima_entries - 2) < re[k] ? b + (minima_entries - 2) : re[k];
+        for (uint64_t i = b; i < stop; ++i) {
+          const uint32_t w = col[i];
+          if (w < n && w != (uint32_t)v) (void)hook<Mode>((uint32_t)v, w, p);
+        }
+      }
     }
   }
 }
@@ -888,6 +974,7 @@ inline unsigned blocks_for(uint64_t n, int cu_count, int per_cu) {
 constexpr uint64_t kDenseDegree = 16;              // dense: m >= 16 n
 constexpr uint64_t kHalveMinVertices = 1ull << 24;  // halving from this many vertices
 constexpr uint64_t kWideIlpMaxEntries = 1ull << 31; // dense one-pass ILP 2 below this many entries
+constexpr uint64_t kSkipMinDegree = 6;              // sparse graphs skip from m >= 6 n
 
 struct Plan {
   int cache;     // sampled-neighbour cache in use
@@ -901,6 +988,9 @@ struct Plan {
   int init_ilp;  // rows per thread per step of the sampling init pass
   int frontier_chunks;  // chunks per wave step of the frontier sweep
   int csr;       // one-pass hook from the CSR (k_hook_pair_csr / k_hook_rem_csr), no sample arrays
+  int skip1;     // own-entry links skip neighbour 1; the frontier sweep starts at entry 1
+  int minima_entries;  // ... and local minima unite with this many entries of their row
+  int compress_ilp;    // vertices per thread per step of the pointer-jumping passes
 };
 
 inline Plan plan_for(uint32_t n, uint64_t m, const Knobs &K = g_knobs) {
@@ -941,6 +1031,35 @@ inline Plan plan_for(uint32_t n, uint64_t m, const Knobs &K = g_knobs) {
   // reads sit in front of every latency-bound union.
   const int csr_auto = !pl.rem && K.hook_mode == 4 && big && dense && pl.hook_ilp == 1;
   pl.csr = pl.fused && (pl.rem || K.hook_mode == 4) && (K.csr_hook < 0 ? csr_auto : K.csr_hook);
+  // Skipping neighbour 1 of linked vertices (Knobs::skip_linked; the
+  // lockstep hooks and Rem's unions from the sample arrays) trades the hook's
+  // second walk per vertex for a frontier sweep over the few vertices the
+  // local minima's links leave outside the giant tree:
+  //  * the lockstep hook from the CSR, with one-hop links and 16 workgroups
+  //    per CU: Chung-Lu 65.6M 3.21 -> 2.49 ms per solve, Chung-Lu 40M
+  //    2.08 -> 1.61, R-MAT 27 (4.2e9 entries) 4.18 -> 3.92 (4.18 being the
+  //    plan without skipping at 8-16 workgroups per CU; 5.68 at 4); without
+  //    skipping, 16 workgroups per CU lose on Chung-Lu (3.21 -> 3.38);
+  //  * Rem's unions on sparse graphs of at least 6 entries per vertex: ER
+  //    4M (degree 8) 390 -> 241 us, ER 32M 2.24 -> 1.66 ms;
+  //  * it loses where the minima's links do not join the trees: the 8192^2
+  //    lattice (2.8 entries per vertex) 5.2 -> 8.8 ms, and R-MAT 22/24/25
+  //    (163 -> 204 us, 667 -> 698, 1063 -> 1172), whose trees come out
+  //    deeper, so the frontier sweep costs more than the hook saves.
+  //    Chung-Lu 16.4M on the sample-array lockstep hook gains (818 -> 687
+  //    us) but shares that plan with R-MAT 24/25, which keeps both unions.
+  //  * Local minima uniting with more of their row (minima_entries 3, 4)
+  //    shrink the frontier but stall their waves: Chung-Lu 65.6M 2.73 ->
+  //    3.02 ms, ER 32M 1.66 -> 1.85 ms.
+  const bool skip_ok = pl.fused && (pl.rem ? !pl.csr : K.hook_mode == 4);
+  const bool skip_auto = (pl.csr && !pl.rem) || (pl.rem && !pl.csr && !dense && m >= kSkipMinDegree * (uint64_t)n);
+  pl.skip1 = skip_ok ? (K.skip_linked < 0 ? (skip_auto ? 2 : 0) : K.skip_linked) : 0;
+  pl.minima_entries = K.minima_entries ? K.minima_entries : 2;
+  if (!K.blocks_per_cu && pl.skip1 && pl.csr) pl.hook_bpc = 16;
+  // the last pointer-jumping pass runs after a skipping hook (the frontier
+  // sweep links the vertices left outside the giant tree): 4 vertices per
+  // thread, Chung-Lu 65.6M 87 -> 75 us
+  pl.compress_ilp = K.stream_ilp > 1 ? K.stream_ilp : (pl.skip1 ? 4 : 1);
   return pl;
 }
 
@@ -968,6 +1087,8 @@ extern "C" int cc_hip_components_set(const char *key, int value) {
   else if (k == "init_ilp" && (value == 0 || ilp_ok(value))) g_knobs.init_ilp = value;
   else if (k == "frontier_chunks" && (value == 0 || ilp_ok(value))) g_knobs.frontier_chunks = value;
   else if (k == "csr_hook" && value >= -1 && value <= 1) g_knobs.csr_hook = value;
+  else if (k == "skip_linked" && value >= -1 && value <= 3) g_knobs.skip_linked = value;
+  else if (k == "minima_entries" && (value == 0 || (value >= 2 && value <= 8))) g_knobs.minima_entries = value;
   else return ccx::set_error(4, "cc_hip_components_set: bad key or value (%s=%d)", key, value);
   return 0;
 }
@@ -991,6 +1112,8 @@ extern "C" int cc_hip_components_get(const char *key) {
   if (k == "init_ilp") return g_knobs.init_ilp;
   if (k == "frontier_chunks") return g_knobs.frontier_chunks;
   if (k == "csr_hook") return g_knobs.csr_hook;
+  if (k == "skip_linked") return g_knobs.skip_linked;
+  if (k == "minima_entries") return g_knobs.minima_entries;
   return -1;
 }
 
@@ -1020,6 +1143,9 @@ extern "C" int cc_hip_components_plan_get(uint32_t n, uint64_t m, const char *ke
   if (k == "init_ilp") return pl.init_ilp;
   if (k == "frontier_chunks") return pl.frontier_chunks;
   if (k == "csr_hook") return pl.csr;
+  if (k == "skip_linked") return pl.skip1;
+  if (k == "minima_entries") return pl.minima_entries;
+  if (k == "compress_ilp") return pl.compress_ilp;
   if (k == "hook_blocks_per_cu") return pl.hook_bpc;
   if (k == "giant_early") return pl.cache && !pl.fused && g_knobs.giant_early;
   if (k == "hook_mode") return g_knobs.hook_mode;
@@ -1049,6 +1175,46 @@ extern "C" int cc_hip_components_tune(int sampled_neighbours, int blocks_per_cu,
 
 namespace {
 
+// Launches of the one-pass hooks from the sample arrays, by template
+// arguments chosen at run time.
+template <int Splice, bool Nt>
+int launch_hook_rem(int skip, unsigned nb, hipStream_t s, const uint32_t *s0, const uint32_t *s1,
+                    uint32_t n, uint32_t *p, const uint64_t *row_ptr, const uint32_t *col,
+                    uint32_t minima_entries) {
+  if (skip == 2)
+    CCX_LAUNCH("cc_hook_sample", s, k_hook_rem<Splice, Nt, 2><<<nb, kBlock, 0, s>>>(s0, s1, n, p, row_ptr, col, minima_entries));
+  else if (skip)
+    CCX_LAUNCH("cc_hook_sample", s, k_hook_rem<Splice, Nt, 1><<<nb, kBlock, 0, s>>>(s0, s1, n, p, row_ptr, col, minima_entries));
+  else
+    CCX_LAUNCH("cc_hook_sample", s, k_hook_rem<Splice, Nt, 0><<<nb, kBlock, 0, s>>>(s0, s1, n, p));
+  return 0;
+}
+
+template <int Mode, int ILP, bool Halve, bool Nt>
+int launch_hook_pair(int skip, unsigned nb, hipStream_t s, const uint32_t *s0, const uint32_t *s1,
+                     uint32_t n, uint32_t *p) {
+  if constexpr (Mode == 4) {
+    if (skip == 2) {
+      CCX_LAUNCH("cc_hook_sample", s, k_hook_pair<Mode, ILP, Halve, Nt, 2><<<nb, kBlock, 0, s>>>(s0, s1, n, p));
+      return 0;
+    }
+    if (skip) {
+      CCX_LAUNCH("cc_hook_sample", s, k_hook_pair<Mode, ILP, Halve, Nt, 1><<<nb, kBlock, 0, s>>>(s0, s1, n, p));
+      return 0;
+    }
+  }
+  CCX_LAUNCH("cc_hook_sample", s, k_hook_pair<Mode, ILP, Halve, Nt><<<nb, kBlock, 0, s>>>(s0, s1, n, p));
+  return 0;
+}
+
+template <int Mode, bool Halve, bool Nt>
+int launch_hook_pair_ilp(int ilp, int skip, unsigned nb, hipStream_t s, const uint32_t *s0,
+                         const uint32_t *s1, uint32_t n, uint32_t *p) {
+  if (ilp == 4) return launch_hook_pair<Mode, 4, Halve, Nt>(skip, nb, s, s0, s1, n, p);
+  if (ilp == 2) return launch_hook_pair<Mode, 2, Halve, Nt>(skip, nb, s, s0, s1, n, p);
+  return launch_hook_pair<Mode, 1, Halve, Nt>(skip, nb, s, s0, s1, n, p);
+}
+
 // The whole solve on stream s with knobs K.  words[0..1]: the sampled giant
 // root and the frontier sweep's link flag (written before they are read);
 // smp: rounds * n words for the sampled-neighbour cache when the plan uses
@@ -1065,6 +1231,7 @@ int solve_with(const Knobs &K, const ccx::Context *c, const uint64_t *d_row_ptr,
   const unsigned ns = blocks_for(n, c->cu_count, K.stream_blocks_per_cu);
   const uint32_t rounds = (uint32_t)K.sampled_neighbours;
   const int ilp = K.stream_ilp;
+  const uint32_t me = (uint32_t)plan.minima_entries;
 
   if (m == 0) {
     CCX_LAUNCH("cc_init", s, k_init<<<ns, kBlock, 0, s>>>(n, d_labels));
@@ -1073,8 +1240,9 @@ int solve_with(const Knobs &K, const ccx::Context *c, const uint64_t *d_row_ptr,
   // gate: nullptr for a pass between hook rounds, the frontier's link flag
   // for the last pass
   auto compress = [&](const uint32_t *gate) -> int {
-    if (ilp == 4) CCX_LAUNCH("cc_compress", s, k_compress_ilp<4><<<ns, kBlock, 0, s>>>(n, d_labels, gate));
-    else if (ilp == 2) CCX_LAUNCH("cc_compress", s, k_compress_ilp<2><<<ns, kBlock, 0, s>>>(n, d_labels, gate));
+    const int cilp = plan.compress_ilp;
+    if (cilp == 4) CCX_LAUNCH("cc_compress", s, k_compress_ilp<4><<<ns, kBlock, 0, s>>>(n, d_labels, gate));
+    else if (cilp == 2) CCX_LAUNCH("cc_compress", s, k_compress_ilp<2><<<ns, kBlock, 0, s>>>(n, d_labels, gate));
     else CCX_LAUNCH("cc_compress", s, k_compress<<<ns, kBlock, 0, s>>>(n, d_labels, gate));
     return 0;
   };
@@ -1092,17 +1260,23 @@ int solve_with(const Knobs &K, const ccx::Context *c, const uint64_t *d_row_ptr,
       CCX_LAUNCH("cc_hook_sample", s, k_hook_rem_csr<2><<<nb, kBlock, 0, s>>>(d_row_ptr, d_col_idx, n, d_labels));
     else if (plan.rem)
       CCX_LAUNCH("cc_hook_sample", s, k_hook_rem_csr<0><<<nb, kBlock, 0, s>>>(d_row_ptr, d_col_idx, n, d_labels));
-#define CCX_HOOK_CSR(H)                                                                            \
+#define CCX_HOOK_CSR(H, K1)                                                                          \
   do {                                                                                             \
     if (hook_ilp == 4)                                                                             \
-      CCX_LAUNCH("cc_hook_sample", s, k_hook_pair_csr<4, 4, H><<<nb, kBlock, 0, s>>>(d_row_ptr, d_col_idx, n, d_labels)); \
+      CCX_LAUNCH("cc_hook_sample", s, k_hook_pair_csr<4, 4, H, K1><<<nb, kBlock, 0, s>>>(d_row_ptr, d_col_idx, n, d_labels, me)); \
     else if (hook_ilp == 2)                                                                        \
-      CCX_LAUNCH("cc_hook_sample", s, k_hook_pair_csr<4, 2, H><<<nb, kBlock, 0, s>>>(d_row_ptr, d_col_idx, n, d_labels)); \
+      CCX_LAUNCH("cc_hook_sample", s, k_hook_pair_csr<4, 2, H, K1><<<nb, kBlock, 0, s>>>(d_row_ptr, d_col_idx, n, d_labels, me)); \
     else                                                                                           \
-      CCX_LAUNCH("cc_hook_sample", s, k_hook_pair_csr<4, 1, H><<<nb, kBlock, 0, s>>>(d_row_ptr, d_col_idx, n, d_labels)); \
+      CCX_LAUNCH("cc_hook_sample", s, k_hook_pair_csr<4, 1, H, K1><<<nb, kBlock, 0, s>>>(d_row_ptr, d_col_idx, n, d_labels, me)); \
   } while (0)
-    else if (plan.halve) CCX_HOOK_CSR(true);
-    else CCX_HOOK_CSR(false);
+    else if (plan.halve && plan.skip1 == 3) CCX_HOOK_CSR(true, 3);
+    else if (plan.halve && plan.skip1 == 2) CCX_HOOK_CSR(true, 2);
+    else if (plan.halve && plan.skip1) CCX_HOOK_CSR(true, 1);
+    else if (plan.halve) CCX_HOOK_CSR(true, 0);
+    else if (plan.skip1 == 3) CCX_HOOK_CSR(false, 3);
+    else if (plan.skip1 == 2) CCX_HOOK_CSR(false, 2);
+    else if (plan.skip1) CCX_HOOK_CSR(false, 1);
+    else CCX_HOOK_CSR(false, 0);
 #undef CCX_HOOK_CSR
   } else if (cache) {
     if (!smp) return ccx::set_error(4, "solve: no sample buffer");
@@ -1147,38 +1321,32 @@ int solve_with(const Knobs &K, const ccx::Context *c, const uint64_t *d_row_ptr,
     // hooked above
   } else if (plan.fused) {
     const uint32_t *s1 = smp + n;
-#define CCX_HOOK_PAIR(M, H)                                                                        \
-  do {                                                                                             \
-    if (hook_ilp == 4)                                                                             \
-      CCX_LAUNCH("cc_hook_sample", s, k_hook_pair<M, 4, H><<<nb, kBlock, 0, s>>>(smp, s1, n, d_labels)); \
-    else if (hook_ilp == 2)                                                                        \
-      CCX_LAUNCH("cc_hook_sample", s, k_hook_pair<M, 2, H><<<nb, kBlock, 0, s>>>(smp, s1, n, d_labels)); \
-    else                                                                                           \
-      CCX_LAUNCH("cc_hook_sample", s, k_hook_pair<M, 1, H><<<nb, kBlock, 0, s>>>(smp, s1, n, d_labels)); \
-  } while (0)
-    if (mode == 0)
+    const int sk = plan.skip1;
+    int rc = 0;
+    if (mode == 0) {
       CCX_LAUNCH("cc_hook_sample", s, k_hook_pair<0, 2><<<nb, kBlock, 0, s>>>(smp, s1, n, d_labels));
-    else if (plan.rem && plan.splice == 1 && (plan.nt & 4))
-      CCX_LAUNCH("cc_hook_sample", s, k_hook_rem<1, true><<<nb, kBlock, 0, s>>>(smp, s1, n, d_labels));
-    else if (plan.rem && plan.splice == 1)
-      CCX_LAUNCH("cc_hook_sample", s, k_hook_rem<1><<<nb, kBlock, 0, s>>>(smp, s1, n, d_labels));
-    else if (plan.rem && plan.splice == 2 && (plan.nt & 4))
-      CCX_LAUNCH("cc_hook_sample", s, k_hook_rem<2, true><<<nb, kBlock, 0, s>>>(smp, s1, n, d_labels));
-    else if (plan.rem && plan.splice == 2)
-      CCX_LAUNCH("cc_hook_sample", s, k_hook_rem<2><<<nb, kBlock, 0, s>>>(smp, s1, n, d_labels));
-    else if (plan.rem)
-      CCX_LAUNCH("cc_hook_sample", s, k_hook_rem<0><<<nb, kBlock, 0, s>>>(smp, s1, n, d_labels));
-    else if (mode == 4 && plan.halve && (plan.nt & 4) && hook_ilp == 4)
-      CCX_LAUNCH("cc_hook_sample", s, k_hook_pair<4, 4, true, true><<<nb, kBlock, 0, s>>>(smp, s1, n, d_labels));
-    else if (mode == 4 && plan.halve && (plan.nt & 4) && hook_ilp == 2)
-      CCX_LAUNCH("cc_hook_sample", s, k_hook_pair<4, 2, true, true><<<nb, kBlock, 0, s>>>(smp, s1, n, d_labels));
-    else if (mode == 4 && plan.halve && (plan.nt & 4))
-      CCX_LAUNCH("cc_hook_sample", s, k_hook_pair<4, 1, true, true><<<nb, kBlock, 0, s>>>(smp, s1, n, d_labels));
-    else if (mode == 1 && plan.halve) CCX_HOOK_PAIR(1, true);
-    else if (mode == 1) CCX_HOOK_PAIR(1, false);
-    else if (plan.halve) CCX_HOOK_PAIR(4, true);
-    else CCX_HOOK_PAIR(4, false);
-#undef CCX_HOOK_PAIR
+    } else if (plan.rem) {
+      const bool nt = (plan.nt & 4) != 0;
+      if (plan.splice == 1)
+        rc = nt ? launch_hook_rem<1, true>(sk, nb, s, smp, s1, n, d_labels, d_row_ptr, d_col_idx, me)
+                : launch_hook_rem<1, false>(sk, nb, s, smp, s1, n, d_labels, d_row_ptr, d_col_idx, me);
+      else if (plan.splice == 2)
+        rc = nt ? launch_hook_rem<2, true>(sk, nb, s, smp, s1, n, d_labels, d_row_ptr, d_col_idx, me)
+                : launch_hook_rem<2, false>(sk, nb, s, smp, s1, n, d_labels, d_row_ptr, d_col_idx, me);
+      else
+        rc = nt ? launch_hook_rem<0, true>(sk, nb, s, smp, s1, n, d_labels, d_row_ptr, d_col_idx, me)
+                : launch_hook_rem<0, false>(sk, nb, s, smp, s1, n, d_labels, d_row_ptr, d_col_idx, me);
+    } else if (mode == 1) {
+      rc = plan.halve ? launch_hook_pair_ilp<1, true, false>(hook_ilp, 0, nb, s, smp, s1, n, d_labels)
+                      : launch_hook_pair_ilp<1, false, false>(hook_ilp, 0, nb, s, smp, s1, n, d_labels);
+    } else if (plan.halve && (plan.nt & 4)) {
+      rc = launch_hook_pair_ilp<4, true, true>(hook_ilp, sk, nb, s, smp, s1, n, d_labels);
+    } else if (plan.halve) {
+      rc = launch_hook_pair_ilp<4, true, false>(hook_ilp, sk, nb, s, smp, s1, n, d_labels);
+    } else {
+      rc = launch_hook_pair_ilp<4, false, false>(hook_ilp, sk, nb, s, smp, s1, n, d_labels);
+    }
+    if (rc) return rc;
   } else {
     for (uint32_t r = 0; r < rounds; ++r) {
       if (r > 0)
@@ -1229,7 +1397,9 @@ int solve_with(const Knobs &K, const ccx::Context *c, const uint64_t *d_row_ptr,
   // Exact sampling (modes 0-2) links neighbours 0..rounds-1 of every vertex,
   // so the sweep starts after them; lossy sampling (mode 3) links nothing
   // for sure, so the sweep covers every neighbour.
-  const uint32_t first = (mode == 3) ? 0u : rounds;
+  // With skipping, neighbour 1 of a linked vertex was not united, so the
+  // sweep starts at entry 1 (entry 0 is always united).
+  const uint32_t first = (mode == 3) ? 0u : (plan.skip1 ? 1u : rounds);
 #define CCX_FRONTIER(M)                                                                            \
   do {                                                                                             \
     if (plan.frontier_chunks == 4)                                                                 \

@@ -15,12 +15,14 @@ import bench  # noqa: E402
 import cc_mpi_amd as cm  # noqa: E402
 
 
-def _brute_stats(G):
+def _brute_stats(G, skip=False):
     """deg_gt0/deg_gt1/up0 by a loop over the vertices; the sampled forest
     (every vertex joined with its neighbours 0 and 1) by a union-find; the
     giant tree is the largest one (the graphs below make it hold most
     vertices, so the sampled mode is it); frontier = vertices outside it
-    with more than two neighbours, and their entries past the first two."""
+    with more than two neighbours, and their entries past the first two.
+    skip: only local minima (neighbour 0 above them) join neighbour 1, and
+    the frontier counts entries past the first."""
     n = G.n
     rp = G.row_ptr.astype(np.int64)
     ci = G.col_idx.astype(np.int64)
@@ -32,14 +34,15 @@ def _brute_stats(G):
             x = parent[x]
         return x
 
-    d0 = d1 = up0 = 0
+    d0 = d1 = up0 = m1 = 0
     for v in range(n):
         deg = rp[v + 1] - rp[v]
         d0 += deg > 0
         d1 += deg > 1
-        if deg > 0 and ci[rp[v]] > v:
-            up0 += 1
-        for r in range(min(2, deg)):
+        minimum = deg > 0 and ci[rp[v]] > v
+        up0 += minimum
+        m1 += minimum and deg > 1
+        for r in range(min(2 if (minimum or not skip) else 1, deg)):
             a, b = find(v), find(int(ci[rp[v] + r]))
             if a != b:
                 parent[max(a, b)] = min(a, b)
@@ -47,12 +50,14 @@ def _brute_stats(G):
     vals, counts = np.unique(roots, return_counts=True)
     giant = vals[np.argmax(counts)]
     fv = fe = 0
+    past = 1 if skip else 2
     for v in range(n):
         deg = rp[v + 1] - rp[v]
-        if roots[v] != giant and deg > 2:
+        if roots[v] != giant and deg > past:
             fv += 1
-            fe += deg - 2
-    return dict(deg_gt0=d0, deg_gt1=d1, up0=up0, frontier_vertices=fv, frontier_edges=fe)
+            fe += deg - past
+    return dict(deg_gt0=d0, deg_gt1=d1, up0=up0, minima_gt1=m1, frontier_vertices=fv,
+                frontier_edges=fe)
 
 
 def _two_part_graph(seed):
@@ -82,13 +87,14 @@ def _graph(n, row_ptr, col_idx):
 
 def test_frontier_stats_match_a_union_find_restatement():
     for seed in (3, 4):
-        G = _two_part_graph(seed)
-        got = bench.frontier_stats(G, two_pass=False)
-        want = _brute_stats(G)
-        for k, v in want.items():
-            assert got[k] == v, (seed, k, got[k], v)
-        assert want["frontier_vertices"] >= 9  # the clique block is outside the giant
-        assert 0.5 < got["giant_fraction"] < 1.0
+        for skip in (False, True):
+            G = _two_part_graph(seed)
+            got = bench.frontier_stats(G, two_pass=False, skip=skip)
+            want = _brute_stats(G, skip)
+            for k, v in want.items():
+                assert got[k] == v, (seed, skip, k, got[k], v)
+            assert want["frontier_vertices"] >= 9  # the clique block is outside the giant
+            assert 0.5 < got["giant_fraction"] < 1.0
 
 
 def test_bench_refuses_more_than_one_gpu():

@@ -101,20 +101,22 @@ def test_chung_lu(gpu):
     check_labels(G, dropin(G))
 
 
-@pytest.mark.parametrize("kind,rem,splice,halve", [
-    ("er", 1, 1, 0), ("grid", 1, 1, 0), ("rmat24", 0, 0, 1)])
-def test_default_plan_at_scale(gpu, kind, rem, splice, halve):
+@pytest.mark.parametrize("kind,rem,splice,halve,skip", [
+    ("er", 1, 1, 0, 0), ("er8", 1, 1, 0, 2), ("grid", 1, 1, 0, 0), ("rmat24", 0, 0, 1, 0)])
+def test_default_plan_at_scale(gpu, kind, rem, splice, halve, skip):
     """Graphs of 2^24 vertices: sparse ones take Rem's unions with splicing,
-    dense ones the lockstep hook with path halving, by default; labels,
+    dense ones the lockstep hook with path halving, by default, and sparse
+    ones of degree 6 and more skip neighbour 1 of linked vertices; labels,
     component count and size histogram equal the oracle's."""
     from cc_mpi_amd import devgen
     lib = nat.gpu_lib()
     D = (devgen.erdos_renyi(1 << 24, 3.0, seed=41) if kind == "er"
+         else devgen.erdos_renyi(1 << 24, 8.0, seed=44) if kind == "er8"
          else devgen.grid(4096, 4096, 0.7, seed=42) if kind == "grid"
          else devgen.rmat(24, 16, seed=43))
     plan = {k: lib.cc_hip_components_plan_get(D.n, D.m, k.encode())
-            for k in ("fused", "rem", "rem_splice", "halve")}
-    assert plan == dict(fused=1, rem=rem, rem_splice=splice, halve=halve)
+            for k in ("fused", "rem", "rem_splice", "halve", "skip_linked")}
+    assert plan == dict(fused=1, rem=rem, rem_splice=splice, halve=halve, skip_linked=skip)
     got = D.solve()
     G = D.to_host()
     D.close()
@@ -247,10 +249,26 @@ PATHS = [  # every code path of the solve: (knobs) -> same labels
     dict(csr_hook=1, rem=1, rem_splice=0), dict(csr_hook=1, rem=1, rem_splice=1),
     dict(csr_hook=1, rem=1, rem_splice=2), dict(csr_hook=0, rem=0, halve=1, hook_ilp=1),
     dict(csr_hook=1, rem=0, halve=1, hook_ilp=1, blocks_per_cu=1),
+    # linked vertices skip neighbour 1 (1), or also take neighbour 0's parent
+    # (2); local minima unite with minima_entries entries; off (0) where the
+    # plan would skip
+    dict(csr_hook=1, rem=0, halve=1, hook_ilp=1, skip_linked=1),
+    dict(csr_hook=1, rem=0, halve=1, hook_ilp=1, skip_linked=2),
+    dict(csr_hook=1, rem=0, halve=0, hook_ilp=2, skip_linked=2),
+    dict(csr_hook=1, rem=0, halve=1, hook_ilp=4, skip_linked=2, blocks_per_cu=1),
+    dict(csr_hook=1, rem=0, halve=1, hook_ilp=1, skip_linked=2, minima_entries=3),
+    dict(csr_hook=1, rem=0, halve=1, hook_ilp=2, skip_linked=1, minima_entries=8),
+    dict(rem=1, rem_splice=0, skip_linked=1), dict(rem=1, rem_splice=1, skip_linked=2),
+    dict(rem=1, rem_splice=2, skip_linked=2, nt=4), dict(rem=1, rem_splice=1, skip_linked=1, nt=4),
+    dict(rem=1, rem_splice=0, skip_linked=2, minima_entries=4),
+    dict(rem=1, rem_splice=1, skip_linked=1, minima_entries=8, blocks_per_cu=1),
+    dict(rem=0, halve=1, hook_ilp=1, skip_linked=2), dict(rem=0, halve=1, hook_ilp=2, skip_linked=1),
+    dict(rem=0, halve=0, hook_ilp=4, skip_linked=2), dict(rem=0, halve=1, hook_ilp=2, skip_linked=2, nt=5),
+    dict(skip_linked=2, stream_ilp=2), dict(skip_linked=0),
 ]
 DEFAULTS = dict(sample_cache=1, fused_pair=-1, hook_ilp=0, hook_mode=4, sampled_neighbours=2,
                 stream_ilp=1, blocks_per_cu=0, giant_early=1, halve=-1, rem=-1, rem_splice=-1,
-                nt=-1, init_ilp=0, frontier_chunks=0, csr_hook=-1)
+                nt=-1, init_ilp=0, frontier_chunks=0, csr_hook=-1, skip_linked=-1, minima_entries=0)
 
 
 @pytest.mark.parametrize("knobs", PATHS, ids=lambda k: ",".join(f"{a}={b}" for a, b in k.items()))
