@@ -698,10 +698,11 @@ __device__ __forceinline__ bool link_first(uint32_t v, uint32_t w0, uint32_t n,
 }
 
 //
-// Skip1: a vertex whose own-entry link to neighbour 0 succeeds skips
-// neighbour 1; a local minimum (neighbour 0 above it) or a vertex whose
-// link failed unites with both, and a local minimum also with entries
-// 2 .. minima_entries-1 of its row (the reasoning at k_hook_rem).
+// Skip1: a vertex whose own-entry link to neighbour 0 succeeds (Skip1 == 2:
+// a link under neighbour 0's parent, one hop) skips neighbour 1; a local
+// minimum (neighbour 0 above it) or a vertex whose link failed unites with
+// both, and a local minimum also with entries 2 .. minima_entries-1 of its
+// row (the reasoning at k_hook_rem).
 template <int Mode, int ILP, bool Halve, int Skip1 = 0>
 __global__ __launch_bounds__(kBlock) void k_hook_pair_csr(const uint64_t *__restrict__ row_ptr,
                                                           const uint32_t *__restrict__ col,
@@ -717,6 +718,10 @@ __global__ __launch_bounds__(kBlock) void k_hook_pair_csr(const uint64_t *__rest
       rb[k] = v < n ? row_ptr[v] : 0;
       re[k] = v < n ? row_ptr[v + 1] : 0;
       w0[k] = rb[k] < re[k] ? col[rb[k]] : kNoNeighbour;
+      // read with neighbour 0 even where skipping will not use it: a read
+      // after the link's CAS, for the vertices that do, measured slower
+      // (Chung-Lu 65.6M hook 2.11 -> 2.16 ms), and reading it only for the
+      // local minima costs the frontier sweep more than it saves
       w1[k] = rb[k] + 1 < re[k] ? col[rb[k] + 1] : kNoNeighbour;
     }
     uint32_t x[3 * ILP];
@@ -734,9 +739,7 @@ __global__ __launch_bounds__(kBlock) void k_hook_pair_csr(const uint64_t *__rest
         act[ILP + k] = live && link_first((uint32_t)v, w0[k], n, p);
       }
       act[2 * ILP + k] = live && w1[k] < n && w1[k] != (uint32_t)v && (!Skip1 || act[ILP + k]);
-      // Skip1 == 3: a linked vertex still walks (and, with Halve, halves)
-      // its own path, which keeps the trees flat for the frontier sweep
-      act[k] = act[ILP + k] || act[2 * ILP + k] || (Skip1 == 3 && live && w0[k] < (uint32_t)v);
+      act[k] = act[ILP + k] || act[2 * ILP + k];
       x[k] = (uint32_t)v;
       x[ILP + k] = w0[k];
       x[2 * ILP + k] = w1[k];
@@ -1087,7 +1090,7 @@ extern "C" int cc_hip_components_set(const char *key, int value) {
   else if (k == "init_ilp" && (value == 0 || ilp_ok(value))) g_knobs.init_ilp = value;
   else if (k == "frontier_chunks" && (value == 0 || ilp_ok(value))) g_knobs.frontier_chunks = value;
   else if (k == "csr_hook" && value >= -1 && value <= 1) g_knobs.csr_hook = value;
-  else if (k == "skip_linked" && value >= -1 && value <= 3) g_knobs.skip_linked = value;
+  else if (k == "skip_linked" && value >= -1 && value <= 2) g_knobs.skip_linked = value;
   else if (k == "minima_entries" && (value == 0 || (value >= 2 && value <= 8))) g_knobs.minima_entries = value;
   else return ccx::set_error(4, "cc_hip_components_set: bad key or value (%s=%d)", key, value);
   return 0;
@@ -1269,11 +1272,9 @@ int solve_with(const Knobs &K, const ccx::Context *c, const uint64_t *d_row_ptr,
     else                                                                                           \
       CCX_LAUNCH("cc_hook_sample", s, k_hook_pair_csr<4, 1, H, K1><<<nb, kBlock, 0, s>>>(d_row_ptr, d_col_idx, n, d_labels, me)); \
   } while (0)
-    else if (plan.halve && plan.skip1 == 3) CCX_HOOK_CSR(true, 3);
     else if (plan.halve && plan.skip1 == 2) CCX_HOOK_CSR(true, 2);
     else if (plan.halve && plan.skip1) CCX_HOOK_CSR(true, 1);
     else if (plan.halve) CCX_HOOK_CSR(true, 0);
-    else if (plan.skip1 == 3) CCX_HOOK_CSR(false, 3);
     else if (plan.skip1 == 2) CCX_HOOK_CSR(false, 2);
     else if (plan.skip1) CCX_HOOK_CSR(false, 1);
     else CCX_HOOK_CSR(false, 0);
