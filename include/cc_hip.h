@@ -139,7 +139,8 @@ int cc_hip_components_tune(int sampled_neighbours, int blocks_per_cu, int hook_m
  *   "blocks_per_cu"           sampled-hook grid cap; 0 (default) = 8 for
  *                             Rem's unions unless they splice on every step
  *                             (rem_splice 1), and when the lockstep hook
- *                             walks several roots per thread, else 4
+ *                             walks several roots per thread; 16 for the
+ *                             skipping hook from the CSR; else 4
  *   "hook_ilp"                roots walked per thread in lockstep in the
  *                             sampled hook: 1, 2, 4, or 0 (default) = 1 when
  *                             m < 16 n, else 2 in the one-pass hook (1 from
@@ -147,7 +148,9 @@ int cc_hip_components_tune(int sampled_neighbours, int blocks_per_cu, int hook_m
  *   "frontier_blocks_per_cu"  grid cap of the frontier sweep (default 8)
  *   "stream_blocks_per_cu"    grid cap of init/compress (default 8)
  *   "stream_ilp"              vertices per thread per step in init/compress
- *                             without the cache (1 default, 2, 4)
+ *                             without the cache (1 default, 2, 4); with 1 the
+ *                             pointer-jumping passes take 4 after a skipping
+ *                             hook
  *   "nt"                      non-temporal streaming, a bitmask: 1 the init
  *                             pass stores the sample arrays non-temporally,
  *                             2 it loads the column ids non-temporally, 4 the
@@ -161,6 +164,24 @@ int cc_hip_components_tune(int sampled_neighbours, int blocks_per_cu, int hook_m
  *   "frontier_chunks"         64-vertex chunks per wave step of the frontier
  *                             sweep, their root walks in lockstep: 1, 2, 4, or
  *                             0 (default) = 2
+ *   "csr_hook"                the one-pass hook reads each row's first two
+ *                             entries itself after an identity init, with no
+ *                             sample arrays: 1, 0, or -1 (default) = on for
+ *                             the lockstep hook with one root per thread
+ *                             (n >= 2^24, m >= 16 n, m >= 2^31)
+ *   "skip_linked"             a vertex linked under its neighbour 0 by its
+ *                             own entry skips the union with neighbour 1, so
+ *                             only the local minima (neighbour 0 above them)
+ *                             walk, and the frontier sweep starts at entry 1:
+ *                             0 off, 1 on, 2 on with the link going to
+ *                             neighbour 0's parent (one hop); -1 (default) =
+ *                             2 for the lockstep hook from the CSR and for
+ *                             Rem's unions on sparse graphs with m >= 6 n
+ *                             (m < 16 n), else 0; applies to the one-pass
+ *                             hooks from the sample arrays and to the
+ *                             lockstep hook from the CSR
+ *   "minima_entries"          entries of its row a local minimum unites with
+ *                             when skipping: 2..8, or 0 (default) = 2
  * Returns nonzero for an unknown key or out-of-range value; _get returns -1
  * for an unknown key. */
 int cc_hip_components_set(const char *key, int value);
@@ -173,8 +194,9 @@ int cc_hip_components_plan(uint32_t n, uint64_t m, int *cache, int *fused, int *
                            int *hook_blocks_per_cu);
 /* One field of that plan by name: "cache", "fused", "rem", "rem_splice",
  * "halve", "hook_ilp", "hook_blocks_per_cu", "giant_early" (in effect for
- * this graph), "hook_mode", "nt", "init_ilp", "frontier_chunks".  Returns -1
- * for an unknown key. */
+ * this graph), "hook_mode", "nt", "init_ilp", "frontier_chunks", "csr_hook",
+ * "skip_linked", "minima_entries", "compress_ilp" (vertices per thread of
+ * the pointer-jumping passes).  Returns -1 for an unknown key. */
 int cc_hip_components_plan_get(uint32_t n, uint64_t m, const char *key);
 /* Whether the last solve on `stream` (NULL = the library stream) ran its
  * final pointer-jumping pass: *ran = 1 if its frontier sweep linked a root,
