@@ -238,16 +238,27 @@ __global__ __launch_bounds__(kBlock) void k_hook_neighbour(const uint64_t *__res
   }
 }
 
-// Pointer jumping: p[v] = root(v).  Run with no hooks in flight.  With a
-// gate, the pass is skipped when the gate word is 0: the frontier sweep
-// sets it only if it linked a root, and otherwise has already written every
-// vertex's final root.
+// Pointer jumping: p[v] = root(v).  Run with no hooks in flight.  With the
+// solve's slots (slots[0] the sampled giant root, slots[1] the frontier
+// sweep's link flag) it is the last pass: skipped when the flag is 0 (the
+// sweep linked no root and has already written every vertex's final
+// root), and otherwise an entry equal to the giant root needs no walk when
+// that root is still a root (the sweep's write-back pointed nearly every
+// vertex at it), which leaves most vertices a single streamed load.
+__device__ __forceinline__ uint32_t known_root(const uint32_t *__restrict__ slots,
+                                               const uint32_t *__restrict__ p) {
+  if (!slots) return 0xFFFFFFFFu;
+  const uint32_t g = slots[0];
+  return p[g] == g ? g : 0xFFFFFFFFu;
+}
+
 __global__ __launch_bounds__(kBlock) void k_compress(uint32_t n, uint32_t *__restrict__ p,
-                                                     const uint32_t *__restrict__ gate) {
-  if (gate && *gate == 0) return;
+                                                     const uint32_t *__restrict__ slots) {
+  if (slots && slots[1] == 0) return;
+  const uint32_t g = known_root(slots, p);
   CCX_GRID_STRIDE(v, n) {
     const uint32_t pv = p[v];
-    if (pv == v) continue;
+    if (pv == v || pv == g) continue;
     const uint32_t r = find_root(pv, p);
     if (r != pv) p[v] = r;
   }
@@ -396,8 +407,9 @@ __global__ __launch_bounds__(kBlock) void k_hook_neighbour_ilp(const uint64_t *_
 
 template <int ILP>
 __global__ __launch_bounds__(kBlock) void k_compress_ilp(uint32_t n, uint32_t *__restrict__ p,
-                                                         const uint32_t *__restrict__ gate) {
-  if (gate && *gate == 0) return;
+                                                         const uint32_t *__restrict__ slots) {
+  if (slots && slots[1] == 0) return;
+  const uint32_t g = known_root(slots, p);
   const uint64_t S = (uint64_t)gridDim.x * kBlock;
   for (uint64_t v0 = (uint64_t)blockIdx.x * kBlock + threadIdx.x; v0 < n; v0 += S * ILP) {
     uint32_t x[ILP], first[ILP];
@@ -408,7 +420,7 @@ __global__ __launch_bounds__(kBlock) void k_compress_ilp(uint32_t n, uint32_t *_
       act[k] = v < n;
       x[k] = act[k] ? p[v] : 0;
       first[k] = x[k];
-      act[k] = act[k] && x[k] != (uint32_t)v;
+      act[k] = act[k] && x[k] != (uint32_t)v && x[k] != g;
     }
     roots_lockstep<ILP>(x, act, p);
 #pragma unroll
@@ -1240,8 +1252,8 @@ int solve_with(const Knobs &K, const ccx::Context *c, const uint64_t *d_row_ptr,
     CCX_LAUNCH("cc_init", s, k_init<<<ns, kBlock, 0, s>>>(n, d_labels));
     return 0;
   }
-  // gate: nullptr for a pass between hook rounds, the frontier's link flag
-  // for the last pass
+  // slots: nullptr for a pass between hook rounds, the giant root and the
+  // frontier's link flag for the last pass
   auto compress = [&](const uint32_t *gate) -> int {
     const int cilp = plan.compress_ilp;
     if (cilp == 4) CCX_LAUNCH("cc_compress", s, k_compress_ilp<4><<<ns, kBlock, 0, s>>>(n, d_labels, gate));
@@ -1416,7 +1428,7 @@ int solve_with(const Knobs &K, const ccx::Context *c, const uint64_t *d_row_ptr,
   else if (mode == 4) CCX_FRONTIER(4);
   else CCX_FRONTIER(3);
 #undef CCX_FRONTIER
-  return compress(giant + 1);
+  return compress(giant);
 }
 
 }  // namespace
