@@ -35,7 +35,10 @@
 //      remaining neighbours (indices 2..deg-1, or 1.. after skipping).
 //      Edges between two vertices already under g are skipped; an edge
 //      between a g-vertex and another vertex is still seen from the other
-//      end because the graph is symmetric.  Each wave64 takes 64
+//      end because the graph is symmetric (with the knob symmetric = 0, for
+//      a CSR holding one-directional entries, every vertex hooks with its
+//      remaining neighbours, which gives the weakly connected components
+//      the reference's push-pull propagation computes on such input).  Each wave64 takes 64
 //      consecutive vertices, compacts the active ones with a wave prefix
 //      sum of their remaining degrees and sweeps the concatenated neighbour
 //      lists 64 entries at a time, so the col_idx reads of a wave are
@@ -105,6 +108,11 @@ struct Knobs {
   // entries of its row a local minimum (neighbour 0 above it) unites with
   // when skipping is on (2..8; 0 = auto)
   int minima_entries = 0;
+  // 1: the CSR stores both directions of every edge, so the frontier sweep
+  // may skip the giant tree's members; 0: it unites every vertex's entries,
+  // which gives the reference's labels (weakly connected components) on a
+  // CSR that stores some edges in one direction only
+  int symmetric = 1;
 };
 // Process-wide settings (cc_hip_components_set / _tune); the entry point
 // without allocations always runs with the defaults.
@@ -876,7 +884,8 @@ __global__ __launch_bounds__(kBlock) void k_hook_frontier(const uint64_t *__rest
                                                           const uint32_t *__restrict__ col,
                                                           uint32_t n, uint32_t k_first,
                                                           uint32_t *__restrict__ p,
-                                                          uint32_t *__restrict__ slots) {
+                                                          uint32_t *__restrict__ slots,
+                                                          uint32_t every_vertex = 0) {
   __shared__ uint64_t s_excl[kBlock / kWave][kWave];
   __shared__ uint64_t s_begin[kBlock / kWave][kWave];
   const int lane = threadIdx.x & (kWave - 1);
@@ -917,7 +926,7 @@ __global__ __launch_bounds__(kBlock) void k_hook_frontier(const uint64_t *__rest
       uint64_t work = 0, begin = 0;
       if (v < n) {
         if (cur[j] != pv[j]) p[v] = cur[j];
-        if (!hit[j]) {
+        if (!hit[j] || every_vertex) {
           const uint64_t b = row_ptr[v] + k_first;
           const uint64_t e = row_ptr[v + 1];
           if (e > b) {
@@ -1104,6 +1113,7 @@ extern "C" int cc_hip_components_set(const char *key, int value) {
   else if (k == "csr_hook" && value >= -1 && value <= 1) g_knobs.csr_hook = value;
   else if (k == "skip_linked" && value >= -1 && value <= 2) g_knobs.skip_linked = value;
   else if (k == "minima_entries" && (value == 0 || (value >= 2 && value <= 8))) g_knobs.minima_entries = value;
+  else if (k == "symmetric" && (value == 0 || value == 1)) g_knobs.symmetric = value;
   else return ccx::set_error(4, "cc_hip_components_set: bad key or value (%s=%d)", key, value);
   return 0;
 }
@@ -1129,6 +1139,7 @@ extern "C" int cc_hip_components_get(const char *key) {
   if (k == "csr_hook") return g_knobs.csr_hook;
   if (k == "skip_linked") return g_knobs.skip_linked;
   if (k == "minima_entries") return g_knobs.minima_entries;
+  if (k == "symmetric") return g_knobs.symmetric;
   return -1;
 }
 
@@ -1412,15 +1423,20 @@ int solve_with(const Knobs &K, const ccx::Context *c, const uint64_t *d_row_ptr,
   // for sure, so the sweep covers every neighbour.
   // With skipping, neighbour 1 of a linked vertex was not united, so the
   // sweep starts at entry 1 (entry 0 is always united).
-  const uint32_t first = (mode == 3) ? 0u : (plan.skip1 ? 1u : rounds);
+  // One-directional entries: the giant's members must unite theirs too,
+  // from the first entry that not every vertex united (the early giant's
+  // members skipped rounds 1.. of the two-pass plan).
+  const uint32_t every = K.symmetric ? 0u : 1u;
+  const uint32_t first =
+      (mode == 3) ? 0u : ((plan.skip1 || (every && early_giant)) ? 1u : rounds);
 #define CCX_FRONTIER(M)                                                                            \
   do {                                                                                             \
     if (plan.frontier_chunks == 4)                                                                 \
-      CCX_LAUNCH("cc_hook_frontier", s, k_hook_frontier<M, 4><<<nf, kBlock, 0, s>>>(d_row_ptr, d_col_idx, n, first, d_labels, giant)); \
+      CCX_LAUNCH("cc_hook_frontier", s, k_hook_frontier<M, 4><<<nf, kBlock, 0, s>>>(d_row_ptr, d_col_idx, n, first, d_labels, giant, every)); \
     else if (plan.frontier_chunks == 2)                                                            \
-      CCX_LAUNCH("cc_hook_frontier", s, k_hook_frontier<M, 2><<<nf, kBlock, 0, s>>>(d_row_ptr, d_col_idx, n, first, d_labels, giant)); \
+      CCX_LAUNCH("cc_hook_frontier", s, k_hook_frontier<M, 2><<<nf, kBlock, 0, s>>>(d_row_ptr, d_col_idx, n, first, d_labels, giant, every)); \
     else                                                                                           \
-      CCX_LAUNCH("cc_hook_frontier", s, k_hook_frontier<M, 1><<<nf, kBlock, 0, s>>>(d_row_ptr, d_col_idx, n, first, d_labels, giant)); \
+      CCX_LAUNCH("cc_hook_frontier", s, k_hook_frontier<M, 1><<<nf, kBlock, 0, s>>>(d_row_ptr, d_col_idx, n, first, d_labels, giant, every)); \
   } while (0)
   if (mode == 0) CCX_FRONTIER(0);
   else if (mode == 1) CCX_FRONTIER(1);

@@ -182,6 +182,12 @@ int cc_hip_components_tune(int sampled_neighbours, int blocks_per_cu, int hook_m
  *                             lockstep hook from the CSR
  *   "minima_entries"          entries of its row a local minimum unites with
  *                             when skipping: 2..8, or 0 (default) = 2
+ *   "symmetric"               1 (default): the CSR stores both directions of
+ *                             every edge, and the frontier sweep skips the
+ *                             giant tree's members; 0: every vertex unites
+ *                             all its entries, giving weakly connected
+ *                             components of a CSR with one-directional
+ *                             entries (a union per entry)
  * Returns nonzero for an unknown key or out-of-range value; _get returns -1
  * for an unknown key. */
 int cc_hip_components_set(const char *key, int value);

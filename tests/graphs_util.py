@@ -61,3 +61,25 @@ def bfs_min_labels(G):
                     lab[w] = s
                     stack.append(w)
     return lab
+
+
+def one_directional(G, keep_both, seed):
+    """G with only one direction kept of each edge, except a fraction
+    `keep_both` stored both ways: a CSR whose weakly connected components
+    are G's components, but whose rows no longer see every edge."""
+    rng = np.random.default_rng(seed)
+    deg = np.diff(G.row_ptr).astype(np.int64)
+    src = np.repeat(np.arange(G.n, dtype=np.int64), deg)
+    dst = G.col_idx.astype(np.int64)
+    lo, hi = np.minimum(src, dst), np.maximum(src, dst)
+    # one draw per undirected edge, shared by its two entries
+    h = (lo * 0x9E3779B1 + hi * 0x85EBCA77 + seed) & 0xFFFFFFFF
+    both = (h % 1000) < keep_both * 1000
+    forward = ((h >> 10) & 1) == 1
+    keep = both | (forward == (src < dst))
+    rows = [[] for _ in range(G.n)]
+    for s, d in zip(src[keep].tolist(), dst[keep].tolist()):
+        rows[s].append(d)
+    for r in rows:
+        rng.shuffle(r)
+    return csr_raw(G.n, rows)

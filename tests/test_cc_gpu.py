@@ -11,7 +11,7 @@ import pytest
 import cc_mpi_amd as cm
 from cc_mpi_amd import _native as nat
 import oracle
-from graphs_util import complete, csr_raw, disjoint_cliques, path, star
+from graphs_util import complete, csr_raw, disjoint_cliques, one_directional, path, star
 
 pytestmark = pytest.mark.gpu
 
@@ -292,6 +292,40 @@ def test_every_solve_path(gpu, knobs):
         for k, v in DEFAULTS.items():
             nat.check(lib.cc_hip_components_set(k.encode(), v), k)
 
+
+
+@pytest.mark.parametrize("knobs", [
+    dict(), dict(csr_hook=1, rem=0, halve=1, hook_ilp=1, skip_linked=2),
+    dict(csr_hook=1, rem=0, halve=1, hook_ilp=2, skip_linked=0),
+    dict(rem=1, rem_splice=1, skip_linked=2), dict(rem=1, rem_splice=2, skip_linked=0),
+    dict(rem=0, halve=1, hook_ilp=2, skip_linked=2),
+    dict(sample_cache=1, fused_pair=0, hook_ilp=2, hook_mode=1, giant_early=1),
+    dict(sample_cache=1, fused_pair=0, hook_ilp=1, hook_mode=4, giant_early=1, sampled_neighbours=3),
+    dict(sample_cache=0, fused_pair=0, hook_ilp=1, hook_mode=3)],
+    ids=lambda k: ",".join(f"{a}={b}" for a, b in k.items()) or "default")
+def test_one_directional_entries_with_symmetric_off(gpu, knobs):
+    """With the knob symmetric = 0 the solve unites every stored entry, so a
+    CSR holding some edges in one direction only gets its weakly connected
+    components (the reference's propagation has no schedule-independent
+    answer there; tests/test_oracle.py::test_reference_needs_both_directions)."""
+    lib = nat.gpu_lib()
+    graphs = [one_directional(cm.rmat(13, 16, seed=51), 0.2, 1),
+              one_directional(cm.grid(90, 110, 0.7, seed=52), 0.1, 2),
+              one_directional(cm.erdos_renyi(30000, 3.0, seed=53), 0.3, 3),
+              one_directional(cm.chung_lu(1 << 15, 1 << 18, 2.1, seed=54), 0.0, 4)]
+    try:
+        for k, v in dict(knobs, symmetric=0).items():
+            nat.check(lib.cc_hip_components_set(k.encode(), v), k)
+        for G in graphs:
+            D = cm.DeviceGraph(G)
+            got = D.solve()
+            D.close()
+            want = oracle.cc_min_label(G.row_ptr, G.col_idx)
+            bad = np.flatnonzero(got != want)
+            assert bad.size == 0, f"{bad.size} labels differ, first {bad[:5]}"
+    finally:
+        for k, v in dict(DEFAULTS, symmetric=1).items():
+            nat.check(lib.cc_hip_components_set(k.encode(), v), k)
 
 
 def test_drop_ins_from_four_host_threads(gpu):

@@ -82,6 +82,35 @@ def test_lp_restatement_matches_reference(threads, chunk):
         assert np.array_equal(mine, uf)
 
 
+def test_reference_needs_both_directions():
+    """The reference's propagation (src/cc_pthreads.c:32-73) pushes a label
+    to the neighbours only when the vertex itself just lowered it, so on a
+    CSR that stores some edges in one direction only its fixed point depends
+    on the schedule: thread count and chunk size change the labels, and none
+    of them is the weakly connected components.  With both directions stored
+    every schedule gives the per-component minimum.  This is why the drop-in
+    takes symmetric input as the contract (the knob symmetric = 0 gives the
+    weak components instead, which no reference schedule does)."""
+    from graphs_util import one_directional
+    if oracle.ref_lib() is None:
+        pytest.skip("reference build absent")
+    schedules = ((1, 4096), (4, 1), (8, 64), (8, 4096), (3, 7))
+    varied = 0
+    for seed, G in enumerate((cm.rmat(12, 8, seed=21), cm.grid(48, 48, 0.7, seed=22),
+                              cm.erdos_renyi(20000, 2.5, seed=23))):
+        uf = oracle.cc_min_label(G.row_ptr, G.col_idx)
+        for t, c in schedules:
+            assert np.array_equal(oracle.ref_cc_pthreads(G.row_ptr, G.col_idx, t, c), uf)
+        D = one_directional(G, 0.2, seed)
+        assert D.m < G.m
+        weak = oracle.cc_min_label(D.row_ptr, D.col_idx)
+        assert np.array_equal(weak, uf)  # dropping a direction keeps weak connectivity
+        outs = [oracle.ref_cc_pthreads(D.row_ptr, D.col_idx, t, c) for t, c in schedules]
+        assert not any(np.array_equal(o, weak) for o in outs)
+        varied += any(not np.array_equal(o, outs[0]) for o in outs[1:])
+    assert varied >= 1
+
+
 def test_lp_restatement_without_reference():
     G = cm.rmat(12, 16, seed=2)
     for t, c in ((1, 4096), (5, 1), (6, 7)):
