@@ -733,6 +733,18 @@ def main() -> int:
                    "h2d_seconds": round(h2d_s, 3),
                    "h2d_inclusive_gteps": round(G.m / (h2d_s + ms / 1e3) / 1e9, 4)},
         "roofline": roofline,
+        # Not the roofline: one full propagation sweep of the reference's
+        # algorithm (row offsets, every column id, a label gather per entry,
+        # own label read and written) priced in algorithmic bytes, against
+        # the time of the whole solve.  A value above 1 means the solve ends
+        # before one such sweep could at the HBM peak; the reference needs a
+        # sweep per round.
+        "full_sweep_equivalent": {
+            "bytes": 8.0 * (G.n + 1) + 8.0 * G.m + 8.0 * G.n,
+            "gbs": round((8.0 * (G.n + 1) + 8.0 * G.m + 8.0 * G.n) / (ms * 1e6), 1),
+            "frac_of_peak": round((8.0 * (G.n + 1) + 8.0 * G.m + 8.0 * G.n) / (ms * 1e6)
+                                  / HBM_PEAK_GBS, 4),
+        },
         "cpu_baseline": cpu,
         "kernels": kernel_table,
         "frontier": st,
