@@ -923,6 +923,243 @@ struct SplitRows {
   }
 };
 
+// The same split by partitioning instead of probing (boundary_split 2).
+// The probing split above reads one random bitmap word per dominant record
+// (638M reads of a few-MB bitmap), which the L2s serve at a fixed request
+// rate.  Here the dominant records' remotes are first scattered into
+// buckets of 2^shift consecutive remotes, so that one workgroup per bucket
+// sets that bucket's bits in LDS and stores the window's words once:
+//   count:   per tile of kPartTile records, how many dominant records fall
+//            in each bucket and how many others there are (LDS counters);
+//   (the counts, stored tile-major, are transposed to bucket-major; two
+//    exclusive scans give every tile its place in each bucket region and
+//    in the others' list; the bucket offsets are transposed back);
+//   scatter: the same tiles again; the tile's dominant remotes are grouped
+//            by bucket in LDS and each bucket's run is stored in one piece
+//            in its region; each other record, packed as in SplitRows,
+//            goes to the others' list;
+//   bitmap:  one workgroup per bucket.
+// Order inside a tile is not kept (positions come from LDS atomics): the
+// bitmap is a set and the others are sorted afterwards, so the result is
+// the same bytes as the probing split's.
+constexpr uint64_t kPartTile = 16384;
+constexpr int kPartUnroll = 8;
+constexpr uint32_t kPartMaxBuckets = 1024;
+constexpr int kPartMaxShift = 17;  // 2^17 remotes = 4096 words = 16 KiB of LDS per bucket
+constexpr int kPartBitmapThreads = 1024;
+
+__global__ __launch_bounds__(kThreads) void k_part_count(
+    const uint64_t *__restrict__ rec, uint64_t count, const uint32_t *__restrict__ packed,
+    uint32_t n, const uint32_t *__restrict__ g_rank, int shift, uint32_t nb, uint64_t tiles,
+    uint32_t *__restrict__ cnt_dom, uint32_t *__restrict__ cnt_other) {
+  __shared__ uint32_t c[kPartMaxBuckets];
+  __shared__ uint32_t c_other;
+  const uint32_t g = *g_rank;
+  const int lane = threadIdx.x & (kWave - 1);
+  for (uint64_t tile = blockIdx.x; tile < tiles; tile += gridDim.x) {
+    for (uint32_t b = threadIdx.x; b < nb; b += kThreads) c[b] = 0;
+    if (threadIdx.x == 0) c_other = 0;
+    __syncthreads();
+    const uint64_t base = tile * kPartTile;
+    for (uint64_t r = 0; r < kPartTile; r += (uint64_t)kThreads * kPartUnroll) {
+      uint64_t v[kPartUnroll];
+      uint32_t rank[kPartUnroll];
+#pragma unroll
+      for (int u = 0; u < kPartUnroll; ++u) {
+        const uint64_t j = base + r + (uint64_t)u * kThreads + threadIdx.x;
+        v[u] = j < count ? __builtin_nontemporal_load(&rec[j]) : 0;
+      }
+#pragma unroll
+      for (int u = 0; u < kPartUnroll; ++u) {
+        const uint64_t j = base + r + (uint64_t)u * kThreads + threadIdx.x;
+        const uint32_t li = (uint32_t)v[u];
+        rank[u] = (j < count && li < n) ? packed[li] : 0u;
+      }
+#pragma unroll
+      for (int u = 0; u < kPartUnroll; ++u) {
+        const uint64_t j = base + r + (uint64_t)u * kThreads + threadIdx.x;
+        const bool dom = j < count && rank[u] == g;
+        if (dom) atomicAdd(&c[(uint32_t)(v[u] >> 32) >> shift], 1u);
+        const uint64_t other = __ballot(j < count && !dom);
+        if (lane == 0 && other) atomicAdd(&c_other, (uint32_t)__popcll(other));
+      }
+    }
+    __syncthreads();
+    for (uint32_t b = threadIdx.x; b < nb; b += kThreads) cnt_dom[tile * nb + b] = c[b];
+    if (threadIdx.x == 0) cnt_other[tile] = c_other;
+    __syncthreads();
+  }
+}
+
+// Exclusive prefix sums over a 512-thread workgroup (x from every thread);
+// returns this thread's prefix and sets *total.  Uses `wsum` (8 words of
+// LDS); ends with a barrier.
+constexpr int kPartScatterThreads = 512;
+__device__ __forceinline__ uint32_t block_excl_scan_512(uint32_t x, uint32_t *wsum,
+                                                       uint32_t *total) {
+  const int lane = threadIdx.x & (kWave - 1);
+  const int w = threadIdx.x / kWave;
+  uint32_t incl = x;
+#pragma unroll
+  for (int d = 1; d < kWave; d <<= 1) {
+    const uint32_t up = __shfl_up(incl, d, kWave);
+    if (lane >= d) incl += up;
+  }
+  if (lane == kWave - 1) wsum[w] = incl;
+  __syncthreads();
+  uint32_t before = 0, all = 0;
+#pragma unroll
+  for (int k = 0; k < kPartScatterThreads / kWave; ++k) {
+    const uint32_t t = wsum[k];
+    before += (k < w) ? t : 0u;
+    all += t;
+  }
+  *total = all;
+  __syncthreads();
+  return before + incl - x;
+}
+
+// Scatter by tiles of kPartTile records: the tile's dominant remotes are
+// first grouped by bucket in LDS (bucket b's run starts at the prefix sum
+// of the tile's counts, which the count pass stored), then each run is
+// written to the bucket's region in one contiguous piece, so the stores
+// coalesce; the others go straight to their list.
+__global__ __launch_bounds__(kPartScatterThreads) void k_part_scatter(
+    const uint64_t *__restrict__ rec, uint64_t count, const uint32_t *__restrict__ packed,
+    uint32_t n, const uint32_t *__restrict__ g_rank, int shift, uint32_t nb, uint64_t tiles,
+    int bits, const uint32_t *__restrict__ cnt_dom, const uint64_t *__restrict__ off_tile,
+    const uint64_t *__restrict__ off_other, uint32_t *__restrict__ remotes,
+    uint64_t *__restrict__ others) {
+  __shared__ uint32_t stage[kPartTile];
+  __shared__ uint64_t base_b[kPartMaxBuckets];
+  __shared__ uint32_t loc[kPartMaxBuckets];
+  __shared__ uint32_t cur[kPartMaxBuckets];
+  __shared__ uint32_t wsum[kPartScatterThreads / kWave];
+  __shared__ uint32_t c_other;
+  constexpr uint32_t kPer = kPartMaxBuckets / kPartScatterThreads;  // buckets per thread in the scan
+  const uint32_t g = *g_rank;
+  const int lane = threadIdx.x & (kWave - 1);
+  const uint64_t below = (lane == 0) ? 0 : (~0ull >> (kWave - lane));  // lanes under this one
+  for (uint64_t tile = blockIdx.x; tile < tiles; tile += gridDim.x) {
+    // this tile's runs: bucket b at loc[b] in stage
+    uint32_t mine[kPer];
+    uint32_t part = 0;
+#pragma unroll
+    for (uint32_t k = 0; k < kPer; ++k) {
+      const uint32_t b = threadIdx.x * kPer + k;
+      mine[k] = b < nb ? cnt_dom[tile * nb + b] : 0u;
+      part += mine[k];
+    }
+    uint32_t tile_dom = 0;
+    uint32_t at = block_excl_scan_512(part, wsum, &tile_dom);
+#pragma unroll
+    for (uint32_t k = 0; k < kPer; ++k) {
+      const uint32_t b = threadIdx.x * kPer + k;
+      if (b < nb) {
+        loc[b] = at;
+        cur[b] = at;
+        base_b[b] = off_tile[tile * nb + b];
+      }
+      at += mine[k];
+    }
+    if (threadIdx.x == 0) c_other = 0;
+    __syncthreads();
+    const uint64_t base_o = off_other[tile];
+    const uint64_t base = tile * kPartTile;
+    for (uint64_t r = 0; r < kPartTile; r += (uint64_t)kPartScatterThreads * kPartUnroll) {
+      uint64_t v[kPartUnroll];
+      uint32_t rank[kPartUnroll];
+#pragma unroll
+      for (int u = 0; u < kPartUnroll; ++u) {
+        const uint64_t j = base + r + (uint64_t)u * kPartScatterThreads + threadIdx.x;
+        v[u] = j < count ? __builtin_nontemporal_load(&rec[j]) : 0;
+      }
+#pragma unroll
+      for (int u = 0; u < kPartUnroll; ++u) {
+        const uint64_t j = base + r + (uint64_t)u * kPartScatterThreads + threadIdx.x;
+        const uint32_t li = (uint32_t)v[u];
+        rank[u] = (j < count && li < n) ? packed[li] : 0u;
+      }
+#pragma unroll
+      for (int u = 0; u < kPartUnroll; ++u) {
+        const uint64_t j = base + r + (uint64_t)u * kPartScatterThreads + threadIdx.x;
+        const bool dom = j < count && rank[u] == g;
+        const uint32_t remote = (uint32_t)(v[u] >> 32);
+        if (dom) stage[atomicAdd(&cur[remote >> shift], 1u)] = remote;
+        const bool keep = j < count && !dom;
+        const uint64_t other = __ballot(keep);
+        uint32_t first = 0;
+        if (lane == 0 && other) first = atomicAdd(&c_other, (uint32_t)__popcll(other));
+        first = __shfl(first, 0, kWave);
+        if (keep)
+          others[base_o + first + (uint32_t)__popcll(other & below)] =
+              ((uint64_t)remote << bits) | rank[u];
+      }
+    }
+    __syncthreads();
+    for (uint32_t i = threadIdx.x; i < tile_dom; i += kPartScatterThreads) {
+      const uint32_t x = stage[i];
+      const uint32_t b = x >> shift;
+      remotes[base_b[b] + (i - loc[b])] = x;
+    }
+    __syncthreads();
+  }
+}
+
+// out[c * rows + r] = in[r * cols + c] for an rows x cols matrix, through
+// 32 x 32 LDS tiles (the count pass stores each tile's bucket counts
+// contiguously; the scan needs them bucket-major, and the scatter pass the
+// resulting offsets tile-major again).
+template <typename T>
+__global__ __launch_bounds__(kThreads) void k_transpose(const T *__restrict__ in, uint64_t rows,
+                                                         uint64_t cols, T *__restrict__ out) {
+  __shared__ T t[32][33];
+  const uint64_t tiles_c = (cols + 31) / 32;
+  const uint64_t tiles_r = (rows + 31) / 32;
+  const int tx = threadIdx.x & 31, ty = threadIdx.x / 32;  // 32 x 8 threads
+  for (uint64_t blk = blockIdx.x; blk < tiles_r * tiles_c; blk += gridDim.x) {
+    const uint64_t r0 = (blk / tiles_c) * 32, c0 = (blk % tiles_c) * 32;
+    for (int k = ty; k < 32; k += kThreads / 32)
+      if (r0 + k < rows && c0 + tx < cols) t[k][tx] = in[(r0 + k) * cols + c0 + tx];
+    __syncthreads();
+    for (int k = ty; k < 32; k += kThreads / 32)
+      if (c0 + k < cols && r0 + tx < rows) out[(c0 + k) * rows + r0 + tx] = t[tx][k];
+    __syncthreads();
+  }
+}
+
+// One workgroup per bucket: the bucket's remotes set bits of a window of
+// `ww` words in LDS, which is then stored over the bitmap's words
+// [b * ww, (b + 1) * ww) (clipped to `words`); every bitmap word belongs to
+// exactly one bucket, so nothing else writes it.
+__global__ __launch_bounds__(kPartBitmapThreads) void k_part_bitmap(
+    const uint32_t *__restrict__ remotes, const uint64_t *__restrict__ off_dom, uint64_t tiles,
+    uint32_t ww, uint64_t words, uint32_t *__restrict__ bitmap) {
+  __shared__ uint32_t win[1u << (kPartMaxShift - 5)];
+  const uint64_t b = blockIdx.x;
+  for (uint32_t i = threadIdx.x; i < ww; i += kPartBitmapThreads) win[i] = 0;
+  __syncthreads();
+  const uint64_t lo = off_dom[b * tiles];
+  const uint64_t hi = off_dom[(b + 1) * tiles];
+  const uint32_t mask = ww - 1;
+  uint64_t k = lo + threadIdx.x;
+  for (; k + 3 * kPartBitmapThreads < hi; k += 4 * kPartBitmapThreads) {
+    uint32_t r[4];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) r[u] = __builtin_nontemporal_load(&remotes[k + u * kPartBitmapThreads]);
+#pragma unroll
+    for (int u = 0; u < 4; ++u) atomicOr(&win[(r[u] >> 5) & mask], 1u << (r[u] & 31));
+  }
+  for (; k < hi; k += kPartBitmapThreads) {
+    const uint32_t r = remotes[k];
+    atomicOr(&win[(r >> 5) & mask], 1u << (r & 31));
+  }
+  __syncthreads();
+  const uint64_t w0 = b * ww;
+  for (uint32_t i = threadIdx.x; i < ww && w0 + i < words; i += kPartBitmapThreads)
+    bitmap[w0 + i] = win[i];
+}
+
 // Index of rep r in the ascending rep list (rep_count if absent).
 __device__ __forceinline__ uint32_t rep_index(const uint32_t *reps, uint32_t rep_count, uint32_t r) {
   uint32_t lo = 0, hi = rep_count;
@@ -1201,7 +1438,11 @@ namespace {
 // 1.65 ms for 256 x 32 and 1.89 ms for 1024 x 8.
 int g_sort_items = 16;     // keys per thread (8, 16, 24, 32)
 int g_sort_threads = 512;  // threads per workgroup (256; 512 with 8 or 16 items; 1024 with 8)
-int g_boundary_split = 1;  // boundary build: dominant rep through a bitmap (0: sort every record)
+// boundary build: dominant rep through a bitmap, set by partitioning the
+// remotes into LDS windows (2; at the 4-rank share of the 65.6M-vertex
+// graph the split takes 5.3 ms against 7.5 ms) or by probing (1); 0 sorts
+// every record
+int g_boundary_split = 2;
 int g_split_dominant_rows = 32;  // rows of 64 records per wave step in the bitmap split (16, 32)
 // workgroups for `threads` threads, at most per_cu per CU
 unsigned grid_cap(uint64_t threads, int per_cu);
@@ -1230,7 +1471,7 @@ extern "C" int cc_hip_sort_set(const char *key, int value) {
     ccx::g_compact_rows = value;
     return 0;
   }
-  if (key && strcmp(key, "boundary_split") == 0 && (value == 0 || value == 1)) {
+  if (key && strcmp(key, "boundary_split") == 0 && (value >= 0 && value <= 2)) {
     g_boundary_split = value;
     return 0;
   }
@@ -1555,13 +1796,86 @@ int build_split_dominant(uint64_t *d_records, uint64_t *d_alt, uint64_t count,
   uint32_t *wcnt = (uint32_t *)(bm + map_bytes);
   uint64_t *woff = (uint64_t *)(bm + 2 * map_bytes);
   int rc = 0;
+  // partitioned split: buckets of 2^shift remotes, at most kPartMaxBuckets
+  // of them (else the probing split runs)
+  const uint64_t max_remote = st->h_stage[1] >> 32;
+  int rbits = 1;
+  while (rbits < 64 && (max_remote >> rbits) != 0) ++rbits;
+  int shift = rbits - 8;
+  if (shift < 5) shift = 5;
+  if (shift > kPartMaxShift) shift = kPartMaxShift;
+  const uint64_t nb = (words * 32 + (1ull << shift) - 1) >> shift;
+  const bool part = g_boundary_split == 2 && nb <= kPartMaxBuckets;
+  const uint64_t tiles = (count + kPartTile - 1) / kPartTile;
+  char *pt = nullptr;
+  uint32_t *cnt_dom = nullptr, *cnt_bucket = nullptr, *cnt_other = nullptr;
+  uint64_t *off_dom = nullptr, *off_tile = nullptr, *off_other = nullptr;
+  if (part) {
+    // counts tile-major and bucket-major (u32), offsets bucket-major (with
+    // the total) and tile-major (u64), the others' counts and offsets
+    const size_t a = (nb * tiles * 4 + 7) & ~(size_t)7;
+    const size_t b2 = (tiles * 4 + 7) & ~(size_t)7;
+    const size_t pt_bytes = 2 * a + b2 + (nb * tiles + 1) * 8 + nb * tiles * 8 + (tiles + 1) * 8;
+    if (hipMalloc(&pt, pt_bytes) != hipSuccess) {
+      (void)hipFree(bm);
+      return ccx::set_error(3, "cc_hip_boundary_build_device: hipMalloc of %zu bytes failed", pt_bytes);
+    }
+    cnt_dom = (uint32_t *)pt;
+    cnt_bucket = (uint32_t *)(pt + a);
+    cnt_other = (uint32_t *)(pt + 2 * a);
+    off_dom = (uint64_t *)(pt + 2 * a + b2);
+    off_tile = off_dom + nb * tiles + 1;
+    off_other = off_tile + nb * tiles;
+  }
   do {
-    if (hipMemsetAsync(bitmap, 0, words * 4, s) != hipSuccess) {
+    uint64_t n_other = 0;
+    if (part) {
+      const uint64_t gcap = grid_cap(tiles * kThreads, 8);
+      const unsigned gt = (unsigned)(tiles < gcap ? tiles : gcap);
+      CCX_LAUNCH_RC(rc, "split_part_count", s,
+                    k_part_count<<<gt, kThreads, 0, s>>>(d_records, count, packed, n_local, d_g, shift,
+                                                         (uint32_t)nb, tiles, cnt_dom, cnt_other));
+      if (rc) break;
+      const unsigned gtr = (unsigned)grid_cap(((tiles + 31) / 32) * ((nb + 31) / 32) * kThreads, 8);
+      CCX_LAUNCH_RC(rc, "split_part_transpose", s,
+                    k_transpose<uint32_t><<<gtr, kThreads, 0, s>>>(cnt_dom, tiles, nb, cnt_bucket));
+      if (rc) break;
+      if ((rc = cc_hip_exclusive_scan_u32_device(cnt_bucket, nb * tiles, off_dom, s))) break;
+      CCX_LAUNCH_RC(rc, "split_part_transpose", s,
+                    k_transpose<uint64_t><<<gtr, kThreads, 0, s>>>(off_dom, nb, tiles, off_tile));
+      if (rc) break;
+      if ((rc = cc_hip_exclusive_scan_u32_device(cnt_other, tiles, off_other, s))) break;
+      uint64_t *h = st->h_stage;
+      if (hipMemcpyAsync(&h[0], off_dom + nb * tiles, 8, hipMemcpyDeviceToHost, s) != hipSuccess ||
+          hipMemcpyAsync(&h[1], off_other + tiles, 8, hipMemcpyDeviceToHost, s) != hipSuccess ||
+          hipStreamSynchronize(s) != hipSuccess) {
+        rc = ccx::set_error(1, "cc_hip_boundary_build_device: reading the split counts failed");
+        break;
+      }
+      const uint64_t n_dom = h[0];
+      n_other = h[1];
+      if (n_dom + n_other != count) {
+        rc = ccx::set_error(1, "cc_hip_boundary_build_device: split counted %llu + %llu of %llu records",
+                            (unsigned long long)n_dom, (unsigned long long)n_other,
+                            (unsigned long long)count);
+        break;
+      }
+      // the others fill d_alt[0, n_other); the dominant remotes (4 bytes
+      // each) go right after them, which fits: 8 n_other + 4 n_dom <= 8 count
+      uint32_t *remotes = (uint32_t *)(d_alt + n_other);
+      CCX_LAUNCH_RC(rc, "split_part_scatter", s,
+                    k_part_scatter<<<gt, kPartScatterThreads, 0, s>>>(
+                        d_records, count, packed, n_local, d_g, shift, (uint32_t)nb, tiles, bits,
+                        cnt_dom, off_tile, off_other, remotes, d_alt));
+      if (rc) break;
+      CCX_LAUNCH_RC(rc, "split_part_bitmap", s,
+                    k_part_bitmap<<<(unsigned)nb, kPartBitmapThreads, 0, s>>>(
+                        remotes, off_dom, tiles, (uint32_t)(1u << (shift - 5)), words, bitmap));
+      if (rc) break;
+    } else if (hipMemsetAsync(bitmap, 0, words * 4, s) != hipSuccess) {
       rc = ccx::set_error(1, "cc_hip_boundary_build_device: bitmap clear failed");
       break;
-    }
-    uint64_t n_other = 0;
-    if (g_split_dominant_rows == 16)
+    } else if (g_split_dominant_rows == 16)
       rc = run_compact_rows("split_dominant",
                             SplitRows<16>{d_records, packed, n_local, bits, d_g, bitmap, d_alt},
                             count, s, st, &n_other);
@@ -1625,6 +1939,7 @@ int build_split_dominant(uint64_t *d_records, uint64_t *d_alt, uint64_t count,
   } while (0);
   (void)hipStreamSynchronize(s);
   (void)hipFree(bm);
+  if (pt) (void)hipFree(pt);
   return rc;
 }
 }  // namespace

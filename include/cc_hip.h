@@ -238,8 +238,10 @@ int cc_hip_sort_u64_device(uint64_t *d_keys, uint64_t *d_alt, uint64_t count,
  * records of the most frequent rep (sampled) only set bits of a bitmap over
  * the remotes, and the other records alone are sorted, then merged with the
  * set bits read in order ("boundary_split" of cc_hip_sort_set).  Temporary
- * device memory: 12 bytes per local vertex and about 2 bytes per possible
- * remote id.  *out_count receives the kept count; the records are in d_alt when
+ * device memory: 12 bytes per local vertex, about 2 bytes per possible
+ * remote id, and for the partitioned bitmap 24 bytes per (bucket of 2^17
+ * remote ids, tile of 16,384 records) pair (0.75 bytes per record at 512
+ * buckets).  *out_count receives the kept count; the records are in d_alt when
  * *result_in_alt is 1, else in d_records; the other buffer is clobbered.
  * Blocking. */
 int cc_hip_boundary_build_device(uint64_t *d_records, uint64_t *d_alt, uint64_t count,
@@ -256,11 +258,15 @@ int cc_hip_boundary_build_device(uint64_t *d_records, uint64_t *d_alt, uint64_t 
  * where the next two knobs apply); "dedup_staged" (0 default: each thread
  * stores its own items; 1: outputs staged in LDS and stored as contiguous
  * runs); "dedup_items" (16 or 32 (default) records per thread); of the
- * boundary build: "boundary_split" (1 default: the records of the most
- * frequent rep go through a bitmap over the remotes and only the rest are
- * sorted, then the two sorted lists are merged; 0: every record sorted)
- * and "split_dominant_rows" (16 or 32 (default) rows of 64 records per
- * wave step of that bitmap pass); and of the split: "split_staged" (as dedup_staged; staging measured
+ * boundary build: "boundary_split" (the records of the most frequent rep
+ * go through a bitmap over the remotes and only the rest are sorted, then
+ * the two sorted lists are merged; 2 default: the bitmap is built by
+ * partitioning those records' remotes into buckets of 2^17 ids, each set in
+ * LDS by one workgroup (remote ids needing more than 1024 buckets take
+ * mode 1); 1: each record probes and sets its bit in the bitmap in memory;
+ * 0: every record sorted) and "split_dominant_rows" (16 or 32 (default)
+ * rows of 64 records per wave step of the probing pass); and of the split:
+ * "split_staged" (as dedup_staged; staging measured
  * slower at 854M entries because the LDS costs occupancy).  The result
  * never depends on them.  _set returns nonzero for a bad key or value, _get
  * -1 for an unknown key. */

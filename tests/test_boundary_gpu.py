@@ -307,13 +307,15 @@ def test_delta_unpack_equals_oracle(gpu, seed, nsrc):
     d_ghost.close()
 
 
-@pytest.mark.parametrize("split,rows,items", [(1, 1, 32), (0, 1, 32), (1, 0, 16), (0, 0, 32)])
+@pytest.mark.parametrize("split,rows,items", [(1, 1, 32), (0, 1, 32), (1, 0, 16), (0, 0, 32),
+                                               (2, 1, 32), (2, 0, 16)])
 @pytest.mark.parametrize("roots", [1, 2, 3, 1000, 65537, "all"])
 def test_boundary_build_matches_three_steps(gpu, roots, split, rows, items):
     """cc_hip_boundary_build_device (reps packed by their rank among the
     local roots; with split, the most frequent rep's records through a
-    bitmap over the remotes, merged back) gives the bytes of rep fill ->
-    sort -> dedup."""
+    bitmap over the remotes, set by probing (1) or by partitioning the
+    remotes into LDS windows (2), merged back) gives the bytes of rep fill
+    -> sort -> dedup."""
     import ctypes
     lib = nat.gpu_lib()
     rng = np.random.default_rng(11 if roots == "all" else roots)
@@ -343,13 +345,15 @@ def test_boundary_build_matches_three_steps(gpu, roots, split, rows, items):
         b.close()
 
 
-@pytest.mark.parametrize("split,split_rows", [(1, 32), (1, 16), (0, 32)])
+@pytest.mark.parametrize("split,split_rows", [(1, 32), (1, 16), (0, 32), (2, 32)])
 @pytest.mark.parametrize("count,remote_bits", [(1, 26), (7, 26), (1000, 32), (500_000, 32),
                                                (2_000_003, 26)])
 def test_boundary_build_giant_component(gpu, split, split_rows, count, remote_bits):
     """The shape of a real rank's boundary: one local component holds most
     vertices (so most records carry its root), the rest are small; remotes
-    up to 2^32 - 1.  Same bytes as rep fill -> sort -> dedup."""
+    up to 2^32 - 1 (with split 2 these need more buckets than the
+    partitioned split takes, so it falls back to probing).  Same bytes as
+    rep fill -> sort -> dedup."""
     import ctypes
     lib = nat.gpu_lib()
     rng = np.random.default_rng(count + remote_bits)
