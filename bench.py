@@ -193,12 +193,14 @@ def kernel_bytes(name: str, n: int, st: dict, plan: dict) -> float | None:
         return 8.0 * (n + 1) + 4.0 * n + 2.0 * (d0 + d1) + 2.0 * (up0 + d1)
     if name == "cc_compress":
         # The last pass follows the frontier sweep, which already pointed
-        # every vertex it resolved at its root: it reads every entry and,
-        # for each non-root, the parent's entry to see that it is a root
-        # (`components` = roots of the final forest); nothing need be
-        # written.  On the two-pass path a full pass (read, gather, write
-        # back: 12n) runs between the rounds as well; averaged.
-        last = 4.0 * n + 4.0 * (n - st["components"])
+        # every vertex it resolved at its root: it reads every entry, and an
+        # entry at the giant root (every vertex of the giant tree) needs no
+        # more; a vertex outside the giant tree reads its parent's entry to
+        # see that it is a root.  Nothing need be written.  On the two-pass
+        # path a full pass (read, gather, write back: 12n) runs between the
+        # rounds as well; averaged.
+        outside = n * (1.0 - st["giant_fraction"])
+        last = 4.0 * n + 4.0 * outside
         return last if plan["fused"] else (12.0 * n + last) / 2.0
     if name == "cc_hook_frontier":
         # every vertex resolves its root (own entry + one gather, written back);
