@@ -44,8 +44,10 @@ WORKLOADS = {
     "rmat22": dict(kind="rmat", scale=22, ef=16, seed=1002),
     # Erdos-Renyi 65,536 vertices, average degree 8.
     "er64k": dict(kind="er", n=65536, deg=8.0, seed=1001),
-    # 8192 x 8192 lattice with 30% of edges deleted.
+    # 8192 x 8192 lattice with 30% of edges deleted, ids permuted.
     "grid8k": dict(kind="grid", rows=8192, cols=8192, keep=0.7, seed=1004),
+    # The same lattice numbered row by row (id = row * 8192 + column).
+    "grid8k_rows": dict(kind="grid", rows=8192, cols=8192, keep=0.7, seed=1004, permute=False),
     # Chung-Lu power law (gamma 2.1), 65,601,024 vertices; 1,921,400,000
     # sampled pairs leave 1.80e9 undirected edges (3.60e9 CSR entries) once
     # self-loops and duplicates are dropped.
@@ -66,9 +68,10 @@ def make_device_graph(name: str):
         return devgen.erdos_renyi(w["n"], w["deg"], seed=w["seed"]), (
             f"Erdos-Renyi n={w['n']} avg degree {w['deg']:g}, seed {w['seed']}")
     if w["kind"] == "grid":
-        return devgen.grid(w["rows"], w["cols"], w["keep"], seed=w["seed"]), (
-            f"{w['rows']}x{w['cols']} grid, {1 - w['keep']:.0%} edges deleted, permuted, "
-            f"seed {w['seed']}")
+        permute = w.get("permute", True)
+        return devgen.grid(w["rows"], w["cols"], w["keep"], seed=w["seed"], permute=permute), (
+            f"{w['rows']}x{w['cols']} grid, {1 - w['keep']:.0%} edges deleted, "
+            f"{'permuted' if permute else 'numbered row by row'}, seed {w['seed']}")
     return devgen.chung_lu(w["n"], w["pairs"], w["gamma"], seed=w["seed"]), (
         f"Chung-Lu n={w['n']} pairs={w['pairs']} gamma {w['gamma']}, permuted, seed {w['seed']}")
 
@@ -132,6 +135,38 @@ def frontier_stats(G, two_pass: bool = True, skip: bool = False):
                 minima_gt1=minima_gt1, round1_outside_early_giant=live1)
 
 
+def tile_stats(G, shift: int, labels: np.ndarray) -> dict:
+    """What the vertex-tile plan handles on G with tiles of 2^shift ids:
+    pairs between tiles (entries to a smaller tile; the CSR is symmetric),
+    local roots (the components of the entries inside each tile, computed
+    here with scipy) and how many of them the unions between tiles link,
+    and the vertices whose local root is not their final root."""
+    import scipy.sparse as sp
+    from scipy.sparse.csgraph import connected_components
+    n = G.n
+    rp = G.row_ptr.astype(np.int64)
+    src = np.repeat(np.arange(n, dtype=np.int64), np.diff(rp))
+    dst = G.col_idx.astype(np.int64)
+    ok = dst < n
+    st, dt = src >> shift, dst >> shift
+    cross = int(np.count_nonzero(ok & (dt < st)))
+    inside = ok & (dt == st) & (dst != src)
+    A = sp.coo_matrix((np.ones(int(inside.sum()), dtype=np.int8), (src[inside], dst[inside])),
+                      shape=(n, n)).tocsr()
+    del src, dst, st, dt, ok, inside
+    k, comp = connected_components(A, directed=False)
+    low = np.full(k, n, dtype=np.int64)
+    np.minimum.at(low, comp, np.arange(n, dtype=np.int64))
+    local_root = low[comp]
+    is_root = local_root == np.arange(n)
+    final = labels.astype(np.int64)
+    roots = int(np.count_nonzero(is_root))
+    return dict(tile_shift=shift, tiles=(n + (1 << shift) - 1) >> shift, cross_pairs=cross,
+                local_roots=roots,
+                linked_local_roots=roots - int(np.count_nonzero(final == np.arange(n))),
+                moved=int(np.count_nonzero(~is_root & (final != local_root))))
+
+
 def kernel_bytes(name: str, n: int, st: dict, plan: dict) -> float | None:
     """Algorithmic HBM bytes of one launch of kernel `name` (model stated in
     the design notes): each array element the algorithm must touch, counted
@@ -139,6 +174,28 @@ def kernel_bytes(name: str, n: int, st: dict, plan: dict) -> float | None:
     (cc_hip_components_plan), with two sampled rounds.  d0/d1 = vertices
     with at least 1/2 neighbours; up0 = vertices whose neighbour 0 is above
     them (the only ones round 0 hooks; the others were linked by init)."""
+    if plan.get("tiles_ran"):
+        # the vertex-tile plan; the sampled path's kernels returned at once
+        if not name.startswith("cc_tile_") or name == "cc_tile_probe":
+            return None
+        m, cross, nt = st["m"], st["cross_pairs"], st["tiles"]
+        if name == "cc_tile_local":
+            # row offsets and every column id once, every parent written,
+            # the root bitmap, the pairs between tiles and the tile counts
+            return 8.0 * (n + 1) + 4.0 * m + 4.0 * n + n / 8.0 + 8.0 * cross + 4.0 * nt
+        if name == "cc_tile_cross":
+            # tile counts, the pairs, both endpoints' parents
+            return 4.0 * nt + 8.0 * cross + 8.0 * cross
+        if name == "cc_tile_roots":
+            # the bitmap, then each linked local root's entry read and written
+            return n / 8.0 + 8.0 * st["linked_local_roots"]
+        if name == "cc_tile_final":
+            # every entry, the local root's entry for every other vertex, the
+            # entries that move
+            return 4.0 * n + 4.0 * (n - st["local_roots"]) + 4.0 * st["moved"]
+        return None
+    if name.startswith("cc_tile_"):
+        return None  # the tile plan's kernels returned at once
     d0, d1, up0 = st["deg_gt0"], st["deg_gt1"], st["up0"]
     if name == "cc_init":
         return 4.0 * n
@@ -214,7 +271,7 @@ def solve_plan(lib, n: int, m: int) -> dict:
     """What the solve does on this graph (cc_hip_components_plan)."""
     keys = ("cache", "fused", "rem", "rem_splice", "halve", "hook_ilp", "hook_blocks_per_cu",
             "giant_early", "hook_mode", "nt", "init_ilp", "frontier_chunks", "csr_hook",
-            "skip_linked", "minima_entries", "compress_ilp")
+            "skip_linked", "minima_entries", "compress_ilp", "tiles", "tile_shift", "tile_pairs")
     return {k: int(lib.cc_hip_components_plan_get(n, m, k.encode())) for k in keys}
 
 
@@ -641,14 +698,21 @@ def main() -> int:
     ran = ctypes.c_int(0)
     nat.check(lib.cc_hip_components_last_pass_ran(None, ctypes.byref(ran)), "last pass")
     last_pass_ran = bool(ran.value)
+    nat.check(lib.cc_hip_components_tiles_ran(None, ctypes.byref(ran)), "tiles ran")
+    tiles_ran = bool(ran.value)
     D.close()
 
     plan = solve_plan(lib, G.n, G.m)
+    plan["tiles_ran"] = int(tiles_ran)
     log(f"solve plan: {plan}")
     t0 = time.perf_counter()
-    st = frontier_stats(G, two_pass=not plan["fused"], skip=bool(plan["skip_linked"]))
+    if tiles_ran:
+        st = tile_stats(G, plan["tile_shift"], labels)
+        st["m"] = G.m
+    else:
+        st = frontier_stats(G, two_pass=not plan["fused"], skip=bool(plan["skip_linked"]))
     st["components"] = int(np.count_nonzero(labels == np.arange(G.n, dtype=np.uint32)))
-    log(f"frontier: {st} ({time.perf_counter() - t0:.1f}s)")
+    log(f"{'tiles' if tiles_ran else 'frontier'}: {st} ({time.perf_counter() - t0:.1f}s)")
     roofline = None
     if dominant:
         b = kernel_bytes(dominant, G.n, st, plan)
@@ -670,7 +734,11 @@ def main() -> int:
     for k, v in per.items():
         row = {"launches_per_solve": v["launches"] / args.steps, "avg_us": round(v["avg_us"], 2)}
         kb = kernel_bytes(k, G.n, st, plan)
-        if k == "cc_compress" and plan["fused"] and not last_pass_ran:
+        if plan["tiles"] == 2 and (k.startswith("cc_tile_") != tiles_ran) and k != "cc_tile_probe":
+            # both plans are launched and the probe's switch word sends the
+            # kernels of the one not taken back at once
+            row["switched_off"] = True
+        elif k == "cc_compress" and plan["fused"] and not last_pass_ran:
             # the frontier sweep linked nothing, so the last pass only read
             # its gate word and returned
             row["gated_off"] = True
@@ -691,11 +759,13 @@ def main() -> int:
         # lattice from the same generator (same keep fraction and seed) and
         # the full graph's labels are checked against the oracle.
         import oracle
-        sample = cm.grid(side, side, w["keep"], seed=w["seed"])
+        sample = cm.grid(side, side, w["keep"], seed=w["seed"], permute=w.get("permute", True))
         log(f"timing the CPU baseline on a {side}x{side} sample lattice ...")
         _, cpu = cpu_baseline(sample, args.cpu_budget, args.cpu_runs, args.cpu_impl)
         cpu["sample"] = (f"{side}x{side} lattice from the same generator (keep {w['keep']}, "
-                         f"seed {w['seed']}), n={sample.n} m={sample.m}: " + cpu["sample"])
+                         f"seed {w['seed']}, "
+                         f"{'permuted' if w.get('permute', True) else 'numbered row by row'}), "
+                         f"n={sample.n} m={sample.m}: " + cpu["sample"])
         cpu.pop("seconds")
         parity = bool(np.array_equal(oracle_labels(G, w["kind"]), labels))
     elif not args.no_cpu_baseline:
@@ -749,7 +819,7 @@ def main() -> int:
         },
         "cpu_baseline": cpu,
         "kernels": kernel_table,
-        "frontier": st,
+        ("tiles" if tiles_ran else "frontier"): st,
         "solve_plan": plan,
         "parity_bit_exact": parity,
         "components": comp,

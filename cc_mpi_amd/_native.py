@@ -94,6 +94,7 @@ def gpu_lib() -> ctypes.CDLL:
     _sig(lib, "cc_hip_components_plan", c_int, [c_u32, c_u64, P_int, P_int, P_int, P_int])
     _sig(lib, "cc_hip_components_plan_get", c_int, [c_u32, c_u64, ctypes.c_char_p])
     _sig(lib, "cc_hip_components_last_pass_ran", c_int, [c_void_p, P_int])
+    _sig(lib, "cc_hip_components_tiles_ran", c_int, [c_void_p, P_int])
     _sig(lib, "compute_connected_components_pthreads", None,
          [ctypes.POINTER(CSRGraphC), P_u32, c_int, c_int])
     _sig(lib, "cc_hip_sort_u64_device", c_int,

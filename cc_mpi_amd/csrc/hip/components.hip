@@ -113,6 +113,13 @@ struct Knobs {
   // which gives the reference's labels (weakly connected components) on a
   // CSR that stores some edges in one direction only
   int symmetric = 1;
+  // vertex tiles, the plan for graphs whose ids are local (k_tile_local):
+  // -1 a probe on the device decides, for sparse graphs of 2^20+ vertices
+  // on the plan that unites by Rem's algorithm from the sample arrays; 0
+  // never; 1 always
+  int tiles = -1;
+  int tile_shift = 0;  // log2 of the vertices per tile (6..15; 0 = auto, 15)
+  int tile_pairs = 0;  // pairs per tile segment of the cross list (0 = auto, one per vertex)
 };
 // Process-wide settings (cc_hip_components_set / _tune); the entry point
 // without allocations always runs with the defaults.
@@ -260,8 +267,13 @@ __device__ __forceinline__ uint32_t known_root(const uint32_t *__restrict__ slot
   return p[g] == g ? g : 0xFFFFFFFFu;
 }
 
+// sw: the plan switch word (see k_tile_probe), if both plans are launched:
+// the pass runs only when it holds sw_run.
 __global__ __launch_bounds__(kBlock) void k_compress(uint32_t n, uint32_t *__restrict__ p,
-                                                     const uint32_t *__restrict__ slots) {
+                                                     const uint32_t *__restrict__ slots,
+                                                     const uint32_t *__restrict__ sw = nullptr,
+                                                     uint32_t sw_run = 0) {
+  if (sw && sw[0] != sw_run) return;
   if (slots && slots[1] == 0) return;
   const uint32_t g = known_root(slots, p);
   CCX_GRID_STRIDE(v, n) {
@@ -415,7 +427,10 @@ __global__ __launch_bounds__(kBlock) void k_hook_neighbour_ilp(const uint64_t *_
 
 template <int ILP>
 __global__ __launch_bounds__(kBlock) void k_compress_ilp(uint32_t n, uint32_t *__restrict__ p,
-                                                         const uint32_t *__restrict__ slots) {
+                                                         const uint32_t *__restrict__ slots,
+                                                         const uint32_t *__restrict__ sw = nullptr,
+                                                         uint32_t sw_run = 0) {
+  if (sw && sw[0] != sw_run) return;
   if (slots && slots[1] == 0) return;
   const uint32_t g = known_root(slots, p);
   const uint64_t S = (uint64_t)gridDim.x * kBlock;
@@ -447,11 +462,15 @@ constexpr uint32_t kNoNeighbour = 0xFFFFFFFFu;
 // Nt: bit 1 stores the sample arrays non-temporally, bit 2 loads the
 // column ids non-temporally (both are streamed once; the parent array
 // written here is what the hook gathers from next).
+// off: the plan switch word (see k_tile_probe); nonzero means the tile
+// plan took the solve, and this pass returns at once.
 template <int Rounds, int Nt = 0>
 __global__ __launch_bounds__(kBlock) void k_init_sample(const uint64_t *__restrict__ row_ptr,
                                                         const uint32_t *__restrict__ col,
                                                         uint32_t n, uint32_t *__restrict__ p,
-                                                        uint32_t *__restrict__ s) {
+                                                        uint32_t *__restrict__ s,
+                                                        const uint32_t *__restrict__ off = nullptr) {
+  if (off && off[0]) return;
   CCX_GRID_STRIDE(v, n) {
     const uint64_t b = row_ptr[v];
     const uint64_t e = row_ptr[v + 1];
@@ -662,7 +681,9 @@ __global__ __launch_bounds__(kBlock) void k_hook_rem(const uint32_t *__restrict_
                                                      uint32_t *__restrict__ p,
                                                      const uint64_t *__restrict__ row_ptr = nullptr,
                                                      const uint32_t *__restrict__ col = nullptr,
-                                                     uint32_t minima_entries = 2) {
+                                                     uint32_t minima_entries = 2,
+                                                     const uint32_t *__restrict__ off = nullptr) {
+  if (off && off[0]) return;
   CCX_GRID_STRIDE(v, n) {
     const uint32_t w0 = Nt ? __builtin_nontemporal_load(&s0[v]) : s0[v];
     const uint32_t w1 = Nt ? __builtin_nontemporal_load(&s1[v]) : s1[v];
@@ -818,12 +839,14 @@ __device__ __forceinline__ uint32_t sample_vertex(uint32_t i, uint32_t n) {
 // root so the choice is a function of the forest alone.  One thread per
 // sample, so the root walks (chains of dependent gathers) all run at once.
 __global__ __launch_bounds__(kGiantSamples) void k_sample_giant(uint32_t n, const uint32_t *__restrict__ p,
-                                                         uint32_t *__restrict__ giant) {
+                                                         uint32_t *__restrict__ giant,
+                                                         const uint32_t *__restrict__ off = nullptr) {
   constexpr uint32_t kSlots = 2 * kGiantSamples;
   constexpr uint32_t kEmpty = 0xFFFFFFFFu;
   __shared__ uint32_t keys[kSlots];
   __shared__ uint32_t cnts[kSlots];
   __shared__ uint64_t best[kGiantSamples / kWave];
+  if (off && off[0]) return;
   const int t = threadIdx.x;
   for (uint32_t i = t; i < kSlots; i += kGiantSamples) {
     keys[i] = kEmpty;
@@ -863,6 +886,7 @@ __global__ __launch_bounds__(kGiantSamples) void k_sample_giant(uint32_t n, cons
       if (best[w] > b) b = best[w];
     giant[0] = ~(uint32_t)b;
     giant[1] = 0;  // the frontier sweep's link flag (see k_hook_frontier)
+    giant[2] = 0;  // the plan switch: the sampled path took this solve
   }
 }
 
@@ -885,9 +909,11 @@ __global__ __launch_bounds__(kBlock) void k_hook_frontier(const uint64_t *__rest
                                                           uint32_t n, uint32_t k_first,
                                                           uint32_t *__restrict__ p,
                                                           uint32_t *__restrict__ slots,
-                                                          uint32_t every_vertex = 0) {
+                                                          uint32_t every_vertex = 0,
+                                                          const uint32_t *__restrict__ off = nullptr) {
   __shared__ uint64_t s_excl[kBlock / kWave][kWave];
   __shared__ uint64_t s_begin[kBlock / kWave][kWave];
+  if (off && off[0]) return;
   const int lane = threadIdx.x & (kWave - 1);
   const int wid = threadIdx.x / kWave;
   const uint32_t giant = slots[0];
@@ -974,6 +1000,361 @@ inline unsigned blocks_for(uint64_t n, int cu_count, int per_cu) {
   return (unsigned)(need < cap ? need : cap);
 }
 
+// ---- Vertex tiles: the plan for graphs whose ids are local ------------------
+//
+// On a graph whose edges mostly join vertices with nearby ids (a lattice or
+// mesh numbered row by row, a road network in a space-filling-curve order),
+// the sampled hook builds long parent chains (a row of consecutive ids links
+// one vertex under the next) and every union walks them: an 8192^2 lattice
+// with 30% of its edges deleted, numbered row by row, took 14.6 ms per solve
+// that way against 5.2 ms with its ids permuted.  Such graphs take this plan:
+//   1. k_tile_local: one workgroup per tile of 2^shift consecutive ids
+//      streams the tile's rows (one contiguous slice of col_idx) and unites
+//      every entry that stays inside the tile in an LDS parent array (a
+//      vertex still a root links straight under its smaller neighbour, the
+//      rest by Rem's unions).
+//      Entries that leave the tile go to the tile's segment of a pair list
+//      (with a symmetric CSR only those to smaller ids: the other direction
+//      is seen from the other end).  Then each vertex's entry is set to its
+//      local root (the smallest id of its component inside the tile) and the
+//      local roots are marked in a bitmap.
+//   2. k_tile_cross: one workgroup per tile unites each pair's two trees
+//      (rem_unite<1> on the parent array), once per distinct pair of parents: the
+//      first thread to enter a pair in an LDS table hooks it and the others
+//      skip it (on a lattice nearly every pair between two tiles joins the
+//      same two local roots).  A tile whose pairs did not fit its segment
+//      reads its rows again instead.
+//   3. k_tile_roots: each local root that was linked takes its root, with
+//      path halving by atomicMin, so no store can raise an entry and every
+//      local root ends holding exactly its root.
+//   4. k_tile_final: every other vertex takes its local root's entry (one
+//      hop): after step 1 only local roots are ever parents or change.
+// Every link goes from a larger root to a smaller one, in LDS and in HBM, so
+// each root is its component's minimum, as on the sampled path.
+// When the plan is chosen automatically, k_tile_probe decides on the device
+// from the rows of 1024 sampled vertices: tiles when at most a quarter of
+// their entries leave the vertex's tile.  Its answer is a switch word that
+// every kernel of both plans reads; both plans are launched and the kernels
+// of the one not taken return at once, so the solve never waits on the host.
+
+constexpr int kTileBlock = 1024;
+constexpr uint32_t kTileMaxShift = 15;  // 32,768 vertices: 128 KiB of LDS parents
+constexpr uint32_t kTileMinShift = 6;   // a tile covers whole 64-bit words of the root bitmap
+// Tiles up to 2^14 vertices take a 64 KiB LDS parent array (two workgroups
+// per CU).  The default is 2^15 all the same: on the 8192^2 lattice
+// numbered row by row, 2^14-vertex tiles (two rows) fall apart into many
+// more local components than 2^15-vertex ones (four rows), and the unions
+// between tiles cost more than the occupancy gains (2.9 against 1.55 ms per
+// solve).
+constexpr uint32_t kTileSmallShift = 14;
+constexpr uint32_t kTileDefaultShift = 15;
+constexpr uint32_t kTileMaxPairs = 1u << 24;
+constexpr int kTileProbeSamples = 1024;
+constexpr int kTileProbeEntries = 8;  // entries read per sampled row
+constexpr uint32_t kTileSeenBits = 12;  // pairs entered per tile in the cross pass
+constexpr uint32_t kTileSeen = 1u << kTileSeenBits;
+constexpr unsigned long long kTileNoPair = ~0ull;  // parents are below 2^32 - 1
+
+__device__ __forceinline__ uint32_t tile_ld(const uint32_t *a) {
+  return __hip_atomic_load(a, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+}
+__device__ __forceinline__ void tile_st(uint32_t *a, uint32_t v) {
+  __hip_atomic_store(a, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+}
+
+// Root of x among the LDS parents, halving the path on the way.  Only roots
+// are ever linked (by CAS), so a halving store, which gives a non-root its
+// grandparent, cannot undo a link.
+__device__ __forceinline__ uint32_t tile_find(uint32_t *par, uint32_t x) {
+  for (;;) {
+    const uint32_t px = tile_ld(&par[x]);
+    if (px == x) return x;
+    const uint32_t ppx = tile_ld(&par[px]);
+    if (ppx == px) return px;
+    tile_st(&par[x], ppx);
+    x = ppx;
+  }
+}
+
+// Union in LDS by Rem's algorithm with splicing (as rem_unite<1> in HBM):
+// the two paths are climbed together, the one with the larger parent
+// stepping, whose entry is first lowered (atomicMin) to the other path's
+// parent, which both unites and shortens; a root is linked by a CAS that
+// holds only while it is still a root.  The walk stops where the paths
+// meet.
+__device__ __forceinline__ void tile_unite(uint32_t *par, uint32_t x, uint32_t y) {
+  uint32_t px = tile_ld(&par[x]);
+  uint32_t py = tile_ld(&par[y]);
+  while (px != py) {
+    if (px < py) {
+      uint32_t t = x;
+      x = y;
+      y = t;
+      t = px;
+      px = py;
+      py = t;
+    }
+    if (x == px) {  // x is a root, py < x
+      const uint32_t old = atomicCAS(&par[x], x, py);
+      if (old == x) return;
+      px = old;
+      continue;
+    }
+    atomicMin(&par[x], py);
+    x = px;
+    px = tile_ld(&par[x]);
+  }
+}
+
+// f(vl, w) for every entry w of the tile's rows (vl = the row's local
+// index), one row per lane: the lanes of a wave take 64 consecutive rows,
+// each thread kTileRows rows per step (vl, vl + kTileBlock, ...), and all of
+// the step's row offsets, then the first kTileRowEntries entries of each of
+// its rows, are loaded before f runs on any of them; a row's end offset is
+// the next lane's start offset.  Longer rows continue in a loop, so f runs
+// on diverged lanes (its ballots count the lanes that run it).
+constexpr int kTileRows = 4;
+constexpr int kTileRowEntries = 4;
+template <class F>
+__device__ __forceinline__ void tile_rows_lanes(const uint64_t *__restrict__ row_ptr,
+                                                const uint32_t *__restrict__ col, uint32_t t0,
+                                                uint32_t tn, F &&f) {
+  const int lane = threadIdx.x & (kWave - 1);
+  const uint32_t wbase = (threadIdx.x / kWave) * kWave;
+  for (uint32_t base = wbase; base < tn; base += kTileRows * kTileBlock) {  // wave-uniform
+    uint64_t b[kTileRows], e[kTileRows];
+#pragma unroll
+    for (int r = 0; r < kTileRows; ++r) {
+      const uint32_t i = base + r * kTileBlock + lane;
+      b[r] = row_ptr[t0 + (i < tn ? i : tn)];
+      e[r] = lane == kWave - 1 ? row_ptr[t0 + (i + 1 < tn ? i + 1 : tn)] : 0;
+    }
+#pragma unroll
+    for (int r = 0; r < kTileRows; ++r) {
+      const uint64_t nx = __shfl_down(b[r], 1, kWave);
+      if (lane != kWave - 1) e[r] = nx;
+    }
+    uint32_t w[kTileRows][kTileRowEntries];
+#pragma unroll
+    for (int r = 0; r < kTileRows; ++r)
+#pragma unroll
+      for (int k = 0; k < kTileRowEntries; ++k)
+        w[r][k] = b[r] + k < e[r] ? col[b[r] + k] : 0xFFFFFFFFu;
+#pragma unroll
+    for (int r = 0; r < kTileRows; ++r) {
+      const uint32_t vl = base + r * kTileBlock + lane;
+#pragma unroll
+      for (int k = 0; k < kTileRowEntries; ++k) f(b[r] + k < e[r], vl, w[r][k]);
+      for (uint64_t j = b[r] + kTileRowEntries; j < e[r]; ++j) f(true, vl, col[j]);
+    }
+  }
+}
+
+// Step 1.  Sym: the CSR is symmetric (Knobs::symmetric).  LdsShift: the LDS
+// parent array's size (shift <= LdsShift).  counts[tile] is the number of
+// pairs the tile produced, also those past its segment's cap.
+template <int Sym, uint32_t LdsShift>
+__global__ __launch_bounds__(kTileBlock) void k_tile_local(
+    const uint64_t *__restrict__ row_ptr, const uint32_t *__restrict__ col, uint32_t n,
+    uint32_t shift, uint32_t *__restrict__ p, uint32_t *__restrict__ bits,
+    uint32_t *__restrict__ counts, uint2 *__restrict__ pairs, uint32_t cap,
+    const uint32_t *__restrict__ sw) {
+  __shared__ uint32_t par[1u << LdsShift];
+  __shared__ uint32_t s_count;
+  if (sw[0] != 1) return;
+  const uint32_t t0 = blockIdx.x << shift;
+  const uint32_t tn = min(n - t0, 1u << shift);
+  const int lane = threadIdx.x & (kWave - 1);
+  const int wid = threadIdx.x / kWave;
+  for (uint32_t i = threadIdx.x; i < tn; i += kTileBlock) par[i] = i;
+  if (threadIdx.x == 0) s_count = 0;
+  __syncthreads();
+  uint2 *seg = pairs + (size_t)blockIdx.x * cap;
+  tile_rows_lanes(row_ptr, col, t0, tn, [&](bool have, uint32_t vl, uint32_t w) {
+              bool out = false;
+              if (have && w < n) {
+                const uint32_t wl = w - t0;  // inside the tile iff below tn
+                if (wl < tn) {
+                  // A root v links straight under a smaller neighbour (its
+                  // tree holds only ids >= v, so w is in another tree); its
+                  // other entries, and a v already linked, take a union.
+                  if (wl < vl) {
+                    if (tile_ld(&par[vl]) != vl || atomicCAS(&par[vl], vl, wl) != vl)
+                      tile_unite(par, vl, wl);
+                  } else if (!Sym && wl != vl) {
+                    tile_unite(par, vl, wl);
+                  }
+                } else {
+                  out = !Sym || w < t0;
+                }
+              }
+              const uint64_t bal = __ballot(out);
+              if (bal == 0) return;
+              const int leader = __builtin_ctzll(bal);
+              uint32_t base = 0;
+              if (lane == leader) base = atomicAdd(&s_count, (uint32_t)__popcll(bal));
+              base = __shfl(base, leader, kWave);  // (the leader runs this call)
+              if (out) {
+                const uint32_t slot = base + (uint32_t)__popcll(bal & ((1ull << lane) - 1));
+                if (slot < cap) seg[slot] = make_uint2(t0 + vl, w);
+              }
+            });
+  __syncthreads();
+  for (uint32_t i0 = wid * kWave; i0 < tn; i0 += kTileBlock) {
+    const uint32_t i = i0 + lane;
+    bool root = false;
+    if (i < tn) {
+      const uint32_t r = tile_find(par, i);
+      p[t0 + i] = t0 + r;
+      root = r == i;
+    }
+    const uint64_t b = __ballot(root);
+    const uint32_t word = (t0 + i0) >> 5;
+    if (lane == 0) bits[word] = (uint32_t)b;
+    if (lane == 1 && i0 + 32 < tn) bits[word + 1] = (uint32_t)(b >> 32);
+  }
+  if (threadIdx.x == 0) counts[blockIdx.x] = s_count;
+}
+
+// Union of the trees of v and w (a pair of v's tile), unless another thread
+// of this workgroup entered the same two parents in the table first: that
+// thread unites them, so they are united by the time the kernel ends.  A
+// slot taken by another pair is overwritten (its later repeats then unite
+// again, which is only extra work).  The union is Rem's with splicing from
+// the two parents, local roots: it climbs only local roots and writes only
+// their entries, with local roots, so every other vertex keeps its local
+// root as parent.
+__device__ __forceinline__ void tile_cross_unite(uint32_t v, uint32_t w,
+                                                 uint32_t *__restrict__ p,
+                                                 unsigned long long *seen) {
+  const uint32_t a = p[v];
+  const uint32_t b = p[w];
+  if (a == b) return;
+  const unsigned long long key = ((unsigned long long)a << 32) | b;
+  const uint32_t h = (uint32_t)((key * 0x9E3779B97F4A7C15ull) >> (64 - kTileSeenBits));
+  const unsigned long long old = atomicCAS(&seen[h], kTileNoPair, key);
+  if (old == key) return;
+  if (old != kTileNoPair) atomicExch(&seen[h], key);
+  rem_unite<1>(a, b, p);
+}
+
+// Step 2.
+template <int Sym>
+__global__ __launch_bounds__(kTileBlock) void k_tile_cross(
+    const uint64_t *__restrict__ row_ptr, const uint32_t *__restrict__ col, uint32_t n,
+    uint32_t shift, uint32_t *__restrict__ p, const uint32_t *__restrict__ counts,
+    const uint2 *__restrict__ pairs, uint32_t cap, const uint32_t *__restrict__ sw) {
+  __shared__ unsigned long long seen[kTileSeen];
+  if (sw[0] != 1) return;
+  for (uint32_t i = threadIdx.x; i < kTileSeen; i += kTileBlock) seen[i] = kTileNoPair;
+  __syncthreads();
+  const uint32_t t0 = blockIdx.x << shift;
+  const uint32_t tn = min(n - t0, 1u << shift);
+  const uint32_t cnt = counts[blockIdx.x];
+  if (cnt <= cap) {
+    const uint2 *seg = pairs + (size_t)blockIdx.x * cap;
+    for (uint32_t i = threadIdx.x; i < cnt; i += kTileBlock) {
+      const uint2 pr = seg[i];
+      tile_cross_unite(pr.x, pr.y, p, seen);
+    }
+    return;
+  }
+  tile_rows_lanes(row_ptr, col, t0, tn, [&](bool have, uint32_t vl, uint32_t w) {
+    if (have && w < n && w - t0 >= tn && (!Sym || w < t0)) tile_cross_unite(t0 + vl, w, p, seen);
+  });
+}
+
+// Step 3: every local root (bit set) that was linked takes its root.  No
+// links happen here, so the walk ends at the true root; the halving and the
+// final store are atomicMin, which never raises an entry, so each local root
+// ends holding exactly its root whatever other walks store through it.
+__global__ __launch_bounds__(kBlock) void k_tile_roots(uint32_t n, uint32_t *__restrict__ p,
+                                                       const uint32_t *__restrict__ bits,
+                                                       const uint32_t *__restrict__ sw) {
+  if (sw[0] != 1) return;
+  const uint32_t words = (uint32_t)(((uint64_t)n + 31) >> 5);
+  CCX_GRID_STRIDE(i, words) {
+    uint32_t b = bits[i];
+    while (b) {
+      const uint32_t r = i * 32 + (uint32_t)__builtin_ctz(b);
+      b &= b - 1;
+      uint32_t x = p[r];
+      if (x == r) continue;
+      for (;;) {
+        const uint32_t px = p[x];
+        if (px == x) break;
+        const uint32_t ppx = p[px];
+        if (ppx == px) {
+          x = px;
+          break;
+        }
+        (void)__hip_atomic_fetch_min(&p[x], ppx, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        x = ppx;
+      }
+      (void)__hip_atomic_fetch_min(&p[r], x, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
+  }
+}
+
+// Step 4, four vertices per thread per step: p[v] = p[p[v]].  A vertex's
+// entry is its local root's (unchanged since step 1), whose entry is its
+// root after step 3; local roots themselves already hold their root.
+__global__ __launch_bounds__(kBlock) void k_tile_final(uint32_t n, uint32_t *__restrict__ p,
+                                                       const uint32_t *__restrict__ sw) {
+  constexpr int ILP = 4;
+  if (sw[0] != 1) return;
+  const uint64_t S = (uint64_t)gridDim.x * kBlock;
+  for (uint64_t v0 = (uint64_t)blockIdx.x * kBlock + threadIdx.x; v0 < n; v0 += S * ILP) {
+    uint32_t x[ILP], y[ILP];
+#pragma unroll
+    for (int k = 0; k < ILP; ++k) {
+      const uint64_t v = v0 + k * S;
+      x[k] = v < n ? p[v] : (uint32_t)v;
+    }
+#pragma unroll
+    for (int k = 0; k < ILP; ++k) {
+      const uint64_t v = v0 + k * S;
+      y[k] = (v < n && x[k] != (uint32_t)v) ? p[x[k]] : x[k];
+    }
+#pragma unroll
+    for (int k = 0; k < ILP; ++k)
+      if (y[k] != x[k]) p[v0 + k * S] = y[k];
+  }
+}
+
+// One workgroup, one sampled vertex per thread: words[2] = 1 if the tile
+// plan runs (force, or at most a quarter of the sampled rows' first
+// kTileProbeEntries entries leave their vertex's tile), else 0.
+__global__ __launch_bounds__(kTileProbeSamples) void k_tile_probe(
+    const uint64_t *__restrict__ row_ptr, const uint32_t *__restrict__ col, uint32_t n,
+    uint32_t shift, int force, uint32_t *__restrict__ words) {
+  __shared__ uint32_t s_tot, s_out;
+  if (threadIdx.x == 0) {
+    s_tot = 0;
+    s_out = 0;
+  }
+  __syncthreads();
+  if (!force) {
+    const uint32_t v = sample_vertex(threadIdx.x, n);
+    const uint64_t b = row_ptr[v];
+    const uint64_t e = row_ptr[v + 1];
+    uint32_t w[kTileProbeEntries];
+#pragma unroll
+    for (int k = 0; k < kTileProbeEntries; ++k) w[k] = b + k < e ? col[b + k] : 0xFFFFFFFFu;
+    uint32_t tot = 0, out = 0;
+#pragma unroll
+    for (int k = 0; k < kTileProbeEntries; ++k)
+      if (w[k] < n && w[k] != v) {
+        ++tot;
+        out += (w[k] >> shift) != (v >> shift);
+      }
+    atomicAdd(&s_tot, tot);
+    atomicAdd(&s_out, out);
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) words[2] = (force || (s_tot > 0 && 4u * s_out <= s_tot)) ? 1u : 0u;
+}
+
 // Defaults measured on R-MAT scales 20-25, Chung-Lu 8.2M-65.6M vertices,
 // ER 4M/32M (degree 8) and 2048^2 / 8192^2 grids (70% of edges kept):
 //  * the frontier sweep and init/compress want 8 blocks per CU, one vertex
@@ -999,6 +1380,7 @@ constexpr uint64_t kDenseDegree = 16;              // dense: m >= 16 n
 constexpr uint64_t kHalveMinVertices = 1ull << 24;  // halving from this many vertices
 constexpr uint64_t kWideIlpMaxEntries = 1ull << 31; // dense one-pass ILP 2 below this many entries
 constexpr uint64_t kSkipMinDegree = 6;              // sparse graphs skip from m >= 6 n
+constexpr uint64_t kTileMinVertices = 1ull << 20;   // the tile probe runs from this many vertices
 
 struct Plan {
   int cache;     // sampled-neighbour cache in use
@@ -1015,6 +1397,9 @@ struct Plan {
   int skip1;     // own-entry links skip neighbour 1; the frontier sweep starts at entry 1
   int minima_entries;  // ... and local minima unite with this many entries of their row
   int compress_ilp;    // vertices per thread per step of the pointer-jumping passes
+  int tiles;           // vertex tiles: 0 no, 1 always, 2 if the probe finds local ids
+  uint32_t tile_shift; // log2 of the vertices per tile
+  uint32_t tile_cap;   // pairs per tile segment
 };
 
 inline Plan plan_for(uint32_t n, uint64_t m, const Knobs &K = g_knobs) {
@@ -1084,7 +1469,28 @@ inline Plan plan_for(uint32_t n, uint64_t m, const Knobs &K = g_knobs) {
   // sweep links the vertices left outside the giant tree): 4 vertices per
   // thread, Chung-Lu 65.6M 87 -> 75 us
   pl.compress_ilp = K.stream_ilp > 1 ? K.stream_ilp : (pl.skip1 ? 4 : 1);
+  // Vertex tiles (k_tile_local).  The probe runs where the sampled path is
+  // the one whose kernels read the plan switch: the sparse plan uniting by
+  // Rem's algorithm from the sample arrays.  Below 2^20 vertices a solve
+  // is short whichever plan runs.
+  const bool tile_shape = pl.cache && pl.fused && pl.rem && !pl.csr && K.hook_mode == 4 &&
+                          K.sampled_neighbours == 2 && pl.init_ilp == 1;
+  const bool tile_auto = tile_shape && !dense && n >= kTileMinVertices;
+  pl.tiles = K.tiles > 0 ? 1 : (K.tiles == 0 ? 0 : (tile_auto ? 2 : 0));
+  pl.tile_shift = K.tile_shift ? (uint32_t)K.tile_shift : kTileDefaultShift;
+  pl.tile_cap = K.tile_pairs ? (uint32_t)K.tile_pairs : (1u << pl.tile_shift);
   return pl;
+}
+
+// Words of the tile plan's workspace: per-tile pair counts and the local
+// root bitmap (rounded up to 256 bytes), then the pair segments.
+inline uint64_t tile_head_words(uint32_t n, const Plan &pl) {
+  const uint64_t nt = ((uint64_t)n + (1u << pl.tile_shift) - 1) >> pl.tile_shift;
+  return (nt + ((uint64_t)n + 31) / 32 + 63) & ~63ull;
+}
+inline uint64_t tile_words(uint32_t n, const Plan &pl) {
+  const uint64_t nt = ((uint64_t)n + (1u << pl.tile_shift) - 1) >> pl.tile_shift;
+  return tile_head_words(n, pl) + 2 * nt * pl.tile_cap;
 }
 
 inline bool ilp_ok(int v) { return v == 1 || v == 2 || v == 4; }
@@ -1114,6 +1520,10 @@ extern "C" int cc_hip_components_set(const char *key, int value) {
   else if (k == "skip_linked" && value >= -1 && value <= 2) g_knobs.skip_linked = value;
   else if (k == "minima_entries" && (value == 0 || (value >= 2 && value <= 8))) g_knobs.minima_entries = value;
   else if (k == "symmetric" && (value == 0 || value == 1)) g_knobs.symmetric = value;
+  else if (k == "tiles" && value >= -1 && value <= 1) g_knobs.tiles = value;
+  else if (k == "tile_shift" && (value == 0 || (value >= (int)kTileMinShift && value <= (int)kTileMaxShift)))
+    g_knobs.tile_shift = value;
+  else if (k == "tile_pairs" && value >= 0 && value <= (int)kTileMaxPairs) g_knobs.tile_pairs = value;
   else return ccx::set_error(4, "cc_hip_components_set: bad key or value (%s=%d)", key, value);
   return 0;
 }
@@ -1140,6 +1550,9 @@ extern "C" int cc_hip_components_get(const char *key) {
   if (k == "skip_linked") return g_knobs.skip_linked;
   if (k == "minima_entries") return g_knobs.minima_entries;
   if (k == "symmetric") return g_knobs.symmetric;
+  if (k == "tiles") return g_knobs.tiles;
+  if (k == "tile_shift") return g_knobs.tile_shift;
+  if (k == "tile_pairs") return g_knobs.tile_pairs;
   return -1;
 }
 
@@ -1151,6 +1564,19 @@ extern "C" int cc_hip_components_last_pass_ran(void *stream, int *ran) {
   if (!st) return 3;
   uint32_t flag = 0;
   CCX_TRY(hipMemcpyAsync(&flag, st->d_words + 1, sizeof(flag), hipMemcpyDeviceToHost, s));
+  CCX_TRY(hipStreamSynchronize(s));
+  *ran = flag != 0;
+  return 0;
+}
+
+extern "C" int cc_hip_components_tiles_ran(void *stream, int *ran) {
+  if (!ccx::ctx()) return 2;
+  if (!ran) return ccx::set_error(4, "cc_hip_components_tiles_ran: null pointer");
+  hipStream_t s = ccx::pick_stream(stream);
+  ccx::StreamState *st = ccx::stream_state(s);
+  if (!st) return 3;
+  uint32_t flag = 0;
+  CCX_TRY(hipMemcpyAsync(&flag, st->d_words + 2, sizeof(flag), hipMemcpyDeviceToHost, s));
   CCX_TRY(hipStreamSynchronize(s));
   *ran = flag != 0;
   return 0;
@@ -1175,6 +1601,9 @@ extern "C" int cc_hip_components_plan_get(uint32_t n, uint64_t m, const char *ke
   if (k == "hook_blocks_per_cu") return pl.hook_bpc;
   if (k == "giant_early") return pl.cache && !pl.fused && g_knobs.giant_early;
   if (k == "hook_mode") return g_knobs.hook_mode;
+  if (k == "tiles") return pl.tiles;
+  if (k == "tile_shift") return (int)pl.tile_shift;
+  if (k == "tile_pairs") return (int)pl.tile_cap;
   return -1;
 }
 
@@ -1206,13 +1635,13 @@ namespace {
 template <int Splice, bool Nt>
 int launch_hook_rem(int skip, unsigned nb, hipStream_t s, const uint32_t *s0, const uint32_t *s1,
                     uint32_t n, uint32_t *p, const uint64_t *row_ptr, const uint32_t *col,
-                    uint32_t minima_entries) {
+                    uint32_t minima_entries, const uint32_t *off) {
   if (skip == 2)
-    CCX_LAUNCH("cc_hook_sample", s, k_hook_rem<Splice, Nt, 2><<<nb, kBlock, 0, s>>>(s0, s1, n, p, row_ptr, col, minima_entries));
+    CCX_LAUNCH("cc_hook_sample", s, k_hook_rem<Splice, Nt, 2><<<nb, kBlock, 0, s>>>(s0, s1, n, p, row_ptr, col, minima_entries, off));
   else if (skip)
-    CCX_LAUNCH("cc_hook_sample", s, k_hook_rem<Splice, Nt, 1><<<nb, kBlock, 0, s>>>(s0, s1, n, p, row_ptr, col, minima_entries));
+    CCX_LAUNCH("cc_hook_sample", s, k_hook_rem<Splice, Nt, 1><<<nb, kBlock, 0, s>>>(s0, s1, n, p, row_ptr, col, minima_entries, off));
   else
-    CCX_LAUNCH("cc_hook_sample", s, k_hook_rem<Splice, Nt, 0><<<nb, kBlock, 0, s>>>(s0, s1, n, p));
+    CCX_LAUNCH("cc_hook_sample", s, k_hook_rem<Splice, Nt, 0><<<nb, kBlock, 0, s>>>(s0, s1, n, p, row_ptr, col, minima_entries, off));
   return 0;
 }
 
@@ -1241,11 +1670,12 @@ int launch_hook_pair_ilp(int ilp, int skip, unsigned nb, hipStream_t s, const ui
   return launch_hook_pair<Mode, 1, Halve, Nt>(skip, nb, s, s0, s1, n, p);
 }
 
-// The whole solve on stream s with knobs K.  words[0..1]: the sampled giant
-// root and the frontier sweep's link flag (written before they are read);
-// smp: rounds * n words for the sampled-neighbour cache when the plan uses
-// it (written by the init pass before the hooks read it).  Allocates
-// nothing itself.
+// The whole solve on stream s with knobs K.  words[0..2]: the sampled giant
+// root, the frontier sweep's link flag and the plan switch (each written
+// before it is read); smp: rounds * n words for the sampled-neighbour cache
+// when the plan uses it (written by the init pass before the hooks read
+// it), or tile_words() for the tile plan, whichever is larger when both are
+// launched (only one of them does work).  Allocates nothing itself.
 int solve_with(const Knobs &K, const ccx::Context *c, const uint64_t *d_row_ptr,
                const uint32_t *d_col_idx, uint32_t n, uint64_t m, uint32_t *d_labels,
                hipStream_t s, uint32_t *words, uint32_t *smp) {
@@ -1265,11 +1695,42 @@ int solve_with(const Knobs &K, const ccx::Context *c, const uint64_t *d_row_ptr,
   }
   // slots: nullptr for a pass between hook rounds, the giant root and the
   // frontier's link flag for the last pass
+  // The tile plan: launched first; with the probe deciding, the sampled
+  // path's kernels below read the same switch word and return at once when
+  // the tiles took the solve.
+  uint32_t *sw = words + 2;
+  const uint32_t *off = plan.tiles == 2 ? sw : nullptr;
+  if (plan.tiles) {
+    if (!smp) return ccx::set_error(4, "solve: no tile workspace");
+    const uint32_t shift = plan.tile_shift;
+    const uint32_t cap = plan.tile_cap;
+    const uint32_t nt = (uint32_t)(((uint64_t)n + (1u << shift) - 1) >> shift);
+    uint32_t *counts = smp;
+    uint32_t *bits = smp + nt;
+    uint2 *pairs = (uint2 *)(smp + tile_head_words(n, plan));
+    const unsigned nw = blocks_for(((uint64_t)n + 31) / 32, c->cu_count, K.stream_blocks_per_cu);
+    CCX_LAUNCH("cc_tile_probe", s, k_tile_probe<<<1, kTileProbeSamples, 0, s>>>(d_row_ptr, d_col_idx, n, shift, plan.tiles == 1, words));
+#define CCX_TILE_LOCAL(SYM, LDS)                                                                   \
+  CCX_LAUNCH("cc_tile_local", s, k_tile_local<SYM, LDS><<<nt, kTileBlock, 0, s>>>(d_row_ptr, d_col_idx, n, shift, d_labels, bits, counts, pairs, cap, sw))
+    if (K.symmetric) {
+      if (shift > kTileSmallShift) CCX_TILE_LOCAL(1, kTileMaxShift);
+      else CCX_TILE_LOCAL(1, kTileSmallShift);
+      CCX_LAUNCH("cc_tile_cross", s, k_tile_cross<1><<<nt, kTileBlock, 0, s>>>(d_row_ptr, d_col_idx, n, shift, d_labels, counts, pairs, cap, sw));
+    } else {
+      if (shift > kTileSmallShift) CCX_TILE_LOCAL(0, kTileMaxShift);
+      else CCX_TILE_LOCAL(0, kTileSmallShift);
+      CCX_LAUNCH("cc_tile_cross", s, k_tile_cross<0><<<nt, kTileBlock, 0, s>>>(d_row_ptr, d_col_idx, n, shift, d_labels, counts, pairs, cap, sw));
+    }
+#undef CCX_TILE_LOCAL
+    CCX_LAUNCH("cc_tile_roots", s, k_tile_roots<<<nw, kBlock, 0, s>>>(n, d_labels, bits, sw));
+    CCX_LAUNCH("cc_tile_final", s, k_tile_final<<<ns, kBlock, 0, s>>>(n, d_labels, sw));
+    if (plan.tiles == 1) return 0;
+  }
   auto compress = [&](const uint32_t *gate) -> int {
     const int cilp = plan.compress_ilp;
-    if (cilp == 4) CCX_LAUNCH("cc_compress", s, k_compress_ilp<4><<<ns, kBlock, 0, s>>>(n, d_labels, gate));
-    else if (cilp == 2) CCX_LAUNCH("cc_compress", s, k_compress_ilp<2><<<ns, kBlock, 0, s>>>(n, d_labels, gate));
-    else CCX_LAUNCH("cc_compress", s, k_compress<<<ns, kBlock, 0, s>>>(n, d_labels, gate));
+    if (cilp == 4) CCX_LAUNCH("cc_compress", s, k_compress_ilp<4><<<ns, kBlock, 0, s>>>(n, d_labels, gate, off));
+    else if (cilp == 2) CCX_LAUNCH("cc_compress", s, k_compress_ilp<2><<<ns, kBlock, 0, s>>>(n, d_labels, gate, off));
+    else CCX_LAUNCH("cc_compress", s, k_compress<<<ns, kBlock, 0, s>>>(n, d_labels, gate, off));
     return 0;
   };
   const int mode = K.hook_mode;
@@ -1305,7 +1766,7 @@ int solve_with(const Knobs &K, const ccx::Context *c, const uint64_t *d_row_ptr,
   } else if (cache) {
     if (!smp) return ccx::set_error(4, "solve: no sample buffer");
     switch (rounds) {
-      case 1: CCX_LAUNCH("cc_init_min", s, k_init_sample<1><<<ns, kBlock, 0, s>>>(d_row_ptr, d_col_idx, n, d_labels, smp)); break;
+      case 1: CCX_LAUNCH("cc_init_min", s, k_init_sample<1><<<ns, kBlock, 0, s>>>(d_row_ptr, d_col_idx, n, d_labels, smp, off)); break;
       case 2:
         if (plan.init_ilp == 2 || plan.init_ilp == 4) {
 #define CCX_INIT2(NT)                                                                              \
@@ -1325,14 +1786,14 @@ int solve_with(const Knobs &K, const ccx::Context *c, const uint64_t *d_row_ptr,
           break;
         }
         switch (plan.nt & 3) {
-          case 1: CCX_LAUNCH("cc_init_min", s, k_init_sample<2, 1><<<ns, kBlock, 0, s>>>(d_row_ptr, d_col_idx, n, d_labels, smp)); break;
-          case 2: CCX_LAUNCH("cc_init_min", s, k_init_sample<2, 2><<<ns, kBlock, 0, s>>>(d_row_ptr, d_col_idx, n, d_labels, smp)); break;
-          case 3: CCX_LAUNCH("cc_init_min", s, k_init_sample<2, 3><<<ns, kBlock, 0, s>>>(d_row_ptr, d_col_idx, n, d_labels, smp)); break;
-          default: CCX_LAUNCH("cc_init_min", s, k_init_sample<2><<<ns, kBlock, 0, s>>>(d_row_ptr, d_col_idx, n, d_labels, smp)); break;
+          case 1: CCX_LAUNCH("cc_init_min", s, k_init_sample<2, 1><<<ns, kBlock, 0, s>>>(d_row_ptr, d_col_idx, n, d_labels, smp, off)); break;
+          case 2: CCX_LAUNCH("cc_init_min", s, k_init_sample<2, 2><<<ns, kBlock, 0, s>>>(d_row_ptr, d_col_idx, n, d_labels, smp, off)); break;
+          case 3: CCX_LAUNCH("cc_init_min", s, k_init_sample<2, 3><<<ns, kBlock, 0, s>>>(d_row_ptr, d_col_idx, n, d_labels, smp, off)); break;
+          default: CCX_LAUNCH("cc_init_min", s, k_init_sample<2><<<ns, kBlock, 0, s>>>(d_row_ptr, d_col_idx, n, d_labels, smp, off)); break;
         }
         break;
-      case 3: CCX_LAUNCH("cc_init_min", s, k_init_sample<3><<<ns, kBlock, 0, s>>>(d_row_ptr, d_col_idx, n, d_labels, smp)); break;
-      default: CCX_LAUNCH("cc_init_min", s, k_init_sample<4><<<ns, kBlock, 0, s>>>(d_row_ptr, d_col_idx, n, d_labels, smp)); break;
+      case 3: CCX_LAUNCH("cc_init_min", s, k_init_sample<3><<<ns, kBlock, 0, s>>>(d_row_ptr, d_col_idx, n, d_labels, smp, off)); break;
+      default: CCX_LAUNCH("cc_init_min", s, k_init_sample<4><<<ns, kBlock, 0, s>>>(d_row_ptr, d_col_idx, n, d_labels, smp, off)); break;
     }
   } else if (ilp == 4) {
     CCX_LAUNCH("cc_init_min", s, k_init_min_ilp<4><<<ns, kBlock, 0, s>>>(d_row_ptr, d_col_idx, n, d_labels));
@@ -1352,14 +1813,14 @@ int solve_with(const Knobs &K, const ccx::Context *c, const uint64_t *d_row_ptr,
     } else if (plan.rem) {
       const bool nt = (plan.nt & 4) != 0;
       if (plan.splice == 1)
-        rc = nt ? launch_hook_rem<1, true>(sk, nb, s, smp, s1, n, d_labels, d_row_ptr, d_col_idx, me)
-                : launch_hook_rem<1, false>(sk, nb, s, smp, s1, n, d_labels, d_row_ptr, d_col_idx, me);
+        rc = nt ? launch_hook_rem<1, true>(sk, nb, s, smp, s1, n, d_labels, d_row_ptr, d_col_idx, me, off)
+                : launch_hook_rem<1, false>(sk, nb, s, smp, s1, n, d_labels, d_row_ptr, d_col_idx, me, off);
       else if (plan.splice == 2)
-        rc = nt ? launch_hook_rem<2, true>(sk, nb, s, smp, s1, n, d_labels, d_row_ptr, d_col_idx, me)
-                : launch_hook_rem<2, false>(sk, nb, s, smp, s1, n, d_labels, d_row_ptr, d_col_idx, me);
+        rc = nt ? launch_hook_rem<2, true>(sk, nb, s, smp, s1, n, d_labels, d_row_ptr, d_col_idx, me, off)
+                : launch_hook_rem<2, false>(sk, nb, s, smp, s1, n, d_labels, d_row_ptr, d_col_idx, me, off);
       else
-        rc = nt ? launch_hook_rem<0, true>(sk, nb, s, smp, s1, n, d_labels, d_row_ptr, d_col_idx, me)
-                : launch_hook_rem<0, false>(sk, nb, s, smp, s1, n, d_labels, d_row_ptr, d_col_idx, me);
+        rc = nt ? launch_hook_rem<0, true>(sk, nb, s, smp, s1, n, d_labels, d_row_ptr, d_col_idx, me, off)
+                : launch_hook_rem<0, false>(sk, nb, s, smp, s1, n, d_labels, d_row_ptr, d_col_idx, me, off);
     } else if (mode == 1) {
       rc = plan.halve ? launch_hook_pair_ilp<1, true, false>(hook_ilp, 0, nb, s, smp, s1, n, d_labels)
                       : launch_hook_pair_ilp<1, false, false>(hook_ilp, 0, nb, s, smp, s1, n, d_labels);
@@ -1417,7 +1878,7 @@ int solve_with(const Knobs &K, const ccx::Context *c, const uint64_t *d_row_ptr,
   // (sampled again after early skipping: the giant tree has since absorbed
   // other trees under smaller roots, whose members would not pass through
   // the early root)
-  CCX_LAUNCH("cc_sample_giant", s, k_sample_giant<<<1, kGiantSamples, 0, s>>>(n, d_labels, giant));
+  CCX_LAUNCH("cc_sample_giant", s, k_sample_giant<<<1, kGiantSamples, 0, s>>>(n, d_labels, giant, off));
   // Exact sampling (modes 0-2) links neighbours 0..rounds-1 of every vertex,
   // so the sweep starts after them; lossy sampling (mode 3) links nothing
   // for sure, so the sweep covers every neighbour.
@@ -1432,11 +1893,11 @@ int solve_with(const Knobs &K, const ccx::Context *c, const uint64_t *d_row_ptr,
 #define CCX_FRONTIER(M)                                                                            \
   do {                                                                                             \
     if (plan.frontier_chunks == 4)                                                                 \
-      CCX_LAUNCH("cc_hook_frontier", s, k_hook_frontier<M, 4><<<nf, kBlock, 0, s>>>(d_row_ptr, d_col_idx, n, first, d_labels, giant, every)); \
+      CCX_LAUNCH("cc_hook_frontier", s, k_hook_frontier<M, 4><<<nf, kBlock, 0, s>>>(d_row_ptr, d_col_idx, n, first, d_labels, giant, every, off)); \
     else if (plan.frontier_chunks == 2)                                                            \
-      CCX_LAUNCH("cc_hook_frontier", s, k_hook_frontier<M, 2><<<nf, kBlock, 0, s>>>(d_row_ptr, d_col_idx, n, first, d_labels, giant, every)); \
+      CCX_LAUNCH("cc_hook_frontier", s, k_hook_frontier<M, 2><<<nf, kBlock, 0, s>>>(d_row_ptr, d_col_idx, n, first, d_labels, giant, every, off)); \
     else                                                                                           \
-      CCX_LAUNCH("cc_hook_frontier", s, k_hook_frontier<M, 1><<<nf, kBlock, 0, s>>>(d_row_ptr, d_col_idx, n, first, d_labels, giant, every)); \
+      CCX_LAUNCH("cc_hook_frontier", s, k_hook_frontier<M, 1><<<nf, kBlock, 0, s>>>(d_row_ptr, d_col_idx, n, first, d_labels, giant, every, off)); \
   } while (0)
   if (mode == 0) CCX_FRONTIER(0);
   else if (mode == 1) CCX_FRONTIER(1);
@@ -1447,6 +1908,19 @@ int solve_with(const Knobs &K, const ccx::Context *c, const uint64_t *d_row_ptr,
   return compress(giant);
 }
 
+}  // namespace
+
+namespace {
+// Words of temporary memory the solve needs beyond the control words: the
+// sample cache, the tile workspace, or the larger of the two when the probe
+// decides between them.
+uint64_t work_words(uint32_t n, uint64_t m, const Knobs &K, const Plan &pl) {
+  if (m == 0) return 0;
+  if (pl.tiles == 1) return tile_words(n, pl);
+  const uint64_t cache = (pl.cache && !pl.csr) ? (uint64_t)K.sampled_neighbours * n : 0;
+  const uint64_t tiles = pl.tiles ? tile_words(n, pl) : 0;
+  return cache > tiles ? cache : tiles;
+}
 }  // namespace
 
 extern "C" int cc_hip_components_device(const uint64_t *d_row_ptr, const uint32_t *d_col_idx,
@@ -1467,8 +1941,9 @@ extern "C" int cc_hip_components_device(const uint64_t *d_row_ptr, const uint32_
   const Knobs K = g_knobs;
   uint32_t *smp = nullptr;
   const Plan pl = plan_for(n, m, K);
-  if (m > 0 && pl.cache && !pl.csr) {
-    smp = (uint32_t *)ccx::stream_tmp(st, (size_t)K.sampled_neighbours * n * sizeof(uint32_t));
+  const uint64_t words = work_words(n, m, K, pl);
+  if (words) {
+    smp = (uint32_t *)ccx::stream_tmp(st, words * sizeof(uint32_t));
     if (!smp) return 3;
   }
   return solve_with(K, c, d_row_ptr, d_col_idx, n, m, d_labels, s, st->d_words, smp);
@@ -1478,23 +1953,30 @@ extern "C" int cc_hip_components_device(const uint64_t *d_row_ptr, const uint32_
 //
 // cc_hip_components_device_noalloc runs the default plan with its scratch in
 // the file-scope array `scratch` below instead of the stream's state: 64
-// control words (the sampled giant root and the frontier's link flag; both
-// written before they are read) followed by the sampled-neighbour cache, 2
-// words per vertex (written by the init pass before the hook reads it).  So
-// nothing needs clearing between calls.  Every call on one GPU shares that
-// GPU's copy of the array: calls that overlap on one GPU race.
+// control words (the sampled giant root, the frontier's link flag and the
+// plan switch; each written before it is read) followed by the
+// sampled-neighbour cache, 2 words per vertex (written by the init pass
+// before the hook reads it), or the tile plan's workspace (pair counts and
+// root bitmap written by the local pass before they are read).  So nothing
+// needs clearing between calls.  Every call on one GPU shares that GPU's
+// copy of the array: calls that overlap on one GPU race.
 
 namespace {
 constexpr uint32_t kNoAllocMaxVertices = 1u << 22;  // 4,194,304 vertices
 constexpr size_t kNoAllocControlWords = 64;
-constexpr size_t kScratchWords = kNoAllocControlWords + 2 * (size_t)kNoAllocMaxVertices;
+// the larger of the sample cache (2 words per vertex) and the tile plan's
+// workspace at the default tile shape, for up to kNoAllocMaxVertices
+constexpr size_t kNoAllocWorkWords =
+    2 * (size_t)kNoAllocMaxVertices +
+    ((((size_t)kNoAllocMaxVertices >> kTileDefaultShift) + kNoAllocMaxVertices / 32 + 63) & ~(size_t)63);
+constexpr size_t kScratchWords = kNoAllocControlWords + kNoAllocWorkWords;
 constexpr int kMaxDevices = 64;
 // Device address of `scratch` on each device, from hipGetSymbolAddress at
 // initialisation (the array's name is not a device address in host code).
 uint32_t *g_scratch_dev[kMaxDevices];
 }  // namespace
 
-// 32 MiB + 256 B, held on every GPU the library runs on until the process ends.
+// 32.5 MiB, held on every GPU the library runs on until the process ends.
 static __device__ __align__(256) uint32_t scratch[kScratchWords];
 
 namespace ccx {
@@ -1531,6 +2013,9 @@ extern "C" int cc_hip_components_device_noalloc(const uint64_t *d_row_ptr,
   if (!ptr)
     return ccx::set_error(2, "cc_hip_components_device_noalloc: scratch array not set up");
   const hipStream_t s = stream ? (hipStream_t)stream : c->stream;
+  if (work_words(n, m, kDefaultKnobs, plan_for(n, m, kDefaultKnobs)) > kNoAllocWorkWords)
+    return ccx::set_error(4, "cc_hip_components_device_noalloc: the plan for n=%u needs more "
+                             "scratch than the array holds", n);
   const ccx::ProfMute mute;  // no timing events either
   return solve_with(kDefaultKnobs, c, d_row_ptr, d_col_idx, n, m, d_labels, s, ptr,
                     ptr + kNoAllocControlWords);

@@ -84,8 +84,8 @@ int cc_hip_components_device(const uint64_t *d_row_ptr, const uint32_t *d_col_id
  * reference: it is for callers that cannot allocate device memory (code
  * under HIP graph capture, tight serving loops, harnesses that forbid
  * hipMalloc).  Runs the default plan (whatever cc_hip_components_set has
- * tuned) with its scratch in a fixed device array of the library (32 MiB +
- * 256 B per GPU, held until the process ends) instead of the stream's state;
+ * tuned) with its scratch in a fixed device array of the library (32.5 MiB
+ * per GPU, held until the process ends) instead of the stream's state;
  * calls no hipMalloc, hipFree or other allocating routine, records no timing
  * events, and never initialises the library: cc_hip_init must have run
  * (return 2 otherwise).  n <= 4,194,304 (return 4 beyond).  Same labels as
@@ -188,6 +188,26 @@ int cc_hip_components_tune(int sampled_neighbours, int blocks_per_cu, int hook_m
  *                             all its entries, giving weakly connected
  *                             components of a CSR with one-directional
  *                             entries (a union per entry)
+ *   "tiles"                   the vertex-tile plan for graphs whose ids are
+ *                             local (a lattice or mesh numbered row by row):
+ *                             each workgroup unites the entries inside its
+ *                             tile of consecutive ids in LDS, then the pairs
+ *                             between tiles are united in HBM: 1 always, 0
+ *                             never, -1 (default) = a probe on the device
+ *                             picks it when at most a quarter of 4096
+ *                             sampled rows' entries leave their tile, on
+ *                             sparse graphs (m < 16 n) of 2^20+ vertices
+ *                             that the default sampled plan would give to
+ *                             Rem's unions from the sample arrays; the solve
+ *                             never waits for the probe on the host
+ *   "tile_shift"              log2 of the vertices per tile: 6..15, or 0
+ *                             (default) = 15 (a 128 KiB LDS parent array;
+ *                             up to 14, 64 KiB)
+ *   "tile_pairs"              pairs between tiles kept per tile (a tile with
+ *                             more reads its rows again): 1..2^24, or 0
+ *                             (default) = one per vertex of the tile; the
+ *                             plan's temporary memory is about 8 bytes per
+ *                             vertex at the default
  * Returns nonzero for an unknown key or out-of-range value; _get returns -1
  * for an unknown key. */
 int cc_hip_components_set(const char *key, int value);
@@ -202,13 +222,18 @@ int cc_hip_components_plan(uint32_t n, uint64_t m, int *cache, int *fused, int *
  * "halve", "hook_ilp", "hook_blocks_per_cu", "giant_early" (in effect for
  * this graph), "hook_mode", "nt", "init_ilp", "frontier_chunks", "csr_hook",
  * "skip_linked", "minima_entries", "compress_ilp" (vertices per thread of
- * the pointer-jumping passes).  Returns -1 for an unknown key. */
+ * the pointer-jumping passes), "tiles" (0 no, 1 always, 2 the probe
+ * decides), "tile_shift", "tile_pairs".  Returns -1 for an unknown key. */
 int cc_hip_components_plan_get(uint32_t n, uint64_t m, const char *key);
 /* Whether the last solve on `stream` (NULL = the library stream) ran its
  * final pointer-jumping pass: *ran = 1 if its frontier sweep linked a root,
  * 0 if the pass was skipped because every vertex already held its final
  * root.  Blocking (waits for the stream). */
 int cc_hip_components_last_pass_ran(void *stream, int *ran);
+/* Whether the last solve with entries on `stream` (NULL = the library
+ * stream) ran the vertex-tile plan: *ran = 1 if so, 0 if the sampled plan
+ * ran.  Blocking (waits for the stream). */
+int cc_hip_components_tiles_ran(void *stream, int *ran);
 
 /* Same result through host buffers (H2D, solve, D2H; blocking). */
 int cc_hip_components_host(const uint64_t *row_ptr, const uint32_t *col_idx,

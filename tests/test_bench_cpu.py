@@ -97,6 +97,40 @@ def test_frontier_stats_match_a_union_find_restatement():
             assert 0.5 < got["giant_fraction"] < 1.0
 
 
+def test_tile_stats_match_a_union_find_restatement():
+    """The vertex-tile byte model's inputs (pairs to smaller tiles, local
+    roots, linked local roots, vertices whose local root is not final) on
+    small lattices numbered row by row and permuted, against a plain
+    union-find over the entries inside each tile."""
+    import oracle
+    for permute, shift in ((False, 7), (True, 6), (False, 9)):
+        G = cm.grid(40, 64, 0.7, seed=3, permute=permute)
+        lab = oracle.cc_min_label(G.row_ptr, G.col_idx)
+        st = bench.tile_stats(G, shift, lab)
+        n, T = G.n, 1 << shift
+        par = list(range(n))
+
+        def find(x):
+            while par[x] != x:
+                x = par[x]
+            return x
+
+        cross = 0
+        for v in range(n):
+            for w in G.col_idx[G.row_ptr[v]:G.row_ptr[v + 1]].tolist():
+                cross += w // T < v // T
+                if w // T == v // T and w != v:
+                    a, b = find(v), find(w)
+                    if a != b:
+                        par[max(a, b)] = min(a, b)
+        local = [find(v) for v in range(n)]
+        roots = sum(local[v] == v for v in range(n))
+        assert st["cross_pairs"] == cross
+        assert st["local_roots"] == roots
+        assert st["linked_local_roots"] == roots - int(np.count_nonzero(lab == np.arange(n)))
+        assert st["moved"] == sum(local[v] != v and lab[v] != local[v] for v in range(n))
+
+
 def test_bench_refuses_more_than_one_gpu():
     r = subprocess.run([sys.executable, os.path.join(ROOT, "bench.py"), "--gpus", "2"],
                        capture_output=True, text=True, timeout=120)

@@ -101,23 +101,31 @@ def test_chung_lu(gpu):
     check_labels(G, dropin(G))
 
 
-@pytest.mark.parametrize("kind,rem,splice,halve,skip", [
-    ("er", 1, 1, 0, 0), ("er8", 1, 1, 0, 2), ("grid", 1, 1, 0, 0), ("rmat24", 0, 0, 1, 0)])
-def test_default_plan_at_scale(gpu, kind, rem, splice, halve, skip):
+@pytest.mark.parametrize("kind,rem,splice,halve,skip,tiles,ran", [
+    ("er", 1, 1, 0, 0, 2, 0), ("er8", 1, 1, 0, 2, 2, 0), ("grid", 1, 1, 0, 0, 2, 0),
+    ("grid_rows", 1, 1, 0, 0, 2, 1), ("rmat24", 0, 0, 1, 0, 0, 0)])
+def test_default_plan_at_scale(gpu, kind, rem, splice, halve, skip, tiles, ran):
     """Graphs of 2^24 vertices: sparse ones take Rem's unions with splicing,
     dense ones the lockstep hook with path halving, by default, and sparse
-    ones of degree 6 and more skip neighbour 1 of linked vertices; labels,
+    ones of degree 6 and more skip neighbour 1 of linked vertices; on sparse
+    ones a probe on the device picks the vertex-tile plan where ids are
+    local (the lattice numbered row by row) and not elsewhere; labels,
     component count and size histogram equal the oracle's."""
     from cc_mpi_amd import devgen
     lib = nat.gpu_lib()
     D = (devgen.erdos_renyi(1 << 24, 3.0, seed=41) if kind == "er"
          else devgen.erdos_renyi(1 << 24, 8.0, seed=44) if kind == "er8"
          else devgen.grid(4096, 4096, 0.7, seed=42) if kind == "grid"
+         else devgen.grid(4096, 4096, 0.7, seed=45, permute=False) if kind == "grid_rows"
          else devgen.rmat(24, 16, seed=43))
     plan = {k: lib.cc_hip_components_plan_get(D.n, D.m, k.encode())
-            for k in ("fused", "rem", "rem_splice", "halve", "skip_linked")}
-    assert plan == dict(fused=1, rem=rem, rem_splice=splice, halve=halve, skip_linked=skip)
+            for k in ("fused", "rem", "rem_splice", "halve", "skip_linked", "tiles")}
+    assert plan == dict(fused=1, rem=rem, rem_splice=splice, halve=halve, skip_linked=skip,
+                        tiles=tiles)
     got = D.solve()
+    took = ctypes.c_int(-1)
+    nat.check(lib.cc_hip_components_tiles_ran(None, ctypes.byref(took)), "tiles ran")
+    assert took.value == ran
     G = D.to_host()
     D.close()
     want = check_labels(G, got)
@@ -265,10 +273,17 @@ PATHS = [  # every code path of the solve: (knobs) -> same labels
     dict(rem=0, halve=1, hook_ilp=1, skip_linked=2), dict(rem=0, halve=1, hook_ilp=2, skip_linked=1),
     dict(rem=0, halve=0, hook_ilp=4, skip_linked=2), dict(rem=0, halve=1, hook_ilp=2, skip_linked=2, nt=5),
     dict(skip_linked=2, stream_ilp=2), dict(skip_linked=0),
+    # vertex tiles: whole plan forced on every graph, at several tile sizes,
+    # with pair segments small enough that tiles overflow and read their rows
+    # again; and forced off
+    dict(tiles=1), dict(tiles=1, tile_shift=6), dict(tiles=1, tile_shift=9, tile_pairs=1),
+    dict(tiles=1, tile_shift=12, tile_pairs=64), dict(tiles=1, tile_shift=7, stream_blocks_per_cu=1),
+    dict(tiles=0),
 ]
 DEFAULTS = dict(sample_cache=1, fused_pair=-1, hook_ilp=0, hook_mode=4, sampled_neighbours=2,
                 stream_ilp=1, blocks_per_cu=0, giant_early=1, halve=-1, rem=-1, rem_splice=-1,
-                nt=-1, init_ilp=0, frontier_chunks=0, csr_hook=-1, skip_linked=-1, minima_entries=0)
+                nt=-1, init_ilp=0, frontier_chunks=0, csr_hook=-1, skip_linked=-1, minima_entries=0,
+                tiles=-1, tile_shift=0, tile_pairs=0, stream_blocks_per_cu=8)
 
 
 @pytest.mark.parametrize("knobs", PATHS, ids=lambda k: ",".join(f"{a}={b}" for a, b in k.items()))
@@ -277,7 +292,8 @@ def test_every_solve_path(gpu, knobs):
     lib = nat.gpu_lib()
     graphs = [cm.rmat(15, 16, seed=31), cm.erdos_renyi(50000, 3.0, seed=32),
               cm.grid(120, 150, 0.6, seed=33), disjoint_cliques(200, 5, seed=34),
-              cm.chung_lu(1 << 16, 1 << 19, 2.1, seed=35)]
+              cm.chung_lu(1 << 16, 1 << 19, 2.1, seed=35),
+              cm.grid(130, 170, 0.6, seed=36, permute=False)]
     try:
         for k, v in knobs.items():
             nat.check(lib.cc_hip_components_set(k.encode(), v), k)
@@ -301,7 +317,8 @@ def test_every_solve_path(gpu, knobs):
     dict(rem=0, halve=1, hook_ilp=2, skip_linked=2),
     dict(sample_cache=1, fused_pair=0, hook_ilp=2, hook_mode=1, giant_early=1),
     dict(sample_cache=1, fused_pair=0, hook_ilp=1, hook_mode=4, giant_early=1, sampled_neighbours=3),
-    dict(sample_cache=0, fused_pair=0, hook_ilp=1, hook_mode=3)],
+    dict(sample_cache=0, fused_pair=0, hook_ilp=1, hook_mode=3),
+    dict(tiles=1, tile_shift=7), dict(tiles=1, tile_shift=6, tile_pairs=1)],
     ids=lambda k: ",".join(f"{a}={b}" for a, b in k.items()) or "default")
 def test_one_directional_entries_with_symmetric_off(gpu, knobs):
     """With the knob symmetric = 0 the solve unites every stored entry, so a
@@ -311,6 +328,7 @@ def test_one_directional_entries_with_symmetric_off(gpu, knobs):
     lib = nat.gpu_lib()
     graphs = [one_directional(cm.rmat(13, 16, seed=51), 0.2, 1),
               one_directional(cm.grid(90, 110, 0.7, seed=52), 0.1, 2),
+              one_directional(cm.grid(80, 120, 0.7, seed=55, permute=False), 0.2, 5),
               one_directional(cm.erdos_renyi(30000, 3.0, seed=53), 0.3, 3),
               one_directional(cm.chung_lu(1 << 15, 1 << 18, 2.1, seed=54), 0.0, 4)]
     try:

@@ -7,7 +7,8 @@
   pinned against the reference's own build), with the component count and
   size histogram.
 * The 8192 x 8192 lattice with 30% of its edges deleted (high diameter, many
-  components), against the oracle's union-find.
+  components), with permuted ids and numbered row by row, against the
+  oracle's union-find.
 * Rank 0 of a 4-rank vertex-block partition of the same Chung-Lu graph at
   real shape (16,400,256 owned vertices): split, local CC, send lists,
   boundary build and ghosts against their definitions and the oracle, then
@@ -66,13 +67,21 @@ def test_chung_lu_65m_full_graph(chung_lu):
     assert np.array_equal(D.solve(), truth)
 
 
-def test_grid_8192_30pct_deleted(gpu):
-    print("building the 8192^2 lattice on the device", flush=True)
-    D = devgen.grid(8192, 8192, 0.7, seed=1004)
+@pytest.mark.parametrize("permute", [True, False], ids=["permuted", "row_major"])
+def test_grid_8192_30pct_deleted(gpu, permute):
+    """Both numberings: permuted ids (the sampled path) and row by row (the
+    probe picks the vertex-tile plan)."""
+    import ctypes
+    from cc_mpi_amd import _native as nat
+    print(f"building the 8192^2 lattice on the device (permute={permute})", flush=True)
+    D = devgen.grid(8192, 8192, 0.7, seed=1004, permute=permute)
     G = D.to_host()
     truth = oracle.cc_min_label(G.row_ptr, G.col_idx)
     got = D.solve()
+    ran = ctypes.c_int(-1)
+    nat.check(nat.gpu_lib().cc_hip_components_tiles_ran(None, ctypes.byref(ran)), "tiles ran")
     D.close()
+    assert ran.value == (0 if permute else 1)
     bad = np.flatnonzero(got != truth)
     assert bad.size == 0, f"{bad.size} labels differ, first {bad[:5]}"
     count, hist = histogram(got)

@@ -43,9 +43,12 @@ def test_noalloc_equals_device_solve(gpu, use_stream):
         nat.check(lib.cc_hip_stream_create(ctypes.byref(s)), "stream")
         stream = s.value
     # the largest graph the scratch holds, then smaller ones
+    # (the lattice numbered row by row takes the vertex-tile plan, whose
+    # workspace is the larger one)
     graphs = [devgen.rmat(22, 16, seed=1002), devgen.grid(300, 200, 0.7, seed=5),
+              devgen.grid(2048, 2048, 0.7, seed=6, permute=False),
               devgen.erdos_renyi(70_000, 3.0, seed=8)]
-    assert graphs[0].n == MAX_VERTICES
+    assert graphs[0].n == MAX_VERTICES and graphs[2].n == MAX_VERTICES
     outs = [cm.DeviceBuffer(4 * G.n) for G in graphs]
     wants = [G.solve() for G in graphs]  # cc_hip_components_device, library stream
     for G, want in zip(graphs[1:], wants[1:]):
