@@ -190,9 +190,9 @@ def kernel_bytes(name: str, n: int, st: dict, plan: dict) -> float | None:
             # the bitmap, then each linked local root's entry read and written
             return n / 8.0 + 8.0 * st["linked_local_roots"]
         if name == "cc_tile_final":
-            # every entry, the local root's entry for every other vertex, the
-            # entries that move
-            return 4.0 * n + 4.0 * (n - st["local_roots"]) + 4.0 * st["moved"]
+            # every entry, each local root's entry (read by all of its
+            # members, counted once), the entries that move
+            return 4.0 * n + 4.0 * st["local_roots"] + 4.0 * st["moved"]
         return None
     if name.startswith("cc_tile_"):
         return None  # the tile plan's kernels returned at once

@@ -1007,10 +1007,11 @@ inline unsigned blocks_for(uint64_t n, int cu_count, int per_cu) {
 // the sampled hook builds long parent chains (a row of consecutive ids links
 // one vertex under the next) and every union walks them: an 8192^2 lattice
 // with 30% of its edges deleted, numbered row by row, took 14.6 ms per solve
-// that way against 5.2 ms with its ids permuted.  Such graphs take this plan:
+// that way against 5.2 ms with its ids permuted; this plan takes 1.20 ms:
 //   1. k_tile_local: one workgroup per tile of 2^shift consecutive ids
 //      streams the tile's rows (one contiguous slice of col_idx) and unites
-//      every entry that stays inside the tile in an LDS parent array (a
+//      every entry that stays inside the tile in an LDS array of 16-bit
+//      local parents (a
 //      vertex still a root links straight under its smaller neighbour, the
 //      rest by Rem's unions).
 //      Entries that leave the tile go to the tile's segment of a pair list
@@ -1038,16 +1039,16 @@ inline unsigned blocks_for(uint64_t n, int cu_count, int per_cu) {
 // of the one not taken return at once, so the solve never waits on the host.
 
 constexpr int kTileBlock = 1024;
-constexpr uint32_t kTileMaxShift = 15;  // 32,768 vertices: 128 KiB of LDS parents
+constexpr uint32_t kTileMaxShift = 16;  // 65,536 vertices: 16-bit local ids, 128 KiB of LDS parents
 constexpr uint32_t kTileMinShift = 6;   // a tile covers whole 64-bit words of the root bitmap
-// Tiles up to 2^14 vertices take a 64 KiB LDS parent array (two workgroups
-// per CU).  The default is 2^15 all the same: on the 8192^2 lattice
-// numbered row by row, 2^14-vertex tiles (two rows) fall apart into many
-// more local components than 2^15-vertex ones (four rows), and the unions
-// between tiles cost more than the occupancy gains (2.9 against 1.55 ms per
-// solve).
-constexpr uint32_t kTileSmallShift = 14;
-constexpr uint32_t kTileDefaultShift = 15;
+// Tiles up to 2^15 vertices take a 64 KiB LDS parent array (two workgroups
+// per CU), larger ones 128 KiB (one).  The default is the largest tile: on
+// the 8192^2 lattice numbered row by row a 2^16-vertex tile holds eight
+// rows and rarely falls apart, while narrower tiles split into many local
+// components whose unions between tiles cost more than the extra
+// occupancy gains (1.20 ms per solve at 2^16, 1.58 at 2^15, 3.2 at 2^14).
+constexpr uint32_t kTileSmallShift = 15;
+constexpr uint32_t kTileDefaultShift = 16;
 constexpr uint32_t kTileMaxPairs = 1u << 24;
 constexpr int kTileProbeSamples = 1024;
 constexpr int kTileProbeEntries = 8;  // entries read per sampled row
@@ -1055,54 +1056,72 @@ constexpr uint32_t kTileSeenBits = 12;  // pairs entered per tile in the cross p
 constexpr uint32_t kTileSeen = 1u << kTileSeenBits;
 constexpr unsigned long long kTileNoPair = ~0ull;  // parents are below 2^32 - 1
 
-__device__ __forceinline__ uint32_t tile_ld(const uint32_t *a) {
-  return __hip_atomic_load(a, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
-}
-__device__ __forceinline__ void tile_st(uint32_t *a, uint32_t v) {
-  __hip_atomic_store(a, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
-}
+// The LDS parents of a tile as 16-bit local ids, two to a 32-bit word:
+// vertex i is half i >> (Shift - 1) of word i mod 2^(Shift - 1), so
+// consecutive ids, which neighbouring lanes link at the same time, sit in
+// different words.  Loads and stores are 16-bit LDS accesses; a CAS goes
+// through a 32-bit CAS of the word, retried while only the other half
+// changed.
+template <uint32_t Shift>
+struct TilePar {
+  static constexpr uint32_t kHalf = 1u << (Shift - 1);
+  uint32_t *w;
+  __device__ __forceinline__ uint16_t *at(uint32_t i) const {
+    return reinterpret_cast<uint16_t *>(w) + (((i & (kHalf - 1)) << 1) | (i >> (Shift - 1)));
+  }
+  __device__ __forceinline__ uint32_t ld(uint32_t i) const {
+    return __hip_atomic_load(at(i), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+  }
+  __device__ __forceinline__ void st(uint32_t i, uint32_t v) const {
+    __hip_atomic_store(at(i), (uint16_t)v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+  }
+  // Replaces i's parent by v if it is `expect`; returns the parent found.
+  __device__ __forceinline__ uint32_t cas(uint32_t i, uint32_t expect, uint32_t v) const {
+    uint32_t *word = &w[i & (kHalf - 1)];
+    const uint32_t sh = (i >> (Shift - 1)) << 4;
+    uint32_t old = __hip_atomic_load(word, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+    for (;;) {
+      const uint32_t cur = (old >> sh) & 0xFFFFu;
+      if (cur != expect) return cur;
+      const uint32_t got = atomicCAS(word, old, (old & ~(0xFFFFu << sh)) | (v << sh));
+      if (got == old) return expect;
+      old = got;
+    }
+  }
+};
 
 // Root of x among the LDS parents, halving the path on the way.  Only roots
 // are ever linked (by CAS), so a halving store, which gives a non-root its
 // grandparent, cannot undo a link.
-__device__ __forceinline__ uint32_t tile_find(uint32_t *par, uint32_t x) {
+template <class P>
+__device__ __forceinline__ uint32_t tile_find(const P &par, uint32_t x) {
   for (;;) {
-    const uint32_t px = tile_ld(&par[x]);
+    const uint32_t px = par.ld(x);
     if (px == x) return x;
-    const uint32_t ppx = tile_ld(&par[px]);
+    const uint32_t ppx = par.ld(px);
     if (ppx == px) return px;
-    tile_st(&par[x], ppx);
+    par.st(x, ppx);
     x = ppx;
   }
 }
 
-// Union in LDS by Rem's algorithm with splicing (as rem_unite<1> in HBM):
-// the two paths are climbed together, the one with the larger parent
-// stepping, whose entry is first lowered (atomicMin) to the other path's
-// parent, which both unites and shortens; a root is linked by a CAS that
-// holds only while it is still a root.  The walk stops where the paths
-// meet.
-__device__ __forceinline__ void tile_unite(uint32_t *par, uint32_t x, uint32_t y) {
-  uint32_t px = tile_ld(&par[x]);
-  uint32_t py = tile_ld(&par[y]);
-  while (px != py) {
-    if (px < py) {
-      uint32_t t = x;
-      x = y;
-      y = t;
-      t = px;
-      px = py;
-      py = t;
-    }
-    if (x == px) {  // x is a root, py < x
-      const uint32_t old = atomicCAS(&par[x], x, py);
-      if (old == x) return;
-      px = old;
-      continue;
-    }
-    atomicMin(&par[x], py);
-    x = px;
-    px = tile_ld(&par[x]);
+// Union in LDS: both roots found with path halving (plain 16-bit stores),
+// then the larger linked under the smaller by a CAS that holds only while it
+// is still a root; otherwise the walk resumes from it.  (Rem's unions, which
+// lower an entry at every step, cost a 32-bit CAS loop per step on 16-bit
+// entries: the local pass took 1,305 us with them on the 8192^2 lattice
+// numbered row by row, 2^15-vertex tiles.)
+template <class P>
+__device__ __forceinline__ void tile_unite(const P &par, uint32_t a, uint32_t b) {
+  for (;;) {
+    a = tile_find(par, a);
+    b = tile_find(par, b);
+    if (a == b) return;
+    const uint32_t hi = a > b ? a : b;
+    const uint32_t lo = a > b ? b : a;
+    if (par.cas(hi, hi, lo) == hi) return;
+    a = hi;
+    b = lo;
   }
 }
 
@@ -1112,7 +1131,9 @@ __device__ __forceinline__ void tile_unite(uint32_t *par, uint32_t x, uint32_t y
 // the step's row offsets, then the first kTileRowEntries entries of each of
 // its rows, are loaded before f runs on any of them; a row's end offset is
 // the next lane's start offset.  Longer rows continue in a loop, so f runs
-// on diverged lanes (its ballots count the lanes that run it).
+// on diverged lanes (its ballots count the lanes that run it).  (Loading a
+// step ahead of f measured slower: the unions, not the loads, bound the
+// pass.)
 constexpr int kTileRows = 4;
 constexpr int kTileRowEntries = 4;
 template <class F>
@@ -1151,7 +1172,7 @@ __device__ __forceinline__ void tile_rows_lanes(const uint64_t *__restrict__ row
 }
 
 // Step 1.  Sym: the CSR is symmetric (Knobs::symmetric).  LdsShift: the LDS
-// parent array's size (shift <= LdsShift).  counts[tile] is the number of
+// parent array's size, 2^LdsShift 16-bit entries (shift <= LdsShift).  counts[tile] is the number of
 // pairs the tile produced, also those past its segment's cap.
 template <int Sym, uint32_t LdsShift>
 __global__ __launch_bounds__(kTileBlock) void k_tile_local(
@@ -1159,14 +1180,16 @@ __global__ __launch_bounds__(kTileBlock) void k_tile_local(
     uint32_t shift, uint32_t *__restrict__ p, uint32_t *__restrict__ bits,
     uint32_t *__restrict__ counts, uint2 *__restrict__ pairs, uint32_t cap,
     const uint32_t *__restrict__ sw) {
-  __shared__ uint32_t par[1u << LdsShift];
+  constexpr uint32_t kHalf = TilePar<LdsShift>::kHalf;
+  __shared__ uint32_t words[kHalf];
   __shared__ uint32_t s_count;
   if (sw[0] != 1) return;
+  const TilePar<LdsShift> par{words};
   const uint32_t t0 = blockIdx.x << shift;
   const uint32_t tn = min(n - t0, 1u << shift);
   const int lane = threadIdx.x & (kWave - 1);
   const int wid = threadIdx.x / kWave;
-  for (uint32_t i = threadIdx.x; i < tn; i += kTileBlock) par[i] = i;
+  for (uint32_t j = threadIdx.x; j < kHalf; j += kTileBlock) words[j] = j | ((j + kHalf) << 16);
   if (threadIdx.x == 0) s_count = 0;
   __syncthreads();
   uint2 *seg = pairs + (size_t)blockIdx.x * cap;
@@ -1175,11 +1198,12 @@ __global__ __launch_bounds__(kTileBlock) void k_tile_local(
               if (have && w < n) {
                 const uint32_t wl = w - t0;  // inside the tile iff below tn
                 if (wl < tn) {
-                  // A root v links straight under a smaller neighbour (its
-                  // tree holds only ids >= v, so w is in another tree); its
-                  // other entries, and a v already linked, take a union.
+                  // A root v links straight under a smaller neighbour's
+                  // root (v's tree holds only ids >= v, so w is in another
+                  // tree), which keeps v one hop from its root; its other
+                  // entries, and a v already linked, take a union.
                   if (wl < vl) {
-                    if (tile_ld(&par[vl]) != vl || atomicCAS(&par[vl], vl, wl) != vl)
+                    if (par.ld(vl) != vl || par.cas(vl, vl, tile_find(par, wl)) != vl)
                       tile_unite(par, vl, wl);
                   } else if (!Sym && wl != vl) {
                     tile_unite(par, vl, wl);

@@ -200,9 +200,9 @@ int cc_hip_components_tune(int sampled_neighbours, int blocks_per_cu, int hook_m
  *                             that the default sampled plan would give to
  *                             Rem's unions from the sample arrays; the solve
  *                             never waits for the probe on the host
- *   "tile_shift"              log2 of the vertices per tile: 6..15, or 0
- *                             (default) = 15 (a 128 KiB LDS parent array;
- *                             up to 14, 64 KiB)
+ *   "tile_shift"              log2 of the vertices per tile: 6..16, or 0
+ *                             (default) = 16 (16-bit local parents: a
+ *                             128 KiB LDS array; up to 15, 64 KiB)
  *   "tile_pairs"              pairs between tiles kept per tile (a tile with
  *                             more reads its rows again): 1..2^24, or 0
  *                             (default) = one per vertex of the tile; the

@@ -278,6 +278,7 @@ PATHS = [  # every code path of the solve: (knobs) -> same labels
     # again; and forced off
     dict(tiles=1), dict(tiles=1, tile_shift=6), dict(tiles=1, tile_shift=9, tile_pairs=1),
     dict(tiles=1, tile_shift=12, tile_pairs=64), dict(tiles=1, tile_shift=7, stream_blocks_per_cu=1),
+    dict(tiles=1, tile_shift=16), dict(tiles=1, tile_shift=16, tile_pairs=3),
     dict(tiles=0),
 ]
 DEFAULTS = dict(sample_cache=1, fused_pair=-1, hook_ilp=0, hook_mode=4, sampled_neighbours=2,
