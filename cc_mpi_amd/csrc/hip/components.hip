@@ -1033,8 +1033,8 @@ inline unsigned blocks_for(uint64_t n, int cu_count, int per_cu) {
 // Every link goes from a larger root to a smaller one, in LDS and in HBM, so
 // each root is its component's minimum, as on the sampled path.
 // When the plan is chosen automatically, k_tile_probe decides on the device
-// from the rows of 1024 sampled vertices: tiles when at most a quarter of
-// their entries leave the vertex's tile.  Its answer is a switch word that
+// from the rows of 1024 sampled vertices: tiles when at most 35% of their
+// entries leave the vertex's tile.  Its answer is a switch word that
 // every kernel of both plans reads; both plans are launched and the kernels
 // of the one not taken return at once, so the solve never waits on the host.
 
@@ -1052,6 +1052,15 @@ constexpr uint32_t kTileDefaultShift = 16;
 constexpr uint32_t kTileMaxPairs = 1u << 24;
 constexpr int kTileProbeSamples = 1024;
 constexpr int kTileProbeEntries = 8;  // entries read per sampled row
+// The plan runs when at most 35% of the sampled entries leave their tile.
+// Measured on 2^24-vertex graphs (sampled plan against tiles, per solve):
+// a 256^3 lattice numbered row by row (a third of its entries cross tiles)
+// 2.30 against 0.59 ms; random banded graphs of degree 8 whose edges span
+// up to 32K / 48K / 64K ids (about 25 / 37 / 50% crossing) 1.13 / 1.11 /
+// 1.10 against 1.25 / 1.41 / 1.58 ms; degree 4 and 32K ids 5.12 against
+// 0.78 ms.
+constexpr uint32_t kTileProbeNum = 7;
+constexpr uint32_t kTileProbeDen = 20;
 constexpr uint32_t kTileSeenBits = 12;  // pairs entered per tile in the cross pass
 constexpr uint32_t kTileSeen = 1u << kTileSeenBits;
 constexpr unsigned long long kTileNoPair = ~0ull;  // parents are below 2^32 - 1
@@ -1347,8 +1356,8 @@ __global__ __launch_bounds__(kBlock) void k_tile_final(uint32_t n, uint32_t *__r
 }
 
 // One workgroup, one sampled vertex per thread: words[2] = 1 if the tile
-// plan runs (force, or at most a quarter of the sampled rows' first
-// kTileProbeEntries entries leave their vertex's tile), else 0.
+// plan runs (force, or at most kTileProbeNum / kTileProbeDen of the sampled
+// rows' first kTileProbeEntries entries leave their vertex's tile), else 0.
 __global__ __launch_bounds__(kTileProbeSamples) void k_tile_probe(
     const uint64_t *__restrict__ row_ptr, const uint32_t *__restrict__ col, uint32_t n,
     uint32_t shift, int force, uint32_t *__restrict__ words) {
@@ -1376,7 +1385,8 @@ __global__ __launch_bounds__(kTileProbeSamples) void k_tile_probe(
     atomicAdd(&s_out, out);
   }
   __syncthreads();
-  if (threadIdx.x == 0) words[2] = (force || (s_tot > 0 && 4u * s_out <= s_tot)) ? 1u : 0u;
+  if (threadIdx.x == 0)
+    words[2] = (force || (s_tot > 0 && kTileProbeDen * s_out <= kTileProbeNum * s_tot)) ? 1u : 0u;
 }
 
 // Defaults measured on R-MAT scales 20-25, Chung-Lu 8.2M-65.6M vertices,

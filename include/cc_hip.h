@@ -194,8 +194,9 @@ int cc_hip_components_tune(int sampled_neighbours, int blocks_per_cu, int hook_m
  *                             tile of consecutive ids in LDS, then the pairs
  *                             between tiles are united in HBM: 1 always, 0
  *                             never, -1 (default) = a probe on the device
- *                             picks it when at most a quarter of 4096
- *                             sampled rows' entries leave their tile, on
+ *                             picks it when at most 35% of the first 8
+ *                             entries of 1024 sampled rows leave their
+ *                             tile, on
  *                             sparse graphs (m < 16 n) of 2^20+ vertices
  *                             that the default sampled plan would give to
  *                             Rem's unions from the sample arrays; the solve

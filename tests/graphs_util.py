@@ -42,6 +42,32 @@ def disjoint_cliques(k, size, seed=0):
     return from_pairs(n, src, dst, perm=perm, name=f"{k}xK{size}")
 
 
+def lattice3d(side, keep, seed):
+    """side^3 lattice numbered row by row (id = x + side*y + side^2*z), each
+    of its 6-neighbour edges kept with probability `keep`."""
+    rng = np.random.default_rng(seed)
+    n = side ** 3
+    v = np.arange(n, dtype=np.int64)
+    coord = (v % side, (v // side) % side, v // (side * side))
+    src, dst = [], []
+    for c, step in zip(coord, (1, side, side * side)):
+        sel = v[(c + 1 < side) & (rng.random(n) < keep)]
+        src.append(sel)
+        dst.append(sel + step)
+    return from_pairs(n, np.concatenate(src), np.concatenate(dst), name=f"lattice3d-{side}")
+
+
+def banded(n, degree, width, seed):
+    """n*degree/2 random edges, each from a uniform vertex to one up to
+    `width` ids above it (dropped past n)."""
+    rng = np.random.default_rng(seed)
+    m = n * degree // 2
+    a = rng.integers(0, n, m)
+    b = a + rng.integers(1, width, m)
+    ok = b < n
+    return from_pairs(n, a[ok], b[ok], name=f"banded-{n}-d{degree}-w{width}")
+
+
 def bfs_min_labels(G):
     """Brute force: BFS from each unvisited vertex in id order (so the
     starting vertex is the component minimum)."""

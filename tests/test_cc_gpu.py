@@ -11,7 +11,8 @@ import pytest
 import cc_mpi_amd as cm
 from cc_mpi_amd import _native as nat
 import oracle
-from graphs_util import complete, csr_raw, disjoint_cliques, one_directional, path, star
+from graphs_util import (banded, complete, csr_raw, disjoint_cliques, lattice3d, one_directional,
+                         path, star)
 
 pytestmark = pytest.mark.gpu
 
@@ -181,6 +182,29 @@ def test_device_resident_repeat(gpu):
     for _ in range(5):
         assert np.array_equal(D.solve(), first)
     D.close()
+
+
+@pytest.mark.parametrize("make,tiles", [
+    (lambda: lattice3d(256, 0.5, seed=61), 1),          # a third of the entries cross tiles
+    (lambda: banded(1 << 21, 4, 1 << 13, seed=62), 1),   # local, low degree
+    (lambda: banded(1 << 21, 8, 1 << 20, seed=63), 0),   # edges spanning 2^20 ids
+    (lambda: cm.grid(1448, 1448, 0.7, seed=64), 0),      # permuted ids
+], ids=["lattice3d", "banded_local", "banded_wide", "grid_permuted"])
+def test_probe_choice_on_local_and_nonlocal_graphs(gpu, make, tiles):
+    """Sparse graphs of 2^20+ vertices: the probe on the device picks the
+    vertex-tile plan where ids are local and the sampled plan elsewhere;
+    labels are exact either way."""
+    lib = nat.gpu_lib()
+    G = make()
+    assert G.n >= 1 << 20
+    assert lib.cc_hip_components_plan_get(G.n, G.m, b"tiles") == 2
+    D = cm.DeviceGraph(G)
+    got = D.solve()
+    took = ctypes.c_int(-1)
+    nat.check(lib.cc_hip_components_tiles_ran(None, ctypes.byref(took)), "tiles ran")
+    D.close()
+    assert took.value == tiles
+    check_labels(G, got)
 
 
 def test_rmat22_full_size(gpu):
