@@ -56,6 +56,40 @@ def test_edge_cases(gpu):
     assert dropin(G).tolist() == [0, 0, 0, 0]
 
 
+@pytest.mark.parametrize("knobs", [dict(tiles=1), dict(tiles=1, tile_shift=6, tile_pairs=1),
+                                   dict(tiles=1, tile_shift=6, symmetric=0)],
+                         ids=lambda k: ",".join(f"{a}={b}" for a, b in k.items()))
+def test_edge_cases_vertex_tiles(gpu, knobs):
+    """The hand-built CSRs above with the vertex-tile plan forced: one tile
+    or several of 64 ids, with pair segments of one entry."""
+    lib = nat.gpu_lib()
+    cases = [
+        (csr_raw(1, [[]]), [0]),
+        (csr_raw(5, [[], [], [], [], []]), [0, 1, 2, 3, 4]),
+        (csr_raw(6, [[1, 1, 0], [0, 0, 1], [77], [4], [3, 3, 4], [5]]), [0, 0, 2, 3, 3, 5]),
+        (csr_raw(4, [[3, 2], [2], [1, 0], [0]]), [0, 0, 0, 0]),
+    ]
+    # rows across tiles of 64: a path 0-70-130-200 and a pair 65-199,
+    # duplicates, self-loops, entries past n and unsorted rows
+    n = 210
+    adj = [[] for _ in range(n)]
+    for a, b in ((0, 70), (70, 130), (130, 200), (65, 199), (5, 6), (6, 5), (63, 64)):
+        adj[a].append(b)
+        adj[b].append(a)
+    adj[9] += [9, 999, 9]
+    adj[130] = adj[130][::-1]
+    want = oracle.cc_min_label(csr_raw(n, adj).row_ptr, csr_raw(n, adj).col_idx).tolist()
+    cases.append((csr_raw(n, adj), want))
+    try:
+        for k, v in knobs.items():
+            nat.check(lib.cc_hip_components_set(k.encode(), v), k)
+        for G, want in cases:
+            assert dropin(G).tolist() == want
+    finally:
+        for k, v in dict(tiles=-1, tile_shift=0, tile_pairs=0, symmetric=1).items():
+            nat.check(lib.cc_hip_components_set(k.encode(), v), k)
+
+
 def test_empty_graph(gpu):
     G = cm.Graph(np.zeros(1, dtype=np.uint64), np.zeros(0, dtype=np.uint32))
     assert cm.compute_connected_components_pthreads(G).size == 0
