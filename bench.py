@@ -76,7 +76,17 @@ def make_device_graph(name: str):
         f"Chung-Lu n={w['n']} pairs={w['pairs']} gamma {w['gamma']}, permuted, seed {w['seed']}")
 
 
-def frontier_stats(G, two_pass: bool = True, skip: bool = False):
+def low_bound(n: int, shift: int) -> int:
+    """Ids below this bound are hooked first by the low-first hook (0: the
+    plan hooks every vertex in one pass); the device's rule."""
+    if not shift:
+        return 0
+    L = ((n >> shift) + 63) & ~63
+    L = max(L, 64)
+    return 0 if L >= n else L
+
+
+def frontier_stats(G, two_pass: bool = True, skip: bool = False, low: int = 0):
     """Size of the frontier the hooking sweep visits: vertices not in the
     sampled giant tree after hooking each vertex with neighbours 0 and 1.
     The trees of that sampled forest are the components of the sampled edge
@@ -86,7 +96,11 @@ def frontier_stats(G, two_pass: bool = True, skip: bool = False):
     above the vertex), and the sweep starts at entry 1.  (A vertex whose
     own link fails because another thread linked it first also unites
     neighbour 1 on the device; that depends on timing and is rare, so the
-    model leaves it out.)"""
+    model leaves it out.)  low: the low-first hook's bound L; the counts
+    are then also split at L (suffix _lo for ids below L, _hi for the rest),
+    and `marked_hi` counts the linked vertices at or above L whose
+    neighbour 0 lies below L in the sampled giant tree, the ones the low-id
+    map links without a parent gather."""
     import scipy.sparse as sp
     from scipy.sparse.csgraph import connected_components
     n = G.n
@@ -120,6 +134,22 @@ def frontier_stats(G, two_pass: bool = True, skip: bool = False):
     active = front & (rem > 0)
     up0 = int(np.count_nonzero(minimum))
     minima_gt1 = int(np.count_nonzero(minimum & (deg > 1)))
+    split = {}
+    if low:
+        lo = np.zeros(n, dtype=bool)
+        lo[:low] = True
+        has = deg > 0
+        w0 = np.full(n, n, dtype=np.int64)
+        w0[has0] = first
+        linked = has & ~minimum
+        marked = linked & ~lo & (w0 < low)
+        marked[marked] = comp[w0[marked]] == giant
+        for tag, sel in (("lo", lo), ("hi", ~lo)):
+            split[f"deg_gt0_{tag}"] = int(np.count_nonzero(has & sel))
+            split[f"up0_{tag}"] = int(np.count_nonzero(minimum & sel))
+            split[f"minima_gt1_{tag}"] = int(np.count_nonzero(minimum & (deg > 1) & sel))
+        split["marked_hi"] = int(np.count_nonzero(marked))
+        split["low_bound"] = int(low)
     live1 = None
     if two_pass:
         # forest after round 0 alone (neighbour 0 of every vertex) and its
@@ -132,7 +162,7 @@ def frontier_stats(G, two_pass: bool = True, skip: bool = False):
     return dict(frontier_vertices=int(active.sum()), frontier_edges=int(rem[active].sum()),
                 giant_fraction=float((~front).mean()),
                 deg_gt0=int((deg > 0).sum()), deg_gt1=int((deg > 1).sum()), up0=up0,
-                minima_gt1=minima_gt1, round1_outside_early_giant=live1)
+                minima_gt1=minima_gt1, round1_outside_early_giant=live1, **split)
 
 
 def tile_stats(G, shift: int, labels: np.ndarray) -> dict:
@@ -206,6 +236,26 @@ def kernel_bytes(name: str, n: int, st: dict, plan: dict) -> float | None:
             return 8.0 * (n + 1) + 4.0 * (d0 + d1) + 4.0 * n + 8.0 * n
         # row_ptr, first col entry of each non-isolated vertex, own parent write
         return 8.0 * (n + 1) + 4.0 * d0 + 4.0 * n
+    L = st.get("low_bound", 0)
+    if name in ("cc_hook_low", "cc_hook_sample") and L:
+        # the low-first hook: each pass is the one-hop skipping hook from
+        # the CSR over its ids (below L, or L..n-1): row_ptr, the first col
+        # entry of each row and the second of each local minimum, own parent
+        # read and written, one parent gather per neighbour the minima's
+        # walks unite, and neighbour 0's parent for each linked vertex,
+        # except, at or above L, those the low-id map links under g0 (the
+        # map read once instead: L/8 bytes)
+        tag = "lo" if name == "cc_hook_low" else "hi"
+        cnt = L if tag == "lo" else n - L
+        d0_, up0_, m1_ = st[f"deg_gt0_{tag}"], st[f"up0_{tag}"], st[f"minima_gt1_{tag}"]
+        hop = d0_ - up0_ - (st["marked_hi"] if tag == "hi" else 0)
+        extra = L / 8.0 if tag == "hi" else 0.0
+        return (8.0 * (cnt + 1) + 4.0 * (d0_ + m1_) + 8.0 * cnt + 4.0 * hop
+                + 4.0 * (up0_ + m1_) + extra)
+    if name == "cc_low_bits" and L:
+        # each id below L: its entry, its root's entry (one gather), the
+        # map written
+        return 8.0 * L + L / 8.0
     if name == "cc_hook_sample":
         skip = plan.get("skip_linked", 0)
         m1 = st.get("minima_gt1", 0)
@@ -271,7 +321,8 @@ def solve_plan(lib, n: int, m: int) -> dict:
     """What the solve does on this graph (cc_hip_components_plan)."""
     keys = ("cache", "fused", "rem", "rem_splice", "halve", "hook_ilp", "hook_blocks_per_cu",
             "giant_early", "hook_mode", "nt", "init_ilp", "frontier_chunks", "csr_hook",
-            "skip_linked", "minima_entries", "compress_ilp", "tiles", "tile_shift", "tile_pairs")
+            "skip_linked", "minima_entries", "compress_ilp", "tiles", "tile_shift", "tile_pairs",
+            "low_shift")
     return {k: int(lib.cc_hip_components_plan_get(n, m, k.encode())) for k in keys}
 
 
@@ -710,7 +761,8 @@ def main() -> int:
         st = tile_stats(G, plan["tile_shift"], labels)
         st["m"] = G.m
     else:
-        st = frontier_stats(G, two_pass=not plan["fused"], skip=bool(plan["skip_linked"]))
+        st = frontier_stats(G, two_pass=not plan["fused"], skip=bool(plan["skip_linked"]),
+                            low=low_bound(G.n, plan["low_shift"]))
     st["components"] = int(np.count_nonzero(labels == np.arange(G.n, dtype=np.uint32)))
     log(f"{'tiles' if tiles_ran else 'frontier'}: {st} ({time.perf_counter() - t0:.1f}s)")
     roofline = None

@@ -28,7 +28,12 @@
 //      as its own (one hop): the trees then hang from the local minima
 //      (vertices below all their neighbours), which unite with both
 //      sampled neighbours, and those links join nearly every tree into the
-//      giant one while only the minima walk.
+//      giant one while only the minima walk.  On the largest of them the
+//      ids below n/16 are hooked first; a bitmap of which of them then hang
+//      under their most frequent root g0 (1 bit per id, small enough to
+//      stay in every XCD's L2) lets each other vertex whose neighbour 0 is
+//      marked link straight under g0 instead of gathering neighbour 0's
+//      parent from HBM.
 //   3. The most frequent root among 1024 sampled vertices is taken as the
 //      giant root g.
 //   4. Frontier sweep: every vertex whose parent is not g hooks with all its
@@ -120,6 +125,11 @@ struct Knobs {
   int tiles = -1;
   int tile_shift = 0;  // log2 of the vertices per tile (6..15; 0 = auto, 15)
   int tile_pairs = 0;  // pairs per tile segment of the cross list (0 = auto, one per vertex)
+  // low-first hook (the skipping one-hop hook from the CSR): the ids below
+  // n >> low_shift are hooked first, then a map of which of them hang
+  // under their giant root lets the rest link there without reading
+  // neighbour 0's parent (-1 = auto, 0 = off, 1..8)
+  int low_shift = -1;
 };
 // Process-wide settings (cc_hip_components_set / _tune); the entry point
 // without allocations always runs with the defaults.
@@ -458,6 +468,11 @@ __global__ __launch_bounds__(kBlock) void k_compress_ilp(uint32_t n, uint32_t *_
 // (they share that line) into one coalesced array per round, so the rounds
 // stream 4 bytes per vertex instead of gathering a line plus an offset.
 constexpr uint32_t kNoNeighbour = 0xFFFFFFFFu;
+// Low-first hook (Knobs::low_shift) over the ids below n >> 4 on the plan
+// that takes it (the skipping one-hop hook from the CSR).  Per solve:
+// Chung-Lu 65.6M 2.45 -> 2.27 ms (shifts 1/2/3/5: 2.37/2.30/2.30/2.30),
+// Chung-Lu 40M 1.58 -> 1.47-1.51, R-MAT 27 3.84 -> 3.78.
+constexpr int kLowShiftAuto = 4;
 
 // Nt: bit 1 stores the sample arrays non-temporally, bit 2 loads the
 // column ids non-temporally (both are streamed once; the parent array
@@ -744,20 +759,34 @@ __device__ __forceinline__ bool link_first(uint32_t v, uint32_t w0, uint32_t n,
 // minimum (neighbour 0 above it) or a vertex whose link failed unites with
 // both, and a local minimum also with entries 2 .. minima_entries-1 of its
 // row (the reasoning at k_hook_rem).
-template <int Mode, int ILP, bool Halve, int Skip1 = 0>
+//
+// The launch covers the vertices [v_lo, v_hi) (v_hi clipped to n).  With
+// Map, a low-id map (bits; the low-first hook, k_low_bits): a vertex
+// whose neighbour 0 lies in the map's range [0, nbits) with its bit set,
+// i.e. under the giant root g0 = g0p[0] sampled among the low ids, links
+// straight under g0 (g0 <= neighbour 0 < v, as a root is its tree's
+// minimum), reading the map, which stays in the L2, instead of neighbour
+// 0's parent, a line of the parent array.
+template <int Mode, int ILP, bool Halve, int Skip1 = 0, bool Map = false>
 __global__ __launch_bounds__(kBlock) void k_hook_pair_csr(const uint64_t *__restrict__ row_ptr,
                                                           const uint32_t *__restrict__ col,
                                                           uint32_t n, uint32_t *__restrict__ p,
-                                                          uint32_t minima_entries = 2) {
+                                                          uint32_t minima_entries = 2,
+                                                          uint32_t v_lo = 0, uint32_t v_hi = 0xFFFFFFFFu,
+                                                          const uint32_t *__restrict__ bits = nullptr,
+                                                          uint32_t nbits = 0,
+                                                          const uint32_t *__restrict__ g0p = nullptr) {
   const uint64_t S = (uint64_t)gridDim.x * kBlock;
-  for (uint64_t v0 = (uint64_t)blockIdx.x * kBlock + threadIdx.x; v0 < n; v0 += S * ILP) {
+  const uint64_t hi = v_hi < n ? v_hi : n;
+  const uint32_t g0 = Map ? g0p[0] : 0u;
+  for (uint64_t v0 = (uint64_t)v_lo + blockIdx.x * kBlock + threadIdx.x; v0 < hi; v0 += S * ILP) {
     uint32_t w0[ILP], w1[ILP];
     uint64_t rb[ILP], re[ILP];
 #pragma unroll
     for (int k = 0; k < ILP; ++k) {
       const uint64_t v = v0 + k * S;
-      rb[k] = v < n ? row_ptr[v] : 0;
-      re[k] = v < n ? row_ptr[v + 1] : 0;
+      rb[k] = v < hi ? row_ptr[v] : 0;
+      re[k] = v < hi ? row_ptr[v + 1] : 0;
       w0[k] = rb[k] < re[k] ? col[rb[k]] : kNoNeighbour;
       // read with neighbour 0 even where skipping will not use it: a read
       // after the link's CAS, for the vertices that do, measured slower
@@ -770,12 +799,19 @@ __global__ __launch_bounds__(kBlock) void k_hook_pair_csr(const uint64_t *__rest
 #pragma unroll
     for (int k = 0; k < ILP; ++k) {
       const uint64_t v = v0 + k * S;
-      const bool live = v < n;
+      const bool live = v < hi;
       if (Skip1 == 2 && live && w0[k] < (uint32_t)v) {
         // one hop: link v under neighbour 0's parent (an ancestor of
-        // neighbour 0, below v) by the CAS of its own entry
-        const uint32_t q = p[w0[k]];
-        act[ILP + k] = atomicCAS(&p[v], (uint32_t)v, q < w0[k] ? q : w0[k]) != (uint32_t)v;
+        // neighbour 0, below v) by the CAS of its own entry; under g0 where
+        // the low-id map says neighbour 0 hangs there
+        uint32_t q;
+        if (Map && w0[k] < nbits && ((bits[w0[k] >> 5] >> (w0[k] & 31)) & 1u)) {
+          q = g0;
+        } else {
+          q = p[w0[k]];
+          q = q < w0[k] ? q : w0[k];
+        }
+        act[ILP + k] = atomicCAS(&p[v], (uint32_t)v, q) != (uint32_t)v;
       } else {
         act[ILP + k] = live && link_first((uint32_t)v, w0[k], n, p);
       }
@@ -801,7 +837,7 @@ __global__ __launch_bounds__(kBlock) void k_hook_pair_csr(const uint64_t *__rest
 #pragma unroll
       for (int k = 0; k < ILP; ++k) {
         const uint64_t v = v0 + k * S;
-        if (v >= n || w0[k] >= n || w0[k] <= (uint32_t)v) continue;
+        if (v >= hi || w0[k] >= n || w0[k] <= (uint32_t)v) continue;
         const uint64_t b = rb[k] + 2;
         const uint64_t stop = b + (minima_entries - 2) < re[k] ? b + (minima_entries - 2) : re[k];
         for (uint64_t i = b; i < stop; ++i) {
@@ -822,6 +858,35 @@ __global__ __launch_bounds__(kBlock) void k_hook_rem_csr(const uint64_t *__restr
     csr_first_two(row_ptr, col, v, n, w0, w1);
     if (link_first((uint32_t)v, w0, n, p)) rem_unite<Splice>((uint32_t)v, w0, p);
     if (w1 < n && w1 != (uint32_t)v) rem_unite<Splice>((uint32_t)v, w1, p);
+  }
+}
+
+// The low-first hook's map over the low ids [0, L): bit u is set iff u's
+// root is g0 = g0p[0], the giant root sampled among [0, L) after their
+// hook.  Each entry of [0, L) is also pointed at its root, so later walks
+// through the low ids take one hop; plain stores suffice, since nothing
+// else changes the forest during this pass and a concurrent walker reads
+// either value, both ancestors.  Every wave covers 64 consecutive ids
+// (256-thread workgroups, grid stride a multiple of 256), so one ballot
+// is one 64-bit word of the map.
+__global__ __launch_bounds__(kBlock) void k_low_bits(uint32_t L, uint32_t *__restrict__ p,
+                                                     const uint32_t *__restrict__ g0p,
+                                                     uint64_t *__restrict__ bits) {
+  const uint32_t g0 = g0p[0];
+  const uint32_t whole = (L + 63u) & ~63u;
+  for (uint32_t u = blockIdx.x * kBlock + threadIdx.x; u < whole; u += gridDim.x * kBlock) {
+    bool in = false;
+    if (u < L) {
+      const uint32_t pu = p[u];
+      uint32_t r = pu;
+      if (pu != u) {
+        r = find_root(pu, p);
+        if (r != pu) p[u] = r;
+      }
+      in = r == g0;
+    }
+    const uint64_t b = __ballot(in);
+    if ((threadIdx.x & (kWave - 1)) == 0) bits[u >> 6] = b;
   }
 }
 
@@ -1434,6 +1499,7 @@ struct Plan {
   int tiles;           // vertex tiles: 0 no, 1 always, 2 if the probe finds local ids
   uint32_t tile_shift; // log2 of the vertices per tile
   uint32_t tile_cap;   // pairs per tile segment
+  uint32_t low_shift;  // low-first hook over ids below n >> low_shift (0: one pass)
 };
 
 inline Plan plan_for(uint32_t n, uint64_t m, const Knobs &K = g_knobs) {
@@ -1513,7 +1579,18 @@ inline Plan plan_for(uint32_t n, uint64_t m, const Knobs &K = g_knobs) {
   pl.tiles = K.tiles > 0 ? 1 : (K.tiles == 0 ? 0 : (tile_auto ? 2 : 0));
   pl.tile_shift = K.tile_shift ? (uint32_t)K.tile_shift : kTileDefaultShift;
   pl.tile_cap = K.tile_pairs ? (uint32_t)K.tile_pairs : (1u << pl.tile_shift);
+  const bool low_ok = pl.csr && !pl.rem && pl.halve && pl.skip1 == 2 && pl.hook_ilp == 1;
+  pl.low_shift = low_ok ? (uint32_t)(K.low_shift < 0 ? kLowShiftAuto : K.low_shift) : 0u;
   return pl;
+}
+
+// The low-first hook's bound: ids below it are hooked first and mapped.  A
+// multiple of 64 (whole map words), at least 64, below n.
+inline uint32_t low_bound(uint32_t n, const Plan &pl) {
+  if (!pl.low_shift) return 0;
+  uint32_t L = ((n >> pl.low_shift) + 63u) & ~63u;
+  if (L < 64u) L = 64u;
+  return L >= n ? 0u : L;
 }
 
 // Words of the tile plan's workspace: per-tile pair counts and the local
@@ -1558,6 +1635,7 @@ extern "C" int cc_hip_components_set(const char *key, int value) {
   else if (k == "tile_shift" && (value == 0 || (value >= (int)kTileMinShift && value <= (int)kTileMaxShift)))
     g_knobs.tile_shift = value;
   else if (k == "tile_pairs" && value >= 0 && value <= (int)kTileMaxPairs) g_knobs.tile_pairs = value;
+  else if (k == "low_shift" && value >= -1 && value <= 8) g_knobs.low_shift = value;
   else return ccx::set_error(4, "cc_hip_components_set: bad key or value (%s=%d)", key, value);
   return 0;
 }
@@ -1587,6 +1665,7 @@ extern "C" int cc_hip_components_get(const char *key) {
   if (k == "tiles") return g_knobs.tiles;
   if (k == "tile_shift") return g_knobs.tile_shift;
   if (k == "tile_pairs") return g_knobs.tile_pairs;
+  if (k == "low_shift") return g_knobs.low_shift;
   return -1;
 }
 
@@ -1638,6 +1717,7 @@ extern "C" int cc_hip_components_plan_get(uint32_t n, uint64_t m, const char *ke
   if (k == "tiles") return pl.tiles;
   if (k == "tile_shift") return (int)pl.tile_shift;
   if (k == "tile_pairs") return (int)pl.tile_cap;
+  if (k == "low_shift") return (int)pl.low_shift;
   return -1;
 }
 
@@ -1790,6 +1870,19 @@ int solve_with(const Knobs &K, const ccx::Context *c, const uint64_t *d_row_ptr,
     else                                                                                           \
       CCX_LAUNCH("cc_hook_sample", s, k_hook_pair_csr<4, 1, H, K1><<<nb, kBlock, 0, s>>>(d_row_ptr, d_col_idx, n, d_labels, me)); \
   } while (0)
+    else if (plan.halve && plan.skip1 == 2 && low_bound(n, plan)) {
+      // low-first: the ids below L, their giant root g0 (words[4]), the map
+      // of which of them hang under g0, then the rest of the ids
+      const uint32_t L = low_bound(n, plan);
+      if (!smp) return ccx::set_error(4, "solve: no low-id map workspace");
+      const unsigned nbl = blocks_for(L, c->cu_count, plan.hook_bpc);
+      const unsigned nbh = blocks_for(n - L, c->cu_count, plan.hook_bpc);
+      const unsigned nsl = blocks_for(L, c->cu_count, K.stream_blocks_per_cu);
+      CCX_LAUNCH("cc_hook_low", s, k_hook_pair_csr<4, 1, true, 2><<<nbl, kBlock, 0, s>>>(d_row_ptr, d_col_idx, n, d_labels, me, 0u, L));
+      CCX_LAUNCH("cc_sample_low", s, k_sample_giant<<<1, kGiantSamples, 0, s>>>(L, d_labels, words + 4));
+      CCX_LAUNCH("cc_low_bits", s, k_low_bits<<<nsl, kBlock, 0, s>>>(L, d_labels, words + 4, (uint64_t *)smp));
+      CCX_LAUNCH("cc_hook_sample", s, k_hook_pair_csr<4, 1, true, 2, true><<<nbh, kBlock, 0, s>>>(d_row_ptr, d_col_idx, n, d_labels, me, L, n, smp, L, words + 4));
+    }
     else if (plan.halve && plan.skip1 == 2) CCX_HOOK_CSR(true, 2);
     else if (plan.halve && plan.skip1) CCX_HOOK_CSR(true, 1);
     else if (plan.halve) CCX_HOOK_CSR(true, 0);
@@ -1951,6 +2044,7 @@ namespace {
 uint64_t work_words(uint32_t n, uint64_t m, const Knobs &K, const Plan &pl) {
   if (m == 0) return 0;
   if (pl.tiles == 1) return tile_words(n, pl);
+  if (pl.csr && pl.low_shift) return 2ull * (((uint64_t)low_bound(n, pl) + 63) / 64);
   const uint64_t cache = (pl.cache && !pl.csr) ? (uint64_t)K.sampled_neighbours * n : 0;
   const uint64_t tiles = pl.tiles ? tile_words(n, pl) : 0;
   return cache > tiles ? cache : tiles;

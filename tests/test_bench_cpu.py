@@ -97,6 +97,63 @@ def test_frontier_stats_match_a_union_find_restatement():
             assert 0.5 < got["giant_fraction"] < 1.0
 
 
+def test_low_split_counts_match_a_loop():
+    """The low-first hook's byte-model inputs: the counts split at the bound
+    L, and the linked vertices at or above L whose neighbour 0 lies below L
+    in the giant tree of the sampled forest (the skipping hook's edge set),
+    by a loop over the vertices and a union-find."""
+    for seed, shift in ((5, 2), (6, 4)):
+        G = _two_part_graph(seed)
+        n = G.n
+        L = bench.low_bound(n, shift)
+        assert L % 64 == 0 and 0 < L < n
+        got = bench.frontier_stats(G, two_pass=False, skip=True, low=L)
+        rp = G.row_ptr.astype(np.int64)
+        ci = G.col_idx.astype(np.int64)
+        parent = list(range(n))
+
+        def find(x):
+            while parent[x] != x:
+                x = parent[x]
+            return x
+
+        want = {f"{k}_{t}": 0 for k in ("deg_gt0", "up0", "minima_gt1") for t in ("lo", "hi")}
+        for v in range(n):
+            deg = rp[v + 1] - rp[v]
+            tag = "lo" if v < L else "hi"
+            minimum = deg > 0 and ci[rp[v]] > v
+            want[f"deg_gt0_{tag}"] += deg > 0
+            want[f"up0_{tag}"] += minimum
+            want[f"minima_gt1_{tag}"] += minimum and deg > 1
+            for r in range(min(2 if minimum else 1, deg)):
+                a, b = find(v), find(int(ci[rp[v] + r]))
+                if a != b:
+                    parent[max(a, b)] = min(a, b)
+        roots = np.array([find(v) for v in range(n)])
+        vals, counts = np.unique(roots, return_counts=True)
+        giant = vals[np.argmax(counts)]
+        marked = 0
+        for v in range(L, n):
+            if rp[v + 1] > rp[v]:
+                w0 = int(ci[rp[v]])
+                marked += w0 < v and w0 < L and roots[w0] == giant
+        want["marked_hi"] = marked
+        for k, v in want.items():
+            assert got[k] == v, (seed, k, got[k], v)
+        assert got["low_bound"] == L and marked > 0
+
+
+def test_low_bound_rule():
+    # ids below n >> shift rounded up to whole 64-id map words, at least 64,
+    # none when that reaches n
+    assert bench.low_bound(65601024, 4) == 4100096
+    assert bench.low_bound(210, 8) == 64
+    assert bench.low_bound(210, 1) == 128
+    assert bench.low_bound(64, 1) == 0
+    assert bench.low_bound(50, 8) == 0
+    assert bench.low_bound(1 << 20, 0) == 0
+
+
 def test_tile_stats_match_a_union_find_restatement():
     """The vertex-tile byte model's inputs (pairs to smaller tiles, local
     roots, linked local roots, vertices whose local root is not final) on

@@ -63,6 +63,43 @@ def test_edge_cases_vertex_tiles(gpu, knobs):
     """The hand-built CSRs above with the vertex-tile plan forced: one tile
     or several of 64 ids, with pair segments of one entry."""
     lib = nat.gpu_lib()
+    try:
+        for k, v in knobs.items():
+            nat.check(lib.cc_hip_components_set(k.encode(), v), k)
+        for G, want in edge_case_graphs():
+            assert dropin(G).tolist() == want
+    finally:
+        for k, v in dict(tiles=-1, tile_shift=0, tile_pairs=0, symmetric=1).items():
+            nat.check(lib.cc_hip_components_set(k.encode(), v), k)
+
+
+@pytest.mark.parametrize("shift", [1, 2, 8])
+def test_edge_cases_low_first_hook(gpu, shift):
+    """The same CSRs through the low-first hook (the one-hop skipping hook
+    from the CSR with the low-id map): graphs below 64 vertices run one pass,
+    the 210-vertex one hooks ids below 64 or 128 first, whose map then
+    serves the rest (entries past n, self-loops and unsorted rows included),
+    and a 70-vertex path whose ids above 64 all link through the map."""
+    lib = nat.gpu_lib()
+    knobs = dict(csr_hook=1, rem=0, halve=1, hook_ilp=1, skip_linked=2, low_shift=shift)
+    n = 70
+    path = [[] for _ in range(n)]
+    for a in range(n - 1):
+        path[a].append(a + 1)
+        path[a + 1].append(a)
+    cases = edge_case_graphs() + [(csr_raw(n, path), [0] * n)]
+    try:
+        for k, v in knobs.items():
+            nat.check(lib.cc_hip_components_set(k.encode(), v), k)
+        for G, want in cases:
+            assert dropin(G).tolist() == want
+    finally:
+        for k, v in dict(csr_hook=-1, rem=-1, halve=-1, hook_ilp=0, skip_linked=-1,
+                         low_shift=-1).items():
+            nat.check(lib.cc_hip_components_set(k.encode(), v), k)
+
+
+def edge_case_graphs():
     cases = [
         (csr_raw(1, [[]]), [0]),
         (csr_raw(5, [[], [], [], [], []]), [0, 1, 2, 3, 4]),
@@ -80,14 +117,7 @@ def test_edge_cases_vertex_tiles(gpu, knobs):
     adj[130] = adj[130][::-1]
     want = oracle.cc_min_label(csr_raw(n, adj).row_ptr, csr_raw(n, adj).col_idx).tolist()
     cases.append((csr_raw(n, adj), want))
-    try:
-        for k, v in knobs.items():
-            nat.check(lib.cc_hip_components_set(k.encode(), v), k)
-        for G, want in cases:
-            assert dropin(G).tolist() == want
-    finally:
-        for k, v in dict(tiles=-1, tile_shift=0, tile_pairs=0, symmetric=1).items():
-            nat.check(lib.cc_hip_components_set(k.encode(), v), k)
+    return cases
 
 
 def test_empty_graph(gpu):
@@ -324,6 +354,14 @@ PATHS = [  # every code path of the solve: (knobs) -> same labels
     dict(csr_hook=1, rem=0, halve=1, hook_ilp=4, skip_linked=2, blocks_per_cu=1),
     dict(csr_hook=1, rem=0, halve=1, hook_ilp=1, skip_linked=2, minima_entries=3),
     dict(csr_hook=1, rem=0, halve=1, hook_ilp=2, skip_linked=1, minima_entries=8),
+    # the low-first hook (the one-hop skipping hook from the CSR, one root
+    # per thread): off, half of the ids first, the smallest bound (64 ids,
+    # grid-stride loops of many steps), with more entries for local minima
+    dict(csr_hook=1, rem=0, halve=1, hook_ilp=1, skip_linked=2, low_shift=0),
+    dict(csr_hook=1, rem=0, halve=1, hook_ilp=1, skip_linked=2, low_shift=1),
+    dict(csr_hook=1, rem=0, halve=1, hook_ilp=1, skip_linked=2, low_shift=8, blocks_per_cu=1),
+    dict(csr_hook=1, rem=0, halve=1, hook_ilp=1, skip_linked=2, low_shift=2, minima_entries=3,
+         stream_blocks_per_cu=1),
     dict(rem=1, rem_splice=0, skip_linked=1), dict(rem=1, rem_splice=1, skip_linked=2),
     dict(rem=1, rem_splice=2, skip_linked=2, nt=4), dict(rem=1, rem_splice=1, skip_linked=1, nt=4),
     dict(rem=1, rem_splice=0, skip_linked=2, minima_entries=4),
@@ -342,7 +380,7 @@ PATHS = [  # every code path of the solve: (knobs) -> same labels
 DEFAULTS = dict(sample_cache=1, fused_pair=-1, hook_ilp=0, hook_mode=4, sampled_neighbours=2,
                 stream_ilp=1, blocks_per_cu=0, giant_early=1, halve=-1, rem=-1, rem_splice=-1,
                 nt=-1, init_ilp=0, frontier_chunks=0, csr_hook=-1, skip_linked=-1, minima_entries=0,
-                tiles=-1, tile_shift=0, tile_pairs=0, stream_blocks_per_cu=8)
+                tiles=-1, tile_shift=0, tile_pairs=0, stream_blocks_per_cu=8, low_shift=-1)
 
 
 @pytest.mark.parametrize("knobs", PATHS, ids=lambda k: ",".join(f"{a}={b}" for a, b in k.items()))
@@ -371,6 +409,7 @@ def test_every_solve_path(gpu, knobs):
 
 @pytest.mark.parametrize("knobs", [
     dict(), dict(csr_hook=1, rem=0, halve=1, hook_ilp=1, skip_linked=2),
+    dict(csr_hook=1, rem=0, halve=1, hook_ilp=1, skip_linked=2, low_shift=1),
     dict(csr_hook=1, rem=0, halve=1, hook_ilp=2, skip_linked=0),
     dict(rem=1, rem_splice=1, skip_linked=2), dict(rem=1, rem_splice=2, skip_linked=0),
     dict(rem=0, halve=1, hook_ilp=2, skip_linked=2),
