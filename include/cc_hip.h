@@ -209,6 +209,16 @@ int cc_hip_components_tune(int sampled_neighbours, int blocks_per_cu, int hook_m
  *                             (default) = one per vertex of the tile; the
  *                             plan's temporary memory is about 8 bytes per
  *                             vertex at the default
+ *   "low_shift"               the low-first hook, on the one-hop skipping
+ *                             hook from the CSR with one root per thread:
+ *                             the ids below L = n >> low_shift (rounded up
+ *                             to 64) are hooked first, a bitmap marks which
+ *                             of them hang under their most frequent root
+ *                             g0, and each other vertex whose neighbour 0
+ *                             is marked links under g0 without gathering
+ *                             neighbour 0's parent: 1..8, 0 off (one pass),
+ *                             or -1 (default) = 4; temporary memory L/8
+ *                             bytes
  * Returns nonzero for an unknown key or out-of-range value; _get returns -1
  * for an unknown key. */
 int cc_hip_components_set(const char *key, int value);
@@ -224,7 +234,8 @@ int cc_hip_components_plan(uint32_t n, uint64_t m, int *cache, int *fused, int *
  * this graph), "hook_mode", "nt", "init_ilp", "frontier_chunks", "csr_hook",
  * "skip_linked", "minima_entries", "compress_ilp" (vertices per thread of
  * the pointer-jumping passes), "tiles" (0 no, 1 always, 2 the probe
- * decides), "tile_shift", "tile_pairs".  Returns -1 for an unknown key. */
+ * decides), "tile_shift", "tile_pairs", "low_shift" (0: no low-first
+ * hook).  Returns -1 for an unknown key. */
 int cc_hip_components_plan_get(uint32_t n, uint64_t m, const char *key);
 /* Whether the last solve on `stream` (NULL = the library stream) ran its
  * final pointer-jumping pass: *ran = 1 if its frontier sweep linked a root,
