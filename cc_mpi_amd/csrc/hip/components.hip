@@ -130,6 +130,10 @@ struct Knobs {
   // under their giant root lets the rest link there without reading
   // neighbour 0's parent (-1 = auto, 0 = off, 1..8)
   int low_shift = -1;
+  // the low-first hook's passes queue the unions a vertex still owes after
+  // its own-entry link per wave and run them 64 at a time (k_hook_queued):
+  // 1, 0, or -1 = auto
+  int split_walks = -1;
 };
 // Process-wide settings (cc_hip_components_set / _tune); the entry point
 // without allocations always runs with the defaults.
@@ -473,6 +477,10 @@ constexpr uint32_t kNoNeighbour = 0xFFFFFFFFu;
 // Chung-Lu 65.6M 2.45 -> 2.27 ms (shifts 1/2/3/5: 2.37/2.30/2.30/2.30),
 // Chung-Lu 40M 1.58 -> 1.47-1.51, R-MAT 27 3.84 -> 3.78.
 constexpr int kLowShiftAuto = 4;
+// Queued owed unions (Knobs::split_walks): R-MAT 27 3.80 -> 3.46 ms per
+// solve, but Chung-Lu 65.6M 2.27 -> 2.32 and Chung-Lu 40M 1.47 -> 1.50, so
+// off by default (a knob for graphs with many local minima).
+constexpr int kSplitWalksAuto = 0;
 
 // Nt: bit 1 stores the sample arrays non-temporally, bit 2 loads the
 // column ids non-temporally (both are streamed once; the parent array
@@ -800,6 +808,7 @@ __global__ __launch_bounds__(kBlock) void k_hook_pair_csr(const uint64_t *__rest
     for (int k = 0; k < ILP; ++k) {
       const uint64_t v = v0 + k * S;
       const bool live = v < hi;
+      bool via_g0 = false;
       if (Skip1 == 2 && live && w0[k] < (uint32_t)v) {
         // one hop: link v under neighbour 0's parent (an ancestor of
         // neighbour 0, below v) by the CAS of its own entry; under g0 where
@@ -807,6 +816,7 @@ __global__ __launch_bounds__(kBlock) void k_hook_pair_csr(const uint64_t *__rest
         uint32_t q;
         if (Map && w0[k] < nbits && ((bits[w0[k] >> 5] >> (w0[k] & 31)) & 1u)) {
           q = g0;
+          via_g0 = true;
         } else {
           q = p[w0[k]];
           q = q < w0[k] ? q : w0[k];
@@ -815,10 +825,16 @@ __global__ __launch_bounds__(kBlock) void k_hook_pair_csr(const uint64_t *__rest
       } else {
         act[ILP + k] = live && link_first((uint32_t)v, w0[k], n, p);
       }
+      // A link that failed (another union took v's entry first) unites v
+      // with both neighbours, walking from g0 where the map marks neighbour
+      // 0 (the same tree, a short walk).  Uniting such a vertex with
+      // neighbour 0 alone (which the sweep from entry 1 would allow) left
+      // more trees apart: Chung-Lu 65.6M 2.27 -> 2.31 ms per solve, R-MAT
+      // 27 3.80 -> 4.17 ms, its frontier sweep 582 -> 1,076 us.
       act[2 * ILP + k] = live && w1[k] < n && w1[k] != (uint32_t)v && (!Skip1 || act[ILP + k]);
       act[k] = act[ILP + k] || act[2 * ILP + k];
       x[k] = (uint32_t)v;
-      x[ILP + k] = w0[k];
+      x[ILP + k] = via_g0 ? g0 : w0[k];
       x[2 * ILP + k] = w1[k];
     }
     if (Halve) roots_lockstep_halve<3 * ILP>(x, act, p);
@@ -847,6 +863,109 @@ __global__ __launch_bounds__(kBlock) void k_hook_pair_csr(const uint64_t *__rest
       }
     }
   }
+}
+
+// The unions a vertex of the one-hop skipping hook still owes after its
+// own-entry link: with a (neighbour 0, or g0 where the map marks it) and
+// with b (neighbour 1, or kNoNeighbour), the three roots walked in lockstep
+// with halving, exactly as k_hook_pair_csr does for such a vertex.
+__device__ __forceinline__ void unite_owed(uint32_t v, uint32_t a, uint32_t b, uint32_t n,
+                                           uint32_t *__restrict__ p) {
+  uint32_t x[3] = {v, a, b};
+  const bool act[3] = {true, true, b < n && b != v};
+  roots_lockstep_halve<3>(x, act, p);
+  uint32_t rv = x[0];
+  if (rv != x[1]) {
+    hook_roots<4>(rv, x[1], p);
+    rv = rv < x[1] ? rv : x[1];
+  }
+  if (act[2] && rv != x[2]) hook_roots<4>(rv, x[2], p);
+}
+
+// Queued form of k_hook_pair_csr<4, 1, true, 2, Map> (minima_entries 2).
+// Each lane does the own-entry link (row offsets, the row's first two
+// entries, the map bit or neighbour 0's parent, the CAS); a vertex that
+// still owes unions (a local minimum, or a link that failed) goes to its
+// wave's queue in LDS, and whenever 64 are queued the wave's lanes run 64
+// of them at once.  In one pass, a wave waits for its slowest lane at
+// every step, and nearly every step holds a walker.  The queue needs no
+// atomics: waves cover 64 consecutive ids, so the loop, the ballots and
+// the queue length are wave-uniform, and a wave's LDS operations complete
+// in order (the fences below make the compiler wait for them).
+constexpr int kQueue = 2 * kWave;
+template <bool Map>
+__global__ __launch_bounds__(kBlock) void k_hook_queued(const uint64_t *__restrict__ row_ptr,
+                                                        const uint32_t *__restrict__ col,
+                                                        uint32_t n, uint32_t *__restrict__ p,
+                                                        uint32_t v_lo, uint32_t v_hi,
+                                                        const uint32_t *__restrict__ bits,
+                                                        uint32_t nbits,
+                                                        const uint32_t *__restrict__ g0p) {
+  __shared__ uint32_t qv[kBlock / kWave][kQueue];
+  __shared__ uint32_t qa[kBlock / kWave][kQueue];
+  __shared__ uint32_t qb[kBlock / kWave][kQueue];
+  const uint64_t S = (uint64_t)gridDim.x * kBlock;
+  const uint64_t hi = v_hi < n ? v_hi : n;
+  const uint32_t g0 = Map ? g0p[0] : 0u;
+  const uint32_t lane = threadIdx.x & (kWave - 1);
+  const uint32_t w = threadIdx.x / kWave;
+  uint32_t queued = 0;  // wave-uniform, below kWave between steps
+  for (uint64_t base = (uint64_t)v_lo + blockIdx.x * kBlock + (threadIdx.x - lane); base < hi;
+       base += S) {
+    const uint64_t v = base + lane;
+    const bool live = v < hi;
+    const uint64_t rb = live ? row_ptr[v] : 0;
+    const uint64_t re = live ? row_ptr[v + 1] : 0;
+    const uint32_t w0 = rb < re ? col[rb] : kNoNeighbour;
+    const uint32_t w1 = rb + 1 < re ? col[rb + 1] : kNoNeighbour;
+    bool owe = false;
+    uint32_t a = w0;
+    if (live && w0 < n && w0 != (uint32_t)v) {
+      if (w0 < (uint32_t)v) {
+        uint32_t q;
+        bool marked = false;
+        if (Map && w0 < nbits && ((bits[w0 >> 5] >> (w0 & 31)) & 1u)) {
+          q = g0;
+          marked = true;
+        } else {
+          q = p[w0];
+          q = q < w0 ? q : w0;
+        }
+        if (atomicCAS(&p[v], (uint32_t)v, q) != (uint32_t)v) {
+          owe = true;
+          if (marked) a = g0;
+        }
+      } else {
+        owe = true;  // a local minimum
+      }
+    }
+    const uint64_t mask = __ballot(owe);
+    if (mask) {
+      if (owe) {
+        const uint32_t at = queued + (uint32_t)__popcll(mask & ((1ull << lane) - 1ull));
+        qv[w][at] = (uint32_t)v;
+        qa[w][at] = a;
+        qb[w][at] = w1;
+      }
+      queued += (uint32_t)__popcll(mask);
+      __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+      __builtin_amdgcn_wave_barrier();
+      if (queued >= kWave) {
+        // 64 owed unions, one per lane; the rest moves to the front
+        const uint32_t tv = qv[w][lane], ta = qa[w][lane], tb = qb[w][lane];
+        queued -= kWave;
+        if (lane < queued) {
+          qv[w][lane] = qv[w][kWave + lane];
+          qa[w][lane] = qa[w][kWave + lane];
+          qb[w][lane] = qb[w][kWave + lane];
+        }
+        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        unite_owed(tv, ta, tb, n, p);
+      }
+    }
+  }
+  if (lane < queued) unite_owed(qv[w][lane], qa[w][lane], qb[w][lane], n, p);
 }
 
 template <int Splice>
@@ -1500,6 +1619,7 @@ struct Plan {
   uint32_t tile_shift; // log2 of the vertices per tile
   uint32_t tile_cap;   // pairs per tile segment
   uint32_t low_shift;  // low-first hook over ids below n >> low_shift (0: one pass)
+  int split_walks;     // its passes split into links and a walk list
 };
 
 inline Plan plan_for(uint32_t n, uint64_t m, const Knobs &K = g_knobs) {
@@ -1581,11 +1701,18 @@ inline Plan plan_for(uint32_t n, uint64_t m, const Knobs &K = g_knobs) {
   pl.tile_cap = K.tile_pairs ? (uint32_t)K.tile_pairs : (1u << pl.tile_shift);
   const bool low_ok = pl.csr && !pl.rem && pl.halve && pl.skip1 == 2 && pl.hook_ilp == 1;
   pl.low_shift = low_ok ? (uint32_t)(K.low_shift < 0 ? kLowShiftAuto : K.low_shift) : 0u;
+  pl.split_walks = pl.low_shift && pl.minima_entries == 2
+                       ? (K.split_walks < 0 ? kSplitWalksAuto : K.split_walks) : 0;
   return pl;
 }
 
 // The low-first hook's bound: ids below it are hooked first and mapped.  A
 // multiple of 64 (whole map words), at least 64, below n.
+inline uint32_t low_bound(uint32_t n, const Plan &pl);
+// Words of the low-id map, rounded up to 256 bytes.
+inline uint64_t low_map_words(uint32_t n, const Plan &pl) {
+  return ((2ull * (((uint64_t)low_bound(n, pl) + 63) / 64)) + 63) & ~63ull;
+}
 inline uint32_t low_bound(uint32_t n, const Plan &pl) {
   if (!pl.low_shift) return 0;
   uint32_t L = ((n >> pl.low_shift) + 63u) & ~63u;
@@ -1636,6 +1763,7 @@ extern "C" int cc_hip_components_set(const char *key, int value) {
     g_knobs.tile_shift = value;
   else if (k == "tile_pairs" && value >= 0 && value <= (int)kTileMaxPairs) g_knobs.tile_pairs = value;
   else if (k == "low_shift" && value >= -1 && value <= 8) g_knobs.low_shift = value;
+  else if (k == "split_walks" && value >= -1 && value <= 1) g_knobs.split_walks = value;
   else return ccx::set_error(4, "cc_hip_components_set: bad key or value (%s=%d)", key, value);
   return 0;
 }
@@ -1666,6 +1794,7 @@ extern "C" int cc_hip_components_get(const char *key) {
   if (k == "tile_shift") return g_knobs.tile_shift;
   if (k == "tile_pairs") return g_knobs.tile_pairs;
   if (k == "low_shift") return g_knobs.low_shift;
+  if (k == "split_walks") return g_knobs.split_walks;
   return -1;
 }
 
@@ -1718,6 +1847,7 @@ extern "C" int cc_hip_components_plan_get(uint32_t n, uint64_t m, const char *ke
   if (k == "tile_shift") return (int)pl.tile_shift;
   if (k == "tile_pairs") return (int)pl.tile_cap;
   if (k == "low_shift") return (int)pl.low_shift;
+  if (k == "split_walks") return pl.split_walks;
   return -1;
 }
 
@@ -1878,10 +2008,18 @@ int solve_with(const Knobs &K, const ccx::Context *c, const uint64_t *d_row_ptr,
       const unsigned nbl = blocks_for(L, c->cu_count, plan.hook_bpc);
       const unsigned nbh = blocks_for(n - L, c->cu_count, plan.hook_bpc);
       const unsigned nsl = blocks_for(L, c->cu_count, K.stream_blocks_per_cu);
+      if (plan.split_walks) {
+        // the owed unions queued per wave and run 64 at a time
+        CCX_LAUNCH("cc_hook_low", s, k_hook_queued<false><<<nbl, kBlock, 0, s>>>(d_row_ptr, d_col_idx, n, d_labels, 0u, L, nullptr, 0u, nullptr));
+        CCX_LAUNCH("cc_sample_low", s, k_sample_giant<<<1, kGiantSamples, 0, s>>>(L, d_labels, words + 4));
+        CCX_LAUNCH("cc_low_bits", s, k_low_bits<<<nsl, kBlock, 0, s>>>(L, d_labels, words + 4, (uint64_t *)smp));
+        CCX_LAUNCH("cc_hook_sample", s, k_hook_queued<true><<<nbh, kBlock, 0, s>>>(d_row_ptr, d_col_idx, n, d_labels, L, n, smp, L, words + 4));
+      } else {
       CCX_LAUNCH("cc_hook_low", s, k_hook_pair_csr<4, 1, true, 2><<<nbl, kBlock, 0, s>>>(d_row_ptr, d_col_idx, n, d_labels, me, 0u, L));
       CCX_LAUNCH("cc_sample_low", s, k_sample_giant<<<1, kGiantSamples, 0, s>>>(L, d_labels, words + 4));
       CCX_LAUNCH("cc_low_bits", s, k_low_bits<<<nsl, kBlock, 0, s>>>(L, d_labels, words + 4, (uint64_t *)smp));
       CCX_LAUNCH("cc_hook_sample", s, k_hook_pair_csr<4, 1, true, 2, true><<<nbh, kBlock, 0, s>>>(d_row_ptr, d_col_idx, n, d_labels, me, L, n, smp, L, words + 4));
+      }
     }
     else if (plan.halve && plan.skip1 == 2) CCX_HOOK_CSR(true, 2);
     else if (plan.halve && plan.skip1) CCX_HOOK_CSR(true, 1);
@@ -2044,7 +2182,10 @@ namespace {
 uint64_t work_words(uint32_t n, uint64_t m, const Knobs &K, const Plan &pl) {
   if (m == 0) return 0;
   if (pl.tiles == 1) return tile_words(n, pl);
-  if (pl.csr && pl.low_shift) return 2ull * (((uint64_t)low_bound(n, pl) + 63) / 64);
+  if (pl.csr && pl.low_shift) {
+    const uint64_t map = low_map_words(n, pl);
+    return map;
+  }
   const uint64_t cache = (pl.cache && !pl.csr) ? (uint64_t)K.sampled_neighbours * n : 0;
   const uint64_t tiles = pl.tiles ? tile_words(n, pl) : 0;
   return cache > tiles ? cache : tiles;

@@ -219,6 +219,13 @@ int cc_hip_components_tune(int sampled_neighbours, int blocks_per_cu, int hook_m
  *                             neighbour 0's parent: 1..8, 0 off (one pass),
  *                             or -1 (default) = 4; temporary memory L/8
  *                             bytes
+ *   "split_walks"             with the low-first hook: each wave queues, in
+ *                             LDS, the vertices that still owe unions after
+ *                             their own-entry link (local minima, failed
+ *                             links) and runs them 64 at a time, so no wave
+ *                             waits on one walker per step: 1, 0, or -1
+ *                             (default) = 0 (faster on R-MAT 27, slower on
+ *                             the Chung-Lu graphs)
  * Returns nonzero for an unknown key or out-of-range value; _get returns -1
  * for an unknown key. */
 int cc_hip_components_set(const char *key, int value);
@@ -235,7 +242,7 @@ int cc_hip_components_plan(uint32_t n, uint64_t m, int *cache, int *fused, int *
  * "skip_linked", "minima_entries", "compress_ilp" (vertices per thread of
  * the pointer-jumping passes), "tiles" (0 no, 1 always, 2 the probe
  * decides), "tile_shift", "tile_pairs", "low_shift" (0: no low-first
- * hook).  Returns -1 for an unknown key. */
+ * hook), "split_walks".  Returns -1 for an unknown key. */
 int cc_hip_components_plan_get(uint32_t n, uint64_t m, const char *key);
 /* Whether the last solve on `stream` (NULL = the library stream) ran its
  * final pointer-jumping pass: *ran = 1 if its frontier sweep linked a root,

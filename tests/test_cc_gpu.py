@@ -73,15 +73,16 @@ def test_edge_cases_vertex_tiles(gpu, knobs):
             nat.check(lib.cc_hip_components_set(k.encode(), v), k)
 
 
-@pytest.mark.parametrize("shift", [1, 2, 8])
-def test_edge_cases_low_first_hook(gpu, shift):
+@pytest.mark.parametrize("shift,split", [(1, 0), (2, 0), (8, 0), (1, 1), (8, 1)])
+def test_edge_cases_low_first_hook(gpu, shift, split):
     """The same CSRs through the low-first hook (the one-hop skipping hook
     from the CSR with the low-id map): graphs below 64 vertices run one pass,
     the 210-vertex one hooks ids below 64 or 128 first, whose map then
     serves the rest (entries past n, self-loops and unsorted rows included),
     and a 70-vertex path whose ids above 64 all link through the map."""
     lib = nat.gpu_lib()
-    knobs = dict(csr_hook=1, rem=0, halve=1, hook_ilp=1, skip_linked=2, low_shift=shift)
+    knobs = dict(csr_hook=1, rem=0, halve=1, hook_ilp=1, skip_linked=2, low_shift=shift,
+                 split_walks=split)
     n = 70
     path = [[] for _ in range(n)]
     for a in range(n - 1):
@@ -95,7 +96,7 @@ def test_edge_cases_low_first_hook(gpu, shift):
             assert dropin(G).tolist() == want
     finally:
         for k, v in dict(csr_hook=-1, rem=-1, halve=-1, hook_ilp=0, skip_linked=-1,
-                         low_shift=-1).items():
+                         low_shift=-1, split_walks=-1).items():
             nat.check(lib.cc_hip_components_set(k.encode(), v), k)
 
 
@@ -362,6 +363,11 @@ PATHS = [  # every code path of the solve: (knobs) -> same labels
     dict(csr_hook=1, rem=0, halve=1, hook_ilp=1, skip_linked=2, low_shift=8, blocks_per_cu=1),
     dict(csr_hook=1, rem=0, halve=1, hook_ilp=1, skip_linked=2, low_shift=2, minima_entries=3,
          stream_blocks_per_cu=1),
+    # ... split into own-entry links and a dense pass over the walk list
+    dict(csr_hook=1, rem=0, halve=1, hook_ilp=1, skip_linked=2, low_shift=4, split_walks=1),
+    dict(csr_hook=1, rem=0, halve=1, hook_ilp=1, skip_linked=2, low_shift=1, split_walks=1,
+         blocks_per_cu=1),
+    dict(csr_hook=1, rem=0, halve=1, hook_ilp=1, skip_linked=2, low_shift=8, split_walks=0),
     dict(rem=1, rem_splice=0, skip_linked=1), dict(rem=1, rem_splice=1, skip_linked=2),
     dict(rem=1, rem_splice=2, skip_linked=2, nt=4), dict(rem=1, rem_splice=1, skip_linked=1, nt=4),
     dict(rem=1, rem_splice=0, skip_linked=2, minima_entries=4),
@@ -380,7 +386,8 @@ PATHS = [  # every code path of the solve: (knobs) -> same labels
 DEFAULTS = dict(sample_cache=1, fused_pair=-1, hook_ilp=0, hook_mode=4, sampled_neighbours=2,
                 stream_ilp=1, blocks_per_cu=0, giant_early=1, halve=-1, rem=-1, rem_splice=-1,
                 nt=-1, init_ilp=0, frontier_chunks=0, csr_hook=-1, skip_linked=-1, minima_entries=0,
-                tiles=-1, tile_shift=0, tile_pairs=0, stream_blocks_per_cu=8, low_shift=-1)
+                tiles=-1, tile_shift=0, tile_pairs=0, stream_blocks_per_cu=8, low_shift=-1,
+                split_walks=-1)
 
 
 @pytest.mark.parametrize("knobs", PATHS, ids=lambda k: ",".join(f"{a}={b}" for a, b in k.items()))
@@ -410,6 +417,7 @@ def test_every_solve_path(gpu, knobs):
 @pytest.mark.parametrize("knobs", [
     dict(), dict(csr_hook=1, rem=0, halve=1, hook_ilp=1, skip_linked=2),
     dict(csr_hook=1, rem=0, halve=1, hook_ilp=1, skip_linked=2, low_shift=1),
+    dict(csr_hook=1, rem=0, halve=1, hook_ilp=1, skip_linked=2, low_shift=2, split_walks=1),
     dict(csr_hook=1, rem=0, halve=1, hook_ilp=2, skip_linked=0),
     dict(rem=1, rem_splice=1, skip_linked=2), dict(rem=1, rem_splice=2, skip_linked=0),
     dict(rem=0, halve=1, hook_ilp=2, skip_linked=2),
