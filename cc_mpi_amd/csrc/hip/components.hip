@@ -477,10 +477,11 @@ constexpr uint32_t kNoNeighbour = 0xFFFFFFFFu;
 // Chung-Lu 65.6M 2.45 -> 2.27 ms (shifts 1/2/3/5: 2.37/2.30/2.30/2.30),
 // Chung-Lu 40M 1.58 -> 1.47-1.51, R-MAT 27 3.84 -> 3.78.
 constexpr int kLowShiftAuto = 4;
-// Queued owed unions (Knobs::split_walks): R-MAT 27 3.80 -> 3.46 ms per
-// solve, but Chung-Lu 65.6M 2.27 -> 2.32 and Chung-Lu 40M 1.47 -> 1.50, so
-// off by default (a knob for graphs with many local minima).
-constexpr int kSplitWalksAuto = 0;
+// Queued owed unions (Knobs::split_walks), on below 2^31 entries
+// (kSplitWalksMaxEntries), per solve: R-MAT 24 0.547 -> 0.493 ms, R-MAT 25
+// 1.05 -> 0.905, R-MAT 26 2.00 -> 1.785, Chung-Lu 20M 0.749 -> 0.737,
+// Chung-Lu 32M 1.26 -> 1.22; above, Chung-Lu 65.6M 2.27 -> 2.32 and
+// Chung-Lu 40M 1.47 -> 1.50 lose (R-MAT 27 3.80 -> 3.46 would gain).
 
 // Nt: bit 1 stores the sample arrays non-temporally, bit 2 loads the
 // column ids non-temporally (both are streamed once; the parent array
@@ -1597,6 +1598,7 @@ __global__ __launch_bounds__(kTileProbeSamples) void k_tile_probe(
 constexpr uint64_t kDenseDegree = 16;              // dense: m >= 16 n
 constexpr uint64_t kHalveMinVertices = 1ull << 24;  // halving from this many vertices
 constexpr uint64_t kWideIlpMaxEntries = 1ull << 31; // dense one-pass ILP 2 below this many entries
+constexpr uint64_t kSplitWalksMaxEntries = 1ull << 31;  // queued owed unions below this many entries
 constexpr uint64_t kSkipMinDegree = 6;              // sparse graphs skip from m >= 6 n
 constexpr uint64_t kTileMinVertices = 1ull << 20;   // the tile probe runs from this many vertices
 
@@ -1632,7 +1634,13 @@ inline Plan plan_for(uint32_t n, uint64_t m, const Knobs &K = g_knobs) {
   const int splice_auto = big ? 1 : (dense ? 2 : 0);
   pl.splice = pl.rem ? (K.rem_splice < 0 ? splice_auto : K.rem_splice) : 0;
   pl.halve = pl.fused && !pl.rem && K.hook_mode != 0 && (K.halve < 0 ? big : K.halve);
-  const int wide = pl.fused ? (m < kWideIlpMaxEntries ? 2 : 1) : 4;
+  // Dense graphs of 2^24+ vertices take the one-pass hook from the CSR
+  // (one root per thread, below), at every size; the lockstep hook from
+  // the sample arrays with two roots per thread, which those below 2^31
+  // entries took before the low-first hook, is what `wide` still gives
+  // where knobs pick it.
+  const bool csr_shape = pl.fused && big && dense && K.hook_mode == 4;
+  const int wide = pl.fused ? (m < kWideIlpMaxEntries && !csr_shape ? 2 : 1) : 4;
   pl.hook_ilp = pl.rem ? 1 : (K.hook_ilp ? K.hook_ilp : (dense ? wide : 1));
   const int bpc_auto = pl.rem ? (pl.splice == 1 ? 4 : 8) : (pl.hook_ilp >= 2 ? 8 : 4);
   pl.hook_bpc = K.blocks_per_cu ? K.blocks_per_cu : bpc_auto;
@@ -1650,14 +1658,19 @@ inline Plan plan_for(uint32_t n, uint64_t m, const Knobs &K = g_knobs) {
   // 103 -> 99 us; four lose on every graph (register pressure)
   pl.frontier_chunks = K.frontier_chunks ? K.frontier_chunks : 2;
   // The lockstep hook with one root per thread (dense graphs of 2^24+
-  // vertices past 2^31 entries) reads the rows itself instead of sample
-  // arrays: Chung-Lu 65.6M (3.60e9 entries) 3.30 -> 3.22 ms per solve,
-  // Chung-Lu 40M (2.31e9) 2.04 -> 1.98 ms, Chung-Lu 65.6M at gamma 2.3
-  // (3.82e9) 3.76 -> 3.60 ms.  The other plans keep the sampling init pass:
-  // with two roots per thread it is mixed (R-MAT 24 0.67 -> 0.63 ms, R-MAT
-  // 25 1.07 -> 1.14, R-MAT 26 2.27 -> 2.30), and Rem's unions lose badly
-  // (R-MAT 22 hook 93 -> 181 us, 8192^2 grid 3.76 -> 5.67 ms), since the row
-  // reads sit in front of every latency-bound union.
+  // vertices) reads the rows itself instead of sample arrays: Chung-Lu
+  // 65.6M (3.60e9 entries) 3.30 -> 3.22 ms per solve, Chung-Lu 40M (2.31e9)
+  // 2.04 -> 1.98 ms, Chung-Lu 65.6M at gamma 2.3 (3.82e9) 3.76 -> 3.60 ms.
+  // With skipping, the low-first map and, below 2^31 entries, the queued
+  // walks, it also beats the two-roots-per-thread hook from the sample
+  // arrays that dense graphs below 2^31 entries took before: R-MAT 24
+  // 0.685 -> 0.493 ms, R-MAT 25 1.06 -> 0.905, R-MAT 26 2.22 -> 1.785,
+  // Chung-Lu 20M 1.50 -> 0.737, Chung-Lu 32M 2.22 -> 1.22.  The other plans
+  // keep the sampling init pass: below 2^24 vertices and on sparse graphs
+  // the CSR plan loses (R-MAT 22 0.160 -> 0.198 ms, ER 32M degree 8 1.76 ->
+  // 2.19, 8192^2 grid 5.20 -> 7.99), and Rem's unions from the CSR lose
+  // badly (R-MAT 22 hook 93 -> 181 us, 8192^2 grid 3.76 -> 5.67 ms), since
+  // the row reads sit in front of every latency-bound union.
   const int csr_auto = !pl.rem && K.hook_mode == 4 && big && dense && pl.hook_ilp == 1;
   pl.csr = pl.fused && (pl.rem || K.hook_mode == 4) && (K.csr_hook < 0 ? csr_auto : K.csr_hook);
   // Skipping neighbour 1 of linked vertices (Knobs::skip_linked; the
@@ -1702,7 +1715,8 @@ inline Plan plan_for(uint32_t n, uint64_t m, const Knobs &K = g_knobs) {
   const bool low_ok = pl.csr && !pl.rem && pl.halve && pl.skip1 == 2 && pl.hook_ilp == 1;
   pl.low_shift = low_ok ? (uint32_t)(K.low_shift < 0 ? kLowShiftAuto : K.low_shift) : 0u;
   pl.split_walks = pl.low_shift && pl.minima_entries == 2
-                       ? (K.split_walks < 0 ? kSplitWalksAuto : K.split_walks) : 0;
+                       ? (K.split_walks < 0 ? (m < kSplitWalksMaxEntries ? 1 : 0) : K.split_walks)
+                       : 0;
   return pl;
 }
 

@@ -169,10 +169,11 @@ def test_chung_lu(gpu):
 
 @pytest.mark.parametrize("kind,rem,splice,halve,skip,tiles,ran", [
     ("er", 1, 1, 0, 0, 2, 0), ("er8", 1, 1, 0, 2, 2, 0), ("grid", 1, 1, 0, 0, 2, 0),
-    ("grid_rows", 1, 1, 0, 0, 2, 1), ("rmat24", 0, 0, 1, 0, 0, 0)])
+    ("grid_rows", 1, 1, 0, 0, 2, 1), ("rmat24", 0, 0, 1, 2, 0, 0)])
 def test_default_plan_at_scale(gpu, kind, rem, splice, halve, skip, tiles, ran):
     """Graphs of 2^24 vertices: sparse ones take Rem's unions with splicing,
-    dense ones the lockstep hook with path halving, by default, and sparse
+    dense ones the lockstep hook from the CSR with path halving, skipping,
+    the low-first map and queued walks, by default, and sparse
     ones of degree 6 and more skip neighbour 1 of linked vertices; on sparse
     ones a probe on the device picks the vertex-tile plan where ids are
     local (the lattice numbered row by row) and not elsewhere; labels,
@@ -185,9 +186,12 @@ def test_default_plan_at_scale(gpu, kind, rem, splice, halve, skip, tiles, ran):
          else devgen.grid(4096, 4096, 0.7, seed=45, permute=False) if kind == "grid_rows"
          else devgen.rmat(24, 16, seed=43))
     plan = {k: lib.cc_hip_components_plan_get(D.n, D.m, k.encode())
-            for k in ("fused", "rem", "rem_splice", "halve", "skip_linked", "tiles")}
+            for k in ("fused", "rem", "rem_splice", "halve", "skip_linked", "tiles",
+                      "csr_hook", "low_shift", "split_walks")}
+    dense = kind == "rmat24"
     assert plan == dict(fused=1, rem=rem, rem_splice=splice, halve=halve, skip_linked=skip,
-                        tiles=tiles)
+                        tiles=tiles, csr_hook=int(dense), low_shift=4 if dense else 0,
+                        split_walks=int(dense))
     got = D.solve()
     took = ctypes.c_int(-1)
     nat.check(lib.cc_hip_components_tiles_ran(None, ctypes.byref(took)), "tiles ran")
