@@ -143,8 +143,8 @@ int cc_hip_components_tune(int sampled_neighbours, int blocks_per_cu, int hook_m
  *                             skipping hook from the CSR; else 4
  *   "hook_ilp"                roots walked per thread in lockstep in the
  *                             sampled hook: 1, 2, 4, or 0 (default) = 1 when
- *                             m < 16 n, else 2 in the one-pass hook (1 from
- *                             m >= 2^31), 4 per round
+ *                             m < 16 n or n >= 2^24, else 2 in the one-pass
+ *                             hook, 4 per round
  *   "frontier_blocks_per_cu"  grid cap of the frontier sweep (default 8)
  *   "stream_blocks_per_cu"    grid cap of init/compress (default 8)
  *   "stream_ilp"              vertices per thread per step in init/compress
@@ -168,7 +168,7 @@ int cc_hip_components_tune(int sampled_neighbours, int blocks_per_cu, int hook_m
  *                             entries itself after an identity init, with no
  *                             sample arrays: 1, 0, or -1 (default) = on for
  *                             the lockstep hook with one root per thread
- *                             (n >= 2^24, m >= 16 n, m >= 2^31)
+ *                             (n >= 2^24, m >= 16 n)
  *   "skip_linked"             a vertex linked under its neighbour 0 by its
  *                             own entry skips the union with neighbour 1, so
  *                             only the local minima (neighbour 0 above them)
@@ -224,8 +224,8 @@ int cc_hip_components_tune(int sampled_neighbours, int blocks_per_cu, int hook_m
  *                             their own-entry link (local minima, failed
  *                             links) and runs them 64 at a time, so no wave
  *                             waits on one walker per step: 1, 0, or -1
- *                             (default) = 0 (faster on R-MAT 27, slower on
- *                             the Chung-Lu graphs)
+ *                             (default) = 1 below 2^31 entries, else 0
+ *                             (above it the Chung-Lu graphs lose ~2%)
  * Returns nonzero for an unknown key or out-of-range value; _get returns -1
  * for an unknown key. */
 int cc_hip_components_set(const char *key, int value);
