@@ -1697,7 +1697,11 @@ inline Plan plan_for(uint32_t n, uint64_t m, const Knobs &K = g_knobs) {
   const bool skip_auto = (pl.csr && !pl.rem) || (pl.rem && !pl.csr && !dense && m >= kSkipMinDegree * (uint64_t)n);
   pl.skip1 = skip_ok ? (K.skip_linked < 0 ? (skip_auto ? 2 : 0) : K.skip_linked) : 0;
   pl.minima_entries = K.minima_entries ? K.minima_entries : 2;
-  if (!K.blocks_per_cu && pl.skip1 && pl.csr) pl.hook_bpc = 16;
+  // 48 workgroups per CU for the skipping hook from the CSR (16 before the
+  // low-first plan): Chung-Lu 65.6M 2.27 -> 2.23 ms per solve, Chung-Lu
+  // 32M 1.22 -> 1.18, Chung-Lu 40M 1.48 -> 1.45, R-MAT 26 1.78 -> 1.72
+  // (64: 2.22, 1.17, 1.45, 1.75)
+  if (!K.blocks_per_cu && pl.skip1 && pl.csr) pl.hook_bpc = 48;
   // the last pointer-jumping pass runs after a skipping hook (the frontier
   // sweep links the vertices left outside the giant tree): 4 vertices per
   // thread, Chung-Lu 65.6M 87 -> 75 us

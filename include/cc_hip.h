@@ -139,7 +139,7 @@ int cc_hip_components_tune(int sampled_neighbours, int blocks_per_cu, int hook_m
  *   "blocks_per_cu"           sampled-hook grid cap; 0 (default) = 8 for
  *                             Rem's unions unless they splice on every step
  *                             (rem_splice 1), and when the lockstep hook
- *                             walks several roots per thread; 16 for the
+ *                             walks several roots per thread; 48 for the
  *                             skipping hook from the CSR; else 4
  *   "hook_ilp"                roots walked per thread in lockstep in the
  *                             sampled hook: 1, 2, 4, or 0 (default) = 1 when

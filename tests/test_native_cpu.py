@@ -132,10 +132,12 @@ def test_plan_defaults():
     """The default plan: cache on; one-pass hook; Rem's unions below 2^24
     vertices (8 workgroups per CU; dense graphs splice each vertex's own
     entry) and on sparse graphs above (splicing every step, 4 per CU);
-    otherwise the lockstep hook with path halving, two roots per thread
-    below 2^31 entries.  Linked vertices skip neighbour 1 (with one-hop
-    links) on the lockstep hook from the CSR (16 workgroups per CU) and on
-    sparse graphs of degree 6 and more, not on the lattice or R-MAT."""
+    otherwise the lockstep hook from the CSR with path halving and one root
+    per thread.  Linked vertices skip neighbour 1 (with one-hop links) on
+    that hook (48 workgroups per CU) and on sparse graphs of degree 6 and
+    more, not on the lattice or R-MAT 22.  The hook from the CSR runs
+    low-first (ids below n/16 first), with the owed unions queued per wave
+    below 2^31 entries."""
     lib = nat.gpu_lib()
     def plan(n, m):
         v = [ctypes.c_int() for _ in range(4)]
@@ -145,14 +147,20 @@ def test_plan_defaults():
     def get(n, m):
         return tuple(lib.cc_hip_components_plan_get(n, m, k) for k in (
             b"rem", b"rem_splice", b"halve", b"skip_linked", b"compress_ilp", b"csr_hook"))
+
+    def low(n, m):
+        return tuple(lib.cc_hip_components_plan_get(n, m, k) for k in (b"low_shift", b"split_walks"))
     # R-MAT 22, Chung-Lu 65.6M, ER 4M degree 8, ER 4M degree 3, 8192^2
     # grid, R-MAT 24, R-MAT 27
     assert plan(4194304, 128313032) == (1, 1, 1, 8) and get(4194304, 128313032) == (1, 2, 0, 0, 1, 0)
-    assert plan(65601024, 3600152054) == (1, 1, 1, 16) and get(65601024, 3600152054) == (0, 0, 1, 2, 4, 1)
+    assert plan(65601024, 3600152054) == (1, 1, 1, 48) and get(65601024, 3600152054) == (0, 0, 1, 2, 4, 1)
+    assert low(65601024, 3600152054) == (4, 0) and low(4194304, 128313032) == (0, 0)
     assert plan(4194304, 33554406) == (1, 1, 1, 8) and get(4194304, 33554406) == (1, 0, 0, 2, 4, 0)
     assert plan(4194304, 12582900) == (1, 1, 1, 8) and get(4194304, 12582900) == (1, 0, 0, 0, 1, 0)
     assert plan(67108864, 187873588) == (1, 1, 1, 4) and get(67108864, 187873588) == (1, 1, 0, 0, 1, 0)
-    assert plan(16777216, 520769534) == (1, 1, 2, 8) and get(16777216, 520769534) == (0, 0, 1, 0, 1, 0)
-    assert plan(134217728, 4223308732) == (1, 1, 1, 16) and get(134217728, 4223308732) == (0, 0, 1, 2, 4, 1)
+    assert plan(16777216, 520769534) == (1, 1, 1, 48) and get(16777216, 520769534) == (0, 0, 1, 2, 4, 1)
+    assert low(16777216, 520769534) == (4, 1) and low(67108864, 187873588) == (0, 0)
+    assert plan(134217728, 4223308732) == (1, 1, 1, 48) and get(134217728, 4223308732) == (0, 0, 1, 2, 4, 1)
+    assert low(134217728, 4223308732) == (4, 0)
     assert lib.cc_hip_components_plan_get(1, 1, b"hook_mode") == 4
     assert lib.cc_hip_components_plan_get(1, 1, b"nonsense") == -1
