@@ -205,6 +205,45 @@ def test_default_plan_at_scale(gpu, kind, rem, splice, halve, skip, tiles, ran):
                           np.bincount(want, minlength=G.n)[roots])
 
 
+@pytest.mark.parametrize("kind", ["rmat24_unpermuted", "dense_blocks"])
+def test_low_first_plan_at_scale(gpu, kind):
+    """Dense graphs of 2^24 vertices on the default plan (the low-first hook
+    from the CSR with queued walks) where its map has little to work with:
+    R-MAT 24 with ids not permuted, so the low ids are the hubs; and 64
+    disjoint random blocks of 20 entries per vertex with permuted ids, so
+    the giant root of the low ids holds about 1/64 of the vertices and most
+    links find neighbour 0 unmarked.  Labels, component count and sizes
+    equal the oracle's."""
+    from cc_mpi_amd import devgen
+    lib = nat.gpu_lib()
+    if kind == "rmat24_unpermuted":
+        D = devgen.rmat(24, 16, seed=47, permute=False)
+        G = D.to_host()
+    else:
+        n, blocks = 1 << 24, 64
+        size = n // blocks
+        rng = np.random.default_rng(48)
+        pairs = 10 * n
+        base = rng.integers(0, blocks, pairs, dtype=np.uint32) * np.uint32(size)
+        src = base + rng.integers(0, size, pairs, dtype=np.uint32)
+        dst = base + rng.integers(0, size, pairs, dtype=np.uint32)
+        G = cm.from_pairs(n, src, dst, perm=rng.permutation(n).astype(np.uint32))
+        del base, src, dst
+        D = cm.DeviceGraph(G)
+    assert G.m >= 16 * G.n and G.m < 1 << 31
+    plan = {k: lib.cc_hip_components_plan_get(G.n, G.m, k.encode())
+            for k in ("csr_hook", "skip_linked", "low_shift", "split_walks")}
+    assert plan == dict(csr_hook=1, skip_linked=2, low_shift=4, split_walks=1)
+    got = D.solve()
+    D.close()
+    want = check_labels(G, got)
+    roots = np.flatnonzero(want == np.arange(G.n))
+    if kind == "dense_blocks":
+        assert roots.size >= blocks
+    assert np.array_equal(np.bincount(got, minlength=G.n)[roots],
+                          np.bincount(want, minlength=G.n)[roots])
+
+
 def test_last_pass_gate(gpu):
     """The last pointer-jumping pass is skipped when the frontier sweep links
     no root; labels are exact either way, and both cases occur here."""
