@@ -18,17 +18,18 @@
 //   2. Sampled hooking: every vertex is united with its neighbour 0 (only
 //      where step 1 did not already link them) and its neighbour 1, in one
 //      pass: by Rem's unions (both paths climbed together, stopping where
-//      they meet) below 2^24 vertices and on sparse graphs, else by walking
-//      v's root and both neighbours' roots in lockstep with path halving
+//      they meet) on sparse graphs and on dense ones below 2^23 vertices,
+//      else by walking v's root and both neighbours' roots in lockstep with
+//      path halving, reading each row's first entries from the CSR
 //      (or, as a tuning choice, a pass per neighbour with pointer jumping
 //      between).  After this the forest already holds the giant component
-//      of a power-law or random graph in one tree.  On the largest dense
-//      graphs and on sparse graphs of degree 6 and more, a vertex linked
+//      of a power-law or random graph in one tree.  On those dense graphs
+//      and on sparse graphs of degree 6 and more, a vertex linked
 //      under neighbour 0 skips neighbour 1 and takes neighbour 0's parent
 //      as its own (one hop): the trees then hang from the local minima
 //      (vertices below all their neighbours), which unite with both
 //      sampled neighbours, and those links join nearly every tree into the
-//      giant one while only the minima walk.  On the largest of them the
+//      giant one while only the minima walk.  On the dense ones the
 //      ids below n/16 are hooked first; a bitmap of which of them then hang
 //      under their most frequent root g0 (1 bit per id, small enough to
 //      stay in every XCD's L2) lets each other vertex whose neighbour 0 is
@@ -1599,6 +1600,7 @@ constexpr uint64_t kDenseDegree = 16;              // dense: m >= 16 n
 constexpr uint64_t kHalveMinVertices = 1ull << 24;  // halving from this many vertices
 constexpr uint64_t kWideIlpMaxEntries = 1ull << 31; // dense one-pass ILP 2 below this many entries
 constexpr uint64_t kSplitWalksMaxEntries = 1ull << 31;  // queued owed unions below this many entries
+constexpr uint64_t kCsrMinVertices = 1ull << 23;       // dense graphs from here: the hook from the CSR
 constexpr uint64_t kSkipMinDegree = 6;              // sparse graphs skip from m >= 6 n
 constexpr uint64_t kTileMinVertices = 1ull << 20;   // the tile probe runs from this many vertices
 
@@ -1630,16 +1632,17 @@ inline Plan plan_for(uint32_t n, uint64_t m, const Knobs &K = g_knobs) {
   const bool big = n >= kHalveMinVertices;
   pl.cache = K.sample_cache && (K.hook_mode == 0 || K.hook_mode == 1 || K.hook_mode == 4);
   pl.fused = pl.cache && K.sampled_neighbours == 2 && (K.fused_pair < 0 ? 1 : K.fused_pair);
-  pl.rem = pl.fused && K.hook_mode != 0 && (K.rem < 0 ? (!big || !dense) : K.rem);
+  // Dense graphs of 2^23+ vertices take the one-pass hook from the CSR
+  // (one root per thread, path halving, below), at every size; the
+  // lockstep hook from the sample arrays with two roots per thread, which
+  // those of 2^24+ vertices below 2^31 entries took before the low-first
+  // hook, is what `wide` still gives where knobs pick it, and those below
+  // 2^24 vertices took Rem's unions.
+  const bool csr_shape = pl.fused && dense && n >= kCsrMinVertices && K.hook_mode == 4;
+  pl.rem = pl.fused && K.hook_mode != 0 && (K.rem < 0 ? (!big || !dense) && !csr_shape : K.rem);
   const int splice_auto = big ? 1 : (dense ? 2 : 0);
   pl.splice = pl.rem ? (K.rem_splice < 0 ? splice_auto : K.rem_splice) : 0;
-  pl.halve = pl.fused && !pl.rem && K.hook_mode != 0 && (K.halve < 0 ? big : K.halve);
-  // Dense graphs of 2^24+ vertices take the one-pass hook from the CSR
-  // (one root per thread, below), at every size; the lockstep hook from
-  // the sample arrays with two roots per thread, which those below 2^31
-  // entries took before the low-first hook, is what `wide` still gives
-  // where knobs pick it.
-  const bool csr_shape = pl.fused && big && dense && K.hook_mode == 4;
+  pl.halve = pl.fused && !pl.rem && K.hook_mode != 0 && (K.halve < 0 ? big || csr_shape : K.halve);
   const int wide = pl.fused ? (m < kWideIlpMaxEntries && !csr_shape ? 2 : 1) : 4;
   pl.hook_ilp = pl.rem ? 1 : (K.hook_ilp ? K.hook_ilp : (dense ? wide : 1));
   const int bpc_auto = pl.rem ? (pl.splice == 1 ? 4 : 8) : (pl.hook_ilp >= 2 ? 8 : 4);
@@ -1666,12 +1669,14 @@ inline Plan plan_for(uint32_t n, uint64_t m, const Knobs &K = g_knobs) {
   // arrays that dense graphs below 2^31 entries took before: R-MAT 24
   // 0.685 -> 0.493 ms, R-MAT 25 1.06 -> 0.905, R-MAT 26 2.22 -> 1.785,
   // Chung-Lu 20M 1.50 -> 0.737, Chung-Lu 32M 2.22 -> 1.22.  The other plans
-  // keep the sampling init pass: below 2^24 vertices and on sparse graphs
-  // the CSR plan loses (R-MAT 22 0.160 -> 0.198 ms, ER 32M degree 8 1.76 ->
-  // 2.19, 8192^2 grid 5.20 -> 7.99), and Rem's unions from the CSR lose
+  // keep the sampling init pass.  On dense graphs the CSR plan also wins
+  // from 2^23 vertices (R-MAT 23 0.313 -> 0.281 ms, Chung-Lu 8.2M 0.413 ->
+  // 0.381, 12M 0.609 -> 0.513, 16.4M 0.80 -> 0.68), but it loses below
+  // that and on sparse graphs (R-MAT 22 0.160 -> 0.198 ms, ER 32M degree 8
+  // 1.76 -> 2.19, 8192^2 grid 5.20 -> 7.99), and Rem's unions from the CSR lose
   // badly (R-MAT 22 hook 93 -> 181 us, 8192^2 grid 3.76 -> 5.67 ms), since
   // the row reads sit in front of every latency-bound union.
-  const int csr_auto = !pl.rem && K.hook_mode == 4 && big && dense && pl.hook_ilp == 1;
+  const int csr_auto = csr_shape && !pl.rem && pl.hook_ilp == 1;
   pl.csr = pl.fused && (pl.rem || K.hook_mode == 4) && (K.csr_hook < 0 ? csr_auto : K.csr_hook);
   // Skipping neighbour 1 of linked vertices (Knobs::skip_linked; the
   // lockstep hooks and Rem's unions from the sample arrays) trades the hook's
@@ -1700,8 +1705,9 @@ inline Plan plan_for(uint32_t n, uint64_t m, const Knobs &K = g_knobs) {
   // 48 workgroups per CU for the skipping hook from the CSR (16 before the
   // low-first plan): Chung-Lu 65.6M 2.27 -> 2.23 ms per solve, Chung-Lu
   // 32M 1.22 -> 1.18, Chung-Lu 40M 1.48 -> 1.45, R-MAT 26 1.78 -> 1.72
-  // (64: 2.22, 1.17, 1.45, 1.75)
-  if (!K.blocks_per_cu && pl.skip1 && pl.csr) pl.hook_bpc = 48;
+  // (64: 2.22, 1.17, 1.45, 1.75); below 2^24 vertices 16 (R-MAT 23 0.287
+  // -> 0.281 ms, Chung-Lu 12M 0.531 -> 0.513, Chung-Lu 8.2M 0.406 -> 0.381)
+  if (!K.blocks_per_cu && pl.skip1 && pl.csr) pl.hook_bpc = big ? 48 : 16;
   // the last pointer-jumping pass runs after a skipping hook (the frontier
   // sweep links the vertices left outside the giant tree): 4 vertices per
   // thread, Chung-Lu 65.6M 87 -> 75 us

@@ -129,11 +129,12 @@ def test_rmat_sizes():
 
 
 def test_plan_defaults():
-    """The default plan: cache on; one-pass hook; Rem's unions below 2^24
-    vertices (8 workgroups per CU; dense graphs splice each vertex's own
-    entry) and on sparse graphs above (splicing every step, 4 per CU);
-    otherwise the lockstep hook from the CSR with path halving and one root
-    per thread.  Linked vertices skip neighbour 1 (with one-hop links) on
+    """The default plan: cache on; one-pass hook; Rem's unions on sparse
+    graphs (splicing every step and 4 workgroups per CU from 2^24 vertices,
+    8 below) and on dense graphs below 2^23 vertices (each vertex's own
+    entry spliced, 8 per CU);
+    dense ones of 2^23+ vertices the lockstep hook from the CSR with path
+    halving and one root per thread.  Linked vertices skip neighbour 1 (with one-hop links) on
     that hook (48 workgroups per CU) and on sparse graphs of degree 6 and
     more, not on the lattice or R-MAT 22.  The hook from the CSR runs
     low-first (ids below n/16 first), with the owed unions queued per wave
@@ -162,5 +163,9 @@ def test_plan_defaults():
     assert low(16777216, 520769534) == (4, 1) and low(67108864, 187873588) == (0, 0)
     assert plan(134217728, 4223308732) == (1, 1, 1, 48) and get(134217728, 4223308732) == (0, 0, 1, 2, 4, 1)
     assert low(134217728, 4223308732) == (4, 0)
+    # R-MAT 23 and Chung-Lu 16.4M: dense from 2^23 vertices, 16 workgroups
+    # per CU below 2^24
+    assert plan(8388608, 256000000) == (1, 1, 1, 16) and get(8388608, 256000000) == (0, 0, 1, 2, 4, 1)
+    assert plan(16400256, 865053518) == (1, 1, 1, 16) and low(16400256, 865053518) == (4, 1)
     assert lib.cc_hip_components_plan_get(1, 1, b"hook_mode") == 4
     assert lib.cc_hip_components_plan_get(1, 1, b"nonsense") == -1
