@@ -122,7 +122,7 @@ int cc_hip_components_tune(int sampled_neighbours, int blocks_per_cu, int hook_m
  *                             paths climbed together, stopping where they
  *                             meet) instead of walking all three roots in
  *                             lockstep: 1, 0, or -1 (default) = on when
- *                             n < 2^24 or m < 16 n; not in mode 0
+ *                             m < 16 n, or n < 2^23; not in mode 0
  *   "rem_splice"              splicing in those unions (a step lowers the
  *                             entry it leaves to the other side's parent):
  *                             1 on every step, 2 on the vertex's own entry
@@ -131,7 +131,8 @@ int cc_hip_components_tune(int sampled_neighbours, int blocks_per_cu, int hook_m
  *   "halve"                   path halving (atomic min of each visited
  *                             entry to its grandparent) in the lockstep
  *                             one-pass hook's root walks: 1, 0, or -1
- *                             (default) = on when n >= 2^24; not in mode 0
+ *                             (default) = on when n >= 2^24, or n >= 2^23
+ *                             and m >= 16 n; not in mode 0
  *   "giant_early"             1 (default): on the two-pass cached path,
  *                             sample the giant root after round 0 and skip
  *                             its members in round 1; 0: sample it once,
@@ -140,10 +141,11 @@ int cc_hip_components_tune(int sampled_neighbours, int blocks_per_cu, int hook_m
  *                             Rem's unions unless they splice on every step
  *                             (rem_splice 1), and when the lockstep hook
  *                             walks several roots per thread; 48 for the
- *                             skipping hook from the CSR; else 4
+ *                             skipping hook from the CSR (16 below 2^24
+ *                             vertices); else 4
  *   "hook_ilp"                roots walked per thread in lockstep in the
  *                             sampled hook: 1, 2, 4, or 0 (default) = 1 when
- *                             m < 16 n or n >= 2^24, else 2 in the one-pass
+ *                             m < 16 n or n >= 2^23, else 2 in the one-pass
  *                             hook, 4 per round
  *   "frontier_blocks_per_cu"  grid cap of the frontier sweep (default 8)
  *   "stream_blocks_per_cu"    grid cap of init/compress (default 8)
@@ -168,7 +170,7 @@ int cc_hip_components_tune(int sampled_neighbours, int blocks_per_cu, int hook_m
  *                             entries itself after an identity init, with no
  *                             sample arrays: 1, 0, or -1 (default) = on for
  *                             the lockstep hook with one root per thread
- *                             (n >= 2^24, m >= 16 n)
+ *                             (n >= 2^23, m >= 16 n)
  *   "skip_linked"             a vertex linked under its neighbour 0 by its
  *                             own entry skips the union with neighbour 1, so
  *                             only the local minima (neighbour 0 above them)
