@@ -281,6 +281,34 @@ def test_solve_on_caller_streams(gpu):
     D.close()
 
 
+def test_low_first_plan_on_concurrent_streams(gpu):
+    """Two dense graphs of 2^23 vertices (the low-first hook from the CSR,
+    queued walks) solved at the same time on two caller streams, three times
+    each with no wait in between: each stream keeps its own giant root, map
+    and workspace, so both label arrays equal the oracle's."""
+    from cc_mpi_amd import devgen
+    lib = nat.gpu_lib()
+    graphs = [devgen.rmat(23, 16, seed=61), devgen.chung_lu(1 << 23, 245000000, 2.1, seed=62)]
+    for D in graphs:
+        assert lib.cc_hip_components_plan_get(D.n, D.m, b"csr_hook") == 1
+        assert lib.cc_hip_components_plan_get(D.n, D.m, b"low_shift") == 4
+    streams = [ctypes.c_void_p(), ctypes.c_void_p()]
+    for s in streams:
+        nat.check(lib.cc_hip_stream_create(ctypes.byref(s)), "stream create")
+    try:
+        for _ in range(3):
+            for D, s in zip(graphs, streams):
+                D.solve_async(s)
+        got = [D.labels.download(np.empty(D.n, dtype=np.uint32), s)
+               for D, s in zip(graphs, streams)]
+    finally:
+        for s in streams:
+            nat.check(lib.cc_hip_stream_destroy(s), "stream destroy")
+    for D, g in zip(graphs, got):
+        check_labels(D.to_host(), g)
+        D.close()
+
+
 def test_device_resident_repeat(gpu):
     # same labels on repeated solves of a resident graph, and equal to the oracle
     G = cm.rmat(16, 16, seed=5)
